@@ -1,0 +1,171 @@
+/*
+ * pdlp_hip.h — C ABI of libpdlp_hip.so, the MI355X (gfx950) restarted-PDHG engine.
+ *
+ * The reference has no native layer: its PDHG hot path is the Python loop of
+ * solver/lp/pdhg.py:209-244 (CPU, SciPy csr_matvec + NumPy) and
+ * solver/lp/pdhg_gpu.py:196-282, 378-427 (library kernels dispatched one by
+ * one from Python, KKT on the host every 50 iterations).  Each entry point
+ * below names the reference code it replaces.  The Python drop-in
+ * (sovereign_solver_amd.lp.pdhg / pdhg_gpu / batch) binds these with ctypes.
+ *
+ * Conventions
+ *   - every pointer is a device pointer owned by the caller; the library
+ *     allocates no device memory (workspaces are passed in);
+ *   - every call is asynchronous on the given stream (hipStream_t passed as
+ *     void*; NULL = default stream);
+ *   - return value: 0 on success, a positive hipError_t from the HIP runtime,
+ *     or a negative PDLP_E* code for argument errors; pdlp_last_error()
+ *     returns a thread-local message for the last failure;
+ *   - no C++ exception crosses the ABI.
+ */
+#ifndef PDLP_HIP_H
+#define PDLP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDLP_ABI_VERSION 1
+
+/* precision of iterates, matrix values and problem vectors */
+#define PDLP_F32 0
+#define PDLP_F64 1
+
+/* negative return codes */
+#define PDLP_E_ARG    (-1)   /* bad argument (null pointer, size, enum) */
+#define PDLP_E_TUNE   (-2)   /* unsupported tuning value */
+#define PDLP_E_SHAPE  (-3)   /* inconsistent sizes */
+#define PDLP_E_LIMIT  (-4)   /* size beyond a documented maximum */
+
+/* run status kept in PdlpCtrl.status */
+#define PDLP_RUNNING    0
+#define PDLP_OPTIMAL    1
+#define PDLP_NUMERICAL  3    /* NaN/Inf in a KKT evaluation */
+
+/* row kinds of the box form */
+#define PDLP_ROW_EQ 0        /* K_i x  = b_i, dual free     */
+#define PDLP_ROW_LE 1        /* K_i x <= b_i, dual y_i <= 0 */
+
+/* Number of doubles of PdlpDev.partials the engine needs. */
+#define PDLP_PARTIAL_BLOCKS 1024
+#define PDLP_PARTIALS_LEN   (2 * PDLP_PARTIAL_BLOCKS * 4)
+
+/*
+ * Device control block, one per instance, device-resident.  The engine
+ * updates it; the host reads it back (a few dozen bytes) between blocks of
+ * iterations.  Replaces the host-side scalars of pdhg.py:198-207
+ * (window_len, current_kkt, status, iters).
+ */
+typedef struct PdlpCtrl {
+    int32_t iter;            /* iterations executed                          */
+    int32_t window;          /* iterations in the current averaging window   */
+    int32_t status;          /* PDLP_RUNNING / PDLP_OPTIMAL / PDLP_NUMERICAL */
+    int32_t restarts;
+    int32_t done_iter;       /* iteration at which status left RUNNING       */
+    int32_t restart_pending; /* set by the check, consumed by the apply step */
+    int32_t apply_window;    /* window length the pending average divides by */
+    int32_t checks;
+    double ref_err;          /* KKT at the last restart point (pdhg.py:203,244) */
+    double last_err, rp, rd, gap;
+    double tol;
+    double norm_b, norm_c;   /* max(1,||b||), max(1,||c||) of the reference form */
+    double tau_scale;        /* multiplies tau_j (primal weight); 1 = reference */
+    double sigma_scale;      /* multiplies sigma_i; 1 = reference               */
+    double p_obj, d_obj;     /* objectives of the last check (reference form)   */
+} PdlpCtrl;
+
+/*
+ * One LP on the device.  Box form: min c'x, K_i x (= | <=) b_i, lo <= x <= hi.
+ * The reference form of pdhg.py:62-125 is the special case with every row
+ * '=', lo = 0, hi = +inf; the same kernels run both.
+ * Arrays of type "prec" are float (PDLP_F32) or double (PDLP_F64).
+ */
+typedef struct PdlpDev {
+    int32_t m, n, nnz;
+    int32_t prec;
+    /* K (m x n) CSR and K^T (n x m) CSR; column indices sorted per row */
+    const int32_t *row_ptr, *col_idx; const void *val;
+    const int32_t *t_row_ptr, *t_col_idx; const void *t_val;
+    const void *c;            /* n, prec                                   */
+    const void *lo, *hi;      /* n, prec; NULL means 0 and +inf            */
+    const void *b;            /* m, prec                                   */
+    const int8_t *row_kind;   /* m; NULL means every row PDLP_ROW_EQ       */
+    /* FP64 data of the KKT yardstick (reference formula on the lifted form) */
+    const double *c64;        /* n                                          */
+    const double *b64;        /* m                                          */
+    const double *b_shift;    /* m: b - K*lo_shift (right-hand side after the shift) */
+    const double *lo_shift;   /* n: finite lower bound, else 0; NULL means 0 */
+    const double *ub_span;    /* n: hi - lo_shift where hi is finite, else +inf; NULL means +inf */
+    /* state */
+    void *x, *xbar, *y;       /* n, n, m; prec */
+    double *x_sum, *y_sum;    /* n, m */
+    double *partials;         /* PDLP_PARTIALS_LEN doubles of workspace */
+    PdlpCtrl *ctrl;
+} PdlpDev;
+
+/* Launch shape, chosen by the autotuner (sovereign_solver_amd.engine.tune). */
+typedef struct PdlpTune {
+    int32_t tpr_primal;   /* lanes per K^T row: 1,2,4,8,16,32,64 */
+    int32_t tpr_dual;     /* lanes per K row:   1,2,4,8,16,32,64 */
+    int32_t block;        /* threads per workgroup: multiple of 64 in [64, 1024] */
+    int32_t check_every;  /* iterations between KKT checks; 50 = reference */
+    int32_t xcd_remap;    /* bit 0: primal step, bit 1: dual step — give each XCD
+                             a contiguous range of rows */
+    int32_t exact;        /* 1: sequential row sums, no FMA contraction (bitwise
+                             agreement with SciPy/NumPy in PDLP_F64); needs tpr 1 */
+} PdlpTune;
+
+int         pdlp_abi_version(void);
+const char* pdlp_last_error(void);
+
+/* y = A x, A in CSR (prec values), x,y prec.  The SpMV of pdhg.py:211,217
+ * (SciPy csr_matvec) and pdhg_gpu.py:215,222,245,252 (cuSPARSE / torch.mv). */
+int pdlp_spmv_csr(int32_t prec, const int32_t* row_ptr, const int32_t* col_idx,
+                  const void* val, const void* x, void* y, int32_t nrows,
+                  int32_t tpr, void* stream);
+
+/* y = A x with FP32 A, FP64 x and FP64 accumulation: the products of the KKT
+ * check (pdhg.py:186,188) when the iterate engine runs in FP32. */
+int pdlp_spmv_csr_f64acc(const int32_t* row_ptr, const int32_t* col_idx,
+                         const float* val, const double* x, double* y,
+                         int32_t nrows, int32_t tpr, void* stream);
+
+/* Reset the state for a solve from (x, y) already in dev->x / dev->y
+ * (zeros = the reference's cold start, pdhg.py:175-176): clears the sums and
+ * counters and evaluates the KKT of the starting point into ctrl->ref_err
+ * (pdhg.py:203).  ctrl->tol, norm_b, norm_c, tau_scale, sigma_scale must be
+ * set by the caller beforehand. */
+int pdlp_init(const PdlpDev* dev, const PdlpTune* tune, void* stream);
+
+/* One primal step (pdhg.py:211; pdhg_gpu.py:215-216):
+ *   x+ = clip(x - tau.(c - K^T y), lo, hi),  xbar = 2x+ - x,  x_sum += x+
+ * tau_j = tau_scale * 0.999 / max(sum_i |K_ij|, 1e-4) is formed from the K^T
+ * row in registers (pdhg.py:131-147), never stored. */
+int pdlp_primal_step(const PdlpDev* dev, const PdlpTune* tune, void* stream);
+
+/* One dual step (pdhg.py:217; pdhg_gpu.py:222-223):
+ *   y+ = proj(y + sigma.(b - K xbar)),  y_sum += y+
+ * proj keeps y_i <= 0 on PDLP_ROW_LE rows. */
+int pdlp_dual_step(const PdlpDev* dev, const PdlpTune* tune, void* stream);
+
+/* KKT of the window average, termination and restart (pdhg.py:225-244,
+ * replacing pdhg_gpu.py:381-427 and its host round trip). */
+int pdlp_kkt_check(const PdlpDev* dev, const PdlpTune* tune, void* stream);
+
+/* iters x (primal, dual), then a KKT check when check != 0. */
+int pdlp_run(const PdlpDev* dev, const PdlpTune* tune, int32_t iters,
+             int32_t check, void* stream);
+
+/* Capture `blocks` x pdlp_run(dev, tune, tune->check_every, 1) in a HIP graph.
+ * *exec receives an opaque handle for pdlp_graph_launch / pdlp_graph_destroy. */
+int pdlp_graph_capture(const PdlpDev* dev, const PdlpTune* tune, int32_t blocks,
+                       void* stream, void** exec);
+int pdlp_graph_launch(void* exec, void* stream);
+int pdlp_graph_destroy(void* exec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PDLP_HIP_H */

@@ -1,0 +1,103 @@
+// Shared device helpers of libpdlp_hip.so (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+#include <string>
+
+#include "../../include/pdlp_hip.h"
+
+namespace pdlp {
+
+constexpr int kWave = 64;
+
+// Thread-local message behind pdlp_last_error().
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Sum over aligned groups of W lanes inside one wavefront (W divides 64).
+template <int W, typename A>
+__device__ __forceinline__ A group_sum(A v) {
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
+    return v;
+}
+
+// Exact (non-contracted, round-to-nearest) arithmetic for the bitwise mode.
+__device__ __forceinline__ double add_rn(double a, double b) { return __dadd_rn(a, b); }
+__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ float add_rn(float a, float b) { return __fadd_rn(a, b); }
+__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
+
+// Workgroup index after the optional XCD remap.  Consecutive workgroups are
+// dispatched round-robin over the 8 XCDs; with the remap, XCD k receives the
+// contiguous index range [k*per, (k+1)*per), so rows that share gathered
+// cache lines also share an L2.  The grid must be a multiple of 8 then.
+__device__ __forceinline__ int block_index(int remap) {
+    int b = blockIdx.x;
+    if (remap) {
+        int per = gridDim.x >> 3;
+        b = (b & 7) * per + (b >> 3);
+    }
+    return b;
+}
+
+// Dot product of CSR row `row` with a gathered vector, plus the row's
+// absolute sum, by a group of TPR lanes (lane = index inside the group).
+// Exact mode: one lane, ascending column order, no FMA — the summation of
+// SciPy's csr_matvec.
+template <int TPR, bool EXACT, typename A, typename V, typename G>
+__device__ __forceinline__ void row_dot(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                        const V* __restrict__ val, int row, int lane, G gather,
+                                        A& dot, A& asum) {
+    const int beg = rp[row], end = rp[row + 1];
+    A d = A(0), s = A(0);
+    if constexpr (EXACT) {
+        for (int k = beg; k < end; ++k) {
+            const A a = (A)val[k];
+            d = add_rn(d, mul_rn(a, gather(ci[k])));
+            s = add_rn(s, fabs(a));
+        }
+    } else {
+        for (int k = beg + lane; k < end; k += TPR) {
+            const A a = (A)val[k];
+            d += a * gather(ci[k]);
+            s += fabs(a);
+        }
+        if constexpr (TPR > 1) {
+            d = group_sum<TPR>(d);
+            s = group_sum<TPR>(s);
+        }
+    }
+    dot = d;
+    asum = s;
+}
+
+// Block-wide sum of NV doubles (blockDim.x a multiple of 64, <= 1024).
+// Result valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* >= 16*NV */) {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = group_sum<kWave>(v[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) lds[w * NV + k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double t = 0.0;
+            for (int i = 0; i < nw; ++i) t += lds[i * NV + k];
+            v[k] = t;
+        }
+    }
+    __syncthreads();
+}
+
+}  // namespace pdlp

@@ -1,0 +1,485 @@
+// Single-instance restarted PDHG on gfx950.
+//
+// Replaces the per-iteration Python dispatch of solver/lp/pdhg.py:209-244 and
+// solver/lp/pdhg_gpu.py:196-282, 378-427.  Per iteration two kernels run:
+//   primal: one K^T row (= one column of K) per group of TPR lanes; the
+//           gathered K^T y, the step tau_j (from the same row's |values|), the
+//           box projection, the extrapolation and the ergodic sum are fused;
+//   dual:   one K row per group; gathers xbar, forms sigma_i, projects y.
+// Every check_every iterations four small kernels evaluate the reference KKT
+// of the window average in FP64 (two partial-sum passes, one finalize that
+// decides termination/restart, one apply that restarts to the average).
+// The control block lives on the device; kernels return at once after the
+// status leaves RUNNING, so blocks of iterations can be enqueued (and
+// captured in a HIP graph) ahead of the host's status poll.
+
+#include "pdlp_common.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace pdlp {
+
+namespace {
+
+template <typename T>
+__device__ __forceinline__ T clamp_box(T v, const T* lo, const T* hi, int j) {
+    // Order matters only for crossed bounds (lo > hi), which validation rejects.
+    if (lo != nullptr) v = v < lo[j] ? lo[j] : v;
+    else v = v < T(0) ? T(0) : v;
+    if (hi != nullptr) v = v > hi[j] ? hi[j] : v;
+    return v;
+}
+
+// ---------------------------------------------------------------- primal step
+template <typename T, int TPR, bool EXACT>
+__global__ void __launch_bounds__(1024) primal_kernel(PdlpDev d, int remap) {
+    const PdlpCtrl* ctrl = d.ctrl;
+    if (ctrl->status != PDLP_RUNNING) return;
+    const T tau_scale = (T)ctrl->tau_scale;
+    const T* __restrict__ tv = (const T*)d.t_val;
+    const T* __restrict__ y = (const T*)d.y;
+    const T* __restrict__ c = (const T*)d.c;
+    const T* lo = (const T*)d.lo;
+    const T* hi = (const T*)d.hi;
+    T* __restrict__ x = (T*)d.x;
+    T* __restrict__ xbar = (T*)d.xbar;
+    const int groups_per_block = blockDim.x / TPR;
+    const int lane = threadIdx.x % TPR;
+    const int j = block_index(remap) * groups_per_block + threadIdx.x / TPR;
+    if (j >= d.n) return;
+    T aty, asum;
+    row_dot<TPR, EXACT, T>(d.t_row_ptr, d.t_col_idx, tv, j, lane,
+                           [&](int k) { return y[k]; }, aty, asum);
+    if (lane != 0) return;
+    const T colsum = asum > T(1e-4) ? asum : T(1e-4);
+    T tau = T(0.999) / colsum;
+    if (tau_scale != T(1)) tau *= tau_scale;
+    const T xj = x[j];
+    T xn;
+    if constexpr (EXACT) {
+        xn = add_rn(xj, -mul_rn(tau, add_rn(c[j], -aty)));
+        xn = clamp_box(xn, lo, hi, j);
+        xbar[j] = add_rn(mul_rn(T(2), xn), -xj);
+    } else {
+        xn = clamp_box(xj - tau * (c[j] - aty), lo, hi, j);
+        xbar[j] = T(2) * xn - xj;
+    }
+    x[j] = xn;
+    d.x_sum[j] += (double)xn;
+}
+
+// ------------------------------------------------------------------ dual step
+template <typename T, int TPR, bool EXACT>
+__global__ void __launch_bounds__(1024) dual_kernel(PdlpDev d, int remap) {
+    PdlpCtrl* ctrl = d.ctrl;
+    if (ctrl->status != PDLP_RUNNING) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {   // one writer; nobody else reads these here
+        ctrl->iter += 1;
+        ctrl->window += 1;
+    }
+    const T sigma_scale = (T)ctrl->sigma_scale;
+    const T* __restrict__ v = (const T*)d.val;
+    const T* __restrict__ xbar = (const T*)d.xbar;
+    const T* __restrict__ b = (const T*)d.b;
+    T* __restrict__ y = (T*)d.y;
+    const int groups_per_block = blockDim.x / TPR;
+    const int lane = threadIdx.x % TPR;
+    const int i = block_index(remap) * groups_per_block + threadIdx.x / TPR;
+    if (i >= d.m) return;
+    T ax, asum;
+    row_dot<TPR, EXACT, T>(d.row_ptr, d.col_idx, v, i, lane,
+                           [&](int k) { return xbar[k]; }, ax, asum);
+    if (lane != 0) return;
+    const T rowsum = asum > T(1e-4) ? asum : T(1e-4);
+    T sigma = T(0.999) / rowsum;
+    if (sigma_scale != T(1)) sigma *= sigma_scale;
+    T yn;
+    if constexpr (EXACT) yn = add_rn(y[i], mul_rn(sigma, add_rn(b[i], -ax)));
+    else yn = y[i] + sigma * (b[i] - ax);
+    if (d.row_kind != nullptr && d.row_kind[i] == PDLP_ROW_LE && yn > T(0)) yn = T(0);
+    y[i] = yn;
+    d.y_sum[i] += (double)yn;
+}
+
+// ------------------------------------------------------------- KKT partials
+// Partial layout: partials[part*BLOCKS*4 + blk*4 + k], k = 0 rp^2, 1 rd^2,
+// 2 primal objective, 3 dual objective.  part 0 = rows, part 1 = columns.
+// src: 0 = current iterate, 1 = window average (sum / window).
+
+template <typename T, int TPR>
+__global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src) {
+    __shared__ double lds[16 * 4];
+    const PdlpCtrl* ctrl = d.ctrl;
+    if (ctrl->status != PDLP_RUNNING) return;
+    const double w = src ? (double)ctrl->window : 1.0;
+    const T* __restrict__ v = (const T*)d.val;
+    const T* __restrict__ x = (const T*)d.x;
+    const T* __restrict__ y = (const T*)d.y;
+    const double* __restrict__ xs = d.x_sum;
+    const double* __restrict__ ys = d.y_sum;
+    const int gpb = blockDim.x / TPR, lane = threadIdx.x % TPR;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * gpb + threadIdx.x / TPR; i < d.m; i += gridDim.x * gpb) {
+        double ax, asum;
+        if (src) row_dot<TPR, false, double>(d.row_ptr, d.col_idx, v, i, lane,
+                                             [&](int k) { return xs[k] / w; }, ax, asum);
+        else row_dot<TPR, false, double>(d.row_ptr, d.col_idx, v, i, lane,
+                                         [&](int k) { return (double)x[k]; }, ax, asum);
+        if (lane == 0) {
+            const bool le = d.row_kind != nullptr && d.row_kind[i] == PDLP_ROW_LE;
+            double res = ax - d.b64[i];
+            if (le && res < 0.0) res = 0.0;
+            const double yi = src ? ys[i] / w : (double)y[i];
+            acc[0] += res * res;
+            if (le && yi > 0.0) acc[1] += yi * yi;      // slack column reduced cost -y_i
+            acc[3] += d.b_shift[i] * yi;
+        }
+    }
+    block_sum<4>(acc, lds);
+    if (threadIdx.x == 0) {
+        double* p = d.partials + (size_t)blockIdx.x * 4;
+        for (int k = 0; k < 4; ++k) p[k] = acc[k];
+    }
+}
+
+template <typename T, int TPR>
+__global__ void __launch_bounds__(256) kkt_cols_kernel(PdlpDev d, int src) {
+    __shared__ double lds[16 * 4];
+    const PdlpCtrl* ctrl = d.ctrl;
+    if (ctrl->status != PDLP_RUNNING) return;
+    const double w = src ? (double)ctrl->window : 1.0;
+    const T* __restrict__ tv = (const T*)d.t_val;
+    const T* __restrict__ x = (const T*)d.x;
+    const T* __restrict__ y = (const T*)d.y;
+    const double* __restrict__ xs = d.x_sum;
+    const double* __restrict__ ys = d.y_sum;
+    const int gpb = blockDim.x / TPR, lane = threadIdx.x % TPR;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j = blockIdx.x * gpb + threadIdx.x / TPR; j < d.n; j += gridDim.x * gpb) {
+        double aty, asum;
+        if (src) row_dot<TPR, false, double>(d.t_row_ptr, d.t_col_idx, tv, j, lane,
+                                             [&](int k) { return ys[k] / w; }, aty, asum);
+        else row_dot<TPR, false, double>(d.t_row_ptr, d.t_col_idx, tv, j, lane,
+                                         [&](int k) { return (double)y[k]; }, aty, asum);
+        if (lane == 0) {
+            const double r = d.c64[j] - aty;
+            const double span = d.ub_span != nullptr ? d.ub_span[j] : INFINITY;
+            const double rneg = r < 0.0 ? r : 0.0;
+            if (isfinite(span)) acc[3] += span * rneg;   // upper-bound row dual w_j = min(r_j, 0)
+            else acc[1] += rneg * rneg;
+            const double xj = src ? xs[j] / w : (double)x[j];
+            const double sh = d.lo_shift != nullptr ? d.lo_shift[j] : 0.0;
+            acc[2] += d.c64[j] * (xj - sh);
+        }
+    }
+    block_sum<4>(acc, lds);
+    if (threadIdx.x == 0) {
+        double* p = d.partials + (size_t)(PDLP_PARTIAL_BLOCKS + blockIdx.x) * 4;
+        for (int k = 0; k < 4; ++k) p[k] = acc[k];
+    }
+}
+
+// mode: 0 = initial point (sets ref_err), 1 = check of the window average.
+__global__ void __launch_bounds__(256) kkt_finalize_kernel(PdlpDev d, int nb_rows, int nb_cols, int mode) {
+    __shared__ double lds[16 * 4];
+    PdlpCtrl* ctrl = d.ctrl;
+    if (ctrl->status != PDLP_RUNNING) {
+        if (threadIdx.x == 0) ctrl->restart_pending = 0;
+        return;
+    }
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nb_rows; b += blockDim.x)
+        for (int k = 0; k < 4; ++k) acc[k] += d.partials[(size_t)b * 4 + k];
+    for (int b = threadIdx.x; b < nb_cols; b += blockDim.x)
+        for (int k = 0; k < 4; ++k) acc[k] += d.partials[(size_t)(PDLP_PARTIAL_BLOCKS + b) * 4 + k];
+    block_sum<4>(acc, lds);
+    if (threadIdx.x != 0) return;
+    const double rp = sqrt(acc[0]) / ctrl->norm_b;
+    const double rd = sqrt(acc[1]) / ctrl->norm_c;
+    const double po = acc[2], dobj = acc[3];
+    const double gap = fabs(po - dobj) / (1.0 + fabs(po) + fabs(dobj));
+    const double err = fmax(rp, fmax(rd, gap));
+    ctrl->rp = rp;
+    ctrl->rd = rd;
+    ctrl->gap = gap;
+    ctrl->p_obj = po;
+    ctrl->d_obj = dobj;
+    ctrl->last_err = err;
+    ctrl->restart_pending = 0;
+    if (!isfinite(err) || !isfinite(po) || !isfinite(dobj)) {
+        ctrl->status = PDLP_NUMERICAL;
+        ctrl->done_iter = ctrl->iter;
+        return;
+    }
+    if (mode == 0) {
+        ctrl->ref_err = err;
+        return;
+    }
+    ctrl->checks += 1;
+    if (err < ctrl->tol) {                       // pdhg.py:231-235
+        ctrl->status = PDLP_OPTIMAL;
+        ctrl->done_iter = ctrl->iter;
+        ctrl->restart_pending = 1;
+        ctrl->apply_window = ctrl->window;
+    } else if (err < 0.5 * ctrl->ref_err) {      // pdhg.py:238-244
+        ctrl->restart_pending = 1;
+        ctrl->apply_window = ctrl->window;
+        ctrl->window = 0;
+        ctrl->ref_err = err;
+        ctrl->restarts += 1;
+    }
+}
+
+// Restart to (or, on convergence, finish at) the window average.
+template <typename T>
+__global__ void __launch_bounds__(256) apply_kernel(PdlpDev d) {
+    const PdlpCtrl* ctrl = d.ctrl;
+    if (!ctrl->restart_pending) return;
+    const bool keep_sums = ctrl->status == PDLP_OPTIMAL;   // host may read the FP64 average
+    const double w = (double)ctrl->apply_window;
+    T* x = (T*)d.x;
+    T* y = (T*)d.y;
+    const int stride = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < d.n; j += stride) {
+        x[j] = (T)(d.x_sum[j] / w);
+        if (!keep_sums) d.x_sum[j] = 0.0;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.m; i += stride) {
+        y[i] = (T)(d.y_sum[i] / w);
+        if (!keep_sums) d.y_sum[i] = 0.0;
+    }
+}
+
+__global__ void init_kernel(PdlpDev d) {
+    const int stride = gridDim.x * blockDim.x;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < d.n; j += stride) d.x_sum[j] = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.m; i += stride) d.y_sum[i] = 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        PdlpCtrl* c = d.ctrl;
+        c->iter = 0;
+        c->window = 0;
+        c->status = PDLP_RUNNING;
+        c->restarts = 0;
+        c->done_iter = 0;
+        c->restart_pending = 0;
+        c->apply_window = 0;
+        c->checks = 0;
+    }
+}
+
+// ------------------------------------------------------------ launch helpers
+bool valid_tpr(int t) { return t == 1 || t == 2 || t == 4 || t == 8 || t == 16 || t == 32 || t == 64; }
+
+int check_dev(const PdlpDev* d) {
+    if (d == nullptr) return fail(PDLP_E_ARG, "PdlpDev is NULL");
+    if (d->prec != PDLP_F32 && d->prec != PDLP_F64) return fail(PDLP_E_ARG, "prec must be PDLP_F32 or PDLP_F64");
+    if (d->m <= 0 || d->n <= 0 || d->nnz < 0) return fail(PDLP_E_SHAPE, "m and n must be positive");
+    if (!d->row_ptr || !d->col_idx || !d->val || !d->t_row_ptr || !d->t_col_idx || !d->t_val || !d->c ||
+        !d->b || !d->c64 || !d->b64 || !d->b_shift || !d->x || !d->xbar || !d->y || !d->x_sum ||
+        !d->y_sum || !d->partials || !d->ctrl)
+        return fail(PDLP_E_ARG, "a required PdlpDev pointer is NULL");
+    return 0;
+}
+
+int check_tune(const PdlpTune* t) {
+    if (t == nullptr) return fail(PDLP_E_ARG, "PdlpTune is NULL");
+    if (!valid_tpr(t->tpr_primal) || !valid_tpr(t->tpr_dual))
+        return fail(PDLP_E_TUNE, "tpr must be one of 1,2,4,8,16,32,64");
+    if (t->block < 64 || t->block > 1024 || (t->block & 63))
+        return fail(PDLP_E_TUNE, "block must be a multiple of 64 in [64, 1024]");
+    if (t->check_every <= 0) return fail(PDLP_E_TUNE, "check_every must be positive");
+    if (t->exact && (t->tpr_primal != 1 || t->tpr_dual != 1))
+        return fail(PDLP_E_TUNE, "exact mode needs tpr_primal = tpr_dual = 1");
+    return 0;
+}
+
+inline unsigned grid_for(int rows, int tpr, int block, int remap) {
+    const long groups = block / tpr;
+    long g = (rows + groups - 1) / groups;
+    if (g < 1) g = 1;
+    if (remap) g = (g + 7) / 8 * 8;
+    return (unsigned)g;
+}
+
+template <typename T, bool EXACT>
+int launch_primal_t(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
+    const int remap = t.xcd_remap & 1;
+    const unsigned g = grid_for(d.n, t.tpr_primal, t.block, remap);
+    if constexpr (EXACT) {
+        primal_kernel<T, 1, true><<<g, t.block, 0, s>>>(d, remap);
+    } else {
+        switch (t.tpr_primal) {
+            case 1: primal_kernel<T, 1, false><<<g, t.block, 0, s>>>(d, remap); break;
+            case 2: primal_kernel<T, 2, false><<<g, t.block, 0, s>>>(d, remap); break;
+            case 4: primal_kernel<T, 4, false><<<g, t.block, 0, s>>>(d, remap); break;
+            case 8: primal_kernel<T, 8, false><<<g, t.block, 0, s>>>(d, remap); break;
+            case 16: primal_kernel<T, 16, false><<<g, t.block, 0, s>>>(d, remap); break;
+            case 32: primal_kernel<T, 32, false><<<g, t.block, 0, s>>>(d, remap); break;
+            default: primal_kernel<T, 64, false><<<g, t.block, 0, s>>>(d, remap); break;
+        }
+    }
+    return hip_fail(hipGetLastError(), "primal_kernel launch");
+}
+
+template <typename T, bool EXACT>
+int launch_dual_t(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
+    const int remap = (t.xcd_remap >> 1) & 1;
+    const unsigned g = grid_for(d.m, t.tpr_dual, t.block, remap);
+    if constexpr (EXACT) {
+        dual_kernel<T, 1, true><<<g, t.block, 0, s>>>(d, remap);
+    } else {
+        switch (t.tpr_dual) {
+            case 1: dual_kernel<T, 1, false><<<g, t.block, 0, s>>>(d, remap); break;
+            case 2: dual_kernel<T, 2, false><<<g, t.block, 0, s>>>(d, remap); break;
+            case 4: dual_kernel<T, 4, false><<<g, t.block, 0, s>>>(d, remap); break;
+            case 8: dual_kernel<T, 8, false><<<g, t.block, 0, s>>>(d, remap); break;
+            case 16: dual_kernel<T, 16, false><<<g, t.block, 0, s>>>(d, remap); break;
+            case 32: dual_kernel<T, 32, false><<<g, t.block, 0, s>>>(d, remap); break;
+            default: dual_kernel<T, 64, false><<<g, t.block, 0, s>>>(d, remap); break;
+        }
+    }
+    return hip_fail(hipGetLastError(), "dual_kernel launch");
+}
+
+template <typename T>
+int launch_kkt_t(const PdlpDev& d, const PdlpTune& t, int src, int mode, hipStream_t s) {
+    constexpr int B = 256;
+    auto nb = [](int rows, int tpr) {
+        long g = (rows + (B / tpr) - 1) / (B / tpr);
+        return (int)std::max(1L, std::min<long>(g, PDLP_PARTIAL_BLOCKS));
+    };
+    const int nr = nb(d.m, t.tpr_dual), nc = nb(d.n, t.tpr_primal);
+    switch (t.tpr_dual) {
+        case 1: kkt_rows_kernel<T, 1><<<nr, B, 0, s>>>(d, src); break;
+        case 2: kkt_rows_kernel<T, 2><<<nr, B, 0, s>>>(d, src); break;
+        case 4: kkt_rows_kernel<T, 4><<<nr, B, 0, s>>>(d, src); break;
+        case 8: kkt_rows_kernel<T, 8><<<nr, B, 0, s>>>(d, src); break;
+        case 16: kkt_rows_kernel<T, 16><<<nr, B, 0, s>>>(d, src); break;
+        case 32: kkt_rows_kernel<T, 32><<<nr, B, 0, s>>>(d, src); break;
+        default: kkt_rows_kernel<T, 64><<<nr, B, 0, s>>>(d, src); break;
+    }
+    switch (t.tpr_primal) {
+        case 1: kkt_cols_kernel<T, 1><<<nc, B, 0, s>>>(d, src); break;
+        case 2: kkt_cols_kernel<T, 2><<<nc, B, 0, s>>>(d, src); break;
+        case 4: kkt_cols_kernel<T, 4><<<nc, B, 0, s>>>(d, src); break;
+        case 8: kkt_cols_kernel<T, 8><<<nc, B, 0, s>>>(d, src); break;
+        case 16: kkt_cols_kernel<T, 16><<<nc, B, 0, s>>>(d, src); break;
+        case 32: kkt_cols_kernel<T, 32><<<nc, B, 0, s>>>(d, src); break;
+        default: kkt_cols_kernel<T, 64><<<nc, B, 0, s>>>(d, src); break;
+    }
+    kkt_finalize_kernel<<<1, B, 0, s>>>(d, nr, nc, mode);
+    if (mode == 1) {
+        const int work = std::max(d.n, d.m);
+        const int g = std::min((work + B - 1) / B, 4096);
+        apply_kernel<T><<<g, B, 0, s>>>(d);
+    }
+    return hip_fail(hipGetLastError(), "kkt kernels launch");
+}
+
+int primal(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
+    if (d.prec == PDLP_F64)
+        return t.exact ? launch_primal_t<double, true>(d, t, s) : launch_primal_t<double, false>(d, t, s);
+    return t.exact ? launch_primal_t<float, true>(d, t, s) : launch_primal_t<float, false>(d, t, s);
+}
+
+int dual(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
+    if (d.prec == PDLP_F64)
+        return t.exact ? launch_dual_t<double, true>(d, t, s) : launch_dual_t<double, false>(d, t, s);
+    return t.exact ? launch_dual_t<float, true>(d, t, s) : launch_dual_t<float, false>(d, t, s);
+}
+
+int kkt(const PdlpDev& d, const PdlpTune& t, int src, int mode, hipStream_t s) {
+    return d.prec == PDLP_F64 ? launch_kkt_t<double>(d, t, src, mode, s)
+                              : launch_kkt_t<float>(d, t, src, mode, s);
+}
+
+int run(const PdlpDev& d, const PdlpTune& t, int iters, int check, hipStream_t s) {
+    for (int k = 0; k < iters; ++k) {
+        int rc = primal(d, t, s);
+        if (rc) return rc;
+        rc = dual(d, t, s);
+        if (rc) return rc;
+    }
+    return check ? kkt(d, t, 1, 1, s) : 0;
+}
+
+}  // namespace
+
+}  // namespace pdlp
+
+using namespace pdlp;
+
+extern "C" {
+
+int pdlp_init(const PdlpDev* dev, const PdlpTune* tune, void* stream) {
+    int rc = check_dev(dev);
+    if (rc) return rc;
+    if ((rc = check_tune(tune))) return rc;
+    hipStream_t s = as_stream(stream);
+    const int work = std::max(dev->n, dev->m);
+    init_kernel<<<std::min((work + 255) / 256, 4096), 256, 0, s>>>(*dev);
+    if ((rc = hip_fail(hipGetLastError(), "init_kernel launch"))) return rc;
+    return kkt(*dev, *tune, 0, 0, s);
+}
+
+int pdlp_primal_step(const PdlpDev* dev, const PdlpTune* tune, void* stream) {
+    int rc = check_dev(dev);
+    if (rc || (rc = check_tune(tune))) return rc;
+    return primal(*dev, *tune, as_stream(stream));
+}
+
+int pdlp_dual_step(const PdlpDev* dev, const PdlpTune* tune, void* stream) {
+    int rc = check_dev(dev);
+    if (rc || (rc = check_tune(tune))) return rc;
+    return dual(*dev, *tune, as_stream(stream));
+}
+
+int pdlp_kkt_check(const PdlpDev* dev, const PdlpTune* tune, void* stream) {
+    int rc = check_dev(dev);
+    if (rc || (rc = check_tune(tune))) return rc;
+    return kkt(*dev, *tune, 1, 1, as_stream(stream));
+}
+
+int pdlp_run(const PdlpDev* dev, const PdlpTune* tune, int32_t iters, int32_t check, void* stream) {
+    int rc = check_dev(dev);
+    if (rc || (rc = check_tune(tune))) return rc;
+    if (iters < 0) return fail(PDLP_E_ARG, "iters must be >= 0");
+    return run(*dev, *tune, iters, check, as_stream(stream));
+}
+
+int pdlp_graph_capture(const PdlpDev* dev, const PdlpTune* tune, int32_t blocks, void* stream, void** exec) {
+    int rc = check_dev(dev);
+    if (rc || (rc = check_tune(tune))) return rc;
+    if (blocks <= 0 || exec == nullptr) return fail(PDLP_E_ARG, "blocks must be positive and exec non-NULL");
+    if (stream == nullptr) return fail(PDLP_E_ARG, "graph capture needs a non-default stream");
+    hipStream_t s = as_stream(stream);
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+    for (int k = 0; k < blocks && rc == 0; ++k) rc = run(*dev, *tune, tune->check_every, 1, s);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    *exec = (void*)ge;
+    return 0;
+}
+
+int pdlp_graph_launch(void* exec, void* stream) {
+    if (exec == nullptr) return fail(PDLP_E_ARG, "graph exec is NULL");
+    return hip_fail(hipGraphLaunch((hipGraphExec_t)exec, as_stream(stream)), "hipGraphLaunch");
+}
+
+int pdlp_graph_destroy(void* exec) {
+    if (exec == nullptr) return 0;
+    return hip_fail(hipGraphExecDestroy((hipGraphExec_t)exec), "hipGraphExecDestroy");
+}
+
+}  // extern "C"

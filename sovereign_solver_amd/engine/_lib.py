@@ -1,0 +1,118 @@
+"""ctypes binding of libpdlp_hip.so (declared in include/pdlp_hip.h).
+
+The library is built in-tree (``sovereign_solver_amd/_lib/libpdlp_hip.so``).
+There is no fallback: if it is missing or fails to load, :func:`lib` raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "_lib", "libpdlp_hip.so")
+
+PDLP_ABI_VERSION = 1
+PDLP_F32, PDLP_F64 = 0, 1
+PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL = 0, 1, 3
+PDLP_ROW_EQ, PDLP_ROW_LE = 0, 1
+PDLP_PARTIAL_BLOCKS = 1024
+PDLP_PARTIALS_LEN = 2 * PDLP_PARTIAL_BLOCKS * 4
+
+_p = C.c_void_p
+_i32p = C.c_void_p
+
+
+class PdlpCtrl(C.Structure):
+    _fields_ = [
+        ("iter", C.c_int32), ("window", C.c_int32), ("status", C.c_int32), ("restarts", C.c_int32),
+        ("done_iter", C.c_int32), ("restart_pending", C.c_int32), ("apply_window", C.c_int32),
+        ("checks", C.c_int32),
+        ("ref_err", C.c_double), ("last_err", C.c_double), ("rp", C.c_double), ("rd", C.c_double),
+        ("gap", C.c_double), ("tol", C.c_double), ("norm_b", C.c_double), ("norm_c", C.c_double),
+        ("tau_scale", C.c_double), ("sigma_scale", C.c_double), ("p_obj", C.c_double), ("d_obj", C.c_double),
+    ]
+
+
+class PdlpDev(C.Structure):
+    _fields_ = [
+        ("m", C.c_int32), ("n", C.c_int32), ("nnz", C.c_int32), ("prec", C.c_int32),
+        ("row_ptr", _p), ("col_idx", _p), ("val", _p),
+        ("t_row_ptr", _p), ("t_col_idx", _p), ("t_val", _p),
+        ("c", _p), ("lo", _p), ("hi", _p), ("b", _p), ("row_kind", _p),
+        ("c64", _p), ("b64", _p), ("b_shift", _p), ("lo_shift", _p), ("ub_span", _p),
+        ("x", _p), ("xbar", _p), ("y", _p), ("x_sum", _p), ("y_sum", _p),
+        ("partials", _p), ("ctrl", _p),
+    ]
+
+
+class PdlpTune(C.Structure):
+    _fields_ = [
+        ("tpr_primal", C.c_int32), ("tpr_dual", C.c_int32), ("block", C.c_int32),
+        ("check_every", C.c_int32), ("xcd_remap", C.c_int32), ("exact", C.c_int32),
+    ]
+
+
+class PdlpError(RuntimeError):
+    pass
+
+
+_lock = threading.Lock()
+_lib = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "pdlp_abi_version": (C.c_int, []),
+    "pdlp_last_error": (C.c_char_p, []),
+    "pdlp_spmv_csr": (C.c_int, [C.c_int32, _p, _p, _p, _p, _p, C.c_int32, C.c_int32, _p]),
+    "pdlp_spmv_csr_f64acc": (C.c_int, [_p, _p, _p, _p, _p, C.c_int32, C.c_int32, _p]),
+    "pdlp_init": (C.c_int, [C.POINTER(PdlpDev), C.POINTER(PdlpTune), _p]),
+    "pdlp_primal_step": (C.c_int, [C.POINTER(PdlpDev), C.POINTER(PdlpTune), _p]),
+    "pdlp_dual_step": (C.c_int, [C.POINTER(PdlpDev), C.POINTER(PdlpTune), _p]),
+    "pdlp_kkt_check": (C.c_int, [C.POINTER(PdlpDev), C.POINTER(PdlpTune), _p]),
+    "pdlp_run": (C.c_int, [C.POINTER(PdlpDev), C.POINTER(PdlpTune), C.c_int32, C.c_int32, _p]),
+    "pdlp_graph_capture": (C.c_int, [C.POINTER(PdlpDev), C.POINTER(PdlpTune), C.c_int32, _p,
+                                     C.POINTER(C.c_void_p)]),
+    "pdlp_graph_launch": (C.c_int, [_p, _p]),
+    "pdlp_graph_destroy": (C.c_int, [_p]),
+}
+
+# Extra entry points registered by other host modules (batched engine, MLP).
+EXTRA_SIGS: dict = {}
+
+
+def _bind(L, name, sig):
+    fn = getattr(L, name)
+    fn.restype, fn.argtypes = sig
+
+
+def lib():
+    """The loaded library; raises PdlpError if it is missing or stale."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise PdlpError(f"{LIB_PATH} is missing: build it with "
+                            "`python -m sovereign_solver_amd.csrc.build` (this engine has no CPU path)")
+        L = C.CDLL(LIB_PATH)
+        for name, sig in {**_SIGS, **EXTRA_SIGS}.items():
+            _bind(L, name, sig)
+        v = L.pdlp_abi_version()
+        if v != PDLP_ABI_VERSION:
+            raise PdlpError(f"libpdlp_hip.so ABI {v} != expected {PDLP_ABI_VERSION}; rebuild it")
+        _lib = L
+        return L
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().pdlp_last_error().decode(errors="replace")
+        raise PdlpError(f"{what or 'pdlp call'} failed ({rc}): {msg}")
+
+
+def exported_symbols() -> list:
+    return sorted({**_SIGS, **EXTRA_SIGS}.keys())

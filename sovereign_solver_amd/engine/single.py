@@ -1,0 +1,238 @@
+"""Single-instance PDHG engine: one LP resident in HBM, iterated by HIP kernels.
+
+The host enqueues blocks of ``check_every`` iterations (each block = the
+primal and dual kernels ``check_every`` times, then the FP64 KKT check with
+the device-side termination/restart decision), captured in HIP graphs of
+1, 2, 4, … blocks.  It reads back only the control block (~128 bytes) after
+each graph, one graph behind, so the GPU always has the next graph queued.
+After convergence the kernels return immediately, so the at most one graph
+of surplus work costs little and the reported iteration count is exact.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .device import dev_tensor, ptr, require_gpu, stream_handle, struct_from_bytes
+from .forms import HostForm
+from .tune import Tune, heuristic_tune
+
+_STATUS = {L.PDLP_RUNNING: "iteration_limit", L.PDLP_OPTIMAL: "optimal", L.PDLP_NUMERICAL: "numerical_error"}
+
+MAX_GRAPH_BLOCKS = 16
+
+
+class SingleEngine:
+    """Device copy of one :class:`HostForm` plus its PDHG state."""
+
+    def __init__(self, form: HostForm, precision: str = "fp32", tune: Optional[Tune] = None,
+                 check_every: int = 50, exact: bool = False, use_graphs: bool = True):
+        if precision not in ("fp32", "fp64"):
+            raise ValueError("precision must be 'fp32' or 'fp64'")
+        self.lib = L.lib()
+        self.device = require_gpu()
+        self.form = form
+        self.precision = precision
+        self.m, self.n, self.nnz = form.m, form.n, form.nnz
+        self.tune = tune or heuristic_tune(self.m, self.n, self.nnz, check_every, exact)
+        if exact and (self.tune.tpr_primal != 1 or self.tune.tpr_dual != 1 or not self.tune.exact):
+            raise ValueError("exact mode needs Tune(tpr_primal=1, tpr_dual=1, exact=1)")
+        self.use_graphs = use_graphs
+        self.stream = torch.cuda.Stream(device=self.device)
+        T = torch.float32 if precision == "fp32" else torch.float64
+        self.dtype = T
+        dv = self.device
+        f64 = torch.float64
+        i32 = torch.int32
+        with torch.cuda.stream(self.stream):
+            K, KT = form.K, form.KT
+            self.row_ptr = dev_tensor(K.indptr, i32, dv)
+            self.col_idx = dev_tensor(K.indices, i32, dv)
+            self.val = dev_tensor(K.data, T, dv)
+            self.t_row_ptr = dev_tensor(KT.indptr, i32, dv)
+            self.t_col_idx = dev_tensor(KT.indices, i32, dv)
+            self.t_val = dev_tensor(KT.data, T, dv)
+            self.c = dev_tensor(form.c, T, dv)
+            self.lo = None if form.lo is None else dev_tensor(form.lo, T, dv)
+            self.hi = None if form.hi is None else dev_tensor(form.hi, T, dv)
+            self.b = dev_tensor(form.b, T, dv)
+            self.row_kind = None if form.row_kind is None else dev_tensor(form.row_kind, torch.int8, dv)
+            self.c64 = dev_tensor(form.c, f64, dv)
+            self.b64 = dev_tensor(form.b, f64, dv)
+            self.b_shift = dev_tensor(form.b_shift, f64, dv)
+            self.lo_shift = None if form.lo_shift is None else dev_tensor(form.lo_shift, f64, dv)
+            self.ub_span = None if form.ub_span is None else dev_tensor(form.ub_span, f64, dv)
+            self.x = torch.zeros(self.n, dtype=T, device=dv)
+            self.xbar = torch.zeros(self.n, dtype=T, device=dv)
+            self.y = torch.zeros(self.m, dtype=T, device=dv)
+            self.x_sum = torch.zeros(self.n, dtype=f64, device=dv)
+            self.y_sum = torch.zeros(self.m, dtype=f64, device=dv)
+            self.partials = torch.zeros(L.PDLP_PARTIALS_LEN, dtype=f64, device=dv)
+            self.ctrl = torch.zeros(C.sizeof(L.PdlpCtrl), dtype=torch.uint8, device=dv)
+        self.ctrl_host = torch.zeros((2, C.sizeof(L.PdlpCtrl)), dtype=torch.uint8).pin_memory()
+        self.stream.synchronize()
+        self.dev = L.PdlpDev(
+            m=self.m, n=self.n, nnz=self.nnz, prec=L.PDLP_F32 if precision == "fp32" else L.PDLP_F64,
+            row_ptr=ptr(self.row_ptr), col_idx=ptr(self.col_idx), val=ptr(self.val),
+            t_row_ptr=ptr(self.t_row_ptr), t_col_idx=ptr(self.t_col_idx), t_val=ptr(self.t_val),
+            c=ptr(self.c), lo=ptr(self.lo), hi=ptr(self.hi), b=ptr(self.b), row_kind=ptr(self.row_kind),
+            c64=ptr(self.c64), b64=ptr(self.b64), b_shift=ptr(self.b_shift), lo_shift=ptr(self.lo_shift),
+            ub_span=ptr(self.ub_span), x=ptr(self.x), xbar=ptr(self.xbar), y=ptr(self.y),
+            x_sum=ptr(self.x_sum), y_sum=ptr(self.y_sum), partials=ptr(self.partials), ctrl=ptr(self.ctrl))
+        self._graphs = {}
+        self._graph_tune = None
+
+    # ------------------------------------------------------------------ utils
+    @property
+    def sh(self):
+        return stream_handle(self.stream)
+
+    def _ctune(self, t: Optional[Tune] = None):
+        return (t or self.tune).to_c()
+
+    def set_tune(self, t: Tune) -> None:
+        self.tune = t
+        self.drop_graphs()
+
+    def drop_graphs(self) -> None:
+        for g in self._graphs.values():
+            self.lib.pdlp_graph_destroy(g)
+        self._graphs = {}
+
+    def __del__(self):
+        try:
+            self.drop_graphs()
+        except Exception:
+            pass
+
+    def _graph(self, blocks: int):
+        g = self._graphs.get(blocks)
+        if g is None:
+            h = C.c_void_p()
+            ct = self._ctune()
+            L.check(self.lib.pdlp_graph_capture(C.byref(self.dev), C.byref(ct), blocks, self.sh, C.byref(h)),
+                    "pdlp_graph_capture")
+            g = self._graphs[blocks] = h
+        return g
+
+    def read_ctrl(self) -> L.PdlpCtrl:
+        with torch.cuda.stream(self.stream):
+            self.ctrl_host[0].copy_(self.ctrl, non_blocking=True)
+        self.stream.synchronize()
+        return struct_from_bytes(L.PdlpCtrl, self.ctrl_host[0])
+
+    def write_ctrl(self, st: L.PdlpCtrl) -> None:
+        raw = torch.frombuffer(bytearray(bytes(st)), dtype=torch.uint8)
+        with torch.cuda.stream(self.stream):
+            self.ctrl.copy_(raw.to(self.device, non_blocking=False))
+
+    # ------------------------------------------------------------------ solve
+    def start(self, tol: float, x0=None, y0=None, tau_scale: float = 1.0, sigma_scale: float = 1.0) -> None:
+        with torch.cuda.stream(self.stream):
+            if x0 is None:
+                self.x.zero_()
+            else:
+                self.x.copy_(torch.as_tensor(np.asarray(x0), dtype=self.dtype).to(self.device))
+            if y0 is None:
+                self.y.zero_()
+            else:
+                self.y.copy_(torch.as_tensor(np.asarray(y0), dtype=self.dtype).to(self.device))
+        st = L.PdlpCtrl()
+        st.tol = tol
+        st.norm_b = self.form.norm_b
+        st.norm_c = self.form.norm_c
+        st.tau_scale = tau_scale
+        st.sigma_scale = sigma_scale
+        self.write_ctrl(st)
+        ct = self._ctune()
+        L.check(self.lib.pdlp_init(C.byref(self.dev), C.byref(ct), self.sh), "pdlp_init")
+
+    def run_iterations(self, max_iters: int) -> L.PdlpCtrl:
+        """Enqueue up to ``max_iters`` iterations; stop early once the device leaves RUNNING."""
+        ce = self.tune.check_every
+        done = 0
+        blocks = 1
+        inflight = []          # (slot, event)
+        slot = 0
+        final = None
+        ct = self._ctune()
+        while True:
+            left = (max_iters - done) // ce
+            if left == 0:
+                break
+            g = min(blocks, left)
+            with torch.cuda.stream(self.stream):
+                if self.use_graphs:
+                    if g & (g - 1):                  # not a power of two: largest power below
+                        g = 1 << (g.bit_length() - 1)
+                    L.check(self.lib.pdlp_graph_launch(self._graph(g), self.sh), "pdlp_graph_launch")
+                else:
+                    for _ in range(g):
+                        L.check(self.lib.pdlp_run(C.byref(self.dev), C.byref(ct), ce, 1, self.sh), "pdlp_run")
+                self.ctrl_host[slot].copy_(self.ctrl, non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(self.stream)
+            done += g * ce
+            inflight.append((slot, ev))
+            slot ^= 1
+            blocks = min(blocks * 2, MAX_GRAPH_BLOCKS)
+            if len(inflight) == 2:
+                s0, e0 = inflight.pop(0)
+                e0.synchronize()
+                st = struct_from_bytes(L.PdlpCtrl, self.ctrl_host[s0])
+                if st.status != L.PDLP_RUNNING:
+                    final = st
+                    break
+        if final is None:
+            tail = max_iters - done
+            if tail > 0:
+                with torch.cuda.stream(self.stream):
+                    L.check(self.lib.pdlp_run(C.byref(self.dev), C.byref(ct), tail, 0, self.sh), "pdlp_run")
+        self.stream.synchronize()
+        return self.read_ctrl()
+
+    def solution(self, st: L.PdlpCtrl):
+        """Host copies of (x, y): the FP64 window average at convergence, else the last iterate."""
+        with torch.cuda.stream(self.stream):
+            if st.status == L.PDLP_OPTIMAL:
+                w = float(st.apply_window)
+                x = (self.x_sum / w).cpu()
+                y = (self.y_sum / w).cpu()
+            else:
+                x = self.x.to(torch.float64).cpu()
+                y = self.y.to(torch.float64).cpu()
+        self.stream.synchronize()
+        return x.numpy(), y.numpy()
+
+    def solve(self, tol: float, max_iters: int, x0=None, y0=None, tau_scale=1.0, sigma_scale=1.0):
+        self.start(tol, x0, y0, tau_scale, sigma_scale)
+        st = self.run_iterations(max_iters)
+        x, y = self.solution(st)
+        iters = st.done_iter if st.status != L.PDLP_RUNNING else st.iter
+        return dict(status=_STATUS[st.status], x=x, y=y, iterations=int(iters), rel_kkt=st.last_err,
+                    rp=st.rp, rd=st.rd, gap=st.gap, restarts=int(st.restarts), ctrl=st)
+
+    # --------------------------------------------------------------- timing
+    def time_kernel(self, which: str, t: Tune, reps: int = 20) -> float:
+        """Mean µs per launch of one step kernel under tuning ``t`` (HIP events on this stream)."""
+        ct = t.to_c()
+        fn = {"primal": self.lib.pdlp_primal_step, "dual": self.lib.pdlp_dual_step}[which]
+        st = L.PdlpCtrl()
+        st.tol = 0.0
+        st.norm_b, st.norm_c, st.tau_scale, st.sigma_scale = self.form.norm_b, self.form.norm_c, 1.0, 1.0
+        self.write_ctrl(st)
+        with torch.cuda.stream(self.stream):
+            for _ in range(3):
+                L.check(fn(C.byref(self.dev), C.byref(ct), self.sh), which)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(self.stream)
+            for _ in range(reps):
+                L.check(fn(C.byref(self.dev), C.byref(ct), self.sh), which)
+            e1.record(self.stream)
+        self.stream.synchronize()
+        return e0.elapsed_time(e1) * 1000.0 / reps

@@ -1,0 +1,103 @@
+"""Drop-in for the reference's ``solve_lp_pdhg`` (``solver/lp/pdhg.py:152-259``).
+
+Same call and result: ``solve_lp_pdhg(problem, max_iters=MAX_PDHG_ITERS,
+tol=PDHG_TOL) -> SolveResult`` with ``status in {"optimal",
+"iteration_limit"}``, ``x`` in the original variable space clipped to the
+bounds, the objective in the original sense and the iteration count.  The
+work runs in HIP kernels on the GPU (there is no CPU path: without a HIP
+device or the built library the call raises).
+
+Keyword-only extensions (defaults keep the reference's contract):
+
+``formulation``
+    ``"box"`` (default): the engine's native form, bounds by projection,
+    ``<=`` rows by a sign-constrained dual; ``"reference"``: the reference's
+    slack/upper-bound-row standard form, for trajectory-level parity.
+``precision``
+    ``"fp32"`` iterates and SpMV with FP64 sums, KKT and objective, or
+    ``"fp64"`` throughout; ``"auto"`` (default) picks fp32 when
+    ``tol >= 1e-5`` and fp64 below, since FP32 iterates stall near 1e-6.
+``exact``
+    With ``formulation="reference", precision="fp64"``: one lane per row,
+    ascending summation and no FMA contraction, i.e. SciPy's ``csr_matvec``
+    arithmetic; the iterates then match the reference's bit for bit up to
+    the KKT reductions (whose order differs at the 1e-16 level).
+``x0, y0, step_scale``
+    Warm start (box form: x in the original space, y ordered ``[A_ub; A_eq]``)
+    and a primal weight ω: τ ← ω·τ, σ ← σ/ω (ω = 1 is the reference's step).
+``check_every``, ``tune``
+    KKT/restart cadence (50 = reference) and launch shape (default: the
+    autotuner's heuristic).
+
+The reference returns no duals and no residuals; the result additionally
+carries ``y``, ``rel_kkt``, ``rp``, ``rd``, ``gap`` and ``restarts``.
+"""
+
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+
+from ..config import MAX_PDHG_ITERS, PDHG_TOL
+from ..engine.forms import extract, make_form
+from ..engine.single import SingleEngine
+from ..engine.tune import Tune, heuristic_tune
+from ..problem import Problem
+from ..result import SolveResult
+
+
+def _precision(precision: str, tol: float) -> str:
+    if precision == "auto":
+        return "fp32" if tol >= 1e-5 else "fp64"
+    if precision not in ("fp32", "fp64"):
+        raise ValueError("precision must be 'auto', 'fp32' or 'fp64'")
+    return precision
+
+
+def _warm_start(form, p: Problem, x0, y0):
+    if x0 is None and y0 is None:
+        return None, None
+    if form.kind != "box":
+        raise ValueError("warm starts are supported with formulation='box'")
+    x = None if x0 is None else np.clip(np.asarray(x0, float), p.lb, p.ub)
+    y = None
+    if y0 is not None:
+        y = np.asarray(y0, float).copy()
+        if form.m_ub:
+            y[: form.m_ub] = np.minimum(y[: form.m_ub], 0.0)
+    return x, y
+
+
+def solve_lp_pdhg(
+    problem: Problem,
+    max_iters: int = MAX_PDHG_ITERS,
+    tol: float = PDHG_TOL,
+    *,
+    formulation: str = "box",
+    precision: str = "auto",
+    exact: bool = False,
+    x0: Optional[np.ndarray] = None,
+    y0: Optional[np.ndarray] = None,
+    step_scale: float = 1.0,
+    check_every: int = 50,
+    tune: Optional[Tune] = None,
+    use_graphs: bool = True,
+) -> SolveResult:
+    form = make_form(problem, formulation)
+    if form.m == 0 or form.n == 0:                      # pdhg.py:171-172
+        return SolveResult("optimal", np.zeros(problem.n_vars), 0.0, 0)
+    prec = _precision(precision, tol)
+    if exact:
+        tune = Tune(1, 1, 256, check_every, 0, 1)
+    elif tune is None:
+        tune = heuristic_tune(form.m, form.n, form.nnz, check_every)
+    eng = SingleEngine(form, prec, tune=tune, check_every=check_every, exact=exact, use_graphs=use_graphs)
+    try:
+        xs, ys = _warm_start(form, problem, x0, y0)
+        r = eng.solve(tol, max_iters, xs, ys, tau_scale=step_scale, sigma_scale=1.0 / step_scale)
+    finally:
+        eng.drop_graphs()
+    x, obj, y = extract(form, problem, r["x"], r["y"])
+    return SolveResult(r["status"], x, obj, r["iterations"], y=y, rel_kkt=r["rel_kkt"], rp=r["rp"],
+                       rd=r["rd"], gap=r["gap"], restarts=r["restarts"])

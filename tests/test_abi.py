@@ -1,0 +1,59 @@
+"""The C ABI library loads and exports every entry point the headers declare
+(no GPU needed: only version / error / argument-validation calls)."""
+
+import ctypes as C
+import glob
+import os
+import re
+
+import pytest
+
+from sovereign_solver_amd.engine import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        txt = open(h).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        names |= set(re.findall(r"\b(pdlp_\w+)\s*\(", txt))
+    return sorted(names)
+
+
+def test_library_built():
+    assert os.path.exists(L.LIB_PATH), "run __graft_entry__.build() first"
+
+
+def test_every_declared_symbol_exported_and_bound():
+    lib = L.lib()
+    decl = declared_symbols()
+    assert len(decl) >= 10
+    raw = C.CDLL(L.LIB_PATH)
+    for name in decl:
+        assert hasattr(raw, name), f"{name} declared in include/ but not exported"
+    bound = set(L.exported_symbols())
+    missing = [d for d in decl if d not in bound]
+    assert not missing, f"declared but not bound in _lib.py: {missing}"
+    assert lib.pdlp_abi_version() == L.PDLP_ABI_VERSION
+
+
+def test_struct_sizes_match_header():
+    # PdlpCtrl: 8 int32 + 12 double; PdlpDev: 4 int32 + 23 pointers; PdlpTune: 6 int32
+    assert C.sizeof(L.PdlpCtrl) == 8 * 4 + 12 * 8
+    assert C.sizeof(L.PdlpDev) == 4 * 4 + 23 * 8
+    assert C.sizeof(L.PdlpTune) == 6 * 4
+
+
+def test_argument_errors_without_device():
+    lib = L.lib()
+    rc = lib.pdlp_run(None, None, 1, 1, None)
+    assert rc == -1
+    assert b"NULL" in lib.pdlp_last_error()
+    t = L.PdlpTune(3, 1, 256, 50, 0, 0)
+    d = L.PdlpDev()
+    rc = lib.pdlp_primal_step(C.byref(d), C.byref(t), None)
+    assert rc in (-1, -2, -3)
+    with pytest.raises(L.PdlpError):
+        L.check(rc, "primal")

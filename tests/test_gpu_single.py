@@ -1,0 +1,204 @@
+"""GPU parity of the single-instance engine (needs an MI355X).
+
+Reference form, FP64, exact arithmetic: the reference's published PDHG rows
+(status, iteration count, objective to 8 digits) and the oracle trajectory.
+Box form, FP32/FP64: termination at the requested relative KKT measured on
+the host by the reference formula, objective against HiGHS (FP64 simplex).
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+from scipy.optimize import linprog
+
+from oracle import pdhg_ref, yardstick
+from sovereign_solver_amd import solve_lp_pdhg, solve_lp_pdhg_gpu, get_available_device
+from sovereign_solver_amd.engine import _lib as L
+from sovereign_solver_amd.problem import Problem
+from sovereign_solver_amd.workloads import random_sparse_lp, transportation_lp
+from tests.golden.stress_instances import golden_problem, load_published
+
+pytestmark = pytest.mark.gpu
+PUB = load_published()
+
+
+def _highs_obj(p):
+    r = linprog(p.c, A_ub=p.A_ub if p.n_ineq else None, b_ub=p.b_ub if p.n_ineq else None,
+                A_eq=p.A_eq if p.n_eq else None, b_eq=p.b_eq if p.n_eq else None,
+                bounds=list(zip(p.lb, [None if np.isinf(u) else u for u in p.ub])), method="highs-ds")
+    assert r.status == 0
+    return r.fun
+
+
+# ----------------------------------------------------------------- SpMV kernel
+@pytest.mark.parametrize("tpr", [1, 2, 4, 8, 16, 32, 64])
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_spmv_matches_scipy(gpu, tpr, prec):
+    rng = np.random.default_rng(tpr)
+    A = sp.random(3000, 1700, density=0.01, random_state=tpr, format="csr")
+    A.data = rng.standard_normal(A.nnz)
+    A.indices = A.indices.astype(np.int32)
+    A.indptr = A.indptr.astype(np.int32)
+    x = rng.standard_normal(1700)
+    dt = torch.float32 if prec == "fp32" else torch.float64
+    d = lambda a, t: torch.from_numpy(np.ascontiguousarray(a)).to(gpu, t)
+    rp, ci, v, xv = d(A.indptr, torch.int32), d(A.indices, torch.int32), d(A.data, dt), d(x, dt)
+    y = torch.zeros(3000, dtype=dt, device=gpu)
+    L.check(L.lib().pdlp_spmv_csr(L.PDLP_F32 if prec == "fp32" else L.PDLP_F64, rp.data_ptr(), ci.data_ptr(),
+                                  v.data_ptr(), xv.data_ptr(), y.data_ptr(), 3000, tpr, None), "spmv")
+    torch.cuda.synchronize()
+    got = y.double().cpu().numpy()
+    if prec == "fp64":
+        Ad, xd = A, x
+        np.testing.assert_allclose(got, Ad @ xd, rtol=0, atol=1e-12 * (abs(A) @ abs(x)).max())
+    else:
+        A32 = A.copy()
+        A32.data = A.data.astype(np.float32).astype(np.float64)
+        x32 = x.astype(np.float32).astype(np.float64)
+        k = np.diff(A.indptr).max()
+        bound = (k + 2) * 2.0**-24 * (abs(A32) @ abs(x32)) + 1e-30
+        assert np.all(np.abs(got - A32 @ x32) <= bound)
+
+
+def test_spmv_f64acc(gpu):
+    rng = np.random.default_rng(0)
+    A = sp.random(500, 800, density=0.05, random_state=0, format="csr")
+    A.data = rng.standard_normal(A.nnz).astype(np.float32).astype(np.float64)
+    x = rng.standard_normal(800)
+    d = lambda a, t: torch.from_numpy(np.ascontiguousarray(a)).to(gpu, t)
+    # every device buffer stays referenced until the kernel has finished: a
+    # temporary freed after data_ptr() is recycled by the next upload
+    y = torch.zeros(500, dtype=torch.float64, device=gpu)
+    rp = d(A.indptr.astype(np.int32), torch.int32)
+    ci = d(A.indices.astype(np.int32), torch.int32)
+    v = d(A.data, torch.float32)
+    xv = d(x, torch.float64)
+    assert rp.numel() == 501 and int(rp[-1]) == ci.numel() == v.numel() and xv.numel() == 800
+    L.check(L.lib().pdlp_spmv_csr_f64acc(rp.data_ptr(), ci.data_ptr(), v.data_ptr(), xv.data_ptr(),
+                                         y.data_ptr(), 500, 8, None), "f64acc")
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(y.cpu().numpy(), A @ x, rtol=1e-12, atol=1e-12)
+
+
+# ---------------------------------------------- reference form, exact FP64
+@pytest.mark.parametrize("name", ["blending", "afiro", "random_n50_m30", "brandy", "adlittle"])
+def test_exact_reference_form_reproduces_published_rows(gpu, name):
+    row = PUB[(name, "pdhg")]
+    r = solve_lp_pdhg(golden_problem(name), max_iters=int(row["max_iters"]), tol=1e-6,
+                      formulation="reference", precision="fp64", exact=True)
+    assert r.status == row["status"]
+    assert r.iterations == int(row["iterations"])
+    pub = float(row["objective"])
+    assert abs(r.objective - pub) <= 6e-9 * max(1.0, abs(pub)), (r.objective, pub)
+
+
+def test_exact_reference_form_sparse_n500(gpu):
+    row = PUB[("sparse_n500_m250", "pdhg")]
+    r = solve_lp_pdhg(golden_problem("sparse_n500_m250"), formulation="reference", precision="fp64", exact=True)
+    assert (r.status, r.iterations) == ("optimal", int(row["iterations"]))
+    assert abs(r.objective - float(row["objective"])) <= 6e-9 * abs(float(row["objective"]))
+
+
+def test_exact_trajectory_matches_oracle_iterates(gpu):
+    p = random_sparse_lp(60, 150, 0.05, seed=4)
+    for iters in (1, 50, 1234):
+        o = pdhg_ref.solve(p, max_iters=iters, tol=1e-12)
+        g = solve_lp_pdhg(p, max_iters=iters, tol=1e-12, formulation="reference", precision="fp64", exact=True)
+        assert g.status == o.status == "iteration_limit"
+        np.testing.assert_allclose(g.x, o.x, rtol=1e-13, atol=1e-13)
+
+
+# ------------------------------------------------------------- box form
+@pytest.mark.parametrize("seed", [0, 1])
+def test_box_fp64_random_lp(gpu, seed):
+    p = random_sparse_lp(200, 500, 0.02, seed=seed)
+    r = solve_lp_pdhg(p, tol=1e-6, precision="fp64")
+    assert r.status == "optimal"
+    err = yardstick.rel_kkt(p, r.x, r.y)[0]
+    assert err <= 1.0001e-6
+    ref = _highs_obj(p)
+    assert abs(r.objective - ref) <= 1e-4 * (1 + abs(ref))
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_box_fp32_random_lp(gpu, seed):
+    p = random_sparse_lp(200, 500, 0.02, seed=seed)
+    r = solve_lp_pdhg(p, tol=1e-4, precision="fp32")
+    assert r.status == "optimal"
+    err = yardstick.rel_kkt(p, r.x, r.y)[0]
+    assert err <= 1.02e-4, err
+    assert abs(r.rel_kkt - err) <= 1e-6 + 1e-3 * err
+    ref = _highs_obj(p)
+    assert abs(r.objective - ref) <= 1e-3 * (1 + abs(ref))
+
+
+def test_box_fp32_transport_small(gpu):
+    p = transportation_lp(50, 100, seed=0)
+    r = solve_lp_pdhg(p, tol=1e-4, precision="fp32", max_iters=200_000)
+    assert r.status == "optimal"
+    assert yardstick.rel_kkt(p, r.x, r.y)[0] <= 1.02e-4
+    ref = _highs_obj(p)
+    assert abs(r.objective - ref) <= 2e-3 * (1 + abs(ref))
+
+
+def test_graphs_and_direct_launches_agree_bitwise(gpu):
+    p = random_sparse_lp(100, 250, 0.03, seed=9)
+    a = solve_lp_pdhg(p, tol=1e-4, precision="fp32", use_graphs=True)
+    b = solve_lp_pdhg(p, tol=1e-4, precision="fp32", use_graphs=False)
+    assert a.iterations == b.iterations
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+
+
+def test_warm_start_from_solution_stops_at_first_check(gpu):
+    p = random_sparse_lp(100, 250, 0.03, seed=3)
+    a = solve_lp_pdhg(p, tol=1e-4, precision="fp64")
+    b = solve_lp_pdhg(p, tol=1e-4, precision="fp64", x0=a.x, y0=a.y)
+    assert b.status == "optimal" and b.iterations <= 100 < a.iterations
+
+
+def test_max_iters_not_multiple_of_check(gpu):
+    p = random_sparse_lp(50, 120, 0.05, seed=1)
+    o = pdhg_ref.solve(p, max_iters=137, tol=1e-12)
+    g = solve_lp_pdhg(p, max_iters=137, tol=1e-12, formulation="reference", precision="fp64", exact=True)
+    assert g.iterations == 137 and g.status == "iteration_limit"
+    np.testing.assert_allclose(g.x, o.x, rtol=1e-13, atol=1e-13)
+
+
+# ------------------------------------------------------- drop-in behaviour
+def _make(c, A_ub, b_ub):
+    A = np.array(A_ub, float)
+    n = A.shape[1]
+    return Problem(c=np.array(c, float), A_ub=sp.csr_matrix(A), b_ub=np.array(b_ub, float),
+                   A_eq=sp.csr_matrix((0, n)), b_eq=np.zeros(0), lb=np.zeros(n), ub=np.full(n, np.inf),
+                   integer_mask=np.zeros(n, bool))
+
+
+@pytest.mark.parametrize("c,A,b,expected", [
+    ([-3, -5], [[1, 0], [0, 2], [3, 2]], [4, 12, 18], -36.0),
+    ([1, 1], [[1, 0], [0, 1]], [3, 3], 0.0),
+    ([-1, -2], [[1, 1]], [4], -8.0),
+])
+def test_reference_test_lps(gpu, c, A, b, expected):
+    """The LPs of tests/test_pdhg.py:81-99 with tight checks instead of |Δ|<0.5."""
+    p = _make(c, A, b)
+    r = solve_lp_pdhg_gpu(p, device="auto")
+    assert r.status == "optimal"
+    assert abs(r.objective - expected) <= 1e-4 * (1 + abs(expected))
+    assert np.all(r.x >= -1e-12)
+
+
+def test_device_resolution(gpu):
+    assert get_available_device() == "cuda"
+    p = _make([-1.0, -2.0], [[1.0, 1.0]], [4.0])
+    with pytest.raises(RuntimeError):
+        solve_lp_pdhg_gpu(p, device="cpu")
+    with pytest.raises(ValueError):
+        solve_lp_pdhg_gpu(p, device="tpu")
+
+
+def test_empty_problem_returns_optimal_zeros(gpu):
+    r = solve_lp_pdhg(Problem.empty(3))
+    assert r.status == "optimal" and r.iterations == 0 and np.array_equal(r.x, np.zeros(3))

@@ -1,0 +1,125 @@
+"""Pin the CPU oracle before trusting it (no GPU needed).
+
+* published reference results (benchmark_comparison.csv) reproduced by the
+  FP64 restatement of solver/lp/pdhg.py;
+* closed forms of the reference standard form (pdhg.py:62-125) and the step
+  sizes (pdhg.py:131-147);
+* the yardstick lift against the plain formula and at known optima.
+"""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.optimize import linprog
+
+from oracle import pdhg_ref, yardstick
+from sovereign_solver_amd.problem import Problem
+from sovereign_solver_amd.workloads import random_sparse_lp
+from tests.golden.stress_instances import golden_problem, load_published
+
+PUB = load_published()
+
+FAST_ROWS = ["blending", "afiro", "random_n50_m30", "brandy", "adlittle"]
+
+
+@pytest.mark.parametrize("name", FAST_ROWS + ["sparse_n500_m250"])
+def test_oracle_reproduces_published_pdhg_rows(name):
+    row = PUB[(name, "pdhg")]
+    r = pdhg_ref.solve(golden_problem(name), max_iters=int(row["max_iters"]))
+    assert r.status == row["status"]
+    assert r.iterations == int(row["iterations"])
+    # the CSV prints 8 decimals
+    assert abs(r.objective - float(row["objective"])) <= 6e-9 * max(1.0, abs(float(row["objective"])))
+
+
+def test_oracle_badly_scaled_row_within_one_iteration_drift():
+    """badly_scaled_1e8 stops at the 100,000-iteration limit while the objective
+    still drifts by ~0.00998 per iteration; the published value lies between
+    the oracle's values at 100,000 and 100,001 iterations (documented
+    discrepancy of at most one iteration's drift)."""
+    p = golden_problem("badly_scaled_1e8")
+    pub = float(PUB[("badly_scaled_1e8", "pdhg")]["objective"])
+    a = pdhg_ref.solve(p, max_iters=100_000)
+    assert a.status == "iteration_limit" and a.iterations == 100_000
+    step = 0.00998
+    assert abs(a.objective - pub) <= 1.01 * step
+
+
+def test_published_pdhg_objectives_near_lp_optimum():
+    for name in ["afiro", "blending", "random_n50_m30", "sparse_n500_m250"]:
+        opt = float(PUB[(name, "lp_optimum")]["objective"])
+        got = float(PUB[(name, "pdhg")]["objective"])
+        assert abs(got - opt) <= 1e-6 * max(1.0, abs(opt))
+
+
+def _tiny_problem():
+    # min -x0 - 2 x1  s.t. x0 + x1 <= 4 ; x0 - x1 = -1 ; 0 <= x0 <= 3, 1 <= x1
+    return Problem(c=np.array([-1.0, -2.0]), A_ub=sp.csr_matrix([[1.0, 1.0]]), b_ub=np.array([4.0]),
+                   A_eq=sp.csr_matrix([[1.0, -1.0]]), b_eq=np.array([-1.0]), lb=np.array([0.0, 1.0]),
+                   ub=np.array([3.0, np.inf]), integer_mask=np.zeros(2, bool))
+
+
+def test_standard_form_closed_forms():
+    p = _tiny_problem()
+    sf = pdhg_ref.to_standard_form(p)
+    m_ub, m_eq, nF = 1, 1, 1
+    assert sf.A.shape == (m_ub + nF + m_eq, 2 + m_ub + nF)
+    assert sf.A.nnz == p.A_ub.nnz + p.A_eq.nnz + nF + (m_ub + nF)
+    # b = [b_ub - A_ub lb; ub-lb on F; b_eq - A_eq lb] with row negation where < -1e-6
+    raw = np.array([4.0 - 1.0, 3.0 - 0.0, -1.0 - (0.0 - 1.0)])
+    assert np.allclose(sf.b, np.abs(raw))
+    # lift invariant: any feasible x gives A z = b
+    for x in ([0.5, 1.5], [1.0, 2.0], [1.5, 2.5]):
+        x = np.array(x)
+        z = np.concatenate([x - sf.lb_shift, [4.0 - x.sum()], [3.0 - x[0]]])
+        assert np.allclose(sf.A @ z, sf.b)
+
+
+def test_step_size_bound_on_random_lp():
+    p = random_sparse_lp(20, 30, 0.2, seed=3)
+    sf = pdhg_ref.to_standard_form(p)
+    tau, sigma = pdhg_ref.step_sizes(sf.A)
+    M = np.sqrt(sigma)[:, None] * sf.A.toarray() * np.sqrt(tau)[None, :]
+    assert np.linalg.norm(M, 2) <= 0.999 + 1e-12
+    # closed form
+    assert np.allclose(tau, 0.999 / np.maximum(abs(sf.A).sum(axis=0).A1, 1e-4))
+
+
+def test_yardstick_identity_lift_matches_formula():
+    # no bounds, no inequality rows: the lift is the identity
+    rng = np.random.default_rng(1)
+    A = sp.random(6, 9, density=0.5, random_state=1, format="csr")
+    n = 9
+    p = Problem(c=rng.standard_normal(n), A_ub=sp.csr_matrix((0, n)), b_ub=np.zeros(0), A_eq=A,
+                b_eq=np.abs(rng.standard_normal(6)) + 0.1, lb=np.zeros(n), ub=np.full(n, np.inf),
+                integer_mask=np.zeros(n, bool))
+    x = np.abs(rng.standard_normal(n))
+    y = rng.standard_normal(6)
+    got = yardstick.rel_kkt(p, x, y)
+    sf = pdhg_ref.to_standard_form(p)
+    want = pdhg_ref.kkt(sf.A, sf.A.T.tocsr(), sf.b, sf.c, max(1, np.linalg.norm(sf.b)),
+                        max(1, np.linalg.norm(sf.c)), x, y)
+    assert np.allclose(got, want, rtol=1e-14, atol=1e-15)
+
+
+def test_yardstick_hand_computed():
+    # min x, 0 <= x <= 2, row x <= 1 (y<=0).  At x=0, y=0: r = 1, no residuals,
+    # p = 0, d = 0 -> all zero.  At x=1, y=-1: r = 2; p=1, d = b*y = -1 -> gap 2/3.
+    p = Problem(c=np.array([1.0]), A_ub=sp.csr_matrix([[1.0]]), b_ub=np.array([1.0]),
+                A_eq=sp.csr_matrix((0, 1)), b_eq=np.zeros(0), lb=np.array([0.0]), ub=np.array([2.0]),
+                integer_mask=np.zeros(1, bool))
+    assert np.allclose(yardstick.rel_kkt(p, [0.0], [0.0]), 0.0)
+    err, rp, rd, gap = yardstick.rel_kkt(p, [1.0], [-1.0])
+    assert rp == 0.0 and rd == 0.0 and np.isclose(gap, 2.0 / 3.0)
+    # infeasible row: x = 2 violates x <= 1 by 1; b_ref = [1, 2] -> norm sqrt(5)
+    err, rp, rd, gap = yardstick.rel_kkt(p, [2.0], [0.0])
+    assert np.isclose(rp, 1.0 / np.sqrt(5.0))
+
+
+def test_yardstick_zero_at_highs_optimum():
+    p = random_sparse_lp(30, 60, 0.1, seed=5)
+    r = linprog(p.c, A_eq=p.A_eq, b_eq=p.b_eq, bounds=list(zip(p.lb, p.ub)), method="highs")
+    assert r.status == 0
+    y = r.eqlin.marginals          # HiGHS: d obj / d b
+    err = yardstick.rel_kkt(p, r.x, y)[0]
+    assert err < 1e-8
