@@ -100,16 +100,20 @@ typedef struct PdlpDev {
     const double *ub_span;    /* n: hi - lo_shift where hi is finite, else +inf; NULL means +inf */
     /* state */
     void *x, *xbar, *y;       /* n, n, m; prec */
-    double *x_sum, *y_sum;    /* n, m */
+    double *x_sum, *y_sum;    /* n, m: FP64 ergodic sums of the window */
+    float  *x_psum, *y_psum;  /* n, m: PDLP_F32 only — FP32 sums of the iterations since
+                                 the last check, folded into x_sum/y_sum at every check;
+                                 NULL = add into x_sum/y_sum every iteration (PDLP_F64) */
     double *partials;         /* PDLP_PARTIALS_LEN doubles of workspace */
     PdlpCtrl *ctrl;
 } PdlpDev;
 
 /* Launch shape, chosen by the autotuner (sovereign_solver_amd.engine.tune). */
 typedef struct PdlpTune {
-    int32_t tpr_primal;   /* lanes per K^T row: 1,2,4,8,16,32,64 */
-    int32_t tpr_dual;     /* lanes per K row:   1,2,4,8,16,32,64 */
-    int32_t block;        /* threads per workgroup: multiple of 64 in [64, 1024] */
+    int32_t tpr_primal;   /* lanes per K^T row: 1,2,4,...,64, or 128/256/512/1024
+                             = one row per workgroup of that many threads */
+    int32_t tpr_dual;     /* lanes per K row, same choices */
+    int32_t block;        /* threads per workgroup for tpr <= 64: multiple of 64 in [64, 1024] */
     int32_t check_every;  /* iterations between KKT checks; 50 = reference */
     int32_t xcd_remap;    /* bit 0: primal step, bit 1: dual step — give each XCD
                              a contiguous range of rows */
@@ -164,6 +168,72 @@ int pdlp_graph_capture(const PdlpDev* dev, const PdlpTune* tune, int32_t blocks,
                        void* stream, void** exec);
 int pdlp_graph_launch(void* exec, void* stream);
 int pdlp_graph_destroy(void* exec);
+
+/* ----------------------------------------------------------------------------
+ * Batched engine: thousands of independent box-form LPs in one persistent launch.
+ *
+ * The reference solves LPs one at a time (solver/lp/pdhg.py:152; B&B node LPs
+ * one by one at solver/milp/branch_and_bound.py:367-374); this is the batched
+ * replacement of that loop.  All LPs are packed into shared arrays ("arena");
+ * each PdlpBatchLP holds one LP's offsets.  Row pointers of an LP are relative
+ * to its nnz_off, column indices relative to its col_off.
+ *
+ * One workgroup owns one LP at a time and takes the next from the work queue
+ * (`order`, typically largest first) when it finishes.  The iterates live in
+ * LDS; per iteration one pass over the LP's CSR rows forms K xbar, the dual
+ * update, and scatters K^T y+ into an LDS accumulator for the next primal
+ * step.  Termination and restarts follow pdhg.py:225-244 per LP, with the
+ * relative KKT of the reference-form lift in FP64.
+ * ------------------------------------------------------------------------- */
+
+typedef struct PdlpBatchLP {
+    int32_t m, n;
+    int32_t tpr;            /* lanes per row: 1,2,4,8,16,32,64 */
+    int32_t flags;          /* reserved, 0 */
+    int64_t row_off;        /* first row of this LP in the row arrays        */
+    int64_t col_off;        /* first column of this LP in the column arrays  */
+    int64_t rp_off;         /* first of its m+1 row pointers                 */
+    int64_t nnz_off;        /* first of its nonzeros                         */
+    double norm_b, norm_c;  /* reference-form normalisers                     */
+    double tau_scale;       /* primal weight: tau *= w, sigma /= w            */
+    double sigma_scale;
+} PdlpBatchLP;
+
+typedef struct PdlpBatchOut {
+    int32_t status;         /* PDLP_RUNNING (= iteration limit), PDLP_OPTIMAL, PDLP_NUMERICAL */
+    int32_t iterations;
+    int32_t restarts;
+    int32_t checks;
+    double rel_kkt, rp, rd, gap, p_obj, d_obj;
+} PdlpBatchOut;
+
+typedef struct PdlpBatch {
+    int32_t count;          /* number of LPs                                  */
+    int32_t check_every;    /* 50 = reference cadence                          */
+    int32_t max_iters;
+    int32_t block;          /* threads per workgroup: 256, 512 or 1024        */
+    int32_t lds_bytes;      /* dynamic LDS per workgroup: >= max over LPs of 8*(m+n) */
+    int32_t grid;           /* persistent workgroups (0 = 2 per CU)           */
+    double tol;
+    const PdlpBatchLP* lps; /* count descriptors                              */
+    const int32_t* order;   /* count LP indices, processing order             */
+    int32_t* queue;         /* one int32 work counter, zero before the launch */
+    /* matrices */
+    const int32_t* row_ptr; const int32_t* col_idx; const float* val;
+    /* per column: c, bounds (+-inf allowed), FP64 c, lower-bound shift, (hi - shift) or +inf */
+    const float *c, *lo, *hi;
+    const double *c64, *lo_shift, *ub_span;
+    /* per row: b, kind, FP64 b, FP64 b - K*lo_shift */
+    const float* b; const int8_t* row_kind; const double *b64, *b_shift;
+    /* state: x and y hold the warm start on entry (zeros = cold start);
+       on exit x_sum / y_sum hold each LP's solution in FP64 */
+    float* x; float* x_psum; double* x_sum; float* tau;   /* per column */
+    float* y; double* y_sum;                               /* per row    */
+    PdlpBatchOut* out;
+} PdlpBatch;
+
+/* Solve every LP of the batch (one launch). */
+int pdlp_batch_solve(const PdlpBatch* batch, void* stream);
 
 #ifdef __cplusplus
 }

@@ -12,5 +12,7 @@ from .result import SolveResult
 from .io.mps import read_mps
 from .lp.pdhg import solve_lp_pdhg
 from .lp.pdhg_gpu import get_available_device, solve_lp_pdhg_gpu
+from .lp.batch import solve_lp_pdhg_batch
 
-__all__ = ["Problem", "SolveResult", "read_mps", "solve_lp_pdhg", "solve_lp_pdhg_gpu", "get_available_device"]
+__all__ = ["Problem", "SolveResult", "read_mps", "solve_lp_pdhg", "solve_lp_pdhg_gpu", "get_available_device",
+           "solve_lp_pdhg_batch"]

@@ -77,6 +77,42 @@ __device__ __forceinline__ void row_dot(const int32_t* __restrict__ rp, const in
     asum = s;
 }
 
+// Row dot product by a whole workgroup (TPR = blockDim.x > 64, one row per
+// workgroup): strided partial sums, wavefront reduction, then the wavefront
+// results through LDS.  Result valid in thread 0.  `lds` holds >= 2*16 A.
+template <int TPR, typename A, typename V, typename G>
+__device__ __forceinline__ void row_dot_block(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                              const V* __restrict__ val, int row, G gather, A& dot, A& asum,
+                                              A* lds) {
+    static_assert(TPR > 64 && TPR <= 1024 && TPR % 64 == 0, "block row width");
+    const int beg = rp[row], end = rp[row + 1];
+    A d = A(0), s = A(0);
+    for (int k = beg + (int)threadIdx.x; k < end; k += TPR) {
+        const A a = (A)val[k];
+        d += a * gather(ci[k]);
+        s += fabs(a);
+    }
+    d = group_sum<kWave>(d);
+    s = group_sum<kWave>(s);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        lds[2 * w] = d;
+        lds[2 * w + 1] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        d = A(0);
+        s = A(0);
+#pragma unroll
+        for (int i = 0; i < TPR / kWave; ++i) {
+            d += lds[2 * i];
+            s += lds[2 * i + 1];
+        }
+    }
+    dot = d;
+    asum = s;
+}
+
 // Block-wide sum of NV doubles (blockDim.x a multiple of 64, <= 1024).
 // Result valid in thread 0.
 template <int NV>

@@ -6,17 +6,19 @@
 //           gathered K^T y, the step tau_j (from the same row's |values|), the
 //           box projection, the extrapolation and the ergodic sum are fused;
 //   dual:   one K row per group; gathers xbar, forms sigma_i, projects y.
-// Every check_every iterations four small kernels evaluate the reference KKT
-// of the window average in FP64 (two partial-sum passes, one finalize that
-// decides termination/restart, one apply that restarts to the average).
-// The control block lives on the device; kernels return at once after the
-// status leaves RUNNING, so blocks of iterations can be enqueued (and
-// captured in a HIP graph) ahead of the host's status poll.
+// A group is 1..64 lanes of a wavefront, or a whole workgroup (128..1024
+// threads) for long rows.  Every check_every iterations four small kernels
+// evaluate the reference KKT of the window average in FP64 (two partial-sum
+// passes, one finalize that decides termination/restart, one apply that
+// restarts to the average and folds the FP32 partial sums).  The control
+// block lives on the device; kernels return at once after the status leaves
+// RUNNING, so blocks of iterations can be enqueued (and captured in a HIP
+// graph) ahead of the host's status poll.
 
 #include "pdlp_common.hpp"
 
 #include <algorithm>
-#include <cstring>
+#include <type_traits>
 
 namespace pdlp {
 
@@ -24,11 +26,33 @@ namespace {
 
 template <typename T>
 __device__ __forceinline__ T clamp_box(T v, const T* lo, const T* hi, int j) {
-    // Order matters only for crossed bounds (lo > hi), which validation rejects.
     if (lo != nullptr) v = v < lo[j] ? lo[j] : v;
     else v = v < T(0) ? T(0) : v;
     if (hi != nullptr) v = v > hi[j] ? hi[j] : v;
     return v;
+}
+
+// Row of this group and the lane inside the group.
+template <int TPR>
+__device__ __forceinline__ void group_of(int remap, int& row, int& lane) {
+    if constexpr (TPR > 64) {
+        row = block_index(remap);
+        lane = threadIdx.x;
+    } else {
+        row = block_index(remap) * (blockDim.x / TPR) + threadIdx.x / TPR;
+        lane = threadIdx.x % TPR;
+    }
+}
+
+template <int TPR, bool EXACT, typename T, typename G>
+__device__ __forceinline__ void any_row_dot(const int32_t* rp, const int32_t* ci, const T* val, int row, int lane,
+                                            G gather, T& dot, T& asum) {
+    if constexpr (TPR > 64) {
+        __shared__ T lds[2 * 16];
+        row_dot_block<TPR, T>(rp, ci, val, row, gather, dot, asum, lds);
+    } else {
+        row_dot<TPR, EXACT, T>(rp, ci, val, row, lane, gather, dot, asum);
+    }
 }
 
 // ---------------------------------------------------------------- primal step
@@ -37,36 +61,32 @@ __global__ void __launch_bounds__(1024) primal_kernel(PdlpDev d, int remap) {
     const PdlpCtrl* ctrl = d.ctrl;
     if (ctrl->status != PDLP_RUNNING) return;
     const T tau_scale = (T)ctrl->tau_scale;
-    const T* __restrict__ tv = (const T*)d.t_val;
     const T* __restrict__ y = (const T*)d.y;
-    const T* __restrict__ c = (const T*)d.c;
-    const T* lo = (const T*)d.lo;
-    const T* hi = (const T*)d.hi;
-    T* __restrict__ x = (T*)d.x;
-    T* __restrict__ xbar = (T*)d.xbar;
-    const int groups_per_block = blockDim.x / TPR;
-    const int lane = threadIdx.x % TPR;
-    const int j = block_index(remap) * groups_per_block + threadIdx.x / TPR;
-    if (j >= d.n) return;
+    int j, lane;
+    group_of<TPR>(remap, j, lane);
+    if (j >= d.n) return;                               // whole groups leave together
     T aty, asum;
-    row_dot<TPR, EXACT, T>(d.t_row_ptr, d.t_col_idx, tv, j, lane,
-                           [&](int k) { return y[k]; }, aty, asum);
+    any_row_dot<TPR, EXACT, T>(d.t_row_ptr, d.t_col_idx, (const T*)d.t_val, j, lane,
+                               [&](int k) { return y[k]; }, aty, asum);
     if (lane != 0) return;
+    const T* c = (const T*)d.c;
+    T* x = (T*)d.x;
+    T* xbar = (T*)d.xbar;
     const T colsum = asum > T(1e-4) ? asum : T(1e-4);
     T tau = T(0.999) / colsum;
     if (tau_scale != T(1)) tau *= tau_scale;
     const T xj = x[j];
     T xn;
     if constexpr (EXACT) {
-        xn = add_rn(xj, -mul_rn(tau, add_rn(c[j], -aty)));
-        xn = clamp_box(xn, lo, hi, j);
+        xn = clamp_box(add_rn(xj, -mul_rn(tau, add_rn(c[j], -aty))), (const T*)d.lo, (const T*)d.hi, j);
         xbar[j] = add_rn(mul_rn(T(2), xn), -xj);
     } else {
-        xn = clamp_box(xj - tau * (c[j] - aty), lo, hi, j);
+        xn = clamp_box(xj - tau * (c[j] - aty), (const T*)d.lo, (const T*)d.hi, j);
         xbar[j] = T(2) * xn - xj;
     }
     x[j] = xn;
-    d.x_sum[j] += (double)xn;
+    if (d.x_psum != nullptr) d.x_psum[j] += (float)xn;
+    else d.x_sum[j] += (double)xn;
 }
 
 // ------------------------------------------------------------------ dual step
@@ -74,23 +94,21 @@ template <typename T, int TPR, bool EXACT>
 __global__ void __launch_bounds__(1024) dual_kernel(PdlpDev d, int remap) {
     PdlpCtrl* ctrl = d.ctrl;
     if (ctrl->status != PDLP_RUNNING) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {   // one writer; nobody else reads these here
+    if (blockIdx.x == 0 && threadIdx.x == 0) {   // single writer; read only by later kernels
         ctrl->iter += 1;
         ctrl->window += 1;
     }
     const T sigma_scale = (T)ctrl->sigma_scale;
-    const T* __restrict__ v = (const T*)d.val;
     const T* __restrict__ xbar = (const T*)d.xbar;
-    const T* __restrict__ b = (const T*)d.b;
-    T* __restrict__ y = (T*)d.y;
-    const int groups_per_block = blockDim.x / TPR;
-    const int lane = threadIdx.x % TPR;
-    const int i = block_index(remap) * groups_per_block + threadIdx.x / TPR;
+    int i, lane;
+    group_of<TPR>(remap, i, lane);
     if (i >= d.m) return;
     T ax, asum;
-    row_dot<TPR, EXACT, T>(d.row_ptr, d.col_idx, v, i, lane,
-                           [&](int k) { return xbar[k]; }, ax, asum);
+    any_row_dot<TPR, EXACT, T>(d.row_ptr, d.col_idx, (const T*)d.val, i, lane,
+                               [&](int k) { return xbar[k]; }, ax, asum);
     if (lane != 0) return;
+    const T* b = (const T*)d.b;
+    T* y = (T*)d.y;
     const T rowsum = asum > T(1e-4) ? asum : T(1e-4);
     T sigma = T(0.999) / rowsum;
     if (sigma_scale != T(1)) sigma *= sigma_scale;
@@ -99,13 +117,18 @@ __global__ void __launch_bounds__(1024) dual_kernel(PdlpDev d, int remap) {
     else yn = y[i] + sigma * (b[i] - ax);
     if (d.row_kind != nullptr && d.row_kind[i] == PDLP_ROW_LE && yn > T(0)) yn = T(0);
     y[i] = yn;
-    d.y_sum[i] += (double)yn;
+    if (d.y_psum != nullptr) d.y_psum[i] += (float)yn;
+    else d.y_sum[i] += (double)yn;
 }
 
 // ------------------------------------------------------------- KKT partials
 // Partial layout: partials[part*BLOCKS*4 + blk*4 + k], k = 0 rp^2, 1 rd^2,
 // 2 primal objective, 3 dual objective.  part 0 = rows, part 1 = columns.
-// src: 0 = current iterate, 1 = window average (sum / window).
+// src: 0 = current iterate, 1 = window average ((sum + partial sum) / window).
+
+__device__ __forceinline__ double avg_of(const double* s, const float* ps, int k, double w) {
+    return ps != nullptr ? (s[k] + (double)ps[k]) / w : s[k] / w;
+}
 
 template <typename T, int TPR>
 __global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src) {
@@ -116,21 +139,19 @@ __global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src) {
     const T* __restrict__ v = (const T*)d.val;
     const T* __restrict__ x = (const T*)d.x;
     const T* __restrict__ y = (const T*)d.y;
-    const double* __restrict__ xs = d.x_sum;
-    const double* __restrict__ ys = d.y_sum;
     const int gpb = blockDim.x / TPR, lane = threadIdx.x % TPR;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int i = blockIdx.x * gpb + threadIdx.x / TPR; i < d.m; i += gridDim.x * gpb) {
         double ax, asum;
         if (src) row_dot<TPR, false, double>(d.row_ptr, d.col_idx, v, i, lane,
-                                             [&](int k) { return xs[k] / w; }, ax, asum);
+                                             [&](int k) { return avg_of(d.x_sum, d.x_psum, k, w); }, ax, asum);
         else row_dot<TPR, false, double>(d.row_ptr, d.col_idx, v, i, lane,
                                          [&](int k) { return (double)x[k]; }, ax, asum);
         if (lane == 0) {
             const bool le = d.row_kind != nullptr && d.row_kind[i] == PDLP_ROW_LE;
             double res = ax - d.b64[i];
             if (le && res < 0.0) res = 0.0;
-            const double yi = src ? ys[i] / w : (double)y[i];
+            const double yi = src ? avg_of(d.y_sum, d.y_psum, i, w) : (double)y[i];
             acc[0] += res * res;
             if (le && yi > 0.0) acc[1] += yi * yi;      // slack column reduced cost -y_i
             acc[3] += d.b_shift[i] * yi;
@@ -152,14 +173,12 @@ __global__ void __launch_bounds__(256) kkt_cols_kernel(PdlpDev d, int src) {
     const T* __restrict__ tv = (const T*)d.t_val;
     const T* __restrict__ x = (const T*)d.x;
     const T* __restrict__ y = (const T*)d.y;
-    const double* __restrict__ xs = d.x_sum;
-    const double* __restrict__ ys = d.y_sum;
     const int gpb = blockDim.x / TPR, lane = threadIdx.x % TPR;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int j = blockIdx.x * gpb + threadIdx.x / TPR; j < d.n; j += gridDim.x * gpb) {
         double aty, asum;
         if (src) row_dot<TPR, false, double>(d.t_row_ptr, d.t_col_idx, tv, j, lane,
-                                             [&](int k) { return ys[k] / w; }, aty, asum);
+                                             [&](int k) { return avg_of(d.y_sum, d.y_psum, k, w); }, aty, asum);
         else row_dot<TPR, false, double>(d.t_row_ptr, d.t_col_idx, tv, j, lane,
                                          [&](int k) { return (double)y[k]; }, aty, asum);
         if (lane == 0) {
@@ -168,7 +187,7 @@ __global__ void __launch_bounds__(256) kkt_cols_kernel(PdlpDev d, int src) {
             const double rneg = r < 0.0 ? r : 0.0;
             if (isfinite(span)) acc[3] += span * rneg;   // upper-bound row dual w_j = min(r_j, 0)
             else acc[1] += rneg * rneg;
-            const double xj = src ? xs[j] / w : (double)x[j];
+            const double xj = src ? avg_of(d.x_sum, d.x_psum, j, w) : (double)x[j];
             const double sh = d.lo_shift != nullptr ? d.lo_shift[j] : 0.0;
             acc[2] += d.c64[j] * (xj - sh);
         }
@@ -231,30 +250,56 @@ __global__ void __launch_bounds__(256) kkt_finalize_kernel(PdlpDev d, int nb_row
     }
 }
 
-// Restart to (or, on convergence, finish at) the window average.
+// After each check: restart to (or, on convergence, finish at) the window
+// average, and fold the FP32 partial sums of the last check_every iterations
+// into the FP64 window sums.  At convergence the FP64 sums are kept (folded)
+// so the host can read the exact average.
 template <typename T>
 __global__ void __launch_bounds__(256) apply_kernel(PdlpDev d) {
     const PdlpCtrl* ctrl = d.ctrl;
-    if (!ctrl->restart_pending) return;
-    const bool keep_sums = ctrl->status == PDLP_OPTIMAL;   // host may read the FP64 average
+    const bool pending = ctrl->restart_pending != 0;
+    if (!pending && d.x_psum == nullptr) return;
+    const bool keep_sums = ctrl->status == PDLP_OPTIMAL;
     const double w = (double)ctrl->apply_window;
     T* x = (T*)d.x;
     T* y = (T*)d.y;
     const int stride = gridDim.x * blockDim.x;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < d.n; j += stride) {
-        x[j] = (T)(d.x_sum[j] / w);
-        if (!keep_sums) d.x_sum[j] = 0.0;
+        double s = d.x_sum[j];
+        if (d.x_psum != nullptr) {
+            s += (double)d.x_psum[j];
+            d.x_psum[j] = 0.f;
+        }
+        if (pending) {
+            x[j] = (T)(s / w);
+            if (!keep_sums) s = 0.0;
+        }
+        d.x_sum[j] = s;
     }
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.m; i += stride) {
-        y[i] = (T)(d.y_sum[i] / w);
-        if (!keep_sums) d.y_sum[i] = 0.0;
+        double s = d.y_sum[i];
+        if (d.y_psum != nullptr) {
+            s += (double)d.y_psum[i];
+            d.y_psum[i] = 0.f;
+        }
+        if (pending) {
+            y[i] = (T)(s / w);
+            if (!keep_sums) s = 0.0;
+        }
+        d.y_sum[i] = s;
     }
 }
 
 __global__ void init_kernel(PdlpDev d) {
     const int stride = gridDim.x * blockDim.x;
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < d.n; j += stride) d.x_sum[j] = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.m; i += stride) d.y_sum[i] = 0.0;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < d.n; j += stride) {
+        d.x_sum[j] = 0.0;
+        if (d.x_psum != nullptr) d.x_psum[j] = 0.f;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.m; i += stride) {
+        d.y_sum[i] = 0.0;
+        if (d.y_psum != nullptr) d.y_psum[i] = 0.f;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         PdlpCtrl* c = d.ctrl;
         c->iter = 0;
@@ -269,7 +314,29 @@ __global__ void init_kernel(PdlpDev d) {
 }
 
 // ------------------------------------------------------------ launch helpers
-bool valid_tpr(int t) { return t == 1 || t == 2 || t == 4 || t == 8 || t == 16 || t == 32 || t == 64; }
+template <int V>
+using ic = std::integral_constant<int, V>;
+
+// Calls f(ic<TPR>{}) for a supported lanes-per-row value.
+template <typename F>
+int with_tpr(int tpr, F&& f) {
+    switch (tpr) {
+        case 1: return f(ic<1>{});
+        case 2: return f(ic<2>{});
+        case 4: return f(ic<4>{});
+        case 8: return f(ic<8>{});
+        case 16: return f(ic<16>{});
+        case 32: return f(ic<32>{});
+        case 64: return f(ic<64>{});
+        case 128: return f(ic<128>{});
+        case 256: return f(ic<256>{});
+        case 512: return f(ic<512>{});
+        case 1024: return f(ic<1024>{});
+        default: return fail(PDLP_E_TUNE, "tpr must be a power of two in [1, 1024]");
+    }
+}
+
+bool valid_tpr(int t) { return t >= 1 && t <= 1024 && (t & (t - 1)) == 0; }
 
 int check_dev(const PdlpDev* d) {
     if (d == nullptr) return fail(PDLP_E_ARG, "PdlpDev is NULL");
@@ -279,13 +346,17 @@ int check_dev(const PdlpDev* d) {
         !d->b || !d->c64 || !d->b64 || !d->b_shift || !d->x || !d->xbar || !d->y || !d->x_sum ||
         !d->y_sum || !d->partials || !d->ctrl)
         return fail(PDLP_E_ARG, "a required PdlpDev pointer is NULL");
+    if ((d->x_psum == nullptr) != (d->y_psum == nullptr))
+        return fail(PDLP_E_ARG, "x_psum and y_psum must both be set or both be NULL");
+    if (d->x_psum != nullptr && d->prec != PDLP_F32)
+        return fail(PDLP_E_ARG, "partial sums are for PDLP_F32 only");
     return 0;
 }
 
 int check_tune(const PdlpTune* t) {
     if (t == nullptr) return fail(PDLP_E_ARG, "PdlpTune is NULL");
     if (!valid_tpr(t->tpr_primal) || !valid_tpr(t->tpr_dual))
-        return fail(PDLP_E_TUNE, "tpr must be one of 1,2,4,8,16,32,64");
+        return fail(PDLP_E_TUNE, "tpr must be a power of two in [1, 1024]");
     if (t->block < 64 || t->block > 1024 || (t->block & 63))
         return fail(PDLP_E_TUNE, "block must be a multiple of 64 in [64, 1024]");
     if (t->check_every <= 0) return fail(PDLP_E_TUNE, "check_every must be positive");
@@ -294,30 +365,35 @@ int check_tune(const PdlpTune* t) {
     return 0;
 }
 
-inline unsigned grid_for(int rows, int tpr, int block, int remap) {
-    const long groups = block / tpr;
-    long g = (rows + groups - 1) / groups;
+// Workgroups and threads per workgroup for `rows` rows at `tpr` lanes each.
+inline void shape_for(int rows, int tpr, int block, int remap, unsigned& grid, unsigned& threads) {
+    long g;
+    if (tpr > 64) {
+        threads = (unsigned)tpr;
+        g = rows;
+    } else {
+        threads = (unsigned)block;
+        const long groups = block / tpr;
+        g = (rows + groups - 1) / groups;
+    }
     if (g < 1) g = 1;
     if (remap) g = (g + 7) / 8 * 8;
-    return (unsigned)g;
+    grid = (unsigned)g;
 }
 
 template <typename T, bool EXACT>
 int launch_primal_t(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
     const int remap = t.xcd_remap & 1;
-    const unsigned g = grid_for(d.n, t.tpr_primal, t.block, remap);
+    unsigned g, nt;
+    shape_for(d.n, t.tpr_primal, t.block, remap, g, nt);
     if constexpr (EXACT) {
-        primal_kernel<T, 1, true><<<g, t.block, 0, s>>>(d, remap);
+        primal_kernel<T, 1, true><<<g, nt, 0, s>>>(d, remap);
     } else {
-        switch (t.tpr_primal) {
-            case 1: primal_kernel<T, 1, false><<<g, t.block, 0, s>>>(d, remap); break;
-            case 2: primal_kernel<T, 2, false><<<g, t.block, 0, s>>>(d, remap); break;
-            case 4: primal_kernel<T, 4, false><<<g, t.block, 0, s>>>(d, remap); break;
-            case 8: primal_kernel<T, 8, false><<<g, t.block, 0, s>>>(d, remap); break;
-            case 16: primal_kernel<T, 16, false><<<g, t.block, 0, s>>>(d, remap); break;
-            case 32: primal_kernel<T, 32, false><<<g, t.block, 0, s>>>(d, remap); break;
-            default: primal_kernel<T, 64, false><<<g, t.block, 0, s>>>(d, remap); break;
-        }
+        int rc = with_tpr(t.tpr_primal, [&](auto k) {
+            primal_kernel<T, decltype(k)::value, false><<<g, nt, 0, s>>>(d, remap);
+            return 0;
+        });
+        if (rc) return rc;
     }
     return hip_fail(hipGetLastError(), "primal_kernel launch");
 }
@@ -325,19 +401,16 @@ int launch_primal_t(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
 template <typename T, bool EXACT>
 int launch_dual_t(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
     const int remap = (t.xcd_remap >> 1) & 1;
-    const unsigned g = grid_for(d.m, t.tpr_dual, t.block, remap);
+    unsigned g, nt;
+    shape_for(d.m, t.tpr_dual, t.block, remap, g, nt);
     if constexpr (EXACT) {
-        dual_kernel<T, 1, true><<<g, t.block, 0, s>>>(d, remap);
+        dual_kernel<T, 1, true><<<g, nt, 0, s>>>(d, remap);
     } else {
-        switch (t.tpr_dual) {
-            case 1: dual_kernel<T, 1, false><<<g, t.block, 0, s>>>(d, remap); break;
-            case 2: dual_kernel<T, 2, false><<<g, t.block, 0, s>>>(d, remap); break;
-            case 4: dual_kernel<T, 4, false><<<g, t.block, 0, s>>>(d, remap); break;
-            case 8: dual_kernel<T, 8, false><<<g, t.block, 0, s>>>(d, remap); break;
-            case 16: dual_kernel<T, 16, false><<<g, t.block, 0, s>>>(d, remap); break;
-            case 32: dual_kernel<T, 32, false><<<g, t.block, 0, s>>>(d, remap); break;
-            default: dual_kernel<T, 64, false><<<g, t.block, 0, s>>>(d, remap); break;
-        }
+        int rc = with_tpr(t.tpr_dual, [&](auto k) {
+            dual_kernel<T, decltype(k)::value, false><<<g, nt, 0, s>>>(d, remap);
+            return 0;
+        });
+        if (rc) return rc;
     }
     return hip_fail(hipGetLastError(), "dual_kernel launch");
 }
@@ -349,25 +422,19 @@ int launch_kkt_t(const PdlpDev& d, const PdlpTune& t, int src, int mode, hipStre
         long g = (rows + (B / tpr) - 1) / (B / tpr);
         return (int)std::max(1L, std::min<long>(g, PDLP_PARTIAL_BLOCKS));
     };
-    const int nr = nb(d.m, t.tpr_dual), nc = nb(d.n, t.tpr_primal);
-    switch (t.tpr_dual) {
-        case 1: kkt_rows_kernel<T, 1><<<nr, B, 0, s>>>(d, src); break;
-        case 2: kkt_rows_kernel<T, 2><<<nr, B, 0, s>>>(d, src); break;
-        case 4: kkt_rows_kernel<T, 4><<<nr, B, 0, s>>>(d, src); break;
-        case 8: kkt_rows_kernel<T, 8><<<nr, B, 0, s>>>(d, src); break;
-        case 16: kkt_rows_kernel<T, 16><<<nr, B, 0, s>>>(d, src); break;
-        case 32: kkt_rows_kernel<T, 32><<<nr, B, 0, s>>>(d, src); break;
-        default: kkt_rows_kernel<T, 64><<<nr, B, 0, s>>>(d, src); break;
-    }
-    switch (t.tpr_primal) {
-        case 1: kkt_cols_kernel<T, 1><<<nc, B, 0, s>>>(d, src); break;
-        case 2: kkt_cols_kernel<T, 2><<<nc, B, 0, s>>>(d, src); break;
-        case 4: kkt_cols_kernel<T, 4><<<nc, B, 0, s>>>(d, src); break;
-        case 8: kkt_cols_kernel<T, 8><<<nc, B, 0, s>>>(d, src); break;
-        case 16: kkt_cols_kernel<T, 16><<<nc, B, 0, s>>>(d, src); break;
-        case 32: kkt_cols_kernel<T, 32><<<nc, B, 0, s>>>(d, src); break;
-        default: kkt_cols_kernel<T, 64><<<nc, B, 0, s>>>(d, src); break;
-    }
+    // the check's reductions use at most a wavefront per row
+    const int td = std::min(t.tpr_dual, 64), tp = std::min(t.tpr_primal, 64);
+    const int nr = nb(d.m, td), nc = nb(d.n, tp);
+    int rc = with_tpr(td, [&](auto k) {
+        if constexpr (decltype(k)::value <= 64) kkt_rows_kernel<T, decltype(k)::value><<<nr, B, 0, s>>>(d, src);
+        return 0;
+    });
+    if (rc) return rc;
+    rc = with_tpr(tp, [&](auto k) {
+        if constexpr (decltype(k)::value <= 64) kkt_cols_kernel<T, decltype(k)::value><<<nc, B, 0, s>>>(d, src);
+        return 0;
+    });
+    if (rc) return rc;
     kkt_finalize_kernel<<<1, B, 0, s>>>(d, nr, nc, mode);
     if (mode == 1) {
         const int work = std::max(d.n, d.m);
@@ -414,8 +481,7 @@ extern "C" {
 
 int pdlp_init(const PdlpDev* dev, const PdlpTune* tune, void* stream) {
     int rc = check_dev(dev);
-    if (rc) return rc;
-    if ((rc = check_tune(tune))) return rc;
+    if (rc || (rc = check_tune(tune))) return rc;
     hipStream_t s = as_stream(stream);
     const int work = std::max(dev->n, dev->m);
     init_kernel<<<std::min((work + 255) / 256, 4096), 256, 0, s>>>(*dev);

@@ -43,7 +43,7 @@ class PdlpDev(C.Structure):
         ("c", _p), ("lo", _p), ("hi", _p), ("b", _p), ("row_kind", _p),
         ("c64", _p), ("b64", _p), ("b_shift", _p), ("lo_shift", _p), ("ub_span", _p),
         ("x", _p), ("xbar", _p), ("y", _p), ("x_sum", _p), ("y_sum", _p),
-        ("partials", _p), ("ctrl", _p),
+        ("x_psum", _p), ("y_psum", _p), ("partials", _p), ("ctrl", _p),
     ]
 
 
@@ -51,6 +51,20 @@ class PdlpTune(C.Structure):
     _fields_ = [
         ("tpr_primal", C.c_int32), ("tpr_dual", C.c_int32), ("block", C.c_int32),
         ("check_every", C.c_int32), ("xcd_remap", C.c_int32), ("exact", C.c_int32),
+    ]
+
+
+class PdlpBatch(C.Structure):
+    _fields_ = [
+        ("count", C.c_int32), ("check_every", C.c_int32), ("max_iters", C.c_int32), ("block", C.c_int32),
+        ("lds_bytes", C.c_int32), ("grid", C.c_int32), ("tol", C.c_double),
+        ("lps", C.c_void_p), ("order", C.c_void_p), ("queue", C.c_void_p),
+        ("row_ptr", C.c_void_p), ("col_idx", C.c_void_p), ("val", C.c_void_p),
+        ("c", C.c_void_p), ("lo", C.c_void_p), ("hi", C.c_void_p),
+        ("c64", C.c_void_p), ("lo_shift", C.c_void_p), ("ub_span", C.c_void_p),
+        ("b", C.c_void_p), ("row_kind", C.c_void_p), ("b64", C.c_void_p), ("b_shift", C.c_void_p),
+        ("x", C.c_void_p), ("x_psum", C.c_void_p), ("x_sum", C.c_void_p), ("tau", C.c_void_p),
+        ("y", C.c_void_p), ("y_sum", C.c_void_p), ("out", C.c_void_p),
     ]
 
 
@@ -76,10 +90,9 @@ _SIGS = {
                                      C.POINTER(C.c_void_p)]),
     "pdlp_graph_launch": (C.c_int, [_p, _p]),
     "pdlp_graph_destroy": (C.c_int, [_p]),
+    "pdlp_batch_solve": (C.c_int, [C.POINTER(PdlpBatch), _p]),
 }
 
-# Extra entry points registered by other host modules (batched engine, MLP).
-EXTRA_SIGS: dict = {}
 
 
 def _bind(L, name, sig):
@@ -99,7 +112,7 @@ def lib():
             raise PdlpError(f"{LIB_PATH} is missing: build it with "
                             "`python -m sovereign_solver_amd.csrc.build` (this engine has no CPU path)")
         L = C.CDLL(LIB_PATH)
-        for name, sig in {**_SIGS, **EXTRA_SIGS}.items():
+        for name, sig in _SIGS.items():
             _bind(L, name, sig)
         v = L.pdlp_abi_version()
         if v != PDLP_ABI_VERSION:
@@ -115,4 +128,4 @@ def check(rc: int, what: str = "") -> None:
 
 
 def exported_symbols() -> list:
-    return sorted({**_SIGS, **EXTRA_SIGS}.keys())
+    return sorted(_SIGS.keys())

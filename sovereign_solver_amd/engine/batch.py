@@ -1,0 +1,207 @@
+"""Batched PDHG engine: many independent LPs, one persistent HIP launch.
+
+The LPs are packed into one arena of device arrays (CSR rows, per-column and
+per-row vectors, FP64 yardstick data) with a descriptor per LP; the kernel
+(``pdlp_batch_solve``) gives each workgroup one LP at a time from a work queue
+ordered largest-first, so the biggest LPs start first and the small ones fill
+in behind them.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+import scipy.sparse as sp
+import torch
+
+from . import _lib as L
+from ._lib import PdlpBatch
+from .device import dev_tensor, ptr, require_gpu, stream_handle
+from .forms import ROW_EQ, ROW_LE
+from ..problem import Problem
+
+LP_DTYPE = np.dtype([
+    ("m", np.int32), ("n", np.int32), ("tpr", np.int32), ("flags", np.int32),
+    ("row_off", np.int64), ("col_off", np.int64), ("rp_off", np.int64), ("nnz_off", np.int64),
+    ("norm_b", np.float64), ("norm_c", np.float64), ("tau_scale", np.float64), ("sigma_scale", np.float64),
+], align=True)
+
+OUT_DTYPE = np.dtype([
+    ("status", np.int32), ("iterations", np.int32), ("restarts", np.int32), ("checks", np.int32),
+    ("rel_kkt", np.float64), ("rp", np.float64), ("rd", np.float64), ("gap", np.float64),
+    ("p_obj", np.float64), ("d_obj", np.float64),
+], align=True)
+
+
+_STATUS = {L.PDLP_RUNNING: "iteration_limit", L.PDLP_OPTIMAL: "optimal", L.PDLP_NUMERICAL: "numerical_error"}
+
+
+def lanes_per_row(mean_len: float) -> int:
+    """About two entries per lane, a power of two in [1, 64]."""
+    t = 1
+    while t * 2 <= mean_len / 2 and t < 64:
+        t *= 2
+    return t
+
+
+class PackedBatch:
+    """Host-side arena of B box-form LPs (same layout as the device arena)."""
+
+    def __init__(self, problems: Sequence[Problem]):
+        self.problems = list(problems)
+        B = len(self.problems)
+        ms = np.empty(B, np.int64)
+        ns = np.empty(B, np.int64)
+        nnzs = np.empty(B, np.int64)
+        Ks = []
+        for k, p in enumerate(self.problems):
+            K = p.A_eq if p.n_ineq == 0 else (p.A_ub if p.n_eq == 0 else sp.vstack([p.A_ub, p.A_eq], format="csr"))
+            K = sp.csr_matrix(K, dtype=np.float64)
+            K.sum_duplicates()
+            K.sort_indices()
+            Ks.append(K)
+            ms[k], ns[k], nnzs[k] = K.shape[0], K.shape[1], K.nnz
+            if ms[k] == 0 or ns[k] == 0:
+                raise ValueError(f"LP {k}: the batched engine needs m > 0 and n > 0")
+        self.ms, self.ns, self.nnzs = ms, ns, nnzs
+        row_off = np.concatenate([[0], np.cumsum(ms)[:-1]])
+        col_off = np.concatenate([[0], np.cumsum(ns)[:-1]])
+        rp_off = np.concatenate([[0], np.cumsum(ms + 1)[:-1]])
+        nnz_off = np.concatenate([[0], np.cumsum(nnzs)[:-1]])
+        self.row_off, self.col_off, self.nnz_off = row_off, col_off, nnz_off
+        M, N, Z = int(ms.sum()), int(ns.sum()), int(nnzs.sum())
+        if Z >= 2**31 or N >= 2**31:
+            raise ValueError("batch beyond int32 indexing of the arena")
+        self.M, self.N, self.Z = M, N, Z
+        self.row_ptr = np.concatenate([K.indptr.astype(np.int32) for K in Ks])
+        self.col_idx = np.concatenate([K.indices.astype(np.int32) for K in Ks])
+        self.val = np.concatenate([K.data for K in Ks])
+        c = np.empty(N)
+        lo = np.empty(N)
+        hi = np.empty(N)
+        b = np.empty(M)
+        kind = np.empty(M, np.int8)
+        b_shift = np.empty(M)
+        desc = np.zeros(B, LP_DTYPE)
+        for k, (p, K) in enumerate(zip(self.problems, Ks)):
+            cs, rs = slice(col_off[k], col_off[k] + ns[k]), slice(row_off[k], row_off[k] + ms[k])
+            c[cs] = p.c
+            lo[cs] = p.lb
+            hi[cs] = p.ub
+            bk = np.concatenate([np.asarray(p.b_ub, float), np.asarray(p.b_eq, float)])
+            b[rs] = bk
+            kind[rs] = np.concatenate([np.full(p.n_ineq, ROW_LE, np.int8), np.full(p.n_eq, ROW_EQ, np.int8)])
+            lbf = np.where(np.isfinite(p.lb), p.lb, 0.0)
+            bs = bk - K @ lbf
+            b_shift[rs] = bs
+            fin = np.isfinite(p.ub)
+            span = (p.ub - lbf)[fin]
+            desc[k] = (ms[k], ns[k], lanes_per_row(nnzs[k] / ms[k]), 0, row_off[k], col_off[k], rp_off[k],
+                       nnz_off[k], max(1.0, float(np.sqrt(bs @ bs + span @ span))),
+                       max(1.0, float(np.linalg.norm(p.c))), 1.0, 1.0)
+        lo_shift = np.where(np.isfinite(lo), lo, 0.0)
+        ub_span = np.where(np.isfinite(hi), hi - lo_shift, np.inf)
+        self.c, self.lo, self.hi, self.b, self.kind = c, lo, hi, b, kind
+        self.lo_shift, self.ub_span, self.b_shift = lo_shift, ub_span, b_shift
+        self.desc = desc
+        self.order = np.argsort(-nnzs, kind="stable").astype(np.int32)     # largest first
+        self.lds_bytes = int(((8 * (ms + ns)).max() + 15) // 16 * 16)
+
+    def algorithmic_bytes_per_iteration(self) -> float:
+        """Bytes one iteration of every LP moves from HBM at minimum: the CSR once
+        (values, column indices, row pointers) and the per-column state."""
+        return float(self.Z * 8 + (self.M + len(self.problems)) * 4 + self.N * (4 * 6 + 4 * 2) + self.M * 5)
+
+
+class BatchEngine:
+    def __init__(self, packed: PackedBatch, block: int = 256, grid: int = 0):
+        self.lib = L.lib()
+        self.device = require_gpu()
+        self.pk = packed
+        self.block = block
+        self.grid = grid
+        self.stream = torch.cuda.Stream(device=self.device)
+        dv = self.device
+        f32, f64, i32 = torch.float32, torch.float64, torch.int32
+        pk = packed
+        with torch.cuda.stream(self.stream):
+            self.t = {
+                "lps": torch.from_numpy(pk.desc.view(np.uint8).copy()).to(dv),
+                "order": dev_tensor(pk.order, i32, dv),
+                "queue": torch.zeros(1, dtype=i32, device=dv),
+                "row_ptr": dev_tensor(pk.row_ptr, i32, dv),
+                "col_idx": dev_tensor(pk.col_idx, i32, dv),
+                "val": dev_tensor(pk.val, f32, dv),
+                "c": dev_tensor(pk.c, f32, dv), "lo": dev_tensor(pk.lo, f32, dv), "hi": dev_tensor(pk.hi, f32, dv),
+                "c64": dev_tensor(pk.c, f64, dv), "lo_shift": dev_tensor(pk.lo_shift, f64, dv),
+                "ub_span": dev_tensor(pk.ub_span, f64, dv),
+                "b": dev_tensor(pk.b, f32, dv), "row_kind": dev_tensor(pk.kind, torch.int8, dv),
+                "b64": dev_tensor(pk.b, f64, dv), "b_shift": dev_tensor(pk.b_shift, f64, dv),
+                "x": torch.zeros(pk.N, dtype=f32, device=dv), "x_psum": torch.zeros(pk.N, dtype=f32, device=dv),
+                "x_sum": torch.zeros(pk.N, dtype=f64, device=dv), "tau": torch.zeros(pk.N, dtype=f32, device=dv),
+                "y": torch.zeros(pk.M, dtype=f32, device=dv), "y_sum": torch.zeros(pk.M, dtype=f64, device=dv),
+                "out": torch.zeros(len(pk.problems) * OUT_DTYPE.itemsize, dtype=torch.uint8, device=dv),
+            }
+        self.stream.synchronize()
+
+    def set_warm_start(self, x0: Optional[np.ndarray] = None, y0: Optional[np.ndarray] = None,
+                       step_scale: Optional[np.ndarray] = None) -> None:
+        """Packed starting point (length N / M arrays; None = zeros) and per-LP primal weights."""
+        with torch.cuda.stream(self.stream):
+            if x0 is None:          # cold start: the origin projected onto the box (x = lb when lb >= 0)
+                self.t["x"].copy_(torch.from_numpy(np.clip(0.0, self.pk.lo, self.pk.hi).astype(np.float32))
+                                  .to(self.device))
+            else:
+                self.t["x"].copy_(torch.from_numpy(np.asarray(x0, np.float32)).to(self.device))
+            if y0 is None:
+                self.t["y"].zero_()
+            else:
+                self.t["y"].copy_(torch.from_numpy(np.asarray(y0, np.float32)).to(self.device))
+            desc = self.pk.desc.copy()
+            if step_scale is not None:
+                w = np.asarray(step_scale, float)
+                desc["tau_scale"] = w
+                desc["sigma_scale"] = 1.0 / w
+            self.t["lps"].copy_(torch.from_numpy(desc.view(np.uint8).copy()).to(self.device))
+
+    def _args(self, tol: float, max_iters: int, check_every: int) -> PdlpBatch:
+        t = self.t
+        a = PdlpBatch(count=len(self.pk.problems), check_every=check_every, max_iters=max_iters, block=self.block,
+                      lds_bytes=self.pk.lds_bytes, grid=self.grid, tol=tol)
+        for name in ("lps", "order", "queue", "row_ptr", "col_idx", "val", "c", "lo", "hi", "c64", "lo_shift",
+                     "ub_span", "b", "row_kind", "b64", "b_shift", "x", "x_psum", "x_sum", "tau", "y", "y_sum",
+                     "out"):
+            setattr(a, name, ptr(t[name]))
+        return a
+
+    def launch(self, tol: float, max_iters: int, check_every: int = 50) -> None:
+        """Enqueue one batched solve on the engine stream (inputs already in place)."""
+        self._a = self._args(tol, max_iters, check_every)
+        with torch.cuda.stream(self.stream):
+            self.t["queue"].zero_()
+            L.check(self.lib.pdlp_batch_solve(C.byref(self._a), stream_handle(self.stream)), "pdlp_batch_solve")
+
+    def results(self):
+        self.stream.synchronize()
+        out = np.frombuffer(self.t["out"].cpu().numpy().tobytes(), dtype=OUT_DTYPE)
+        xs = self.t["x_sum"].cpu().numpy()
+        ys = self.t["y_sum"].cpu().numpy()
+        return out, xs, ys
+
+    def solve(self, tol: float, max_iters: int, check_every: int = 50, x0=None, y0=None, step_scale=None):
+        """Cold start (x = y = 0) unless a packed warm start is given."""
+        self.set_warm_start(x0, y0, step_scale)
+        self.launch(tol, max_iters, check_every)
+        return self.results()
+
+    def unpack(self, out, xs, ys):
+        """Per-LP (status, x, y, out-record) in each LP's own variable/row space."""
+        pk = self.pk
+        res = []
+        for k in range(len(pk.problems)):
+            x = xs[pk.col_off[k]: pk.col_off[k] + pk.ns[k]]
+            y = ys[pk.row_off[k]: pk.row_off[k] + pk.ms[k]]
+            res.append((_STATUS[int(out["status"][k])], x, y, out[k]))
+        return res

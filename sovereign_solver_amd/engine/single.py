@@ -72,6 +72,10 @@ class SingleEngine:
             self.y = torch.zeros(self.m, dtype=T, device=dv)
             self.x_sum = torch.zeros(self.n, dtype=f64, device=dv)
             self.y_sum = torch.zeros(self.m, dtype=f64, device=dv)
+            # FP32 engine: FP32 sums over each check block, folded into the FP64 sums at every check
+            fp32 = precision == "fp32"
+            self.x_psum = torch.zeros(self.n, dtype=torch.float32, device=dv) if fp32 else None
+            self.y_psum = torch.zeros(self.m, dtype=torch.float32, device=dv) if fp32 else None
             self.partials = torch.zeros(L.PDLP_PARTIALS_LEN, dtype=f64, device=dv)
             self.ctrl = torch.zeros(C.sizeof(L.PdlpCtrl), dtype=torch.uint8, device=dv)
         self.ctrl_host = torch.zeros((2, C.sizeof(L.PdlpCtrl)), dtype=torch.uint8).pin_memory()
@@ -83,7 +87,8 @@ class SingleEngine:
             c=ptr(self.c), lo=ptr(self.lo), hi=ptr(self.hi), b=ptr(self.b), row_kind=ptr(self.row_kind),
             c64=ptr(self.c64), b64=ptr(self.b64), b_shift=ptr(self.b_shift), lo_shift=ptr(self.lo_shift),
             ub_span=ptr(self.ub_span), x=ptr(self.x), xbar=ptr(self.xbar), y=ptr(self.y),
-            x_sum=ptr(self.x_sum), y_sum=ptr(self.y_sum), partials=ptr(self.partials), ctrl=ptr(self.ctrl))
+            x_sum=ptr(self.x_sum), y_sum=ptr(self.y_sum), x_psum=ptr(self.x_psum), y_psum=ptr(self.y_psum),
+            partials=ptr(self.partials), ctrl=ptr(self.ctrl))
         self._graphs = {}
         self._graph_tune = None
 
@@ -134,8 +139,11 @@ class SingleEngine:
     # ------------------------------------------------------------------ solve
     def start(self, tol: float, x0=None, y0=None, tau_scale: float = 1.0, sigma_scale: float = 1.0) -> None:
         with torch.cuda.stream(self.stream):
-            if x0 is None:
+            if x0 is None and self.lo is None:
                 self.x.zero_()
+            elif x0 is None:        # cold start: the origin projected onto the box (x = lb when lb >= 0)
+                x0 = np.clip(0.0, self.form.lo, self.form.hi)
+                self.x.copy_(torch.as_tensor(x0, dtype=self.dtype).to(self.device))
             else:
                 self.x.copy_(torch.as_tensor(np.asarray(x0), dtype=self.dtype).to(self.device))
             if y0 is None:

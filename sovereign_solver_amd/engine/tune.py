@@ -21,7 +21,7 @@ import threading
 from dataclasses import asdict, dataclass
 
 ARCH = "gfx950"
-TPR_CHOICES = (1, 2, 4, 8, 16, 32, 64)
+TPR_CHOICES = (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024)
 
 
 @dataclass(frozen=True)
@@ -39,10 +39,13 @@ class Tune:
 
 
 def _tpr_for(mean_len: float) -> int:
+    """Lanes per row: about one lane per 2–4 entries, a workgroup per row above 256."""
     if mean_len <= 1.0:
         return 1
     t = 1 << int(math.floor(math.log2(mean_len)))
-    return max(1, min(64, t))
+    if t > 64:                     # long rows: a workgroup per row, ~4 entries per lane
+        return max(128, min(1024, t // 4))
+    return max(1, t)
 
 
 def heuristic_tune(m: int, n: int, nnz: int, check_every: int = 50, exact: bool = False) -> Tune:

@@ -1,0 +1,66 @@
+"""Host-side packing of the batched engine (no GPU)."""
+
+import numpy as np
+
+from sovereign_solver_amd.engine.batch import LP_DTYPE, OUT_DTYPE, PackedBatch, lanes_per_row
+from sovereign_solver_amd.lp.batch import pack_warm_start
+from sovereign_solver_amd.workloads import random_sparse_lp, transportation_lp, uneven_batch, uneven_sizes
+
+
+def test_descriptor_layout_matches_header():
+    # PdlpBatchLP: 4 int32, 4 int64, 4 double; PdlpBatchOut: 4 int32, 6 double
+    assert LP_DTYPE.itemsize == 4 * 4 + 4 * 8 + 4 * 8
+    assert LP_DTYPE.fields["row_off"][1] == 16 and LP_DTYPE.fields["norm_b"][1] == 48
+    assert OUT_DTYPE.itemsize == 4 * 4 + 6 * 8
+    assert OUT_DTYPE.fields["rel_kkt"][1] == 16
+
+
+def test_packing_offsets_and_contents():
+    probs = [transportation_lp(3, 4, seed=0), random_sparse_lp(5, 9, 0.3, seed=1), random_sparse_lp(7, 6, 0.4, seed=2)]
+    pk = PackedBatch(probs)
+    assert pk.M == sum(p.n_constraints for p in probs) and pk.N == sum(p.n_vars for p in probs)
+    for k, p in enumerate(probs):
+        d = pk.desc[k]
+        m, n = d["m"], d["n"]
+        rp = pk.row_ptr[d["rp_off"]: d["rp_off"] + m + 1]
+        assert rp[0] == 0 and rp[-1] == pk.nnzs[k]
+        ci = pk.col_idx[d["nnz_off"]: d["nnz_off"] + rp[-1]]
+        va = pk.val[d["nnz_off"]: d["nnz_off"] + rp[-1]]
+        dense = np.zeros((m, n))
+        for i in range(m):
+            dense[i, ci[rp[i]:rp[i + 1]]] = va[rp[i]:rp[i + 1]]
+        full = np.vstack([p.A_ub.toarray(), p.A_eq.toarray()])
+        assert np.array_equal(dense, full)
+        assert np.array_equal(pk.c[d["col_off"]: d["col_off"] + n], p.c)
+        kinds = pk.kind[d["row_off"]: d["row_off"] + m]
+        assert list(kinds) == [1] * p.n_ineq + [0] * p.n_eq
+    assert pk.lds_bytes >= max(8 * (p.n_constraints + p.n_vars) for p in probs)
+    assert list(pk.order) == list(np.argsort(-pk.nnzs, kind="stable"))
+
+
+def test_uneven_family_statistics():
+    ms = uneven_sizes(4096)
+    assert ms.min() >= 50 and ms.max() <= 2000
+    # log-uniform: median near sqrt(50*2000) = 316
+    assert 280 < np.median(ms) < 360
+    b = uneven_batch(8, seed=1)
+    for p in b:
+        m, n = p.A_eq.shape
+        assert n == round(2.5 * m)
+        assert abs(p.A_eq.nnz - round(0.01 * m * n)) <= 1
+
+
+def test_lanes_per_row():
+    assert lanes_per_row(1.2) == 1 and lanes_per_row(4) == 2 and lanes_per_row(50) == 16
+    assert lanes_per_row(10_000) == 64
+
+
+def test_warm_start_packing_clips_and_projects():
+    probs = [random_sparse_lp(4, 6, 0.5, seed=3), transportation_lp(2, 3, seed=1)]
+    pk = PackedBatch(probs)
+    ws = [(np.full(6, 100.0), np.ones(4), 2.0), (np.full(6, -1.0), np.ones(5))]
+    x0, y0, w = pack_warm_start(pk, ws)
+    assert np.allclose(x0[:6], probs[0].ub)            # clipped to ub
+    assert np.all(x0[6:] == 0.0)                        # clipped to lb = 0
+    assert np.all(y0[4:6] == 0.0) and np.all(y0[6:] == 1.0)   # <= rows projected to y <= 0
+    assert w[0] == 2.0 and w[1] == 1.0
