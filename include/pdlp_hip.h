@@ -235,6 +235,66 @@ typedef struct PdlpBatch {
 /* Solve every LP of the batch (one launch). */
 int pdlp_batch_solve(const PdlpBatch* batch, void* stream);
 
+/* ----------------------------------------------------------------------------
+ * Learned warm start: a 3-layer MLP (d_in -> d_h -> d_h -> d_out, ReLU) whose
+ * GEMMs run on BF16 MFMA (v_mfma_f32_32x32x16_bf16) with FP32 accumulation.
+ * The reference has no learned component (its PDHG always starts from 0,
+ * pdhg.py:175-176); these kernels are new.  BF16 values are passed as
+ * uint16_t bit patterns.
+ * ------------------------------------------------------------------------- */
+
+/* C[M,N] = epi(alpha * op(A)[M,K] op(B)[K,N] (+ C if accumulate))
+ * op(A)(m,k) = trans_a ? A[k*lda + m] : A[m*lda + k]   (BF16)
+ * op(B)(k,n) = trans_b ? B[n*ldb + k] : B[k*ldb + n]   (BF16)
+ * epilogue, in order: + bias[n] (FP32, may be NULL); ReLU if relu;
+ * zero where mask[m*ldc + n] <= 0 (BF16, may be NULL; the ReLU derivative);
+ * stores to C (FP32, may be NULL) and/or Cb (BF16, may be NULL), both ldc. */
+typedef struct PdlpGemm {
+    int32_t M, N, K;
+    int32_t lda, ldb, ldc;
+    int32_t trans_a, trans_b;
+    int32_t relu, accumulate;
+    float alpha;
+    int32_t pad;
+    const uint16_t* A;
+    const uint16_t* B;
+    const float* bias;
+    const uint16_t* mask;
+    float* C;
+    uint16_t* Cb;
+} PdlpGemm;
+
+int pdlp_gemm_bf16(const PdlpGemm* g, void* stream);
+
+typedef struct PdlpMlp {
+    int32_t batch, d_in, d_h, d_out;
+    const uint16_t* x;                 /* batch x d_in                              */
+    const uint16_t *w1, *w2, *w3;      /* BF16 weights [d_in][d_h], [d_h][d_h], [d_h][d_out] */
+    const float *b1, *b2, *b3;         /* FP32 biases                                */
+    uint16_t *h1, *h2;                 /* batch x d_h, post-ReLU (BF16)              */
+    float* out;                        /* batch x d_out (FP32)                       */
+    /* backward only */
+    const float* target;               /* batch x d_out                              */
+    const float* out_weight;           /* d_out loss weights, NULL = 1               */
+    float* dout;                       /* batch x d_out scratch (FP32)               */
+    uint16_t* dout_b;                  /* batch x d_out scratch (BF16)               */
+    uint16_t *dz2, *dz1;               /* batch x d_h scratch (BF16)                 */
+    float *gw1, *gw2, *gw3;            /* weight gradients (FP32, same layout as w)  */
+    float *gb1, *gb2, *gb3;            /* bias gradients (FP32)                      */
+    float* loss;                       /* 1 float: sum_b,j w_j (out - target)^2 / (batch*d_out) */
+} PdlpMlp;
+
+/* h1 = relu(x w1 + b1), h2 = relu(h1 w2 + b2), out = h2 w3 + b3. */
+int pdlp_mlp_fwd_bf16(const PdlpMlp* mlp, void* stream);
+/* Loss of the last forward pass and all weight/bias gradients. */
+int pdlp_mlp_bwd_bf16(const PdlpMlp* mlp, void* stream);
+/* Adam step on FP32 master weights w (n values) with gradient g, moments m, v;
+ * refreshes the BF16 copy wb (may be NULL).  step is 1-based. */
+int pdlp_adam_step(float* w, const float* g, float* m, float* v, uint16_t* wb, int64_t n, float lr,
+                   float beta1, float beta2, float eps, int32_t step, void* stream);
+/* y = bf16(x), round to nearest even. */
+int pdlp_cast_bf16(const float* x, uint16_t* y, int64_t n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,26 @@ class PdlpBatch(C.Structure):
     ]
 
 
+class PdlpGemm(C.Structure):
+    _fields_ = [
+        ("M", C.c_int32), ("N", C.c_int32), ("K", C.c_int32),
+        ("lda", C.c_int32), ("ldb", C.c_int32), ("ldc", C.c_int32),
+        ("trans_a", C.c_int32), ("trans_b", C.c_int32), ("relu", C.c_int32), ("accumulate", C.c_int32),
+        ("alpha", C.c_float), ("pad", C.c_int32),
+        ("A", _p), ("B", _p), ("bias", _p), ("mask", _p), ("C", _p), ("Cb", _p),
+    ]
+
+
+class PdlpMlp(C.Structure):
+    _fields_ = [
+        ("batch", C.c_int32), ("d_in", C.c_int32), ("d_h", C.c_int32), ("d_out", C.c_int32),
+        ("x", _p), ("w1", _p), ("w2", _p), ("w3", _p), ("b1", _p), ("b2", _p), ("b3", _p),
+        ("h1", _p), ("h2", _p), ("out", _p),
+        ("target", _p), ("out_weight", _p), ("dout", _p), ("dout_b", _p), ("dz2", _p), ("dz1", _p),
+        ("gw1", _p), ("gw2", _p), ("gw3", _p), ("gb1", _p), ("gb2", _p), ("gb3", _p), ("loss", _p),
+    ]
+
+
 class PdlpError(RuntimeError):
     pass
 
@@ -91,6 +111,12 @@ _SIGS = {
     "pdlp_graph_launch": (C.c_int, [_p, _p]),
     "pdlp_graph_destroy": (C.c_int, [_p]),
     "pdlp_batch_solve": (C.c_int, [C.POINTER(PdlpBatch), _p]),
+    "pdlp_gemm_bf16": (C.c_int, [C.POINTER(PdlpGemm), _p]),
+    "pdlp_mlp_fwd_bf16": (C.c_int, [C.POINTER(PdlpMlp), _p]),
+    "pdlp_mlp_bwd_bf16": (C.c_int, [C.POINTER(PdlpMlp), _p]),
+    "pdlp_adam_step": (C.c_int, [_p, _p, _p, _p, _p, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float,
+                                 C.c_int32, _p]),
+    "pdlp_cast_bf16": (C.c_int, [_p, _p, C.c_int64, _p]),
 }
 
 

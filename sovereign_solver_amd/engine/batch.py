@@ -56,28 +56,45 @@ class PackedBatch:
         ns = np.empty(B, np.int64)
         nnzs = np.empty(B, np.int64)
         Ks = []
+        # LPs whose (A_ub, A_eq) are the same objects share one stored CSR
+        mat_of = np.empty(B, np.int64)
+        uniq = {}
+        stored = []
         for k, p in enumerate(self.problems):
-            K = p.A_eq if p.n_ineq == 0 else (p.A_ub if p.n_eq == 0 else sp.vstack([p.A_ub, p.A_eq], format="csr"))
-            K = sp.csr_matrix(K, dtype=np.float64)
-            K.sum_duplicates()
-            K.sort_indices()
+            key = (id(p.A_ub), id(p.A_eq))
+            if key not in uniq:
+                K = p.A_eq if p.n_ineq == 0 else (p.A_ub if p.n_eq == 0 else
+                                                  sp.vstack([p.A_ub, p.A_eq], format="csr"))
+                K = sp.csr_matrix(K, dtype=np.float64)
+                K.sum_duplicates()
+                K.sort_indices()
+                uniq[key] = len(stored)
+                stored.append(K)
+            mat_of[k] = uniq[key]
+            K = stored[mat_of[k]]
             Ks.append(K)
             ms[k], ns[k], nnzs[k] = K.shape[0], K.shape[1], K.nnz
             if ms[k] == 0 or ns[k] == 0:
                 raise ValueError(f"LP {k}: the batched engine needs m > 0 and n > 0")
         self.ms, self.ns, self.nnzs = ms, ns, nnzs
+        self.n_matrices = len(stored)
         row_off = np.concatenate([[0], np.cumsum(ms)[:-1]])
         col_off = np.concatenate([[0], np.cumsum(ns)[:-1]])
-        rp_off = np.concatenate([[0], np.cumsum(ms + 1)[:-1]])
-        nnz_off = np.concatenate([[0], np.cumsum(nnzs)[:-1]])
+        s_m = np.array([K.shape[0] for K in stored], np.int64)
+        s_z = np.array([K.nnz for K in stored], np.int64)
+        s_rp = np.concatenate([[0], np.cumsum(s_m + 1)[:-1]])
+        s_nz = np.concatenate([[0], np.cumsum(s_z)[:-1]])
+        rp_off = s_rp[mat_of]
+        nnz_off = s_nz[mat_of]
         self.row_off, self.col_off, self.nnz_off = row_off, col_off, nnz_off
         M, N, Z = int(ms.sum()), int(ns.sum()), int(nnzs.sum())
-        if Z >= 2**31 or N >= 2**31:
+        if int(s_z.sum()) >= 2**31 or N >= 2**31:
             raise ValueError("batch beyond int32 indexing of the arena")
         self.M, self.N, self.Z = M, N, Z
-        self.row_ptr = np.concatenate([K.indptr.astype(np.int32) for K in Ks])
-        self.col_idx = np.concatenate([K.indices.astype(np.int32) for K in Ks])
-        self.val = np.concatenate([K.data for K in Ks])
+        self.Z_stored = int(s_z.sum())
+        self.row_ptr = np.concatenate([K.indptr.astype(np.int32) for K in stored])
+        self.col_idx = np.concatenate([K.indices.astype(np.int32) for K in stored])
+        self.val = np.concatenate([K.data for K in stored])
         c = np.empty(N)
         lo = np.empty(N)
         hi = np.empty(N)

@@ -1,0 +1,162 @@
+"""Warm-start MLP driven by the HIP kernels of libpdlp_hip.so.
+
+Three layers d_in → d_h → d_h → d_out with ReLU; BF16 weights and
+activations feed MFMA GEMMs with FP32 accumulation; FP32 master weights,
+biases, gradients and Adam moments.  Torch tensors only hold the buffers;
+every arithmetic step (forward, loss gradient, backward GEMMs, bias
+gradients, Adam, casts) is a kernel of the library.
+
+Parameter layout in one flat FP32 buffer: W1 [d_in][d_h], b1, W2 [d_h][d_h],
+b2, W3 [d_h][d_out], b3 (weights [fan_in][fan_out]).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+import torch
+
+from ..engine import _lib as L
+from ..engine.device import require_gpu, stream_handle
+
+
+class WarmStartMLP:
+    def __init__(self, d_in: int, d_h: int, d_out: int, seed: int = 0, device=None):
+        self.lib = L.lib()
+        self.device = device or require_gpu()
+        self.d_in, self.d_h, self.d_out = d_in, d_h, d_out
+        shapes = [("W1", (d_in, d_h)), ("b1", (d_h,)), ("W2", (d_h, d_h)), ("b2", (d_h,)),
+                  ("W3", (d_h, d_out)), ("b3", (d_out,))]
+        self.offsets = {}
+        off = 0
+        for name, shp in shapes:
+            self.offsets[name] = (off, shp)
+            off += int(np.prod(shp))
+        self.n_params = off
+        rng = np.random.default_rng(np.random.SeedSequence([7, seed]))
+        host = np.zeros(off, np.float32)
+        for name, (o, shp) in self.offsets.items():
+            if name.startswith("W"):
+                host[o: o + int(np.prod(shp))] = (rng.standard_normal(int(np.prod(shp))) *
+                                                  np.sqrt(2.0 / shp[0])).astype(np.float32)
+        dv = self.device
+        self.stream = torch.cuda.Stream(device=dv)
+        with torch.cuda.stream(self.stream):
+            self.w = torch.from_numpy(host).to(dv)
+            self.wb = torch.empty(off, dtype=torch.bfloat16, device=dv)
+            self.g = torch.zeros(off, dtype=torch.float32, device=dv)
+            self.m = torch.zeros(off, dtype=torch.float32, device=dv)
+            self.v = torch.zeros(off, dtype=torch.float32, device=dv)
+            self.loss = torch.zeros(1, dtype=torch.float32, device=dv)
+        self.step = 0
+        self._bufs = {}
+        self.refresh_bf16()
+
+    # ---------------------------------------------------------------- helpers
+    @property
+    def sh(self):
+        return stream_handle(self.stream)
+
+    def _p(self, name, t):
+        o, _ = self.offsets[name]
+        return t.data_ptr() + o * t.element_size()
+
+    def refresh_bf16(self):
+        with torch.cuda.stream(self.stream):
+            L.check(self.lib.pdlp_cast_bf16(self.w.data_ptr(), self.wb.data_ptr(), self.n_params, self.sh), "cast")
+
+    def params_numpy(self) -> dict:
+        self.stream.synchronize()
+        w = self.w.cpu().numpy()
+        return {k: w[o: o + int(np.prod(s))].reshape(s).astype(np.float64) for k, (o, s) in self.offsets.items()}
+
+    def set_params(self, P: dict):
+        host = np.zeros(self.n_params, np.float32)
+        for k, (o, s) in self.offsets.items():
+            host[o: o + int(np.prod(s))] = np.asarray(P[k], np.float32).reshape(-1)
+        with torch.cuda.stream(self.stream):
+            self.w.copy_(torch.from_numpy(host).to(self.device))
+        self.refresh_bf16()
+
+    def grads_numpy(self) -> dict:
+        self.stream.synchronize()
+        g = self.g.cpu().numpy()
+        return {k: g[o: o + int(np.prod(s))].reshape(s).astype(np.float64) for k, (o, s) in self.offsets.items()}
+
+    def _buffers(self, B: int):
+        b = self._bufs.get(B)
+        if b is None:
+            dv, bf, f32 = self.device, torch.bfloat16, torch.float32
+            with torch.cuda.stream(self.stream):
+                b = dict(h1=torch.empty(B, self.d_h, dtype=bf, device=dv),
+                         h2=torch.empty(B, self.d_h, dtype=bf, device=dv),
+                         out=torch.empty(B, self.d_out, dtype=f32, device=dv),
+                         dout=torch.empty(B, self.d_out, dtype=f32, device=dv),
+                         dout_b=torch.empty(B, self.d_out, dtype=bf, device=dv),
+                         dz2=torch.empty(B, self.d_h, dtype=bf, device=dv),
+                         dz1=torch.empty(B, self.d_h, dtype=bf, device=dv))
+            self._bufs[B] = b
+        return b
+
+    def _desc(self, xb: torch.Tensor, target: Optional[torch.Tensor] = None,
+              out_weight: Optional[torch.Tensor] = None) -> L.PdlpMlp:
+        B = xb.shape[0]
+        if xb.dtype != torch.bfloat16 or xb.shape[1] != self.d_in or not xb.is_contiguous():
+            raise ValueError("input must be a contiguous [B, d_in] bfloat16 device tensor")
+        bu = self._buffers(B)
+        d = L.PdlpMlp(batch=B, d_in=self.d_in, d_h=self.d_h, d_out=self.d_out, x=xb.data_ptr(),
+                      w1=self._p("W1", self.wb), w2=self._p("W2", self.wb), w3=self._p("W3", self.wb),
+                      b1=self._p("b1", self.w), b2=self._p("b2", self.w), b3=self._p("b3", self.w),
+                      h1=bu["h1"].data_ptr(), h2=bu["h2"].data_ptr(), out=bu["out"].data_ptr())
+        if target is not None:
+            if target.shape != (B, self.d_out) or target.dtype != torch.float32 or not target.is_contiguous():
+                raise ValueError("target must be a contiguous [B, d_out] float32 device tensor")
+            d.target = target.data_ptr()
+            d.out_weight = 0 if out_weight is None else out_weight.data_ptr()
+            d.dout, d.dout_b = bu["dout"].data_ptr(), bu["dout_b"].data_ptr()
+            d.dz2, d.dz1 = bu["dz2"].data_ptr(), bu["dz1"].data_ptr()
+            d.gw1, d.gw2, d.gw3 = self._p("W1", self.g), self._p("W2", self.g), self._p("W3", self.g)
+            d.gb1, d.gb2, d.gb3 = self._p("b1", self.g), self._p("b2", self.g), self._p("b3", self.g)
+            d.loss = self.loss.data_ptr()
+        return d
+
+    # ------------------------------------------------------------------- API
+    def forward(self, xb: torch.Tensor) -> torch.Tensor:
+        """[B, d_out] FP32 output (a view of an internal buffer, valid until the next call)."""
+        d = self._desc(xb)
+        with torch.cuda.stream(self.stream):
+            L.check(self.lib.pdlp_mlp_fwd_bf16(C.byref(d), self.sh), "mlp fwd")
+        return self._buffers(xb.shape[0])["out"]
+
+    def loss_and_grad(self, xb: torch.Tensor, target: torch.Tensor, out_weight=None) -> torch.Tensor:
+        d = self._desc(xb, target, out_weight)
+        with torch.cuda.stream(self.stream):
+            L.check(self.lib.pdlp_mlp_fwd_bf16(C.byref(d), self.sh), "mlp fwd")
+            L.check(self.lib.pdlp_mlp_bwd_bf16(C.byref(d), self.sh), "mlp bwd")
+        return self.loss
+
+    def adam(self, lr: float = 1e-3, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8):
+        self.step += 1
+        with torch.cuda.stream(self.stream):
+            L.check(self.lib.pdlp_adam_step(self.w.data_ptr(), self.g.data_ptr(), self.m.data_ptr(),
+                                            self.v.data_ptr(), self.wb.data_ptr(), self.n_params, lr, beta1, beta2,
+                                            eps, self.step, self.sh), "adam")
+
+    def save(self, path: str, extra: Optional[dict] = None):
+        """Weights (and extra arrays, e.g. target normalisation) as safetensors."""
+        from safetensors.numpy import save_file
+        t = {k: np.ascontiguousarray(v.astype(np.float32)) for k, v in self.params_numpy().items()}
+        for k, v in (extra or {}).items():
+            t[k] = np.ascontiguousarray(np.asarray(v, np.float32))
+        save_file(t, path)
+
+    @staticmethod
+    def load(path: str):
+        from safetensors.numpy import load_file
+        t = load_file(path)
+        m = WarmStartMLP(t["W1"].shape[0], t["W1"].shape[1], t["W3"].shape[1])
+        m.set_params({k: t[k] for k in ("W1", "b1", "W2", "b2", "W3", "b3")})
+        extra = {k: v for k, v in t.items() if k not in m.offsets}
+        return m, extra

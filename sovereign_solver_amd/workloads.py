@@ -83,3 +83,133 @@ def uneven_batch(B: int = 4096, m_lo: int = 50, m_hi: int = 2000, density: float
     ms = uneven_sizes(B, m_lo, m_hi, seed)
     return [random_sparse_lp(int(m), int(round(2.5 * m)), density, seed=seed * 1_000_003 + k, family=2)
             for k, m in enumerate(ms)]
+
+
+# demand profile shape of demos/power_economic_dispatch.py:78-83 (24 hourly values)
+_DEMAND_SHAPE = np.array([300, 280, 270, 260, 280, 350, 450, 600, 700, 750, 780, 800,
+                          790, 800, 780, 750, 720, 700, 680, 700, 720, 680, 600, 450], float)
+
+
+class DispatchFamily:
+    """Parametric economic-dispatch LPs sharing one constraint matrix.
+
+    G generators × T periods, variables p[t*G + g] in [0, Pmax_g]; rows
+    (all ``<=``): T demand rows ``-Σ_g p_gt ≤ -D_t`` and, per generator and
+    period step, ramp-up ``p_gt − p_g,t−1 ≤ R_g`` and ramp-down
+    ``p_g,t−1 − p_gt ≤ R_g`` with R = 0.25·Pmax.  Base data (seed 0):
+    Pmax ~ U[50,400], cost ~ U[20,60], demand shaped like the reference demo's
+    profile scaled to a peak of 0.7·ΣPmax.  Instance k multiplies each D_t and
+    each cost_g by 1 + U[−0.2, 0.2] (seed [3, k]).  p = Pmax is feasible for
+    every instance, so all are feasible and bounded.
+    """
+
+    def __init__(self, G: int = 200, T: int = 24, base_seed: int = 0):
+        rng = _rng(3, 10**9 + base_seed)
+        self.G, self.T = G, T
+        self.pmax = rng.uniform(50.0, 400.0, G)
+        self.ramp = 0.25 * self.pmax
+        self.cost = rng.uniform(20.0, 60.0, G)
+        shape = np.interp(np.linspace(0, 23, T), np.arange(24), _DEMAND_SHAPE)
+        self.demand = shape / shape.max() * 0.7 * self.pmax.sum()
+        n = G * T
+        rows, cols, vals = [], [], []
+        for t in range(T):                                   # demand rows
+            rows.append(np.full(G, t)); cols.append(t * G + np.arange(G)); vals.append(-np.ones(G))
+        r0 = T
+        g = np.arange(G)
+        for t in range(1, T):                                # ramp up / down
+            up = r0 + 2 * ((t - 1) * G + g)
+            rows += [up, up, up + 1, up + 1]
+            cols += [t * G + g, (t - 1) * G + g, (t - 1) * G + g, t * G + g]
+            vals += [np.ones(G), -np.ones(G), np.ones(G), -np.ones(G)]
+        m = T + 2 * G * (T - 1)
+        self.A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(m, n))
+        self.A.sort_indices()
+        self.m, self.n = m, n
+        self.no_eq = sp.csr_matrix((0, n))
+
+    def params(self, k: int):
+        """(demand, cost) of instance k."""
+        rng = _rng(3, k)
+        d = self.demand * (1.0 + rng.uniform(-0.2, 0.2, self.T))
+        c = self.cost * (1.0 + rng.uniform(-0.2, 0.2, self.G))
+        return d, c
+
+    def features(self, d, c) -> np.ndarray:
+        """Network input: demand and cost relative to the base instance."""
+        return np.concatenate([d / self.demand - 1.0, c / self.cost - 1.0])
+
+    def problem(self, k: int) -> Problem:
+        d, c = self.params(k)
+        return self.problem_from(d, c, name=f"dispatch_{k}")
+
+    def rhs(self, d) -> np.ndarray:
+        G, T = self.G, self.T
+        ramp = np.repeat(np.tile(self.ramp, T - 1), 2)       # [up, down] per (t, g)
+        return np.concatenate([-d, ramp])
+
+    def objective(self, c) -> np.ndarray:
+        return np.tile(c, self.T)
+
+    def problem_from(self, d, c, name: str = "dispatch") -> Problem:
+        n = self.n
+        # every instance references the same matrix objects (the batched engine stores them once)
+        return Problem(c=self.objective(c), A_ub=self.A, b_ub=self.rhs(d), A_eq=self.no_eq,
+                       b_eq=np.zeros(0), lb=np.zeros(n), ub=np.tile(self.pmax, self.T),
+                       integer_mask=np.zeros(n, dtype=bool), name=name)
+
+
+class SetCoverNodes:
+    """LP relaxations of a set-covering MILP and branch-and-bound nodes.
+
+    Base (seed [4, 0]): binary A (rows × cols) with ``round(density·rows·cols)``
+    ones at distinct positions, every row covered at least twice; c ~ U[1,100];
+    rows ``−A x ≤ −1``, 0 ≤ x ≤ 1 (``benchmarks/miplib_benchmark.py:133-157``
+    form).  Node k (seed [4, 1+k]) fixes f ~ U{1..50} distinct variables to
+    0 or 1 (lb = ub), tightening bounds only, as node problems are built in
+    ``solver/milp/branch_and_bound.py:95-113``.  Fixings that leave a row with
+    no free or one-fixed variable are redrawn, so every node is feasible.
+    """
+
+    def __init__(self, rows: int = 1000, cols: int = 10000, density: float = 0.005, base_seed: int = 0):
+        rng = _rng(4, base_seed)
+        k = int(round(density * rows * cols))
+        pos = np.sort(rng.choice(rows * cols, size=k, replace=False))
+        r, c = pos // cols, pos % cols
+        A = sp.csr_matrix((np.ones(k), (r, c)), shape=(rows, cols)).tolil()
+        cnt = np.asarray(A.sum(axis=1)).ravel()
+        for i in np.flatnonzero(cnt < 2):
+            extra = rng.choice(cols, size=2, replace=False)
+            A[i, extra] = 1.0
+        self.A = sp.csr_matrix(A)
+        self.A.sum_duplicates()
+        self.A.data[:] = 1.0
+        self.A.sort_indices()
+        self.c = rng.uniform(1.0, 100.0, cols)
+        self.rows, self.cols = rows, cols
+        # shared by every node problem (the batched engine stores them once)
+        self.negA = sp.csr_matrix(-self.A)
+        self.no_eq = sp.csr_matrix((0, cols))
+
+    def root(self) -> Problem:
+        return self.node_from(np.zeros(self.cols), np.ones(self.cols), name="setcover_root")
+
+    def node_from(self, lb, ub, name="setcover_node") -> Problem:
+        return Problem(c=self.c.copy(), A_ub=self.negA, b_ub=-np.ones(self.rows), A_eq=self.no_eq,
+                       b_eq=np.zeros(0), lb=np.asarray(lb, float), ub=np.asarray(ub, float),
+                       integer_mask=np.zeros(self.cols, dtype=bool), name=name)
+
+    def node(self, k: int) -> Problem:
+        rng = _rng(4, 1 + k)
+        for _ in range(100):
+            f = int(rng.integers(1, 51))
+            idx = rng.choice(self.cols, size=f, replace=False)
+            val = rng.integers(0, 2, size=f).astype(float)
+            lb = np.zeros(self.cols)
+            ub = np.ones(self.cols)
+            lb[idx] = val
+            ub[idx] = val
+            # every row must keep a variable that can be 1
+            if np.all(self.A @ ub >= 1.0):
+                return self.node_from(lb, ub, name=f"setcover_node_{k}")
+        raise RuntimeError("could not draw a feasible node")

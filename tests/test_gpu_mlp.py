@@ -1,0 +1,104 @@
+"""GPU checks of the BF16-MFMA MLP kernels against the FP64 oracle (needs an MI355X)."""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp_ref
+from sovereign_solver_amd.engine import _lib as L
+from sovereign_solver_amd.learned.mlp import WarmStartMLP
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(dev).to(torch.bfloat16).contiguous()
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (70, 131, 45), (5, 300, 17)])
+def test_gemm_matches_fp64(gpu, ta, tb, M, N, K):
+    rng = np.random.default_rng(M * 7 + N + K)
+    A = mlp_ref.bf16_round(rng.standard_normal((M, K)))
+    B = mlp_ref.bf16_round(rng.standard_normal((K, N)))
+    bias = rng.standard_normal(N).astype(np.float32)
+    At = _bf(A.T if ta else A, gpu)
+    Bt = _bf(B.T if tb else B, gpu)
+    bt = torch.from_numpy(bias).to(gpu)
+    Ct = torch.zeros(M, N, dtype=torch.float32, device=gpu)
+    g = L.PdlpGemm(M=M, N=N, K=K, lda=(M if ta else K), ldb=(K if tb else N), ldc=N, trans_a=ta, trans_b=tb,
+                   relu=0, accumulate=0, alpha=1.0, A=At.data_ptr(), B=Bt.data_ptr(), bias=bt.data_ptr(),
+                   mask=0, C=Ct.data_ptr(), Cb=0)
+    L.check(L.lib().pdlp_gemm_bf16(C.byref(g), None), "gemm")
+    torch.cuda.synchronize()
+    want = A @ B + bias
+    got = Ct.cpu().numpy()
+    bound = (K + 2) * 2.0**-23 * (np.abs(A) @ np.abs(B) + np.abs(bias)) + 1e-6
+    assert np.all(np.abs(got - want) <= bound), np.max(np.abs(got - want) / bound)
+
+
+def test_gemm_identity_pins_lane_layout(gpu):
+    """A x I = A and I x B = B: a transposed or shifted operand/result layout breaks these."""
+    rng = np.random.default_rng(3)
+    A = mlp_ref.bf16_round(rng.standard_normal((64, 64)))
+    I = np.eye(64)
+    for left, right in ((A, I), (I, A)):
+        At, Bt = _bf(left, gpu), _bf(right, gpu)
+        Ct = torch.zeros(64, 64, dtype=torch.float32, device=gpu)
+        g = L.PdlpGemm(M=64, N=64, K=64, lda=64, ldb=64, ldc=64, alpha=1.0, A=At.data_ptr(), B=Bt.data_ptr(),
+                       C=Ct.data_ptr())
+        L.check(L.lib().pdlp_gemm_bf16(C.byref(g), None), "gemm")
+        torch.cuda.synchronize()
+        assert np.array_equal(Ct.cpu().numpy(), A)
+
+
+def test_mlp_forward_backward_vs_oracle(gpu):
+    rng = np.random.default_rng(0)
+    B, di, dh, do = 96, 24, 64, 200
+    mlp = WarmStartMLP(di, dh, do, seed=1)
+    P = mlp.params_numpy()
+    P["b1"] = rng.standard_normal(dh) * 0.1
+    P["b3"] = rng.standard_normal(do) * 0.1
+    mlp.set_params(P)
+    x = rng.standard_normal((B, di))
+    t = rng.standard_normal((B, do))
+    w = rng.uniform(0.5, 2.0, do)
+    xb = _bf(mlp_ref.bf16_round(x), gpu)
+    tt = torch.from_numpy(t.astype(np.float32)).to(gpu)
+    wt = torch.from_numpy(w.astype(np.float32)).to(gpu)
+    loss = float(mlp.loss_and_grad(xb, tt, wt).item())
+    G = mlp.grads_numpy()
+    P32 = {k: v.astype(np.float32).astype(np.float64) for k, v in mlp.params_numpy().items()}
+    ref_loss, ref_g, ref_out = mlp_ref.loss_and_grads(x, t.astype(np.float32).astype(np.float64), **P32, w=w,
+                                                      rnd=mlp_ref.bf16_round)
+    assert abs(loss - ref_loss) <= 1e-3 * ref_loss
+    for k in ref_g:
+        err = np.linalg.norm(G[k] - ref_g[k]) / max(np.linalg.norm(ref_g[k]), 1e-30)
+        assert err < 2e-2, (k, err)
+
+
+def test_adam_step_matches_formula(gpu):
+    mlp = WarmStartMLP(8, 16, 4, seed=2)
+    w0 = mlp.w.cpu().numpy().astype(np.float64)
+    g = np.random.default_rng(1).standard_normal(mlp.n_params).astype(np.float32)
+    mlp.g.copy_(torch.from_numpy(g).to(gpu))
+    mlp.adam(lr=1e-2)
+    mlp.stream.synchronize()
+    m = 0.1 * g
+    v = 0.001 * g.astype(np.float64) ** 2
+    want = w0 - 1e-2 * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
+    np.testing.assert_allclose(mlp.w.cpu().numpy(), want, rtol=1e-5, atol=1e-6)
+    wb = mlp.wb.float().cpu().numpy()
+    np.testing.assert_allclose(wb, mlp_ref.bf16_round(mlp.w.cpu().numpy()), rtol=0, atol=0)
+
+
+def test_training_reduces_loss(gpu):
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((2048, 16))
+    Wt = rng.standard_normal((16, 40))
+    T = np.tanh(X @ Wt) + 0.5 * (X[:, :1] ** 2)
+    from sovereign_solver_amd.learned.dispatch import train
+    _, _, _, losses = train(X, T, d_h=64, epochs=15, batch=256, lr=3e-3, step_weight=1.0)
+    assert losses[-1] < 0.25 * losses[0], losses
