@@ -46,14 +46,14 @@ def primal_bytes(form, elem: int) -> float:
         b += 2 * n * elem                     # lo, hi
     b += 2 * n * elem                         # x read + write
     b += n * elem                             # xbar write
-    b += 2 * n * 8                            # x_sum read + write (FP64)
+    b += 2 * n * 4                            # FP32 partial sum read + write
     return float(b)
 
 
 def dual_bytes(form, elem: int) -> float:
     n, m, nnz = form.n, form.m, form.nnz
     b = nnz * (elem + 4) + (m + 1) * 4 + n * elem   # K CSR + xbar gathered once
-    b += m * elem + 2 * m * elem + 2 * m * 8        # b, y r/w, y_sum r/w
+    b += m * elem + 2 * m * elem + 2 * m * 4        # b, y r/w, FP32 partial sum r/w
     if form.row_kind is not None:
         b += m
     return float(b)
@@ -221,7 +221,104 @@ def bench_batch(args):
                      "traffic": None, "bytes_per_launch": bytes_alg, "us_per_launch": times[-1][1] * 1e6},
         "cpu_baseline": cpu,
     }
+    if args.dump:
+        np.savez(args.dump, m=pk.ms, n=pk.ns, nnz=pk.nnzs, iterations=outs[-1]["iterations"],
+                 status=outs[-1]["status"], restarts=outs[-1]["restarts"])
+    calib_copy(args)
     print(json.dumps(out), flush=True)
+
+
+def bench_learned(args):
+    """Learned warm start on the economic-dispatch family (200 generators x 24 periods)."""
+    import numpy as np
+    import torch
+    from sovereign_solver_amd.engine.forms import box_form
+    from sovereign_solver_amd.engine.single import SingleEngine
+    from sovereign_solver_amd.learned import dispatch as D
+    from sovereign_solver_amd.workloads import DispatchFamily
+
+    fam = DispatchFamily()
+    log(f"dispatch family: m={fam.m} n={fam.n} nnz={fam.A.nnz}")
+    train_ids = np.arange(args.train)
+    eval_ids = 10**6 + np.arange(args.eval)
+    t0 = time.perf_counter()
+    lab = D.label(fam, train_ids, tol=args.tol, max_iters=args.max_iters)
+    t_label = time.perf_counter() - t0
+    ok = int((lab.status == "optimal").sum())
+    log(f"labelled {len(train_ids)} instances in {t_label:.1f}s (engine {lab.seconds:.1f}s), optimal {ok}, "
+        f"iterations mean {lab.iterations.mean():.0f}")
+    keep = lab.status == "optimal"
+    feats = D.features_of(fam, train_ids)[keep]
+    targets = D.targets_of(lab)[keep]
+    t1 = time.perf_counter()
+    mlp, nin, nout, losses = D.train(feats, targets, d_h=512, epochs=args.epochs, batch=256, lr=1e-3, log=log)
+    t_train = time.perf_counter() - t1
+    log(f"trained in {t_train:.1f}s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+
+    # held-out: cold vs learned warm start, same batched engine
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    warm = D.predict_warm_start(mlp, nin, nout, fam, eval_ids, use_step=not args.no_step)
+    t_pred = time.perf_counter() - t2
+    ev = D.evaluate(fam, eval_ids, warm, tol=args.tol, max_iters=args.max_iters)
+    n_eval = len(eval_ids)
+    cold_lps = ev["cold_ok"] / ev["cold_s"]
+    warm_lps = ev["warm_ok"] / (ev["warm_s"] + t_pred)
+    log(f"held-out {n_eval}: cold {ev['cold_ok']} ok, iters mean {ev['cold_iters'].mean():.0f}, {ev['cold_s']:.2f}s "
+        f"({cold_lps:.1f} LPs/s); warm {ev['warm_ok']} ok, iters mean {ev['warm_iters'].mean():.0f}, "
+        f"{ev['warm_s']:.2f}s + predict {t_pred:.2f}s ({warm_lps:.1f} LPs/s)")
+    # objective error of the warm results against HiGHS dual simplex on a sample
+    from scipy.optimize import linprog
+    errs = []
+    for k in range(min(8, n_eval)):
+        p = fam.problem(int(eval_ids[k]))
+        h = linprog(p.c, A_ub=p.A_ub, b_ub=p.b_ub, bounds=list(zip(p.lb, p.ub)), method="highs-ds")
+        errs.append(abs(ev["warm"][k].objective - h.fun) / (1 + abs(h.fun)))
+    # plain single-instance GPU PDHG on a sample of the held-out set
+    t_plain, ok_plain = 0.0, 0
+    for k in eval_ids[: args.plain_sample]:
+        se = SingleEngine(box_form(fam.problem(int(k))), "fp32")
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        r = se.solve(args.tol, args.max_iters)
+        t_plain += time.perf_counter() - t3
+        ok_plain += r["status"] == "optimal"
+        se.drop_graphs()
+    plain = ok_plain / t_plain
+    log(f"plain single-instance GPU PDHG: {ok_plain}/{args.plain_sample} optimal, {plain:.2f} LPs/s; "
+        f"objective rel err vs HiGHS (warm, 8 LPs) max {max(errs):.2e}")
+    out = {
+        "metric": "LPs/sec to 1e-4 rel KKT: learned+batched vs plain PDHG, 1 GPU; SpMV HBM GB/s",
+        "value": warm_lps, "unit": "LPs/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": (ev["warm_s"] + t_pred) * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 (MLP bf16 x bf16 -> fp32)", "data": "synthetic dispatch family",
+        "config": {"workload": f"learned warm start, economic dispatch 200x24 (m={fam.m}, n={fam.n}), "
+                               f"{args.train} PDHG-labelled training instances, {n_eval} held-out, batched",
+                   "train_label_s": t_label, "train_s": t_train, "epochs": args.epochs,
+                   "loss_first_last": [losses[0], losses[-1]],
+                   "cold_batched_lps_per_s": cold_lps, "warm_batched_lps_per_s": warm_lps,
+                   "cold_iters_mean": float(ev["cold_iters"].mean()), "warm_iters_mean": float(ev["warm_iters"].mean()),
+                   "cold_ok": int(ev["cold_ok"]), "warm_ok": int(ev["warm_ok"]),
+                   "plain_single_gpu_lps_per_s": plain, "plain_sample": args.plain_sample,
+                   "speedup_vs_plain": warm_lps / plain if plain > 0 else None,
+                   "objective_rel_err_vs_highs_max": max(errs), "use_step": not args.no_step},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def calib_copy(args):
+    """Three 1 GiB device-to-device copies: known bytes for the HBM counter scale."""
+    if not args.calib_copy:
+        return
+    import torch
+    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("calibration copies: 3 x 1 GiB read + 1 GiB written")
 
 
 def main():
@@ -229,9 +326,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="transport", choices=["transport", "batch"])
+    ap.add_argument("--workload", default="transport", choices=["transport", "batch", "learned"])
+    ap.add_argument("--train", type=int, default=4096)
+    ap.add_argument("--eval", type=int, default=1024)
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--no-step", action="store_true", help="learned: do not use the predicted step")
     ap.add_argument("--batch", type=int, default=4096)
-    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--block", type=int, default=0, help="batch workgroup size (0 = by LDS need)")
     ap.add_argument("--plain-sample", type=int, default=32)
     ap.add_argument("--cpu-cap-s", type=float, default=20.0)
     ap.add_argument("--sources", type=int, default=1000)
@@ -242,7 +343,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ref-iters", action="store_true",
                     help="skip the reference-form GPU run that counts the reference algorithm's iterations")
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="use the heuristic launch shape")
+    ap.add_argument("--no-highs", action="store_true", help="skip the HiGHS objective reference")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="end with 1 GiB device copies (calibration of the HBM byte counters)")
+    ap.add_argument("--dump", default="", help="batch workload: write per-LP sizes/iterations (.npz)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -251,6 +356,8 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "learned":
+        return bench_learned(args)
 
     import numpy as np
     import torch
@@ -265,7 +372,7 @@ def main():
     log(f"built transport LP {args.sources}x{args.sinks}: m={form.m} n={form.n} nnz={form.nnz} "
         f"({time.perf_counter() - t0:.1f}s)")
     eng = SingleEngine(form, "fp32")
-    if args.autotune:
+    if not args.no_autotune:
         eng.set_tune(T.autotune(eng))
     log(f"tune {eng.tune}")
 
@@ -292,14 +399,17 @@ def main():
     x, y = results[-1]["x"], results[-1]["y"]
     obj = float(p.c @ x)
     obj_ref = None
-    try:
+    if args.no_highs:
+        pass
+    else:
+      try:
         from scipy.optimize import linprog
         t1 = time.perf_counter()
         lr = linprog(p.c, A_ub=p.A_ub, b_ub=p.b_ub, A_eq=p.A_eq, b_eq=p.b_eq, bounds=(0, None), method="highs-ds")
         if lr.status == 0:
             obj_ref = float(lr.fun)
         log(f"HiGHS-ds objective {obj_ref} ({time.perf_counter() - t1:.1f}s)")
-    except Exception as e:                       # pragma: no cover
+      except Exception as e:                       # pragma: no cover
         log(f"HiGHS reference unavailable: {e}")
 
     # roofline of the dominant kernel: event-timed back-to-back launches
@@ -345,6 +455,7 @@ def main():
                      "dual_kernel": {"achieved": gbs_d, "bytes_per_launch": bd, "us_per_launch": us_d}},
         "cpu_baseline": cpu,
     }
+    calib_copy(args)
     print(json.dumps(out), flush=True)
 
 

@@ -119,6 +119,8 @@ typedef struct PdlpTune {
                              a contiguous range of rows */
     int32_t exact;        /* 1: sequential row sums, no FMA contraction (bitwise
                              agreement with SciPy/NumPy in PDLP_F64); needs tpr 1 */
+    int32_t block_dual;   /* threads per workgroup of the dual step for tpr_dual <= 64;
+                             0 = same as block */
 } PdlpTune;
 
 int         pdlp_abi_version(void);
@@ -168,6 +170,12 @@ int pdlp_graph_capture(const PdlpDev* dev, const PdlpTune* tune, int32_t blocks,
                        void* stream, void** exec);
 int pdlp_graph_launch(void* exec, void* stream);
 int pdlp_graph_destroy(void* exec);
+
+/* Capture `reps` back-to-back launches of one step kernel (which: 0 = primal,
+ * 1 = dual) in a HIP graph: the timing harness of the autotuner and of the
+ * roofline figure (per-kernel time without host launch gaps). */
+int pdlp_graph_capture_step(const PdlpDev* dev, const PdlpTune* tune, int32_t which, int32_t reps,
+                            void* stream, void** exec);
 
 /* ----------------------------------------------------------------------------
  * Batched engine: thousands of independent box-form LPs in one persistent launch.

@@ -359,6 +359,8 @@ int check_tune(const PdlpTune* t) {
         return fail(PDLP_E_TUNE, "tpr must be a power of two in [1, 1024]");
     if (t->block < 64 || t->block > 1024 || (t->block & 63))
         return fail(PDLP_E_TUNE, "block must be a multiple of 64 in [64, 1024]");
+    if (t->block_dual != 0 && (t->block_dual < 64 || t->block_dual > 1024 || (t->block_dual & 63)))
+        return fail(PDLP_E_TUNE, "block_dual must be 0 or a multiple of 64 in [64, 1024]");
     if (t->check_every <= 0) return fail(PDLP_E_TUNE, "check_every must be positive");
     if (t->exact && (t->tpr_primal != 1 || t->tpr_dual != 1))
         return fail(PDLP_E_TUNE, "exact mode needs tpr_primal = tpr_dual = 1");
@@ -402,7 +404,7 @@ template <typename T, bool EXACT>
 int launch_dual_t(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
     const int remap = (t.xcd_remap >> 1) & 1;
     unsigned g, nt;
-    shape_for(d.m, t.tpr_dual, t.block, remap, g, nt);
+    shape_for(d.m, t.tpr_dual, (t.block_dual > 0 ? t.block_dual : t.block), remap, g, nt);
     if constexpr (EXACT) {
         dual_kernel<T, 1, true><<<g, nt, 0, s>>>(d, remap);
     } else {
@@ -523,6 +525,32 @@ int pdlp_graph_capture(const PdlpDev* dev, const PdlpTune* tune, int32_t blocks,
     hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
     for (int k = 0; k < blocks && rc == 0; ++k) rc = run(*dev, *tune, tune->check_every, 1, s);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    *exec = (void*)ge;
+    return 0;
+}
+
+int pdlp_graph_capture_step(const PdlpDev* dev, const PdlpTune* tune, int32_t which, int32_t reps, void* stream,
+                            void** exec) {
+    int rc = check_dev(dev);
+    if (rc || (rc = check_tune(tune))) return rc;
+    if (reps <= 0 || exec == nullptr || (which != 0 && which != 1))
+        return fail(PDLP_E_ARG, "reps > 0, which in {0, 1} and exec non-NULL");
+    if (stream == nullptr) return fail(PDLP_E_ARG, "graph capture needs a non-default stream");
+    hipStream_t s = as_stream(stream);
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+    for (int k = 0; k < reps && rc == 0; ++k) rc = which == 0 ? primal(*dev, *tune, s) : dual(*dev, *tune, s);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(s, &g);
     if (rc) {

@@ -50,7 +50,7 @@ class PdlpDev(C.Structure):
 class PdlpTune(C.Structure):
     _fields_ = [
         ("tpr_primal", C.c_int32), ("tpr_dual", C.c_int32), ("block", C.c_int32),
-        ("check_every", C.c_int32), ("xcd_remap", C.c_int32), ("exact", C.c_int32),
+        ("check_every", C.c_int32), ("xcd_remap", C.c_int32), ("exact", C.c_int32), ("block_dual", C.c_int32),
     ]
 
 
@@ -110,6 +110,8 @@ _SIGS = {
                                      C.POINTER(C.c_void_p)]),
     "pdlp_graph_launch": (C.c_int, [_p, _p]),
     "pdlp_graph_destroy": (C.c_int, [_p]),
+    "pdlp_graph_capture_step": (C.c_int, [C.POINTER(PdlpDev), C.POINTER(PdlpTune), C.c_int32, C.c_int32, _p,
+                                          C.POINTER(C.c_void_p)]),
     "pdlp_batch_solve": (C.c_int, [C.POINTER(PdlpBatch), _p]),
     "pdlp_gemm_bf16": (C.c_int, [C.POINTER(PdlpGemm), _p]),
     "pdlp_mlp_fwd_bf16": (C.c_int, [C.POINTER(PdlpMlp), _p]),

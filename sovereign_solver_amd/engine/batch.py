@@ -133,11 +133,12 @@ class PackedBatch:
 
 
 class BatchEngine:
-    def __init__(self, packed: PackedBatch, block: int = 256, grid: int = 0):
+    def __init__(self, packed: PackedBatch, block: int = 0, grid: int = 0):
         self.lib = L.lib()
         self.device = require_gpu()
         self.pk = packed
-        self.block = block
+        # block 0: 256 threads while two workgroups fit a CU's 160 KB of LDS, else 512
+        self.block = block or (256 if packed.lds_bytes <= 80 * 1024 else 512)
         self.grid = grid
         self.stream = torch.cuda.Stream(device=self.device)
         dv = self.device

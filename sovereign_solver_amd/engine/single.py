@@ -24,7 +24,9 @@ from .tune import Tune, heuristic_tune
 
 _STATUS = {L.PDLP_RUNNING: "iteration_limit", L.PDLP_OPTIMAL: "optimal", L.PDLP_NUMERICAL: "numerical_error"}
 
-MAX_GRAPH_BLOCKS = 16
+# Largest graph (in check blocks).  Small graphs bound the surplus of no-op
+# launches after convergence (at most two graphs are in flight).
+MAX_GRAPH_BLOCKS = 4
 
 
 class SingleEngine:
@@ -227,20 +229,24 @@ class SingleEngine:
 
     # --------------------------------------------------------------- timing
     def time_kernel(self, which: str, t: Tune, reps: int = 20) -> float:
-        """Mean µs per launch of one step kernel under tuning ``t`` (HIP events on this stream)."""
+        """Mean µs per launch of one step kernel under tuning ``t``: a HIP graph of
+        ``reps`` back-to-back launches timed with HIP events on this stream."""
         ct = t.to_c()
-        fn = {"primal": self.lib.pdlp_primal_step, "dual": self.lib.pdlp_dual_step}[which]
         st = L.PdlpCtrl()
         st.tol = 0.0
         st.norm_b, st.norm_c, st.tau_scale, st.sigma_scale = self.form.norm_b, self.form.norm_c, 1.0, 1.0
         self.write_ctrl(st)
-        with torch.cuda.stream(self.stream):
-            for _ in range(3):
-                L.check(fn(C.byref(self.dev), C.byref(ct), self.sh), which)
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(self.stream)
-            for _ in range(reps):
-                L.check(fn(C.byref(self.dev), C.byref(ct), self.sh), which)
-            e1.record(self.stream)
-        self.stream.synchronize()
+        h = C.c_void_p()
+        L.check(self.lib.pdlp_graph_capture_step(C.byref(self.dev), C.byref(ct), {"primal": 0, "dual": 1}[which],
+                                                 reps, self.sh, C.byref(h)), "pdlp_graph_capture_step")
+        try:
+            with torch.cuda.stream(self.stream):
+                L.check(self.lib.pdlp_graph_launch(h, self.sh), "warm-up launch")
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(self.stream)
+                L.check(self.lib.pdlp_graph_launch(h, self.sh), "timed launch")
+                e1.record(self.stream)
+            self.stream.synchronize()
+        finally:
+            self.lib.pdlp_graph_destroy(h)
         return e0.elapsed_time(e1) * 1000.0 / reps

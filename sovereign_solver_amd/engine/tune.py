@@ -32,10 +32,12 @@ class Tune:
     check_every: int = 50
     xcd_remap: int = 0
     exact: int = 0
+    block_dual: int = 0
 
     def to_c(self):
         from ._lib import PdlpTune
-        return PdlpTune(self.tpr_primal, self.tpr_dual, self.block, self.check_every, self.xcd_remap, self.exact)
+        return PdlpTune(self.tpr_primal, self.tpr_dual, self.block, self.check_every, self.xcd_remap, self.exact,
+                        self.block_dual)
 
 
 def _tpr_for(mean_len: float) -> int:
@@ -127,12 +129,15 @@ def autotune(engine, reps: int = 20, use_cache: bool = True) -> Tune:
                 best_p, best_pt = (tp, remap), us
     best_d, best_dt = None, float("inf")
     for td in _neighbours(base.tpr_dual):
-        for remap in (0, 1):
-            t = Tune(best_p[0], td, 256, base.check_every, remap << 1, 0)
-            us = engine.time_kernel("dual", t, reps)
-            timings[f"dual/tpr{td}/remap{remap}"] = us
-            if us < best_dt:
-                best_d, best_dt = (td, remap), us
-    t = Tune(best_p[0], best_d[0], 256, base.check_every, best_p[1] | (best_d[1] << 1), 0)
+        # several rows per workgroup share gathered cache lines in L1 when rows are short enough
+        blocks = (256, 1024) if td <= 64 else (0,)
+        for bd in blocks:
+            for remap in (0, 1):
+                t = Tune(best_p[0], td, 256, base.check_every, remap << 1, 0, bd)
+                us = engine.time_kernel("dual", t, reps)
+                timings[f"dual/tpr{td}/block{bd}/remap{remap}"] = us
+                if us < best_dt:
+                    best_d, best_dt = (td, remap, bd), us
+    t = Tune(best_p[0], best_d[0], 256, base.check_every, best_p[1] | (best_d[1] << 1), 0, best_d[2])
     _store_cache(key, t, timings)
     return t

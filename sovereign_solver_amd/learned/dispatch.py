@@ -60,7 +60,7 @@ class Labels:
 
 
 def solve_family(fam: DispatchFamily, ids, tol: float, max_iters: int, warm=None, chunk: int = 2048,
-                 block: int = 256):
+                 block: int = 0):
     """Batched solves of family instances; returns SolveResults and engine seconds (launch to results)."""
     res = []
     secs = 0.0

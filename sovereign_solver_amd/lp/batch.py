@@ -68,7 +68,7 @@ def solve_lp_pdhg_batch(
     max_iters: int = MAX_PDHG_ITERS,
     *,
     check_every: int = 50,
-    block: int = 256,
+    block: int = 0,
 ) -> list:
     problems = list(problems)
     if not problems:
