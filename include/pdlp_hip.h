@@ -48,9 +48,24 @@ extern "C" {
 #define PDLP_ROW_EQ 0        /* K_i x  = b_i, dual free     */
 #define PDLP_ROW_LE 1        /* K_i x <= b_i, dual y_i <= 0 */
 
-/* Number of doubles of PdlpDev.partials the engine needs. */
+/* restart schemes (PdlpTune.scheme, PdlpBatchLP.flags bit 0) */
+#define PDLP_SCHEME_REFERENCE 0  /* pdhg.py:225-244: window average, restart when
+                                    KKT < 0.5 x KKT at the last restart, fixed steps */
+#define PDLP_SCHEME_ADAPTIVE  1  /* candidate = better of average / current iterate;
+                                    restart on sufficient (0.2) or necessary (0.8)
+                                    decay without progress, or when the window
+                                    exceeds 0.36 x all iterations; at each restart
+                                    the primal weight moves (geometric mean) toward
+                                    the ratio |dx| / |dy| of the distances moved
+                                    since the last restart, in the step-size metric */
+#define PDLP_SCHEME_ADAPTIVE_FIXED 2  /* adaptive restarts, primal weight kept fixed */
+#define PDLP_SCHEME_ADAPTIVE_START 3  /* adaptive restarts, weight toward |x - x0| / |y - y0|
+                                         with (x0, y0) the starting point */
+
+/* Number of doubles of PdlpDev.partials the engine needs:
+ * 2 candidates x 2 passes x PDLP_PARTIAL_BLOCKS blocks x 6 sums. */
 #define PDLP_PARTIAL_BLOCKS 1024
-#define PDLP_PARTIALS_LEN   (2 * PDLP_PARTIAL_BLOCKS * 4)
+#define PDLP_PARTIALS_LEN   (4 * PDLP_PARTIAL_BLOCKS * 6)
 
 /*
  * Device control block, one per instance, device-resident.  The engine
@@ -71,9 +86,13 @@ typedef struct PdlpCtrl {
     double last_err, rp, rd, gap;
     double tol;
     double norm_b, norm_c;   /* max(1,||b||), max(1,||c||) of the reference form */
-    double tau_scale;        /* multiplies tau_j (primal weight); 1 = reference */
-    double sigma_scale;      /* multiplies sigma_i; 1 = reference               */
+    double tau_scale;        /* multiplies tau_j (primal weight w); 1 = reference */
+    double sigma_scale;      /* multiplies sigma_i (1/w); 1 = reference          */
     double p_obj, d_obj;     /* objectives of the last check (reference form)   */
+    /* adaptive scheme */
+    double err_prev;         /* candidate KKT at the previous check (+inf after a restart) */
+    int32_t cand;            /* candidate of the pending restart: 0 current, 1 average */
+    int32_t last_restart;    /* iteration of the last restart                   */
 } PdlpCtrl;
 
 /*
@@ -106,6 +125,8 @@ typedef struct PdlpDev {
                                  NULL = add into x_sum/y_sum every iteration (PDLP_F64) */
     double *partials;         /* PDLP_PARTIALS_LEN doubles of workspace */
     PdlpCtrl *ctrl;
+    void *x_last, *y_last;    /* n, m; prec: last restart point (adaptive scheme; may be
+                                 NULL for the reference scheme) */
 } PdlpDev;
 
 /* Launch shape, chosen by the autotuner (sovereign_solver_amd.engine.tune). */
@@ -121,6 +142,7 @@ typedef struct PdlpTune {
                              agreement with SciPy/NumPy in PDLP_F64); needs tpr 1 */
     int32_t block_dual;   /* threads per workgroup of the dual step for tpr_dual <= 64;
                              0 = same as block */
+    int32_t scheme;       /* PDLP_SCHEME_REFERENCE or PDLP_SCHEME_ADAPTIVE */
 } PdlpTune;
 
 int         pdlp_abi_version(void);
@@ -197,7 +219,10 @@ int pdlp_graph_capture_step(const PdlpDev* dev, const PdlpTune* tune, int32_t wh
 typedef struct PdlpBatchLP {
     int32_t m, n;
     int32_t tpr;            /* lanes per row: 1,2,4,8,16,32,64 */
-    int32_t flags;          /* reserved, 0 */
+    int32_t flags;          /* bit 0: adaptive restarts (else the reference scheme);
+                               bit 1: keep the primal weight fixed;
+                               bit 2: weight from the distances to the starting point
+                                      (else: moved since the last restart) */
     int64_t row_off;        /* first row of this LP in the row arrays        */
     int64_t col_off;        /* first column of this LP in the column arrays  */
     int64_t rp_off;         /* first of its m+1 row pointers                 */
@@ -213,6 +238,7 @@ typedef struct PdlpBatchOut {
     int32_t restarts;
     int32_t checks;
     double rel_kkt, rp, rd, gap, p_obj, d_obj;
+    double weight;          /* primal weight at exit (to resume a related solve) */
 } PdlpBatchOut;
 
 typedef struct PdlpBatch {
@@ -238,6 +264,7 @@ typedef struct PdlpBatch {
     float* x; float* x_psum; double* x_sum; float* tau;   /* per column */
     float* y; double* y_sum;                               /* per row    */
     PdlpBatchOut* out;
+    float* x_last; float* y_last;   /* per column / row: last restart point (scratch) */
 } PdlpBatch;
 
 /* Solve every LP of the batch (one launch). */

@@ -24,8 +24,6 @@
 namespace pdlp {
 namespace {
 
-constexpr int kDecContinue = 0, kDecOptimal = 1, kDecRestart = 2, kDecNumerical = 3;
-
 struct LPView {
     int m, n;
     const int32_t* rp;     // m+1, relative to nnz base
@@ -40,8 +38,12 @@ struct LPView {
     double* x_sum;
     float* y;
     double* y_sum;
+    float *x_last, *y_last;   // last restart point (adaptive scheme)
     double norm_b, norm_c;
-    float tau_scale, sigma_scale;
+    float weight;             // starting primal weight w: tau = w tau_hat, sigma = sigma_hat / w
+    bool adaptive;            // adaptive restarts (flags bit 0)
+    bool fixed_weight;        // keep the primal weight (flags bit 1)
+    bool move_ref;            // distances since the last restart (flags bit 2 clear) or from the start
 };
 
 __device__ __forceinline__ LPView view_of(const PdlpBatch& a, const PdlpBatchLP& lp) {
@@ -67,10 +69,14 @@ __device__ __forceinline__ LPView view_of(const PdlpBatch& a, const PdlpBatchLP&
     v.tau = a.tau + lp.col_off;
     v.y = a.y + lp.row_off;
     v.y_sum = a.y_sum + lp.row_off;
+    v.x_last = a.x_last + lp.col_off;
+    v.y_last = a.y_last + lp.row_off;
     v.norm_b = lp.norm_b;
     v.norm_c = lp.norm_c;
-    v.tau_scale = (float)lp.tau_scale;
-    v.sigma_scale = (float)lp.sigma_scale;
+    v.weight = (float)lp.tau_scale;
+    v.adaptive = (lp.flags & 1) != 0;
+    v.fixed_weight = (lp.flags & 2) != 0;
+    v.move_ref = (lp.flags & 4) == 0;
     return v;
 }
 
@@ -86,48 +92,64 @@ __device__ __forceinline__ void scatter_rows(const LPView& v, A* acc, F row_scal
     }
 }
 
-// Block-wide sum of 4 doubles, result broadcast to all threads.
-template <int NT>
-__device__ __forceinline__ void block_sum4_all(double (&acc)[4], double* red) {
+// Block-wide sum of NV doubles, result broadcast to all threads.
+template <int NT, int NV>
+__device__ __forceinline__ void block_sum_all(double (&acc)[NV], double* red) {
     constexpr int NW = NT / kWave;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) acc[k] = group_sum<kWave>(acc[k]);
+    for (int k = 0; k < NV; ++k) acc[k] = group_sum<kWave>(acc[k]);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0)
 #pragma unroll
-        for (int k = 0; k < 4; ++k) red[w * 4 + k] = acc[k];
+        for (int k = 0; k < NV; ++k) red[w * NV + k] = acc[k];
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < NV; ++k) {
         double t = 0.0;
-        for (int i = 0; i < NW; ++i) t += red[i * 4 + k];
+        for (int i = 0; i < NW; ++i) t += red[i * NV + k];
         acc[k] = t;
     }
     __syncthreads();
 }
 
+constexpr int NS = 6;   // rp^2, rd^2, primal obj, dual obj, |dx|^2, |dy|^2
+
 // Reference KKT of (x, y): src 0 = current iterate, 1 = window average (w).
-// Uses R as n doubles; returns (rp, rd, p_obj, d_obj) sums in acc on all threads.
+// Uses R as n doubles.  acc (all threads): rp^2, rd^2, primal and dual
+// objective, and for the adaptive scheme the squared distances of the point
+// from the last restart point in the step-size metric (dx^2/tau_hat, dy^2/sigma_hat).
 template <int NT, int TPR>
-__device__ void kkt_eval(const LPView& v, float* R, const float* yl, int src, double w, double (&acc)[4],
+__device__ void kkt_eval(const LPView& v, float* R, const float* yl, int src, double w, double (&acc)[NS],
                          double* red) {
     double* R64 = reinterpret_cast<double*>(R);
-    acc[0] = acc[1] = acc[2] = acc[3] = 0.0;
-    // C1: primal point into LDS (FP64), primal objective
+#pragma unroll
+    for (int k = 0; k < NS; ++k) acc[k] = 0.0;
+    // C1: primal point into LDS (FP64), primal objective, primal distance
     for (int j = threadIdx.x; j < v.n; j += NT) {
         const double xj = src ? v.x_sum[j] / w : (double)v.x[j];
         R64[j] = xj;
         acc[2] += v.c64[j] * (xj - v.lo_shift[j]);
+        if (v.adaptive) {
+            const double dx = xj - (double)v.x_last[j];
+            acc[4] += dx * dx / (double)v.tau[j];
+        }
     }
     __syncthreads();
-    // C2: rows — primal residual, dual objective, slack dual residual
+    // C2: rows — primal residual, dual objective, slack dual residual, dual distance
     {
         const int g = threadIdx.x / TPR, lane = threadIdx.x % TPR;
         for (int i = g; i < v.m; i += NT / TPR) {
             const int beg = v.rp[i], end = v.rp[i + 1];
-            double d = 0.0;
-            for (int k = beg + lane; k < end; k += TPR) d += (double)v.val[k] * R64[v.ci[k]];
-            if constexpr (TPR > 1) d = group_sum<TPR>(d);
+            double d = 0.0, as = 0.0;
+            for (int k = beg + lane; k < end; k += TPR) {
+                const double kv = (double)v.val[k];
+                d += kv * R64[v.ci[k]];
+                as += fabs(kv);
+            }
+            if constexpr (TPR > 1) {
+                d = group_sum<TPR>(d);
+                as = group_sum<TPR>(as);
+            }
             if (lane == 0) {
                 const bool le = v.kind[i] == PDLP_ROW_LE;
                 double res = d - v.b64[i];
@@ -136,6 +158,10 @@ __device__ void kkt_eval(const LPView& v, float* R, const float* yl, int src, do
                 acc[0] += res * res;
                 if (le && yi > 0.0) acc[1] += yi * yi;
                 acc[3] += v.b_shift[i] * yi;
+                if (v.adaptive) {
+                    const double dy = yi - (double)v.y_last[i];
+                    acc[5] += dy * dy * (as > 1e-4 ? as : 1e-4) / 0.999;
+                }
             }
         }
     }
@@ -153,7 +179,7 @@ __device__ void kkt_eval(const LPView& v, float* R, const float* yl, int src, do
         if (isfinite(span)) acc[3] += span * rneg;
         else acc[1] += rneg * rneg;
     }
-    block_sum4_all<NT>(acc, red);
+    block_sum_all<NT, NS>(acc, red);
 }
 
 // aty = K^T yl (float), R's upper half.
@@ -166,8 +192,26 @@ __device__ __forceinline__ void rebuild_aty(const LPView& v, float* R, const flo
     __syncthreads();
 }
 
+struct Kkt {
+    double rp, rd, gap, err, po, dobj, dx, dy;
+};
+
+__device__ __forceinline__ Kkt kkt_of(const double (&s)[NS], const LPView& v) {
+    Kkt k;
+    k.rp = sqrt(s[0]) / v.norm_b;
+    k.rd = sqrt(s[1]) / v.norm_c;
+    k.po = s[2];
+    k.dobj = s[3];
+    k.gap = fabs(k.po - k.dobj) / (1.0 + fabs(k.po) + fabs(k.dobj));
+    k.err = fmax(k.rp, fmax(k.rd, k.gap));
+    if (!isfinite(k.po) || !isfinite(k.dobj)) k.err = NAN;
+    k.dx = sqrt(s[4]);
+    k.dy = sqrt(s[5]);
+    return k;
+}
+
 template <int NT, int TPR>
-__device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red, int* sdec) {
+__device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red) {
     const PdlpBatchLP lp = a.lps[k];
     const LPView v = view_of(a, lp);
     float* R = lds;
@@ -176,16 +220,18 @@ __device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red, in
     float* yl = R + 2 * v.n;
     float* ypl = yl + v.m;
 
-    // ---- set-up: y into LDS, sums cleared, tau from column sums of |K|
+    // ---- set-up: y into LDS, sums cleared, restart point = start, tau_hat from column sums of |K|
     for (int i = threadIdx.x; i < v.m; i += NT) {
         yl[i] = v.y[i];
         ypl[i] = 0.f;
         v.y_sum[i] = 0.0;
+        v.y_last[i] = v.y[i];
     }
     for (int j = threadIdx.x; j < v.n; j += NT) {
         aty[j] = 0.f;
         v.x_sum[j] = 0.0;
         v.x_psum[j] = 0.f;
+        v.x_last[j] = v.x[j];
     }
     __syncthreads();
     {
@@ -196,33 +242,29 @@ __device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red, in
     __syncthreads();
     for (int j = threadIdx.x; j < v.n; j += NT) {
         const float s = aty[j];
-        v.tau[j] = v.tau_scale * (0.999f / (s > 1e-4f ? s : 1e-4f));
+        v.tau[j] = 0.999f / (s > 1e-4f ? s : 1e-4f);
     }
     __syncthreads();
 
-    double acc[4];
+    double acc[NS], acc2[NS];
     kkt_eval<NT, TPR>(v, R, yl, 0, 1.0, acc, red);
-    auto err_of = [&](const double (&s)[4], double& rp, double& rd, double& gap) {
-        rp = sqrt(s[0]) / v.norm_b;
-        rd = sqrt(s[1]) / v.norm_c;
-        gap = fabs(s[2] - s[3]) / (1.0 + fabs(s[2]) + fabs(s[3]));
-        return fmax(rp, fmax(rd, gap));
-    };
-    double rp, rd, gap;
-    double err = err_of(acc, rp, rd, gap);
-    double ref_err = err;
-    int status = isfinite(err) ? PDLP_RUNNING : PDLP_NUMERICAL;
-    int window = 0, restarts = 0, checks = 0, it = 0, done_it = 0;
+    Kkt kk = kkt_of(acc, v);
+    double ref_err = kk.err, err_prev = INFINITY;
+    int status = isfinite(kk.err) ? PDLP_RUNNING : PDLP_NUMERICAL;
+    int window = 0, restarts = 0, checks = 0, it = 0, done_it = 0, last_restart = 0;
+    bool end_on_avg = false;
     double w_final = 1.0;
+    float wgt = v.weight;
     rebuild_aty<NT, TPR>(v, R, yl);
 
     const int ce = a.check_every;
     const int g = threadIdx.x / TPR, lane = threadIdx.x % TPR;
     while (status == PDLP_RUNNING && it < a.max_iters) {
+        const float inv_wgt = 1.f / wgt;
         // ---- column pass
         for (int j = threadIdx.x; j < v.n; j += NT) {
             const float xo = v.x[j];
-            float xn = xo - v.tau[j] * (v.c[j] - aty[j]);
+            float xn = xo - (wgt * v.tau[j]) * (v.c[j] - aty[j]);
             xn = xn < v.lo[j] ? v.lo[j] : xn;
             xn = xn > v.hi[j] ? v.hi[j] : xn;
             xb[j] = 2.f * xn - xo;
@@ -244,7 +286,7 @@ __device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red, in
                 d = group_sum<TPR>(d);
                 s = group_sum<TPR>(s);
             }
-            const float sigma = v.sigma_scale * (0.999f / (s > 1e-4f ? s : 1e-4f));
+            const float sigma = inv_wgt * (0.999f / (s > 1e-4f ? s : 1e-4f));
             float yn = yl[i] + sigma * (v.b[i] - d);
             if (v.kind[i] == PDLP_ROW_LE && yn > 0.f) yn = 0.f;
             if (yn != 0.f)
@@ -259,7 +301,7 @@ __device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red, in
         ++window;
         if (window % ce != 0) continue;
 
-        // ---- check: fold partial sums, KKT of the average, decision
+        // ---- check: fold partial sums, KKT of the candidate(s), decision
         for (int j = threadIdx.x; j < v.n; j += NT) {
             v.x_sum[j] += (double)v.x_psum[j];
             v.x_psum[j] = 0.f;
@@ -271,45 +313,69 @@ __device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red, in
         __syncthreads();
         const double w = (double)window;
         kkt_eval<NT, TPR>(v, R, yl, 1, w, acc, red);
-        err = err_of(acc, rp, rd, gap);
+        Kkt ka = kkt_of(acc, v);
+        bool use_avg = true;
+        if (v.adaptive) {                       // the better of average and current iterate
+            kkt_eval<NT, TPR>(v, R, yl, 0, 1.0, acc2, red);
+            const Kkt kc = kkt_of(acc2, v);
+            if (!isfinite(ka.err) || kc.err < ka.err) {
+                use_avg = false;
+                ka = kc;
+#pragma unroll
+                for (int q = 0; q < NS; ++q) acc[q] = acc2[q];
+            }
+        }
+        kk = ka;
         ++checks;
-        int dec;
-        if (!isfinite(err) || !isfinite(acc[2]) || !isfinite(acc[3])) dec = kDecNumerical;
-        else if (err < a.tol) dec = kDecOptimal;
-        else if (err < 0.5 * ref_err) dec = kDecRestart;
-        else dec = kDecContinue;
         // every thread computed the same values from the broadcast sums
-        if (dec == kDecNumerical) {
+        if (!isfinite(kk.err)) {
             status = PDLP_NUMERICAL;
             done_it = it;
             break;
         }
-        if (dec == kDecOptimal) {
+        if (kk.err < a.tol) {                   // pdhg.py:231-235
             status = PDLP_OPTIMAL;
             done_it = it;
+            end_on_avg = use_avg;
             w_final = w;
             break;
         }
-        if (dec == kDecRestart) {
-            ref_err = err;
+        bool restart;
+        if (!v.adaptive) {
+            restart = kk.err < 0.5 * ref_err;   // pdhg.py:238-244
+        } else {
+            restart = kk.err <= 0.2 * ref_err || (kk.err <= 0.8 * ref_err && kk.err > err_prev) ||
+                      (double)(it - last_restart) >= 0.36 * (double)it;
+        }
+        if (restart) {
+            if (v.adaptive && !v.fixed_weight && kk.dx > 1e-10 && kk.dy > 1e-10) {
+                const double lw = 0.5 * log(kk.dx / kk.dy) + 0.5 * log((double)wgt);
+                wgt = (float)fmin(fmax(exp(lw), 1e-8), 1e8);
+            }
+            ref_err = kk.err;
+            err_prev = INFINITY;
+            last_restart = it;
             ++restarts;
             for (int j = threadIdx.x; j < v.n; j += NT) {
-                v.x[j] = (float)(v.x_sum[j] / w);
+                if (use_avg) v.x[j] = (float)(v.x_sum[j] / w);
                 v.x_sum[j] = 0.0;
+                if (v.move_ref) v.x_last[j] = v.x[j];
             }
             for (int i = threadIdx.x; i < v.m; i += NT) {
-                yl[i] = (float)(v.y_sum[i] / w);
+                if (use_avg) yl[i] = (float)(v.y_sum[i] / w);
                 v.y_sum[i] = 0.0;
+                if (v.move_ref) v.y_last[i] = yl[i];
             }
             window = 0;
             __syncthreads();
+        } else {
+            err_prev = kk.err;
         }
         rebuild_aty<NT, TPR>(v, R, yl);
     }
-    (void)sdec;
 
     // ---- outputs: FP64 solution in x_sum / y_sum
-    if (status == PDLP_OPTIMAL) {
+    if (status == PDLP_OPTIMAL && end_on_avg) {
         for (int j = threadIdx.x; j < v.n; j += NT) v.x_sum[j] = v.x_sum[j] / w_final;
         for (int i = threadIdx.x; i < v.m; i += NT) v.y_sum[i] = v.y_sum[i] / w_final;
     } else {
@@ -324,12 +390,13 @@ __device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red, in
         o.iterations = done_it;
         o.restarts = restarts;
         o.checks = checks;
-        o.rel_kkt = err;
-        o.rp = rp;
-        o.rd = rd;
-        o.gap = gap;
-        o.p_obj = acc[2];
-        o.d_obj = acc[3];
+        o.rel_kkt = kk.err;
+        o.rp = kk.rp;
+        o.rd = kk.rd;
+        o.gap = kk.gap;
+        o.p_obj = kk.po;
+        o.d_obj = kk.dobj;
+        o.weight = (double)wgt;
         a.out[k] = o;
     }
     __syncthreads();
@@ -338,9 +405,8 @@ __device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red, in
 template <int NT>
 __global__ void __launch_bounds__(NT) batch_kernel(PdlpBatch a) {
     extern __shared__ float lds[];
-    __shared__ double red[4 * (NT / kWave)];
+    __shared__ double red[NS * (NT / kWave)];
     __shared__ int s_q;
-    __shared__ int s_dec;
     for (;;) {
         if (threadIdx.x == 0) s_q = atomicAdd(a.queue, 1);
         __syncthreads();
@@ -350,13 +416,13 @@ __global__ void __launch_bounds__(NT) batch_kernel(PdlpBatch a) {
         if (q >= a.count) return;                 // every wave of the workgroup leaves here
         const int k = __builtin_amdgcn_readfirstlane(a.order[q]);
         switch (a.lps[k].tpr) {
-            case 1: solve_one<NT, 1>(a, k, lds, red, &s_dec); break;
-            case 2: solve_one<NT, 2>(a, k, lds, red, &s_dec); break;
-            case 4: solve_one<NT, 4>(a, k, lds, red, &s_dec); break;
-            case 8: solve_one<NT, 8>(a, k, lds, red, &s_dec); break;
-            case 16: solve_one<NT, 16>(a, k, lds, red, &s_dec); break;
-            case 32: solve_one<NT, 32>(a, k, lds, red, &s_dec); break;
-            default: solve_one<NT, 64>(a, k, lds, red, &s_dec); break;
+            case 1: solve_one<NT, 1>(a, k, lds, red); break;
+            case 2: solve_one<NT, 2>(a, k, lds, red); break;
+            case 4: solve_one<NT, 4>(a, k, lds, red); break;
+            case 8: solve_one<NT, 8>(a, k, lds, red); break;
+            case 16: solve_one<NT, 16>(a, k, lds, red); break;
+            case 32: solve_one<NT, 32>(a, k, lds, red); break;
+            default: solve_one<NT, 64>(a, k, lds, red); break;
         }
     }
 }
@@ -372,7 +438,7 @@ extern "C" int pdlp_batch_solve(const PdlpBatch* a, void* stream) {
     if (a->count == 0) return 0;
     if (!a->lps || !a->order || !a->queue || !a->row_ptr || !a->col_idx || !a->val || !a->c || !a->lo ||
         !a->hi || !a->c64 || !a->lo_shift || !a->ub_span || !a->b || !a->row_kind || !a->b64 || !a->b_shift ||
-        !a->x || !a->x_psum || !a->x_sum || !a->tau || !a->y || !a->y_sum || !a->out)
+        !a->x || !a->x_psum || !a->x_sum || !a->tau || !a->y || !a->y_sum || !a->out || !a->x_last || !a->y_last)
         return fail(PDLP_E_ARG, "a required PdlpBatch pointer is NULL");
     if (a->check_every <= 0 || a->max_iters < 0) return fail(PDLP_E_ARG, "check_every > 0 and max_iters >= 0");
     if (a->lds_bytes <= 0 || a->lds_bytes > 160 * 1024)

@@ -122,9 +122,19 @@ __global__ void __launch_bounds__(1024) dual_kernel(PdlpDev d, int remap) {
 }
 
 // ------------------------------------------------------------- KKT partials
-// Partial layout: partials[part*BLOCKS*4 + blk*4 + k], k = 0 rp^2, 1 rd^2,
-// 2 primal objective, 3 dual objective.  part 0 = rows, part 1 = columns.
-// src: 0 = current iterate, 1 = window average ((sum + partial sum) / window).
+// A check evaluates one or two candidates: src 0 = current iterate, src 1 =
+// window average ((FP64 sum + FP32 partial sum) / window).  Each workgroup
+// writes NP FP64 sums to partials[((src*2 + part)*BLOCKS + blk)*NP + k], part
+// 0 = rows, 1 = columns; k = 0 rp^2, 1 rd^2, 2 primal objective, 3 dual
+// objective, 4 |dx|^2 and 5 |dy|^2: the distance of the candidate from the
+// last restart point in the step-size metric (sum dx_j^2 / tau_hat_j and
+// sum dy_i^2 / sigma_hat_i), used by the adaptive primal weight.
+
+constexpr int NP = 6;
+
+__device__ __forceinline__ double* part_ptr(const PdlpDev& d, int src, int part, int blk) {
+    return d.partials + ((size_t)(src * 2 + part) * PDLP_PARTIAL_BLOCKS + blk) * NP;
+}
 
 __device__ __forceinline__ double avg_of(const double* s, const float* ps, int k, double w) {
     return ps != nullptr ? (s[k] + (double)ps[k]) / w : s[k] / w;
@@ -132,15 +142,16 @@ __device__ __forceinline__ double avg_of(const double* s, const float* ps, int k
 
 template <typename T, int TPR>
 __global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src) {
-    __shared__ double lds[16 * 4];
+    __shared__ double lds[16 * NP];
     const PdlpCtrl* ctrl = d.ctrl;
     if (ctrl->status != PDLP_RUNNING) return;
     const double w = src ? (double)ctrl->window : 1.0;
     const T* __restrict__ v = (const T*)d.val;
     const T* __restrict__ x = (const T*)d.x;
     const T* __restrict__ y = (const T*)d.y;
+    const T* ylast = (const T*)d.y_last;
     const int gpb = blockDim.x / TPR, lane = threadIdx.x % TPR;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int i = blockIdx.x * gpb + threadIdx.x / TPR; i < d.m; i += gridDim.x * gpb) {
         double ax, asum;
         if (src) row_dot<TPR, false, double>(d.row_ptr, d.col_idx, v, i, lane,
@@ -155,26 +166,31 @@ __global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src) {
             acc[0] += res * res;
             if (le && yi > 0.0) acc[1] += yi * yi;      // slack column reduced cost -y_i
             acc[3] += d.b_shift[i] * yi;
+            if (ylast != nullptr) {
+                const double dy = yi - (double)ylast[i];
+                acc[5] += dy * dy * (asum > 1e-4 ? asum : 1e-4) / 0.999;
+            }
         }
     }
-    block_sum<4>(acc, lds);
+    block_sum<NP>(acc, lds);
     if (threadIdx.x == 0) {
-        double* p = d.partials + (size_t)blockIdx.x * 4;
-        for (int k = 0; k < 4; ++k) p[k] = acc[k];
+        double* p = part_ptr(d, src, 0, blockIdx.x);
+        for (int k = 0; k < NP; ++k) p[k] = acc[k];
     }
 }
 
 template <typename T, int TPR>
 __global__ void __launch_bounds__(256) kkt_cols_kernel(PdlpDev d, int src) {
-    __shared__ double lds[16 * 4];
+    __shared__ double lds[16 * NP];
     const PdlpCtrl* ctrl = d.ctrl;
     if (ctrl->status != PDLP_RUNNING) return;
     const double w = src ? (double)ctrl->window : 1.0;
     const T* __restrict__ tv = (const T*)d.t_val;
     const T* __restrict__ x = (const T*)d.x;
     const T* __restrict__ y = (const T*)d.y;
+    const T* xlast = (const T*)d.x_last;
     const int gpb = blockDim.x / TPR, lane = threadIdx.x % TPR;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int j = blockIdx.x * gpb + threadIdx.x / TPR; j < d.n; j += gridDim.x * gpb) {
         double aty, asum;
         if (src) row_dot<TPR, false, double>(d.t_row_ptr, d.t_col_idx, tv, j, lane,
@@ -190,79 +206,150 @@ __global__ void __launch_bounds__(256) kkt_cols_kernel(PdlpDev d, int src) {
             const double xj = src ? avg_of(d.x_sum, d.x_psum, j, w) : (double)x[j];
             const double sh = d.lo_shift != nullptr ? d.lo_shift[j] : 0.0;
             acc[2] += d.c64[j] * (xj - sh);
+            if (xlast != nullptr) {
+                const double dx = xj - (double)xlast[j];
+                acc[4] += dx * dx * (asum > 1e-4 ? asum : 1e-4) / 0.999;
+            }
         }
     }
-    block_sum<4>(acc, lds);
+    block_sum<NP>(acc, lds);
     if (threadIdx.x == 0) {
-        double* p = d.partials + (size_t)(PDLP_PARTIAL_BLOCKS + blockIdx.x) * 4;
-        for (int k = 0; k < 4; ++k) p[k] = acc[k];
+        double* p = part_ptr(d, src, 1, blockIdx.x);
+        for (int k = 0; k < NP; ++k) p[k] = acc[k];
     }
 }
 
-// mode: 0 = initial point (sets ref_err), 1 = check of the window average.
-__global__ void __launch_bounds__(256) kkt_finalize_kernel(PdlpDev d, int nb_rows, int nb_cols, int mode) {
-    __shared__ double lds[16 * 4];
+struct KktVal {
+    double rp, rd, gap, err, po, dobj, dx, dy;
+};
+
+__device__ KktVal kkt_value(const double (&s)[NP], const PdlpCtrl* ctrl) {
+    KktVal k;
+    k.rp = sqrt(s[0]) / ctrl->norm_b;
+    k.rd = sqrt(s[1]) / ctrl->norm_c;
+    k.po = s[2];
+    k.dobj = s[3];
+    k.gap = fabs(k.po - k.dobj) / (1.0 + fabs(k.po) + fabs(k.dobj));
+    k.err = fmax(k.rp, fmax(k.rd, k.gap));
+    if (!isfinite(k.po) || !isfinite(k.dobj)) k.err = NAN;
+    k.dx = sqrt(s[4]);
+    k.dy = sqrt(s[5]);
+    return k;
+}
+
+// mode 0: initial point (candidate 0 only; sets the reference KKT).
+// mode 1: check; `cands` = bit mask of the evaluated candidates (bit c = src c).
+__global__ void __launch_bounds__(256) kkt_finalize_kernel(PdlpDev d, int nb_rows, int nb_cols, int mode,
+                                                           int scheme, int cands) {
+    __shared__ double lds[16 * NP];
     PdlpCtrl* ctrl = d.ctrl;
     if (ctrl->status != PDLP_RUNNING) {
         if (threadIdx.x == 0) ctrl->restart_pending = 0;
         return;
     }
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < nb_rows; b += blockDim.x)
-        for (int k = 0; k < 4; ++k) acc[k] += d.partials[(size_t)b * 4 + k];
-    for (int b = threadIdx.x; b < nb_cols; b += blockDim.x)
-        for (int k = 0; k < 4; ++k) acc[k] += d.partials[(size_t)(PDLP_PARTIAL_BLOCKS + b) * 4 + k];
-    block_sum<4>(acc, lds);
+    double sums[2][NP];
+    for (int c = 0; c < 2; ++c) {
+        double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (cands & (1 << c)) {
+            for (int b = threadIdx.x; b < nb_rows; b += blockDim.x) {
+                const double* p = part_ptr(d, c, 0, b);
+                for (int k = 0; k < NP; ++k) acc[k] += p[k];
+            }
+            for (int b = threadIdx.x; b < nb_cols; b += blockDim.x) {
+                const double* p = part_ptr(d, c, 1, b);
+                for (int k = 0; k < NP; ++k) acc[k] += p[k];
+            }
+        }
+        block_sum<NP>(acc, lds);
+        for (int k = 0; k < NP; ++k) sums[c][k] = acc[k];
+    }
     if (threadIdx.x != 0) return;
-    const double rp = sqrt(acc[0]) / ctrl->norm_b;
-    const double rd = sqrt(acc[1]) / ctrl->norm_c;
-    const double po = acc[2], dobj = acc[3];
-    const double gap = fabs(po - dobj) / (1.0 + fabs(po) + fabs(dobj));
-    const double err = fmax(rp, fmax(rd, gap));
-    ctrl->rp = rp;
-    ctrl->rd = rd;
-    ctrl->gap = gap;
-    ctrl->p_obj = po;
-    ctrl->d_obj = dobj;
-    ctrl->last_err = err;
+    const KktVal k0 = kkt_value(sums[0], ctrl), k1 = kkt_value(sums[1], ctrl);
+    int c;
+    if (cands == 1) c = 0;
+    else if (cands == 2) c = 1;
+    else c = (isfinite(k1.err) && !(k0.err < k1.err)) ? 1 : 0;   // the better candidate; NaN loses
+    const KktVal& k = c ? k1 : k0;
+    ctrl->rp = k.rp;
+    ctrl->rd = k.rd;
+    ctrl->gap = k.gap;
+    ctrl->p_obj = k.po;
+    ctrl->d_obj = k.dobj;
+    ctrl->last_err = k.err;
     ctrl->restart_pending = 0;
-    if (!isfinite(err) || !isfinite(po) || !isfinite(dobj)) {
+    if (!isfinite(k.err)) {
         ctrl->status = PDLP_NUMERICAL;
         ctrl->done_iter = ctrl->iter;
         return;
     }
     if (mode == 0) {
-        ctrl->ref_err = err;
+        ctrl->ref_err = k.err;
+        ctrl->err_prev = INFINITY;
+        ctrl->last_restart = ctrl->iter;
         return;
     }
     ctrl->checks += 1;
-    if (err < ctrl->tol) {                       // pdhg.py:231-235
+    if (k.err < ctrl->tol) {                     // pdhg.py:231-235
         ctrl->status = PDLP_OPTIMAL;
         ctrl->done_iter = ctrl->iter;
         ctrl->restart_pending = 1;
+        ctrl->cand = c;
         ctrl->apply_window = ctrl->window;
-    } else if (err < 0.5 * ctrl->ref_err) {      // pdhg.py:238-244
-        ctrl->restart_pending = 1;
-        ctrl->apply_window = ctrl->window;
-        ctrl->window = 0;
-        ctrl->ref_err = err;
-        ctrl->restarts += 1;
+        return;
     }
+    bool restart;
+    if (scheme == PDLP_SCHEME_REFERENCE) {
+        restart = k.err < 0.5 * ctrl->ref_err;   // pdhg.py:238-244
+    } else {
+        const int len = ctrl->iter - ctrl->last_restart;
+        restart = k.err <= 0.2 * ctrl->ref_err ||
+                  (k.err <= 0.8 * ctrl->ref_err && k.err > ctrl->err_prev) ||
+                  (double)len >= 0.36 * (double)ctrl->iter;
+    }
+    if (!restart) {
+        ctrl->err_prev = k.err;
+        return;
+    }
+    if ((scheme == PDLP_SCHEME_ADAPTIVE || scheme == PDLP_SCHEME_ADAPTIVE_START) && k.dx > 1e-10 &&
+        k.dy > 1e-10) {
+        // primal weight w (tau = w tau_hat, sigma = sigma_hat / w): geometric mean of the
+        // old weight and the distance ratio (from the start point, or since the last restart)
+        const double lw = 0.5 * log(k.dx / k.dy) + 0.5 * log(ctrl->tau_scale);
+        const double wt = fmin(fmax(exp(lw), 1e-8), 1e8);
+        ctrl->tau_scale = wt;
+        ctrl->sigma_scale = 1.0 / wt;
+    }
+    // 2: the restart point becomes the reference of the next distances
+    ctrl->restart_pending = scheme == PDLP_SCHEME_ADAPTIVE_START ? 1 : 2;
+    ctrl->cand = c;
+    ctrl->apply_window = ctrl->window;
+    ctrl->window = 0;
+    ctrl->ref_err = k.err;
+    ctrl->err_prev = INFINITY;
+    ctrl->last_restart = ctrl->iter;
+    ctrl->restarts += 1;
 }
 
-// After each check: restart to (or, on convergence, finish at) the window
-// average, and fold the FP32 partial sums of the last check_every iterations
-// into the FP64 window sums.  At convergence the FP64 sums are kept (folded)
-// so the host can read the exact average.
+// After each check: restart to (or, on convergence, finish at) the chosen
+// candidate — the window average or the current iterate — and fold the FP32
+// partial sums of the last check_every iterations into the FP64 window sums.
+// A restart also records the new reference point (x_last, y_last) of the
+// weight update, except with PDLP_SCHEME_ADAPTIVE_START (reference = start).  At
+// convergence the FP64 sums are kept (folded) so the host can read the exact
+// average.
 template <typename T>
 __global__ void __launch_bounds__(256) apply_kernel(PdlpDev d) {
     const PdlpCtrl* ctrl = d.ctrl;
     const bool pending = ctrl->restart_pending != 0;
     if (!pending && d.x_psum == nullptr) return;
     const bool keep_sums = ctrl->status == PDLP_OPTIMAL;
+    const bool to_avg = ctrl->cand == 1;
     const double w = (double)ctrl->apply_window;
     T* x = (T*)d.x;
     T* y = (T*)d.y;
+    T* xl = (T*)d.x_last;
+    T* yl = (T*)d.y_last;
+    const bool move_ref = ctrl->restart_pending == 2 && xl != nullptr && yl != nullptr;
     const int stride = gridDim.x * blockDim.x;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < d.n; j += stride) {
         double s = d.x_sum[j];
@@ -271,8 +358,9 @@ __global__ void __launch_bounds__(256) apply_kernel(PdlpDev d) {
             d.x_psum[j] = 0.f;
         }
         if (pending) {
-            x[j] = (T)(s / w);
+            if (to_avg) x[j] = (T)(s / w);
             if (!keep_sums) s = 0.0;
+            if (move_ref) xl[j] = x[j];
         }
         d.x_sum[j] = s;
     }
@@ -283,22 +371,26 @@ __global__ void __launch_bounds__(256) apply_kernel(PdlpDev d) {
             d.y_psum[i] = 0.f;
         }
         if (pending) {
-            y[i] = (T)(s / w);
+            if (to_avg) y[i] = (T)(s / w);
             if (!keep_sums) s = 0.0;
+            if (move_ref) yl[i] = y[i];
         }
         d.y_sum[i] = s;
     }
 }
 
+template <typename T>
 __global__ void init_kernel(PdlpDev d) {
     const int stride = gridDim.x * blockDim.x;
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < d.n; j += stride) {
         d.x_sum[j] = 0.0;
         if (d.x_psum != nullptr) d.x_psum[j] = 0.f;
+        if (d.x_last != nullptr) ((T*)d.x_last)[j] = ((const T*)d.x)[j];
     }
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.m; i += stride) {
         d.y_sum[i] = 0.0;
         if (d.y_psum != nullptr) d.y_psum[i] = 0.f;
+        if (d.y_last != nullptr) ((T*)d.y_last)[i] = ((const T*)d.y)[i];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         PdlpCtrl* c = d.ctrl;
@@ -310,6 +402,9 @@ __global__ void init_kernel(PdlpDev d) {
         c->restart_pending = 0;
         c->apply_window = 0;
         c->checks = 0;
+        c->cand = 1;
+        c->last_restart = 0;
+        c->err_prev = INFINITY;
     }
 }
 
@@ -364,6 +459,18 @@ int check_tune(const PdlpTune* t) {
     if (t->check_every <= 0) return fail(PDLP_E_TUNE, "check_every must be positive");
     if (t->exact && (t->tpr_primal != 1 || t->tpr_dual != 1))
         return fail(PDLP_E_TUNE, "exact mode needs tpr_primal = tpr_dual = 1");
+    if (t->scheme < PDLP_SCHEME_REFERENCE || t->scheme > PDLP_SCHEME_ADAPTIVE_START)
+        return fail(PDLP_E_TUNE, "scheme must be one of the PDLP_SCHEME_* values");
+    if (t->exact && t->scheme != PDLP_SCHEME_REFERENCE)
+        return fail(PDLP_E_TUNE, "exact mode reproduces the reference scheme only");
+    return 0;
+}
+
+int check_all(const PdlpDev* d, const PdlpTune* t) {
+    int rc = check_dev(d);
+    if (rc || (rc = check_tune(t))) return rc;
+    if (t->scheme != PDLP_SCHEME_REFERENCE && (d->x_last == nullptr || d->y_last == nullptr))
+        return fail(PDLP_E_ARG, "the adaptive scheme needs x_last and y_last");
     return 0;
 }
 
@@ -418,7 +525,7 @@ int launch_dual_t(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
 }
 
 template <typename T>
-int launch_kkt_t(const PdlpDev& d, const PdlpTune& t, int src, int mode, hipStream_t s) {
+int launch_kkt_t(const PdlpDev& d, const PdlpTune& t, int mode, hipStream_t s) {
     constexpr int B = 256;
     auto nb = [](int rows, int tpr) {
         long g = (rows + (B / tpr) - 1) / (B / tpr);
@@ -427,17 +534,23 @@ int launch_kkt_t(const PdlpDev& d, const PdlpTune& t, int src, int mode, hipStre
     // the check's reductions use at most a wavefront per row
     const int td = std::min(t.tpr_dual, 64), tp = std::min(t.tpr_primal, 64);
     const int nr = nb(d.m, td), nc = nb(d.n, tp);
-    int rc = with_tpr(td, [&](auto k) {
-        if constexpr (decltype(k)::value <= 64) kkt_rows_kernel<T, decltype(k)::value><<<nr, B, 0, s>>>(d, src);
-        return 0;
-    });
-    if (rc) return rc;
-    rc = with_tpr(tp, [&](auto k) {
-        if constexpr (decltype(k)::value <= 64) kkt_cols_kernel<T, decltype(k)::value><<<nc, B, 0, s>>>(d, src);
-        return 0;
-    });
-    if (rc) return rc;
-    kkt_finalize_kernel<<<1, B, 0, s>>>(d, nr, nc, mode);
+    // candidates: the initial point is "current"; a reference check uses the
+    // average only; an adaptive check evaluates both
+    const int cands = mode == 0 ? 1 : (t.scheme != PDLP_SCHEME_REFERENCE ? 3 : 2);
+    for (int src = 0; src < 2; ++src) {
+        if (!(cands & (1 << src))) continue;
+        int rc = with_tpr(td, [&](auto k) {
+            if constexpr (decltype(k)::value <= 64) kkt_rows_kernel<T, decltype(k)::value><<<nr, B, 0, s>>>(d, src);
+            return 0;
+        });
+        if (rc) return rc;
+        rc = with_tpr(tp, [&](auto k) {
+            if constexpr (decltype(k)::value <= 64) kkt_cols_kernel<T, decltype(k)::value><<<nc, B, 0, s>>>(d, src);
+            return 0;
+        });
+        if (rc) return rc;
+    }
+    kkt_finalize_kernel<<<1, B, 0, s>>>(d, nr, nc, mode, t.scheme, cands);
     if (mode == 1) {
         const int work = std::max(d.n, d.m);
         const int g = std::min((work + B - 1) / B, 4096);
@@ -458,9 +571,8 @@ int dual(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
     return t.exact ? launch_dual_t<float, true>(d, t, s) : launch_dual_t<float, false>(d, t, s);
 }
 
-int kkt(const PdlpDev& d, const PdlpTune& t, int src, int mode, hipStream_t s) {
-    return d.prec == PDLP_F64 ? launch_kkt_t<double>(d, t, src, mode, s)
-                              : launch_kkt_t<float>(d, t, src, mode, s);
+int kkt(const PdlpDev& d, const PdlpTune& t, int mode, hipStream_t s) {
+    return d.prec == PDLP_F64 ? launch_kkt_t<double>(d, t, mode, s) : launch_kkt_t<float>(d, t, mode, s);
 }
 
 int run(const PdlpDev& d, const PdlpTune& t, int iters, int check, hipStream_t s) {
@@ -470,7 +582,7 @@ int run(const PdlpDev& d, const PdlpTune& t, int iters, int check, hipStream_t s
         rc = dual(d, t, s);
         if (rc) return rc;
     }
-    return check ? kkt(d, t, 1, 1, s) : 0;
+    return check ? kkt(d, t, 1, s) : 0;
 }
 
 }  // namespace
@@ -482,43 +594,44 @@ using namespace pdlp;
 extern "C" {
 
 int pdlp_init(const PdlpDev* dev, const PdlpTune* tune, void* stream) {
-    int rc = check_dev(dev);
-    if (rc || (rc = check_tune(tune))) return rc;
+    int rc = check_all(dev, tune);
+    if (rc) return rc;
     hipStream_t s = as_stream(stream);
     const int work = std::max(dev->n, dev->m);
-    init_kernel<<<std::min((work + 255) / 256, 4096), 256, 0, s>>>(*dev);
+    if (dev->prec == PDLP_F64) init_kernel<double><<<std::min((work + 255) / 256, 4096), 256, 0, s>>>(*dev);
+    else init_kernel<float><<<std::min((work + 255) / 256, 4096), 256, 0, s>>>(*dev);
     if ((rc = hip_fail(hipGetLastError(), "init_kernel launch"))) return rc;
-    return kkt(*dev, *tune, 0, 0, s);
+    return kkt(*dev, *tune, 0, s);
 }
 
 int pdlp_primal_step(const PdlpDev* dev, const PdlpTune* tune, void* stream) {
-    int rc = check_dev(dev);
-    if (rc || (rc = check_tune(tune))) return rc;
+    int rc = check_all(dev, tune);
+    if (rc) return rc;
     return primal(*dev, *tune, as_stream(stream));
 }
 
 int pdlp_dual_step(const PdlpDev* dev, const PdlpTune* tune, void* stream) {
-    int rc = check_dev(dev);
-    if (rc || (rc = check_tune(tune))) return rc;
+    int rc = check_all(dev, tune);
+    if (rc) return rc;
     return dual(*dev, *tune, as_stream(stream));
 }
 
 int pdlp_kkt_check(const PdlpDev* dev, const PdlpTune* tune, void* stream) {
-    int rc = check_dev(dev);
-    if (rc || (rc = check_tune(tune))) return rc;
-    return kkt(*dev, *tune, 1, 1, as_stream(stream));
+    int rc = check_all(dev, tune);
+    if (rc) return rc;
+    return kkt(*dev, *tune, 1, as_stream(stream));
 }
 
 int pdlp_run(const PdlpDev* dev, const PdlpTune* tune, int32_t iters, int32_t check, void* stream) {
-    int rc = check_dev(dev);
-    if (rc || (rc = check_tune(tune))) return rc;
+    int rc = check_all(dev, tune);
+    if (rc) return rc;
     if (iters < 0) return fail(PDLP_E_ARG, "iters must be >= 0");
     return run(*dev, *tune, iters, check, as_stream(stream));
 }
 
 int pdlp_graph_capture(const PdlpDev* dev, const PdlpTune* tune, int32_t blocks, void* stream, void** exec) {
-    int rc = check_dev(dev);
-    if (rc || (rc = check_tune(tune))) return rc;
+    int rc = check_all(dev, tune);
+    if (rc) return rc;
     if (blocks <= 0 || exec == nullptr) return fail(PDLP_E_ARG, "blocks must be positive and exec non-NULL");
     if (stream == nullptr) return fail(PDLP_E_ARG, "graph capture needs a non-default stream");
     hipStream_t s = as_stream(stream);
@@ -542,8 +655,8 @@ int pdlp_graph_capture(const PdlpDev* dev, const PdlpTune* tune, int32_t blocks,
 
 int pdlp_graph_capture_step(const PdlpDev* dev, const PdlpTune* tune, int32_t which, int32_t reps, void* stream,
                             void** exec) {
-    int rc = check_dev(dev);
-    if (rc || (rc = check_tune(tune))) return rc;
+    int rc = check_all(dev, tune);
+    if (rc) return rc;
     if (reps <= 0 || exec == nullptr || (which != 0 && which != 1))
         return fail(PDLP_E_ARG, "reps > 0, which in {0, 1} and exec non-NULL");
     if (stream == nullptr) return fail(PDLP_E_ARG, "graph capture needs a non-default stream");

@@ -18,7 +18,8 @@ PDLP_F32, PDLP_F64 = 0, 1
 PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL = 0, 1, 3
 PDLP_ROW_EQ, PDLP_ROW_LE = 0, 1
 PDLP_PARTIAL_BLOCKS = 1024
-PDLP_PARTIALS_LEN = 2 * PDLP_PARTIAL_BLOCKS * 4
+PDLP_PARTIALS_LEN = 4 * PDLP_PARTIAL_BLOCKS * 6
+PDLP_SCHEME_REFERENCE, PDLP_SCHEME_ADAPTIVE = 0, 1
 
 _p = C.c_void_p
 _i32p = C.c_void_p
@@ -32,6 +33,7 @@ class PdlpCtrl(C.Structure):
         ("ref_err", C.c_double), ("last_err", C.c_double), ("rp", C.c_double), ("rd", C.c_double),
         ("gap", C.c_double), ("tol", C.c_double), ("norm_b", C.c_double), ("norm_c", C.c_double),
         ("tau_scale", C.c_double), ("sigma_scale", C.c_double), ("p_obj", C.c_double), ("d_obj", C.c_double),
+        ("err_prev", C.c_double), ("cand", C.c_int32), ("last_restart", C.c_int32),
     ]
 
 
@@ -43,7 +45,7 @@ class PdlpDev(C.Structure):
         ("c", _p), ("lo", _p), ("hi", _p), ("b", _p), ("row_kind", _p),
         ("c64", _p), ("b64", _p), ("b_shift", _p), ("lo_shift", _p), ("ub_span", _p),
         ("x", _p), ("xbar", _p), ("y", _p), ("x_sum", _p), ("y_sum", _p),
-        ("x_psum", _p), ("y_psum", _p), ("partials", _p), ("ctrl", _p),
+        ("x_psum", _p), ("y_psum", _p), ("partials", _p), ("ctrl", _p), ("x_last", _p), ("y_last", _p),
     ]
 
 
@@ -51,6 +53,7 @@ class PdlpTune(C.Structure):
     _fields_ = [
         ("tpr_primal", C.c_int32), ("tpr_dual", C.c_int32), ("block", C.c_int32),
         ("check_every", C.c_int32), ("xcd_remap", C.c_int32), ("exact", C.c_int32), ("block_dual", C.c_int32),
+        ("scheme", C.c_int32),
     ]
 
 
@@ -65,6 +68,7 @@ class PdlpBatch(C.Structure):
         ("b", C.c_void_p), ("row_kind", C.c_void_p), ("b64", C.c_void_p), ("b_shift", C.c_void_p),
         ("x", C.c_void_p), ("x_psum", C.c_void_p), ("x_sum", C.c_void_p), ("tau", C.c_void_p),
         ("y", C.c_void_p), ("y_sum", C.c_void_p), ("out", C.c_void_p),
+        ("x_last", C.c_void_p), ("y_last", C.c_void_p),
     ]
 
 

@@ -31,7 +31,7 @@ LP_DTYPE = np.dtype([
 OUT_DTYPE = np.dtype([
     ("status", np.int32), ("iterations", np.int32), ("restarts", np.int32), ("checks", np.int32),
     ("rel_kkt", np.float64), ("rp", np.float64), ("rd", np.float64), ("gap", np.float64),
-    ("p_obj", np.float64), ("d_obj", np.float64),
+    ("p_obj", np.float64), ("d_obj", np.float64), ("weight", np.float64),
 ], align=True)
 
 
@@ -102,6 +102,13 @@ class PackedBatch:
         kind = np.empty(M, np.int8)
         b_shift = np.empty(M)
         desc = np.zeros(B, LP_DTYPE)
+        # starting primal weight of the adaptive scheme: ||sqrt(sigma_hat) b|| / ||sqrt(tau_hat) c||
+        sums = {}
+        for i, K in enumerate(stored):
+            aK = abs(K)
+            sums[i] = (np.maximum(np.asarray(aK.sum(axis=0)).ravel(), 1e-4),
+                       np.maximum(np.asarray(aK.sum(axis=1)).ravel(), 1e-4))
+        self.init_weight = np.ones(B)
         for k, (p, K) in enumerate(zip(self.problems, Ks)):
             cs, rs = slice(col_off[k], col_off[k] + ns[k]), slice(row_off[k], row_off[k] + ms[k])
             c[cs] = p.c
@@ -118,6 +125,11 @@ class PackedBatch:
             desc[k] = (ms[k], ns[k], lanes_per_row(nnzs[k] / ms[k]), 0, row_off[k], col_off[k], rp_off[k],
                        nnz_off[k], max(1.0, float(np.sqrt(bs @ bs + span @ span))),
                        max(1.0, float(np.linalg.norm(p.c))), 1.0, 1.0)
+            colsum, rowsum = sums[mat_of[k]]
+            nb = float(np.sqrt(np.sum(bk * bk * 0.999 / rowsum)))
+            nc = float(np.sqrt(np.sum(np.asarray(p.c, float) ** 2 * 0.999 / colsum)))
+            if nb > 1e-12 and nc > 1e-12:
+                self.init_weight[k] = float(np.clip(nb / nc, 1e-6, 1e6))
         lo_shift = np.where(np.isfinite(lo), lo, 0.0)
         ub_span = np.where(np.isfinite(hi), hi - lo_shift, np.inf)
         self.c, self.lo, self.hi, self.b, self.kind = c, lo, hi, b, kind
@@ -133,10 +145,17 @@ class PackedBatch:
 
 
 class BatchEngine:
-    def __init__(self, packed: PackedBatch, block: int = 0, grid: int = 0):
+    """Device arena of a :class:`PackedBatch`.  ``scheme``: "adaptive" (default)
+    or "reference" restarts for every LP (see ``include/pdlp_hip.h``)."""
+
+    def __init__(self, packed: PackedBatch, block: int = 0, grid: int = 0, scheme: str = "adaptive"):
+        if scheme not in ("adaptive", "reference", "adaptive_fixed", "adaptive_start"):
+            raise ValueError("scheme must be 'adaptive', 'adaptive_fixed', 'adaptive_start' or 'reference'")
         self.lib = L.lib()
         self.device = require_gpu()
         self.pk = packed
+        self.adaptive = scheme != "reference"
+        self.flags = {"reference": 0, "adaptive": 1, "adaptive_fixed": 3, "adaptive_start": 5}[scheme]
         # block 0: 256 threads while two workgroups fit a CU's 160 KB of LDS, else 512
         self.block = block or (256 if packed.lds_bytes <= 80 * 1024 else 512)
         self.grid = grid
@@ -161,12 +180,15 @@ class BatchEngine:
                 "x_sum": torch.zeros(pk.N, dtype=f64, device=dv), "tau": torch.zeros(pk.N, dtype=f32, device=dv),
                 "y": torch.zeros(pk.M, dtype=f32, device=dv), "y_sum": torch.zeros(pk.M, dtype=f64, device=dv),
                 "out": torch.zeros(len(pk.problems) * OUT_DTYPE.itemsize, dtype=torch.uint8, device=dv),
+                "x_last": torch.zeros(pk.N, dtype=f32, device=dv), "y_last": torch.zeros(pk.M, dtype=f32, device=dv),
             }
         self.stream.synchronize()
 
     def set_warm_start(self, x0: Optional[np.ndarray] = None, y0: Optional[np.ndarray] = None,
                        step_scale: Optional[np.ndarray] = None) -> None:
-        """Packed starting point (length N / M arrays; None = zeros) and per-LP primal weights."""
+        """Packed starting point (length N / M arrays; None = cold start) and per-LP
+        starting primal weights (None = 1 for the reference scheme, the norm-ratio
+        estimate for the adaptive one)."""
         with torch.cuda.stream(self.stream):
             if x0 is None:          # cold start: the origin projected onto the box (x = lb when lb >= 0)
                 self.t["x"].copy_(torch.from_numpy(np.clip(0.0, self.pk.lo, self.pk.hi).astype(np.float32))
@@ -178,10 +200,13 @@ class BatchEngine:
             else:
                 self.t["y"].copy_(torch.from_numpy(np.asarray(y0, np.float32)).to(self.device))
             desc = self.pk.desc.copy()
-            if step_scale is not None:
-                w = np.asarray(step_scale, float)
-                desc["tau_scale"] = w
-                desc["sigma_scale"] = 1.0 / w
+            desc["flags"] = self.flags
+            w = self.pk.init_weight if self.adaptive else np.ones(len(desc))
+            if step_scale is not None:            # NaN entries keep the default
+                ss = np.asarray(step_scale, float)
+                w = np.where(np.isfinite(ss), ss, w)
+            desc["tau_scale"] = w
+            desc["sigma_scale"] = 1.0 / w
             self.t["lps"].copy_(torch.from_numpy(desc.view(np.uint8).copy()).to(self.device))
 
     def _args(self, tol: float, max_iters: int, check_every: int) -> PdlpBatch:
@@ -190,7 +215,7 @@ class BatchEngine:
                       lds_bytes=self.pk.lds_bytes, grid=self.grid, tol=tol)
         for name in ("lps", "order", "queue", "row_ptr", "col_idx", "val", "c", "lo", "hi", "c64", "lo_shift",
                      "ub_span", "b", "row_kind", "b64", "b_shift", "x", "x_psum", "x_sum", "tau", "y", "y_sum",
-                     "out"):
+                     "out", "x_last", "y_last"):
             setattr(a, name, ptr(t[name]))
         return a
 

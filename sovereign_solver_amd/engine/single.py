@@ -33,7 +33,8 @@ class SingleEngine:
     """Device copy of one :class:`HostForm` plus its PDHG state."""
 
     def __init__(self, form: HostForm, precision: str = "fp32", tune: Optional[Tune] = None,
-                 check_every: int = 50, exact: bool = False, use_graphs: bool = True):
+                 check_every: int = 50, exact: bool = False, use_graphs: bool = True,
+                 scheme: Optional[int] = None):
         if precision not in ("fp32", "fp64"):
             raise ValueError("precision must be 'fp32' or 'fp64'")
         self.lib = L.lib()
@@ -42,6 +43,8 @@ class SingleEngine:
         self.precision = precision
         self.m, self.n, self.nnz = form.m, form.n, form.nnz
         self.tune = tune or heuristic_tune(self.m, self.n, self.nnz, check_every, exact)
+        if scheme is not None:
+            self.tune = self.tune.with_(scheme=int(scheme))
         if exact and (self.tune.tpr_primal != 1 or self.tune.tpr_dual != 1 or not self.tune.exact):
             raise ValueError("exact mode needs Tune(tpr_primal=1, tpr_dual=1, exact=1)")
         self.use_graphs = use_graphs
@@ -80,6 +83,9 @@ class SingleEngine:
             self.y_psum = torch.zeros(self.m, dtype=torch.float32, device=dv) if fp32 else None
             self.partials = torch.zeros(L.PDLP_PARTIALS_LEN, dtype=f64, device=dv)
             self.ctrl = torch.zeros(C.sizeof(L.PdlpCtrl), dtype=torch.uint8, device=dv)
+            # last restart point (adaptive scheme)
+            self.x_last = torch.zeros(self.n, dtype=T, device=dv)
+            self.y_last = torch.zeros(self.m, dtype=T, device=dv)
         self.ctrl_host = torch.zeros((2, C.sizeof(L.PdlpCtrl)), dtype=torch.uint8).pin_memory()
         self.stream.synchronize()
         self.dev = L.PdlpDev(
@@ -90,7 +96,7 @@ class SingleEngine:
             c64=ptr(self.c64), b64=ptr(self.b64), b_shift=ptr(self.b_shift), lo_shift=ptr(self.lo_shift),
             ub_span=ptr(self.ub_span), x=ptr(self.x), xbar=ptr(self.xbar), y=ptr(self.y),
             x_sum=ptr(self.x_sum), y_sum=ptr(self.y_sum), x_psum=ptr(self.x_psum), y_psum=ptr(self.y_psum),
-            partials=ptr(self.partials), ctrl=ptr(self.ctrl))
+            partials=ptr(self.partials), ctrl=ptr(self.ctrl), x_last=ptr(self.x_last), y_last=ptr(self.y_last))
         self._graphs = {}
         self._graph_tune = None
 
@@ -206,10 +212,28 @@ class SingleEngine:
         self.stream.synchronize()
         return self.read_ctrl()
 
+    def initial_weight(self) -> float:
+        """Starting primal weight of the adaptive scheme: ‖√σ̂∘b‖ / ‖√τ̂∘c‖.
+
+        The ratio of the right-hand side and objective norms in the metric of the
+        diagonal step sizes (τ̂_j = 0.999/colsum_j, σ̂_i = 0.999/rowsum_i); 1 when
+        either norm vanishes."""
+        f = self.form
+        absK = abs(f.K)
+        colsum = np.maximum(np.asarray(absK.sum(axis=0)).ravel(), 1e-4)
+        rowsum = np.maximum(np.asarray(absK.sum(axis=1)).ravel(), 1e-4)
+        nb = float(np.sqrt(np.sum(f.b * f.b * 0.999 / rowsum)))
+        nc = float(np.sqrt(np.sum(f.c * f.c * 0.999 / colsum)))
+        if nb <= 1e-12 or nc <= 1e-12:
+            return 1.0
+        return float(np.clip(nb / nc, 1e-6, 1e6))
+
     def solution(self, st: L.PdlpCtrl):
-        """Host copies of (x, y): the FP64 window average at convergence, else the last iterate."""
+        """Host copies of (x, y): the FP64 window average when the solve ended on
+        the average, else the iterate (the current point, or the last iterate at
+        the iteration limit)."""
         with torch.cuda.stream(self.stream):
-            if st.status == L.PDLP_OPTIMAL:
+            if st.status == L.PDLP_OPTIMAL and st.cand == 1:
                 w = float(st.apply_window)
                 x = (self.x_sum / w).cpu()
                 y = (self.y_sum / w).cpu()
@@ -219,8 +243,13 @@ class SingleEngine:
         self.stream.synchronize()
         return x.numpy(), y.numpy()
 
-    def solve(self, tol: float, max_iters: int, x0=None, y0=None, tau_scale=1.0, sigma_scale=1.0):
-        self.start(tol, x0, y0, tau_scale, sigma_scale)
+    def solve(self, tol: float, max_iters: int, x0=None, y0=None, step_scale: Optional[float] = None):
+        """One solve.  ``step_scale`` is the starting primal weight w (tau = w tau_hat,
+        sigma = sigma_hat / w); None = 1 for the reference scheme, the norm ratio of
+        :meth:`initial_weight` for the adaptive one."""
+        if step_scale is None:
+            step_scale = self.initial_weight() if self.tune.scheme != L.PDLP_SCHEME_REFERENCE else 1.0
+        self.start(tol, x0, y0, float(step_scale), 1.0 / float(step_scale))
         st = self.run_iterations(max_iters)
         x, y = self.solution(st)
         iters = st.done_iter if st.status != L.PDLP_RUNNING else st.iter

@@ -33,11 +33,17 @@ class Tune:
     xcd_remap: int = 0
     exact: int = 0
     block_dual: int = 0
+    scheme: int = 0             # 0 reference restart rule, 1 adaptive (see include/pdlp_hip.h)
 
     def to_c(self):
         from ._lib import PdlpTune
         return PdlpTune(self.tpr_primal, self.tpr_dual, self.block, self.check_every, self.xcd_remap, self.exact,
-                        self.block_dual)
+                        self.block_dual, self.scheme)
+
+    def with_(self, **kw) -> "Tune":
+        d = asdict(self)
+        d.update(kw)
+        return Tune(**d)
 
 
 def _tpr_for(mean_len: float) -> int:
@@ -116,7 +122,7 @@ def autotune(engine, reps: int = 20, use_cache: bool = True) -> Tune:
     if use_cache:
         hit = cached_tune(engine.m, engine.n, engine.nnz, engine.precision, engine.tune.check_every)
         if hit is not None:
-            return hit
+            return hit.with_(scheme=engine.tune.scheme)
     base = heuristic_tune(engine.m, engine.n, engine.nnz, engine.tune.check_every)
     timings = {}
     best_p, best_pt = None, float("inf")
@@ -140,4 +146,4 @@ def autotune(engine, reps: int = 20, use_cache: bool = True) -> Tune:
                     best_d, best_dt = (td, remap, bd), us
     t = Tune(best_p[0], best_d[0], 256, base.check_every, best_p[1] | (best_d[1] << 1), 0, best_d[2])
     _store_cache(key, t, timings)
-    return t
+    return t.with_(scheme=engine.tune.scheme)

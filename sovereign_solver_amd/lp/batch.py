@@ -5,7 +5,8 @@ The reference has no batch call; its callers loop over ``solve_lp_pdhg``
 (``solver/milp/branch_and_bound.py:367-374``).  This call solves all LPs in one
 persistent HIP launch and returns one :class:`SolveResult` per problem with the
 same fields and semantics as :func:`solve_lp_pdhg` (box form, FP32 iterates,
-FP64 KKT, the reference's restart rule and check cadence).
+FP64 KKT, check cadence 50; ``scheme="adaptive"`` (default) or
+``"reference"`` restarts as in :func:`solve_lp_pdhg`).
 
 ``warm_start`` is an optional list (one entry per problem, ``None`` allowed)
 of ``(x0, y0)`` or ``(x0, y0, step_scale)``: x0 in the original variable
@@ -30,7 +31,7 @@ def pack_warm_start(pk: PackedBatch, warm_start):
         return None, None, None
     x0 = np.zeros(pk.N)
     y0 = np.zeros(pk.M)
-    w = np.ones(len(pk.problems))
+    w = np.full(len(pk.problems), np.nan)     # NaN: the engine's default starting weight
     for k, ws in enumerate(warm_start):
         if ws is None:
             continue
@@ -57,7 +58,8 @@ def results_of(engine: BatchEngine, out, xs, ys) -> list:
             obj = -obj
         res.append(SolveResult(status, x.copy(), obj, int(rec["iterations"]), y=y.copy(),
                                rel_kkt=float(rec["rel_kkt"]), rp=float(rec["rp"]), rd=float(rec["rd"]),
-                               gap=float(rec["gap"]), restarts=int(rec["restarts"])))
+                               gap=float(rec["gap"]), restarts=int(rec["restarts"]),
+                               step_scale=float(rec["weight"])))
     return res
 
 
@@ -69,6 +71,7 @@ def solve_lp_pdhg_batch(
     *,
     check_every: int = 50,
     block: int = 0,
+    scheme: str = "adaptive",
 ) -> list:
     problems = list(problems)
     if not problems:
@@ -80,7 +83,7 @@ def solve_lp_pdhg_batch(
         out[k] = SolveResult("optimal", np.zeros(problems[k].n_vars), 0.0, 0)
     if live:
         pk = PackedBatch([problems[k] for k in live])
-        eng = BatchEngine(pk, block=block)
+        eng = BatchEngine(pk, block=block, scheme=scheme)
         ws = None if warm_start is None else [warm_start[k] for k in live]
         x0, y0, w = pack_warm_start(pk, ws)
         o, xs, ys = eng.solve(tol, max_iters, check_every, x0, y0, w)

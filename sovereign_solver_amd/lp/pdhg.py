@@ -22,9 +22,20 @@ Keyword-only extensions (defaults keep the reference's contract):
     ascending summation and no FMA contraction, i.e. SciPy's ``csr_matvec``
     arithmetic; the iterates then match the reference's bit for bit up to
     the KKT reductions (whose order differs at the 1e-16 level).
+``scheme``
+    ``"reference"``: the reference's restart rule (window average, restart when
+    the KKT falls below half its value at the last restart) with fixed steps;
+    ``"adaptive"``: restart candidate = better of average and current iterate,
+    restarts on sufficient/necessary decay or a long window, and a primal
+    weight updated at each restart from the distances moved since the last
+    one (``"adaptive_start"``: from the distances to the starting point;
+    ``"adaptive_fixed"``: weight kept); ``"auto"`` (default): adaptive for the
+    box form, reference for ``formulation="reference"`` and ``exact``.
 ``x0, y0, step_scale``
     Warm start (box form: x in the original space, y ordered ``[A_ub; A_eq]``)
-    and a primal weight ω: τ ← ω·τ, σ ← σ/ω (ω = 1 is the reference's step).
+    and the starting primal weight ω: τ ← ω·τ, σ ← σ/ω (ω = 1 is the
+    reference's step; None = 1 for the reference scheme, a norm-ratio
+    estimate for the adaptive one).
 ``check_every``, ``tune``
     KKT/restart cadence (50 = reference) and launch shape (default: the
     autotuner's heuristic).
@@ -55,6 +66,17 @@ def _precision(precision: str, tol: float) -> str:
     return precision
 
 
+def _scheme(scheme: str, formulation: str, exact: bool) -> int:
+    if scheme == "auto":
+        return 0 if (exact or formulation == "reference") else 1
+    codes = {"reference": 0, "adaptive": 1, "adaptive_fixed": 2, "adaptive_start": 3}
+    if scheme not in codes:
+        raise ValueError(f"scheme must be 'auto' or one of {sorted(codes)}")
+    if exact and scheme != "reference":
+        raise ValueError("exact=True reproduces the reference scheme only")
+    return codes[scheme]
+
+
 def _warm_start(form, p: Problem, x0, y0):
     if x0 is None and y0 is None:
         return None, None
@@ -79,7 +101,8 @@ def solve_lp_pdhg(
     exact: bool = False,
     x0: Optional[np.ndarray] = None,
     y0: Optional[np.ndarray] = None,
-    step_scale: float = 1.0,
+    step_scale: Optional[float] = None,
+    scheme: str = "auto",
     check_every: int = 50,
     tune: Optional[Tune] = None,
     use_graphs: bool = True,
@@ -88,16 +111,18 @@ def solve_lp_pdhg(
     if form.m == 0 or form.n == 0:                      # pdhg.py:171-172
         return SolveResult("optimal", np.zeros(problem.n_vars), 0.0, 0)
     prec = _precision(precision, tol)
+    sch = _scheme(scheme, formulation, exact)
     if exact:
         tune = Tune(1, 1, 256, check_every, 0, 1)
     elif tune is None:
         tune = heuristic_tune(form.m, form.n, form.nnz, check_every)
-    eng = SingleEngine(form, prec, tune=tune, check_every=check_every, exact=exact, use_graphs=use_graphs)
+    eng = SingleEngine(form, prec, tune=tune, check_every=check_every, exact=exact, use_graphs=use_graphs,
+                       scheme=sch)
     try:
         xs, ys = _warm_start(form, problem, x0, y0)
-        r = eng.solve(tol, max_iters, xs, ys, tau_scale=step_scale, sigma_scale=1.0 / step_scale)
+        r = eng.solve(tol, max_iters, xs, ys, step_scale=step_scale)
     finally:
         eng.drop_graphs()
     x, obj, y = extract(form, problem, r["x"], r["y"])
     return SolveResult(r["status"], x, obj, r["iterations"], y=y, rel_kkt=r["rel_kkt"], rp=r["rp"],
-                       rd=r["rd"], gap=r["gap"], restarts=r["restarts"])
+                       rd=r["rd"], gap=r["gap"], restarts=r["restarts"], step_scale=r["ctrl"].tau_scale)

@@ -7,7 +7,7 @@ space), ``objective`` (original sense), ``iterations``, ``message``,
 the reference never returned: the dual vector ``y`` (rows ordered
 ``[A_ub; A_eq]``), the relative KKT error and its parts measured by the
 reference formula (``solver/lp/pdhg.py:184-196``) on the reference-form lift,
-and the restart count.  Callers that only use the reference fields see no
+the restart count and the primal weight at exit.  Callers that only use the reference fields see no
 difference.
 """
 
@@ -35,6 +35,7 @@ class SolveResult:
         rd: Optional[float] = None,
         gap: Optional[float] = None,
         restarts: int = 0,
+        step_scale: Optional[float] = None,
     ):
         self.status = status
         self.x = x
@@ -49,6 +50,7 @@ class SolveResult:
         self.rd = rd
         self.gap = gap
         self.restarts = restarts
+        self.step_scale = step_scale    # primal weight at exit: pass it back to resume a related solve
 
     def __repr__(self) -> str:
         obj = f"{self.objective:.6g}" if self.objective is not None else "N/A"

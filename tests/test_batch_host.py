@@ -8,10 +8,10 @@ from sovereign_solver_amd.workloads import random_sparse_lp, transportation_lp, 
 
 
 def test_descriptor_layout_matches_header():
-    # PdlpBatchLP: 4 int32, 4 int64, 4 double; PdlpBatchOut: 4 int32, 6 double
+    # PdlpBatchLP: 4 int32, 4 int64, 4 double; PdlpBatchOut: 4 int32, 7 double
     assert LP_DTYPE.itemsize == 4 * 4 + 4 * 8 + 4 * 8
     assert LP_DTYPE.fields["row_off"][1] == 16 and LP_DTYPE.fields["norm_b"][1] == 48
-    assert OUT_DTYPE.itemsize == 4 * 4 + 6 * 8
+    assert OUT_DTYPE.itemsize == 4 * 4 + 7 * 8
     assert OUT_DTYPE.fields["rel_kkt"][1] == 16
 
 
@@ -63,4 +63,4 @@ def test_warm_start_packing_clips_and_projects():
     assert np.allclose(x0[:6], probs[0].ub)            # clipped to ub
     assert np.all(x0[6:] == 0.0)                        # clipped to lb = 0
     assert np.all(y0[4:6] == 0.0) and np.all(y0[6:] == 1.0)   # <= rows projected to y <= 0
-    assert w[0] == 2.0 and w[1] == 1.0
+    assert w[0] == 2.0 and np.isnan(w[1])                # NaN = the engine default

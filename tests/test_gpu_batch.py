@@ -63,11 +63,11 @@ def test_batch_agrees_with_single_engine(gpu):
 def test_batch_warm_start_from_solution(gpu):
     probs = [random_sparse_lp(80, 200, 0.04, seed=s) for s in range(5)]
     cold = solve_lp_pdhg_batch(probs, tol=1e-4)
-    warm = solve_lp_pdhg_batch(probs, warm_start=[(r.x, r.y) for r in cold], tol=1e-4)
+    warm = solve_lp_pdhg_batch(probs, warm_start=[(r.x, r.y, r.step_scale) for r in cold], tol=1e-4)
     for c, w in zip(cold, warm):
         # the cold result sits just under the tolerance and FP32 iterates restart near it,
         # so the warm solve needs a few checks, still far fewer iterations than the cold one
-        assert w.status == "optimal" and w.iterations <= max(250, c.iterations // 4) < c.iterations
+        assert w.status == "optimal" and w.iterations <= c.iterations // 2
 
 
 def test_batch_iteration_limit_and_trivial(gpu):

@@ -155,7 +155,7 @@ def test_graphs_and_direct_launches_agree_bitwise(gpu):
 def test_warm_start_from_solution_stops_at_first_check(gpu):
     p = random_sparse_lp(100, 250, 0.03, seed=3)
     a = solve_lp_pdhg(p, tol=1e-4, precision="fp64")
-    b = solve_lp_pdhg(p, tol=1e-4, precision="fp64", x0=a.x, y0=a.y)
+    b = solve_lp_pdhg(p, tol=1e-4, precision="fp64", x0=a.x, y0=a.y, step_scale=a.step_scale)
     assert b.status == "optimal" and b.iterations <= 100 < a.iterations
 
 
