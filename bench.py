@@ -251,7 +251,8 @@ def bench_learned(args):
     feats = D.features_of(fam, train_ids)[keep]
     targets = D.targets_of(lab)[keep]
     t1 = time.perf_counter()
-    mlp, nin, nout, losses = D.train(feats, targets, d_h=512, epochs=args.epochs, batch=256, lr=1e-3, log=log)
+    mlp, nin, nout, losses = D.train(feats, targets, d_h=512, epochs=args.epochs, batch=256, lr=1e-3, log=log,
+                                     out_blocks=D.output_blocks(fam))
     t_train = time.perf_counter() - t1
     log(f"trained in {t_train:.1f}s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
 
@@ -261,6 +262,19 @@ def bench_learned(args):
     warm = D.predict_warm_start(mlp, nin, nout, fam, eval_ids, use_step=not args.no_step)
     t_pred = time.perf_counter() - t2
     ev = D.evaluate(fam, eval_ids, warm, tol=args.tol, max_iters=args.max_iters)
+    # the same under the reference restart scheme, and without the predicted step
+    ev_ref = D.evaluate(fam, eval_ids, warm, tol=args.tol, max_iters=args.max_iters, scheme="reference")
+    warm_nostep = [(x0, y0, None) for x0, y0, _ in warm]
+    _, t_ns = D.solve_family(fam, eval_ids, args.tol, args.max_iters, warm=warm_nostep)
+    it_ns = np.array([r.iterations for r in _])
+    partial = {}
+    for name, ws in (("x_only", [(x0, None, s) for x0, _, s in warm]), ("y_only", [(None, y0, s) for _, y0, s in warm])):
+        rr, tt = D.solve_family(fam, eval_ids, args.tol, args.max_iters, warm=ws)
+        partial[name] = {"iters_mean": float(np.mean([r.iterations for r in rr])), "s": tt}
+    log(f"partial warm starts (adaptive): {partial}")
+    log(f"reference scheme: cold iters {ev_ref['cold_iters'].mean():.0f} ({ev_ref['cold_s']:.2f}s), warm iters "
+        f"{ev_ref['warm_iters'].mean():.0f} ({ev_ref['warm_s']:.2f}s); adaptive warm without predicted step: "
+        f"iters {it_ns.mean():.0f} ({t_ns:.2f}s)")
     n_eval = len(eval_ids)
     cold_lps = ev["cold_ok"] / ev["cold_s"]
     warm_lps = ev["warm_ok"] / (ev["warm_s"] + t_pred)
@@ -301,7 +315,11 @@ def bench_learned(args):
                    "cold_ok": int(ev["cold_ok"]), "warm_ok": int(ev["warm_ok"]),
                    "plain_single_gpu_lps_per_s": plain, "plain_sample": args.plain_sample,
                    "speedup_vs_plain": warm_lps / plain if plain > 0 else None,
-                   "objective_rel_err_vs_highs_max": max(errs), "use_step": not args.no_step},
+                   "objective_rel_err_vs_highs_max": max(errs), "use_step": not args.no_step,
+                   "reference_scheme": {"cold_iters_mean": float(ev_ref["cold_iters"].mean()),
+                                        "warm_iters_mean": float(ev_ref["warm_iters"].mean()),
+                                        "cold_s": ev_ref["cold_s"], "warm_s": ev_ref["warm_s"]},
+                   "warm_without_step_iters_mean": float(it_ns.mean()), "partial_warm": partial},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -338,7 +356,7 @@ def main():
     ap.add_argument("--sources", type=int, default=1000)
     ap.add_argument("--sinks", type=int, default=2000)
     ap.add_argument("--tol", type=float, default=1e-4)
-    ap.add_argument("--max-iters", type=int, default=200_000)
+    ap.add_argument("--max-iters", type=int, default=100_000, help="per LP (the reference's MAX_PDHG_ITERS)")
     ap.add_argument("--cpu-sample-iters", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ref-iters", action="store_true",

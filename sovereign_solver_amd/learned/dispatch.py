@@ -60,7 +60,7 @@ class Labels:
 
 
 def solve_family(fam: DispatchFamily, ids, tol: float, max_iters: int, warm=None, chunk: int = 2048,
-                 block: int = 0):
+                 block: int = 0, scheme: str = "adaptive"):
     """Batched solves of family instances; returns SolveResults and engine seconds (launch to results)."""
     res = []
     secs = 0.0
@@ -68,7 +68,7 @@ def solve_family(fam: DispatchFamily, ids, tol: float, max_iters: int, warm=None
         part = ids[s: s + chunk]
         probs = [fam.problem(int(k)) for k in part]
         pk = PackedBatch(probs)
-        eng = BatchEngine(pk, block=block)
+        eng = BatchEngine(pk, block=block, scheme=scheme)
         x0, y0, w = pack_warm_start(pk, None if warm is None else warm[s: s + chunk])
         eng.set_warm_start(x0, y0, w)
         torch.cuda.synchronize()
@@ -91,9 +91,21 @@ def label(fam: DispatchFamily, ids, tol: float = 1e-4, max_iters: int = 100_000)
 
 
 class Normaliser:
-    def __init__(self, a: np.ndarray, floor: float = 1e-6):
+    """Per-dimension mean; scale per dimension (default) or one scale per block.
+
+    ``blocks`` (list of slices) gives every dimension of a block the same scale,
+    the RMS deviation of the whole block: a near-constant output (a variable
+    at a bound, up to solver noise) is then not inflated to unit variance."""
+
+    def __init__(self, a: np.ndarray, floor: float = 1e-6, blocks=None):
         self.mean = a.mean(axis=0)
-        self.std = np.maximum(a.std(axis=0), floor)
+        if blocks is None:
+            self.std = np.maximum(a.std(axis=0), floor)
+        else:
+            self.std = np.empty(a.shape[1])
+            for sl in blocks:
+                d = a[:, sl] - self.mean[sl]
+                self.std[sl] = max(float(np.sqrt(np.mean(d * d))), floor)
 
     def fwd(self, a):
         return (a - self.mean) / self.std
@@ -106,11 +118,17 @@ def targets_of(lab: Labels) -> np.ndarray:
     return np.concatenate([lab.x, lab.y, lab.log_step[:, None]], axis=1)
 
 
+def output_blocks(fam: DispatchFamily):
+    """x, y and step blocks of the network output."""
+    return [slice(0, fam.n), slice(fam.n, fam.n + fam.m), slice(fam.n + fam.m, fam.n + fam.m + 1)]
+
+
 def train(feats: np.ndarray, targets: np.ndarray, d_h: int = 512, epochs: int = 20, batch: int = 512,
-          lr: float = 1e-3, seed: int = 0, step_weight: float = 200.0, log=None):
+          lr: float = 1e-3, seed: int = 0, step_weight: float = 200.0, log=None, out_blocks=None):
     """Fit the MLP on normalised (features, targets); returns (mlp, in_norm, out_norm, losses)."""
     dev = torch.device("cuda", torch.cuda.current_device())
-    nin, nout = Normaliser(feats), Normaliser(targets)
+    nin = Normaliser(feats)
+    nout = Normaliser(targets, blocks=out_blocks) if out_blocks is not None else Normaliser(targets)
     X = torch.from_numpy(nin.fwd(feats).astype(np.float32)).to(dev).to(torch.bfloat16)
     T = torch.from_numpy(nout.fwd(targets).astype(np.float32)).to(dev)
     w = np.ones(targets.shape[1], np.float32)
@@ -161,10 +179,11 @@ def predict_warm_start(mlp: WarmStartMLP, nin: Normaliser, nout: Normaliser, fam
     return warm
 
 
-def evaluate(fam: DispatchFamily, ids, warm, tol: float = 1e-4, max_iters: int = 100_000):
+def evaluate(fam: DispatchFamily, ids, warm, tol: float = 1e-4, max_iters: int = 100_000,
+             scheme: str = "adaptive"):
     """Cold vs warm batched solves of the same instances."""
-    cold, t_cold = solve_family(fam, ids, tol, max_iters)
-    hot, t_hot = solve_family(fam, ids, tol, max_iters, warm=warm)
+    cold, t_cold = solve_family(fam, ids, tol, max_iters, scheme=scheme)
+    hot, t_hot = solve_family(fam, ids, tol, max_iters, warm=warm, scheme=scheme)
     return dict(
         cold_iters=np.array([r.iterations for r in cold]), warm_iters=np.array([r.iterations for r in hot]),
         cold_ok=sum(r.status == "optimal" for r in cold), warm_ok=sum(r.status == "optimal" for r in hot),
