@@ -271,6 +271,95 @@ typedef struct PdlpBatch {
 int pdlp_batch_solve(const PdlpBatch* batch, void* stream);
 
 /* ----------------------------------------------------------------------------
+ * Shared-matrix batch ("family"): B box-form LPs with one constraint matrix K
+ * (m x n, row kinds shared) and their own c, b, lo, hi — parametric families
+ * (varying costs and demands) and branch-and-bound nodes (varying bounds),
+ * the node loop of solver/milp/branch_and_bound.py:367-374 and the per-instance
+ * solve_lp_pdhg calls of solver/lp/pdhg.py:152.
+ *
+ * All LPs step in lockstep.  Per-LP vectors live in tiles of 64 LPs, LP index
+ * fastest: element d of LP k in a vector of length L is at
+ *     ((k / 64) * L + d) * 64 + k % 64
+ * so one wavefront updates one row (or column) of 64 LPs with coalesced
+ * loads, and the CSR of K is read with uniform (scalar) loads.  A vector
+ * marked shared (PDLP_FAM_SHARED_* bit) is stored once, length L.
+ *
+ * Per check (every check_every iterations) the window sums are folded, the
+ * reference KKT (pdhg.py:184-196 on the lifted form, FP64) of the window
+ * average and of the current iterate is evaluated for every running LP, and
+ * each LP takes its own termination / restart decision (pdhg.py:225-244, or
+ * the adaptive scheme as in PdlpBatchLP.flags).  Finished LPs are frozen and
+ * skipped by tiles; their FP64 solution is in x_sum / y_sum.
+ * ------------------------------------------------------------------------- */
+
+#define PDLP_FAM_TILE 64
+#define PDLP_FAM_SHARED_C 1
+#define PDLP_FAM_SHARED_B 2
+#define PDLP_FAM_SHARED_LO 4
+#define PDLP_FAM_SHARED_HI 8
+#define PDLP_FAM_SHARED_C64 16
+#define PDLP_FAM_SHARED_B64 32
+#define PDLP_FAM_SHARED_BSHIFT 64
+#define PDLP_FAM_SHARED_LOSHIFT 128
+#define PDLP_FAM_SHARED_UBSPAN 256
+#define PDLP_FAM_KKT_CHUNKS 32     /* partial-sum chunks per KKT pass (max)   */
+#define PDLP_FAM_NSUM 12            /* 6 sums x (window average, current)      */
+
+typedef struct PdlpFamilyLP {
+    int32_t status, iter, window, restarts, checks, done_iter, last_restart, action;
+    double norm_b, norm_c;  /* reference-form normalisers                       */
+    double ref_err, err_prev, w_div;
+    float weight;           /* primal weight: tau = w tau_hat, sigma = sigma_hat / w */
+    int32_t pad;
+} PdlpFamilyLP;
+
+typedef struct PdlpFamily {
+    int32_t m, n, count;    /* rows, columns, LPs                              */
+    int32_t tiles;          /* ceil(count / 64)                                */
+    int32_t check_every;
+    int32_t flags;          /* as PdlpBatchLP.flags, for every LP              */
+    int32_t shared;         /* PDLP_FAM_SHARED_* bits                          */
+    int32_t items_per_wave; /* rows / columns per wavefront in a step, 1..64   */
+    double tol;
+    /* K and K^T, CSR, FP32 values */
+    const int32_t* row_ptr; const int32_t* col_idx; const float* val;
+    const int32_t* t_row_ptr; const int32_t* t_col_idx; const float* t_val;
+    const int8_t* row_kind;       /* m, PDLP_ROW_EQ / PDLP_ROW_LE            */
+    const float* tau_hat;         /* n: 0.999 / max(sum_i |K_ij|, 1e-4)      */
+    const float* sigma_hat;       /* m: 0.999 / max(sum_j |K_ij|, 1e-4)      */
+    const double* row_metric;     /* m: max(sum_j |K_ij|, 1e-4) / 0.999      */
+    /* problem data, tiled per LP or shared */
+    const float *c, *lo, *hi, *b;
+    const double *c64, *lo_shift, *ub_span, *b64, *b_shift;
+    /* state, tiled; x and y hold the start on entry to pdlp_family_init */
+    float *x, *xbar, *x_psum, *x_last;
+    float *y, *y_psum, *y_last;
+    double *x_sum, *y_sum;
+    PdlpFamilyLP* lp;             /* count                                    */
+    PdlpBatchOut* out;            /* count                                    */
+    uint64_t* run_mask;           /* tiles: running LPs of the tile           */
+    int32_t* running;             /* 1: number of running LPs after a check   */
+    double* partials;             /* 2 * PDLP_FAM_KKT_CHUNKS * 12 * tiles * 64 */
+} PdlpFamily;
+
+/* Set up every LP from (x, y), lp[k].norm_b/norm_c/weight as filled by the
+ * host: sums cleared, restart point = start, KKT of the start. */
+int pdlp_family_init(const PdlpFamily* f, void* stream);
+/* `blocks` times: check_every iterations of every running LP, then a check. */
+int pdlp_family_run(const PdlpFamily* f, int32_t blocks, void* stream);
+/* `iters` iterations without a check (the tail below check_every). */
+int pdlp_family_iterate(const PdlpFamily* f, int32_t iters, void* stream);
+/* End: LPs still running report PDLP_RUNNING with their current iterate as
+ * the solution (x_sum / y_sum); `tail` iterations ran since the last check. */
+int pdlp_family_finish(const PdlpFamily* f, int32_t tail, void* stream);
+/* HIP graph of `blocks` x (check_every iterations + check), as pdlp_family_run;
+ * launch with pdlp_graph_launch, free with pdlp_graph_destroy. */
+int pdlp_family_graph_capture(const PdlpFamily* f, int32_t blocks, void* stream, void** exec);
+/* HIP graph of `reps` launches of one step kernel (which: 0 column pass,
+ * 1 row pass), for timing. */
+int pdlp_family_graph_capture_step(const PdlpFamily* f, int32_t which, int32_t reps, void* stream, void** exec);
+
+/* ----------------------------------------------------------------------------
  * Learned warm start: a 3-layer MLP (d_in -> d_h -> d_h -> d_out, ReLU) whose
  * GEMMs run on BF16 MFMA (v_mfma_f32_32x32x16_bf16) with FP32 accumulation.
  * The reference has no learned component (its PDHG always starts from 0,

@@ -1,0 +1,652 @@
+// Shared-matrix batched restarted PDHG ("family" engine).
+//
+// B box-form LPs share one matrix K; they differ in c, b, lo, hi.  The LPs
+// step in lockstep and every per-LP vector is stored in tiles of 64 LPs with
+// the LP index fastest, so a wavefront owns one tile: lane = LP, and one
+// wavefront-wide step updates row i (or column j) of 64 LPs.  K's CSR entries
+// are then the same for every lane (scalar loads) and each gathered value is
+// one coalesced 256-byte read of 64 LPs.  Per iteration:
+//
+//   column pass  x+ = clip(x - w tau_hat (c - K^T y)),  xbar = 2x+ - x
+//   row pass     y+ = proj(y + (sigma_hat / w) (b - K xbar))
+//
+// with FP32 iterates and FP32 partial sums of the window.  Every check_every
+// iterations: the partial sums are folded into FP64 window sums, the
+// reference KKT (solver/lp/pdhg.py:184-196 on the lifted form) of the window
+// average and of the current iterate is reduced in FP64 for every LP, and each
+// LP decides on termination / restart (pdhg.py:225-244, or the adaptive
+// scheme) exactly as the persistent batched kernel does.  Finished LPs are
+// frozen (their lane stops writing; a tile whose LPs are all finished is
+// skipped) and hold their FP64 solution in x_sum / y_sum.
+
+#include "pdlp_common.hpp"
+
+#include <algorithm>
+
+namespace pdlp {
+namespace {
+
+constexpr int T = PDLP_FAM_TILE;     // LPs per tile = lanes per wavefront
+constexpr int NT = 256;              // threads per workgroup
+constexpr int NWV = NT / kWave;      // wavefronts per workgroup
+constexpr int NS2 = PDLP_FAM_NSUM;   // 6 sums x 2 candidates
+constexpr int KCH = PDLP_FAM_KKT_CHUNKS;
+
+static_assert(T == kWave, "one tile per wavefront");
+
+enum Action : int32_t { ACT_NONE = 0, ACT_RESTART_AVG = 1, ACT_RESTART_CUR = 2, ACT_DONE_AVG = 3, ACT_DONE_CUR = 4 };
+
+__device__ __forceinline__ size_t at(int tile, int len, int d, int lane) {
+    return ((size_t)tile * len + d) * T + lane;
+}
+
+template <typename V>
+__device__ __forceinline__ V ldv(const V* p, bool shared, int tile, int len, int d, int lane) {
+    return shared ? p[d] : p[at(tile, len, d, lane)];
+}
+
+// wavefront index inside the workgroup, as a uniform value
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+
+// sum over q in [beg, end) of val[q] * v(LP lane, index idx[q]) in ascending q.
+// The (uniform) index/value pairs are taken 8 at a time so that 8 gathers are
+// in flight together; the summation order stays sequential.
+__device__ __forceinline__ float gather_dot(const int32_t* __restrict__ idx, const float* __restrict__ val, int beg,
+                                            int end, const float* __restrict__ v, int tile, int len, int lane) {
+    float d = 0.f;
+    int q = beg;
+    for (; q + 8 <= end; q += 8) {
+        float g[8], w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            w[u] = val[q + u];
+            g[u] = v[at(tile, len, idx[q + u], lane)];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) d += w[u] * g[u];
+    }
+    for (; q < end; ++q) d += val[q] * v[at(tile, len, idx[q], lane)];
+    return d;
+}
+
+// ---------------------------------------------------------------- iteration
+__global__ void __launch_bounds__(NT) primal_kernel(PdlpFamily f) {
+    const int tile = blockIdx.y;
+    const uint64_t mask = f.run_mask[tile];
+    if (mask == 0) return;
+    const int lane = lane_id(), w = wave_id();
+    const bool act = (mask >> lane) & 1ull;
+    const int k = tile * T + lane;
+    const float wgt = act ? f.lp[k].weight : 1.f;
+    const bool shc = f.shared & PDLP_FAM_SHARED_C, shl = f.shared & PDLP_FAM_SHARED_LO,
+               shh = f.shared & PDLP_FAM_SHARED_HI;
+    const int j0 = (blockIdx.x * NWV + w) * f.items_per_wave;
+    const int j1 = min(j0 + f.items_per_wave, f.n);
+    for (int j = j0; j < j1; ++j) {
+        const int beg = f.t_row_ptr[j], end = f.t_row_ptr[j + 1];
+        const float aty = gather_dot(f.t_col_idx, f.t_val, beg, end, f.y, tile, f.m, lane);
+        const size_t ix = at(tile, f.n, j, lane);
+        const float xo = f.x[ix];
+        float xn = xo - (wgt * f.tau_hat[j]) * (ldv(f.c, shc, tile, f.n, j, lane) - aty);
+        const float lo = ldv(f.lo, shl, tile, f.n, j, lane), hi = ldv(f.hi, shh, tile, f.n, j, lane);
+        xn = xn < lo ? lo : xn;
+        xn = xn > hi ? hi : xn;
+        if (act) {
+            f.x[ix] = xn;
+            f.xbar[ix] = 2.f * xn - xo;
+            f.x_psum[ix] += xn;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(NT) dual_kernel(PdlpFamily f) {
+    const int tile = blockIdx.y;
+    const uint64_t mask = f.run_mask[tile];
+    if (mask == 0) return;
+    const int lane = lane_id(), w = wave_id();
+    const bool act = (mask >> lane) & 1ull;
+    const int k = tile * T + lane;
+    const float inv_wgt = act ? 1.f / f.lp[k].weight : 1.f;
+    const bool shb = f.shared & PDLP_FAM_SHARED_B;
+    const int i0 = (blockIdx.x * NWV + w) * f.items_per_wave;
+    const int i1 = min(i0 + f.items_per_wave, f.m);
+    for (int i = i0; i < i1; ++i) {
+        const int beg = f.row_ptr[i], end = f.row_ptr[i + 1];
+        const float d = gather_dot(f.col_idx, f.val, beg, end, f.xbar, tile, f.n, lane);
+        const size_t ix = at(tile, f.m, i, lane);
+        const float sigma = inv_wgt * f.sigma_hat[i];
+        float yn = f.y[ix] + sigma * (ldv(f.b, shb, tile, f.m, i, lane) - d);
+        if (f.row_kind[i] == PDLP_ROW_LE && yn > 0.f) yn = 0.f;
+        if (act) {
+            f.y[ix] = yn;
+            f.y_psum[ix] += yn;
+        }
+    }
+}
+
+// ------------------------------------------------------------------- checks
+// Per-lane sums of both candidates (window average a[0..6), current a[6..12)),
+// reduced over the workgroup's wavefronts and stored as one chunk.
+__device__ __forceinline__ void store_chunk(const PdlpFamily& f, int chunk, int tile, double (&a)[NS2]) {
+    __shared__ double red[NWV - 1][NS2][T];
+    const int lane = lane_id(), w = wave_id();
+    if (w > 0)
+#pragma unroll
+        for (int s = 0; s < NS2; ++s) red[w - 1][s][lane] = a[s];
+    __syncthreads();
+    if (w == 0) {
+        const size_t cpad = (size_t)f.tiles * T;
+#pragma unroll
+        for (int s = 0; s < NS2; ++s) {
+            double t = a[s];
+#pragma unroll
+            for (int v = 0; v < NWV - 1; ++v) t += red[v][s][lane];
+            f.partials[((size_t)chunk * NS2 + s) * cpad + (size_t)tile * T + lane] = t;
+        }
+    }
+}
+
+// Columns: fold the x partial sums; primal objective and distance of both
+// candidates; K^T y of both candidates (y window sums not folded yet: the row
+// pass folds them after this pass) for the dual residual / bound-row duals.
+__global__ void __launch_bounds__(NT) kkt_cols_kernel(PdlpFamily f, int add) {
+    const int tile = blockIdx.y;
+    const uint64_t mask = f.run_mask[tile];
+    if (mask == 0) return;                      // whole workgroup: no barrier is skipped by part of it
+    const int lane = lane_id(), w = wave_id();
+    const bool act = (mask >> lane) & 1ull;
+    const int k = tile * T + lane;
+    const int win = act ? f.lp[k].window + add : 0;   // window length including this check's iterations
+    const double wd = (double)win;
+    const bool adaptive = f.flags & 1;
+    const bool shc = f.shared & PDLP_FAM_SHARED_C64, shl = f.shared & PDLP_FAM_SHARED_LOSHIFT,
+               shu = f.shared & PDLP_FAM_SHARED_UBSPAN;
+    double a[NS2];
+#pragma unroll
+    for (int s = 0; s < NS2; ++s) a[s] = 0.0;
+    for (int j = blockIdx.x * NWV + w; j < f.n; j += gridDim.x * NWV) {
+        const size_t ix = at(tile, f.n, j, lane);
+        const double xs = f.x_sum[ix] + (double)f.x_psum[ix];
+        if (act) {
+            f.x_sum[ix] = xs;
+            f.x_psum[ix] = 0.f;
+        }
+        const double xc = (double)f.x[ix];
+        const double xa = win > 0 ? xs / wd : xc;
+        const double c = ldv(f.c64, shc, tile, f.n, j, lane);
+        const double ls = ldv(f.lo_shift, shl, tile, f.n, j, lane);
+        a[2] += c * (xa - ls);
+        a[8] += c * (xc - ls);
+        if (adaptive) {
+            const double xl = (double)f.x_last[ix], th = (double)f.tau_hat[j];
+            const double da = xa - xl, dc = xc - xl;
+            a[4] += da * da / th;
+            a[10] += dc * dc / th;
+        }
+        double sa = 0.0, sc = 0.0;
+        const int beg = f.t_row_ptr[j], end = f.t_row_ptr[j + 1];
+        for (int q = beg; q < end; ++q) {
+            const double kv = (double)f.t_val[q];
+            const size_t iy = at(tile, f.m, f.t_col_idx[q], lane);
+            const double yc = (double)f.y[iy];
+            const double ya = win > 0 ? (f.y_sum[iy] + (double)f.y_psum[iy]) / wd : yc;
+            sa += kv * ya;
+            sc += kv * yc;
+        }
+        const double span = ldv(f.ub_span, shu, tile, f.n, j, lane);
+        const double ra = fmin(c - sa, 0.0), rc = fmin(c - sc, 0.0);
+        if (isfinite(span)) {
+            a[3] += span * ra;
+            a[9] += span * rc;
+        } else {
+            a[1] += ra * ra;
+            a[7] += rc * rc;
+        }
+    }
+    store_chunk(f, blockIdx.x, tile, a);
+}
+
+// Rows: fold the y partial sums; primal residual (K x of both candidates, x
+// window sums folded by the column pass), dual objective, sign violation of
+// y on <= rows, dual distance.
+__global__ void __launch_bounds__(NT) kkt_rows_kernel(PdlpFamily f, int add) {
+    const int tile = blockIdx.y;
+    const uint64_t mask = f.run_mask[tile];
+    if (mask == 0) return;
+    const int lane = lane_id(), w = wave_id();
+    const bool act = (mask >> lane) & 1ull;
+    const int k = tile * T + lane;
+    const int win = act ? f.lp[k].window + add : 0;   // window length including this check's iterations
+    const double wd = (double)win;
+    const bool adaptive = f.flags & 1;
+    const bool shb = f.shared & PDLP_FAM_SHARED_B64, shs = f.shared & PDLP_FAM_SHARED_BSHIFT;
+    double a[NS2];
+#pragma unroll
+    for (int s = 0; s < NS2; ++s) a[s] = 0.0;
+    for (int i = blockIdx.x * NWV + w; i < f.m; i += gridDim.x * NWV) {
+        const size_t iy = at(tile, f.m, i, lane);
+        const double ys = f.y_sum[iy] + (double)f.y_psum[iy];
+        if (act) {
+            f.y_sum[iy] = ys;
+            f.y_psum[iy] = 0.f;
+        }
+        const double yc = (double)f.y[iy];
+        const double ya = win > 0 ? ys / wd : yc;
+        double da = 0.0, dc = 0.0;
+        const int beg = f.row_ptr[i], end = f.row_ptr[i + 1];
+        for (int q = beg; q < end; ++q) {
+            const double kv = (double)f.val[q];
+            const size_t jx = at(tile, f.n, f.col_idx[q], lane);
+            const double xc = (double)f.x[jx];
+            const double xa = win > 0 ? f.x_sum[jx] / wd : xc;
+            da += kv * xa;
+            dc += kv * xc;
+        }
+        const bool le = f.row_kind[i] == PDLP_ROW_LE;
+        const double b = ldv(f.b64, shb, tile, f.m, i, lane);
+        double ra = da - b, rc = dc - b;
+        if (le && ra < 0.0) ra = 0.0;
+        if (le && rc < 0.0) rc = 0.0;
+        a[0] += ra * ra;
+        a[6] += rc * rc;
+        if (le && ya > 0.0) a[1] += ya * ya;
+        if (le && yc > 0.0) a[7] += yc * yc;
+        const double bs = ldv(f.b_shift, shs, tile, f.m, i, lane);
+        a[3] += bs * ya;
+        a[9] += bs * yc;
+        if (adaptive) {
+            const double yl = (double)f.y_last[iy], rm = f.row_metric[i];
+            const double da2 = ya - yl, dc2 = yc - yl;
+            a[5] += da2 * da2 * rm;
+            a[11] += dc2 * dc2 * rm;
+        }
+    }
+    store_chunk(f, KCH + blockIdx.x, tile, a);
+}
+
+struct Kkt {
+    double rp, rd, gap, err, po, dobj, dx, dy;
+};
+
+__device__ __forceinline__ Kkt kkt_of(const double* s, double norm_b, double norm_c) {
+    Kkt k;
+    k.rp = sqrt(s[0]) / norm_b;
+    k.rd = sqrt(s[1]) / norm_c;
+    k.po = s[2];
+    k.dobj = s[3];
+    k.gap = fabs(k.po - k.dobj) / (1.0 + fabs(k.po) + fabs(k.dobj));
+    k.err = fmax(k.rp, fmax(k.rd, k.gap));
+    if (!isfinite(k.po) || !isfinite(k.dobj)) k.err = NAN;
+    k.dx = sqrt(s[4]);
+    k.dy = sqrt(s[5]);
+    return k;
+}
+
+__device__ __forceinline__ void write_out(PdlpBatchOut* o, const PdlpFamilyLP& s, const Kkt& kk, int iters) {
+    PdlpBatchOut r;
+    r.status = s.status;
+    r.iterations = iters;
+    r.restarts = s.restarts;
+    r.checks = s.checks;
+    r.rel_kkt = kk.err;
+    r.rp = kk.rp;
+    r.rd = kk.rd;
+    r.gap = kk.gap;
+    r.p_obj = kk.po;
+    r.d_obj = kk.dobj;
+    r.weight = (double)s.weight;
+    *o = r;
+}
+
+// One thread per LP (one wavefront per tile).  mode 0: start (reference error
+// of the starting point); mode 1: a check after check_every iterations.
+__global__ void __launch_bounds__(T) decide_kernel(PdlpFamily f, int mode, int col_chunks, int row_chunks) {
+    const int tile = blockIdx.x, lane = threadIdx.x;
+    const int k = tile * T + lane;
+    const uint64_t mask = f.run_mask[tile];
+    const bool act = (mask >> lane) & 1ull;
+    bool running = false;
+    if (k < f.count) {
+        PdlpFamilyLP s = f.lp[k];
+        if (!act) {
+            s.action = ACT_NONE;
+        } else {
+            const size_t cpad = (size_t)f.tiles * T;
+            double sum[NS2];
+#pragma unroll
+            for (int q = 0; q < NS2; ++q) {
+                double t = 0.0;
+                for (int c = 0; c < col_chunks; ++c) t += f.partials[((size_t)c * NS2 + q) * cpad + k];
+                for (int c = 0; c < row_chunks; ++c) t += f.partials[((size_t)(KCH + c) * NS2 + q) * cpad + k];
+                sum[q] = t;
+            }
+            const bool adaptive = f.flags & 1, fixed_weight = f.flags & 2;
+            Kkt ka = kkt_of(sum, s.norm_b, s.norm_c);
+            const Kkt kc = kkt_of(sum + 6, s.norm_b, s.norm_c);
+            s.action = ACT_NONE;
+            if (mode == 0) {
+                s.ref_err = kc.err;
+                s.err_prev = INFINITY;
+                s.status = isfinite(kc.err) ? PDLP_RUNNING : PDLP_NUMERICAL;
+                if (s.status != PDLP_RUNNING) s.action = ACT_DONE_CUR;
+                write_out(&f.out[k], s, kc, 0);
+            } else {
+                const int it = s.iter + f.check_every;
+                s.iter = it;
+                s.window += f.check_every;
+                bool use_avg = true;
+                if (adaptive && (!isfinite(ka.err) || kc.err < ka.err)) {   // the better candidate
+                    use_avg = false;
+                    ka = kc;
+                }
+                ++s.checks;
+                if (!isfinite(ka.err)) {
+                    s.status = PDLP_NUMERICAL;
+                    s.done_iter = it;
+                    s.action = ACT_DONE_CUR;
+                } else if (ka.err < f.tol) {                                 // pdhg.py:231-235
+                    s.status = PDLP_OPTIMAL;
+                    s.done_iter = it;
+                    s.w_div = (double)s.window;
+                    s.action = use_avg ? ACT_DONE_AVG : ACT_DONE_CUR;
+                } else {
+                    bool restart;
+                    if (!adaptive) {
+                        restart = ka.err < 0.5 * s.ref_err;                  // pdhg.py:238-244
+                    } else {
+                        restart = ka.err <= 0.2 * s.ref_err || (ka.err <= 0.8 * s.ref_err && ka.err > s.err_prev) ||
+                                  (double)(it - s.last_restart) >= 0.36 * (double)it;
+                    }
+                    if (restart) {
+                        if (adaptive && !fixed_weight && ka.dx > 1e-10 && ka.dy > 1e-10) {
+                            const double lw = 0.5 * log(ka.dx / ka.dy) + 0.5 * log((double)s.weight);
+                            s.weight = (float)fmin(fmax(exp(lw), 1e-8), 1e8);
+                        }
+                        s.ref_err = ka.err;
+                        s.err_prev = INFINITY;
+                        s.last_restart = it;
+                        ++s.restarts;
+                        s.w_div = (double)s.window;
+                        s.window = 0;
+                        s.action = use_avg ? ACT_RESTART_AVG : ACT_RESTART_CUR;
+                    } else {
+                        s.err_prev = ka.err;
+                    }
+                }
+                write_out(&f.out[k], s, ka, s.status == PDLP_RUNNING ? it : s.done_iter);
+            }
+            running = s.status == PDLP_RUNNING;
+        }
+        f.lp[k] = s;
+    }
+    const uint64_t run = __ballot(running);
+    if (lane == 0) f.run_mask[tile] = run;
+}
+
+// Vector side of the decisions: restart to the average or the current point
+// (clear the window sums, move the restart point), or store the solution.
+__global__ void __launch_bounds__(NT) apply_kernel(PdlpFamily f) {
+    const int tile = blockIdx.y;
+    const int lane = lane_id(), w = wave_id();
+    const int k = tile * T + lane;
+    const int act = k < f.count ? f.lp[k].action : (int)ACT_NONE;
+    if (__ballot(act != ACT_NONE) == 0) return;           // no barrier in this kernel
+    const double wd = act != ACT_NONE ? f.lp[k].w_div : 1.0;
+    const bool move_ref = (f.flags & 4) == 0;
+    const int per = 16;
+    const int d0 = (blockIdx.x * NWV + w) * per;
+    const int d1 = min(d0 + per, f.n + f.m);
+    for (int d = d0; d < d1; ++d) {
+        const bool col = d < f.n;
+        const size_t ix = col ? at(tile, f.n, d, lane) : at(tile, f.m, d - f.n, lane);
+        float* v = col ? f.x : f.y;
+        double* sum = col ? f.x_sum : f.y_sum;
+        float* last = col ? f.x_last : f.y_last;
+        switch (act) {
+            case ACT_RESTART_AVG: {
+                const float nv = (float)(sum[ix] / wd);
+                v[ix] = nv;
+                sum[ix] = 0.0;
+                if (move_ref) last[ix] = nv;
+                break;
+            }
+            case ACT_RESTART_CUR:
+                sum[ix] = 0.0;
+                if (move_ref) last[ix] = v[ix];
+                break;
+            case ACT_DONE_AVG:
+                sum[ix] = sum[ix] / wd;
+                break;
+            case ACT_DONE_CUR:
+                sum[ix] = (double)v[ix];
+                break;
+            default:
+                break;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(NT) count_kernel(PdlpFamily f) {
+    int c = 0;
+    for (int t = threadIdx.x; t < f.tiles; t += NT) c += __popcll(f.run_mask[t]);
+    c = group_sum<kWave>(c);
+    __shared__ int red[NWV];
+    if (lane_id() == 0) red[wave_id()] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int v = 0; v < NWV; ++v) t += red[v];
+        *f.running = t;
+    }
+}
+
+// ---------------------------------------------------------------- set-up / end
+__global__ void __launch_bounds__(NT) init_vec_kernel(PdlpFamily f) {
+    const int tile = blockIdx.y;
+    const int lane = lane_id(), w = wave_id();
+    const int per = 16;
+    const int d0 = (blockIdx.x * NWV + w) * per;
+    const int d1 = min(d0 + per, f.n + f.m);
+    for (int d = d0; d < d1; ++d) {
+        if (d < f.n) {
+            const size_t ix = at(tile, f.n, d, lane);
+            f.x_sum[ix] = 0.0;
+            f.x_psum[ix] = 0.f;
+            f.x_last[ix] = f.x[ix];
+            f.xbar[ix] = f.x[ix];
+        } else {
+            const size_t ix = at(tile, f.m, d - f.n, lane);
+            f.y_sum[ix] = 0.0;
+            f.y_psum[ix] = 0.f;
+            f.y_last[ix] = f.y[ix];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(T) init_lp_kernel(PdlpFamily f) {
+    const int tile = blockIdx.x, lane = threadIdx.x;
+    const int k = tile * T + lane;
+    const bool valid = k < f.count;
+    if (valid) {
+        PdlpFamilyLP s = f.lp[k];
+        s.status = PDLP_RUNNING;
+        s.iter = s.window = s.restarts = s.checks = s.done_iter = s.last_restart = 0;
+        s.action = ACT_NONE;
+        s.ref_err = INFINITY;
+        s.err_prev = INFINITY;
+        s.w_div = 1.0;
+        f.lp[k] = s;
+    }
+    const uint64_t run = __ballot(valid);
+    if (lane == 0) f.run_mask[tile] = run;
+}
+
+// LPs still running at the end: iteration limit, current iterate as solution.
+__global__ void __launch_bounds__(T) finish_lp_kernel(PdlpFamily f, int tail) {
+    const int tile = blockIdx.x, lane = threadIdx.x;
+    const int k = tile * T + lane;
+    const bool act = (f.run_mask[tile] >> lane) & 1ull;
+    if (k < f.count) {
+        PdlpFamilyLP s = f.lp[k];
+        if (act) {
+            s.iter += tail;
+            s.done_iter = s.iter;
+            s.action = ACT_DONE_CUR;
+            f.out[k].iterations = s.iter;
+            f.out[k].weight = (double)s.weight;
+        } else {
+            s.action = ACT_NONE;
+        }
+        f.lp[k] = s;
+    }
+}
+
+__global__ void __launch_bounds__(T) clear_mask_kernel(PdlpFamily f) {
+    if (threadIdx.x == 0) f.run_mask[blockIdx.x] = 0;
+}
+
+// --------------------------------------------------------------------- host
+int check_family(const PdlpFamily* f) {
+    if (f == nullptr) return fail(PDLP_E_ARG, "PdlpFamily is NULL");
+    if (f->m <= 0 || f->n <= 0 || f->count <= 0) return fail(PDLP_E_SHAPE, "family needs m, n, count > 0");
+    if (f->tiles != (f->count + T - 1) / T) return fail(PDLP_E_SHAPE, "tiles must be ceil(count / 64)");
+    if (f->tiles > 65535) return fail(PDLP_E_LIMIT, "at most 65535 tiles of 64 LPs");
+    if (f->check_every <= 0) return fail(PDLP_E_ARG, "check_every must be > 0");
+    if (f->items_per_wave < 1 || f->items_per_wave > 64) return fail(PDLP_E_TUNE, "items_per_wave in [1, 64]");
+    if (!f->row_ptr || !f->col_idx || !f->val || !f->t_row_ptr || !f->t_col_idx || !f->t_val || !f->row_kind ||
+        !f->tau_hat || !f->sigma_hat || !f->row_metric || !f->c || !f->lo || !f->hi || !f->b || !f->c64 ||
+        !f->lo_shift || !f->ub_span || !f->b64 || !f->b_shift || !f->x || !f->xbar || !f->x_psum || !f->x_last ||
+        !f->y || !f->y_psum || !f->y_last || !f->x_sum || !f->y_sum || !f->lp || !f->out || !f->run_mask ||
+        !f->running || !f->partials)
+        return fail(PDLP_E_ARG, "a required PdlpFamily pointer is NULL");
+    return 0;
+}
+
+int chunks(int items) { return std::max(1, std::min(KCH, (items + NWV - 1) / NWV)); }
+
+dim3 step_grid(const PdlpFamily& f, int items) {
+    const int per_block = NWV * f.items_per_wave;
+    return dim3((unsigned)((items + per_block - 1) / per_block), (unsigned)f.tiles);
+}
+
+dim3 vec_grid(const PdlpFamily& f) {
+    const int per_block = NWV * 16;
+    return dim3((unsigned)((f.n + f.m + per_block - 1) / per_block), (unsigned)f.tiles);
+}
+
+void launch_iterations(const PdlpFamily& f, int iters, hipStream_t s) {
+    const dim3 gp = step_grid(f, f.n), gd = step_grid(f, f.m);
+    for (int it = 0; it < iters; ++it) {
+        primal_kernel<<<gp, NT, 0, s>>>(f);
+        dual_kernel<<<gd, NT, 0, s>>>(f);
+    }
+}
+
+void launch_check(const PdlpFamily& f, int mode, hipStream_t s) {
+    const int cc = chunks(f.n), rc = chunks(f.m);
+    const int add = mode == 0 ? 0 : f.check_every;
+    kkt_cols_kernel<<<dim3(cc, f.tiles), NT, 0, s>>>(f, add);
+    kkt_rows_kernel<<<dim3(rc, f.tiles), NT, 0, s>>>(f, add);
+    decide_kernel<<<f.tiles, T, 0, s>>>(f, mode, cc, rc);
+    apply_kernel<<<vec_grid(f), NT, 0, s>>>(f);
+    count_kernel<<<1, NT, 0, s>>>(f);
+}
+
+}  // namespace
+}  // namespace pdlp
+
+using namespace pdlp;
+
+extern "C" int pdlp_family_init(const PdlpFamily* f, void* stream) {
+    if (int e = check_family(f)) return e;
+    hipStream_t s = as_stream(stream);
+    init_lp_kernel<<<f->tiles, T, 0, s>>>(*f);
+    init_vec_kernel<<<vec_grid(*f), NT, 0, s>>>(*f);
+    launch_check(*f, 0, s);
+    return hip_fail(hipGetLastError(), "pdlp_family_init launch");
+}
+
+extern "C" int pdlp_family_run(const PdlpFamily* f, int32_t blocks, void* stream) {
+    if (int e = check_family(f)) return e;
+    if (blocks < 0) return fail(PDLP_E_ARG, "blocks must be >= 0");
+    hipStream_t s = as_stream(stream);
+    for (int b = 0; b < blocks; ++b) {
+        launch_iterations(*f, f->check_every, s);
+        launch_check(*f, 1, s);
+    }
+    return hip_fail(hipGetLastError(), "pdlp_family_run launch");
+}
+
+extern "C" int pdlp_family_iterate(const PdlpFamily* f, int32_t iters, void* stream) {
+    if (int e = check_family(f)) return e;
+    if (iters < 0) return fail(PDLP_E_ARG, "iters must be >= 0");
+    launch_iterations(*f, iters, as_stream(stream));
+    return hip_fail(hipGetLastError(), "pdlp_family_iterate launch");
+}
+
+extern "C" int pdlp_family_finish(const PdlpFamily* f, int32_t tail, void* stream) {
+    if (int e = check_family(f)) return e;
+    if (tail < 0) return fail(PDLP_E_ARG, "tail must be >= 0");
+    hipStream_t s = as_stream(stream);
+    finish_lp_kernel<<<f->tiles, T, 0, s>>>(*f, tail);
+    apply_kernel<<<vec_grid(*f), NT, 0, s>>>(*f);
+    clear_mask_kernel<<<f->tiles, T, 0, s>>>(*f);
+    count_kernel<<<1, NT, 0, s>>>(*f);
+    return hip_fail(hipGetLastError(), "pdlp_family_finish launch");
+}
+
+namespace {
+
+// Capture `reps` launches of `body` on `stream` into an instantiated graph.
+template <typename F>
+int capture(hipStream_t s, void** exec, F body) {
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+    body();
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s, &g);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail(e, "family graph capture");
+    }
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    *exec = (void*)ge;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int pdlp_family_graph_capture(const PdlpFamily* f, int32_t blocks, void* stream, void** exec) {
+    if (int e = check_family(f)) return e;
+    if (blocks <= 0 || exec == nullptr) return fail(PDLP_E_ARG, "blocks must be positive and exec non-NULL");
+    if (stream == nullptr) return fail(PDLP_E_ARG, "graph capture needs a non-default stream");
+    hipStream_t s = as_stream(stream);
+    return capture(s, exec, [&] {
+        for (int b = 0; b < blocks; ++b) {
+            launch_iterations(*f, f->check_every, s);
+            launch_check(*f, 1, s);
+        }
+    });
+}
+
+extern "C" int pdlp_family_graph_capture_step(const PdlpFamily* f, int32_t which, int32_t reps, void* stream,
+                                              void** exec) {
+    if (int e = check_family(f)) return e;
+    if (reps <= 0 || exec == nullptr || (which != 0 && which != 1))
+        return fail(PDLP_E_ARG, "reps > 0, which in {0, 1} and exec non-NULL");
+    if (stream == nullptr) return fail(PDLP_E_ARG, "graph capture needs a non-default stream");
+    hipStream_t s = as_stream(stream);
+    const dim3 g = step_grid(*f, which == 0 ? f->n : f->m);
+    return capture(s, exec, [&] {
+        for (int r = 0; r < reps; ++r) {
+            if (which == 0) primal_kernel<<<g, NT, 0, s>>>(*f);
+            else dual_kernel<<<g, NT, 0, s>>>(*f);
+        }
+    });
+}

@@ -20,6 +20,11 @@ PDLP_ROW_EQ, PDLP_ROW_LE = 0, 1
 PDLP_PARTIAL_BLOCKS = 1024
 PDLP_PARTIALS_LEN = 4 * PDLP_PARTIAL_BLOCKS * 6
 PDLP_SCHEME_REFERENCE, PDLP_SCHEME_ADAPTIVE = 0, 1
+PDLP_FAM_TILE = 64
+PDLP_FAM_SHARED = {"c": 1, "b": 2, "lo": 4, "hi": 8, "c64": 16, "b64": 32, "b_shift": 64, "lo_shift": 128,
+                   "ub_span": 256}
+PDLP_FAM_KKT_CHUNKS = 32
+PDLP_FAM_NSUM = 12
 
 _p = C.c_void_p
 _i32p = C.c_void_p
@@ -72,6 +77,30 @@ class PdlpBatch(C.Structure):
     ]
 
 
+class PdlpFamilyLP(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("iter", C.c_int32), ("window", C.c_int32), ("restarts", C.c_int32),
+        ("checks", C.c_int32), ("done_iter", C.c_int32), ("last_restart", C.c_int32), ("action", C.c_int32),
+        ("norm_b", C.c_double), ("norm_c", C.c_double), ("ref_err", C.c_double), ("err_prev", C.c_double),
+        ("w_div", C.c_double), ("weight", C.c_float), ("pad", C.c_int32),
+    ]
+
+
+class PdlpFamily(C.Structure):
+    _fields_ = [
+        ("m", C.c_int32), ("n", C.c_int32), ("count", C.c_int32), ("tiles", C.c_int32),
+        ("check_every", C.c_int32), ("flags", C.c_int32), ("shared", C.c_int32), ("items_per_wave", C.c_int32),
+        ("tol", C.c_double),
+        ("row_ptr", _p), ("col_idx", _p), ("val", _p), ("t_row_ptr", _p), ("t_col_idx", _p), ("t_val", _p),
+        ("row_kind", _p), ("tau_hat", _p), ("sigma_hat", _p), ("row_metric", _p),
+        ("c", _p), ("lo", _p), ("hi", _p), ("b", _p),
+        ("c64", _p), ("lo_shift", _p), ("ub_span", _p), ("b64", _p), ("b_shift", _p),
+        ("x", _p), ("xbar", _p), ("x_psum", _p), ("x_last", _p),
+        ("y", _p), ("y_psum", _p), ("y_last", _p), ("x_sum", _p), ("y_sum", _p),
+        ("lp", _p), ("out", _p), ("run_mask", _p), ("running", _p), ("partials", _p),
+    ]
+
+
 class PdlpGemm(C.Structure):
     _fields_ = [
         ("M", C.c_int32), ("N", C.c_int32), ("K", C.c_int32),
@@ -117,6 +146,13 @@ _SIGS = {
     "pdlp_graph_capture_step": (C.c_int, [C.POINTER(PdlpDev), C.POINTER(PdlpTune), C.c_int32, C.c_int32, _p,
                                           C.POINTER(C.c_void_p)]),
     "pdlp_batch_solve": (C.c_int, [C.POINTER(PdlpBatch), _p]),
+    "pdlp_family_init": (C.c_int, [C.POINTER(PdlpFamily), _p]),
+    "pdlp_family_run": (C.c_int, [C.POINTER(PdlpFamily), C.c_int32, _p]),
+    "pdlp_family_iterate": (C.c_int, [C.POINTER(PdlpFamily), C.c_int32, _p]),
+    "pdlp_family_finish": (C.c_int, [C.POINTER(PdlpFamily), C.c_int32, _p]),
+    "pdlp_family_graph_capture": (C.c_int, [C.POINTER(PdlpFamily), C.c_int32, _p, C.POINTER(C.c_void_p)]),
+    "pdlp_family_graph_capture_step": (C.c_int, [C.POINTER(PdlpFamily), C.c_int32, C.c_int32, _p,
+                                                 C.POINTER(C.c_void_p)]),
     "pdlp_gemm_bf16": (C.c_int, [C.POINTER(PdlpGemm), _p]),
     "pdlp_mlp_fwd_bf16": (C.c_int, [C.POINTER(PdlpMlp), _p]),
     "pdlp_mlp_bwd_bf16": (C.c_int, [C.POINTER(PdlpMlp), _p]),

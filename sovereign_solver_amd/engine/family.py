@@ -1,0 +1,433 @@
+"""Shared-matrix batched PDHG engine ("family"): B LPs with one constraint matrix.
+
+Parametric LP families (the same K, varying costs and right-hand sides) and
+branch-and-bound nodes (the same K, varying bounds) step in lockstep on the
+device (``pdlp_family_*`` in ``include/pdlp_hip.h``).  Every per-LP vector is
+stored in tiles of 64 LPs with the LP index fastest, so each wavefront
+updates one row or column of 64 LPs with coalesced loads; vectors equal for
+all LPs are stored once.  Each LP keeps its own primal weight, restart state
+and termination, exactly as in the persistent batched engine.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+import scipy.sparse as sp
+import torch
+
+from . import _lib as L
+from .batch import OUT_DTYPE, _STATUS
+from .device import dev_tensor, ptr, require_gpu, stream_handle
+from .forms import ROW_EQ, ROW_LE
+from ..problem import Problem
+
+TILE = L.PDLP_FAM_TILE
+MAX_GRAPH_BLOCKS = 4
+
+
+def _canonical(A) -> sp.csr_matrix:
+    A = sp.csr_matrix(A, dtype=np.float64)
+    A.sum_duplicates()
+    A.sort_indices()
+    A.indices = A.indices.astype(np.int32, copy=False)
+    A.indptr = A.indptr.astype(np.int32, copy=False)
+    return A
+
+
+def _same_matrix(A, B) -> bool:
+    if A is B:
+        return True
+    A, B = sp.csr_matrix(A), sp.csr_matrix(B)
+    if A.shape != B.shape or A.nnz != B.nnz:
+        return False
+    A, B = _canonical(A), _canonical(B)
+    return (np.array_equal(A.indptr, B.indptr) and np.array_equal(A.indices, B.indices)
+            and np.array_equal(A.data, B.data))
+
+
+def to_tiles(a: np.ndarray, tiles: int) -> np.ndarray:
+    """[B, L] (LP-major) -> flat tiled layout [tiles, L, 64], padded LPs zero."""
+    B, Ln = a.shape
+    pad = np.zeros((tiles * TILE, Ln), a.dtype)
+    pad[:B] = a
+    return np.ascontiguousarray(pad.reshape(tiles, TILE, Ln).transpose(0, 2, 1)).ravel()
+
+
+def from_tiles(flat: np.ndarray, tiles: int, Ln: int, B: int) -> np.ndarray:
+    """Inverse of :func:`to_tiles`: [B, L]."""
+    return flat.reshape(tiles, Ln, TILE).transpose(0, 2, 1).reshape(tiles * TILE, Ln)[:B]
+
+
+class FamilyBatch:
+    """Host data of B box-form LPs that share K = [A_ub; A_eq]."""
+
+    def __init__(self, problems: Sequence[Problem]):
+        self.problems = list(problems)
+        if not self.problems:
+            raise ValueError("empty family")
+        p0 = self.problems[0]
+        self.m_ub, self.m_eq = p0.n_ineq, p0.n_eq
+        for k, p in enumerate(self.problems):
+            if (p.n_ineq, p.n_eq, p.n_vars) != (self.m_ub, self.m_eq, p0.n_vars):
+                raise ValueError(f"LP {k}: shape differs from LP 0")
+            if not (_same_matrix(p.A_ub, p0.A_ub) and _same_matrix(p.A_eq, p0.A_eq)):
+                raise ValueError(f"LP {k}: constraint matrix differs from LP 0 (use the persistent batch engine)")
+        K = _canonical(sp.vstack([sp.csr_matrix(p0.A_ub), sp.csr_matrix(p0.A_eq)], format="csr"))
+        self.K, self.KT = K, _canonical(K.T.tocsr())
+        m, n = K.shape
+        if m == 0 or n == 0:
+            raise ValueError("the family engine needs m > 0 and n > 0")
+        self.m, self.n, self.count = m, n, len(self.problems)
+        self.tiles = (self.count + TILE - 1) // TILE
+        self.kind = np.concatenate([np.full(self.m_ub, ROW_LE, np.int8), np.full(self.m_eq, ROW_EQ, np.int8)])
+        aK = abs(K)
+        colsum = np.asarray(aK.sum(axis=0)).ravel()
+        rowsum = np.asarray(aK.sum(axis=1)).ravel()
+        self.tau_hat = (0.999 / np.maximum(colsum, 1e-4)).astype(np.float32)
+        self.sigma_hat = (0.999 / np.maximum(rowsum, 1e-4)).astype(np.float32)
+        self.row_metric = np.maximum(rowsum, 1e-4) / 0.999
+
+        P = self.problems
+        c = np.stack([np.asarray(p.c, float) for p in P])
+        b = np.stack([np.concatenate([np.asarray(p.b_ub, float), np.asarray(p.b_eq, float)]) for p in P])
+        lo = np.stack([np.asarray(p.lb, float) for p in P])
+        hi = np.stack([np.asarray(p.ub, float) for p in P])
+        lbf = np.where(np.isfinite(lo), lo, 0.0)
+        b_shift = b - (K @ lbf.T).T
+        span = np.where(np.isfinite(hi), hi - lbf, np.inf)
+        fin = np.isfinite(span)
+        self.norm_b = np.maximum(1.0, np.sqrt(np.sum(b_shift * b_shift, axis=1) +
+                                              np.sum(np.where(fin, span, 0.0) ** 2, axis=1)))
+        self.norm_c = np.maximum(1.0, np.linalg.norm(c, axis=1))
+        # starting primal weight of the adaptive scheme: ||sqrt(sigma_hat) b|| / ||sqrt(tau_hat) c||
+        nb = np.sqrt(np.sum(b * b * (0.999 / np.maximum(rowsum, 1e-4)), axis=1))
+        nc = np.sqrt(np.sum(c * c * (0.999 / np.maximum(colsum, 1e-4)), axis=1))
+        ok = (nb > 1e-12) & (nc > 1e-12)
+        self.init_weight = np.where(ok, np.clip(nb / np.where(ok, nc, 1.0), 1e-6, 1e6), 1.0)
+        self.vec = {"c": c, "b": b, "lo": lo, "hi": hi, "c64": c, "b64": b, "b_shift": b_shift,
+                    "lo_shift": lbf, "ub_span": span}
+        self.shared = 0
+        for name, a in self.vec.items():
+            if bool(np.all(a == a[0])):             # (inf == inf holds)
+                self.shared |= L.PDLP_FAM_SHARED[name]
+        self.lo, self.hi = lo, hi
+
+    def is_shared(self, name: str) -> bool:
+        return bool(self.shared & L.PDLP_FAM_SHARED[name])
+
+    def device_vector(self, name: str) -> np.ndarray:
+        a = self.vec[name]
+        return a[0] if self.is_shared(name) else to_tiles(a, self.tiles)
+
+    def algorithmic_bytes_per_iteration(self) -> float:
+        """Bytes one lockstep iteration of all LPs moves at minimum: the shared CSR
+        of K and K^T once, each per-LP vector once per access (FP32)."""
+        m, n, B = self.m, self.n, self.tiles * TILE
+        z = self.K.nnz
+        csr = 2 * (8 * z) + 4 * (m + 1 + n + 1)
+        per = lambda name, L_: L_ * 4 * (1 if self.is_shared(name) else B)
+        # column pass: y gathered once, c, lo, hi, x r/w, xbar w, x partial sum r/w
+        col = 4 * m * B + per("c", n) + per("lo", n) + per("hi", n) + 5 * 4 * n * B
+        # row pass: xbar gathered once, b, y r/w, y partial sum r/w
+        row = 4 * n * B + per("b", m) + 4 * 4 * m * B
+        return float(csr + col + row + 4 * (n + m))
+
+
+class FamilyEngine:
+    """Device state of a :class:`FamilyBatch` and the lockstep solve loop."""
+
+    def __init__(self, fb: FamilyBatch, scheme: str = "adaptive", check_every: int = 50,
+                 items_per_wave: int = 1, use_graphs: bool = False, compaction: bool = True):
+        flags = {"reference": 0, "adaptive": 1, "adaptive_fixed": 3, "adaptive_start": 5}
+        if scheme not in flags:
+            raise ValueError("scheme must be 'adaptive', 'adaptive_fixed', 'adaptive_start' or 'reference'")
+        self.lib = L.lib()
+        self.device = dv = require_gpu()
+        self.fb = fb
+        self.scheme = scheme
+        self.flags = flags[scheme]
+        self.check_every = check_every
+        self.items_per_wave = items_per_wave
+        self.use_graphs = use_graphs
+        self.compaction = compaction
+        self.max_graph_blocks = MAX_GRAPH_BLOCKS
+        self.stream = torch.cuda.Stream(device=dv)
+        f32, f64, i32 = torch.float32, torch.float64, torch.int32
+        B, Tn = fb.count, fb.tiles * TILE
+        n, m = fb.n, fb.m
+        with torch.cuda.stream(self.stream):
+            t = {
+                "row_ptr": dev_tensor(fb.K.indptr, i32, dv), "col_idx": dev_tensor(fb.K.indices, i32, dv),
+                "val": dev_tensor(fb.K.data, f32, dv),
+                "t_row_ptr": dev_tensor(fb.KT.indptr, i32, dv), "t_col_idx": dev_tensor(fb.KT.indices, i32, dv),
+                "t_val": dev_tensor(fb.KT.data, f32, dv),
+                "row_kind": dev_tensor(fb.kind, torch.int8, dv), "tau_hat": dev_tensor(fb.tau_hat, f32, dv),
+                "sigma_hat": dev_tensor(fb.sigma_hat, f32, dv), "row_metric": dev_tensor(fb.row_metric, f64, dv),
+            }
+            for name, ln in (("x", n), ("xbar", n), ("x_psum", n), ("x_last", n),
+                             ("y", m), ("y_psum", m), ("y_last", m)):
+                t[name] = torch.zeros(Tn * ln, dtype=f32, device=dv)
+            t["x_sum"] = torch.zeros(Tn * n, dtype=f64, device=dv)
+            t["y_sum"] = torch.zeros(Tn * m, dtype=f64, device=dv)
+            t["lp"] = torch.zeros(B * C.sizeof(L.PdlpFamilyLP), dtype=torch.uint8, device=dv)
+            t["out"] = torch.zeros(B * OUT_DTYPE.itemsize, dtype=torch.uint8, device=dv)
+            t["run_mask"] = torch.zeros(fb.tiles, dtype=torch.int64, device=dv)
+            t["running"] = torch.zeros(1, dtype=i32, device=dv)
+            t["partials"] = torch.zeros(2 * L.PDLP_FAM_KKT_CHUNKS * L.PDLP_FAM_NSUM * Tn, dtype=f64, device=dv)
+        self.t = t
+        self._upload_data()
+        self.stream.synchronize()
+        # per-LP tiled arrays that compaction moves (window partial sums and xbar
+        # are zero / rewritten at a check boundary and need not move)
+        self._per_lp = [("x", n), ("x_last", n), ("x_sum", n), ("y", m), ("y_last", m), ("y_sum", m)]
+        self._per_lp += [(name, n if name in ("c", "lo", "hi", "c64", "lo_shift", "ub_span") else m)
+                         for name in self._DATA if not fb.is_shared(name)]
+        self.running_host = torch.zeros((2, 1), dtype=i32).pin_memory()
+        self.mask_host = torch.zeros((2, fb.tiles), dtype=torch.int64).pin_memory()
+        self.compactions = 0
+        self.args = L.PdlpFamily(m=m, n=n, count=B, tiles=fb.tiles, check_every=check_every, flags=self.flags,
+                                 shared=fb.shared, items_per_wave=items_per_wave, tol=0.0)
+        for name, _ in L.PdlpFamily._fields_[9:]:
+            setattr(self.args, name, ptr(t[name]))
+        self._graphs = {}
+
+    _DATA = {"c": torch.float32, "lo": torch.float32, "hi": torch.float32, "b": torch.float32,
+             "c64": torch.float64, "lo_shift": torch.float64, "ub_span": torch.float64, "b64": torch.float64,
+             "b_shift": torch.float64}
+
+    def _upload_data(self) -> None:
+        """Problem vectors in the original slot order (compaction permutes them)."""
+        with torch.cuda.stream(self.stream):
+            for name, dt in self._DATA.items():
+                a = torch.from_numpy(np.ascontiguousarray(self.fb.device_vector(name))).to(self.device, dt)
+                if name in self.t:
+                    self.t[name].copy_(a)
+                else:
+                    self.t[name] = a
+
+    @property
+    def sh(self):
+        return stream_handle(self.stream)
+
+    def drop_graphs(self) -> None:
+        for g in self._graphs.values():
+            self.lib.pdlp_graph_destroy(g)
+        self._graphs = {}
+
+    def __del__(self):
+        try:
+            self.drop_graphs()
+        except Exception:
+            pass
+
+    def _graph(self, blocks: int):
+        g = self._graphs.get(blocks)
+        if g is None:
+            h = C.c_void_p()
+            L.check(self.lib.pdlp_family_graph_capture(C.byref(self.args), blocks, self.sh, C.byref(h)),
+                    "pdlp_family_graph_capture")
+            g = self._graphs[blocks] = h
+        return g
+
+    def set_tol(self, tol: float) -> None:
+        if self.args.tol != tol:
+            self.args.tol = tol
+            self.drop_graphs()          # the graphs hold the argument block by value
+
+    # ----------------------------------------------------------------- solve
+    def start(self, tol: float, x0: Optional[np.ndarray] = None, y0: Optional[np.ndarray] = None,
+              weight: Optional[np.ndarray] = None) -> None:
+        """Starting point ([B, n] / [B, m], None = cold start: x = clip(0, lo, hi),
+        y = 0) and per-LP primal weights (None or NaN entries: 1 for the reference
+        scheme, the norm ratio for the adaptive ones)."""
+        fb = self.fb
+        self.set_tol(tol)
+        if self.compactions or self.args.tiles != fb.tiles:
+            self._upload_data()
+            self.args.tiles, self.args.count = fb.tiles, fb.count
+            self.compactions = 0
+            self.drop_graphs()
+        self.slot_lp = np.concatenate([np.arange(fb.count), np.full(fb.tiles * TILE - fb.count, -1)])
+        self.saved = np.zeros(fb.count, bool)
+        self.X_out = np.zeros((fb.count, fb.n))
+        self.Y_out = np.zeros((fb.count, fb.m))
+        self.out_final = np.zeros(fb.count, OUT_DTYPE)
+        x = np.clip(0.0, fb.lo, fb.hi) if x0 is None else np.clip(np.asarray(x0, float), fb.lo, fb.hi)
+        y = np.zeros((fb.count, fb.m)) if y0 is None else np.array(y0, float)
+        y[:, : fb.m_ub] = np.minimum(y[:, : fb.m_ub], 0.0)      # y <= 0 on the <= rows
+        w = fb.init_weight.copy() if self.flags & 1 else np.ones(fb.count)
+        if weight is not None:
+            ww = np.asarray(weight, float)
+            w = np.where(np.isfinite(ww), ww, w)
+        lp = np.zeros(fb.count, dtype=np.dtype([(n_, "<i4") for n_ in
+                                                ("status", "iter", "window", "restarts", "checks", "done_iter",
+                                                 "last_restart", "action")] +
+                                               [(n_, "<f8") for n_ in ("norm_b", "norm_c", "ref_err", "err_prev",
+                                                                       "w_div")] +
+                                               [("weight", "<f4"), ("pad", "<i4")]))
+        assert lp.dtype.itemsize == C.sizeof(L.PdlpFamilyLP)
+        lp["norm_b"], lp["norm_c"], lp["weight"] = fb.norm_b, fb.norm_c, w
+        dv = self.device
+        with torch.cuda.stream(self.stream):
+            self.t["x"].copy_(torch.from_numpy(to_tiles(x.astype(np.float32), fb.tiles)).to(dv))
+            self.t["y"].copy_(torch.from_numpy(to_tiles(y.astype(np.float32), fb.tiles)).to(dv))
+            self.t["lp"].copy_(torch.from_numpy(lp.view(np.uint8).copy()).to(dv))
+        L.check(self.lib.pdlp_family_init(C.byref(self.args), self.sh), "pdlp_family_init")
+
+    def run(self, max_iters: int) -> int:
+        """Enqueue blocks of check_every iterations until every LP has finished or
+        ``max_iters`` is reached; returns the iterations executed.
+
+        The running-LP count and tile masks come back one block behind, so the
+        GPU always has the next block queued.  When finished LPs leave enough
+        lanes idle (the running LPs would fit in at most 3/4 of the live
+        tiles), the engine waits, saves the finished LPs' results and packs the
+        running LPs into the leading tiles (:meth:`compact`)."""
+        ce = self.check_every
+        done, blocks, slot = 0, 1, 0
+        inflight = []
+        finished = False
+        while True:
+            left = (max_iters - done) // ce
+            if left == 0:
+                break
+            g = min(blocks, left)
+            with torch.cuda.stream(self.stream):
+                if self.use_graphs:
+                    g = 1 << (g.bit_length() - 1)
+                    L.check(self.lib.pdlp_graph_launch(self._graph(g), self.sh), "pdlp_graph_launch")
+                else:
+                    L.check(self.lib.pdlp_family_run(C.byref(self.args), g, self.sh), "pdlp_family_run")
+                self.running_host[slot].copy_(self.t["running"], non_blocking=True)
+                self.mask_host[slot, : self.args.tiles].copy_(self.t["run_mask"][: self.args.tiles],
+                                                              non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(self.stream)
+            done += g * ce
+            inflight.append((slot, ev, self.args.tiles))
+            slot ^= 1
+            blocks = min(blocks * 2, self.max_graph_blocks)
+            if len(inflight) == 2:
+                s0, e0, t0 = inflight.pop(0)
+                e0.synchronize()
+                running = int(self.running_host[s0, 0])
+                if running == 0:
+                    finished = True
+                    break
+                live = int(np.count_nonzero(self.mask_host[s0, :t0].numpy()))
+                need = (running + TILE - 1) // TILE
+                if self.compaction and live >= 2 and need <= 0.75 * live:
+                    self.stream.synchronize()
+                    inflight.clear()
+                    self.compact()
+        tail = 0
+        if not finished:
+            tail = max_iters - done
+            if tail > 0:
+                L.check(self.lib.pdlp_family_iterate(C.byref(self.args), tail, self.sh), "pdlp_family_iterate")
+        L.check(self.lib.pdlp_family_finish(C.byref(self.args), tail, self.sh), "pdlp_family_finish")
+        return done + tail
+
+    # ------------------------------------------------------------ compaction
+    def _slot_index(self, perm: torch.Tensor, ln: int, new_tiles: int) -> torch.Tensor:
+        """Flat source index of every element of the first ``new_tiles`` tiles after
+        the slot permutation ``perm`` (new slot -> old slot)."""
+        base = (perm // TILE) * (ln * TILE) + perm % TILE                     # [new_tiles*64]
+        d = torch.arange(ln, device=perm.device, dtype=torch.int64) * TILE
+        return (base.view(new_tiles, 1, TILE) + d.view(1, ln, 1)).reshape(-1)
+
+    def _save(self, slots: np.ndarray) -> None:
+        """Copy the results of the LPs in ``slots`` (current layout) to the host."""
+        if slots.size == 0:
+            return
+        fb = self.fb
+        lps = self.slot_lp[slots]
+        sl = torch.from_numpy(slots.astype(np.int64)).to(self.device)
+        with torch.cuda.stream(self.stream):
+            for name, ln, dst in (("x_sum", fb.n, self.X_out), ("y_sum", fb.m, self.Y_out)):
+                base = (sl // TILE) * (ln * TILE) + sl % TILE
+                d = torch.arange(ln, device=self.device, dtype=torch.int64) * TILE
+                flat = (base.view(-1, 1) + d.view(1, ln)).reshape(-1)
+                dst[lps] = torch.take(self.t[name], flat).view(-1, ln).cpu().numpy()
+            rec = self.t["out"].view(-1, OUT_DTYPE.itemsize).index_select(0, sl).cpu().numpy()
+        self.out_final[lps] = np.frombuffer(rec.tobytes(), dtype=OUT_DTYPE)
+        self.saved[lps] = True
+
+    def compact(self) -> None:
+        """Save the finished LPs of the live tiles and move the running LPs to the
+        first ceil(running / 64) tiles (stream idle, at a check boundary)."""
+        fb = self.fb
+        T, cnt = self.args.tiles, self.args.count
+        mask = self.t["run_mask"][:T].cpu().numpy().view(np.uint64)
+        bits = ((mask[:, None] >> np.arange(TILE, dtype=np.uint64)[None, :]) & np.uint64(1)).astype(bool).ravel()
+        slots = np.arange(T * TILE)
+        valid = slots < cnt
+        run = bits & valid
+        done_slots = slots[valid & ~run & ~self.saved[np.maximum(self.slot_lp, 0)]]
+        self._save(done_slots)
+        keep = slots[run]
+        R = int(keep.size)
+        new_T = (R + TILE - 1) // TILE
+        perm_np = np.concatenate([keep, np.full(new_T * TILE - R, keep[0] if R else 0)]).astype(np.int64)
+        perm = torch.from_numpy(perm_np).to(self.device)
+        with torch.cuda.stream(self.stream):
+            idx = {fb.n: self._slot_index(perm, fb.n, new_T), fb.m: self._slot_index(perm, fb.m, new_T)}
+            for name, ln in self._per_lp:
+                t = self.t[name]
+                moved = torch.take(t, idx[ln])
+                t[: moved.numel()].copy_(moved)
+            for name, size in (("lp", C.sizeof(L.PdlpFamilyLP)), ("out", OUT_DTYPE.itemsize)):
+                v = self.t[name].view(-1, size)
+                moved = v.index_select(0, perm[:R])
+                v[:R].copy_(moved)
+            new_mask = np.zeros(new_T, np.uint64)
+            for t_ in range(new_T):
+                lanes = min(TILE, R - t_ * TILE)
+                new_mask[t_] = np.uint64((1 << lanes) - 1) if lanes < 64 else np.uint64(0xFFFFFFFFFFFFFFFF)
+            self.t["run_mask"][:new_T].copy_(torch.from_numpy(new_mask.view(np.int64)).to(self.device))
+        self.stream.synchronize()
+        self.slot_lp = np.concatenate([self.slot_lp[perm_np[:R]], np.full(new_T * TILE - R, -1)])
+        self.args.tiles = new_T
+        self.args.count = R
+        self.compactions += 1
+        self.drop_graphs()
+
+    def results(self):
+        """(out records, X [B, n] FP64 solutions, Y [B, m]) by LP."""
+        self.stream.synchronize()
+        cnt = self.args.count
+        live = np.arange(cnt)[~self.saved[self.slot_lp[:cnt]]]
+        self._save(live)
+        self.stream.synchronize()
+        return self.out_final.copy(), self.X_out.copy(), self.Y_out.copy()
+
+    def solve(self, tol: float, max_iters: int, x0=None, y0=None, weight=None):
+        self.start(tol, x0, y0, weight)
+        self.run(max_iters)
+        return self.results()
+
+    def unpack(self, out, X, Y):
+        return [(_STATUS[int(out["status"][k])], X[k], Y[k], out[k]) for k in range(self.fb.count)]
+
+    # ---------------------------------------------------------------- timing
+    def time_step(self, which: str, reps: int = 20) -> float:
+        """Mean µs per launch of the column (``"primal"``) or row (``"dual"``) pass at
+        the current state: a HIP graph of ``reps`` launches timed with events on
+        the engine stream (the state keeps moving; the work per launch does not)."""
+        h = C.c_void_p()
+        L.check(self.lib.pdlp_family_graph_capture_step(C.byref(self.args), {"primal": 0, "dual": 1}[which], reps,
+                                                        self.sh, C.byref(h)), "pdlp_family_graph_capture_step")
+        try:
+            with torch.cuda.stream(self.stream):
+                L.check(self.lib.pdlp_graph_launch(h, self.sh), "warm-up launch")
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(self.stream)
+                L.check(self.lib.pdlp_graph_launch(h, self.sh), "timed launch")
+                e1.record(self.stream)
+            e1.synchronize()
+            return e0.elapsed_time(e1) * 1e3 / reps
+        finally:
+            self.lib.pdlp_graph_destroy(h)

@@ -8,6 +8,13 @@ same fields and semantics as :func:`solve_lp_pdhg` (box form, FP32 iterates,
 FP64 KKT, check cadence 50; ``scheme="adaptive"`` (default) or
 ``"reference"`` restarts as in :func:`solve_lp_pdhg`).
 
+``engine``: ``"persistent"`` (one workgroup per LP, any mix of matrices; an
+LP needs 8(m+n) bytes of LDS), ``"family"`` (all LPs share one constraint
+matrix and step in lockstep, 64 LPs per wavefront; no size limit) or
+``"auto"`` (family when every problem holds the same ``A_ub``/``A_eq``
+objects and there are at least 16 of them, or when an LP is too large for
+LDS; else persistent).
+
 ``warm_start`` is an optional list (one entry per problem, ``None`` allowed)
 of ``(x0, y0)`` or ``(x0, y0, step_scale)``: x0 in the original variable
 space, y0 the duals of ``[A_ub; A_eq]`` (y ≤ 0 on the ``<=`` rows), and a
@@ -22,6 +29,7 @@ import numpy as np
 
 from ..config import MAX_PDHG_ITERS
 from ..engine.batch import BatchEngine, PackedBatch
+from ..engine.family import FamilyBatch, FamilyEngine
 from ..problem import Problem
 from ..result import SolveResult
 
@@ -48,10 +56,31 @@ def pack_warm_start(pk: PackedBatch, warm_start):
     return x0, y0, w
 
 
-def results_of(engine: BatchEngine, out, xs, ys) -> list:
+def family_warm_start(fb: FamilyBatch, warm_start):
+    """Per-LP warm starts as [B, n] / [B, m] arrays and weights (NaN = default)."""
+    if warm_start is None:
+        return None, None, None
+    X = np.clip(0.0, fb.lo, fb.hi)
+    Y = np.zeros((fb.count, fb.m))
+    w = np.full(fb.count, np.nan)
+    for k, ws in enumerate(warm_start):
+        if ws is None:
+            continue
+        if ws[0] is not None:
+            X[k] = np.asarray(ws[0], float)
+        if ws[1] is not None:
+            Y[k] = np.asarray(ws[1], float)
+        if len(ws) > 2 and ws[2] is not None:
+            w[k] = float(ws[2])
+    return X, Y, w
+
+
+def results_of(engine, out, xs, ys) -> list:
+    """SolveResults of a persistent (:class:`BatchEngine`) or family engine run."""
+    problems = engine.pk.problems if isinstance(engine, BatchEngine) else engine.fb.problems
     res = []
     for k, (status, x, y, rec) in enumerate(engine.unpack(out, xs, ys)):
-        p = engine.pk.problems[k]
+        p = problems[k]
         x = np.clip(x, np.where(np.isfinite(p.lb), p.lb, -1e30), np.where(np.isfinite(p.ub), p.ub, 1e30))
         obj = float(np.asarray(p.c, float) @ x)
         if p.sense == "max":
@@ -72,7 +101,10 @@ def solve_lp_pdhg_batch(
     check_every: int = 50,
     block: int = 0,
     scheme: str = "adaptive",
+    engine: str = "auto",
 ) -> list:
+    if engine not in ("auto", "persistent", "family"):
+        raise ValueError("engine must be 'auto', 'persistent' or 'family'")
     problems = list(problems)
     if not problems:
         return []
@@ -81,7 +113,20 @@ def solve_lp_pdhg_batch(
     out = [None] * len(problems)
     for k in trivial:                                    # pdhg.py:171-172
         out[k] = SolveResult("optimal", np.zeros(problems[k].n_vars), 0.0, 0)
-    if live:
+    if live and engine == "auto":
+        ps = [problems[k] for k in live]
+        same = all(p.A_ub is ps[0].A_ub and p.A_eq is ps[0].A_eq for p in ps)
+        too_big = max(8 * (p.n_constraints + p.n_vars) for p in ps) > 160 * 1024
+        engine = "family" if (same and len(ps) >= 16) or (too_big and same) else "persistent"
+    if live and engine == "family":
+        fb = FamilyBatch([problems[k] for k in live])
+        eng = FamilyEngine(fb, scheme=scheme, check_every=check_every)
+        ws = None if warm_start is None else [warm_start[k] for k in live]
+        X, Y, w = family_warm_start(fb, ws)
+        o, xs, ys = eng.solve(tol, max_iters, X, Y, w)
+        for k, r in zip(live, results_of(eng, o, xs, ys)):
+            out[k] = r
+    elif live:
         pk = PackedBatch([problems[k] for k in live])
         eng = BatchEngine(pk, block=block, scheme=scheme)
         ws = None if warm_start is None else [warm_start[k] for k in live]

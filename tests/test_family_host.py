@@ -1,0 +1,103 @@
+"""Host side of the shared-matrix (family) engine: tiled layout, shared-vector
+detection, per-LP normalisers and the reference-form yardstick data (no GPU)."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import yardstick
+from sovereign_solver_amd.engine import _lib as L
+from sovereign_solver_amd.engine.family import FamilyBatch, from_tiles, to_tiles
+from sovereign_solver_amd.engine.forms import box_form
+from sovereign_solver_amd.problem import Problem
+from sovereign_solver_amd.workloads import DispatchFamily, SetCoverNodes, random_sparse_lp
+
+
+def test_tiles_round_trip_and_address_formula():
+    rng = np.random.default_rng(0)
+    for B, Ln in ((1, 3), (64, 5), (65, 7), (200, 2)):
+        a = rng.standard_normal((B, Ln))
+        tiles = (B + 63) // 64
+        flat = to_tiles(a, tiles)
+        assert flat.size == tiles * 64 * Ln
+        np.testing.assert_array_equal(from_tiles(flat, tiles, Ln, B), a)
+        # element d of LP k sits at ((k // 64) * L + d) * 64 + k % 64
+        for k in (0, B - 1, B // 2):
+            for d in (0, Ln - 1):
+                assert flat[((k // 64) * Ln + d) * 64 + k % 64] == a[k, d]
+
+
+def test_dispatch_family_shares_bounds_not_costs():
+    fam = DispatchFamily(G=10, T=4)
+    fb = FamilyBatch([fam.problem(k) for k in range(5)])
+    assert fb.is_shared("lo") and fb.is_shared("hi") and fb.is_shared("lo_shift") and fb.is_shared("ub_span")
+    assert not fb.is_shared("c") and not fb.is_shared("b") and not fb.is_shared("b_shift")
+    assert fb.tiles == 1 and fb.K.shape == (fam.m, fam.n)
+
+
+def test_setcover_nodes_share_costs_not_bounds():
+    sc = SetCoverNodes(rows=60, cols=300, density=0.03)
+    fb = FamilyBatch([sc.node(k) for k in range(4)])
+    assert fb.is_shared("c") and fb.is_shared("b") and fb.is_shared("c64")
+    assert not fb.is_shared("lo") and not fb.is_shared("hi") and not fb.is_shared("b_shift")
+
+
+def test_normalisers_match_single_box_form_and_yardstick():
+    """Per-LP norm_b / norm_c equal the single engine's box form, whose KKT data
+    the yardstick pins; step sizes are the Pock-Chambolle diagonal ones."""
+    sc = SetCoverNodes(rows=60, cols=300, density=0.03)
+    probs = [sc.node(k) for k in range(3)]
+    fb = FamilyBatch(probs)
+    for k, p in enumerate(probs):
+        f = box_form(p)
+        assert fb.norm_b[k] == pytest.approx(f.norm_b, rel=1e-14)
+        assert fb.norm_c[k] == pytest.approx(f.norm_c, rel=1e-14)
+        np.testing.assert_allclose(fb.vec["b_shift"][k], f.b_shift, rtol=0, atol=1e-12)
+        # the yardstick normaliser is the same quantity
+        x = np.clip(0.0, p.lb, p.ub)
+        y = np.zeros(p.n_constraints)
+        assert np.isfinite(yardstick.rel_kkt(p, x, y)[0])
+    colsum = np.asarray(abs(fb.K).sum(axis=0)).ravel()
+    np.testing.assert_allclose(fb.tau_hat, 0.999 / np.maximum(colsum, 1e-4), rtol=1e-7)
+
+
+def test_rejects_different_matrices():
+    a, b = random_sparse_lp(20, 40, 0.1, seed=1), random_sparse_lp(20, 40, 0.1, seed=2)
+    with pytest.raises(ValueError, match="matrix differs"):
+        FamilyBatch([a, b])
+
+
+def test_equal_but_distinct_matrix_objects_accepted():
+    p = random_sparse_lp(20, 40, 0.1, seed=1)
+    q = Problem(c=p.c * 2, A_ub=p.A_ub.copy(), b_ub=p.b_ub, A_eq=p.A_eq.copy(), b_eq=p.b_eq, lb=p.lb, ub=p.ub,
+                integer_mask=p.integer_mask)
+    fb = FamilyBatch([p, q])
+    assert fb.count == 2 and not fb.is_shared("c")
+
+
+def test_struct_layouts():
+    import ctypes as C
+    assert C.sizeof(L.PdlpFamilyLP) == 8 * 4 + 5 * 8 + 4 + 4
+    assert C.sizeof(L.PdlpFamily) == 8 * 4 + 8 + 33 * 8
+    lib = L.lib()
+    assert lib.pdlp_family_init(None, None) == -1                               # PDLP_E_ARG
+    assert b"NULL" in lib.pdlp_last_error()
+    f = L.PdlpFamily(m=3, n=4, count=65, tiles=1, check_every=50, items_per_wave=4)
+    assert lib.pdlp_family_run(C.byref(f), 1, None) == -3                    # PDLP_E_SHAPE: tiles
+
+
+def test_compaction_index_moves_slots():
+    """The flat gather index of a slot permutation maps tile layout to tile layout."""
+    import torch
+    from sovereign_solver_amd.engine.family import FamilyEngine
+    rng = np.random.default_rng(3)
+    B, Ln = 150, 7
+    tiles = 3
+    a = rng.standard_normal((B, Ln))
+    flat = torch.from_numpy(to_tiles(a, tiles))
+    keep = np.sort(rng.choice(B, size=70, replace=False))
+    new_T = 2
+    perm = np.concatenate([keep, np.full(new_T * 64 - keep.size, keep[0])])
+    idx = FamilyEngine._slot_index(None, torch.from_numpy(perm), Ln, new_T)
+    moved = torch.take(flat, idx).numpy()
+    np.testing.assert_array_equal(from_tiles(moved, new_T, Ln, keep.size), a[keep])

@@ -1,0 +1,100 @@
+"""GPU parity of the shared-matrix (family) engine (needs an MI355X).
+
+Every LP must stop at the requested relative KKT as measured on the host by
+the reference formula (oracle yardstick), agree with HiGHS (FP64 simplex) on
+the objective, and run the same algorithm as the persistent batched engine
+(iteration counts within a few checks: the summation orders differ).
+"""
+
+import numpy as np
+import pytest
+from scipy.optimize import linprog
+
+from oracle import yardstick
+from sovereign_solver_amd import solve_lp_pdhg_batch
+from sovereign_solver_amd.engine.family import FamilyBatch, FamilyEngine
+from sovereign_solver_amd.workloads import DispatchFamily, SetCoverNodes
+
+pytestmark = pytest.mark.gpu
+
+
+def _highs(p):
+    r = linprog(p.c, A_ub=p.A_ub if p.n_ineq else None, b_ub=p.b_ub if p.n_ineq else None,
+                A_eq=p.A_eq if p.n_eq else None, b_eq=p.b_eq if p.n_eq else None,
+                bounds=list(zip(p.lb, [None if np.isinf(u) else u for u in p.ub])), method="highs-ds")
+    assert r.status == 0
+    return r.fun
+
+
+def _check(probs, res, tol=1e-4, n_highs=6, obj_tol=1e-3):
+    for p, r in zip(probs, res):
+        assert r.status == "optimal", (p.name, r.status, r.iterations)
+        err = yardstick.rel_kkt(p, r.x, r.y)[0]
+        assert err <= 1.02 * tol, (p.name, err, r.rel_kkt)
+        assert np.all(r.x >= p.lb - 1e-9) and np.all(r.x <= p.ub + 1e-9)
+    for p, r in list(zip(probs, res))[:n_highs]:
+        ref = _highs(p)
+        assert abs(r.objective - ref) <= obj_tol * (1 + abs(ref)), (p.name, r.objective, ref)
+
+
+@pytest.mark.parametrize("scheme", ["adaptive", "reference"])
+def test_family_dispatch_small(gpu, scheme):
+    fam = DispatchFamily(G=20, T=6)
+    probs = [fam.problem(k) for k in range(70)]                  # two tiles, the second partial
+    res = solve_lp_pdhg_batch(probs, tol=1e-4, max_iters=100_000, engine="family", scheme=scheme)
+    _check(probs, res)
+
+
+def test_family_setcover_nodes(gpu):
+    sc = SetCoverNodes(rows=100, cols=1000, density=0.01)
+    probs = [sc.node(k) for k in range(40)]
+    res = solve_lp_pdhg_batch(probs, tol=1e-4, max_iters=100_000, engine="family")
+    _check(probs, res)
+
+
+def test_family_agrees_with_persistent_engine(gpu):
+    fam = DispatchFamily(G=30, T=8)
+    probs = [fam.problem(k) for k in range(20)]
+    rf = solve_lp_pdhg_batch(probs, tol=1e-4, engine="family")
+    rp = solve_lp_pdhg_batch(probs, tol=1e-4, engine="persistent")
+    for f, p in zip(rf, rp):
+        assert f.status == p.status == "optimal"
+        assert abs(f.objective - p.objective) <= 1e-3 * (1 + abs(p.objective))
+        assert abs(f.iterations - p.iterations) <= max(200, 0.25 * p.iterations), (f.iterations, p.iterations)
+
+
+def test_family_launch_path_and_compaction_agree_bitwise(gpu):
+    # no atomics and per-lane arithmetic only: neither the launch path (graphs or
+    # direct launches) nor moving LPs between tiles changes any LP's result
+    fam = DispatchFamily(G=12, T=5)
+    fb = FamilyBatch([fam.problem(k) for k in range(150)])
+    runs = []
+    for graphs, compact in ((False, False), (True, False), (False, True)):
+        eng = FamilyEngine(fb, use_graphs=graphs, compaction=compact)
+        runs.append(eng.solve(1e-4, 100_000))
+        if compact:
+            assert eng.compactions >= 1
+    a = runs[0]
+    for b in runs[1:]:
+        assert np.array_equal(a[0]["iterations"], b[0]["iterations"])
+        assert np.array_equal(a[0]["status"], b[0]["status"])
+        assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+
+
+def test_family_iteration_limit_tail(gpu):
+    fam = DispatchFamily(G=12, T=5)
+    probs = [fam.problem(k) for k in range(5)]
+    res = solve_lp_pdhg_batch(probs, tol=1e-12, max_iters=137, engine="family")
+    for p, r in zip(probs, res):
+        assert r.status == "iteration_limit" and r.iterations == 137
+        assert np.all(np.isfinite(r.x)) and np.all(r.x >= p.lb) and np.all(r.x <= p.ub)
+
+
+def test_family_warm_start_from_solution(gpu):
+    sc = SetCoverNodes(rows=80, cols=600, density=0.015)
+    probs = [sc.node(k) for k in range(16)]
+    cold = solve_lp_pdhg_batch(probs, tol=1e-4, engine="family")
+    warm = solve_lp_pdhg_batch(probs, warm_start=[(r.x, r.y, r.step_scale) for r in cold], tol=1e-4,
+                               engine="family")
+    for c, w in zip(cold, warm):
+        assert w.status == "optimal" and w.iterations <= max(50, c.iterations // 2)
