@@ -22,6 +22,8 @@ from __future__ import annotations
 
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import time
@@ -228,13 +230,69 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_learned(args):
-    """Learned warm start on the economic-dispatch family (200 generators x 24 periods)."""
-    import numpy as np
+def objective_errors(probs, results, k_max: int = 8):
+    """max |obj - obj*| / (1 + |obj*|) over the first k_max LPs, obj* by HiGHS dual simplex (FP64)."""
+    from scipy.optimize import linprog
+    errs = []
+    for p, r in list(zip(probs, results))[:k_max]:
+        h = linprog(p.c, A_ub=p.A_ub if p.n_ineq else None, b_ub=p.b_ub if p.n_ineq else None,
+                    A_eq=p.A_eq if p.n_eq else None, b_eq=p.b_eq if p.n_eq else None,
+                    bounds=list(zip(p.lb, [None if np.isinf(u) else u for u in p.ub])), method="highs-ds")
+        if h.status == 0:
+            errs.append(abs(r.objective - h.fun) / (1 + abs(h.fun)))
+    return max(errs) if errs else None
+
+
+def plain_rate(probs, tol, max_iters):
+    """Plain single-instance GPU PDHG (cold start, one LP after another): LPs/s."""
     import torch
     from sovereign_solver_amd.engine.forms import box_form
     from sovereign_solver_amd.engine.single import SingleEngine
-    from sovereign_solver_amd.learned import dispatch as D
+    t, ok = 0.0, 0
+    for p in probs:
+        se = SingleEngine(box_form(p), "fp32")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = se.solve(tol, max_iters)
+        t += time.perf_counter() - t0
+        ok += r["status"] == "optimal"
+        se.drop_graphs()
+    return ok / t if t > 0 else 0.0
+
+
+def cpu_rate(probs, tol, cap_s):
+    """The FP64 CPU oracle (reference-form PDHG, 1 core) over LPs of the sample until cap_s."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    from oracle import pdhg_ref
+    try:
+        from threadpoolctl import threadpool_limits
+        threadpool_limits(1)
+    except Exception:                        # pragma: no cover
+        pass
+    t0 = time.perf_counter()
+    done = 0
+    for p in probs:
+        pdhg_ref.solve(p, max_iters=100_000, tol=tol)
+        done += 1
+        if time.perf_counter() - t0 > cap_s:
+            break
+    tc = time.perf_counter() - t0
+    return {"value": done / tc, "unit": "LPs/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle FP64 reference-form PDHG (restatement of solver/lp/pdhg.py) on {done} LPs of the "
+                       f"same workload, solved to {tol:g} one after another in {tc:.1f}s on 1 core (BLAS threads 1)")}
+
+
+def family_rate(res, secs):
+    ok = sum(r.status == "optimal" for r in res)
+    its = np.array([r.iterations for r in res])
+    return {"lps_per_s": ok / secs, "s": secs, "ok": ok, "iters_mean": float(its.mean()), "iters_max": int(its.max())}
+
+
+def bench_learned(args):
+    """Learned warm start on the economic-dispatch family (200 generators x 24 periods)."""
+    import torch
+    from sovereign_solver_amd.engine.batch import BatchEngine, PackedBatch
+    from sovereign_solver_amd.learned import warmstart as W
     from sovereign_solver_amd.workloads import DispatchFamily
 
     fam = DispatchFamily()
@@ -242,85 +300,137 @@ def bench_learned(args):
     train_ids = np.arange(args.train)
     eval_ids = 10**6 + np.arange(args.eval)
     t0 = time.perf_counter()
-    lab = D.label(fam, train_ids, tol=args.tol, max_iters=args.max_iters)
+    lab = W.label(fam, train_ids, tol=args.tol, max_iters=args.max_iters)
     t_label = time.perf_counter() - t0
-    ok = int((lab.status == "optimal").sum())
-    log(f"labelled {len(train_ids)} instances in {t_label:.1f}s (engine {lab.seconds:.1f}s), optimal {ok}, "
-        f"iterations mean {lab.iterations.mean():.0f}")
     keep = lab.status == "optimal"
-    feats = D.features_of(fam, train_ids)[keep]
-    targets = D.targets_of(lab)[keep]
+    log(f"labelled {len(train_ids)} instances in {t_label:.1f}s (engine {lab.seconds:.1f}s), optimal {int(keep.sum())}, "
+        f"iterations mean {lab.iterations.mean():.0f}")
     t1 = time.perf_counter()
-    mlp, nin, nout, losses = D.train(feats, targets, d_h=512, epochs=args.epochs, batch=256, lr=1e-3, log=log,
-                                     out_blocks=D.output_blocks(fam))
+    mlp, nin, nout, losses = W.train(fam.features_of(train_ids)[keep], W.targets_of(lab)[keep], d_h=512,
+                                     epochs=args.epochs, batch=256, lr=1e-3, out_blocks=W.output_blocks(fam))
     t_train = time.perf_counter() - t1
     log(f"trained in {t_train:.1f}s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    ls = W.LearnedStart(mlp, nin, nout)
+    probs = [fam.problem(int(k)) for k in eval_ids]
 
-    # held-out: cold vs learned warm start, same batched engine
+    W.solve_family(fam, eval_ids[:64], args.tol, args.max_iters)                      # warm-up (build, graphs)
+    cold = family_rate(*W.solve_family(fam, eval_ids, args.tol, args.max_iters))
+    res_x, t_x = W.solve_learned(fam, eval_ids, ls, args.tol, args.max_iters, use_y=False)
+    learned = family_rate(res_x, t_x)
+    learned_full = family_rate(*W.solve_learned(fam, eval_ids, ls, args.tol, args.max_iters))
+    lab_ev = W.label(fam, eval_ids, tol=args.tol, max_iters=args.max_iters)
+    perfect = family_rate(*W.solve_family(fam, eval_ids, args.tol, args.max_iters,
+                                          warm=[(lab_ev.x[k], lab_ev.y[k], float(np.exp(lab_ev.log_step[k])))
+                                                for k in range(len(eval_ids))]))
+    log(f"cold {cold}; learned x+weight {learned}; learned x+y+weight {learned_full}; solution as start {perfect}")
+    pk = PackedBatch(probs)
+    be = BatchEngine(pk)
+    be.set_warm_start(None, None, None)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    warm = D.predict_warm_start(mlp, nin, nout, fam, eval_ids, use_step=not args.no_step)
-    t_pred = time.perf_counter() - t2
-    ev = D.evaluate(fam, eval_ids, warm, tol=args.tol, max_iters=args.max_iters)
-    # the same under the reference restart scheme, and without the predicted step
-    ev_ref = D.evaluate(fam, eval_ids, warm, tol=args.tol, max_iters=args.max_iters, scheme="reference")
-    warm_nostep = [(x0, y0, None) for x0, y0, _ in warm]
-    _, t_ns = D.solve_family(fam, eval_ids, args.tol, args.max_iters, warm=warm_nostep)
-    it_ns = np.array([r.iterations for r in _])
-    partial = {}
-    for name, ws in (("x_only", [(x0, None, s) for x0, _, s in warm]), ("y_only", [(None, y0, s) for _, y0, s in warm])):
-        rr, tt = D.solve_family(fam, eval_ids, args.tol, args.max_iters, warm=ws)
-        partial[name] = {"iters_mean": float(np.mean([r.iterations for r in rr])), "s": tt}
-    log(f"partial warm starts (adaptive): {partial}")
-    log(f"reference scheme: cold iters {ev_ref['cold_iters'].mean():.0f} ({ev_ref['cold_s']:.2f}s), warm iters "
-        f"{ev_ref['warm_iters'].mean():.0f} ({ev_ref['warm_s']:.2f}s); adaptive warm without predicted step: "
-        f"iters {it_ns.mean():.0f} ({t_ns:.2f}s)")
-    n_eval = len(eval_ids)
-    cold_lps = ev["cold_ok"] / ev["cold_s"]
-    warm_lps = ev["warm_ok"] / (ev["warm_s"] + t_pred)
-    log(f"held-out {n_eval}: cold {ev['cold_ok']} ok, iters mean {ev['cold_iters'].mean():.0f}, {ev['cold_s']:.2f}s "
-        f"({cold_lps:.1f} LPs/s); warm {ev['warm_ok']} ok, iters mean {ev['warm_iters'].mean():.0f}, "
-        f"{ev['warm_s']:.2f}s + predict {t_pred:.2f}s ({warm_lps:.1f} LPs/s)")
-    # objective error of the warm results against HiGHS dual simplex on a sample
-    from scipy.optimize import linprog
-    errs = []
-    for k in range(min(8, n_eval)):
-        p = fam.problem(int(eval_ids[k]))
-        h = linprog(p.c, A_ub=p.A_ub, b_ub=p.b_ub, bounds=list(zip(p.lb, p.ub)), method="highs-ds")
-        errs.append(abs(ev["warm"][k].objective - h.fun) / (1 + abs(h.fun)))
-    # plain single-instance GPU PDHG on a sample of the held-out set
-    t_plain, ok_plain = 0.0, 0
-    for k in eval_ids[: args.plain_sample]:
-        se = SingleEngine(box_form(fam.problem(int(k))), "fp32")
-        torch.cuda.synchronize()
-        t3 = time.perf_counter()
-        r = se.solve(args.tol, args.max_iters)
-        t_plain += time.perf_counter() - t3
-        ok_plain += r["status"] == "optimal"
-        se.drop_graphs()
-    plain = ok_plain / t_plain
-    log(f"plain single-instance GPU PDHG: {ok_plain}/{args.plain_sample} optimal, {plain:.2f} LPs/s; "
-        f"objective rel err vs HiGHS (warm, 8 LPs) max {max(errs):.2e}")
+    be.launch(args.tol, args.max_iters)
+    o, _, _ = be.results()
+    persistent = {"lps_per_s": int((o["status"] == 1).sum()) / (time.perf_counter() - t2),
+                  "iters_mean": float(o["iterations"].mean()), "iters_max": int(o["iterations"].max())}
+    del be
+    plain = plain_rate(probs[: args.plain_sample], args.tol, args.max_iters)
+    obj_err = objective_errors(probs, res_x)
+    log(f"persistent cold {persistent}; plain single-instance {plain:.2f} LPs/s; objective rel err vs HiGHS {obj_err}")
+    cpu = None if args.no_cpu_baseline else cpu_rate(probs, args.tol, args.cpu_cap_s)
+    value = learned["lps_per_s"]
     out = {
         "metric": "LPs/sec to 1e-4 rel KKT: learned+batched vs plain PDHG, 1 GPU; SpMV HBM GB/s",
-        "value": warm_lps, "unit": "LPs/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-        "ms_per_step": (ev["warm_s"] + t_pred) * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32 (MLP bf16 x bf16 -> fp32)", "data": "synthetic dispatch family",
+        "value": value, "unit": "LPs/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": learned["s"] * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 iterates, fp64 KKT (MLP bf16 x bf16 -> fp32)", "data": "synthetic dispatch family",
         "config": {"workload": f"learned warm start, economic dispatch 200x24 (m={fam.m}, n={fam.n}), "
-                               f"{args.train} PDHG-labelled training instances, {n_eval} held-out, batched",
+                               f"{args.train} PDHG-labelled training instances, {len(eval_ids)} held-out, "
+                               f"shared-matrix batched engine; value = learned (x + primal weight) start, "
+                               f"network inference and host<->device transfers included",
                    "train_label_s": t_label, "train_s": t_train, "epochs": args.epochs,
                    "loss_first_last": [losses[0], losses[-1]],
-                   "cold_batched_lps_per_s": cold_lps, "warm_batched_lps_per_s": warm_lps,
-                   "cold_iters_mean": float(ev["cold_iters"].mean()), "warm_iters_mean": float(ev["warm_iters"].mean()),
-                   "cold_ok": int(ev["cold_ok"]), "warm_ok": int(ev["warm_ok"]),
+                   "cold_batched": cold, "learned_x_weight": learned, "learned_x_y_weight": learned_full,
+                   "solution_as_start": perfect, "persistent_engine_cold": persistent,
                    "plain_single_gpu_lps_per_s": plain, "plain_sample": args.plain_sample,
-                   "speedup_vs_plain": warm_lps / plain if plain > 0 else None,
-                   "objective_rel_err_vs_highs_max": max(errs), "use_step": not args.no_step,
-                   "reference_scheme": {"cold_iters_mean": float(ev_ref["cold_iters"].mean()),
-                                        "warm_iters_mean": float(ev_ref["warm_iters"].mean()),
-                                        "cold_s": ev_ref["cold_s"], "warm_s": ev_ref["warm_s"]},
-                   "warm_without_step_iters_mean": float(it_ns.mean()), "partial_warm": partial},
-        "roofline": None, "cpu_baseline": None,
+                   "speedup_vs_plain": value / plain if plain > 0 else None,
+                   "learned_vs_cold_batched": value / cold["lps_per_s"],
+                   "objective_rel_err_vs_highs_max": obj_err},
+        "roofline": None, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_nodes(args):
+    """Branch-and-bound node relaxations of a set-covering MILP (1000 x 10000), batched."""
+    import torch
+    from sovereign_solver_amd import solve_lp_pdhg
+    from sovereign_solver_amd.engine.batch import BatchEngine, PackedBatch
+    from sovereign_solver_amd.learned import warmstart as W
+    from sovereign_solver_amd.workloads import SetCoverNodes
+
+    sc = SetCoverNodes()
+    log(f"set cover: rows={sc.rows} cols={sc.cols} nnz={sc.A.nnz}")
+    root = solve_lp_pdhg(sc.root(), tol=args.tol, precision="fp32")
+    log(f"root LP: {root.status} it={root.iterations} obj={root.objective:.4f} weight={root.step_scale:.3g}")
+    node_ids = np.arange(args.nodes)
+    train_ids = 10**6 + np.arange(args.train)
+    probs = [sc.node(int(k)) for k in node_ids]
+    W.solve_family(sc, node_ids[:64], args.tol, args.max_iters)                       # warm-up
+    cold = family_rate(*W.solve_family(sc, node_ids, args.tol, args.max_iters))
+    parent_ws = [(np.clip(root.x, p.lb, p.ub), root.y, root.step_scale) for p in probs]
+    res_p, t_p = W.solve_family(sc, node_ids, args.tol, args.max_iters, warm=parent_ws)
+    parent = family_rate(res_p, t_p)
+    log(f"cold {cold}; parent warm start {parent}")
+    learned = learned_full = None
+    t_label = t_train = 0.0
+    losses = [None]
+    if args.train > 0:
+        t0 = time.perf_counter()
+        lab = W.label(sc, train_ids, tol=args.tol, max_iters=args.max_iters)
+        t_label = time.perf_counter() - t0
+        keep = lab.status == "optimal"
+        t1 = time.perf_counter()
+        mlp, nin, nout, losses = W.train(sc.features_of(train_ids)[keep], W.targets_of(lab)[keep], d_h=512,
+                                         epochs=args.epochs, batch=256, lr=1e-3, out_blocks=W.output_blocks(sc),
+                                         normalise_inputs=False)
+        t_train = time.perf_counter() - t1
+        log(f"labelled {len(train_ids)} nodes in {t_label:.1f}s, trained in {t_train:.1f}s, "
+            f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+        ls = W.LearnedStart(mlp, nin, nout)
+        learned = family_rate(*W.solve_learned(sc, node_ids, ls, args.tol, args.max_iters, use_y=False))
+        learned_full = family_rate(*W.solve_learned(sc, node_ids, ls, args.tol, args.max_iters))
+        log(f"learned x+weight {learned}; learned x+y+weight {learned_full}")
+    pk = PackedBatch(probs)
+    be = BatchEngine(pk)
+    be.set_warm_start(None, None, None)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    be.launch(args.tol, args.max_iters)
+    o, _, _ = be.results()
+    persistent = {"lps_per_s": int((o["status"] == 1).sum()) / (time.perf_counter() - t2),
+                  "iters_mean": float(o["iterations"].mean()), "iters_max": int(o["iterations"].max())}
+    del be
+    plain = plain_rate(probs[: args.plain_sample], args.tol, args.max_iters)
+    obj_err = objective_errors(probs, res_p)
+    log(f"persistent cold {persistent}; plain single-instance {plain:.2f} LPs/s; objective rel err vs HiGHS {obj_err}")
+    cpu = None if args.no_cpu_baseline else cpu_rate(probs, args.tol, args.cpu_cap_s)
+    value = parent["lps_per_s"]
+    out = {
+        "metric": "LPs/sec to 1e-4 rel KKT: learned+batched vs plain PDHG, 1 GPU; SpMV HBM GB/s",
+        "value": value, "unit": "LPs/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": parent["s"] * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 iterates, fp64 KKT", "data": "synthetic set-cover MILP nodes",
+        "config": {"workload": f"branch-and-bound node relaxations, set cover {sc.rows}x{sc.cols} "
+                               f"(nnz {sc.A.nnz}), {len(probs)} nodes with 1-50 fixings, shared-matrix batched "
+                               f"engine warm-started from the parent (root) solution",
+                   "root_iterations": root.iterations, "cold_batched": cold, "parent_warm": parent,
+                   "learned_x_weight": learned, "learned_x_y_weight": learned_full,
+                   "train_nodes": args.train, "train_label_s": t_label, "train_s": t_train,
+                   "loss_first_last": [losses[0], losses[-1]], "persistent_engine_cold": persistent,
+                   "plain_single_gpu_lps_per_s": plain, "plain_sample": args.plain_sample,
+                   "speedup_vs_plain": value / plain if plain > 0 else None,
+                   "objective_rel_err_vs_highs_max": obj_err},
+        "roofline": None, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
@@ -344,7 +454,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="transport", choices=["transport", "batch", "learned"])
+    ap.add_argument("--workload", default="transport", choices=["transport", "batch", "learned", "nodes"])
+    ap.add_argument("--nodes", type=int, default=256)
     ap.add_argument("--train", type=int, default=4096)
     ap.add_argument("--eval", type=int, default=1024)
     ap.add_argument("--epochs", type=int, default=30)
@@ -376,6 +487,8 @@ def main():
         return bench_batch(args)
     if args.workload == "learned":
         return bench_learned(args)
+    if args.workload == "nodes":
+        return bench_nodes(args)
 
     import numpy as np
     import torch

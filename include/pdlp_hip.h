@@ -302,7 +302,8 @@ int pdlp_batch_solve(const PdlpBatch* batch, void* stream);
 #define PDLP_FAM_SHARED_BSHIFT 64
 #define PDLP_FAM_SHARED_LOSHIFT 128
 #define PDLP_FAM_SHARED_UBSPAN 256
-#define PDLP_FAM_KKT_CHUNKS 32     /* partial-sum chunks per KKT pass (max)   */
+#define PDLP_FAM_KKT_CHUNKS 128    /* partial-sum chunks per KKT pass (max)   */
+#define PDLP_FAM_LONG 64            /* rows / columns with more nonzeros run on a whole workgroup */
 #define PDLP_FAM_NSUM 12            /* 6 sums x (window average, current)      */
 
 typedef struct PdlpFamilyLP {
@@ -320,6 +321,10 @@ typedef struct PdlpFamily {
     int32_t flags;          /* as PdlpBatchLP.flags, for every LP              */
     int32_t shared;         /* PDLP_FAM_SHARED_* bits                          */
     int32_t items_per_wave; /* rows / columns per wavefront in a step, 1..64   */
+    int32_t long_rows;      /* the last long_rows rows have more than PDLP_FAM_LONG
+                               nonzeros and run on one workgroup each (the host
+                               numbers the rows so)                               */
+    int32_t long_cols;      /* likewise for the last long_cols columns           */
     double tol;
     /* K and K^T, CSR, FP32 values */
     const int32_t* row_ptr; const int32_t* col_idx; const float* val;
@@ -328,6 +333,8 @@ typedef struct PdlpFamily {
     const float* tau_hat;         /* n: 0.999 / max(sum_i |K_ij|, 1e-4)      */
     const float* sigma_hat;       /* m: 0.999 / max(sum_j |K_ij|, 1e-4)      */
     const double* row_metric;     /* m: max(sum_j |K_ij|, 1e-4) / 0.999      */
+    const int32_t* row_orig;      /* m: caller's index of each row (pdlp_family_set_start) */
+    const int32_t* col_orig;      /* n: caller's index of each column                     */
     /* problem data, tiled per LP or shared */
     const float *c, *lo, *hi, *b;
     const double *c64, *lo_shift, *ub_span, *b64, *b_shift;
@@ -345,6 +352,19 @@ typedef struct PdlpFamily {
 /* Set up every LP from (x, y), lp[k].norm_b/norm_c/weight as filled by the
  * host: sums cleared, restart point = start, KKT of the start. */
 int pdlp_family_init(const PdlpFamily* f, void* stream);
+/* Starting points from a row-major FP32 matrix src[count][ld] (src may be
+ * NULL when x_off, y_off and w_col are all negative: cold start) (e.g. the
+ * warm-start network's output, already on the device): v = src[k*ld + col]
+ * * scale[col] + shift[col] (scale / shift may be NULL); x_j from column
+ * x_off + col_orig[j] clipped to [lo, hi], y_i from column y_off +
+ * row_orig[i] (y_i <= 0 on
+ * PDLP_ROW_LE rows); x_off < 0 / y_off < 0 give the cold values clip(0, lo,
+ * hi) / 0; non-finite values count as 0.  If w_col >= 0 the primal weight of
+ * LP k becomes exp(clamp(v, -8, 8)) of column w_col.  Call after the host has
+ * written lp[] and before pdlp_family_init.  No counterpart in the
+ * reference, whose PDHG always starts at 0 (pdhg.py:175-176). */
+int pdlp_family_set_start(const PdlpFamily* f, const float* src, int64_t ld, int32_t x_off, int32_t y_off,
+                          int32_t w_col, const float* scale, const float* shift, void* stream);
 /* `blocks` times: check_every iterations of every running LP, then a check. */
 int pdlp_family_run(const PdlpFamily* f, int32_t blocks, void* stream);
 /* `iters` iterations without a check (the tail below check_every). */
@@ -355,8 +375,8 @@ int pdlp_family_finish(const PdlpFamily* f, int32_t tail, void* stream);
 /* HIP graph of `blocks` x (check_every iterations + check), as pdlp_family_run;
  * launch with pdlp_graph_launch, free with pdlp_graph_destroy. */
 int pdlp_family_graph_capture(const PdlpFamily* f, int32_t blocks, void* stream, void** exec);
-/* HIP graph of `reps` launches of one step kernel (which: 0 column pass,
- * 1 row pass), for timing. */
+/* HIP graph of `reps` column passes (which = 0) or row passes (which = 1),
+ * each the long-item and the short-item kernel, for timing. */
 int pdlp_family_graph_capture_step(const PdlpFamily* f, int32_t which, int32_t reps, void* stream, void** exec);
 
 /* ----------------------------------------------------------------------------

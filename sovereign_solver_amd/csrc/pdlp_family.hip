@@ -72,6 +72,40 @@ __device__ __forceinline__ float gather_dot(const int32_t* __restrict__ idx, con
 }
 
 // ---------------------------------------------------------------- iteration
+// Column update of column j for the wavefront's 64 LPs, given K^T y.
+__device__ __forceinline__ void column_update(const PdlpFamily& f, int tile, int lane, bool act, float wgt, int j,
+                                              float aty) {
+    const bool shc = f.shared & PDLP_FAM_SHARED_C, shl = f.shared & PDLP_FAM_SHARED_LO,
+               shh = f.shared & PDLP_FAM_SHARED_HI;
+    const size_t ix = at(tile, f.n, j, lane);
+    const float xo = f.x[ix];
+    float xn = xo - (wgt * f.tau_hat[j]) * (ldv(f.c, shc, tile, f.n, j, lane) - aty);
+    const float lo = ldv(f.lo, shl, tile, f.n, j, lane), hi = ldv(f.hi, shh, tile, f.n, j, lane);
+    xn = xn < lo ? lo : xn;
+    xn = xn > hi ? hi : xn;
+    if (act) {
+        f.x[ix] = xn;
+        f.xbar[ix] = 2.f * xn - xo;
+        f.x_psum[ix] += xn;
+    }
+}
+
+// Row update of row i for the wavefront's 64 LPs, given K xbar.
+__device__ __forceinline__ void row_update(const PdlpFamily& f, int tile, int lane, bool act, float inv_wgt, int i,
+                                           float d) {
+    const bool shb = f.shared & PDLP_FAM_SHARED_B;
+    const size_t ix = at(tile, f.m, i, lane);
+    const float sigma = inv_wgt * f.sigma_hat[i];
+    float yn = f.y[ix] + sigma * (ldv(f.b, shb, tile, f.m, i, lane) - d);
+    if (f.row_kind[i] == PDLP_ROW_LE && yn > 0.f) yn = 0.f;
+    if (act) {
+        f.y[ix] = yn;
+        f.y_psum[ix] += yn;
+    }
+}
+
+// Column pass over the short columns [0, n - long_cols),
+// items_per_wave columns per wavefront.
 __global__ void __launch_bounds__(NT) primal_kernel(PdlpFamily f) {
     const int tile = blockIdx.y;
     const uint64_t mask = f.run_mask[tile];
@@ -80,27 +114,16 @@ __global__ void __launch_bounds__(NT) primal_kernel(PdlpFamily f) {
     const bool act = (mask >> lane) & 1ull;
     const int k = tile * T + lane;
     const float wgt = act ? f.lp[k].weight : 1.f;
-    const bool shc = f.shared & PDLP_FAM_SHARED_C, shl = f.shared & PDLP_FAM_SHARED_LO,
-               shh = f.shared & PDLP_FAM_SHARED_HI;
-    const int j0 = (blockIdx.x * NWV + w) * f.items_per_wave;
-    const int j1 = min(j0 + f.items_per_wave, f.n);
-    for (int j = j0; j < j1; ++j) {
-        const int beg = f.t_row_ptr[j], end = f.t_row_ptr[j + 1];
-        const float aty = gather_dot(f.t_col_idx, f.t_val, beg, end, f.y, tile, f.m, lane);
-        const size_t ix = at(tile, f.n, j, lane);
-        const float xo = f.x[ix];
-        float xn = xo - (wgt * f.tau_hat[j]) * (ldv(f.c, shc, tile, f.n, j, lane) - aty);
-        const float lo = ldv(f.lo, shl, tile, f.n, j, lane), hi = ldv(f.hi, shh, tile, f.n, j, lane);
-        xn = xn < lo ? lo : xn;
-        xn = xn > hi ? hi : xn;
-        if (act) {
-            f.x[ix] = xn;
-            f.xbar[ix] = 2.f * xn - xo;
-            f.x_psum[ix] += xn;
-        }
+    const int p0 = (blockIdx.x * NWV + w) * f.items_per_wave;
+    const int p1 = min(p0 + f.items_per_wave, f.n - f.long_cols);
+    for (int p = p0; p < p1; ++p) {
+        const int j = p;
+        const float aty = gather_dot(f.t_col_idx, f.t_val, f.t_row_ptr[j], f.t_row_ptr[j + 1], f.y, tile, f.m, lane);
+        column_update(f, tile, lane, act, wgt, j, aty);
     }
 }
 
+// Row pass over the short rows.
 __global__ void __launch_bounds__(NT) dual_kernel(PdlpFamily f) {
     const int tile = blockIdx.y;
     const uint64_t mask = f.run_mask[tile];
@@ -109,21 +132,54 @@ __global__ void __launch_bounds__(NT) dual_kernel(PdlpFamily f) {
     const bool act = (mask >> lane) & 1ull;
     const int k = tile * T + lane;
     const float inv_wgt = act ? 1.f / f.lp[k].weight : 1.f;
-    const bool shb = f.shared & PDLP_FAM_SHARED_B;
-    const int i0 = (blockIdx.x * NWV + w) * f.items_per_wave;
-    const int i1 = min(i0 + f.items_per_wave, f.m);
-    for (int i = i0; i < i1; ++i) {
-        const int beg = f.row_ptr[i], end = f.row_ptr[i + 1];
-        const float d = gather_dot(f.col_idx, f.val, beg, end, f.xbar, tile, f.n, lane);
-        const size_t ix = at(tile, f.m, i, lane);
-        const float sigma = inv_wgt * f.sigma_hat[i];
-        float yn = f.y[ix] + sigma * (ldv(f.b, shb, tile, f.m, i, lane) - d);
-        if (f.row_kind[i] == PDLP_ROW_LE && yn > 0.f) yn = 0.f;
-        if (act) {
-            f.y[ix] = yn;
-            f.y_psum[ix] += yn;
-        }
+    const int p0 = (blockIdx.x * NWV + w) * f.items_per_wave;
+    const int p1 = min(p0 + f.items_per_wave, f.m - f.long_rows);
+    for (int p = p0; p < p1; ++p) {
+        const int i = p;
+        const float d = gather_dot(f.col_idx, f.val, f.row_ptr[i], f.row_ptr[i + 1], f.xbar, tile, f.n, lane);
+        row_update(f, tile, lane, act, inv_wgt, i, d);
     }
+}
+
+// One long row (or column) per workgroup of NL wavefronts: each wavefront
+// takes a contiguous share of the nonzeros, the shares meet in LDS in
+// wavefront order, and wavefront 0 applies the update.
+constexpr int NL = 8;
+
+__device__ __forceinline__ float split_dot(const int32_t* __restrict__ idx, const float* __restrict__ val, int beg,
+                                           int end, const float* __restrict__ v, int tile, int len, int lane, int w) {
+    __shared__ float part[NL][T];
+    const int per = (end - beg + NL - 1) / NL;
+    const int b = min(beg + w * per, end), e = min(b + per, end);
+    part[w][lane] = gather_dot(idx, val, b, e, v, tile, len, lane);
+    __syncthreads();
+    float d = 0.f;
+    if (w == 0)
+#pragma unroll
+        for (int u = 0; u < NL; ++u) d += part[u][lane];
+    return d;
+}
+
+__global__ void __launch_bounds__(NL * kWave) primal_long_kernel(PdlpFamily f) {
+    const int tile = blockIdx.y;
+    const uint64_t mask = f.run_mask[tile];
+    if (mask == 0) return;
+    const int lane = lane_id(), w = wave_id();
+    const bool act = (mask >> lane) & 1ull;
+    const int j = f.n - f.long_cols + blockIdx.x;
+    const float aty = split_dot(f.t_col_idx, f.t_val, f.t_row_ptr[j], f.t_row_ptr[j + 1], f.y, tile, f.m, lane, w);
+    if (w == 0) column_update(f, tile, lane, act, act ? f.lp[tile * T + lane].weight : 1.f, j, aty);
+}
+
+__global__ void __launch_bounds__(NL * kWave) dual_long_kernel(PdlpFamily f) {
+    const int tile = blockIdx.y;
+    const uint64_t mask = f.run_mask[tile];
+    if (mask == 0) return;
+    const int lane = lane_id(), w = wave_id();
+    const bool act = (mask >> lane) & 1ull;
+    const int i = f.m - f.long_rows + blockIdx.x;
+    const float d = split_dot(f.col_idx, f.val, f.row_ptr[i], f.row_ptr[i + 1], f.xbar, tile, f.n, lane, w);
+    if (w == 0) row_update(f, tile, lane, act, act ? 1.f / f.lp[tile * T + lane].weight : 1.f, i, d);
 }
 
 // ------------------------------------------------------------------- checks
@@ -166,7 +222,8 @@ __global__ void __launch_bounds__(NT) kkt_cols_kernel(PdlpFamily f, int add) {
     double a[NS2];
 #pragma unroll
     for (int s = 0; s < NS2; ++s) a[s] = 0.0;
-    for (int j = blockIdx.x * NWV + w; j < f.n; j += gridDim.x * NWV) {
+    for (int p = blockIdx.x * NWV + w; p < f.n; p += gridDim.x * NWV) {
+        const int j = f.n - 1 - p;                       // long columns first
         const size_t ix = at(tile, f.n, j, lane);
         const double xs = f.x_sum[ix] + (double)f.x_psum[ix];
         if (act) {
@@ -187,6 +244,7 @@ __global__ void __launch_bounds__(NT) kkt_cols_kernel(PdlpFamily f, int add) {
         }
         double sa = 0.0, sc = 0.0;
         const int beg = f.t_row_ptr[j], end = f.t_row_ptr[j + 1];
+#pragma unroll 4
         for (int q = beg; q < end; ++q) {
             const double kv = (double)f.t_val[q];
             const size_t iy = at(tile, f.m, f.t_col_idx[q], lane);
@@ -225,7 +283,8 @@ __global__ void __launch_bounds__(NT) kkt_rows_kernel(PdlpFamily f, int add) {
     double a[NS2];
 #pragma unroll
     for (int s = 0; s < NS2; ++s) a[s] = 0.0;
-    for (int i = blockIdx.x * NWV + w; i < f.m; i += gridDim.x * NWV) {
+    for (int p = blockIdx.x * NWV + w; p < f.m; p += gridDim.x * NWV) {
+        const int i = f.m - 1 - p;                       // long rows first
         const size_t iy = at(tile, f.m, i, lane);
         const double ys = f.y_sum[iy] + (double)f.y_psum[iy];
         if (act) {
@@ -236,6 +295,7 @@ __global__ void __launch_bounds__(NT) kkt_rows_kernel(PdlpFamily f, int add) {
         const double ya = win > 0 ? ys / wd : yc;
         double da = 0.0, dc = 0.0;
         const int beg = f.row_ptr[i], end = f.row_ptr[i + 1];
+#pragma unroll 4
         for (int q = beg; q < end; ++q) {
             const double kv = (double)f.val[q];
             const size_t jx = at(tile, f.n, f.col_idx[q], lane);
@@ -300,28 +360,38 @@ __device__ __forceinline__ void write_out(PdlpBatchOut* o, const PdlpFamilyLP& s
     *o = r;
 }
 
-// One thread per LP (one wavefront per tile).  mode 0: start (reference error
-// of the starting point); mode 1: a check after check_every iterations.
-__global__ void __launch_bounds__(T) decide_kernel(PdlpFamily f, int mode, int col_chunks, int row_chunks) {
-    const int tile = blockIdx.x, lane = threadIdx.x;
+// One workgroup of 12 wavefronts per tile: wavefront q adds up sum q over the
+// chunks (lane = LP), then wavefront 0 takes one decision per LP.
+// mode 0: start (reference error of the starting point); mode 1: a check
+// after check_every iterations.
+__global__ void __launch_bounds__(NS2 * T) decide_kernel(PdlpFamily f, int mode, int col_chunks, int row_chunks) {
+    __shared__ double red[NS2][T];
+    const int tile = blockIdx.x, lane = lane_id(), q = wave_id();
     const int k = tile * T + lane;
     const uint64_t mask = f.run_mask[tile];
     const bool act = (mask >> lane) & 1ull;
+    {
+        const size_t cpad = (size_t)f.tiles * T;
+        double t = 0.0;
+        if (act) {
+#pragma unroll 8
+            for (int c = 0; c < col_chunks; ++c) t += f.partials[((size_t)c * NS2 + q) * cpad + k];
+#pragma unroll 8
+            for (int c = 0; c < row_chunks; ++c) t += f.partials[((size_t)(KCH + c) * NS2 + q) * cpad + k];
+        }
+        red[q][lane] = t;
+    }
+    __syncthreads();
+    if (q != 0) return;                                   // no barrier below
     bool running = false;
     if (k < f.count) {
         PdlpFamilyLP s = f.lp[k];
         if (!act) {
             s.action = ACT_NONE;
         } else {
-            const size_t cpad = (size_t)f.tiles * T;
             double sum[NS2];
 #pragma unroll
-            for (int q = 0; q < NS2; ++q) {
-                double t = 0.0;
-                for (int c = 0; c < col_chunks; ++c) t += f.partials[((size_t)c * NS2 + q) * cpad + k];
-                for (int c = 0; c < row_chunks; ++c) t += f.partials[((size_t)(KCH + c) * NS2 + q) * cpad + k];
-                sum[q] = t;
-            }
+            for (int v = 0; v < NS2; ++v) sum[v] = red[v][lane];
             const bool adaptive = f.flags & 1, fixed_weight = f.flags & 2;
             Kkt ka = kkt_of(sum, s.norm_b, s.norm_c);
             const Kkt kc = kkt_of(sum + 6, s.norm_b, s.norm_c);
@@ -503,6 +573,61 @@ __global__ void __launch_bounds__(T) finish_lp_kernel(PdlpFamily f, int tail) {
     }
 }
 
+// Starting point from a row-major FP32 matrix (a network output, one row per
+// LP): a 64 LP x 64 index block is read row by row (coalesced along the
+// row), transposed through LDS and written to the tiled layout, with the
+// affine de-normalisation and the projections on the way.
+__global__ void __launch_bounds__(NT) set_start_kernel(PdlpFamily f, const float* __restrict__ src, int64_t ld,
+                                                       int x_off, int y_off, const float* __restrict__ scale,
+                                                       const float* __restrict__ shift) {
+    __shared__ float blk[T][T + 1];
+    const int tile = blockIdx.y, lane = lane_id(), w = wave_id();
+    const int d0 = blockIdx.x * T, D = f.n + f.m;
+    for (int r = w; r < T; r += NWV) {
+        const int k = tile * T + r, d = d0 + lane;
+        float v = 0.f;
+        if (k < f.count && d < D) {
+            const int col = d < f.n ? (x_off >= 0 ? x_off + f.col_orig[d] : -1)
+                                    : (y_off >= 0 ? y_off + f.row_orig[d - f.n] : -1);
+            if (col >= 0) {
+                v = src[(int64_t)k * ld + col];
+                if (scale) v *= scale[col];
+                if (shift) v += shift[col];
+                if (!isfinite(v)) v = 0.f;
+            }
+        }
+        blk[r][lane] = v;
+    }
+    __syncthreads();
+    const bool shl = f.shared & PDLP_FAM_SHARED_LO, shh = f.shared & PDLP_FAM_SHARED_HI;
+    for (int dd = w; dd < T; dd += NWV) {
+        const int d = d0 + dd;
+        if (d >= D) break;
+        float v = blk[lane][dd];
+        if (d < f.n) {
+            const float lo = ldv(f.lo, shl, tile, f.n, d, lane), hi = ldv(f.hi, shh, tile, f.n, d, lane);
+            v = v < lo ? lo : v;
+            v = v > hi ? hi : v;
+            f.x[at(tile, f.n, d, lane)] = v;
+        } else {
+            const int i = d - f.n;
+            if (f.row_kind[i] == PDLP_ROW_LE && v > 0.f) v = 0.f;
+            f.y[at(tile, f.m, i, lane)] = v;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(T) set_weight_kernel(PdlpFamily f, const float* __restrict__ src, int64_t ld,
+                                                       int w_col, const float* __restrict__ scale,
+                                                       const float* __restrict__ shift) {
+    const int k = blockIdx.x * T + threadIdx.x;
+    if (k >= f.count) return;
+    float v = src[(int64_t)k * ld + w_col];
+    if (scale) v *= scale[w_col];
+    if (shift) v += shift[w_col];
+    if (isfinite(v)) f.lp[k].weight = (float)exp(fmin(fmax((double)v, -8.0), 8.0));
+}
+
 __global__ void __launch_bounds__(T) clear_mask_kernel(PdlpFamily f) {
     if (threadIdx.x == 0) f.run_mask[blockIdx.x] = 0;
 }
@@ -515,6 +640,9 @@ int check_family(const PdlpFamily* f) {
     if (f->tiles > 65535) return fail(PDLP_E_LIMIT, "at most 65535 tiles of 64 LPs");
     if (f->check_every <= 0) return fail(PDLP_E_ARG, "check_every must be > 0");
     if (f->items_per_wave < 1 || f->items_per_wave > 64) return fail(PDLP_E_TUNE, "items_per_wave in [1, 64]");
+    if (f->long_rows < 0 || f->long_rows > f->m || f->long_cols < 0 || f->long_cols > f->n)
+        return fail(PDLP_E_SHAPE, "long_rows in [0, m] and long_cols in [0, n]");
+    if (!f->row_orig || !f->col_orig) return fail(PDLP_E_ARG, "row_orig / col_orig is NULL");
     if (!f->row_ptr || !f->col_idx || !f->val || !f->t_row_ptr || !f->t_col_idx || !f->t_val || !f->row_kind ||
         !f->tau_hat || !f->sigma_hat || !f->row_metric || !f->c || !f->lo || !f->hi || !f->b || !f->c64 ||
         !f->lo_shift || !f->ub_span || !f->b64 || !f->b_shift || !f->x || !f->xbar || !f->x_psum || !f->x_last ||
@@ -537,10 +665,14 @@ dim3 vec_grid(const PdlpFamily& f) {
 }
 
 void launch_iterations(const PdlpFamily& f, int iters, hipStream_t s) {
-    const dim3 gp = step_grid(f, f.n), gd = step_grid(f, f.m);
+    const int sc = f.n - f.long_cols, sr = f.m - f.long_rows;
+    const dim3 gp = step_grid(f, sc), gd = step_grid(f, sr);
+    const dim3 glp((unsigned)f.long_cols, (unsigned)f.tiles), gld((unsigned)f.long_rows, (unsigned)f.tiles);
     for (int it = 0; it < iters; ++it) {
-        primal_kernel<<<gp, NT, 0, s>>>(f);
-        dual_kernel<<<gd, NT, 0, s>>>(f);
+        if (f.long_cols) primal_long_kernel<<<glp, NL * kWave, 0, s>>>(f);
+        if (sc) primal_kernel<<<gp, NT, 0, s>>>(f);
+        if (f.long_rows) dual_long_kernel<<<gld, NL * kWave, 0, s>>>(f);
+        if (sr) dual_kernel<<<gd, NT, 0, s>>>(f);
     }
 }
 
@@ -549,7 +681,7 @@ void launch_check(const PdlpFamily& f, int mode, hipStream_t s) {
     const int add = mode == 0 ? 0 : f.check_every;
     kkt_cols_kernel<<<dim3(cc, f.tiles), NT, 0, s>>>(f, add);
     kkt_rows_kernel<<<dim3(rc, f.tiles), NT, 0, s>>>(f, add);
-    decide_kernel<<<f.tiles, T, 0, s>>>(f, mode, cc, rc);
+    decide_kernel<<<f.tiles, NS2 * T, 0, s>>>(f, mode, cc, rc);
     apply_kernel<<<vec_grid(f), NT, 0, s>>>(f);
     count_kernel<<<1, NT, 0, s>>>(f);
 }
@@ -584,6 +716,22 @@ extern "C" int pdlp_family_iterate(const PdlpFamily* f, int32_t iters, void* str
     if (iters < 0) return fail(PDLP_E_ARG, "iters must be >= 0");
     launch_iterations(*f, iters, as_stream(stream));
     return hip_fail(hipGetLastError(), "pdlp_family_iterate launch");
+}
+
+extern "C" int pdlp_family_set_start(const PdlpFamily* f, const float* src, int64_t ld, int32_t x_off,
+                                     int32_t y_off, int32_t w_col, const float* scale, const float* shift,
+                                     void* stream) {
+    if (int e = check_family(f)) return e;
+    if (src == nullptr && (x_off >= 0 || y_off >= 0 || w_col >= 0))
+        return fail(PDLP_E_ARG, "src is NULL but a column offset is given");
+    const int64_t need = std::max<int64_t>({(int64_t)x_off + f->n, (int64_t)y_off + f->m, (int64_t)w_col + 1});
+    if (src != nullptr && (ld <= 0 || need > ld))
+        return fail(PDLP_E_SHAPE, "src columns x_off+n, y_off+m, w_col must lie within ld");
+    hipStream_t s = as_stream(stream);
+    const dim3 g((unsigned)((f->n + f->m + T - 1) / T), (unsigned)f->tiles);
+    set_start_kernel<<<g, NT, 0, s>>>(*f, src, ld, x_off, y_off, scale, shift);
+    if (w_col >= 0) set_weight_kernel<<<f->tiles, T, 0, s>>>(*f, src, ld, w_col, scale, shift);
+    return hip_fail(hipGetLastError(), "pdlp_family_set_start launch");
 }
 
 extern "C" int pdlp_family_finish(const PdlpFamily* f, int32_t tail, void* stream) {
@@ -642,11 +790,18 @@ extern "C" int pdlp_family_graph_capture_step(const PdlpFamily* f, int32_t which
         return fail(PDLP_E_ARG, "reps > 0, which in {0, 1} and exec non-NULL");
     if (stream == nullptr) return fail(PDLP_E_ARG, "graph capture needs a non-default stream");
     hipStream_t s = as_stream(stream);
-    const dim3 g = step_grid(*f, which == 0 ? f->n : f->m);
+    const int sc = f->n - f->long_cols, sr = f->m - f->long_rows;
+    const dim3 gp = step_grid(*f, sc), gd = step_grid(*f, sr);
+    const dim3 glp((unsigned)f->long_cols, (unsigned)f->tiles), gld((unsigned)f->long_rows, (unsigned)f->tiles);
     return capture(s, exec, [&] {
         for (int r = 0; r < reps; ++r) {
-            if (which == 0) primal_kernel<<<g, NT, 0, s>>>(*f);
-            else dual_kernel<<<g, NT, 0, s>>>(*f);
+            if (which == 0) {
+                if (f->long_cols) primal_long_kernel<<<glp, NL * kWave, 0, s>>>(*f);
+                if (sc) primal_kernel<<<gp, NT, 0, s>>>(*f);
+            } else {
+                if (f->long_rows) dual_long_kernel<<<gld, NL * kWave, 0, s>>>(*f);
+                if (sr) dual_kernel<<<gd, NT, 0, s>>>(*f);
+            }
         }
     });
 }

@@ -23,7 +23,8 @@ PDLP_SCHEME_REFERENCE, PDLP_SCHEME_ADAPTIVE = 0, 1
 PDLP_FAM_TILE = 64
 PDLP_FAM_SHARED = {"c": 1, "b": 2, "lo": 4, "hi": 8, "c64": 16, "b64": 32, "b_shift": 64, "lo_shift": 128,
                    "ub_span": 256}
-PDLP_FAM_KKT_CHUNKS = 32
+PDLP_FAM_KKT_CHUNKS = 128
+PDLP_FAM_LONG = 64
 PDLP_FAM_NSUM = 12
 
 _p = C.c_void_p
@@ -90,10 +91,10 @@ class PdlpFamily(C.Structure):
     _fields_ = [
         ("m", C.c_int32), ("n", C.c_int32), ("count", C.c_int32), ("tiles", C.c_int32),
         ("check_every", C.c_int32), ("flags", C.c_int32), ("shared", C.c_int32), ("items_per_wave", C.c_int32),
-        ("tol", C.c_double),
+        ("long_rows", C.c_int32), ("long_cols", C.c_int32), ("tol", C.c_double),
         ("row_ptr", _p), ("col_idx", _p), ("val", _p), ("t_row_ptr", _p), ("t_col_idx", _p), ("t_val", _p),
-        ("row_kind", _p), ("tau_hat", _p), ("sigma_hat", _p), ("row_metric", _p),
-        ("c", _p), ("lo", _p), ("hi", _p), ("b", _p),
+        ("row_kind", _p), ("tau_hat", _p), ("sigma_hat", _p), ("row_metric", _p), ("row_orig", _p),
+        ("col_orig", _p), ("c", _p), ("lo", _p), ("hi", _p), ("b", _p),
         ("c64", _p), ("lo_shift", _p), ("ub_span", _p), ("b64", _p), ("b_shift", _p),
         ("x", _p), ("xbar", _p), ("x_psum", _p), ("x_last", _p),
         ("y", _p), ("y_psum", _p), ("y_last", _p), ("x_sum", _p), ("y_sum", _p),
@@ -150,6 +151,8 @@ _SIGS = {
     "pdlp_family_run": (C.c_int, [C.POINTER(PdlpFamily), C.c_int32, _p]),
     "pdlp_family_iterate": (C.c_int, [C.POINTER(PdlpFamily), C.c_int32, _p]),
     "pdlp_family_finish": (C.c_int, [C.POINTER(PdlpFamily), C.c_int32, _p]),
+    "pdlp_family_set_start": (C.c_int, [C.POINTER(PdlpFamily), _p, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
+                                        _p, _p, _p]),
     "pdlp_family_graph_capture": (C.c_int, [C.POINTER(PdlpFamily), C.c_int32, _p, C.POINTER(C.c_void_p)]),
     "pdlp_family_graph_capture_step": (C.c_int, [C.POINTER(PdlpFamily), C.c_int32, C.c_int32, _p,
                                                  C.POINTER(C.c_void_p)]),

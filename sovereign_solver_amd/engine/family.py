@@ -25,7 +25,7 @@ from .forms import ROW_EQ, ROW_LE
 from ..problem import Problem
 
 TILE = L.PDLP_FAM_TILE
-MAX_GRAPH_BLOCKS = 4
+MAX_GRAPH_BLOCKS = 1
 
 
 def _canonical(A) -> sp.csr_matrix:
@@ -62,7 +62,13 @@ def from_tiles(flat: np.ndarray, tiles: int, Ln: int, B: int) -> np.ndarray:
 
 
 class FamilyBatch:
-    """Host data of B box-form LPs that share K = [A_ub; A_eq]."""
+    """Host data of B box-form LPs that share K = [A_ub; A_eq].
+
+    The device numbers rows and columns so that those with more than
+    ``PDLP_FAM_LONG`` entries (one workgroup each) come last:
+    ``row_perm[i]`` / ``col_perm[j]`` is the problem's index of device row i /
+    column j.  Every ``*_dev`` array and every vector in ``vec`` is in device
+    order; :meth:`to_device_cols` / :meth:`from_device_cols` convert."""
 
     def __init__(self, problems: Sequence[Problem]):
         self.problems = list(problems)
@@ -75,14 +81,22 @@ class FamilyBatch:
                 raise ValueError(f"LP {k}: shape differs from LP 0")
             if not (_same_matrix(p.A_ub, p0.A_ub) and _same_matrix(p.A_eq, p0.A_eq)):
                 raise ValueError(f"LP {k}: constraint matrix differs from LP 0 (use the persistent batch engine)")
-        K = _canonical(sp.vstack([sp.csr_matrix(p0.A_ub), sp.csr_matrix(p0.A_eq)], format="csr"))
-        self.K, self.KT = K, _canonical(K.T.tocsr())
-        m, n = K.shape
+        K0 = _canonical(sp.vstack([sp.csr_matrix(p0.A_ub), sp.csr_matrix(p0.A_eq)], format="csr"))
+        self.K_orig = K0
+        m, n = K0.shape
         if m == 0 or n == 0:
             raise ValueError("the family engine needs m > 0 and n > 0")
         self.m, self.n, self.count = m, n, len(self.problems)
         self.tiles = (self.count + TILE - 1) // TILE
-        self.kind = np.concatenate([np.full(self.m_ub, ROW_LE, np.int8), np.full(self.m_eq, ROW_EQ, np.int8)])
+        rl, cl = np.diff(K0.indptr), np.diff(K0.tocsc().indptr)
+        self.row_perm = np.concatenate([np.flatnonzero(rl <= L.PDLP_FAM_LONG), np.flatnonzero(rl > L.PDLP_FAM_LONG)])
+        self.col_perm = np.concatenate([np.flatnonzero(cl <= L.PDLP_FAM_LONG), np.flatnonzero(cl > L.PDLP_FAM_LONG)])
+        self.long_rows = int((rl > L.PDLP_FAM_LONG).sum())
+        self.long_cols = int((cl > L.PDLP_FAM_LONG).sum())
+        K = _canonical(K0[self.row_perm][:, self.col_perm])
+        self.K, self.KT = K, _canonical(K.T.tocsr())
+        kind0 = np.concatenate([np.full(self.m_ub, ROW_LE, np.int8), np.full(self.m_eq, ROW_EQ, np.int8)])
+        self.kind = kind0[self.row_perm]
         aK = abs(K)
         colsum = np.asarray(aK.sum(axis=0)).ravel()
         rowsum = np.asarray(aK.sum(axis=1)).ravel()
@@ -91,10 +105,11 @@ class FamilyBatch:
         self.row_metric = np.maximum(rowsum, 1e-4) / 0.999
 
         P = self.problems
-        c = np.stack([np.asarray(p.c, float) for p in P])
-        b = np.stack([np.concatenate([np.asarray(p.b_ub, float), np.asarray(p.b_eq, float)]) for p in P])
-        lo = np.stack([np.asarray(p.lb, float) for p in P])
-        hi = np.stack([np.asarray(p.ub, float) for p in P])
+        cp, rp = self.col_perm, self.row_perm
+        c = np.stack([np.asarray(p.c, float)[cp] for p in P])
+        b = np.stack([np.concatenate([np.asarray(p.b_ub, float), np.asarray(p.b_eq, float)])[rp] for p in P])
+        lo = np.stack([np.asarray(p.lb, float)[cp] for p in P])
+        hi = np.stack([np.asarray(p.ub, float)[cp] for p in P])
         lbf = np.where(np.isfinite(lo), lo, 0.0)
         b_shift = b - (K @ lbf.T).T
         span = np.where(np.isfinite(hi), hi - lbf, np.inf)
@@ -114,6 +129,22 @@ class FamilyBatch:
             if bool(np.all(a == a[0])):             # (inf == inf holds)
                 self.shared |= L.PDLP_FAM_SHARED[name]
         self.lo, self.hi = lo, hi
+
+    def to_device_cols(self, X: np.ndarray) -> np.ndarray:
+        return X[:, self.col_perm]
+
+    def to_device_rows(self, Y: np.ndarray) -> np.ndarray:
+        return Y[:, self.row_perm]
+
+    def from_device_cols(self, Xd: np.ndarray) -> np.ndarray:
+        X = np.empty_like(Xd)
+        X[:, self.col_perm] = Xd
+        return X
+
+    def from_device_rows(self, Yd: np.ndarray) -> np.ndarray:
+        Y = np.empty_like(Yd)
+        Y[:, self.row_perm] = Yd
+        return Y
 
     def is_shared(self, name: str) -> bool:
         return bool(self.shared & L.PDLP_FAM_SHARED[name])
@@ -140,7 +171,7 @@ class FamilyEngine:
     """Device state of a :class:`FamilyBatch` and the lockstep solve loop."""
 
     def __init__(self, fb: FamilyBatch, scheme: str = "adaptive", check_every: int = 50,
-                 items_per_wave: int = 1, use_graphs: bool = False, compaction: bool = True):
+                 items_per_wave: int = 1, use_graphs: bool = True, compaction: bool = True):
         flags = {"reference": 0, "adaptive": 1, "adaptive_fixed": 3, "adaptive_start": 5}
         if scheme not in flags:
             raise ValueError("scheme must be 'adaptive', 'adaptive_fixed', 'adaptive_start' or 'reference'")
@@ -166,6 +197,10 @@ class FamilyEngine:
                 "t_val": dev_tensor(fb.KT.data, f32, dv),
                 "row_kind": dev_tensor(fb.kind, torch.int8, dv), "tau_hat": dev_tensor(fb.tau_hat, f32, dv),
                 "sigma_hat": dev_tensor(fb.sigma_hat, f32, dv), "row_metric": dev_tensor(fb.row_metric, f64, dv),
+                "row_orig": dev_tensor(fb.row_perm.astype(np.int32), i32, dv),
+                "col_orig": dev_tensor(fb.col_perm.astype(np.int32), i32, dv),
+                "row_dev": dev_tensor(np.argsort(fb.row_perm).astype(np.int32), i32, dv),
+                "col_dev": dev_tensor(np.argsort(fb.col_perm).astype(np.int32), i32, dv),
             }
             for name, ln in (("x", n), ("xbar", n), ("x_psum", n), ("x_last", n),
                              ("y", m), ("y_psum", m), ("y_last", m)):
@@ -176,8 +211,12 @@ class FamilyEngine:
             t["out"] = torch.zeros(B * OUT_DTYPE.itemsize, dtype=torch.uint8, device=dv)
             t["run_mask"] = torch.zeros(fb.tiles, dtype=torch.int64, device=dv)
             t["running"] = torch.zeros(1, dtype=i32, device=dv)
+            t["X_out"] = torch.zeros((B, n), dtype=f64, device=dv)
+            t["Y_out"] = torch.zeros((B, m), dtype=f64, device=dv)
+            t["out_final"] = torch.zeros(B * OUT_DTYPE.itemsize, dtype=torch.uint8, device=dv)
             t["partials"] = torch.zeros(2 * L.PDLP_FAM_KKT_CHUNKS * L.PDLP_FAM_NSUM * Tn, dtype=f64, device=dv)
         self.t = t
+        self._orig = {}
         self._upload_data()
         self.stream.synchronize()
         # per-LP tiled arrays that compaction moves (window partial sums and xbar
@@ -189,24 +228,30 @@ class FamilyEngine:
         self.mask_host = torch.zeros((2, fb.tiles), dtype=torch.int64).pin_memory()
         self.compactions = 0
         self.args = L.PdlpFamily(m=m, n=n, count=B, tiles=fb.tiles, check_every=check_every, flags=self.flags,
-                                 shared=fb.shared, items_per_wave=items_per_wave, tol=0.0)
-        for name, _ in L.PdlpFamily._fields_[9:]:
+                                 shared=fb.shared, items_per_wave=items_per_wave, long_rows=fb.long_rows,
+                                 long_cols=fb.long_cols, tol=0.0)
+        for name, _ in L.PdlpFamily._fields_[11:]:
             setattr(self.args, name, ptr(t[name]))
         self._graphs = {}
+        self._host = None
 
     _DATA = {"c": torch.float32, "lo": torch.float32, "hi": torch.float32, "b": torch.float32,
              "c64": torch.float64, "lo_shift": torch.float64, "ub_span": torch.float64, "b64": torch.float64,
              "b_shift": torch.float64}
 
     def _upload_data(self) -> None:
-        """Problem vectors in the original slot order (compaction permutes them)."""
+        """Problem vectors in the original slot order.  Compaction permutes the
+        working copies of the per-LP ones; device originals restore them."""
         with torch.cuda.stream(self.stream):
             for name, dt in self._DATA.items():
-                a = torch.from_numpy(np.ascontiguousarray(self.fb.device_vector(name))).to(self.device, dt)
                 if name in self.t:
-                    self.t[name].copy_(a)
-                else:
-                    self.t[name] = a
+                    if name in self._orig:
+                        self.t[name].copy_(self._orig[name])
+                    continue
+                a = torch.from_numpy(np.ascontiguousarray(self.fb.device_vector(name))).to(self.device, dt)
+                self.t[name] = a
+                if not self.fb.is_shared(name):
+                    self._orig[name] = a.clone()
 
     @property
     def sh(self):
@@ -238,11 +283,13 @@ class FamilyEngine:
             self.drop_graphs()          # the graphs hold the argument block by value
 
     # ----------------------------------------------------------------- solve
-    def start(self, tol: float, x0: Optional[np.ndarray] = None, y0: Optional[np.ndarray] = None,
-              weight: Optional[np.ndarray] = None) -> None:
-        """Starting point ([B, n] / [B, m], None = cold start: x = clip(0, lo, hi),
-        y = 0) and per-LP primal weights (None or NaN entries: 1 for the reference
-        scheme, the norm ratio for the adaptive ones)."""
+    LP_DTYPE = np.dtype([(n_, "<i4") for n_ in ("status", "iter", "window", "restarts", "checks", "done_iter",
+                                                "last_restart", "action")] +
+                        [(n_, "<f8") for n_ in ("norm_b", "norm_c", "ref_err", "err_prev", "w_div")] +
+                        [("weight", "<f4"), ("pad", "<i4")])
+
+    def _reset(self, tol: float, weight: Optional[np.ndarray]) -> None:
+        """Slot bookkeeping and per-LP records for a new solve (original slot order)."""
         fb = self.fb
         self.set_tol(tol)
         if self.compactions or self.args.tiles != fb.tiles:
@@ -252,29 +299,63 @@ class FamilyEngine:
             self.drop_graphs()
         self.slot_lp = np.concatenate([np.arange(fb.count), np.full(fb.tiles * TILE - fb.count, -1)])
         self.saved = np.zeros(fb.count, bool)
-        self.X_out = np.zeros((fb.count, fb.n))
-        self.Y_out = np.zeros((fb.count, fb.m))
-        self.out_final = np.zeros(fb.count, OUT_DTYPE)
-        x = np.clip(0.0, fb.lo, fb.hi) if x0 is None else np.clip(np.asarray(x0, float), fb.lo, fb.hi)
-        y = np.zeros((fb.count, fb.m)) if y0 is None else np.array(y0, float)
-        y[:, : fb.m_ub] = np.minimum(y[:, : fb.m_ub], 0.0)      # y <= 0 on the <= rows
         w = fb.init_weight.copy() if self.flags & 1 else np.ones(fb.count)
         if weight is not None:
             ww = np.asarray(weight, float)
             w = np.where(np.isfinite(ww), ww, w)
-        lp = np.zeros(fb.count, dtype=np.dtype([(n_, "<i4") for n_ in
-                                                ("status", "iter", "window", "restarts", "checks", "done_iter",
-                                                 "last_restart", "action")] +
-                                               [(n_, "<f8") for n_ in ("norm_b", "norm_c", "ref_err", "err_prev",
-                                                                       "w_div")] +
-                                               [("weight", "<f4"), ("pad", "<i4")]))
+        lp = np.zeros(fb.count, dtype=self.LP_DTYPE)
         assert lp.dtype.itemsize == C.sizeof(L.PdlpFamilyLP)
         lp["norm_b"], lp["norm_c"], lp["weight"] = fb.norm_b, fb.norm_c, w
-        dv = self.device
         with torch.cuda.stream(self.stream):
-            self.t["x"].copy_(torch.from_numpy(to_tiles(x.astype(np.float32), fb.tiles)).to(dv))
-            self.t["y"].copy_(torch.from_numpy(to_tiles(y.astype(np.float32), fb.tiles)).to(dv))
-            self.t["lp"].copy_(torch.from_numpy(lp.view(np.uint8).copy()).to(dv))
+            self.t["lp"].copy_(torch.from_numpy(lp.view(np.uint8).copy()).to(self.device))
+
+    def start(self, tol: float, x0: Optional[np.ndarray] = None, y0: Optional[np.ndarray] = None,
+              weight: Optional[np.ndarray] = None) -> None:
+        """Starting point ([B, n] / [B, m], None = cold start: x = clip(0, lo, hi),
+        y = 0) and per-LP primal weights (None or NaN entries: 1 for the reference
+        scheme, the norm ratio for the adaptive ones)."""
+        fb = self.fb
+        self._reset(tol, weight)
+        dv = self.device
+        if x0 is None and y0 is None:           # cold start written on the device
+            L.check(self.lib.pdlp_family_set_start(C.byref(self.args), None, 0, -1, -1, -1, None, None, self.sh),
+                    "pdlp_family_set_start")
+        else:
+            x = (np.clip(0.0, fb.lo, fb.hi) if x0 is None else
+                 np.clip(fb.to_device_cols(np.asarray(x0, float).reshape(fb.count, fb.n)), fb.lo, fb.hi))
+            if y0 is None:
+                y = np.zeros((fb.count, fb.m))
+            else:
+                y = np.array(y0, float).reshape(fb.count, fb.m)
+                y[:, : fb.m_ub] = np.minimum(y[:, : fb.m_ub], 0.0)      # y <= 0 on the <= rows
+                y = fb.to_device_rows(y)
+            with torch.cuda.stream(self.stream):
+                self.t["x"].copy_(torch.from_numpy(to_tiles(x.astype(np.float32), fb.tiles)).to(dv))
+                self.t["y"].copy_(torch.from_numpy(to_tiles(y.astype(np.float32), fb.tiles)).to(dv))
+        L.check(self.lib.pdlp_family_init(C.byref(self.args), self.sh), "pdlp_family_init")
+
+    def start_from_device(self, tol: float, src: torch.Tensor, x_off: int = 0, y_off: int = -1, w_col: int = -1,
+                          scale: Optional[torch.Tensor] = None, shift: Optional[torch.Tensor] = None,
+                          ready: Optional[torch.cuda.Event] = None) -> None:
+        """Starting points read on the device from a row-major [B, d] FP32 matrix
+        (``pdlp_family_set_start``): x from columns x_off.., y from y_off.. (-1:
+        cold), the primal weight exp(.) of column w_col (-1: default), each
+        column de-normalised as v * scale + shift.  ``ready``: an event the
+        engine stream waits for (the producer's stream)."""
+        fb = self.fb
+        if src.dtype != torch.float32 or src.dim() != 2 or src.shape[0] != fb.count or not src.is_contiguous():
+            raise ValueError("src must be a contiguous [count, d] float32 device tensor")
+        d = src.shape[1]
+        for t_ in (scale, shift):
+            if t_ is not None and (t_.dtype != torch.float32 or t_.numel() != d or not t_.is_contiguous()):
+                raise ValueError("scale / shift must be contiguous float32 vectors of length d")
+        self._reset(tol, None)
+        if ready is not None:
+            self.stream.wait_event(ready)
+        else:
+            self.stream.wait_stream(torch.cuda.current_stream())
+        L.check(self.lib.pdlp_family_set_start(C.byref(self.args), src.data_ptr(), d, x_off, y_off, w_col,
+                                               ptr(scale), ptr(shift), self.sh), "pdlp_family_set_start")
         L.check(self.lib.pdlp_family_init(C.byref(self.args), self.sh), "pdlp_family_init")
 
     def run(self, max_iters: int) -> int:
@@ -340,20 +421,26 @@ class FamilyEngine:
         return (base.view(new_tiles, 1, TILE) + d.view(1, ln, 1)).reshape(-1)
 
     def _save(self, slots: np.ndarray) -> None:
-        """Copy the results of the LPs in ``slots`` (current layout) to the host."""
+        """Copy the results of the LPs in ``slots`` (current layout) into the
+        LP-major device outputs (read back once, by :meth:`results`)."""
         if slots.size == 0:
             return
         fb = self.fb
         lps = self.slot_lp[slots]
-        sl = torch.from_numpy(slots.astype(np.int64)).to(self.device)
+        dv = self.device
+        # every temporary is allocated on the engine stream that uses it, so the
+        # caching allocator cannot hand its memory to other work before that use
         with torch.cuda.stream(self.stream):
-            for name, ln, dst in (("x_sum", fb.n, self.X_out), ("y_sum", fb.m, self.Y_out)):
+            sl = torch.from_numpy(slots.astype(np.int64)).to(dv)
+            li = torch.from_numpy(lps.astype(np.int64)).to(dv)
+            for name, ln, dst, inv in (("x_sum", fb.n, "X_out", "col_dev"), ("y_sum", fb.m, "Y_out", "row_dev")):
+                # element (LP, problem index j) sits at device index inv[j]
                 base = (sl // TILE) * (ln * TILE) + sl % TILE
-                d = torch.arange(ln, device=self.device, dtype=torch.int64) * TILE
+                d = self.t[inv].to(torch.int64) * TILE
                 flat = (base.view(-1, 1) + d.view(1, ln)).reshape(-1)
-                dst[lps] = torch.take(self.t[name], flat).view(-1, ln).cpu().numpy()
-            rec = self.t["out"].view(-1, OUT_DTYPE.itemsize).index_select(0, sl).cpu().numpy()
-        self.out_final[lps] = np.frombuffer(rec.tobytes(), dtype=OUT_DTYPE)
+                self.t[dst].index_copy_(0, li, torch.take(self.t[name], flat).view(-1, ln))
+            rec = self.t["out"].view(-1, OUT_DTYPE.itemsize).index_select(0, sl)
+            self.t["out_final"].view(-1, OUT_DTYPE.itemsize).index_copy_(0, li, rec)
         self.saved[lps] = True
 
     def compact(self) -> None:
@@ -361,7 +448,8 @@ class FamilyEngine:
         first ceil(running / 64) tiles (stream idle, at a check boundary)."""
         fb = self.fb
         T, cnt = self.args.tiles, self.args.count
-        mask = self.t["run_mask"][:T].cpu().numpy().view(np.uint64)
+        with torch.cuda.stream(self.stream):
+            mask = self.t["run_mask"][:T].cpu().numpy().view(np.uint64)
         bits = ((mask[:, None] >> np.arange(TILE, dtype=np.uint64)[None, :]) & np.uint64(1)).astype(bool).ravel()
         slots = np.arange(T * TILE)
         valid = slots < cnt
@@ -372,8 +460,8 @@ class FamilyEngine:
         R = int(keep.size)
         new_T = (R + TILE - 1) // TILE
         perm_np = np.concatenate([keep, np.full(new_T * TILE - R, keep[0] if R else 0)]).astype(np.int64)
-        perm = torch.from_numpy(perm_np).to(self.device)
         with torch.cuda.stream(self.stream):
+            perm = torch.from_numpy(perm_np).to(self.device)
             idx = {fb.n: self._slot_index(perm, fb.n, new_T), fb.m: self._slot_index(perm, fb.m, new_T)}
             for name, ln in self._per_lp:
                 t = self.t[name]
@@ -396,13 +484,22 @@ class FamilyEngine:
         self.drop_graphs()
 
     def results(self):
-        """(out records, X [B, n] FP64 solutions, Y [B, m]) by LP."""
+        """(out records, X [B, n] FP64 solutions, Y [B, m]) by LP, in the problems'
+        own row and column order.  X and Y are views of pinned host buffers that
+        the next solve on this engine overwrites: copy what must outlive it."""
         self.stream.synchronize()
         cnt = self.args.count
         live = np.arange(cnt)[~self.saved[self.slot_lp[:cnt]]]
         self._save(live)
+        if self._host is None:                  # pinned staging, allocated once per engine
+            self._host = {k: torch.empty(self.t[k].shape, dtype=self.t[k].dtype).pin_memory()
+                          for k in ("X_out", "Y_out", "out_final")}
+        with torch.cuda.stream(self.stream):
+            for k, h in self._host.items():
+                h.copy_(self.t[k], non_blocking=True)
         self.stream.synchronize()
-        return self.out_final.copy(), self.X_out.copy(), self.Y_out.copy()
+        out = np.frombuffer(self._host["out_final"].numpy().tobytes(), dtype=OUT_DTYPE).copy()
+        return out, self._host["X_out"].numpy(), self._host["Y_out"].numpy()
 
     def solve(self, tol: float, max_iters: int, x0=None, y0=None, weight=None):
         self.start(tol, x0, y0, weight)

@@ -60,7 +60,7 @@ def family_warm_start(fb: FamilyBatch, warm_start):
     """Per-LP warm starts as [B, n] / [B, m] arrays and weights (NaN = default)."""
     if warm_start is None:
         return None, None, None
-    X = np.clip(0.0, fb.lo, fb.hi)
+    X = np.stack([np.clip(0.0, p.lb, p.ub) for p in fb.problems])     # the problems' own column order
     Y = np.zeros((fb.count, fb.m))
     w = np.full(fb.count, np.nan)
     for k, ws in enumerate(warm_start):

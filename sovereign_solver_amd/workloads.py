@@ -143,6 +143,10 @@ class DispatchFamily:
         d, c = self.params(k)
         return self.problem_from(d, c, name=f"dispatch_{k}")
 
+    def features_of(self, ids) -> np.ndarray:
+        """Network inputs of instances ``ids`` [B, T + G]."""
+        return np.stack([self.features(*self.params(int(k))) for k in ids])
+
     def rhs(self, d) -> np.ndarray:
         G, T = self.G, self.T
         ramp = np.repeat(np.tile(self.ramp, T - 1), 2)       # [up, down] per (t, g)
@@ -190,6 +194,28 @@ class SetCoverNodes:
         # shared by every node problem (the batched engine stores them once)
         self.negA = sp.csr_matrix(-self.A)
         self.no_eq = sp.csr_matrix((0, cols))
+
+    @property
+    def n(self) -> int:
+        return self.cols
+
+    @property
+    def m(self) -> int:
+        return self.rows
+
+    def problem(self, k: int) -> Problem:
+        """Node k (the family interface of the learned warm start)."""
+        return self.node(k)
+
+    def fixings(self, k: int) -> np.ndarray:
+        """+1 for a variable fixed to 1, -1 for one fixed to 0, 0 if free."""
+        p = self.node(k)
+        fixed = p.lb == p.ub
+        return np.where(fixed, 2.0 * p.lb - 1.0, 0.0)
+
+    def features_of(self, ids) -> np.ndarray:
+        """Network inputs of nodes ``ids``: their fixings [B, cols]."""
+        return np.stack([self.fixings(int(k)) for k in ids])
 
     def root(self) -> Problem:
         return self.node_from(np.zeros(self.cols), np.ones(self.cols), name="setcover_root")

@@ -52,7 +52,7 @@ def test_normalisers_match_single_box_form_and_yardstick():
         f = box_form(p)
         assert fb.norm_b[k] == pytest.approx(f.norm_b, rel=1e-14)
         assert fb.norm_c[k] == pytest.approx(f.norm_c, rel=1e-14)
-        np.testing.assert_allclose(fb.vec["b_shift"][k], f.b_shift, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(fb.from_device_rows(fb.vec["b_shift"])[k], f.b_shift, rtol=0, atol=1e-12)
         # the yardstick normaliser is the same quantity
         x = np.clip(0.0, p.lb, p.ub)
         y = np.zeros(p.n_constraints)
@@ -78,7 +78,7 @@ def test_equal_but_distinct_matrix_objects_accepted():
 def test_struct_layouts():
     import ctypes as C
     assert C.sizeof(L.PdlpFamilyLP) == 8 * 4 + 5 * 8 + 4 + 4
-    assert C.sizeof(L.PdlpFamily) == 8 * 4 + 8 + 33 * 8
+    assert C.sizeof(L.PdlpFamily) == 10 * 4 + 8 + 35 * 8
     lib = L.lib()
     assert lib.pdlp_family_init(None, None) == -1                               # PDLP_E_ARG
     assert b"NULL" in lib.pdlp_last_error()
@@ -101,3 +101,24 @@ def test_compaction_index_moves_slots():
     idx = FamilyEngine._slot_index(None, torch.from_numpy(perm), Ln, new_T)
     moved = torch.take(flat, idx).numpy()
     np.testing.assert_array_equal(from_tiles(moved, new_T, Ln, keep.size), a[keep])
+
+
+def test_device_numbering_puts_long_rows_last():
+    """Rows / columns above PDLP_FAM_LONG entries are numbered last on the device,
+    and the permuted K is the problem's K with rows and columns renumbered."""
+    fam = DispatchFamily(G=80, T=4)                  # demand rows have 80 > 64 entries
+    fb = FamilyBatch([fam.problem(k) for k in range(3)])
+    assert fb.long_rows == fam.T and fb.long_cols == 0
+    assert np.all(np.diff(fb.K.indptr)[-fb.long_rows:] > L.PDLP_FAM_LONG)
+    assert np.all(np.diff(fb.K.indptr)[: fb.m - fb.long_rows] <= L.PDLP_FAM_LONG)
+    dense = fb.K_orig.toarray()
+    np.testing.assert_array_equal(fb.K.toarray(), dense[fb.row_perm][:, fb.col_perm])
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((3, fb.n))
+    np.testing.assert_array_equal(fb.from_device_cols(fb.to_device_cols(X)), X)
+    Y = rng.standard_normal((3, fb.m))
+    np.testing.assert_array_equal(fb.from_device_rows(fb.to_device_rows(Y)), Y)
+    # device-order problem data matches the problems' own data
+    p = fam.problem(1)
+    np.testing.assert_array_equal(fb.from_device_cols(fb.vec["c"])[1], p.c)
+    np.testing.assert_array_equal(fb.from_device_rows(fb.vec["b"])[1], p.b_ub)

@@ -69,11 +69,12 @@ def test_family_launch_path_and_compaction_agree_bitwise(gpu):
     fam = DispatchFamily(G=12, T=5)
     fb = FamilyBatch([fam.problem(k) for k in range(150)])
     runs = []
-    for graphs, compact in ((False, False), (True, False), (False, True)):
+    for graphs, compact in ((False, False), (True, False), (False, True), (True, True)):
         eng = FamilyEngine(fb, use_graphs=graphs, compaction=compact)
         runs.append(eng.solve(1e-4, 100_000))
         if compact:
             assert eng.compactions >= 1
+            runs.append(eng.solve(1e-4, 100_000))       # a second solve on the same engine
     a = runs[0]
     for b in runs[1:]:
         assert np.array_equal(a[0]["iterations"], b[0]["iterations"])
@@ -98,3 +99,32 @@ def test_family_warm_start_from_solution(gpu):
                                engine="family")
     for c, w in zip(cold, warm):
         assert w.status == "optimal" and w.iterations <= max(50, c.iterations // 2)
+
+
+def test_family_device_start_matches_host_start(gpu):
+    """pdlp_family_set_start (row-major FP32 source, affine de-normalisation,
+    projections, tiling) gives the same solve as the host-side start."""
+    import torch
+    fam = DispatchFamily(G=12, T=5)
+    fb = FamilyBatch([fam.problem(k) for k in range(70)])
+    rng = np.random.default_rng(0)
+    n, m = fb.n, fb.m
+    raw = rng.standard_normal((70, n + m + 1)).astype(np.float32)
+    scale = rng.uniform(0.5, 2.0, n + m + 1).astype(np.float32)
+    shift = rng.standard_normal(n + m + 1).astype(np.float32)
+    val = raw.astype(np.float64) * scale + shift
+    val32 = (raw * scale + shift).astype(np.float64)       # the kernel's FP32 arithmetic
+    a_eng = FamilyEngine(fb)
+    a_eng.start(1e-4, val32[:, :n], val32[:, n:n + m], np.exp(np.clip(val32[:, -1], -8, 8)))
+    a_eng.run(100_000)
+    a = a_eng.results()
+    b_eng = FamilyEngine(fb)
+    dv = b_eng.device
+    src = torch.from_numpy(raw).to(dv)
+    b_eng.start_from_device(1e-4, src, 0, n, n + m, torch.from_numpy(scale).to(dv), torch.from_numpy(shift).to(dv))
+    b_eng.run(100_000)
+    b = b_eng.results()
+    assert np.all(a[0]["status"] == 1) and np.all(b[0]["status"] == 1)
+    # the host path rounds x0 / y0 / weight through FP64 first: agree to a few checks
+    assert np.all(np.abs(a[0]["iterations"] - b[0]["iterations"]) <= 100)
+    assert np.allclose(val, val32, rtol=1e-5, atol=1e-5)
