@@ -282,6 +282,31 @@ def cpu_rate(probs, tol, cap_s):
                        f"same workload, solved to {tol:g} one after another in {tc:.1f}s on 1 core (BLAS threads 1)")}
 
 
+def family_roofline(probs, tol):
+    """Graph-timed column and row passes of the family engine over all LPs at
+    the cold-start state (every LP running): algorithmic bytes / mean time."""
+    from sovereign_solver_amd.engine.family import FamilyBatch, FamilyEngine
+    fb = FamilyBatch(probs)
+    eng = FamilyEngine(fb)
+    shape = eng.autotune(tol)
+    eng.start(tol)
+    us = {w: eng.time_step(w, reps=20) for w in ("primal", "dual")}
+    del eng
+    b = {"primal": fb.column_pass_bytes(), "dual": fb.row_pass_bytes()}
+    gbs = {w: b[w] / (us[w] * 1e-6) / 1e9 for w in us}
+    dom = max(us, key=us.get)
+    names = {"primal": "column pass (primal_kernel + primal_long_kernel)",
+             "dual": "row pass (dual_kernel + dual_long_kernel)"}
+    log(f"family passes: column {us['primal']:.1f} us ({b['primal'] / 1e6:.1f} MB, {gbs['primal']:.0f} GB/s), "
+        f"row {us['dual']:.1f} us ({b['dual'] / 1e6:.1f} MB, {gbs['dual']:.0f} GB/s)")
+    other = "dual" if dom == "primal" else "primal"
+    return {"bound": "hbm", "kernel": names[dom], "achieved": gbs[dom], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gbs[dom] / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": b[dom], "us_per_launch": us[dom],
+            "launch_shape": shape,
+            "other_pass": {"kernel": names[other], "achieved": gbs[other], "bytes_per_launch": b[other],
+                           "us_per_launch": us[other]}}
+
+
 def family_rate(res, secs):
     ok = sum(r.status == "optimal" for r in res)
     its = np.array([r.iterations for r in res])
@@ -312,6 +337,7 @@ def bench_learned(args):
     log(f"trained in {t_train:.1f}s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
     ls = W.LearnedStart(mlp, nin, nout)
     probs = [fam.problem(int(k)) for k in eval_ids]
+    roof = family_roofline(probs, args.tol)
 
     W.solve_family(fam, eval_ids[:64], args.tol, args.max_iters)                      # warm-up (build, graphs)
     cold = family_rate(*W.solve_family(fam, eval_ids, args.tol, args.max_iters))
@@ -355,8 +381,9 @@ def bench_learned(args):
                    "speedup_vs_plain": value / plain if plain > 0 else None,
                    "learned_vs_cold_batched": value / cold["lps_per_s"],
                    "objective_rel_err_vs_highs_max": obj_err},
-        "roofline": None, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu,
     }
+    calib_copy(args)
     print(json.dumps(out), flush=True)
 
 
@@ -375,6 +402,7 @@ def bench_nodes(args):
     node_ids = np.arange(args.nodes)
     train_ids = 10**6 + np.arange(args.train)
     probs = [sc.node(int(k)) for k in node_ids]
+    roof = family_roofline(probs, args.tol)
     W.solve_family(sc, node_ids[:64], args.tol, args.max_iters)                       # warm-up
     cold = family_rate(*W.solve_family(sc, node_ids, args.tol, args.max_iters))
     parent_ws = [(np.clip(root.x, p.lb, p.ub), root.y, root.step_scale) for p in probs]
@@ -430,8 +458,9 @@ def bench_nodes(args):
                    "plain_single_gpu_lps_per_s": plain, "plain_sample": args.plain_sample,
                    "speedup_vs_plain": value / plain if plain > 0 else None,
                    "objective_rel_err_vs_highs_max": obj_err},
-        "roofline": None, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu,
     }
+    calib_copy(args)
     print(json.dumps(out), flush=True)
 
 

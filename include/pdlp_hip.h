@@ -303,7 +303,8 @@ int pdlp_batch_solve(const PdlpBatch* batch, void* stream);
 #define PDLP_FAM_SHARED_LOSHIFT 128
 #define PDLP_FAM_SHARED_UBSPAN 256
 #define PDLP_FAM_KKT_CHUNKS 128    /* partial-sum chunks per KKT pass (max)   */
-#define PDLP_FAM_LONG 64            /* rows / columns with more nonzeros run on a whole workgroup */
+#define PDLP_FAM_LONG 64            /* default split: rows / columns with more nonzeros run on a
+                                       whole workgroup (the host numbers them last)          */
 #define PDLP_FAM_NSUM 12            /* 6 sums x (window average, current)      */
 
 typedef struct PdlpFamilyLP {
@@ -321,10 +322,14 @@ typedef struct PdlpFamily {
     int32_t flags;          /* as PdlpBatchLP.flags, for every LP              */
     int32_t shared;         /* PDLP_FAM_SHARED_* bits                          */
     int32_t items_per_wave; /* rows / columns per wavefront in a step, 1..64   */
-    int32_t long_rows;      /* the last long_rows rows have more than PDLP_FAM_LONG
-                               nonzeros and run on one workgroup each (the host
-                               numbers the rows so)                               */
+    int32_t long_rows;      /* the last long_rows rows run on one workgroup each (the host
+                               numbers the rows with more than a threshold of
+                               nonzeros, PDLP_FAM_LONG by default, last)          */
     int32_t long_cols;      /* likewise for the last long_cols columns           */
+    int32_t xcd_affinity;   /* 1: pin each tile's workgroups to its own XCD(s), so a tile's
+                               gathered vectors stay in one L2 (workgroups are dispatched
+                               round-robin over the 8 XCDs); 0: 2-D grid (chunk, tile) */
+    int32_t reserved;
     double tol;
     /* K and K^T, CSR, FP32 values */
     const int32_t* row_ptr; const int32_t* col_idx; const float* val;

@@ -71,6 +71,38 @@ __device__ __forceinline__ float gather_dot(const int32_t* __restrict__ idx, con
     return d;
 }
 
+// Workgroup -> (tile, chunk of the tile's rows / columns).  With XCD affinity
+// the grid is 1-D and workgroup b runs on XCD b % 8 (round-robin dispatch):
+// with 8 or more tiles XCD x owns tiles x, x+8, ...; with fewer, each tile
+// owns 8 / tiles XCDs that share its chunks.  P = chunks per tile.
+struct Place {
+    int tile, chunk;
+    bool ok;
+};
+
+__device__ __forceinline__ Place place(const PdlpFamily& f, int P) {
+    Place r;
+    if (!f.xcd_affinity) {
+        r.tile = blockIdx.y;
+        r.chunk = blockIdx.x;
+        r.ok = true;
+        return r;
+    }
+    const int b = blockIdx.x, x = b & 7, slot = b >> 3, T = f.tiles;
+    if (T >= 8) {
+        const int lt = slot / P;
+        r.chunk = slot - lt * P;
+        r.tile = x + 8 * lt;
+        r.ok = r.tile < T;
+    } else {
+        const int g = 8 / T;
+        r.tile = x / g;
+        r.chunk = slot * g + (x - r.tile * g);
+        r.ok = x < T * g && r.chunk < P;
+    }
+    return r;
+}
+
 // ---------------------------------------------------------------- iteration
 // Column update of column j for the wavefront's 64 LPs, given K^T y.
 __device__ __forceinline__ void column_update(const PdlpFamily& f, int tile, int lane, bool act, float wgt, int j,
@@ -106,15 +138,17 @@ __device__ __forceinline__ void row_update(const PdlpFamily& f, int tile, int la
 
 // Column pass over the short columns [0, n - long_cols),
 // items_per_wave columns per wavefront.
-__global__ void __launch_bounds__(NT) primal_kernel(PdlpFamily f) {
-    const int tile = blockIdx.y;
+__global__ void __launch_bounds__(NT) primal_kernel(PdlpFamily f, int P) {
+    const Place pl = place(f, P);
+    if (!pl.ok) return;
+    const int tile = pl.tile;
     const uint64_t mask = f.run_mask[tile];
     if (mask == 0) return;
     const int lane = lane_id(), w = wave_id();
     const bool act = (mask >> lane) & 1ull;
     const int k = tile * T + lane;
     const float wgt = act ? f.lp[k].weight : 1.f;
-    const int p0 = (blockIdx.x * NWV + w) * f.items_per_wave;
+    const int p0 = (pl.chunk * NWV + w) * f.items_per_wave;
     const int p1 = min(p0 + f.items_per_wave, f.n - f.long_cols);
     for (int p = p0; p < p1; ++p) {
         const int j = p;
@@ -124,15 +158,17 @@ __global__ void __launch_bounds__(NT) primal_kernel(PdlpFamily f) {
 }
 
 // Row pass over the short rows.
-__global__ void __launch_bounds__(NT) dual_kernel(PdlpFamily f) {
-    const int tile = blockIdx.y;
+__global__ void __launch_bounds__(NT) dual_kernel(PdlpFamily f, int P) {
+    const Place pl = place(f, P);
+    if (!pl.ok) return;
+    const int tile = pl.tile;
     const uint64_t mask = f.run_mask[tile];
     if (mask == 0) return;
     const int lane = lane_id(), w = wave_id();
     const bool act = (mask >> lane) & 1ull;
     const int k = tile * T + lane;
     const float inv_wgt = act ? 1.f / f.lp[k].weight : 1.f;
-    const int p0 = (blockIdx.x * NWV + w) * f.items_per_wave;
+    const int p0 = (pl.chunk * NWV + w) * f.items_per_wave;
     const int p1 = min(p0 + f.items_per_wave, f.m - f.long_rows);
     for (int p = p0; p < p1; ++p) {
         const int i = p;
@@ -160,24 +196,28 @@ __device__ __forceinline__ float split_dot(const int32_t* __restrict__ idx, cons
     return d;
 }
 
-__global__ void __launch_bounds__(NL * kWave) primal_long_kernel(PdlpFamily f) {
-    const int tile = blockIdx.y;
+__global__ void __launch_bounds__(NL * kWave) primal_long_kernel(PdlpFamily f, int P) {
+    const Place pl = place(f, P);
+    if (!pl.ok) return;
+    const int tile = pl.tile;
     const uint64_t mask = f.run_mask[tile];
     if (mask == 0) return;
     const int lane = lane_id(), w = wave_id();
     const bool act = (mask >> lane) & 1ull;
-    const int j = f.n - f.long_cols + blockIdx.x;
+    const int j = f.n - f.long_cols + pl.chunk;
     const float aty = split_dot(f.t_col_idx, f.t_val, f.t_row_ptr[j], f.t_row_ptr[j + 1], f.y, tile, f.m, lane, w);
     if (w == 0) column_update(f, tile, lane, act, act ? f.lp[tile * T + lane].weight : 1.f, j, aty);
 }
 
-__global__ void __launch_bounds__(NL * kWave) dual_long_kernel(PdlpFamily f) {
-    const int tile = blockIdx.y;
+__global__ void __launch_bounds__(NL * kWave) dual_long_kernel(PdlpFamily f, int P) {
+    const Place pl = place(f, P);
+    if (!pl.ok) return;
+    const int tile = pl.tile;
     const uint64_t mask = f.run_mask[tile];
     if (mask == 0) return;
     const int lane = lane_id(), w = wave_id();
     const bool act = (mask >> lane) & 1ull;
-    const int i = f.m - f.long_rows + blockIdx.x;
+    const int i = f.m - f.long_rows + pl.chunk;
     const float d = split_dot(f.col_idx, f.val, f.row_ptr[i], f.row_ptr[i + 1], f.xbar, tile, f.n, lane, w);
     if (w == 0) row_update(f, tile, lane, act, act ? 1.f / f.lp[tile * T + lane].weight : 1.f, i, d);
 }
@@ -207,8 +247,10 @@ __device__ __forceinline__ void store_chunk(const PdlpFamily& f, int chunk, int 
 // Columns: fold the x partial sums; primal objective and distance of both
 // candidates; K^T y of both candidates (y window sums not folded yet: the row
 // pass folds them after this pass) for the dual residual / bound-row duals.
-__global__ void __launch_bounds__(NT) kkt_cols_kernel(PdlpFamily f, int add) {
-    const int tile = blockIdx.y;
+__global__ void __launch_bounds__(NT) kkt_cols_kernel(PdlpFamily f, int add, int P) {
+    const Place pl = place(f, P);
+    if (!pl.ok) return;
+    const int tile = pl.tile;
     const uint64_t mask = f.run_mask[tile];
     if (mask == 0) return;                      // whole workgroup: no barrier is skipped by part of it
     const int lane = lane_id(), w = wave_id();
@@ -222,7 +264,7 @@ __global__ void __launch_bounds__(NT) kkt_cols_kernel(PdlpFamily f, int add) {
     double a[NS2];
 #pragma unroll
     for (int s = 0; s < NS2; ++s) a[s] = 0.0;
-    for (int p = blockIdx.x * NWV + w; p < f.n; p += gridDim.x * NWV) {
+    for (int p = pl.chunk * NWV + w; p < f.n; p += P * NWV) {
         const int j = f.n - 1 - p;                       // long columns first
         const size_t ix = at(tile, f.n, j, lane);
         const double xs = f.x_sum[ix] + (double)f.x_psum[ix];
@@ -263,14 +305,16 @@ __global__ void __launch_bounds__(NT) kkt_cols_kernel(PdlpFamily f, int add) {
             a[7] += rc * rc;
         }
     }
-    store_chunk(f, blockIdx.x, tile, a);
+    store_chunk(f, pl.chunk, tile, a);
 }
 
 // Rows: fold the y partial sums; primal residual (K x of both candidates, x
 // window sums folded by the column pass), dual objective, sign violation of
 // y on <= rows, dual distance.
-__global__ void __launch_bounds__(NT) kkt_rows_kernel(PdlpFamily f, int add) {
-    const int tile = blockIdx.y;
+__global__ void __launch_bounds__(NT) kkt_rows_kernel(PdlpFamily f, int add, int P) {
+    const Place pl = place(f, P);
+    if (!pl.ok) return;
+    const int tile = pl.tile;
     const uint64_t mask = f.run_mask[tile];
     if (mask == 0) return;
     const int lane = lane_id(), w = wave_id();
@@ -283,7 +327,7 @@ __global__ void __launch_bounds__(NT) kkt_rows_kernel(PdlpFamily f, int add) {
     double a[NS2];
 #pragma unroll
     for (int s = 0; s < NS2; ++s) a[s] = 0.0;
-    for (int p = blockIdx.x * NWV + w; p < f.m; p += gridDim.x * NWV) {
+    for (int p = pl.chunk * NWV + w; p < f.m; p += P * NWV) {
         const int i = f.m - 1 - p;                       // long rows first
         const size_t iy = at(tile, f.m, i, lane);
         const double ys = f.y_sum[iy] + (double)f.y_psum[iy];
@@ -323,7 +367,7 @@ __global__ void __launch_bounds__(NT) kkt_rows_kernel(PdlpFamily f, int add) {
             a[11] += dc2 * dc2 * rm;
         }
     }
-    store_chunk(f, KCH + blockIdx.x, tile, a);
+    store_chunk(f, KCH + pl.chunk, tile, a);
 }
 
 struct Kkt {
@@ -643,6 +687,7 @@ int check_family(const PdlpFamily* f) {
     if (f->long_rows < 0 || f->long_rows > f->m || f->long_cols < 0 || f->long_cols > f->n)
         return fail(PDLP_E_SHAPE, "long_rows in [0, m] and long_cols in [0, n]");
     if (!f->row_orig || !f->col_orig) return fail(PDLP_E_ARG, "row_orig / col_orig is NULL");
+    if (f->xcd_affinity != 0 && f->xcd_affinity != 1) return fail(PDLP_E_TUNE, "xcd_affinity must be 0 or 1");
     if (!f->row_ptr || !f->col_idx || !f->val || !f->t_row_ptr || !f->t_col_idx || !f->t_val || !f->row_kind ||
         !f->tau_hat || !f->sigma_hat || !f->row_metric || !f->c || !f->lo || !f->hi || !f->b || !f->c64 ||
         !f->lo_shift || !f->ub_span || !f->b64 || !f->b_shift || !f->x || !f->xbar || !f->x_psum || !f->x_last ||
@@ -654,9 +699,17 @@ int check_family(const PdlpFamily* f) {
 
 int chunks(int items) { return std::max(1, std::min(KCH, (items + NWV - 1) / NWV)); }
 
-dim3 step_grid(const PdlpFamily& f, int items) {
+// Grid for P chunks per tile (see place()).
+dim3 tile_grid(const PdlpFamily& f, int P) {
+    if (!f.xcd_affinity) return dim3((unsigned)P, (unsigned)f.tiles);
+    const int T = f.tiles;
+    const int per_xcd = T >= 8 ? ((T + 7) / 8) * P : (P + (8 / T) - 1) / (8 / T);
+    return dim3((unsigned)(8 * per_xcd), 1u);
+}
+
+int step_chunks(const PdlpFamily& f, int items) {
     const int per_block = NWV * f.items_per_wave;
-    return dim3((unsigned)((items + per_block - 1) / per_block), (unsigned)f.tiles);
+    return (items + per_block - 1) / per_block;
 }
 
 dim3 vec_grid(const PdlpFamily& f) {
@@ -664,23 +717,30 @@ dim3 vec_grid(const PdlpFamily& f) {
     return dim3((unsigned)((f.n + f.m + per_block - 1) / per_block), (unsigned)f.tiles);
 }
 
+void launch_column_pass(const PdlpFamily& f, hipStream_t s) {
+    const int sc = f.n - f.long_cols, pc = step_chunks(f, sc);
+    if (f.long_cols) primal_long_kernel<<<tile_grid(f, f.long_cols), NL * kWave, 0, s>>>(f, f.long_cols);
+    if (sc) primal_kernel<<<tile_grid(f, pc), NT, 0, s>>>(f, pc);
+}
+
+void launch_row_pass(const PdlpFamily& f, hipStream_t s) {
+    const int sr = f.m - f.long_rows, pr = step_chunks(f, sr);
+    if (f.long_rows) dual_long_kernel<<<tile_grid(f, f.long_rows), NL * kWave, 0, s>>>(f, f.long_rows);
+    if (sr) dual_kernel<<<tile_grid(f, pr), NT, 0, s>>>(f, pr);
+}
+
 void launch_iterations(const PdlpFamily& f, int iters, hipStream_t s) {
-    const int sc = f.n - f.long_cols, sr = f.m - f.long_rows;
-    const dim3 gp = step_grid(f, sc), gd = step_grid(f, sr);
-    const dim3 glp((unsigned)f.long_cols, (unsigned)f.tiles), gld((unsigned)f.long_rows, (unsigned)f.tiles);
     for (int it = 0; it < iters; ++it) {
-        if (f.long_cols) primal_long_kernel<<<glp, NL * kWave, 0, s>>>(f);
-        if (sc) primal_kernel<<<gp, NT, 0, s>>>(f);
-        if (f.long_rows) dual_long_kernel<<<gld, NL * kWave, 0, s>>>(f);
-        if (sr) dual_kernel<<<gd, NT, 0, s>>>(f);
+        launch_column_pass(f, s);
+        launch_row_pass(f, s);
     }
 }
 
 void launch_check(const PdlpFamily& f, int mode, hipStream_t s) {
     const int cc = chunks(f.n), rc = chunks(f.m);
     const int add = mode == 0 ? 0 : f.check_every;
-    kkt_cols_kernel<<<dim3(cc, f.tiles), NT, 0, s>>>(f, add);
-    kkt_rows_kernel<<<dim3(rc, f.tiles), NT, 0, s>>>(f, add);
+    kkt_cols_kernel<<<tile_grid(f, cc), NT, 0, s>>>(f, add, cc);
+    kkt_rows_kernel<<<tile_grid(f, rc), NT, 0, s>>>(f, add, rc);
     decide_kernel<<<f.tiles, NS2 * T, 0, s>>>(f, mode, cc, rc);
     apply_kernel<<<vec_grid(f), NT, 0, s>>>(f);
     count_kernel<<<1, NT, 0, s>>>(f);
@@ -790,18 +850,10 @@ extern "C" int pdlp_family_graph_capture_step(const PdlpFamily* f, int32_t which
         return fail(PDLP_E_ARG, "reps > 0, which in {0, 1} and exec non-NULL");
     if (stream == nullptr) return fail(PDLP_E_ARG, "graph capture needs a non-default stream");
     hipStream_t s = as_stream(stream);
-    const int sc = f->n - f->long_cols, sr = f->m - f->long_rows;
-    const dim3 gp = step_grid(*f, sc), gd = step_grid(*f, sr);
-    const dim3 glp((unsigned)f->long_cols, (unsigned)f->tiles), gld((unsigned)f->long_rows, (unsigned)f->tiles);
     return capture(s, exec, [&] {
         for (int r = 0; r < reps; ++r) {
-            if (which == 0) {
-                if (f->long_cols) primal_long_kernel<<<glp, NL * kWave, 0, s>>>(*f);
-                if (sc) primal_kernel<<<gp, NT, 0, s>>>(*f);
-            } else {
-                if (f->long_rows) dual_long_kernel<<<gld, NL * kWave, 0, s>>>(*f);
-                if (sr) dual_kernel<<<gd, NT, 0, s>>>(*f);
-            }
+            if (which == 0) launch_column_pass(*f, s);
+            else launch_row_pass(*f, s);
         }
     });
 }

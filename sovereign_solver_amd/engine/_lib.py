@@ -91,7 +91,8 @@ class PdlpFamily(C.Structure):
     _fields_ = [
         ("m", C.c_int32), ("n", C.c_int32), ("count", C.c_int32), ("tiles", C.c_int32),
         ("check_every", C.c_int32), ("flags", C.c_int32), ("shared", C.c_int32), ("items_per_wave", C.c_int32),
-        ("long_rows", C.c_int32), ("long_cols", C.c_int32), ("tol", C.c_double),
+        ("long_rows", C.c_int32), ("long_cols", C.c_int32), ("xcd_affinity", C.c_int32), ("reserved", C.c_int32),
+        ("tol", C.c_double),
         ("row_ptr", _p), ("col_idx", _p), ("val", _p), ("t_row_ptr", _p), ("t_col_idx", _p), ("t_val", _p),
         ("row_kind", _p), ("tau_hat", _p), ("sigma_hat", _p), ("row_metric", _p), ("row_orig", _p),
         ("col_orig", _p), ("c", _p), ("lo", _p), ("hi", _p), ("b", _p),

@@ -70,10 +70,11 @@ class FamilyBatch:
     column j.  Every ``*_dev`` array and every vector in ``vec`` is in device
     order; :meth:`to_device_cols` / :meth:`from_device_cols` convert."""
 
-    def __init__(self, problems: Sequence[Problem]):
+    def __init__(self, problems: Sequence[Problem], long_threshold: int = L.PDLP_FAM_LONG):
         self.problems = list(problems)
         if not self.problems:
             raise ValueError("empty family")
+        self.long_threshold = long_threshold
         p0 = self.problems[0]
         self.m_ub, self.m_eq = p0.n_ineq, p0.n_eq
         for k, p in enumerate(self.problems):
@@ -89,10 +90,11 @@ class FamilyBatch:
         self.m, self.n, self.count = m, n, len(self.problems)
         self.tiles = (self.count + TILE - 1) // TILE
         rl, cl = np.diff(K0.indptr), np.diff(K0.tocsc().indptr)
-        self.row_perm = np.concatenate([np.flatnonzero(rl <= L.PDLP_FAM_LONG), np.flatnonzero(rl > L.PDLP_FAM_LONG)])
-        self.col_perm = np.concatenate([np.flatnonzero(cl <= L.PDLP_FAM_LONG), np.flatnonzero(cl > L.PDLP_FAM_LONG)])
-        self.long_rows = int((rl > L.PDLP_FAM_LONG).sum())
-        self.long_cols = int((cl > L.PDLP_FAM_LONG).sum())
+        lt = long_threshold
+        self.row_perm = np.concatenate([np.flatnonzero(rl <= lt), np.flatnonzero(rl > lt)])
+        self.col_perm = np.concatenate([np.flatnonzero(cl <= lt), np.flatnonzero(cl > lt)])
+        self.long_rows = int((rl > lt).sum())
+        self.long_cols = int((cl > lt).sum())
         K = _canonical(K0[self.row_perm][:, self.col_perm])
         self.K, self.KT = K, _canonical(K.T.tocsr())
         kind0 = np.concatenate([np.full(self.m_ub, ROW_LE, np.int8), np.full(self.m_eq, ROW_EQ, np.int8)])
@@ -153,25 +155,34 @@ class FamilyBatch:
         a = self.vec[name]
         return a[0] if self.is_shared(name) else to_tiles(a, self.tiles)
 
-    def algorithmic_bytes_per_iteration(self) -> float:
-        """Bytes one lockstep iteration of all LPs moves at minimum: the shared CSR
-        of K and K^T once, each per-LP vector once per access (FP32)."""
-        m, n, B = self.m, self.n, self.tiles * TILE
-        z = self.K.nnz
-        csr = 2 * (8 * z) + 4 * (m + 1 + n + 1)
+    def column_pass_bytes(self) -> float:
+        """Algorithmic bytes of one column pass over all B LPs: K^T's CSR once
+        (8 B per entry, 4 B per row pointer), tau_hat, y gathered once (4 B per
+        row and LP), c / lo / hi (once if shared), x read and written, xbar
+        written, the x partial sum read and written (4 B each per column and LP)."""
+        m, n, B = self.m, self.n, self.count
         per = lambda name, L_: L_ * 4 * (1 if self.is_shared(name) else B)
-        # column pass: y gathered once, c, lo, hi, x r/w, xbar w, x partial sum r/w
-        col = 4 * m * B + per("c", n) + per("lo", n) + per("hi", n) + 5 * 4 * n * B
-        # row pass: xbar gathered once, b, y r/w, y partial sum r/w
-        row = 4 * n * B + per("b", m) + 4 * 4 * m * B
-        return float(csr + col + row + 4 * (n + m))
+        return float(8 * self.K.nnz + 4 * (n + 1) + 4 * n + 4 * m * B + per("c", n) + per("lo", n) +
+                     per("hi", n) + 5 * 4 * n * B)
+
+    def row_pass_bytes(self) -> float:
+        """Algorithmic bytes of one row pass over all B LPs: K's CSR once,
+        sigma_hat and the row kinds, xbar gathered once, b (once if shared), y
+        read and written, the y partial sum read and written."""
+        m, n, B = self.m, self.n, self.count
+        per = lambda name, L_: L_ * 4 * (1 if self.is_shared(name) else B)
+        return float(8 * self.K.nnz + 4 * (m + 1) + 5 * m + 4 * n * B + per("b", m) + 4 * 4 * m * B)
+
+    def algorithmic_bytes_per_iteration(self) -> float:
+        return self.column_pass_bytes() + self.row_pass_bytes()
 
 
 class FamilyEngine:
     """Device state of a :class:`FamilyBatch` and the lockstep solve loop."""
 
     def __init__(self, fb: FamilyBatch, scheme: str = "adaptive", check_every: int = 50,
-                 items_per_wave: int = 1, use_graphs: bool = True, compaction: bool = True):
+                 items_per_wave: int = 1, use_graphs: bool = True, compaction: bool = True,
+                 xcd_affinity: bool = False):
         flags = {"reference": 0, "adaptive": 1, "adaptive_fixed": 3, "adaptive_start": 5}
         if scheme not in flags:
             raise ValueError("scheme must be 'adaptive', 'adaptive_fixed', 'adaptive_start' or 'reference'")
@@ -229,8 +240,8 @@ class FamilyEngine:
         self.compactions = 0
         self.args = L.PdlpFamily(m=m, n=n, count=B, tiles=fb.tiles, check_every=check_every, flags=self.flags,
                                  shared=fb.shared, items_per_wave=items_per_wave, long_rows=fb.long_rows,
-                                 long_cols=fb.long_cols, tol=0.0)
-        for name, _ in L.PdlpFamily._fields_[11:]:
+                                 long_cols=fb.long_cols, xcd_affinity=int(xcd_affinity), tol=0.0)
+        for name, _ in L.PdlpFamily._fields_[13:]:
             setattr(self.args, name, ptr(t[name]))
         self._graphs = {}
         self._host = None
@@ -508,6 +519,26 @@ class FamilyEngine:
 
     def unpack(self, out, X, Y):
         return [(_STATUS[int(out["status"][k])], X[k], Y[k], out[k]) for k in range(self.fb.count)]
+
+    # ---------------------------------------------------------------- tuning
+    def autotune(self, tol: float = 1e-4, reps: int = 10) -> dict:
+        """Pick rows / columns per wavefront (1, 2) and XCD affinity (off, on) by
+        timing the column and row passes (HIP graphs, events) at the cold-start
+        state with every LP running; keeps the fastest sum.  Placement changes
+        no arithmetic, so results do not depend on the choice."""
+        best = None
+        for ipw in (1, 2):
+            for aff in (0, 1):
+                self.args.items_per_wave, self.args.xcd_affinity = ipw, aff
+                self.start(tol)
+                t = self.time_step("primal", reps) + self.time_step("dual", reps)
+                if best is None or t < best[0]:
+                    best = (t, ipw, aff)
+        self.args.items_per_wave, self.args.xcd_affinity = best[1], best[2]
+        self.items_per_wave = best[1]
+        self.drop_graphs()
+        self.stream.synchronize()
+        return {"items_per_wave": best[1], "xcd_affinity": best[2], "us_per_iteration": best[0]}
 
     # ---------------------------------------------------------------- timing
     def time_step(self, which: str, reps: int = 20) -> float:

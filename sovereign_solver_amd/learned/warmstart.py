@@ -72,6 +72,8 @@ def solve_family(fam, ids, tol: float, max_iters: int, warm=None, chunk: int = 2
         part = ids[s: s + chunk]
         fb = FamilyBatch([fam.problem(int(k)) for k in part])
         eng = FamilyEngine(fb, scheme=scheme)
+        if fb.count >= 256:
+            eng.autotune(tol)
         X, Y, w = family_warm_start(fb, None if warm is None else warm[s: s + chunk])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -219,6 +221,8 @@ def solve_learned(fam, ids, ls: LearnedStart, tol: float, max_iters: int, use_x:
         part = ids[s: s + chunk]
         fb = FamilyBatch([fam.problem(int(k)) for k in part])
         eng = FamilyEngine(fb, scheme=scheme)
+        if fb.count >= 256:
+            eng.autotune(tol)
         feats = fam.features_of(part)
         torch.cuda.synchronize()
         t0 = time.perf_counter()

@@ -121,6 +121,8 @@ def solve_lp_pdhg_batch(
     if live and engine == "family":
         fb = FamilyBatch([problems[k] for k in live])
         eng = FamilyEngine(fb, scheme=scheme, check_every=check_every)
+        if fb.count >= 256:                      # launch shape by timing; a few ms per engine
+            eng.autotune(tol)
         ws = None if warm_start is None else [warm_start[k] for k in live]
         X, Y, w = family_warm_start(fb, ws)
         o, xs, ys = eng.solve(tol, max_iters, X, Y, w)

@@ -128,3 +128,23 @@ def test_family_device_start_matches_host_start(gpu):
     # the host path rounds x0 / y0 / weight through FP64 first: agree to a few checks
     assert np.all(np.abs(a[0]["iterations"] - b[0]["iterations"]) <= 100)
     assert np.allclose(val, val32, rtol=1e-5, atol=1e-5)
+
+
+def test_family_xcd_affinity_and_long_split_agree_bitwise(gpu):
+    """Placement of workgroups (XCD affinity) changes no arithmetic; splitting
+    rows over a workgroup changes only their summation order."""
+    sc = SetCoverNodes(rows=100, cols=1000, density=0.03)
+    probs = [sc.node(k) for k in range(150)]                # 3 tiles (fewer than 8 XCDs)
+    fb = FamilyBatch(probs, long_threshold=16)
+    assert fb.long_rows > 0
+    a = FamilyEngine(fb, xcd_affinity=False).solve(1e-4, 100_000)
+    b = FamilyEngine(fb, xcd_affinity=True).solve(1e-4, 100_000)
+    assert np.array_equal(a[0]["iterations"], b[0]["iterations"])
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    fam = DispatchFamily(G=12, T=5)
+    fb2 = FamilyBatch([fam.problem(k) for k in range(600)])   # 10 tiles (more than 8 XCDs)
+    c = FamilyEngine(fb2, xcd_affinity=False).solve(1e-4, 100_000)
+    d = FamilyEngine(fb2, xcd_affinity=True).solve(1e-4, 100_000)
+    assert np.array_equal(c[1], d[1]) and np.array_equal(c[0]["iterations"], d[0]["iterations"])
+    r = solve_lp_pdhg_batch(probs, tol=1e-4, engine="family")
+    _check(probs, r, n_highs=3)
