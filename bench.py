@@ -180,7 +180,7 @@ def bench_batch(args):
         ok_plain += r["status"] == "optimal"
         se.drop_graphs()
         del se
-    plain = ok_plain / t_plain
+    plain = ok_plain / t_plain if t_plain > 0 else 0.0
     log(f"plain single-instance GPU PDHG: {ok_plain}/{len(sub)} optimal, {plain:.2f} LPs/s")
 
     cpu = None

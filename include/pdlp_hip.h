@@ -93,6 +93,8 @@ typedef struct PdlpCtrl {
     double err_prev;         /* candidate KKT at the previous check (+inf after a restart) */
     int32_t cand;            /* candidate of the pending restart: 0 current, 1 average */
     int32_t last_restart;    /* iteration of the last restart                   */
+    double weight0;          /* starting primal weight (tau_scale at pdlp_init); the adaptive
+                                weight stays within 1e4x of it                  */
 } PdlpCtrl;
 
 /*
@@ -127,6 +129,18 @@ typedef struct PdlpDev {
     PdlpCtrl *ctrl;
     void *x_last, *y_last;    /* n, m; prec: last restart point (adaptive scheme; may be
                                  NULL for the reference scheme) */
+    /* optional SELL-64 copy of some rows of K for the dual step (PdlpTune.sell): slice s
+       holds 64 rows, one per lane, its entries column-major (entry k of lane l at
+       sell_ptr[s] + 64 k + l), so rows whose k-th columns are neighbours (the demand
+       rows of a transportation LP) gather xbar in whole cache lines; the other rows
+       stay with the CSR kernel through csr_rows.  Summation order = CSR order. */
+    int32_t sell_slices;      /* 0: no SELL rows                                    */
+    int32_t n_csr_rows;       /* entries of csr_rows                                */
+    const int32_t *sell_ptr;  /* sell_slices + 1                                    */
+    const int32_t *sell_rows; /* 64 * sell_slices: row of each lane, -1 = padding    */
+    const int32_t *sell_col;  /* sell_ptr[sell_slices]: columns (padding 0)          */
+    const void *sell_val;     /* same length, prec (padding 0)                      */
+    const int32_t *csr_rows;  /* the rows outside the slices                        */
 } PdlpDev;
 
 /* Launch shape, chosen by the autotuner (sovereign_solver_amd.engine.tune). */
@@ -142,7 +156,9 @@ typedef struct PdlpTune {
                              agreement with SciPy/NumPy in PDLP_F64); needs tpr 1 */
     int32_t block_dual;   /* threads per workgroup of the dual step for tpr_dual <= 64;
                              0 = same as block */
-    int32_t scheme;       /* PDLP_SCHEME_REFERENCE or PDLP_SCHEME_ADAPTIVE */
+    int32_t scheme;       /* PDLP_SCHEME_* restart scheme */
+    int32_t sell;         /* 1: dual step on the SELL-64 slices of PdlpDev (if any) */
+    int32_t reserved;
 } PdlpTune;
 
 int         pdlp_abi_version(void);
@@ -312,7 +328,7 @@ typedef struct PdlpFamilyLP {
     double norm_b, norm_c;  /* reference-form normalisers                       */
     double ref_err, err_prev, w_div;
     float weight;           /* primal weight: tau = w tau_hat, sigma = sigma_hat / w */
-    int32_t pad;
+    float weight0;          /* starting weight; the adaptive weight stays within 1e4x of it */
 } PdlpFamilyLP;
 
 typedef struct PdlpFamily {

@@ -350,7 +350,7 @@ __device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red) {
         if (restart) {
             if (v.adaptive && !v.fixed_weight && kk.dx > 1e-10 && kk.dy > 1e-10) {
                 const double lw = 0.5 * log(kk.dx / kk.dy) + 0.5 * log((double)wgt);
-                wgt = (float)fmin(fmax(exp(lw), 1e-8), 1e8);
+                wgt = (float)clamp_weight(exp(lw), (double)v.weight);
             }
             ref_err = kk.err;
             err_prev = INFINITY;

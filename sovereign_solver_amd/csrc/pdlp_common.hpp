@@ -12,6 +12,18 @@ namespace pdlp {
 
 constexpr int kWave = 64;
 
+// The adaptive primal weight stays within [w0 / kWeightRange, w0 * kWeightRange]
+// of its starting value w0.  With FP32 iterates an extreme weight (a tiny tau
+// and a huge sigma, or the reverse) amplifies the rounding of the residual in
+// the dual (primal) step until progress stalls (seen on small, sparse LPs
+// whose weight ran to 1e-5).  Four decades either way leave room for starting
+// estimates that are far off (set-cover relaxations settle near 3,000x theirs).
+constexpr double kWeightRange = 1.0e4;
+
+__host__ __device__ inline double clamp_weight(double w, double w0) {
+    return fmin(fmax(w, w0 / kWeightRange), w0 * kWeightRange);
+}
+
 // Thread-local message behind pdlp_last_error().
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);

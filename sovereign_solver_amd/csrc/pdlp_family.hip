@@ -476,7 +476,7 @@ __global__ void __launch_bounds__(NS2 * T) decide_kernel(PdlpFamily f, int mode,
                     if (restart) {
                         if (adaptive && !fixed_weight && ka.dx > 1e-10 && ka.dy > 1e-10) {
                             const double lw = 0.5 * log(ka.dx / ka.dy) + 0.5 * log((double)s.weight);
-                            s.weight = (float)fmin(fmax(exp(lw), 1e-8), 1e8);
+                            s.weight = (float)clamp_weight(exp(lw), (double)s.weight0);
                         }
                         s.ref_err = ka.err;
                         s.err_prev = INFINITY;
@@ -669,7 +669,11 @@ __global__ void __launch_bounds__(T) set_weight_kernel(PdlpFamily f, const float
     float v = src[(int64_t)k * ld + w_col];
     if (scale) v *= scale[w_col];
     if (shift) v += shift[w_col];
-    if (isfinite(v)) f.lp[k].weight = (float)exp(fmin(fmax((double)v, -8.0), 8.0));
+    if (isfinite(v)) {
+        const float w = (float)exp(fmin(fmax((double)v, -8.0), 8.0));
+        f.lp[k].weight = w;
+        f.lp[k].weight0 = w;
+    }
 }
 
 __global__ void __launch_bounds__(T) clear_mask_kernel(PdlpFamily f) {

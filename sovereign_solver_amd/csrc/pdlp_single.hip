@@ -90,23 +90,9 @@ __global__ void __launch_bounds__(1024) primal_kernel(PdlpDev d, int remap) {
 }
 
 // ------------------------------------------------------------------ dual step
-template <typename T, int TPR, bool EXACT>
-__global__ void __launch_bounds__(1024) dual_kernel(PdlpDev d, int remap) {
-    PdlpCtrl* ctrl = d.ctrl;
-    if (ctrl->status != PDLP_RUNNING) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {   // single writer; read only by later kernels
-        ctrl->iter += 1;
-        ctrl->window += 1;
-    }
-    const T sigma_scale = (T)ctrl->sigma_scale;
-    const T* __restrict__ xbar = (const T*)d.xbar;
-    int i, lane;
-    group_of<TPR>(remap, i, lane);
-    if (i >= d.m) return;
-    T ax, asum;
-    any_row_dot<TPR, EXACT, T>(d.row_ptr, d.col_idx, (const T*)d.val, i, lane,
-                               [&](int k) { return xbar[k]; }, ax, asum);
-    if (lane != 0) return;
+// Dual update of row i from ax = K_i xbar and asum = sum_j |K_ij|.
+template <typename T, bool EXACT>
+__device__ __forceinline__ void dual_update(const PdlpDev& d, T sigma_scale, int i, T ax, T asum) {
     const T* b = (const T*)d.b;
     T* y = (T*)d.y;
     const T rowsum = asum > T(1e-4) ? asum : T(1e-4);
@@ -119,6 +105,62 @@ __global__ void __launch_bounds__(1024) dual_kernel(PdlpDev d, int remap) {
     y[i] = yn;
     if (d.y_psum != nullptr) d.y_psum[i] += (float)yn;
     else d.y_sum[i] += (double)yn;
+}
+
+// CSR dual step over all rows (use_list = 0) or over csr_rows (the rows the
+// SELL slices leave).  Block 0 counts the iteration.
+template <typename T, int TPR, bool EXACT>
+__global__ void __launch_bounds__(1024) dual_kernel(PdlpDev d, int remap, int use_list) {
+    PdlpCtrl* ctrl = d.ctrl;
+    if (ctrl->status != PDLP_RUNNING) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {   // single writer; read only by later kernels
+        ctrl->iter += 1;
+        ctrl->window += 1;
+    }
+    const T sigma_scale = (T)ctrl->sigma_scale;
+    const T* __restrict__ xbar = (const T*)d.xbar;
+    int i, lane;
+    group_of<TPR>(remap, i, lane);
+    if (use_list) {
+        if (i >= d.n_csr_rows) return;
+        i = d.csr_rows[i];
+    } else if (i >= d.m) {
+        return;
+    }
+    T ax, asum;
+    any_row_dot<TPR, EXACT, T>(d.row_ptr, d.col_idx, (const T*)d.val, i, lane,
+                               [&](int k) { return xbar[k]; }, ax, asum);
+    if (lane != 0) return;
+    dual_update<T, EXACT>(d, sigma_scale, i, ax, asum);
+}
+
+// SELL-64 dual step: one wavefront per slice, one row per lane; entry k of
+// every lane is read in one coalesced 256-byte (FP32) access, and the
+// gathers of neighbouring columns share cache lines.  Entries are summed in
+// CSR order, as the one-lane CSR kernel does.
+template <typename T>
+__global__ void __launch_bounds__(256) dual_sell_kernel(PdlpDev d) {
+    const PdlpCtrl* ctrl = d.ctrl;
+    if (ctrl->status != PDLP_RUNNING) return;
+    const int s = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+    if (s >= d.sell_slices) return;
+    const int lane = threadIdx.x & 63;
+    const T sigma_scale = (T)ctrl->sigma_scale;
+    const T* __restrict__ xbar = (const T*)d.xbar;
+    const T* __restrict__ val = (const T*)d.sell_val;
+    const int32_t* __restrict__ col = d.sell_col;
+    const int beg = d.sell_ptr[s], width = (d.sell_ptr[s + 1] - beg) >> 6;
+    T ax = T(0), asum = T(0);
+    int q = beg + lane;
+#pragma unroll 8
+    for (int k = 0; k < width; ++k, q += 64) {
+        const T a = val[q];
+        ax += a * xbar[col[q]];
+        asum += fabs(a);
+    }
+    const int i = d.sell_rows[s * 64 + lane];
+    if (i < 0) return;
+    dual_update<T, false>(d, sigma_scale, i, ax, asum);
 }
 
 // ------------------------------------------------------------- KKT partials
@@ -315,7 +357,7 @@ __global__ void __launch_bounds__(256) kkt_finalize_kernel(PdlpDev d, int nb_row
         // primal weight w (tau = w tau_hat, sigma = sigma_hat / w): geometric mean of the
         // old weight and the distance ratio (from the start point, or since the last restart)
         const double lw = 0.5 * log(k.dx / k.dy) + 0.5 * log(ctrl->tau_scale);
-        const double wt = fmin(fmax(exp(lw), 1e-8), 1e8);
+        const double wt = clamp_weight(exp(lw), ctrl->weight0);
         ctrl->tau_scale = wt;
         ctrl->sigma_scale = 1.0 / wt;
     }
@@ -405,6 +447,7 @@ __global__ void init_kernel(PdlpDev d) {
         c->cand = 1;
         c->last_restart = 0;
         c->err_prev = INFINITY;
+        c->weight0 = c->tau_scale;
     }
 }
 
@@ -445,6 +488,11 @@ int check_dev(const PdlpDev* d) {
         return fail(PDLP_E_ARG, "x_psum and y_psum must both be set or both be NULL");
     if (d->x_psum != nullptr && d->prec != PDLP_F32)
         return fail(PDLP_E_ARG, "partial sums are for PDLP_F32 only");
+    if (d->sell_slices < 0 || d->n_csr_rows < 0 || d->n_csr_rows > d->m)
+        return fail(PDLP_E_SHAPE, "sell_slices >= 0 and n_csr_rows in [0, m]");
+    if (d->sell_slices > 0 && (!d->sell_ptr || !d->sell_rows || !d->sell_col || !d->sell_val ||
+                               (d->n_csr_rows > 0 && !d->csr_rows)))
+        return fail(PDLP_E_ARG, "SELL slices need sell_ptr, sell_rows, sell_col, sell_val and csr_rows");
     return 0;
 }
 
@@ -461,6 +509,7 @@ int check_tune(const PdlpTune* t) {
         return fail(PDLP_E_TUNE, "exact mode needs tpr_primal = tpr_dual = 1");
     if (t->scheme < PDLP_SCHEME_REFERENCE || t->scheme > PDLP_SCHEME_ADAPTIVE_START)
         return fail(PDLP_E_TUNE, "scheme must be one of the PDLP_SCHEME_* values");
+    if (t->sell != 0 && t->sell != 1) return fail(PDLP_E_TUNE, "sell must be 0 or 1");
     if (t->exact && t->scheme != PDLP_SCHEME_REFERENCE)
         return fail(PDLP_E_TUNE, "exact mode reproduces the reference scheme only");
     return 0;
@@ -510,13 +559,17 @@ int launch_primal_t(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
 template <typename T, bool EXACT>
 int launch_dual_t(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
     const int remap = (t.xcd_remap >> 1) & 1;
+    const bool sell = !EXACT && t.sell && d.sell_slices > 0;
     unsigned g, nt;
-    shape_for(d.m, t.tpr_dual, (t.block_dual > 0 ? t.block_dual : t.block), remap, g, nt);
+    // with SELL slices the CSR kernel covers csr_rows only (at least one workgroup: it counts the iteration)
+    shape_for(sell ? std::max(d.n_csr_rows, 1) : d.m, t.tpr_dual, (t.block_dual > 0 ? t.block_dual : t.block),
+              remap, g, nt);
     if constexpr (EXACT) {
-        dual_kernel<T, 1, true><<<g, nt, 0, s>>>(d, remap);
+        dual_kernel<T, 1, true><<<g, nt, 0, s>>>(d, remap, 0);
     } else {
+        if (sell) dual_sell_kernel<T><<<(unsigned)((d.sell_slices + 3) / 4), 256, 0, s>>>(d);
         int rc = with_tpr(t.tpr_dual, [&](auto k) {
-            dual_kernel<T, decltype(k)::value, false><<<g, nt, 0, s>>>(d, remap);
+            dual_kernel<T, decltype(k)::value, false><<<g, nt, 0, s>>>(d, remap, sell ? 1 : 0);
             return 0;
         });
         if (rc) return rc;

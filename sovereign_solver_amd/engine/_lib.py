@@ -39,7 +39,7 @@ class PdlpCtrl(C.Structure):
         ("ref_err", C.c_double), ("last_err", C.c_double), ("rp", C.c_double), ("rd", C.c_double),
         ("gap", C.c_double), ("tol", C.c_double), ("norm_b", C.c_double), ("norm_c", C.c_double),
         ("tau_scale", C.c_double), ("sigma_scale", C.c_double), ("p_obj", C.c_double), ("d_obj", C.c_double),
-        ("err_prev", C.c_double), ("cand", C.c_int32), ("last_restart", C.c_int32),
+        ("err_prev", C.c_double), ("cand", C.c_int32), ("last_restart", C.c_int32), ("weight0", C.c_double),
     ]
 
 
@@ -52,6 +52,8 @@ class PdlpDev(C.Structure):
         ("c64", _p), ("b64", _p), ("b_shift", _p), ("lo_shift", _p), ("ub_span", _p),
         ("x", _p), ("xbar", _p), ("y", _p), ("x_sum", _p), ("y_sum", _p),
         ("x_psum", _p), ("y_psum", _p), ("partials", _p), ("ctrl", _p), ("x_last", _p), ("y_last", _p),
+        ("sell_slices", C.c_int32), ("n_csr_rows", C.c_int32), ("sell_ptr", _p), ("sell_rows", _p),
+        ("sell_col", _p), ("sell_val", _p), ("csr_rows", _p),
     ]
 
 
@@ -59,7 +61,7 @@ class PdlpTune(C.Structure):
     _fields_ = [
         ("tpr_primal", C.c_int32), ("tpr_dual", C.c_int32), ("block", C.c_int32),
         ("check_every", C.c_int32), ("xcd_remap", C.c_int32), ("exact", C.c_int32), ("block_dual", C.c_int32),
-        ("scheme", C.c_int32),
+        ("scheme", C.c_int32), ("sell", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -83,7 +85,7 @@ class PdlpFamilyLP(C.Structure):
         ("status", C.c_int32), ("iter", C.c_int32), ("window", C.c_int32), ("restarts", C.c_int32),
         ("checks", C.c_int32), ("done_iter", C.c_int32), ("last_restart", C.c_int32), ("action", C.c_int32),
         ("norm_b", C.c_double), ("norm_c", C.c_double), ("ref_err", C.c_double), ("err_prev", C.c_double),
-        ("w_div", C.c_double), ("weight", C.c_float), ("pad", C.c_int32),
+        ("w_div", C.c_double), ("weight", C.c_float), ("weight0", C.c_float),
     ]
 
 

@@ -297,7 +297,7 @@ class FamilyEngine:
     LP_DTYPE = np.dtype([(n_, "<i4") for n_ in ("status", "iter", "window", "restarts", "checks", "done_iter",
                                                 "last_restart", "action")] +
                         [(n_, "<f8") for n_ in ("norm_b", "norm_c", "ref_err", "err_prev", "w_div")] +
-                        [("weight", "<f4"), ("pad", "<i4")])
+                        [("weight", "<f4"), ("weight0", "<f4")])
 
     def _reset(self, tol: float, weight: Optional[np.ndarray]) -> None:
         """Slot bookkeeping and per-LP records for a new solve (original slot order)."""
@@ -316,7 +316,7 @@ class FamilyEngine:
             w = np.where(np.isfinite(ww), ww, w)
         lp = np.zeros(fb.count, dtype=self.LP_DTYPE)
         assert lp.dtype.itemsize == C.sizeof(L.PdlpFamilyLP)
-        lp["norm_b"], lp["norm_c"], lp["weight"] = fb.norm_b, fb.norm_c, w
+        lp["norm_b"], lp["norm_c"], lp["weight"], lp["weight0"] = fb.norm_b, fb.norm_c, w, w
         with torch.cuda.stream(self.stream):
             self.t["lp"].copy_(torch.from_numpy(lp.view(np.uint8).copy()).to(self.device))
 

@@ -1,0 +1,100 @@
+"""SELL-64 row slices of a CSR matrix for the dual step.
+
+A slice holds 64 consecutive rows, one per lane of a wavefront, with its
+entries stored column-major (entry k of lane l at ``ptr[s] + 64 k + l``), so
+a wavefront reads entry k of all 64 rows in one coalesced access.  A slice
+pays off when, for most k, the 64 rows' k-th columns are neighbours — their
+gathers of xbar then share cache lines (the demand rows of a transportation
+LP: row j's k-th entry is column k·S + j).  Rows whose slice does not pay off
+stay with the CSR kernel.  Format: Monakov et al. 2010 (sliced ELLPACK),
+Kreutzer et al. 2014 (SELL-C-σ), here with C = 64 and σ = 1 (no sorting:
+consecutive rows keep the neighbour property).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import scipy.sparse as sp
+
+SLICE = 64
+
+
+@dataclass
+class SellSlices:
+    ptr: np.ndarray        # int32, slices + 1: first entry of each slice
+    rows: np.ndarray       # int32, 64 * slices: row of each lane, -1 = padding
+    col: np.ndarray        # int32, entries (padding 0)
+    val: np.ndarray        # float64, entries (padding 0)
+    csr_rows: np.ndarray   # int32: rows outside the slices
+
+    @property
+    def slices(self) -> int:
+        return len(self.ptr) - 1
+
+
+def _windows(rl: np.ndarray):
+    """Candidate slices: 64-row windows inside each run of rows of equal length
+    (at least 32 rows; a shorter last window is padded), then 64-row windows
+    over the remaining rows in order."""
+    m = rl.size
+    change = np.flatnonzero(np.diff(rl)) + 1
+    starts = np.concatenate([[0], change])
+    ends = np.concatenate([change, [m]])
+    used = np.zeros(m, bool)
+    out = []
+    for s0, e0 in zip(starts, ends):
+        if e0 - s0 < SLICE:
+            continue
+        for w in range(s0, e0, SLICE):
+            r = np.arange(w, min(w + SLICE, e0))
+            if r.size >= SLICE // 2:
+                out.append(r)
+                used[r] = True
+    rest = np.flatnonzero(~used)
+    for w in range(0, rest.size - SLICE // 2 + 1, SLICE):
+        out.append(rest[w: w + SLICE])
+    return out
+
+
+def build(K: sp.csr_matrix, line_elems: int = 32, max_lines: float = 4.0, max_pad: float = 1.25) -> SellSlices:
+    """Slices of up to 64 rows whose k-th columns span at most ``max_lines``
+    cache lines of ``line_elems`` elements on average, with at most ``max_pad``
+    padding (width x 64 over the slice's entries); the other rows stay CSR."""
+    m = K.shape[0]
+    indptr, indices, data = K.indptr, K.indices, K.data
+    rl = np.diff(indptr)
+    ptr = [0]
+    rows_out, col_out, val_out = [], [], []
+    in_slice = np.zeros(m, bool)
+    for r in _windows(rl):
+        lens = rl[r]
+        if lens.min() == 0:
+            continue
+        W = int(lens.max())
+        if W * SLICE > max_pad * lens.sum():
+            continue
+        k = np.arange(W)
+        valid = k[None, :] < lens[:, None]                                  # [rows, W]
+        pos = np.minimum(indptr[r][:, None] + k[None, :], len(indices) - 1)
+        cols = np.where(valid, indices[pos], -1)
+        lines = np.sort(np.where(valid, cols // line_elems, -1), axis=0)
+        distinct = 1 + (np.diff(lines, axis=0) != 0).sum(axis=0)           # per k (padding counts once)
+        if distinct.mean() > max_lines:
+            continue
+        pad = SLICE - r.size                                                  # lanes without a row
+        in_slice[r] = True
+        rows_out.append(np.concatenate([r, np.full(pad, -1)]).astype(np.int32))
+        c = np.zeros((SLICE, W), np.int32)
+        v = np.zeros((SLICE, W))
+        c[: r.size] = np.where(valid, cols, 0)
+        v[: r.size] = np.where(valid, data[pos], 0.0)
+        col_out.append(c.T.ravel())                                           # column-major: [W, 64]
+        val_out.append(v.T.ravel())
+        ptr.append(ptr[-1] + W * SLICE)
+    if len(ptr) == 1:
+        return SellSlices(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0),
+                          np.arange(m, dtype=np.int32))
+    return SellSlices(np.asarray(ptr, np.int32), np.concatenate(rows_out), np.concatenate(col_out),
+                      np.concatenate(val_out), np.flatnonzero(~in_slice).astype(np.int32))

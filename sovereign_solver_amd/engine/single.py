@@ -18,6 +18,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
+from . import sell
 from .device import dev_tensor, ptr, require_gpu, stream_handle, struct_from_bytes
 from .forms import HostForm
 from .tune import Tune, heuristic_tune
@@ -86,6 +87,20 @@ class SingleEngine:
             # last restart point (adaptive scheme)
             self.x_last = torch.zeros(self.n, dtype=T, device=dv)
             self.y_last = torch.zeros(self.m, dtype=T, device=dv)
+            # SELL-64 slices of the rows whose gathers coalesce (not in exact mode: the
+            # reference trajectory needs the CSR summation of one lane per row)
+            sl = None if exact else sell.build(K)
+            self.sell_slices = 0 if sl is None else sl.slices
+            if self.sell_slices:
+                self.sell_ptr = dev_tensor(sl.ptr, i32, dv)
+                self.sell_rows = dev_tensor(sl.rows, i32, dv)
+                self.sell_col = dev_tensor(sl.col, i32, dv)
+                self.sell_val = dev_tensor(sl.val, T, dv)
+                self.csr_rows = dev_tensor(sl.csr_rows, i32, dv) if sl.csr_rows.size else None
+                self.n_csr_rows = int(sl.csr_rows.size)
+            else:
+                self.sell_ptr = self.sell_rows = self.sell_col = self.sell_val = self.csr_rows = None
+                self.n_csr_rows = 0
         self.ctrl_host = torch.zeros((2, C.sizeof(L.PdlpCtrl)), dtype=torch.uint8).pin_memory()
         self.stream.synchronize()
         self.dev = L.PdlpDev(
@@ -96,7 +111,10 @@ class SingleEngine:
             c64=ptr(self.c64), b64=ptr(self.b64), b_shift=ptr(self.b_shift), lo_shift=ptr(self.lo_shift),
             ub_span=ptr(self.ub_span), x=ptr(self.x), xbar=ptr(self.xbar), y=ptr(self.y),
             x_sum=ptr(self.x_sum), y_sum=ptr(self.y_sum), x_psum=ptr(self.x_psum), y_psum=ptr(self.y_psum),
-            partials=ptr(self.partials), ctrl=ptr(self.ctrl), x_last=ptr(self.x_last), y_last=ptr(self.y_last))
+            partials=ptr(self.partials), ctrl=ptr(self.ctrl), x_last=ptr(self.x_last), y_last=ptr(self.y_last),
+            sell_slices=self.sell_slices, n_csr_rows=self.n_csr_rows, sell_ptr=ptr(self.sell_ptr),
+            sell_rows=ptr(self.sell_rows), sell_col=ptr(self.sell_col), sell_val=ptr(self.sell_val),
+            csr_rows=ptr(self.csr_rows))
         self._graphs = {}
         self._graph_tune = None
 

@@ -34,11 +34,12 @@ class Tune:
     exact: int = 0
     block_dual: int = 0
     scheme: int = 0             # 0 reference restart rule, 1 adaptive (see include/pdlp_hip.h)
+    sell: int = 0               # 1: dual step on the engine's SELL-64 row slices
 
     def to_c(self):
         from ._lib import PdlpTune
         return PdlpTune(self.tpr_primal, self.tpr_dual, self.block, self.check_every, self.xcd_remap, self.exact,
-                        self.block_dual, self.scheme)
+                        self.block_dual, self.scheme, self.sell, 0)
 
     def with_(self, **kw) -> "Tune":
         d = asdict(self)
@@ -144,6 +145,16 @@ def autotune(engine, reps: int = 20, use_cache: bool = True) -> Tune:
                 timings[f"dual/tpr{td}/block{bd}/remap{remap}"] = us
                 if us < best_dt:
                     best_d, best_dt = (td, remap, bd), us
-    t = Tune(best_p[0], best_d[0], 256, base.check_every, best_p[1] | (best_d[1] << 1), 0, best_d[2])
+    sell = 0
+    if getattr(engine, "sell_slices", 0) > 0:
+        # the SELL slices take the coalescing rows; the CSR kernel keeps the rest
+        for td in _neighbours(base.tpr_dual):
+            for remap in (0, 1):
+                t = Tune(best_p[0], td, 256, base.check_every, remap << 1, 0, 0, 0, 1)
+                us = engine.time_kernel("dual", t, reps)
+                timings[f"dual/sell/tpr{td}/remap{remap}"] = us
+                if us < best_dt:
+                    best_d, best_dt, sell = (td, remap, 0), us, 1
+    t = Tune(best_p[0], best_d[0], 256, base.check_every, best_p[1] | (best_d[1] << 1), 0, best_d[2], 0, sell)
     _store_cache(key, t, timings)
     return t.with_(scheme=engine.tune.scheme)

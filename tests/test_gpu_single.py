@@ -202,3 +202,23 @@ def test_device_resolution(gpu):
 def test_empty_problem_returns_optimal_zeros(gpu):
     r = solve_lp_pdhg(Problem.empty(3))
     assert r.status == "optimal" and r.iterations == 0 and np.array_equal(r.x, np.zeros(3))
+
+
+def test_sell_dual_step_matches_csr_bitwise(gpu):
+    """The SELL-64 dual step sums each row in CSR order, as the one-lane CSR
+    kernel does: the whole solve is bitwise the same."""
+    from sovereign_solver_amd.engine.forms import box_form
+    from sovereign_solver_amd.engine.single import SingleEngine
+    from sovereign_solver_amd.engine.tune import Tune
+    f = box_form(transportation_lp(50, 100, seed=0))
+    base = Tune(tpr_primal=1, tpr_dual=1, block=256, check_every=50)
+    a = SingleEngine(f, "fp32", tune=base).solve(1e-4, 50_000)
+    eng = SingleEngine(f, "fp32", tune=base.with_(sell=1))
+    assert eng.sell_slices >= 1 and eng.n_csr_rows > 0
+    b = eng.solve(1e-4, 50_000)
+    assert a["status"] == b["status"] == "optimal"
+    assert a["iterations"] == b["iterations"]
+    assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["y"], b["y"])
+    # the other lane counts of the CSR part agree to rounding
+    c = SingleEngine(f, "fp32", tune=base.with_(sell=1, tpr_dual=8)).solve(1e-4, 50_000)
+    assert c["status"] == "optimal" and abs(c["iterations"] - a["iterations"]) <= 500
