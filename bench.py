@@ -38,6 +38,18 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(key: str):
+    """HBM bytes per launch of a kernel on this workload, as measured by the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded in profiles/traffic.json
+    (None when that workload was not profiled)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            rec = json.load(f).get(key)
+    except (OSError, ValueError):
+        return None
+    return None if rec is None else float(rec["bytes"])
+
+
 def primal_bytes(form, elem: int) -> float:
     """Algorithmic bytes of one fused primal-step launch."""
     n, m, nnz = form.n, form.m, form.nnz
@@ -220,7 +232,8 @@ def bench_batch(args):
                    "kernel_s": times[-1][1]},
         "roofline": {"bound": "hbm", "kernel": "batch_kernel (persistent, one pass over each LP's CSR per iteration)",
                      "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_launch": bytes_alg, "us_per_launch": times[-1][1] * 1e6},
+                     "traffic": measured_traffic(f"uneven_{args.batch}/batch_kernel"),
+                     "bytes_per_launch": bytes_alg, "us_per_launch": times[-1][1] * 1e6},
         "cpu_baseline": cpu,
     }
     if args.dump:
@@ -282,7 +295,7 @@ def cpu_rate(probs, tol, cap_s):
                        f"same workload, solved to {tol:g} one after another in {tc:.1f}s on 1 core (BLAS threads 1)")}
 
 
-def family_roofline(probs, tol):
+def family_roofline(probs, tol, traffic_key=None):
     """Graph-timed column and row passes of the family engine over all LPs at
     the cold-start state (every LP running): algorithmic bytes / mean time."""
     from sovereign_solver_amd.engine.family import FamilyBatch, FamilyEngine
@@ -301,7 +314,10 @@ def family_roofline(probs, tol):
         f"row {us['dual']:.1f} us ({b['dual'] / 1e6:.1f} MB, {gbs['dual']:.0f} GB/s)")
     other = "dual" if dom == "primal" else "primal"
     return {"bound": "hbm", "kernel": names[dom], "achieved": gbs[dom], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": gbs[dom] / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": b[dom], "us_per_launch": us[dom],
+            "frac": gbs[dom] / HBM_PEAK_GBS,
+            "traffic": measured_traffic(f"{traffic_key}/{'column' if dom == 'primal' else 'row'}_pass")
+            if traffic_key else None,
+            "bytes_per_launch": b[dom], "us_per_launch": us[dom],
             "launch_shape": shape,
             "other_pass": {"kernel": names[other], "achieved": gbs[other], "bytes_per_launch": b[other],
                            "us_per_launch": us[other]}}
@@ -337,7 +353,7 @@ def bench_learned(args):
     log(f"trained in {t_train:.1f}s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
     ls = W.LearnedStart(mlp, nin, nout)
     probs = [fam.problem(int(k)) for k in eval_ids]
-    roof = family_roofline(probs, args.tol)
+    roof = family_roofline(probs, args.tol, f"dispatch_{len(probs)}")
 
     W.solve_family(fam, eval_ids[:64], args.tol, args.max_iters)                      # warm-up (build, graphs)
     cold = family_rate(*W.solve_family(fam, eval_ids, args.tol, args.max_iters))
@@ -402,7 +418,7 @@ def bench_nodes(args):
     node_ids = np.arange(args.nodes)
     train_ids = 10**6 + np.arange(args.train)
     probs = [sc.node(int(k)) for k in node_ids]
-    roof = family_roofline(probs, args.tol)
+    roof = family_roofline(probs, args.tol, f"setcover_{len(probs)}")
     W.solve_family(sc, node_ids[:64], args.tol, args.max_iters)                       # warm-up
     cold = family_rate(*W.solve_family(sc, node_ids, args.tol, args.max_iters))
     parent_ws = [(np.clip(root.x, p.lb, p.ub), root.y, root.step_scale) for p in probs]
@@ -611,7 +627,8 @@ def main():
                    "reference_form_iterations": ref_iters},
         "roofline": {"bound": "hbm", "kernel": "primal_kernel (fused K^T y + step + projection + sums)",
                      "achieved": gbs_p, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs_p / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_launch": bp, "us_per_launch": us_p,
+                     "traffic": measured_traffic(f"transport_{args.sources}x{args.sinks}/primal_kernel"),
+                     "bytes_per_launch": bp, "us_per_launch": us_p,
                      "dual_kernel": {"achieved": gbs_d, "bytes_per_launch": bd, "us_per_launch": us_d}},
         "cpu_baseline": cpu,
     }
