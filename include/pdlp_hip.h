@@ -141,6 +141,11 @@ typedef struct PdlpDev {
     const int32_t *sell_col;  /* sell_ptr[sell_slices]: columns (padding 0)          */
     const void *sell_val;     /* same length, prec (padding 0)                      */
     const int32_t *csr_rows;  /* the rows outside the slices                        */
+    int32_t sell_max_width;   /* largest slice width (entries per lane)              */
+    int32_t sell_reserved;
+    void *sell_part;          /* prec, 64 * sell_slices * ceil(sell_max_width / 8): partial
+                                 dot products of the split SELL step (PdlpTune.sell >= 8) */
+    const void *sell_asum;    /* prec, 64 * sell_slices: sum_j |K_ij| of each lane's row */
 } PdlpDev;
 
 /* Launch shape, chosen by the autotuner (sovereign_solver_amd.engine.tune). */
@@ -157,7 +162,9 @@ typedef struct PdlpTune {
     int32_t block_dual;   /* threads per workgroup of the dual step for tpr_dual <= 64;
                              0 = same as block */
     int32_t scheme;       /* PDLP_SCHEME_* restart scheme */
-    int32_t sell;         /* 1: dual step on the SELL-64 slices of PdlpDev (if any) */
+    int32_t sell;         /* dual step on the SELL-64 slices of PdlpDev (if any): 0 off;
+                             1 one wavefront per slice; k >= 8 (multiple of 8) a wavefront
+                             per k entries of a slice, partial sums added in order */
     int32_t reserved;
 } PdlpTune;
 

@@ -308,10 +308,61 @@ __global__ void __launch_bounds__(NT) kkt_cols_kernel(PdlpFamily f, int add, int
     store_chunk(f, pl.chunk, tile, a);
 }
 
-// Rows: fold the y partial sums; primal residual (K x of both candidates, x
-// window sums folded by the column pass), dual objective, sign violation of
-// y on <= rows, dual distance.
-__global__ void __launch_bounds__(NT) kkt_rows_kernel(PdlpFamily f, int add, int P) {
+// Row i's share of the check sums, given da = K_i x_avg and dc = K_i x (FP64):
+// fold the y partial sums; primal residual, dual objective, sign violation of
+// y on <= rows, dual distance, for both candidates.
+__device__ __forceinline__ void row_terms(const PdlpFamily& f, int tile, int lane, bool act, int win, double wd,
+                                          bool adaptive, int i, double da, double dc, double (&a)[NS2]) {
+    const bool shb = f.shared & PDLP_FAM_SHARED_B64, shs = f.shared & PDLP_FAM_SHARED_BSHIFT;
+    const size_t iy = at(tile, f.m, i, lane);
+    const double ys = f.y_sum[iy] + (double)f.y_psum[iy];
+    if (act) {
+        f.y_sum[iy] = ys;
+        f.y_psum[iy] = 0.f;
+    }
+    const double yc = (double)f.y[iy];
+    const double ya = win > 0 ? ys / wd : yc;
+    const bool le = f.row_kind[i] == PDLP_ROW_LE;
+    const double b = ldv(f.b64, shb, tile, f.m, i, lane);
+    double ra = da - b, rc = dc - b;
+    if (le && ra < 0.0) ra = 0.0;
+    if (le && rc < 0.0) rc = 0.0;
+    a[0] += ra * ra;
+    a[6] += rc * rc;
+    if (le && ya > 0.0) a[1] += ya * ya;
+    if (le && yc > 0.0) a[7] += yc * yc;
+    const double bs = ldv(f.b_shift, shs, tile, f.m, i, lane);
+    a[3] += bs * ya;
+    a[9] += bs * yc;
+    if (adaptive) {
+        const double yl = (double)f.y_last[iy], rm = f.row_metric[i];
+        const double da2 = ya - yl, dc2 = yc - yl;
+        a[5] += da2 * da2 * rm;
+        a[11] += dc2 * dc2 * rm;
+    }
+}
+
+// K x of both candidates over entries [beg, end) of a row (x window sums
+// already folded by the column pass).
+__device__ __forceinline__ void row_products(const PdlpFamily& f, int tile, int lane, int win, double wd, int beg,
+                                             int end, double& da, double& dc) {
+    da = 0.0;
+    dc = 0.0;
+#pragma unroll 4
+    for (int q = beg; q < end; ++q) {
+        const double kv = (double)f.val[q];
+        const size_t jx = at(tile, f.n, f.col_idx[q], lane);
+        const double xc = (double)f.x[jx];
+        const double xa = win > 0 ? f.x_sum[jx] / wd : xc;
+        da += kv * xa;
+        dc += kv * xc;
+    }
+}
+
+// Rows of the check.  The first nl chunks each take one long row (the last nl
+// rows) on the whole workgroup, its entries split over the wavefronts; the
+// other chunks stride over the short rows.
+__global__ void __launch_bounds__(NT) kkt_rows_kernel(PdlpFamily f, int add, int P, int nl) {
     const Place pl = place(f, P);
     if (!pl.ok) return;
     const int tile = pl.tile;
@@ -323,48 +374,37 @@ __global__ void __launch_bounds__(NT) kkt_rows_kernel(PdlpFamily f, int add, int
     const int win = act ? f.lp[k].window + add : 0;   // window length including this check's iterations
     const double wd = (double)win;
     const bool adaptive = f.flags & 1;
-    const bool shb = f.shared & PDLP_FAM_SHARED_B64, shs = f.shared & PDLP_FAM_SHARED_BSHIFT;
     double a[NS2];
 #pragma unroll
     for (int s = 0; s < NS2; ++s) a[s] = 0.0;
-    for (int p = pl.chunk * NWV + w; p < f.m; p += P * NWV) {
-        const int i = f.m - 1 - p;                       // long rows first
-        const size_t iy = at(tile, f.m, i, lane);
-        const double ys = f.y_sum[iy] + (double)f.y_psum[iy];
-        if (act) {
-            f.y_sum[iy] = ys;
-            f.y_psum[iy] = 0.f;
-        }
-        const double yc = (double)f.y[iy];
-        const double ya = win > 0 ? ys / wd : yc;
-        double da = 0.0, dc = 0.0;
+    if (pl.chunk < nl) {                               // uniform over the workgroup
+        __shared__ double sp[2][NWV][T];
+        const int i = f.m - nl + pl.chunk;
         const int beg = f.row_ptr[i], end = f.row_ptr[i + 1];
-#pragma unroll 4
-        for (int q = beg; q < end; ++q) {
-            const double kv = (double)f.val[q];
-            const size_t jx = at(tile, f.n, f.col_idx[q], lane);
-            const double xc = (double)f.x[jx];
-            const double xa = win > 0 ? f.x_sum[jx] / wd : xc;
-            da += kv * xa;
-            dc += kv * xc;
+        const int per = (end - beg + NWV - 1) / NWV;
+        const int b0 = min(beg + w * per, end), e0 = min(b0 + per, end);
+        double da, dc;
+        row_products(f, tile, lane, win, wd, b0, e0, da, dc);
+        sp[0][w][lane] = da;
+        sp[1][w][lane] = dc;
+        __syncthreads();
+        if (w == 0) {
+            da = 0.0;
+            dc = 0.0;
+#pragma unroll
+            for (int v = 0; v < NWV; ++v) {
+                da += sp[0][v][lane];
+                dc += sp[1][v][lane];
+            }
+            row_terms(f, tile, lane, act, win, wd, adaptive, i, da, dc, a);
         }
-        const bool le = f.row_kind[i] == PDLP_ROW_LE;
-        const double b = ldv(f.b64, shb, tile, f.m, i, lane);
-        double ra = da - b, rc = dc - b;
-        if (le && ra < 0.0) ra = 0.0;
-        if (le && rc < 0.0) rc = 0.0;
-        a[0] += ra * ra;
-        a[6] += rc * rc;
-        if (le && ya > 0.0) a[1] += ya * ya;
-        if (le && yc > 0.0) a[7] += yc * yc;
-        const double bs = ldv(f.b_shift, shs, tile, f.m, i, lane);
-        a[3] += bs * ya;
-        a[9] += bs * yc;
-        if (adaptive) {
-            const double yl = (double)f.y_last[iy], rm = f.row_metric[i];
-            const double da2 = ya - yl, dc2 = yc - yl;
-            a[5] += da2 * da2 * rm;
-            a[11] += dc2 * dc2 * rm;
+    } else {
+        const int rows = f.m - nl;
+        for (int p = (pl.chunk - nl) * NWV + w; p < rows; p += (P - nl) * NWV) {
+            const int i = rows - 1 - p;                  // longer rows first when nl = 0
+            double da, dc;
+            row_products(f, tile, lane, win, wd, f.row_ptr[i], f.row_ptr[i + 1], da, dc);
+            row_terms(f, tile, lane, act, win, wd, adaptive, i, da, dc, a);
         }
     }
     store_chunk(f, KCH + pl.chunk, tile, a);
@@ -741,10 +781,12 @@ void launch_iterations(const PdlpFamily& f, int iters, hipStream_t s) {
 }
 
 void launch_check(const PdlpFamily& f, int mode, hipStream_t s) {
-    const int cc = chunks(f.n), rc = chunks(f.m);
+    // long rows get a workgroup each while they take at most half the chunks
+    const int nl = f.long_rows <= KCH / 2 ? f.long_rows : 0;
+    const int cc = chunks(f.n), rc = nl + std::min(chunks(f.m - nl), KCH - nl);
     const int add = mode == 0 ? 0 : f.check_every;
     kkt_cols_kernel<<<tile_grid(f, cc), NT, 0, s>>>(f, add, cc);
-    kkt_rows_kernel<<<tile_grid(f, rc), NT, 0, s>>>(f, add, rc);
+    kkt_rows_kernel<<<tile_grid(f, rc), NT, 0, s>>>(f, add, rc, nl);
     decide_kernel<<<f.tiles, NS2 * T, 0, s>>>(f, mode, cc, rc);
     apply_kernel<<<vec_grid(f), NT, 0, s>>>(f);
     count_kernel<<<1, NT, 0, s>>>(f);

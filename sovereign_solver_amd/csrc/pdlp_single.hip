@@ -491,8 +491,9 @@ int check_dev(const PdlpDev* d) {
     if (d->sell_slices < 0 || d->n_csr_rows < 0 || d->n_csr_rows > d->m)
         return fail(PDLP_E_SHAPE, "sell_slices >= 0 and n_csr_rows in [0, m]");
     if (d->sell_slices > 0 && (!d->sell_ptr || !d->sell_rows || !d->sell_col || !d->sell_val ||
-                               (d->n_csr_rows > 0 && !d->csr_rows)))
-        return fail(PDLP_E_ARG, "SELL slices need sell_ptr, sell_rows, sell_col, sell_val and csr_rows");
+                               (d->n_csr_rows > 0 && !d->csr_rows) || !d->sell_part || !d->sell_asum ||
+                               d->sell_max_width <= 0))
+        return fail(PDLP_E_ARG, "SELL slices need sell_ptr/rows/col/val/part/asum, csr_rows and sell_max_width");
     return 0;
 }
 
@@ -509,7 +510,8 @@ int check_tune(const PdlpTune* t) {
         return fail(PDLP_E_TUNE, "exact mode needs tpr_primal = tpr_dual = 1");
     if (t->scheme < PDLP_SCHEME_REFERENCE || t->scheme > PDLP_SCHEME_ADAPTIVE_START)
         return fail(PDLP_E_TUNE, "scheme must be one of the PDLP_SCHEME_* values");
-    if (t->sell != 0 && t->sell != 1) return fail(PDLP_E_TUNE, "sell must be 0 or 1");
+    if (t->sell < 0 || (t->sell > 1 && (t->sell < 8 || (t->sell & 7))))
+        return fail(PDLP_E_TUNE, "sell must be 0, 1 or a multiple of 8");
     if (t->exact && t->scheme != PDLP_SCHEME_REFERENCE)
         return fail(PDLP_E_TUNE, "exact mode reproduces the reference scheme only");
     return 0;
@@ -556,6 +558,47 @@ int launch_primal_t(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
     return hip_fail(hipGetLastError(), "primal_kernel launch");
 }
 
+// Split SELL-64 dual step, pass 1: wavefront (slice, chunk c) sums entries
+// [c kc, (c+1) kc) of each lane's row into sell_part.
+template <typename T>
+__global__ void __launch_bounds__(256) dual_sell_part_kernel(PdlpDev d, int kc, int chunks) {
+    const PdlpCtrl* ctrl = d.ctrl;
+    if (ctrl->status != PDLP_RUNNING) return;
+    const int s = blockIdx.y;
+    const int c = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+    const int beg = d.sell_ptr[s], width = (d.sell_ptr[s + 1] - beg) >> 6;
+    const int k0 = c * kc;
+    if (c >= chunks || k0 >= width) return;
+    const int k1 = min(width, k0 + kc);
+    const int lane = threadIdx.x & 63;
+    const T* __restrict__ xbar = (const T*)d.xbar;
+    const T* __restrict__ val = (const T*)d.sell_val;
+    const int32_t* __restrict__ col = d.sell_col;
+    T ax = T(0);
+    int q = beg + k0 * 64 + lane;
+#pragma unroll 8
+    for (int k = k0; k < k1; ++k, q += 64) ax += val[q] * xbar[col[q]];
+    ((T*)d.sell_part)[((size_t)s * chunks + c) * 64 + lane] = ax;
+}
+
+// Pass 2: one wavefront per slice adds the chunks in order and updates y.
+template <typename T>
+__global__ void __launch_bounds__(256) dual_sell_finish_kernel(PdlpDev d, int kc, int chunks) {
+    const PdlpCtrl* ctrl = d.ctrl;
+    if (ctrl->status != PDLP_RUNNING) return;
+    const int s = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+    if (s >= d.sell_slices) return;
+    const int lane = threadIdx.x & 63;
+    const int width = (d.sell_ptr[s + 1] - d.sell_ptr[s]) >> 6;
+    const int used = (width + kc - 1) / kc;
+    const T* part = (const T*)d.sell_part + (size_t)s * chunks * 64 + lane;
+    T ax = T(0);
+    for (int c = 0; c < used; ++c) ax += part[(size_t)c * 64];
+    const int i = d.sell_rows[s * 64 + lane];
+    if (i < 0) return;
+    dual_update<T, false>(d, (T)ctrl->sigma_scale, i, ax, ((const T*)d.sell_asum)[s * 64 + lane]);
+}
+
 template <typename T, bool EXACT>
 int launch_dual_t(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
     const int remap = (t.xcd_remap >> 1) & 1;
@@ -567,7 +610,14 @@ int launch_dual_t(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
     if constexpr (EXACT) {
         dual_kernel<T, 1, true><<<g, nt, 0, s>>>(d, remap, 0);
     } else {
-        if (sell) dual_sell_kernel<T><<<(unsigned)((d.sell_slices + 3) / 4), 256, 0, s>>>(d);
+        if (sell && t.sell == 1) {
+            dual_sell_kernel<T><<<(unsigned)((d.sell_slices + 3) / 4), 256, 0, s>>>(d);
+        } else if (sell) {
+            const int kc = t.sell, chunks = (d.sell_max_width + kc - 1) / kc;
+            dual_sell_part_kernel<T><<<dim3((unsigned)((chunks + 3) / 4), (unsigned)d.sell_slices), 256, 0, s>>>(
+                d, kc, chunks);
+            dual_sell_finish_kernel<T><<<(unsigned)((d.sell_slices + 3) / 4), 256, 0, s>>>(d, kc, chunks);
+        }
         int rc = with_tpr(t.tpr_dual, [&](auto k) {
             dual_kernel<T, decltype(k)::value, false><<<g, nt, 0, s>>>(d, remap, sell ? 1 : 0);
             return 0;

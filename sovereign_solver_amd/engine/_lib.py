@@ -53,7 +53,8 @@ class PdlpDev(C.Structure):
         ("x", _p), ("xbar", _p), ("y", _p), ("x_sum", _p), ("y_sum", _p),
         ("x_psum", _p), ("y_psum", _p), ("partials", _p), ("ctrl", _p), ("x_last", _p), ("y_last", _p),
         ("sell_slices", C.c_int32), ("n_csr_rows", C.c_int32), ("sell_ptr", _p), ("sell_rows", _p),
-        ("sell_col", _p), ("sell_val", _p), ("csr_rows", _p),
+        ("sell_col", _p), ("sell_val", _p), ("csr_rows", _p), ("sell_max_width", C.c_int32),
+        ("sell_reserved", C.c_int32), ("sell_part", _p), ("sell_asum", _p),
     ]
 
 

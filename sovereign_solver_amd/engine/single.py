@@ -98,9 +98,19 @@ class SingleEngine:
                 self.sell_val = dev_tensor(sl.val, T, dv)
                 self.csr_rows = dev_tensor(sl.csr_rows, i32, dv) if sl.csr_rows.size else None
                 self.n_csr_rows = int(sl.csr_rows.size)
+                widths = np.diff(sl.ptr) // sell.SLICE
+                self.sell_max_width = int(widths.max())
+                # partial sums of the split step: at most ceil(width / 8) chunks per slice
+                self.sell_part = torch.zeros(sell.SLICE * self.sell_slices * ((self.sell_max_width + 7) // 8),
+                                             dtype=T, device=dv)
+                aK = abs(K)
+                rs = np.asarray(aK.sum(axis=1)).ravel()
+                self.sell_asum = dev_tensor(np.where(sl.rows >= 0, rs[np.maximum(sl.rows, 0)], 1.0), T, dv)
             else:
                 self.sell_ptr = self.sell_rows = self.sell_col = self.sell_val = self.csr_rows = None
+                self.sell_part = self.sell_asum = None
                 self.n_csr_rows = 0
+                self.sell_max_width = 0
         self.ctrl_host = torch.zeros((2, C.sizeof(L.PdlpCtrl)), dtype=torch.uint8).pin_memory()
         self.stream.synchronize()
         self.dev = L.PdlpDev(
@@ -114,7 +124,8 @@ class SingleEngine:
             partials=ptr(self.partials), ctrl=ptr(self.ctrl), x_last=ptr(self.x_last), y_last=ptr(self.y_last),
             sell_slices=self.sell_slices, n_csr_rows=self.n_csr_rows, sell_ptr=ptr(self.sell_ptr),
             sell_rows=ptr(self.sell_rows), sell_col=ptr(self.sell_col), sell_val=ptr(self.sell_val),
-            csr_rows=ptr(self.csr_rows))
+            csr_rows=ptr(self.csr_rows), sell_max_width=self.sell_max_width, sell_part=ptr(self.sell_part),
+            sell_asum=ptr(self.sell_asum))
         self._graphs = {}
         self._graph_tune = None
 

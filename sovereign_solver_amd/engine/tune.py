@@ -34,7 +34,7 @@ class Tune:
     exact: int = 0
     block_dual: int = 0
     scheme: int = 0             # 0 reference restart rule, 1 adaptive (see include/pdlp_hip.h)
-    sell: int = 0               # 1: dual step on the engine's SELL-64 row slices
+    sell: int = 0               # SELL-64 dual step: 0 off, 1 a wavefront per slice, k >= 8 per k entries
 
     def to_c(self):
         from ._lib import PdlpTune
@@ -148,13 +148,14 @@ def autotune(engine, reps: int = 20, use_cache: bool = True) -> Tune:
     sell = 0
     if getattr(engine, "sell_slices", 0) > 0:
         # the SELL slices take the coalescing rows; the CSR kernel keeps the rest
-        for td in _neighbours(base.tpr_dual):
-            for remap in (0, 1):
-                t = Tune(best_p[0], td, 256, base.check_every, remap << 1, 0, 0, 0, 1)
-                us = engine.time_kernel("dual", t, reps)
-                timings[f"dual/sell/tpr{td}/remap{remap}"] = us
-                if us < best_dt:
-                    best_d, best_dt, sell = (td, remap, 0), us, 1
+        for sv in (1, 16, 32, 64, 128):
+            for td in _neighbours(base.tpr_dual):
+                for remap in (0, 1):
+                    t = Tune(best_p[0], td, 256, base.check_every, remap << 1, 0, 0, 0, sv)
+                    us = engine.time_kernel("dual", t, reps)
+                    timings[f"dual/sell{sv}/tpr{td}/remap{remap}"] = us
+                    if us < best_dt:
+                        best_d, best_dt, sell = (td, remap, 0), us, sv
     t = Tune(best_p[0], best_d[0], 256, base.check_every, best_p[1] | (best_d[1] << 1), 0, best_d[2], 0, sell)
     _store_cache(key, t, timings)
     return t.with_(scheme=engine.tune.scheme)

@@ -222,3 +222,22 @@ def test_sell_dual_step_matches_csr_bitwise(gpu):
     # the other lane counts of the CSR part agree to rounding
     c = SingleEngine(f, "fp32", tune=base.with_(sell=1, tpr_dual=8)).solve(1e-4, 50_000)
     assert c["status"] == "optimal" and abs(c["iterations"] - a["iterations"]) <= 500
+
+
+@pytest.mark.parametrize("kc", [8, 32])
+def test_split_sell_dual_step_agrees(gpu, kc):
+    """The split SELL step (partial sums per chunk of kc entries) agrees with the
+    CSR step to rounding and reaches the same optimum."""
+    from sovereign_solver_amd.engine.forms import box_form
+    from sovereign_solver_amd.engine.single import SingleEngine
+    from sovereign_solver_amd.engine.tune import Tune
+    p = transportation_lp(50, 100, seed=0)
+    f = box_form(p)
+    base = Tune(tpr_primal=1, tpr_dual=8, block=256, check_every=50)
+    a = SingleEngine(f, "fp32", tune=base).solve(1e-4, 50_000)
+    b = SingleEngine(f, "fp32", tune=base.with_(sell=kc)).solve(1e-4, 50_000)
+    assert a["status"] == b["status"] == "optimal"
+    assert abs(a["iterations"] - b["iterations"]) <= 500
+    assert yardstick.rel_kkt(p, b["x"], b["y"])[0] <= 1.02e-4
+    ref = _highs_obj(p)
+    assert abs(float(p.c @ b["x"]) - ref) <= 2e-3 * (1 + abs(ref))
