@@ -43,6 +43,9 @@ extern "C" {
 #define PDLP_RUNNING    0
 #define PDLP_OPTIMAL    1
 #define PDLP_NUMERICAL  3    /* NaN/Inf in a KKT evaluation */
+#define PDLP_INFEASIBLE 4    /* primal infeasible (family engine): the dual iterate is a
+                                Farkas certificate, b'y - max_{lo<=x<=hi} (K'y)'x > 0 with
+                                y <= 0 on PDLP_ROW_LE rows */
 
 /* row kinds of the box form */
 #define PDLP_ROW_EQ 0        /* K_i x  = b_i, dual free     */
@@ -328,7 +331,7 @@ int pdlp_batch_solve(const PdlpBatch* batch, void* stream);
 #define PDLP_FAM_KKT_CHUNKS 128    /* partial-sum chunks per KKT pass (max)   */
 #define PDLP_FAM_LONG 64            /* default split: rows / columns with more nonzeros run on a
                                        whole workgroup (the host numbers them last)          */
-#define PDLP_FAM_NSUM 12            /* 6 sums x (window average, current)      */
+#define PDLP_FAM_NSUM 20            /* 10 sums x (window average, current)     */
 
 typedef struct PdlpFamilyLP {
     int32_t status, iter, window, restarts, checks, done_iter, last_restart, action;

@@ -29,7 +29,11 @@ namespace {
 constexpr int T = PDLP_FAM_TILE;     // LPs per tile = lanes per wavefront
 constexpr int NT = 256;              // threads per workgroup
 constexpr int NWV = NT / kWave;      // wavefronts per workgroup
-constexpr int NS2 = PDLP_FAM_NSUM;   // 6 sums x 2 candidates
+constexpr int NSC = PDLP_FAM_NSUM / 2;   // sums per candidate (window average at 0, current at NSC):
+// 0 primal residual^2, 1 dual residual^2, 2 primal objective, 3 dual objective,
+// 4 |dx|^2, 5 |dy|^2 (step-size metric), 6 b_shift'y, 7 max over the shifted box of
+// (K'y)'x, 8 sum |b_shift_i y_i|, 9 sum of the |terms| of 7 (Farkas test of y)
+constexpr int NS2 = PDLP_FAM_NSUM;
 constexpr int KCH = PDLP_FAM_KKT_CHUNKS;
 
 static_assert(T == kWave, "one tile per wavefront");
@@ -244,6 +248,15 @@ __device__ __forceinline__ void store_chunk(const PdlpFamily& f, int chunk, int 
     }
 }
 
+// max of g t over t in [lo - lo_shift, span]: lo - lo_shift is 0 for a finite
+// lower bound, -inf for none; an unbounded maximum is +inf (y is then no
+// certificate).
+__device__ __forceinline__ double support(double g, double span, bool lo_inf) {
+    if (g > 0.0) return isfinite(span) ? g * span : INFINITY;
+    if (g < 0.0 && lo_inf) return INFINITY;
+    return 0.0;
+}
+
 // Columns: fold the x partial sums; primal objective and distance of both
 // candidates; K^T y of both candidates (y window sums not folded yet: the row
 // pass folds them after this pass) for the dual residual / bound-row duals.
@@ -260,7 +273,7 @@ __global__ void __launch_bounds__(NT) kkt_cols_kernel(PdlpFamily f, int add, int
     const double wd = (double)win;
     const bool adaptive = f.flags & 1;
     const bool shc = f.shared & PDLP_FAM_SHARED_C64, shl = f.shared & PDLP_FAM_SHARED_LOSHIFT,
-               shu = f.shared & PDLP_FAM_SHARED_UBSPAN;
+               shu = f.shared & PDLP_FAM_SHARED_UBSPAN, shl2 = f.shared & PDLP_FAM_SHARED_LO;
     double a[NS2];
 #pragma unroll
     for (int s = 0; s < NS2; ++s) a[s] = 0.0;
@@ -277,12 +290,12 @@ __global__ void __launch_bounds__(NT) kkt_cols_kernel(PdlpFamily f, int add, int
         const double c = ldv(f.c64, shc, tile, f.n, j, lane);
         const double ls = ldv(f.lo_shift, shl, tile, f.n, j, lane);
         a[2] += c * (xa - ls);
-        a[8] += c * (xc - ls);
+        a[NSC + 2] += c * (xc - ls);
         if (adaptive) {
             const double xl = (double)f.x_last[ix], th = (double)f.tau_hat[j];
             const double da = xa - xl, dc = xc - xl;
             a[4] += da * da / th;
-            a[10] += dc * dc / th;
+            a[NSC + 4] += dc * dc / th;
         }
         double sa = 0.0, sc = 0.0;
         const int beg = f.t_row_ptr[j], end = f.t_row_ptr[j + 1];
@@ -299,11 +312,18 @@ __global__ void __launch_bounds__(NT) kkt_cols_kernel(PdlpFamily f, int add, int
         const double ra = fmin(c - sa, 0.0), rc = fmin(c - sc, 0.0);
         if (isfinite(span)) {
             a[3] += span * ra;
-            a[9] += span * rc;
+            a[NSC + 3] += span * rc;
         } else {
             a[1] += ra * ra;
-            a[7] += rc * rc;
+            a[NSC + 1] += rc * rc;
         }
+        // Farkas test: max over x - lo_shift in [lo - lo_shift, span] of (K'y)_j (x - lo_shift)_j
+        const bool lo_inf = isinf(ldv(f.lo, shl2, tile, f.n, j, lane));
+        const double ua = support(sa, span, lo_inf), uc = support(sc, span, lo_inf);
+        a[7] += ua;
+        a[9] += fabs(ua);
+        a[NSC + 7] += uc;
+        a[NSC + 9] += fabs(uc);
     }
     store_chunk(f, pl.chunk, tile, a);
 }
@@ -328,17 +348,21 @@ __device__ __forceinline__ void row_terms(const PdlpFamily& f, int tile, int lan
     if (le && ra < 0.0) ra = 0.0;
     if (le && rc < 0.0) rc = 0.0;
     a[0] += ra * ra;
-    a[6] += rc * rc;
+    a[NSC] += rc * rc;
     if (le && ya > 0.0) a[1] += ya * ya;
-    if (le && yc > 0.0) a[7] += yc * yc;
+    if (le && yc > 0.0) a[NSC + 1] += yc * yc;
     const double bs = ldv(f.b_shift, shs, tile, f.m, i, lane);
     a[3] += bs * ya;
-    a[9] += bs * yc;
+    a[NSC + 3] += bs * yc;
+    a[6] += bs * ya;
+    a[8] += fabs(bs * ya);
+    a[NSC + 6] += bs * yc;
+    a[NSC + 8] += fabs(bs * yc);
     if (adaptive) {
         const double yl = (double)f.y_last[iy], rm = f.row_metric[i];
         const double da2 = ya - yl, dc2 = yc - yl;
         a[5] += da2 * da2 * rm;
-        a[11] += dc2 * dc2 * rm;
+        a[NSC + 5] += dc2 * dc2 * rm;
     }
 }
 
@@ -412,7 +436,17 @@ __global__ void __launch_bounds__(NT) kkt_rows_kernel(PdlpFamily f, int add, int
 
 struct Kkt {
     double rp, rd, gap, err, po, dobj, dx, dy;
+    bool infeasible;     // the candidate's y is a Farkas certificate of primal infeasibility
 };
+
+// F(y) = b_shift'y - max_box (K'y)'(x - lo_shift) > 0 proves that no x in the box
+// satisfies the rows (y <= 0 on <= rows holds by projection).  The margin 1e-6
+// of the summed magnitudes exceeds the FP32 rounding of K's values (6e-8) and
+// the FP64 rounding of the sums, so a feasible LP is never flagged.
+__device__ __forceinline__ bool farkas(const double* s) {
+    const double F = s[6] - s[7], scale = s[8] + s[9];
+    return isfinite(F) && F > 0.0 && F > 1e-6 * scale;
+}
 
 __device__ __forceinline__ Kkt kkt_of(const double* s, double norm_b, double norm_c) {
     Kkt k;
@@ -425,6 +459,7 @@ __device__ __forceinline__ Kkt kkt_of(const double* s, double norm_b, double nor
     if (!isfinite(k.po) || !isfinite(k.dobj)) k.err = NAN;
     k.dx = sqrt(s[4]);
     k.dy = sqrt(s[5]);
+    k.infeasible = farkas(s);
     return k;
 }
 
@@ -444,17 +479,17 @@ __device__ __forceinline__ void write_out(PdlpBatchOut* o, const PdlpFamilyLP& s
     *o = r;
 }
 
-// One workgroup of 12 wavefronts per tile: wavefront q adds up sum q over the
-// chunks (lane = LP), then wavefront 0 takes one decision per LP.
-// mode 0: start (reference error of the starting point); mode 1: a check
+// One workgroup of NSC wavefronts per tile: wavefront q adds up sums q and
+// q + NSC over the chunks (lane = LP), then wavefront 0 takes one decision per
+// LP.  mode 0: start (reference error of the starting point); mode 1: a check
 // after check_every iterations.
-__global__ void __launch_bounds__(NS2 * T) decide_kernel(PdlpFamily f, int mode, int col_chunks, int row_chunks) {
+__global__ void __launch_bounds__(NSC * T) decide_kernel(PdlpFamily f, int mode, int col_chunks, int row_chunks) {
     __shared__ double red[NS2][T];
-    const int tile = blockIdx.x, lane = lane_id(), q = wave_id();
+    const int tile = blockIdx.x, lane = lane_id(), w = wave_id();
     const int k = tile * T + lane;
     const uint64_t mask = f.run_mask[tile];
     const bool act = (mask >> lane) & 1ull;
-    {
+    for (int q = w; q < NS2; q += NSC) {
         const size_t cpad = (size_t)f.tiles * T;
         double t = 0.0;
         if (act) {
@@ -466,7 +501,7 @@ __global__ void __launch_bounds__(NS2 * T) decide_kernel(PdlpFamily f, int mode,
         red[q][lane] = t;
     }
     __syncthreads();
-    if (q != 0) return;                                   // no barrier below
+    if (w != 0) return;                                   // no barrier below
     bool running = false;
     if (k < f.count) {
         PdlpFamilyLP s = f.lp[k];
@@ -478,25 +513,31 @@ __global__ void __launch_bounds__(NS2 * T) decide_kernel(PdlpFamily f, int mode,
             for (int v = 0; v < NS2; ++v) sum[v] = red[v][lane];
             const bool adaptive = f.flags & 1, fixed_weight = f.flags & 2;
             Kkt ka = kkt_of(sum, s.norm_b, s.norm_c);
-            const Kkt kc = kkt_of(sum + 6, s.norm_b, s.norm_c);
+            const Kkt kc = kkt_of(sum + NSC, s.norm_b, s.norm_c);
             s.action = ACT_NONE;
             if (mode == 0) {
                 s.ref_err = kc.err;
                 s.err_prev = INFINITY;
-                s.status = isfinite(kc.err) ? PDLP_RUNNING : PDLP_NUMERICAL;
+                s.status = kc.infeasible ? PDLP_INFEASIBLE : isfinite(kc.err) ? PDLP_RUNNING : PDLP_NUMERICAL;
                 if (s.status != PDLP_RUNNING) s.action = ACT_DONE_CUR;
                 write_out(&f.out[k], s, kc, 0);
             } else {
                 const int it = s.iter + f.check_every;
                 s.iter = it;
                 s.window += f.check_every;
+                const bool inf_avg = ka.infeasible, inf_cur = kc.infeasible;
                 bool use_avg = true;
                 if (adaptive && (!isfinite(ka.err) || kc.err < ka.err)) {   // the better candidate
                     use_avg = false;
                     ka = kc;
                 }
                 ++s.checks;
-                if (!isfinite(ka.err)) {
+                if (inf_cur || inf_avg) {                                    // certificate found
+                    s.status = PDLP_INFEASIBLE;
+                    s.done_iter = it;
+                    s.w_div = (double)s.window;
+                    s.action = inf_cur ? ACT_DONE_CUR : ACT_DONE_AVG;
+                } else if (!isfinite(ka.err)) {
                     s.status = PDLP_NUMERICAL;
                     s.done_iter = it;
                     s.action = ACT_DONE_CUR;
@@ -787,7 +828,7 @@ void launch_check(const PdlpFamily& f, int mode, hipStream_t s) {
     const int add = mode == 0 ? 0 : f.check_every;
     kkt_cols_kernel<<<tile_grid(f, cc), NT, 0, s>>>(f, add, cc);
     kkt_rows_kernel<<<tile_grid(f, rc), NT, 0, s>>>(f, add, rc, nl);
-    decide_kernel<<<f.tiles, NS2 * T, 0, s>>>(f, mode, cc, rc);
+    decide_kernel<<<f.tiles, NSC * T, 0, s>>>(f, mode, cc, rc);
     apply_kernel<<<vec_grid(f), NT, 0, s>>>(f);
     count_kernel<<<1, NT, 0, s>>>(f);
 }

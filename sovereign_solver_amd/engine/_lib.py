@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "_lib", "libpdlp_hip.so")
 
 PDLP_ABI_VERSION = 1
 PDLP_F32, PDLP_F64 = 0, 1
-PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL = 0, 1, 3
+PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL, PDLP_INFEASIBLE = 0, 1, 3, 4
 PDLP_ROW_EQ, PDLP_ROW_LE = 0, 1
 PDLP_PARTIAL_BLOCKS = 1024
 PDLP_PARTIALS_LEN = 4 * PDLP_PARTIAL_BLOCKS * 6
@@ -25,7 +25,7 @@ PDLP_FAM_SHARED = {"c": 1, "b": 2, "lo": 4, "hi": 8, "c64": 16, "b64": 32, "b_sh
                    "ub_span": 256}
 PDLP_FAM_KKT_CHUNKS = 128
 PDLP_FAM_LONG = 64
-PDLP_FAM_NSUM = 12
+PDLP_FAM_NSUM = 20
 
 _p = C.c_void_p
 _i32p = C.c_void_p

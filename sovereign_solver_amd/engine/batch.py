@@ -35,7 +35,8 @@ OUT_DTYPE = np.dtype([
 ], align=True)
 
 
-_STATUS = {L.PDLP_RUNNING: "iteration_limit", L.PDLP_OPTIMAL: "optimal", L.PDLP_NUMERICAL: "numerical_error"}
+_STATUS = {L.PDLP_RUNNING: "iteration_limit", L.PDLP_OPTIMAL: "optimal", L.PDLP_NUMERICAL: "numerical_error",
+           L.PDLP_INFEASIBLE: "infeasible"}
 
 
 def lanes_per_row(mean_len: float) -> int:
