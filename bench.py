@@ -456,6 +456,24 @@ def bench_nodes(args):
     del be
     plain = plain_rate(probs[: args.plain_sample], args.tol, args.max_iters)
     obj_err = objective_errors(probs, res_p)
+    # strong branching at the root: 2 x 20 child LPs in one batch, warm-started from the root
+    from sovereign_solver_amd.lp.branching import branch_strong_batched, child_problem, fractional_vars
+    mask = np.ones(sc.cols, bool)
+    rp = sc.root()
+    branch_strong_batched(root.x, mask, rp, rp.lb, rp.ub, root.objective, y=root.y, step_scale=root.step_scale)
+    torch.cuda.synchronize()
+    t4 = time.perf_counter()
+    var, g_down, g_up = branch_strong_batched(root.x, mask, rp, rp.lb, rp.ub, root.objective, y=root.y,
+                                              step_scale=root.step_scale)
+    t_sb = time.perf_counter() - t4
+    n_frac = int(fractional_vars(root.x, mask).size)
+    kids = [child_problem(rp, rp.lb, rp.ub, int(j), d, float(root.x[j]))
+            for j in fractional_vars(root.x, mask)[:2] for d in ("down", "up")]
+    seq = plain_rate(kids, args.tol, args.max_iters)
+    strong = {"fractional_vars": n_frac, "children": 2 * min(20, n_frac), "batched_s": t_sb,
+              "children_per_s_batched": 2 * min(20, n_frac) / t_sb, "children_per_s_single_cold": seq,
+              "branch_var": var, "down_gain": g_down, "up_gain": g_up}
+    log(f"strong branching at the root: {strong}")
     log(f"persistent cold {persistent}; plain single-instance {plain:.2f} LPs/s; objective rel err vs HiGHS {obj_err}")
     cpu = None if args.no_cpu_baseline else cpu_rate(probs, args.tol, args.cpu_cap_s)
     value = parent["lps_per_s"]
@@ -473,7 +491,7 @@ def bench_nodes(args):
                    "loss_first_last": [losses[0], losses[-1]], "persistent_engine_cold": persistent,
                    "plain_single_gpu_lps_per_s": plain, "plain_sample": args.plain_sample,
                    "speedup_vs_plain": value / plain if plain > 0 else None,
-                   "objective_rel_err_vs_highs_max": obj_err},
+                   "objective_rel_err_vs_highs_max": obj_err, "strong_branching_root": strong},
         "roofline": roof, "cpu_baseline": cpu,
     }
     calib_copy(args)

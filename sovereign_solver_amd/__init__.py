@@ -13,6 +13,7 @@ from .io.mps import read_mps
 from .lp.pdhg import solve_lp_pdhg
 from .lp.pdhg_gpu import get_available_device, solve_lp_pdhg_gpu
 from .lp.batch import solve_lp_pdhg_batch
+from .lp.branching import branch_strong_batched
 
 __all__ = ["Problem", "SolveResult", "read_mps", "solve_lp_pdhg", "solve_lp_pdhg_gpu", "get_available_device",
-           "solve_lp_pdhg_batch"]
+           "solve_lp_pdhg_batch", "branch_strong_batched"]
