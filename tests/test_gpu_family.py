@@ -187,3 +187,43 @@ def test_family_detects_infeasible_nodes_with_certificates(gpu):
         else:
             assert r.status == "optimal", (k, r.status)
     assert max(r.iterations for k, r in enumerate(res) if k in bad) < 100_000
+
+
+def _variants(base, count, seed, vary_b=True, vary_c=True):
+    """LPs sharing base's matrix objects, with perturbed right-hand sides / costs
+    (feasible: the right-hand side stays A x0-shaped for equality families)."""
+    from sovereign_solver_amd.problem import Problem
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        b_ub = base.b_ub * (1 + (rng.uniform(-0.1, 0.1, base.b_ub.size) if vary_b else 0))
+        b_eq = base.b_eq
+        c = base.c * (1 + (rng.uniform(-0.2, 0.2, base.c.size) if vary_c else 0))
+        out.append(Problem(c=c, A_ub=base.A_ub, b_ub=b_ub, A_eq=base.A_eq, b_eq=b_eq, lb=base.lb, ub=base.ub,
+                           integer_mask=base.integer_mask, name=f"{base.name}_v{k}"))
+    return out
+
+
+def test_family_equality_rows_and_finite_boxes(gpu):
+    from sovereign_solver_amd.workloads import random_sparse_lp
+    base = random_sparse_lp(80, 200, 0.05, seed=7)          # '=' rows, 0 <= x <= u
+    probs = _variants(base, 66, seed=1, vary_b=False)
+    res = solve_lp_pdhg_batch(probs, tol=1e-4, max_iters=100_000, engine="family")
+    _check(probs, res, n_highs=4)
+
+
+def test_family_mixed_rows_and_infinite_upper_bounds(gpu):
+    from sovereign_solver_amd.problem import Problem
+    from sovereign_solver_amd.workloads import transportation_lp
+    base = transportation_lp(12, 20, seed=3)                 # supply '<=' rows, demand '=' rows, x >= 0
+    rng = np.random.default_rng(2)
+    probs = []
+    for k in range(40):                                       # more supply keeps every instance feasible
+        probs.append(Problem(c=base.c * (1 + rng.uniform(-0.3, 0.3, base.c.size)), A_ub=base.A_ub,
+                             b_ub=base.b_ub * (1 + rng.uniform(0.0, 0.3, base.b_ub.size)), A_eq=base.A_eq,
+                             b_eq=base.b_eq, lb=base.lb, ub=base.ub, integer_mask=base.integer_mask,
+                             name=f"transport_v{k}"))
+    res = solve_lp_pdhg_batch(probs, tol=1e-4, max_iters=200_000, engine="family")
+    _check(probs, res, n_highs=4, obj_tol=2e-3)
+    fb = FamilyBatch(probs)
+    assert fb.is_shared("ub_span") and not fb.is_shared("b")
