@@ -254,7 +254,8 @@ typedef struct PdlpBatchLP {
     int64_t rp_off;         /* first of its m+1 row pointers                 */
     int64_t nnz_off;        /* first of its nonzeros                         */
     double norm_b, norm_c;  /* reference-form normalisers                     */
-    double tau_scale;       /* primal weight: tau *= w, sigma /= w            */
+    double tau_scale;       /* primal weight: tau *= w, sigma /= w; NaN: the kernel takes
+                               ||sqrt(sigma_hat) b|| / ||sqrt(tau_hat) c|| (1 if either is 0) */
     double sigma_scale;
 } PdlpBatchLP;
 

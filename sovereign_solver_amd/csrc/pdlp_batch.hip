@@ -246,6 +246,23 @@ __device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red) {
     }
     __syncthreads();
 
+    // starting primal weight: given, or (NaN) ||sqrt(sigma_hat) b|| / ||sqrt(tau_hat) c||
+    float w_start = v.weight;
+    if (!(w_start > 0.f)) {
+        double nw[2] = {0.0, 0.0};
+        for (int j = threadIdx.x; j < v.n; j += NT) nw[1] += v.c64[j] * v.c64[j] * (double)v.tau[j];
+        const int g = threadIdx.x / TPR, lane = threadIdx.x % TPR;
+        for (int i = g; i < v.m; i += NT / TPR) {
+            double s = 0.0;
+            for (int q = v.rp[i] + lane; q < v.rp[i + 1]; q += TPR) s += fabs((double)v.val[q]);
+            if constexpr (TPR > 1) s = group_sum<TPR>(s);
+            if (lane == 0) nw[0] += v.b64[i] * v.b64[i] * 0.999 / (s > 1e-4 ? s : 1e-4);
+        }
+        block_sum_all<NT, 2>(nw, red);
+        const double nb = sqrt(nw[0]), nc = sqrt(nw[1]);
+        w_start = (nb > 1e-12 && nc > 1e-12) ? (float)fmin(fmax(nb / nc, 1e-6), 1e6) : 1.f;
+    }
+
     double acc[NS], acc2[NS];
     kkt_eval<NT, TPR>(v, R, yl, 0, 1.0, acc, red);
     Kkt kk = kkt_of(acc, v);
@@ -254,7 +271,7 @@ __device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red) {
     int window = 0, restarts = 0, checks = 0, it = 0, done_it = 0, last_restart = 0;
     bool end_on_avg = false;
     double w_final = 1.0;
-    float wgt = v.weight;
+    float wgt = w_start;
     rebuild_aty<NT, TPR>(v, R, yl);
 
     const int ce = a.check_every;
@@ -350,7 +367,7 @@ __device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red) {
         if (restart) {
             if (v.adaptive && !v.fixed_weight && kk.dx > 1e-10 && kk.dy > 1e-10) {
                 const double lw = 0.5 * log(kk.dx / kk.dy) + 0.5 * log((double)wgt);
-                wgt = (float)clamp_weight(exp(lw), (double)v.weight);
+                wgt = (float)clamp_weight(exp(lw), (double)w_start);
             }
             ref_err = kk.err;
             err_prev = INFINITY;

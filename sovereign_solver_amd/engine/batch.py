@@ -47,16 +47,26 @@ def lanes_per_row(mean_len: float) -> int:
     return t
 
 
+def _segment_sums(v: np.ndarray, starts: np.ndarray, lens: np.ndarray) -> np.ndarray:
+    """Sums of the segments v[starts[k] : starts[k] + lens[k]] (0 for empty ones).
+    The segments tile v in order (each starts where the previous one ends)."""
+    out = np.zeros(len(starts))
+    nz = lens > 0
+    if nz.any():
+        st = starts[nz]
+        sums = np.add.reduceat(v, st)          # segment i runs to the next non-empty start
+        end = int(st[-1] + lens[nz][-1])
+        sums[-1] -= v[end:].sum()              # the last one runs to the end of v
+        out[nz] = sums
+    return out
+
+
 class PackedBatch:
     """Host-side arena of B box-form LPs (same layout as the device arena)."""
 
     def __init__(self, problems: Sequence[Problem]):
         self.problems = list(problems)
         B = len(self.problems)
-        ms = np.empty(B, np.int64)
-        ns = np.empty(B, np.int64)
-        nnzs = np.empty(B, np.int64)
-        Ks = []
         # LPs whose (A_ub, A_eq) are the same objects share one stored CSR
         mat_of = np.empty(B, np.int64)
         uniq = {}
@@ -72,17 +82,17 @@ class PackedBatch:
                 uniq[key] = len(stored)
                 stored.append(K)
             mat_of[k] = uniq[key]
-            K = stored[mat_of[k]]
-            Ks.append(K)
-            ms[k], ns[k], nnzs[k] = K.shape[0], K.shape[1], K.nnz
-            if ms[k] == 0 or ns[k] == 0:
-                raise ValueError(f"LP {k}: the batched engine needs m > 0 and n > 0")
+        s_m = np.array([K.shape[0] for K in stored], np.int64)
+        s_n = np.array([K.shape[1] for K in stored], np.int64)
+        s_z = np.array([K.nnz for K in stored], np.int64)
+        ms, ns, nnzs = s_m[mat_of], s_n[mat_of], s_z[mat_of]
+        bad = np.flatnonzero((ms == 0) | (ns == 0))
+        if bad.size:
+            raise ValueError(f"LP {int(bad[0])}: the batched engine needs m > 0 and n > 0")
         self.ms, self.ns, self.nnzs = ms, ns, nnzs
         self.n_matrices = len(stored)
         row_off = np.concatenate([[0], np.cumsum(ms)[:-1]])
         col_off = np.concatenate([[0], np.cumsum(ns)[:-1]])
-        s_m = np.array([K.shape[0] for K in stored], np.int64)
-        s_z = np.array([K.nnz for K in stored], np.int64)
         s_rp = np.concatenate([[0], np.cumsum(s_m + 1)[:-1]])
         s_nz = np.concatenate([[0], np.cumsum(s_z)[:-1]])
         rp_off = s_rp[mat_of]
@@ -96,43 +106,32 @@ class PackedBatch:
         self.row_ptr = np.concatenate([K.indptr.astype(np.int32) for K in stored])
         self.col_idx = np.concatenate([K.indices.astype(np.int32) for K in stored])
         self.val = np.concatenate([K.data for K in stored])
-        c = np.empty(N)
-        lo = np.empty(N)
-        hi = np.empty(N)
-        b = np.empty(M)
-        kind = np.empty(M, np.int8)
-        b_shift = np.empty(M)
+
+        P = self.problems
+        c = np.concatenate([np.asarray(p.c, float) for p in P])
+        lo = np.concatenate([np.asarray(p.lb, float) for p in P])
+        hi = np.concatenate([np.asarray(p.ub, float) for p in P])
+        b = np.concatenate([np.concatenate([np.asarray(p.b_ub, float), np.asarray(p.b_eq, float)]) for p in P])
+        m_ub = np.array([p.n_ineq for p in P], np.int64)
+        lp_of_row = np.repeat(np.arange(B), ms)
+        kind = np.where(np.arange(M) - row_off[lp_of_row] < m_ub[lp_of_row], ROW_LE, ROW_EQ).astype(np.int8)
+        lbf = np.where(np.isfinite(lo), lo, 0.0)
+        b_shift = b.copy()
+        for k in np.flatnonzero(_segment_sums(np.abs(lbf), col_off, ns) > 0):   # only LPs with a lower shift
+            K = stored[mat_of[k]]
+            b_shift[row_off[k]: row_off[k] + ms[k]] -= K @ lbf[col_off[k]: col_off[k] + ns[k]]
+        fin = np.isfinite(hi)
+        span2 = np.where(fin, (hi - lbf) ** 2, 0.0)
         desc = np.zeros(B, LP_DTYPE)
-        # starting primal weight of the adaptive scheme: ||sqrt(sigma_hat) b|| / ||sqrt(tau_hat) c||
-        sums = {}
-        for i, K in enumerate(stored):
-            aK = abs(K)
-            sums[i] = (np.maximum(np.asarray(aK.sum(axis=0)).ravel(), 1e-4),
-                       np.maximum(np.asarray(aK.sum(axis=1)).ravel(), 1e-4))
-        self.init_weight = np.ones(B)
-        for k, (p, K) in enumerate(zip(self.problems, Ks)):
-            cs, rs = slice(col_off[k], col_off[k] + ns[k]), slice(row_off[k], row_off[k] + ms[k])
-            c[cs] = p.c
-            lo[cs] = p.lb
-            hi[cs] = p.ub
-            bk = np.concatenate([np.asarray(p.b_ub, float), np.asarray(p.b_eq, float)])
-            b[rs] = bk
-            kind[rs] = np.concatenate([np.full(p.n_ineq, ROW_LE, np.int8), np.full(p.n_eq, ROW_EQ, np.int8)])
-            lbf = np.where(np.isfinite(p.lb), p.lb, 0.0)
-            bs = bk - K @ lbf
-            b_shift[rs] = bs
-            fin = np.isfinite(p.ub)
-            span = (p.ub - lbf)[fin]
-            desc[k] = (ms[k], ns[k], lanes_per_row(nnzs[k] / ms[k]), 0, row_off[k], col_off[k], rp_off[k],
-                       nnz_off[k], max(1.0, float(np.sqrt(bs @ bs + span @ span))),
-                       max(1.0, float(np.linalg.norm(p.c))), 1.0, 1.0)
-            colsum, rowsum = sums[mat_of[k]]
-            nb = float(np.sqrt(np.sum(bk * bk * 0.999 / rowsum)))
-            nc = float(np.sqrt(np.sum(np.asarray(p.c, float) ** 2 * 0.999 / colsum)))
-            if nb > 1e-12 and nc > 1e-12:
-                self.init_weight[k] = float(np.clip(nb / nc, 1e-6, 1e6))
-        lo_shift = np.where(np.isfinite(lo), lo, 0.0)
-        ub_span = np.where(np.isfinite(hi), hi - lo_shift, np.inf)
+        desc["m"], desc["n"] = ms, ns
+        desc["tpr"] = [lanes_per_row(z / m) for z, m in zip(nnzs, ms)]
+        desc["row_off"], desc["col_off"], desc["rp_off"], desc["nnz_off"] = row_off, col_off, rp_off, nnz_off
+        desc["norm_b"] = np.maximum(1.0, np.sqrt(_segment_sums(b_shift * b_shift, row_off, ms) +
+                                                 _segment_sums(span2, col_off, ns)))
+        desc["norm_c"] = np.maximum(1.0, np.sqrt(_segment_sums(c * c, col_off, ns)))
+        desc["tau_scale"], desc["sigma_scale"] = 1.0, 1.0
+        lo_shift = lbf
+        ub_span = np.where(fin, hi - lo_shift, np.inf)
         self.c, self.lo, self.hi, self.b, self.kind = c, lo, hi, b, kind
         self.lo_shift, self.ub_span, self.b_shift = lo_shift, ub_span, b_shift
         self.desc = desc
@@ -202,7 +201,8 @@ class BatchEngine:
                 self.t["y"].copy_(torch.from_numpy(np.asarray(y0, np.float32)).to(self.device))
             desc = self.pk.desc.copy()
             desc["flags"] = self.flags
-            w = self.pk.init_weight if self.adaptive else np.ones(len(desc))
+            # NaN: the kernel computes the norm-ratio starting weight of the adaptive scheme
+            w = np.full(len(desc), np.nan) if self.adaptive else np.ones(len(desc))
             if step_scale is not None:            # NaN entries keep the default
                 ss = np.asarray(step_scale, float)
                 w = np.where(np.isfinite(ss), ss, w)

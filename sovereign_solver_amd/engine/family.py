@@ -61,6 +61,20 @@ def from_tiles(flat: np.ndarray, tiles: int, Ln: int, B: int) -> np.ndarray:
     return flat.reshape(tiles, Ln, TILE).transpose(0, 2, 1).reshape(tiles * TILE, Ln)[:B]
 
 
+def _rows(vectors, perm) -> np.ndarray:
+    """[B, L] array of the vectors (permuted to device order), or [1, L] when they
+    all hold the same values (same object, or equal entries)."""
+    first = vectors[0]
+    if all(v is first for v in vectors):
+        return np.asarray(first, float)[perm][None, :]
+    last = np.asarray(vectors[-1], float)
+    f0 = np.asarray(first, float)
+    if np.array_equal(last, f0):                   # maybe all equal: check every one
+        if all(np.array_equal(np.asarray(v, float), f0) for v in vectors):
+            return f0[perm][None, :]
+    return np.stack([np.asarray(v, float)[perm] for v in vectors])
+
+
 class FamilyBatch:
     """Host data of B box-form LPs that share K = [A_ub; A_eq].
 
@@ -108,27 +122,30 @@ class FamilyBatch:
 
         P = self.problems
         cp, rp = self.col_perm, self.row_perm
-        c = np.stack([np.asarray(p.c, float)[cp] for p in P])
-        b = np.stack([np.concatenate([np.asarray(p.b_ub, float), np.asarray(p.b_eq, float)])[rp] for p in P])
-        lo = np.stack([np.asarray(p.lb, float)[cp] for p in P])
-        hi = np.stack([np.asarray(p.ub, float)[cp] for p in P])
+        B = self.count
+        # per-LP vectors as [B, L] arrays, or [1, L] when every LP has the same values
+        c = _rows([p.c for p in P], cp)
+        b = _rows([np.concatenate([np.asarray(p.b_ub, float), np.asarray(p.b_eq, float)]) for p in P], rp)
+        lo = _rows([p.lb for p in P], cp)
+        hi = _rows([p.ub for p in P], cp)
         lbf = np.where(np.isfinite(lo), lo, 0.0)
-        b_shift = b - (K @ lbf.T).T
+        b_shift = b if not lbf.any() else b - (K @ lbf.T).T
         span = np.where(np.isfinite(hi), hi - lbf, np.inf)
-        fin = np.isfinite(span)
-        self.norm_b = np.maximum(1.0, np.sqrt(np.sum(b_shift * b_shift, axis=1) +
-                                              np.sum(np.where(fin, span, 0.0) ** 2, axis=1)))
-        self.norm_c = np.maximum(1.0, np.linalg.norm(c, axis=1))
+        fin_span = np.where(np.isfinite(span), span, 0.0)
+        per_lp = lambda v: np.broadcast_to(v, (B,)).copy()
+        self.norm_b = per_lp(np.maximum(1.0, np.sqrt(np.sum(b_shift * b_shift, axis=1) +
+                                                     np.sum(fin_span * fin_span, axis=1))))
+        self.norm_c = per_lp(np.maximum(1.0, np.linalg.norm(c, axis=1)))
         # starting primal weight of the adaptive scheme: ||sqrt(sigma_hat) b|| / ||sqrt(tau_hat) c||
-        nb = np.sqrt(np.sum(b * b * (0.999 / np.maximum(rowsum, 1e-4)), axis=1))
-        nc = np.sqrt(np.sum(c * c * (0.999 / np.maximum(colsum, 1e-4)), axis=1))
+        nb = per_lp(np.sqrt(np.sum(b * b * (0.999 / np.maximum(rowsum, 1e-4)), axis=1)))
+        nc = per_lp(np.sqrt(np.sum(c * c * (0.999 / np.maximum(colsum, 1e-4)), axis=1)))
         ok = (nb > 1e-12) & (nc > 1e-12)
         self.init_weight = np.where(ok, np.clip(nb / np.where(ok, nc, 1.0), 1e-6, 1e6), 1.0)
         self.vec = {"c": c, "b": b, "lo": lo, "hi": hi, "c64": c, "b64": b, "b_shift": b_shift,
                     "lo_shift": lbf, "ub_span": span}
         self.shared = 0
-        for name, a in self.vec.items():
-            if bool(np.all(a == a[0])):             # (inf == inf holds)
+        for name, v in self.vec.items():
+            if v.shape[0] == 1:
                 self.shared |= L.PDLP_FAM_SHARED[name]
         self.lo, self.hi = lo, hi
 
@@ -154,6 +171,10 @@ class FamilyBatch:
     def device_vector(self, name: str) -> np.ndarray:
         a = self.vec[name]
         return a[0] if self.is_shared(name) else to_tiles(a, self.tiles)
+
+    def rows_of(self, name: str) -> np.ndarray:
+        """[B, L] view of a per-LP vector in device order (shared ones broadcast)."""
+        return np.broadcast_to(self.vec[name], (self.count, self.vec[name].shape[1]))
 
     def column_pass_bytes(self) -> float:
         """Algorithmic bytes of one column pass over all B LPs: K^T's CSR once
@@ -332,8 +353,9 @@ class FamilyEngine:
             L.check(self.lib.pdlp_family_set_start(C.byref(self.args), None, 0, -1, -1, -1, None, None, self.sh),
                     "pdlp_family_set_start")
         else:
-            x = (np.clip(0.0, fb.lo, fb.hi) if x0 is None else
-                 np.clip(fb.to_device_cols(np.asarray(x0, float).reshape(fb.count, fb.n)), fb.lo, fb.hi))
+            lo, hi = fb.rows_of("lo"), fb.rows_of("hi")
+            x = (np.clip(0.0, lo, hi) if x0 is None else
+                 np.clip(fb.to_device_cols(np.asarray(x0, float).reshape(fb.count, fb.n)), lo, hi))
             if y0 is None:
                 y = np.zeros((fb.count, fb.m))
             else:

@@ -52,7 +52,7 @@ def test_normalisers_match_single_box_form_and_yardstick():
         f = box_form(p)
         assert fb.norm_b[k] == pytest.approx(f.norm_b, rel=1e-14)
         assert fb.norm_c[k] == pytest.approx(f.norm_c, rel=1e-14)
-        np.testing.assert_allclose(fb.from_device_rows(fb.vec["b_shift"])[k], f.b_shift, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(fb.from_device_rows(fb.rows_of("b_shift"))[k], f.b_shift, rtol=0, atol=1e-12)
         # the yardstick normaliser is the same quantity
         x = np.clip(0.0, p.lb, p.ub)
         y = np.zeros(p.n_constraints)
@@ -120,5 +120,5 @@ def test_device_numbering_puts_long_rows_last():
     np.testing.assert_array_equal(fb.from_device_rows(fb.to_device_rows(Y)), Y)
     # device-order problem data matches the problems' own data
     p = fam.problem(1)
-    np.testing.assert_array_equal(fb.from_device_cols(fb.vec["c"])[1], p.c)
-    np.testing.assert_array_equal(fb.from_device_rows(fb.vec["b"])[1], p.b_ub)
+    np.testing.assert_array_equal(fb.from_device_cols(fb.rows_of("c"))[1], p.c)
+    np.testing.assert_array_equal(fb.from_device_rows(fb.rows_of("b"))[1], p.b_ub)
