@@ -121,15 +121,16 @@ def cpu_baseline(problem, iters_needed: int, sample_iters: int):
 def batch_lp_bytes(pk, out, check_every: int):
     """Algorithmic bytes the batched kernel moves, summed over LPs.
 
-    Per iteration of LP k: its CSR once (8 B per entry + 4 B per row pointer),
-    32 B per column (c, tau, lo, hi, x r/w, x partial sum r/w), 5 B per row
-    (b, row kind).  Per check: three more CSR passes (K x_avg, K^T y_avg,
-    K^T y rebuild), 64 B per column and 41 B per row of FP64 sums and data."""
+    Per iteration of LP k: K and K^T once each (values and indices, 8 B per
+    entry each; the line pointers stay in LDS), 32 B per column (c, tau, lo,
+    hi, x r/w, x partial sum r/w), 5 B per row (b, row kind).  Per check (two
+    candidates): K and K^T twice more, 64 B per column and 41 B per row of
+    FP64 sums and data."""
     import numpy as np
     nnz, m, n = pk.nnzs.astype(float), pk.ms.astype(float), pk.ns.astype(float)
-    csr = 8 * nnz + 4 * (m + 1)
-    per_it = csr + 32 * n + 5 * m
-    per_check = 3 * csr + 64 * n + 41 * m
+    mat = 8 * nnz
+    per_it = 2 * mat + 32 * n + 5 * m
+    per_check = 4 * mat + 64 * n + 41 * m
     its = out["iterations"].astype(float)
     checks = out["checks"].astype(float) + 1
     return float(np.sum(its * per_it + checks * per_check))

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PDLP_ABI_VERSION 1
+#define PDLP_ABI_VERSION 2
 
 /* precision of iterates, matrix values and problem vectors */
 #define PDLP_F32 0
@@ -97,7 +97,7 @@ typedef struct PdlpCtrl {
     int32_t cand;            /* candidate of the pending restart: 0 current, 1 average */
     int32_t last_restart;    /* iteration of the last restart                   */
     double weight0;          /* starting primal weight (tau_scale at pdlp_init); the adaptive
-                                weight stays within 1e4x of it                  */
+                                weight stays within [w0/1e2, w0*1e4]            */
 } PdlpCtrl;
 
 /*
@@ -226,20 +226,25 @@ int pdlp_graph_capture_step(const PdlpDev* dev, const PdlpTune* tune, int32_t wh
                             void* stream, void** exec);
 
 /* ----------------------------------------------------------------------------
- * Batched engine: thousands of independent box-form LPs in one persistent launch.
+ * Batched engine: thousands of independent box-form LPs in persistent launches.
  *
  * The reference solves LPs one at a time (solver/lp/pdhg.py:152; B&B node LPs
  * one by one at solver/milp/branch_and_bound.py:367-374); this is the batched
  * replacement of that loop.  All LPs are packed into shared arrays ("arena");
- * each PdlpBatchLP holds one LP's offsets.  Row pointers of an LP are relative
- * to its nnz_off, column indices relative to its col_off.
+ * each PdlpBatchLP holds one LP's offsets.  Each LP is stored as CSR of K
+ * and CSR of K^T with the same number of entries: row pointers relative to
+ * its nnz_off, column indices relative to its col_off; column pointers (of
+ * K^T) relative to its nnz_off, row indices relative to its row_off.
  *
  * One workgroup owns one LP at a time and takes the next from the work queue
- * (`order`, typically largest first) when it finishes.  The iterates live in
- * LDS; per iteration one pass over the LP's CSR rows forms K xbar, the dual
- * update, and scatters K^T y+ into an LDS accumulator for the next primal
- * step.  Termination and restarts follow pdhg.py:225-244 per LP, with the
- * relative KKT of the reference-form lift in FP64.
+ * (`order`, typically largest first) when it finishes.  Two launches on the
+ * stream: the FP32 part (iterates in FP32, x in global memory, 12*n+8*m+8
+ * bytes of LDS per LP) and the FP64 part (iterates in FP64, x in LDS,
+ * 20*n+12*m+8 bytes), for the LPs small enough.  Per iteration a pass over
+ * the columns (K^T y gathered from LDS, primal step) and a pass over the rows
+ * (K xbar gathered from LDS, dual step).  Termination and restarts follow
+ * pdhg.py:225-244 per LP, with the relative KKT of the reference-form lift in
+ * FP64.
  * ------------------------------------------------------------------------- */
 
 typedef struct PdlpBatchLP {
@@ -249,10 +254,13 @@ typedef struct PdlpBatchLP {
                                bit 1: keep the primal weight fixed;
                                bit 2: weight from the distances to the starting point
                                       (else: moved since the last restart) */
+    int32_t tpc;            /* lanes per column (of K^T): 1,2,4,8,16,32,64 */
+    int32_t reserved;
     int64_t row_off;        /* first row of this LP in the row arrays        */
     int64_t col_off;        /* first column of this LP in the column arrays  */
     int64_t rp_off;         /* first of its m+1 row pointers                 */
-    int64_t nnz_off;        /* first of its nonzeros                         */
+    int64_t nnz_off;        /* first of its nonzeros (in K and in K^T)       */
+    int64_t cp_off;         /* first of its n+1 column pointers (of K^T)     */
     double norm_b, norm_c;  /* reference-form normalisers                     */
     double tau_scale;       /* primal weight: tau *= w, sigma /= w; NaN: the kernel takes
                                ||sqrt(sigma_hat) b|| / ||sqrt(tau_hat) c|| (1 if either is 0) */
@@ -272,15 +280,21 @@ typedef struct PdlpBatch {
     int32_t count;          /* number of LPs                                  */
     int32_t check_every;    /* 50 = reference cadence                          */
     int32_t max_iters;
-    int32_t block;          /* threads per workgroup: 256, 512 or 1024        */
-    int32_t lds_bytes;      /* dynamic LDS per workgroup: >= max over LPs of 8*(m+n) */
-    int32_t grid;           /* persistent workgroups (0 = 2 per CU)           */
+    int32_t block;          /* FP32 part: threads per workgroup, 256 or 512   */
+    int32_t lds_bytes;      /* FP32 part: dynamic LDS per workgroup, >= 12*n+8*m+8 of its LPs
+                               (an LP that does not fit is not run, its output left as it was) */
+    int32_t grid;           /* FP32 part: persistent workgroups (0 = 2 per CU) */
+    int32_t n_resident;     /* the last n_resident LPs of `order` run with FP64 iterates */
+    int32_t block_resident; /* FP64 part: threads per workgroup, 256 or 512 (0 = 256) */
+    int32_t lds_resident;   /* FP64 part: dynamic LDS, >= 20*n+12*m+8 of its LPs */
+    int32_t grid_resident;  /* FP64 part: persistent workgroups (0 = 2 per CU) */
     double tol;
     const PdlpBatchLP* lps; /* count descriptors                              */
-    const int32_t* order;   /* count LP indices, processing order             */
-    int32_t* queue;         /* one int32 work counter, zero before the launch */
-    /* matrices */
+    const int32_t* order;   /* count LP indices, processing order: FP32 part, then FP64 part */
+    int32_t* queue;         /* two int32 work counters (FP32, FP64 part), zero before the launch */
+    /* matrices: K by rows, K^T by rows (= K by columns) */
     const int32_t* row_ptr; const int32_t* col_idx; const float* val;
+    const int32_t* t_col_ptr; const int32_t* t_row_idx; const float* t_val;
     /* per column: c, bounds (+-inf allowed), FP64 c, lower-bound shift, (hi - shift) or +inf */
     const float *c, *lo, *hi;
     const double *c64, *lo_shift, *ub_span;
@@ -292,6 +306,7 @@ typedef struct PdlpBatch {
     float* y; double* y_sum;                               /* per row    */
     PdlpBatchOut* out;
     float* x_last; float* y_last;   /* per column / row: last restart point (scratch) */
+    float* y_psum;                  /* per row: sum of y since the last check (scratch) */
 } PdlpBatch;
 
 /* Solve every LP of the batch (one launch). */
@@ -339,7 +354,7 @@ typedef struct PdlpFamilyLP {
     double norm_b, norm_c;  /* reference-form normalisers                       */
     double ref_err, err_prev, w_div;
     float weight;           /* primal weight: tau = w tau_hat, sigma = sigma_hat / w */
-    float weight0;          /* starting weight; the adaptive weight stays within 1e4x of it */
+    float weight0;          /* starting weight; the adaptive weight stays within [w0/1e2, w0*1e4] */
 } PdlpFamilyLP;
 
 typedef struct PdlpFamily {

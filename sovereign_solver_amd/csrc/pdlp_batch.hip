@@ -1,20 +1,35 @@
-// Batched restarted PDHG: one persistent launch, one workgroup per LP at a time.
+// Batched restarted PDHG: persistent launches, one workgroup per LP at a time.
 //
 // Per LP (box form, rows '=' or '<=', lo <= x <= hi), LDS holds
-//   R   : 2n floats — xbar[0..n) and the K^T y accumulator aty[n..2n);
-//         reused as n doubles during a KKT check
-//   yl  : m floats — the dual iterate
-//   ypl : m floats — its sum since the last check
-// and each iteration is
-//   column pass  x+ = clip(x - tau (c - aty)), xbar = 2x+ - x, aty = 0
-//   row pass     d = K_i xbar; y_i+ = proj(y_i + sigma_i (b_i - d));
-//                aty[j] += K_ij y_i+ for the row's entries (LDS atomics)
-// so the LP's CSR is read once per iteration.  Every check_every iterations
-// the partial sums are folded into the FP64 window sums and the reference KKT
-// of the window average is evaluated in FP64 (pdhg.py:184-196 on the lifted
-// form), followed by the termination/restart rule of pdhg.py:225-244.
-// The float atomics make the summation order of K^T y run-dependent; results
-// agree to rounding, not bit for bit, between runs.
+//   X   : n doubles — xbar during iterations, the FP64 primal point of a KKT check
+//   yl  : m — the dual iterate
+//   rpl : m+1 row pointers, cpl : n+1 column pointers (of the transpose)
+// in two parts of the batch, one launch each:
+//   FP32 part: FP32 iterates, x in global memory, 12n + 8m + 8 bytes of LDS;
+//   FP64 part: FP64 iterates, x in LDS as well, 20n + 12m + 8 bytes.
+// Small LPs with many empty columns drift in FP32 (their restarts see the
+// rounding of the residuals and shrink the primal weight without end) and
+// finish in FP64; for the large LPs, whose size sets the LDS per workgroup,
+// FP32 halves the LDS so two workgroups share a CU.  Each iteration is
+//   column pass  aty_j = sum_i K_ij y_i over column j's entries (CSR of K^T),
+//                x+ = clip(x - tau (c - aty)), xbar = 2x+ - x
+//   row pass     d_i = K_i xbar; y_i+ = proj(y_i + sigma_i (b_i - d_i))
+// Both passes gather from LDS and write one value per row or column, so there
+// are no atomics: the float atomics of a scatter formulation serialise in the
+// LDS and cost as much as the rest of the row pass together.  Every sum runs
+// in a fixed order, so results are the same from run to run.
+//
+// A group of 2^lw lanes (lw per LP and pass, from the line lengths) owns a
+// line (row or column); each lane takes every 2^lw-th entry, the first EL of
+// them loaded at once.  Each group works on two lines per round, so the loads
+// of both are in flight together.  Line pointers come from LDS, so a round
+// waits for global memory once.
+//
+// Every check_every iterations the partial sums are folded into the FP64
+// window sums and the reference KKT of the window average (and, for the
+// adaptive scheme, of the current iterate) is evaluated in FP64 (pdhg.py:184-196
+// on the lifted form), followed by the termination/restart rule of
+// pdhg.py:225-244 or its adaptive variant.
 
 #include "pdlp_common.hpp"
 
@@ -24,71 +39,142 @@
 namespace pdlp {
 namespace {
 
+// Global-memory qualifier for an LP's arrays.  The LP's pointers are formed in
+// the kernel from the descriptor, so without it the compiler treats them as
+// generic and emits flat loads: 64-bit addresses in VGPR pairs, and every flat
+// access also counts against the LDS wait counter.
+#define GMEM __attribute__((address_space(1)))
+
+constexpr int EL = 8;   // entries per lane loaded together
+constexpr int KEL = 4;  // entries per lane loaded together in a KKT evaluation (FP64 sums: registers)
+constexpr int NS = 6;   // KKT sums: rp^2, rd^2, primal obj, dual obj, |dx|^2, |dy|^2
+
 struct LPView {
     int m, n;
-    const int32_t* rp;     // m+1, relative to nnz base
-    const int32_t* ci;     // relative to column base
-    const float* val;
-    const float *c, *lo, *hi;
-    const double *c64, *lo_shift, *ub_span;
-    const float* b;
-    const int8_t* kind;
-    const double *b64, *b_shift;
-    float *x, *x_psum, *tau;
-    double* x_sum;
-    float* y;
-    double* y_sum;
-    float *x_last, *y_last;   // last restart point (adaptive scheme)
+    const GMEM int32_t* rp;     // m+1, relative to the nnz base
+    const GMEM int32_t* ci;     // relative to the column base
+    const GMEM float* val;
+    const GMEM int32_t* cp;     // n+1 column pointers of K^T, relative to the nnz base
+    const GMEM int32_t* ri;     // row index of each K^T entry, relative to the row base
+    const GMEM float* tval;
+    const GMEM float *c, *lo, *hi;
+    const GMEM double *c64, *lo_shift, *ub_span;
+    const GMEM float* b;
+    const GMEM int8_t* kind;
+    const GMEM double *b64, *b_shift;
+    GMEM float *x, *x_psum, *tau;
+    GMEM double* x_sum;
+    GMEM float *y, *y_psum;
+    GMEM double* y_sum;
+    GMEM float *x_last, *y_last;   // last restart point (adaptive scheme)
     double norm_b, norm_c;
     float weight;             // starting primal weight w: tau = w tau_hat, sigma = sigma_hat / w
+    int lr, lc;               // log2 lanes per row / per column
     bool adaptive;            // adaptive restarts (flags bit 0)
     bool fixed_weight;        // keep the primal weight (flags bit 1)
     bool move_ref;            // distances since the last restart (flags bit 2 clear) or from the start
 };
 
+__device__ __forceinline__ int log2_lanes(int t) {
+    return __builtin_amdgcn_readfirstlane(t > 1 ? 31 - __builtin_clz(t) : 0);
+}
+
 __device__ __forceinline__ LPView view_of(const PdlpBatch& a, const PdlpBatchLP& lp) {
     LPView v;
     v.m = lp.m;
     v.n = lp.n;
-    v.rp = a.row_ptr + lp.rp_off;
-    v.ci = a.col_idx + lp.nnz_off;
-    v.val = a.val + lp.nnz_off;
-    v.c = a.c + lp.col_off;
-    v.lo = a.lo + lp.col_off;
-    v.hi = a.hi + lp.col_off;
-    v.c64 = a.c64 + lp.col_off;
-    v.lo_shift = a.lo_shift + lp.col_off;
-    v.ub_span = a.ub_span + lp.col_off;
-    v.b = a.b + lp.row_off;
-    v.kind = a.row_kind + lp.row_off;
-    v.b64 = a.b64 + lp.row_off;
-    v.b_shift = a.b_shift + lp.row_off;
-    v.x = a.x + lp.col_off;
-    v.x_psum = a.x_psum + lp.col_off;
-    v.x_sum = a.x_sum + lp.col_off;
-    v.tau = a.tau + lp.col_off;
-    v.y = a.y + lp.row_off;
-    v.y_sum = a.y_sum + lp.row_off;
-    v.x_last = a.x_last + lp.col_off;
-    v.y_last = a.y_last + lp.row_off;
+    v.rp = (decltype(v.rp))(a.row_ptr + lp.rp_off);
+    v.ci = (decltype(v.ci))(a.col_idx + lp.nnz_off);
+    v.val = (decltype(v.val))(a.val + lp.nnz_off);
+    v.cp = (decltype(v.cp))(a.t_col_ptr + lp.cp_off);
+    v.ri = (decltype(v.ri))(a.t_row_idx + lp.nnz_off);
+    v.tval = (decltype(v.tval))(a.t_val + lp.nnz_off);
+    v.c = (decltype(v.c))(a.c + lp.col_off);
+    v.lo = (decltype(v.lo))(a.lo + lp.col_off);
+    v.hi = (decltype(v.hi))(a.hi + lp.col_off);
+    v.c64 = (decltype(v.c64))(a.c64 + lp.col_off);
+    v.lo_shift = (decltype(v.lo_shift))(a.lo_shift + lp.col_off);
+    v.ub_span = (decltype(v.ub_span))(a.ub_span + lp.col_off);
+    v.b = (decltype(v.b))(a.b + lp.row_off);
+    v.kind = (decltype(v.kind))(a.row_kind + lp.row_off);
+    v.b64 = (decltype(v.b64))(a.b64 + lp.row_off);
+    v.b_shift = (decltype(v.b_shift))(a.b_shift + lp.row_off);
+    v.x = (decltype(v.x))(a.x + lp.col_off);
+    v.x_psum = (decltype(v.x_psum))(a.x_psum + lp.col_off);
+    v.x_sum = (decltype(v.x_sum))(a.x_sum + lp.col_off);
+    v.tau = (decltype(v.tau))(a.tau + lp.col_off);
+    v.y = (decltype(v.y))(a.y + lp.row_off);
+    v.y_sum = (decltype(v.y_sum))(a.y_sum + lp.row_off);
+    v.y_psum = (decltype(v.y_psum))(a.y_psum + lp.row_off);
+    v.x_last = (decltype(v.x_last))(a.x_last + lp.col_off);
+    v.y_last = (decltype(v.y_last))(a.y_last + lp.row_off);
     v.norm_b = lp.norm_b;
     v.norm_c = lp.norm_c;
     v.weight = (float)lp.tau_scale;
+    v.lr = log2_lanes(lp.tpr);
+    v.lc = log2_lanes(lp.tpc);
     v.adaptive = (lp.flags & 1) != 0;
     v.fixed_weight = (lp.flags & 2) != 0;
     v.move_ref = (lp.flags & 4) == 0;
     return v;
 }
 
-// Scatter acc[col] += scale(row) * K_ij over all rows (LDS atomics), TPR lanes per row.
-template <int NT, int TPR, typename A, typename F>
-__device__ __forceinline__ void scatter_rows(const LPView& v, A* acc, F row_scale) {
-    const int g = threadIdx.x / TPR, lane = threadIdx.x % TPR;
-    for (int i = g; i < v.m; i += NT / TPR) {
-        const A s = row_scale(i);
-        if (s == A(0)) continue;                      // uniform across the group
-        const int beg = v.rp[i], end = v.rp[i + 1];
-        for (int k = beg + lane; k < end; k += TPR) atomicAdd(&acc[v.ci[k]], (A)v.val[k] * s);
+// Cross-lane move inside a row of 16 lanes (DPP: no LDS traffic).
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over aligned groups of 2^lw lanes (lw uniform, 0..6); every lane of a
+// group ends with the same sum.  Groups are entirely active or inactive.
+__device__ __forceinline__ float group_sum_rt(float v, int lw) {
+    if (lw >= 1) v += dpp<0xB1>(v);    // quad_perm [1,0,3,2]: lane ^ 1
+    if (lw >= 2) v += dpp<0x4E>(v);    // quad_perm [2,3,0,1]: lane ^ 2
+    if (lw >= 3) v += dpp<0x141>(v);   // row_half_mirror: lane i <-> 7 - i in each 8
+    if (lw >= 4) v += dpp<0x140>(v);   // row_mirror: lane i <-> 15 - i in each 16
+    if (lw >= 5) v += __shfl_xor(v, 16);
+    if (lw >= 6) v += __shfl_xor(v, 32);
+    return v;
+}
+
+__device__ __forceinline__ double group_sum_rt(double v, int lw) {
+    for (int o = 1; o < (1 << lw); o <<= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Two lines at once: s_k += val[q] * vec[idx[q]] and a_k += |val[q]| over the
+// lane's entries of [b_k, e_k) (entries lane, lane + 2^lw, ...).  The first EL
+// entries per lane of both lines are loaded before any is used.
+template <typename T>
+__device__ __forceinline__ void dot2(const GMEM float* val, const GMEM int32_t* idx, const T* vec, int b1, int e1,
+                                     int b2, int e2, int lane, int lw, T& s1, T& a1, T& s2, T& a2) {
+    float v1[EL], v2[EL];
+    int c1[EL], c2[EL];
+#pragma unroll
+    for (int e = 0; e < EL; ++e) {
+        const int q1 = b1 + lane + (e << lw), q2 = b2 + lane + (e << lw);
+        v1[e] = q1 < e1 ? val[q1] : 0.f;
+        c1[e] = q1 < e1 ? idx[q1] : 0;
+        v2[e] = q2 < e2 ? val[q2] : 0.f;
+        c2[e] = q2 < e2 ? idx[q2] : 0;
+    }
+#pragma unroll
+    for (int e = 0; e < EL; ++e) {
+        s1 += (T)v1[e] * vec[c1[e]];
+        a1 += (T)fabsf(v1[e]);
+        s2 += (T)v2[e] * vec[c2[e]];
+        a2 += (T)fabsf(v2[e]);
+    }
+    // lines longer than EL entries per lane (rare: the lane count covers most lines)
+    for (int q = b1 + lane + (EL << lw); q < e1; q += 1 << lw) {
+        const T kv = (T)val[q];
+        s1 += kv * vec[idx[q]];
+        a1 += fabs(kv);
+    }
+    for (int q = b2 + lane + (EL << lw); q < e2; q += 1 << lw) {
+        const T kv = (T)val[q];
+        s2 += kv * vec[idx[q]];
+        a2 += fabs(kv);
     }
 }
 
@@ -112,22 +198,22 @@ __device__ __forceinline__ void block_sum_all(double (&acc)[NV], double* red) {
     __syncthreads();
 }
 
-constexpr int NS = 6;   // rp^2, rd^2, primal obj, dual obj, |dx|^2, |dy|^2
-
 // Reference KKT of (x, y): src 0 = current iterate, 1 = window average (w).
-// Uses R as n doubles.  acc (all threads): rp^2, rd^2, primal and dual
+// Uses X as n doubles.  acc (all threads): rp^2, rd^2, primal and dual
 // objective, and for the adaptive scheme the squared distances of the point
 // from the last restart point in the step-size metric (dx^2/tau_hat, dy^2/sigma_hat).
-template <int NT, int TPR>
-__device__ void kkt_eval(const LPView& v, float* R, const float* yl, int src, double w, double (&acc)[NS],
-                         double* red) {
-    double* R64 = reinterpret_cast<double*>(R);
+template <int NT, typename T>
+__device__ __forceinline__ void kkt_eval(const LPView& v, double* X, const T* xl, const T* yl, const int* rpl,
+                                         const int* cpl, int src, double w, double (&acc)[NS], double* red) {
 #pragma unroll
     for (int k = 0; k < NS; ++k) acc[k] = 0.0;
     // C1: primal point into LDS (FP64), primal objective, primal distance
     for (int j = threadIdx.x; j < v.n; j += NT) {
-        const double xj = src ? v.x_sum[j] / w : (double)v.x[j];
-        R64[j] = xj;
+        double xj;
+        if (src) xj = v.x_sum[j] / w;
+        else if constexpr (std::is_same<T, double>::value) xj = xl[j];
+        else xj = (double)v.x[j];
+        X[j] = xj;
         acc[2] += v.c64[j] * (xj - v.lo_shift[j]);
         if (v.adaptive) {
             const double dx = xj - (double)v.x_last[j];
@@ -137,19 +223,27 @@ __device__ void kkt_eval(const LPView& v, float* R, const float* yl, int src, do
     __syncthreads();
     // C2: rows — primal residual, dual objective, slack dual residual, dual distance
     {
-        const int g = threadIdx.x / TPR, lane = threadIdx.x % TPR;
-        for (int i = g; i < v.m; i += NT / TPR) {
-            const int beg = v.rp[i], end = v.rp[i + 1];
+        const int lw = v.lr, lane = threadIdx.x & ((1 << lw) - 1);
+        for (int i = threadIdx.x >> lw; i < v.m; i += NT >> lw) {
+            const int beg = rpl[i], end = rpl[i + 1];
             double d = 0.0, as = 0.0;
-            for (int k = beg + lane; k < end; k += TPR) {
-                const double kv = (double)v.val[k];
-                d += kv * R64[v.ci[k]];
-                as += fabs(kv);
+            for (int q0 = beg + lane; q0 < end; q0 += KEL << lw) {
+                float kv[KEL];
+                int cc[KEL];
+#pragma unroll
+                for (int e = 0; e < KEL; ++e) {
+                    const int q = q0 + (e << lw);
+                    kv[e] = q < end ? v.val[q] : 0.f;
+                    cc[e] = q < end ? v.ci[q] : 0;
+                }
+#pragma unroll
+                for (int e = 0; e < KEL; ++e) {
+                    d += (double)kv[e] * X[cc[e]];
+                    as += fabs((double)kv[e]);
+                }
             }
-            if constexpr (TPR > 1) {
-                d = group_sum<TPR>(d);
-                as = group_sum<TPR>(as);
-            }
+            d = group_sum_rt(d, lw);
+            as = group_sum_rt(as, lw);
             if (lane == 0) {
                 const bool le = v.kind[i] == PDLP_ROW_LE;
                 double res = d - v.b64[i];
@@ -165,31 +259,40 @@ __device__ void kkt_eval(const LPView& v, float* R, const float* yl, int src, do
             }
         }
     }
-    __syncthreads();
-    // C3: K^T y into LDS (FP64 atomics)
-    for (int j = threadIdx.x; j < v.n; j += NT) R64[j] = 0.0;
-    __syncthreads();
-    scatter_rows<NT, TPR, double>(v, R64, [&](int i) { return src ? v.y_sum[i] / w : (double)yl[i]; });
-    __syncthreads();
-    // C4: columns — dual residual / upper-bound-row duals
-    for (int j = threadIdx.x; j < v.n; j += NT) {
-        const double r = v.c64[j] - R64[j];
-        const double rneg = r < 0.0 ? r : 0.0;
-        const double span = v.ub_span[j];
-        if (isfinite(span)) acc[3] += span * rneg;
-        else acc[1] += rneg * rneg;
+    // C3: columns — reduced costs c - K^T y of the point: dual residual / upper-bound-row duals
+    {
+        const int lw = v.lc, lane = threadIdx.x & ((1 << lw) - 1);
+        for (int j = threadIdx.x >> lw; j < v.n; j += NT >> lw) {
+            const int beg = cpl[j], end = cpl[j + 1];
+            double s = 0.0;
+            for (int q0 = beg + lane; q0 < end; q0 += KEL << lw) {
+                float kv[KEL];
+                int rr[KEL];
+#pragma unroll
+                for (int e = 0; e < KEL; ++e) {
+                    const int q = q0 + (e << lw);
+                    kv[e] = q < end ? v.tval[q] : 0.f;
+                    rr[e] = q < end ? v.ri[q] : 0;
+                }
+                if (src) {
+#pragma unroll
+                    for (int e = 0; e < KEL; ++e) s += (double)kv[e] * (v.y_sum[rr[e]] / w);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < KEL; ++e) s += (double)kv[e] * (double)yl[rr[e]];
+                }
+            }
+            s = group_sum_rt(s, lw);
+            if (lane == 0) {
+                const double r = v.c64[j] - s;
+                const double rneg = r < 0.0 ? r : 0.0;
+                const double span = v.ub_span[j];
+                if (isfinite(span)) acc[3] += span * rneg;
+                else acc[1] += rneg * rneg;
+            }
+        }
     }
     block_sum_all<NT, NS>(acc, red);
-}
-
-// aty = K^T yl (float), R's upper half.
-template <int NT, int TPR>
-__device__ __forceinline__ void rebuild_aty(const LPView& v, float* R, const float* yl) {
-    float* aty = R + v.n;
-    for (int j = threadIdx.x; j < v.n; j += NT) aty[j] = 0.f;
-    __syncthreads();
-    scatter_rows<NT, TPR, float>(v, aty, [&](int i) { return yl[i]; });
-    __syncthreads();
 }
 
 struct Kkt {
@@ -210,39 +313,54 @@ __device__ __forceinline__ Kkt kkt_of(const double (&s)[NS], const LPView& v) {
     return k;
 }
 
-template <int NT, int TPR>
-__device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red) {
+// One LP from start to finish.  T = float: FP32 iterates, x in global memory;
+// T = double: FP64 iterates, x in LDS.
+template <int NT, typename T>
+__device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red) {
+    constexpr bool RES = std::is_same<T, double>::value;
+    constexpr int LINES = RES ? 1 : 2;   // lines per group and round (FP64: one, for registers)
     const PdlpBatchLP lp = a.lps[k];
     const LPView v = view_of(a, lp);
-    float* R = lds;
-    float* xb = R;
-    float* aty = R + v.n;
-    float* yl = R + 2 * v.n;
-    float* ypl = yl + v.m;
+    char* base = reinterpret_cast<char*>(lds);
+    T* xl = reinterpret_cast<T*>(base);                              // resident x (RES only)
+    double* X = reinterpret_cast<double*>(base + (RES ? 8 * v.n : 0));
+    T* xb = reinterpret_cast<T*>(X);
+    T* yl = reinterpret_cast<T*>(base + (RES ? 16 * v.n : 8 * v.n));
+    int* rpl = reinterpret_cast<int*>(yl + v.m);
+    int* cpl = rpl + v.m + 1;
+    const int lr = v.lr, lc = v.lc;
+    const int lane_r = threadIdx.x & ((1 << lr) - 1), lane_c = threadIdx.x & ((1 << lc) - 1);
+    auto x_get = [&](int j) -> T {
+        if constexpr (RES) return xl[j];
+        else return v.x[j];
+    };
+    auto x_set = [&](int j, T x) {
+        if constexpr (RES) xl[j] = x;
+        else v.x[j] = x;
+    };
 
-    // ---- set-up: y into LDS, sums cleared, restart point = start, tau_hat from column sums of |K|
+    // ---- set-up: line pointers, x (resident) and y into LDS, sums cleared, restart point = start
+    for (int i = threadIdx.x; i <= v.m; i += NT) rpl[i] = v.rp[i];
+    for (int j = threadIdx.x; j <= v.n; j += NT) cpl[j] = v.cp[j];
     for (int i = threadIdx.x; i < v.m; i += NT) {
-        yl[i] = v.y[i];
-        ypl[i] = 0.f;
+        yl[i] = (T)v.y[i];
+        v.y_psum[i] = 0.f;
         v.y_sum[i] = 0.0;
         v.y_last[i] = v.y[i];
     }
     for (int j = threadIdx.x; j < v.n; j += NT) {
-        aty[j] = 0.f;
+        if constexpr (RES) xl[j] = (T)v.x[j];
         v.x_sum[j] = 0.0;
         v.x_psum[j] = 0.f;
         v.x_last[j] = v.x[j];
     }
     __syncthreads();
-    {
-        const int g = threadIdx.x / TPR, lane = threadIdx.x % TPR;
-        for (int i = g; i < v.m; i += NT / TPR)
-            for (int q = v.rp[i] + lane; q < v.rp[i + 1]; q += TPR) atomicAdd(&aty[v.ci[q]], fabsf(v.val[q]));
-    }
-    __syncthreads();
-    for (int j = threadIdx.x; j < v.n; j += NT) {
-        const float s = aty[j];
-        v.tau[j] = 0.999f / (s > 1e-4f ? s : 1e-4f);
+    // tau_hat_j = 0.999 / max(sum_i |K_ij|, 1e-4) from the column's entries
+    for (int j = threadIdx.x >> lc; j < v.n; j += NT >> lc) {
+        float s = 0.f;
+        for (int q = cpl[j] + lane_c; q < cpl[j + 1]; q += 1 << lc) s += fabsf(v.tval[q]);
+        s = group_sum_rt(s, lc);
+        if (lane_c == 0) v.tau[j] = 0.999f / (s > 1e-4f ? s : 1e-4f);
     }
     __syncthreads();
 
@@ -251,12 +369,11 @@ __device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red) {
     if (!(w_start > 0.f)) {
         double nw[2] = {0.0, 0.0};
         for (int j = threadIdx.x; j < v.n; j += NT) nw[1] += v.c64[j] * v.c64[j] * (double)v.tau[j];
-        const int g = threadIdx.x / TPR, lane = threadIdx.x % TPR;
-        for (int i = g; i < v.m; i += NT / TPR) {
+        for (int i = threadIdx.x >> lr; i < v.m; i += NT >> lr) {
             double s = 0.0;
-            for (int q = v.rp[i] + lane; q < v.rp[i + 1]; q += TPR) s += fabs((double)v.val[q]);
-            if constexpr (TPR > 1) s = group_sum<TPR>(s);
-            if (lane == 0) nw[0] += v.b64[i] * v.b64[i] * 0.999 / (s > 1e-4 ? s : 1e-4);
+            for (int q = rpl[i] + lane_r; q < rpl[i + 1]; q += 1 << lr) s += fabs((double)v.val[q]);
+            s = group_sum_rt(s, lr);
+            if (lane_r == 0) nw[0] += v.b64[i] * v.b64[i] * 0.999 / (s > 1e-4 ? s : 1e-4);
         }
         block_sum_all<NT, 2>(nw, red);
         const double nb = sqrt(nw[0]), nc = sqrt(nw[1]);
@@ -264,53 +381,94 @@ __device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red) {
     }
 
     double acc[NS], acc2[NS];
-    kkt_eval<NT, TPR>(v, R, yl, 0, 1.0, acc, red);
+    kkt_eval<NT, T>(v, X, xl, yl, rpl, cpl, 0, 1.0, acc, red);
     Kkt kk = kkt_of(acc, v);
     double ref_err = kk.err, err_prev = INFINITY;
     int status = isfinite(kk.err) ? PDLP_RUNNING : PDLP_NUMERICAL;
     int window = 0, restarts = 0, checks = 0, it = 0, done_it = 0, last_restart = 0;
     bool end_on_avg = false;
     double w_final = 1.0;
-    float wgt = w_start;
-    rebuild_aty<NT, TPR>(v, R, yl);
+    T wgt = (T)w_start;
 
     const int ce = a.check_every;
-    const int g = threadIdx.x / TPR, lane = threadIdx.x % TPR;
+    const int sr = NT >> lr, sc = NT >> lc;   // line groups per workgroup
+    const T one = 1, two_ = 2, step_num = (T)0.999, floor_ = (T)1e-4;
     while (status == PDLP_RUNNING && it < a.max_iters) {
-        const float inv_wgt = 1.f / wgt;
-        // ---- column pass
-        for (int j = threadIdx.x; j < v.n; j += NT) {
-            const float xo = v.x[j];
-            float xn = xo - (wgt * v.tau[j]) * (v.c[j] - aty[j]);
-            xn = xn < v.lo[j] ? v.lo[j] : xn;
-            xn = xn > v.hi[j] ? v.hi[j] : xn;
-            xb[j] = 2.f * xn - xo;
-            aty[j] = 0.f;
-            v.x[j] = xn;
-            v.x_psum[j] += xn;
+        const T inv_wgt = one / wgt;
+        // ---- column pass: aty = K^T y (gathered from LDS), primal step, xbar
+        for (int j = threadIdx.x >> lc; j < v.n; j += LINES * sc) {
+            const int j2 = j + sc;
+            const bool two = LINES == 2 && j2 < v.n;
+            const int jb = two ? j2 : j;
+            const T xo1 = x_get(j), t1 = (T)v.tau[j], l1 = (T)v.lo[j], h1 = (T)v.hi[j];
+            const T xo2 = x_get(jb), t2 = (T)v.tau[jb], l2 = (T)v.lo[jb], h2 = (T)v.hi[jb];
+            T c1, c2;
+            if constexpr (RES) {
+                c1 = v.c64[j];
+                c2 = v.c64[jb];
+            } else {
+                c1 = v.c[j];
+                c2 = v.c[jb];
+            }
+            T s1 = 0, s2 = 0, u1 = 0, u2 = 0;
+            dot2<T>(v.tval, v.ri, yl, cpl[j], cpl[j + 1], two ? cpl[j2] : 0, two ? cpl[j2 + 1] : 0, lane_c, lc, s1,
+                    u1, s2, u2);
+            s1 = group_sum_rt(s1, lc);
+            s2 = group_sum_rt(s2, lc);
+            if (lane_c == 0) {
+                T xn = xo1 - (wgt * t1) * (c1 - s1);
+                xn = xn < l1 ? l1 : xn;
+                xn = xn > h1 ? h1 : xn;
+                xb[j] = two_ * xn - xo1;
+                x_set(j, xn);
+                if constexpr (RES) v.x_sum[j] += xn;
+                else v.x_psum[j] += xn;
+                if (two) {
+                    T xm = xo2 - (wgt * t2) * (c2 - s2);
+                    xm = xm < l2 ? l2 : xm;
+                    xm = xm > h2 ? h2 : xm;
+                    xb[j2] = two_ * xm - xo2;
+                    x_set(j2, xm);
+                    if constexpr (RES) v.x_sum[j2] += xm;
+                    else v.x_psum[j2] += xm;
+                }
+            }
         }
         __syncthreads();
-        // ---- row pass: dual update and K^T y+ scatter
-        for (int i = g; i < v.m; i += NT / TPR) {
-            const int beg = v.rp[i], end = v.rp[i + 1];
-            float d = 0.f, s = 0.f;
-            for (int q = beg + lane; q < end; q += TPR) {
-                const float kv = v.val[q];
-                d += kv * xb[v.ci[q]];
-                s += fabsf(kv);
+        // ---- row pass: K xbar (gathered from LDS), dual step
+        for (int i = threadIdx.x >> lr; i < v.m; i += LINES * sr) {
+            const int i2 = i + sr;
+            const bool two = LINES == 2 && i2 < v.m;
+            const int ib = two ? i2 : i;
+            T b1, b2;
+            if constexpr (RES) {
+                b1 = v.b64[i];
+                b2 = v.b64[ib];
+            } else {
+                b1 = v.b[i];
+                b2 = v.b[ib];
             }
-            if constexpr (TPR > 1) {
-                d = group_sum<TPR>(d);
-                s = group_sum<TPR>(s);
-            }
-            const float sigma = inv_wgt * (0.999f / (s > 1e-4f ? s : 1e-4f));
-            float yn = yl[i] + sigma * (v.b[i] - d);
-            if (v.kind[i] == PDLP_ROW_LE && yn > 0.f) yn = 0.f;
-            if (yn != 0.f)
-                for (int q = beg + lane; q < end; q += TPR) atomicAdd(&aty[v.ci[q]], v.val[q] * yn);
-            if (lane == 0) {
+            const bool le1 = v.kind[i] == PDLP_ROW_LE, le2 = v.kind[ib] == PDLP_ROW_LE;
+            T d1 = 0, d2 = 0, a1 = 0, a2 = 0;
+            dot2<T>(v.val, v.ci, xb, rpl[i], rpl[i + 1], two ? rpl[i2] : 0, two ? rpl[i2 + 1] : 0, lane_r, lr, d1,
+                    a1, d2, a2);
+            d1 = group_sum_rt(d1, lr);
+            a1 = group_sum_rt(a1, lr);
+            d2 = group_sum_rt(d2, lr);
+            a2 = group_sum_rt(a2, lr);
+            if (lane_r == 0) {
+                T yn = yl[i] + inv_wgt * (step_num / (a1 > floor_ ? a1 : floor_)) * (b1 - d1);
+                if (le1 && yn > (T)0) yn = 0;
                 yl[i] = yn;
-                ypl[i] += yn;
+                if constexpr (RES) v.y_sum[i] += yn;
+                else v.y_psum[i] += yn;
+                if (two) {
+                    T ym = yl[i2] + inv_wgt * (step_num / (a2 > floor_ ? a2 : floor_)) * (b2 - d2);
+                    if (le2 && ym > (T)0) ym = 0;
+                    yl[i2] = ym;
+                    if constexpr (RES) v.y_sum[i2] += ym;
+                    else v.y_psum[i2] += ym;
+                }
             }
         }
         __syncthreads();
@@ -318,22 +476,24 @@ __device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red) {
         ++window;
         if (window % ce != 0) continue;
 
-        // ---- check: fold partial sums, KKT of the candidate(s), decision
-        for (int j = threadIdx.x; j < v.n; j += NT) {
-            v.x_sum[j] += (double)v.x_psum[j];
-            v.x_psum[j] = 0.f;
+        // ---- check: fold partial sums (FP32 iterates), KKT of the candidate(s), decision
+        if constexpr (!RES) {
+            for (int j = threadIdx.x; j < v.n; j += NT) {
+                v.x_sum[j] += (double)v.x_psum[j];
+                v.x_psum[j] = 0.f;
+            }
+            for (int i = threadIdx.x; i < v.m; i += NT) {
+                v.y_sum[i] += (double)v.y_psum[i];
+                v.y_psum[i] = 0.f;
+            }
+            __syncthreads();
         }
-        for (int i = threadIdx.x; i < v.m; i += NT) {
-            v.y_sum[i] += (double)ypl[i];
-            ypl[i] = 0.f;
-        }
-        __syncthreads();
         const double w = (double)window;
-        kkt_eval<NT, TPR>(v, R, yl, 1, w, acc, red);
+        kkt_eval<NT, T>(v, X, xl, yl, rpl, cpl, 1, w, acc, red);
         Kkt ka = kkt_of(acc, v);
         bool use_avg = true;
         if (v.adaptive) {                       // the better of average and current iterate
-            kkt_eval<NT, TPR>(v, R, yl, 0, 1.0, acc2, red);
+            kkt_eval<NT, T>(v, X, xl, yl, rpl, cpl, 0, 1.0, acc2, red);
             const Kkt kc = kkt_of(acc2, v);
             if (!isfinite(ka.err) || kc.err < ka.err) {
                 use_avg = false;
@@ -367,40 +527,41 @@ __device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red) {
         if (restart) {
             if (v.adaptive && !v.fixed_weight && kk.dx > 1e-10 && kk.dy > 1e-10) {
                 const double lw = 0.5 * log(kk.dx / kk.dy) + 0.5 * log((double)wgt);
-                wgt = (float)clamp_weight(exp(lw), (double)w_start);
+                wgt = (T)clamp_weight(exp(lw), (double)w_start);
             }
             ref_err = kk.err;
             err_prev = INFINITY;
             last_restart = it;
             ++restarts;
             for (int j = threadIdx.x; j < v.n; j += NT) {
-                if (use_avg) v.x[j] = (float)(v.x_sum[j] / w);
+                if (use_avg) x_set(j, (T)(v.x_sum[j] / w));
                 v.x_sum[j] = 0.0;
-                if (v.move_ref) v.x_last[j] = v.x[j];
+                if (v.move_ref) v.x_last[j] = (float)x_get(j);
             }
             for (int i = threadIdx.x; i < v.m; i += NT) {
-                if (use_avg) yl[i] = (float)(v.y_sum[i] / w);
+                if (use_avg) yl[i] = (T)(v.y_sum[i] / w);
                 v.y_sum[i] = 0.0;
-                if (v.move_ref) v.y_last[i] = yl[i];
+                if (v.move_ref) v.y_last[i] = (float)yl[i];
             }
             window = 0;
-            __syncthreads();
         } else {
             err_prev = kk.err;
         }
-        rebuild_aty<NT, TPR>(v, R, yl);
+        __syncthreads();
     }
 
-    // ---- outputs: FP64 solution in x_sum / y_sum
+    // ---- outputs: FP64 solution in x_sum / y_sum, last iterate in x / y
     if (status == PDLP_OPTIMAL && end_on_avg) {
         for (int j = threadIdx.x; j < v.n; j += NT) v.x_sum[j] = v.x_sum[j] / w_final;
         for (int i = threadIdx.x; i < v.m; i += NT) v.y_sum[i] = v.y_sum[i] / w_final;
     } else {
-        for (int j = threadIdx.x; j < v.n; j += NT) v.x_sum[j] = (double)v.x[j];
+        for (int j = threadIdx.x; j < v.n; j += NT) v.x_sum[j] = (double)x_get(j);
         for (int i = threadIdx.x; i < v.m; i += NT) v.y_sum[i] = (double)yl[i];
         if (status == PDLP_RUNNING) done_it = it;
     }
-    for (int i = threadIdx.x; i < v.m; i += NT) v.y[i] = yl[i];
+    if constexpr (RES)
+        for (int j = threadIdx.x; j < v.n; j += NT) v.x[j] = (float)xl[j];
+    for (int i = threadIdx.x; i < v.m; i += NT) v.y[i] = (float)yl[i];
     if (threadIdx.x == 0) {
         PdlpBatchOut o;
         o.status = status;
@@ -419,28 +580,50 @@ __device__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red) {
     __syncthreads();
 }
 
-template <int NT>
-__global__ void __launch_bounds__(NT) batch_kernel(PdlpBatch a) {
+// Persistent workgroups over one part of the work order: LPs order[first ..
+// first + count), taken through the counter *queue.  T = float: FP32 part,
+// T = double: FP64 part (each instantiation has its own register allocation).
+template <int NT, typename T>
+__global__ void __launch_bounds__(NT) batch_kernel(PdlpBatch a, int first, int count, int* queue, int lds_limit) {
     extern __shared__ float lds[];
     __shared__ double red[NS * (NT / kWave)];
     __shared__ int s_q;
+    constexpr long NB = std::is_same<T, double>::value ? 20 : 12, MB = std::is_same<T, double>::value ? 12 : 8;
     for (;;) {
-        if (threadIdx.x == 0) s_q = atomicAdd(a.queue, 1);
+        if (threadIdx.x == 0) s_q = atomicAdd(queue, 1);
         __syncthreads();
         // uniform values: the LP's descriptor and pointers then live in scalar registers
         const int q = __builtin_amdgcn_readfirstlane(s_q);
         __syncthreads();
-        if (q >= a.count) return;                 // every wave of the workgroup leaves here
-        const int k = __builtin_amdgcn_readfirstlane(a.order[q]);
-        switch (a.lps[k].tpr) {
-            case 1: solve_one<NT, 1>(a, k, lds, red); break;
-            case 2: solve_one<NT, 2>(a, k, lds, red); break;
-            case 4: solve_one<NT, 4>(a, k, lds, red); break;
-            case 8: solve_one<NT, 8>(a, k, lds, red); break;
-            case 16: solve_one<NT, 16>(a, k, lds, red); break;
-            case 32: solve_one<NT, 32>(a, k, lds, red); break;
-            default: solve_one<NT, 64>(a, k, lds, red); break;
-        }
+        if (q >= count) return;                   // every wave of the workgroup leaves here
+        const int k = __builtin_amdgcn_readfirstlane(a.order[first + q]);
+        const int m = __builtin_amdgcn_readfirstlane(a.lps[k].m);
+        const int n = __builtin_amdgcn_readfirstlane(a.lps[k].n);
+        // an LP that does not fit the launch's LDS is not run (its output record
+        // stays as it was); the packer sizes the LDS for every LP of the part
+        if (NB * n + MB * m + 8 <= lds_limit) solve_one<NT, T>(a, k, lds, red);
+    }
+}
+
+template <int NT, typename T>
+int launch(const PdlpBatch* a, int first, int count, int* queue, int lds, int grid, hipStream_t s) {
+    const hipError_t e = hipFuncSetAttribute((const void*)batch_kernel<NT, T>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
+    batch_kernel<NT, T><<<grid, NT, lds, s>>>(*a, first, count, queue, lds);
+    return hip_fail(hipGetLastError(), "batch_kernel launch");
+}
+
+template <typename T>
+int launch_part(const PdlpBatch* a, int first, int count, int* queue, int block, int lds, int grid, int cus,
+                hipStream_t s) {
+    if (count == 0) return 0;
+    if (lds <= 0 || lds > 160 * 1024) return fail(PDLP_E_LIMIT, "LDS per workgroup must be in (0, 160 KiB]");
+    const int g = std::min(count, grid > 0 ? grid : 2 * cus);
+    switch (block) {
+        case 256: return launch<256, T>(a, first, count, queue, lds, g, s);
+        case 512: return launch<512, T>(a, first, count, queue, lds, g, s);
+        default: return fail(PDLP_E_TUNE, "block must be 256 or 512");
     }
 }
 
@@ -452,43 +635,23 @@ using namespace pdlp;
 extern "C" int pdlp_batch_solve(const PdlpBatch* a, void* stream) {
     if (a == nullptr) return fail(PDLP_E_ARG, "PdlpBatch is NULL");
     if (a->count < 0) return fail(PDLP_E_ARG, "count must be >= 0");
+    if (a->n_resident < 0 || a->n_resident > a->count) return fail(PDLP_E_ARG, "n_resident must be in [0, count]");
     if (a->count == 0) return 0;
-    if (!a->lps || !a->order || !a->queue || !a->row_ptr || !a->col_idx || !a->val || !a->c || !a->lo ||
-        !a->hi || !a->c64 || !a->lo_shift || !a->ub_span || !a->b || !a->row_kind || !a->b64 || !a->b_shift ||
-        !a->x || !a->x_psum || !a->x_sum || !a->tau || !a->y || !a->y_sum || !a->out || !a->x_last || !a->y_last)
+    if (!a->lps || !a->order || !a->queue || !a->row_ptr || !a->col_idx || !a->val || !a->t_col_ptr ||
+        !a->t_row_idx || !a->t_val || !a->c || !a->lo || !a->hi || !a->c64 || !a->lo_shift || !a->ub_span ||
+        !a->b || !a->row_kind || !a->b64 || !a->b_shift || !a->x || !a->x_psum || !a->x_sum || !a->tau || !a->y ||
+        !a->y_sum || !a->out || !a->x_last || !a->y_last || !a->y_psum)
         return fail(PDLP_E_ARG, "a required PdlpBatch pointer is NULL");
     if (a->check_every <= 0 || a->max_iters < 0) return fail(PDLP_E_ARG, "check_every > 0 and max_iters >= 0");
-    if (a->lds_bytes <= 0 || a->lds_bytes > 160 * 1024)
-        return fail(PDLP_E_LIMIT, "lds_bytes must be in (0, 160 KiB]: an LP needs 8*(m+n) bytes");
     hipStream_t s = as_stream(stream);
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) cus = v;
     }
-    const unsigned grid = (unsigned)std::min(a->count, a->grid > 0 ? a->grid : 2 * cus);
-    hipError_t e;
-    switch (a->block) {
-        case 256:
-            e = hipFuncSetAttribute((const void*)batch_kernel<256>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    a->lds_bytes);
-            if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
-            batch_kernel<256><<<grid, 256, a->lds_bytes, s>>>(*a);
-            break;
-        case 512:
-            e = hipFuncSetAttribute((const void*)batch_kernel<512>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    a->lds_bytes);
-            if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
-            batch_kernel<512><<<grid, 512, a->lds_bytes, s>>>(*a);
-            break;
-        case 1024:
-            e = hipFuncSetAttribute((const void*)batch_kernel<1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    a->lds_bytes);
-            if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
-            batch_kernel<1024><<<grid, 1024, a->lds_bytes, s>>>(*a);
-            break;
-        default:
-            return fail(PDLP_E_TUNE, "block must be 256, 512 or 1024");
-    }
-    return hip_fail(hipGetLastError(), "batch_kernel launch");
+    const int n32 = a->count - a->n_resident;
+    int rc = launch_part<float>(a, 0, n32, a->queue, a->block, a->lds_bytes, a->grid, cus, s);
+    if (rc != 0) return rc;
+    return launch_part<double>(a, n32, a->n_resident, a->queue + 1, a->block_resident ? a->block_resident : 256,
+                               a->lds_resident, a->grid_resident, cus, s);
 }

@@ -13,7 +13,7 @@ import threading
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "_lib", "libpdlp_hip.so")
 
-PDLP_ABI_VERSION = 1
+PDLP_ABI_VERSION = 2
 PDLP_F32, PDLP_F64 = 0, 1
 PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL, PDLP_INFEASIBLE = 0, 1, 3, 4
 PDLP_ROW_EQ, PDLP_ROW_LE = 0, 1
@@ -69,15 +69,17 @@ class PdlpTune(C.Structure):
 class PdlpBatch(C.Structure):
     _fields_ = [
         ("count", C.c_int32), ("check_every", C.c_int32), ("max_iters", C.c_int32), ("block", C.c_int32),
-        ("lds_bytes", C.c_int32), ("grid", C.c_int32), ("tol", C.c_double),
+        ("lds_bytes", C.c_int32), ("grid", C.c_int32), ("n_resident", C.c_int32), ("block_resident", C.c_int32),
+        ("lds_resident", C.c_int32), ("grid_resident", C.c_int32), ("tol", C.c_double),
         ("lps", C.c_void_p), ("order", C.c_void_p), ("queue", C.c_void_p),
         ("row_ptr", C.c_void_p), ("col_idx", C.c_void_p), ("val", C.c_void_p),
+        ("t_col_ptr", C.c_void_p), ("t_row_idx", C.c_void_p), ("t_val", C.c_void_p),
         ("c", C.c_void_p), ("lo", C.c_void_p), ("hi", C.c_void_p),
         ("c64", C.c_void_p), ("lo_shift", C.c_void_p), ("ub_span", C.c_void_p),
         ("b", C.c_void_p), ("row_kind", C.c_void_p), ("b64", C.c_void_p), ("b_shift", C.c_void_p),
         ("x", C.c_void_p), ("x_psum", C.c_void_p), ("x_sum", C.c_void_p), ("tau", C.c_void_p),
         ("y", C.c_void_p), ("y_sum", C.c_void_p), ("out", C.c_void_p),
-        ("x_last", C.c_void_p), ("y_last", C.c_void_p),
+        ("x_last", C.c_void_p), ("y_last", C.c_void_p), ("y_psum", C.c_void_p),
     ]
 
 

@@ -23,8 +23,9 @@ from .forms import ROW_EQ, ROW_LE
 from ..problem import Problem
 
 LP_DTYPE = np.dtype([
-    ("m", np.int32), ("n", np.int32), ("tpr", np.int32), ("flags", np.int32),
-    ("row_off", np.int64), ("col_off", np.int64), ("rp_off", np.int64), ("nnz_off", np.int64),
+    ("m", np.int32), ("n", np.int32), ("tpr", np.int32), ("flags", np.int32), ("tpc", np.int32),
+    ("reserved", np.int32),
+    ("row_off", np.int64), ("col_off", np.int64), ("rp_off", np.int64), ("nnz_off", np.int64), ("cp_off", np.int64),
     ("norm_b", np.float64), ("norm_c", np.float64), ("tau_scale", np.float64), ("sigma_scale", np.float64),
 ], align=True)
 
@@ -39,12 +40,36 @@ _STATUS = {L.PDLP_RUNNING: "iteration_limit", L.PDLP_OPTIMAL: "optimal", L.PDLP_
            L.PDLP_INFEASIBLE: "infeasible"}
 
 
+ENTRIES_PER_LANE = 8     # entries each lane loads at once in a pass (EL in pdlp_batch.hip)
+
+
 def lanes_per_row(mean_len: float) -> int:
-    """About two entries per lane, a power of two in [1, 64]."""
+    """Lanes per line (row or column) for a mean line length: the smallest power
+    of two in [1, 64] whose lanes hold a line of mean + 2 sqrt(mean) entries
+    (most lines of a random sparse matrix) in one load of eight entries each.
+    Fewer lanes per line mean more lines per round."""
+    want = (mean_len + 2.0 * np.sqrt(max(mean_len, 0.0))) / ENTRIES_PER_LANE
     t = 1
-    while t * 2 <= mean_len / 2 and t < 64:
+    while t < want and t < 64:
         t *= 2
     return t
+
+
+def lds_bytes_of(m, n):
+    """LDS bytes one LP needs in the persistent kernel with FP32 iterates: n
+    doubles (xbar / FP64 primal point), m floats (y), m+1 and n+1 line pointers."""
+    return 12 * np.asarray(n) + 8 * np.asarray(m) + 8
+
+
+def lds_bytes_resident(m, n):
+    """LDS bytes with FP64 iterates and x in LDS (the FP64 part): x, xbar (n
+    doubles each), y (m doubles) and the line pointers."""
+    return 20 * np.asarray(n) + 12 * np.asarray(m) + 8
+
+
+# LPs whose FP64 part needs at most this much LDS run with FP64 iterates (two
+# workgroups per CU); larger ones with FP32 iterates.
+RESIDENT_LDS = 64 * 1024
 
 
 def _segment_sums(v: np.ndarray, starts: np.ndarray, lens: np.ndarray) -> np.ndarray:
@@ -106,6 +131,15 @@ class PackedBatch:
         self.row_ptr = np.concatenate([K.indptr.astype(np.int32) for K in stored])
         self.col_idx = np.concatenate([K.indices.astype(np.int32) for K in stored])
         self.val = np.concatenate([K.data for K in stored])
+        # K^T by rows (K by columns), same entry count per matrix
+        stored_t = [K.T.tocsr() for K in stored]
+        for Kt in stored_t:
+            Kt.sort_indices()
+        s_cp = np.concatenate([[0], np.cumsum(s_n + 1)[:-1]])
+        self.cp_off = s_cp[mat_of]
+        self.t_col_ptr = np.concatenate([Kt.indptr.astype(np.int32) for Kt in stored_t])
+        self.t_row_idx = np.concatenate([Kt.indices.astype(np.int32) for Kt in stored_t])
+        self.t_val = np.concatenate([Kt.data for Kt in stored_t])
 
         P = self.problems
         c = np.concatenate([np.asarray(p.c, float) for p in P])
@@ -125,7 +159,9 @@ class PackedBatch:
         desc = np.zeros(B, LP_DTYPE)
         desc["m"], desc["n"] = ms, ns
         desc["tpr"] = [lanes_per_row(z / m) for z, m in zip(nnzs, ms)]
+        desc["tpc"] = [lanes_per_row(z / n) for z, n in zip(nnzs, ns)]
         desc["row_off"], desc["col_off"], desc["rp_off"], desc["nnz_off"] = row_off, col_off, rp_off, nnz_off
+        desc["cp_off"] = self.cp_off
         desc["norm_b"] = np.maximum(1.0, np.sqrt(_segment_sums(b_shift * b_shift, row_off, ms) +
                                                  _segment_sums(span2, col_off, ns)))
         desc["norm_c"] = np.maximum(1.0, np.sqrt(_segment_sums(c * c, col_off, ns)))
@@ -135,20 +171,28 @@ class PackedBatch:
         self.c, self.lo, self.hi, self.b, self.kind = c, lo, hi, b, kind
         self.lo_shift, self.ub_span, self.b_shift = lo_shift, ub_span, b_shift
         self.desc = desc
-        self.order = np.argsort(-nnzs, kind="stable").astype(np.int32)     # largest first
-        self.lds_bytes = int(((8 * (ms + ns)).max() + 15) // 16 * 16)
+        # FP32 part (largest first), then the FP64 part (largest first)
+        self.resident = lds_bytes_resident(ms, ns) <= RESIDENT_LDS
+        by_size = np.argsort(-nnzs, kind="stable")
+        self.order = np.concatenate([by_size[~self.resident[by_size]], by_size[self.resident[by_size]]]).astype(np.int32)
+        self.n_resident = int(self.resident.sum())
+        r16 = lambda v: int((int(v) + 15) // 16 * 16)
+        self.lds_bytes = r16(lds_bytes_of(ms, ns)[~self.resident].max()) if self.n_resident < B else 16
+        self.lds_resident = r16(lds_bytes_resident(ms, ns)[self.resident].max()) if self.n_resident else 16
 
     def algorithmic_bytes_per_iteration(self) -> float:
-        """Bytes one iteration of every LP moves from HBM at minimum: the CSR once
-        (values, column indices, row pointers) and the per-column state."""
-        return float(self.Z * 8 + (self.M + len(self.problems)) * 4 + self.N * (4 * 6 + 4 * 2) + self.M * 5)
+        """Bytes one iteration of every LP moves at minimum: K and K^T once each
+        (values and indices; line pointers stay in LDS) and the per-column and
+        per-row state."""
+        return float(self.Z * 16 + self.N * (4 * 6 + 4 * 2) + self.M * 5)
 
 
 class BatchEngine:
     """Device arena of a :class:`PackedBatch`.  ``scheme``: "adaptive" (default)
     or "reference" restarts for every LP (see ``include/pdlp_hip.h``)."""
 
-    def __init__(self, packed: PackedBatch, block: int = 0, grid: int = 0, scheme: str = "adaptive"):
+    def __init__(self, packed: PackedBatch, block: int = 0, grid: int = 0, scheme: str = "adaptive",
+                 block_resident: int = 256):
         if scheme not in ("adaptive", "reference", "adaptive_fixed", "adaptive_start"):
             raise ValueError("scheme must be 'adaptive', 'adaptive_fixed', 'adaptive_start' or 'reference'")
         self.lib = L.lib()
@@ -156,8 +200,10 @@ class BatchEngine:
         self.pk = packed
         self.adaptive = scheme != "reference"
         self.flags = {"reference": 0, "adaptive": 1, "adaptive_fixed": 3, "adaptive_start": 5}[scheme]
-        # block 0: 256 threads while two workgroups fit a CU's 160 KB of LDS, else 512
-        self.block = block or (256 if packed.lds_bytes <= 80 * 1024 else 512)
+        # FP32 part (the large LPs): 512 threads, one workgroup per CU, measured
+        # faster than two of 256 on the uneven batch; FP64 part: 256 threads
+        self.block = block or 512
+        self.block_resident = block_resident
         self.grid = grid
         self.stream = torch.cuda.Stream(device=self.device)
         dv = self.device
@@ -167,10 +213,13 @@ class BatchEngine:
             self.t = {
                 "lps": torch.from_numpy(pk.desc.view(np.uint8).copy()).to(dv),
                 "order": dev_tensor(pk.order, i32, dv),
-                "queue": torch.zeros(1, dtype=i32, device=dv),
+                "queue": torch.zeros(2, dtype=i32, device=dv),
                 "row_ptr": dev_tensor(pk.row_ptr, i32, dv),
                 "col_idx": dev_tensor(pk.col_idx, i32, dv),
                 "val": dev_tensor(pk.val, f32, dv),
+                "t_col_ptr": dev_tensor(pk.t_col_ptr, i32, dv),
+                "t_row_idx": dev_tensor(pk.t_row_idx, i32, dv),
+                "t_val": dev_tensor(pk.t_val, f32, dv),
                 "c": dev_tensor(pk.c, f32, dv), "lo": dev_tensor(pk.lo, f32, dv), "hi": dev_tensor(pk.hi, f32, dv),
                 "c64": dev_tensor(pk.c, f64, dv), "lo_shift": dev_tensor(pk.lo_shift, f64, dv),
                 "ub_span": dev_tensor(pk.ub_span, f64, dv),
@@ -181,6 +230,7 @@ class BatchEngine:
                 "y": torch.zeros(pk.M, dtype=f32, device=dv), "y_sum": torch.zeros(pk.M, dtype=f64, device=dv),
                 "out": torch.zeros(len(pk.problems) * OUT_DTYPE.itemsize, dtype=torch.uint8, device=dv),
                 "x_last": torch.zeros(pk.N, dtype=f32, device=dv), "y_last": torch.zeros(pk.M, dtype=f32, device=dv),
+                "y_psum": torch.zeros(pk.M, dtype=f32, device=dv),
             }
         self.stream.synchronize()
 
@@ -213,10 +263,12 @@ class BatchEngine:
     def _args(self, tol: float, max_iters: int, check_every: int) -> PdlpBatch:
         t = self.t
         a = PdlpBatch(count=len(self.pk.problems), check_every=check_every, max_iters=max_iters, block=self.block,
-                      lds_bytes=self.pk.lds_bytes, grid=self.grid, tol=tol)
-        for name in ("lps", "order", "queue", "row_ptr", "col_idx", "val", "c", "lo", "hi", "c64", "lo_shift",
+                      lds_bytes=self.pk.lds_bytes, grid=self.grid, n_resident=self.pk.n_resident,
+                      block_resident=self.block_resident, lds_resident=self.pk.lds_resident, grid_resident=0, tol=tol)
+        for name in ("lps", "order", "queue", "row_ptr", "col_idx", "val", "t_col_ptr", "t_row_idx", "t_val",
+                     "c", "lo", "hi", "c64", "lo_shift",
                      "ub_span", "b", "row_kind", "b64", "b_shift", "x", "x_psum", "x_sum", "tau", "y", "y_sum",
-                     "out", "x_last", "y_last"):
+                     "out", "x_last", "y_last", "y_psum"):
             setattr(a, name, ptr(t[name]))
         return a
 

@@ -9,7 +9,7 @@ FP64 KKT, check cadence 50; ``scheme="adaptive"`` (default) or
 ``"reference"`` restarts as in :func:`solve_lp_pdhg`).
 
 ``engine``: ``"persistent"`` (one workgroup per LP, any mix of matrices; an
-LP needs 8(m+n) bytes of LDS), ``"family"`` (all LPs share one constraint
+LP needs 12n+8m+8 bytes of LDS), ``"family"`` (all LPs share one constraint
 matrix and step in lockstep, 64 LPs per wavefront; no size limit) or
 ``"auto"`` (family when every problem holds the same ``A_ub``/``A_eq``
 objects and there are at least 16 of them, or when an LP is too large for
@@ -28,7 +28,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 from ..config import MAX_PDHG_ITERS
-from ..engine.batch import BatchEngine, PackedBatch
+from ..engine.batch import BatchEngine, PackedBatch, lds_bytes_of
 from ..engine.family import FamilyBatch, FamilyEngine
 from ..problem import Problem
 from ..result import SolveResult
@@ -116,7 +116,7 @@ def solve_lp_pdhg_batch(
     if live and engine == "auto":
         ps = [problems[k] for k in live]
         same = all(p.A_ub is ps[0].A_ub and p.A_eq is ps[0].A_eq for p in ps)
-        too_big = max(8 * (p.n_constraints + p.n_vars) for p in ps) > 160 * 1024
+        too_big = max(lds_bytes_of(p.n_constraints, p.n_vars) for p in ps) > 160 * 1024
         engine = "family" if (same and len(ps) >= 16) or (too_big and same) else "persistent"
     if live and engine == "family":
         fb = FamilyBatch([problems[k] for k in live])

@@ -2,15 +2,17 @@
 
 import numpy as np
 
-from sovereign_solver_amd.engine.batch import LP_DTYPE, OUT_DTYPE, PackedBatch, lanes_per_row
+from sovereign_solver_amd.engine.batch import (LP_DTYPE, OUT_DTYPE, PackedBatch, lanes_per_row, lds_bytes_of,
+                                               lds_bytes_resident)
 from sovereign_solver_amd.lp.batch import pack_warm_start
 from sovereign_solver_amd.workloads import random_sparse_lp, transportation_lp, uneven_batch, uneven_sizes
 
 
 def test_descriptor_layout_matches_header():
-    # PdlpBatchLP: 4 int32, 4 int64, 4 double; PdlpBatchOut: 4 int32, 7 double
-    assert LP_DTYPE.itemsize == 4 * 4 + 4 * 8 + 4 * 8
-    assert LP_DTYPE.fields["row_off"][1] == 16 and LP_DTYPE.fields["norm_b"][1] == 48
+    # PdlpBatchLP: 6 int32, 5 int64, 4 double; PdlpBatchOut: 4 int32, 7 double
+    assert LP_DTYPE.itemsize == 6 * 4 + 5 * 8 + 4 * 8
+    assert LP_DTYPE.fields["tpc"][1] == 16 and LP_DTYPE.fields["row_off"][1] == 24
+    assert LP_DTYPE.fields["cp_off"][1] == 56 and LP_DTYPE.fields["norm_b"][1] == 64
     assert OUT_DTYPE.itemsize == 4 * 4 + 7 * 8
     assert OUT_DTYPE.fields["rel_kkt"][1] == 16
 
@@ -31,11 +33,35 @@ def test_packing_offsets_and_contents():
             dense[i, ci[rp[i]:rp[i + 1]]] = va[rp[i]:rp[i + 1]]
         full = np.vstack([p.A_ub.toarray(), p.A_eq.toarray()])
         assert np.array_equal(dense, full)
+        # K^T by rows: column pointers at cp_off, entries at the same nnz_off
+        cp = pk.t_col_ptr[d["cp_off"]: d["cp_off"] + n + 1]
+        assert cp[0] == 0 and cp[-1] == pk.nnzs[k]
+        ri = pk.t_row_idx[d["nnz_off"]: d["nnz_off"] + cp[-1]]
+        tv = pk.t_val[d["nnz_off"]: d["nnz_off"] + cp[-1]]
+        dense_t = np.zeros((n, m))
+        for j in range(n):
+            assert np.all(np.diff(ri[cp[j]:cp[j + 1]]) > 0)      # rows ascending within a column
+            dense_t[j, ri[cp[j]:cp[j + 1]]] = tv[cp[j]:cp[j + 1]]
+        assert np.array_equal(dense_t, full.T)
         assert np.array_equal(pk.c[d["col_off"]: d["col_off"] + n], p.c)
         kinds = pk.kind[d["row_off"]: d["row_off"] + m]
         assert list(kinds) == [1] * p.n_ineq + [0] * p.n_eq
-    assert pk.lds_bytes >= max(8 * (p.n_constraints + p.n_vars) for p in probs)
+    # small LPs: all in the FP64 part, whose LDS covers x, xbar, y and the line pointers
+    assert pk.n_resident == len(probs) and pk.resident.all()
+    assert pk.lds_resident >= max(12 * p.n_constraints + 20 * p.n_vars + 8 for p in probs)
     assert list(pk.order) == list(np.argsort(-pk.nnzs, kind="stable"))
+
+
+def test_fp32_and_fp64_parts():
+    # LPs over 64 KB of FP64-part LDS run in the FP32 part, which comes first in the work order
+    probs = [random_sparse_lp(1100, 2750, 0.002, seed=1), random_sparse_lp(60, 150, 0.05, seed=2),
+             random_sparse_lp(1300, 3250, 0.002, seed=3), random_sparse_lp(200, 500, 0.02, seed=4)]
+    pk = PackedBatch(probs)
+    assert list(pk.resident) == [False, True, False, True] and pk.n_resident == 2
+    assert list(pk.order) == [2, 0, 3, 1]              # FP32 part, then FP64 part, each largest first
+    assert pk.lds_bytes == (int(lds_bytes_of(1300, 3250)) + 15) // 16 * 16
+    assert pk.lds_resident == (int(lds_bytes_resident(200, 500)) + 15) // 16 * 16
+    assert pk.lds_bytes <= 160 * 1024 and pk.lds_resident <= 64 * 1024
 
 
 def test_uneven_family_statistics():
@@ -51,7 +77,9 @@ def test_uneven_family_statistics():
 
 
 def test_lanes_per_row():
-    assert lanes_per_row(1.2) == 1 and lanes_per_row(4) == 2 and lanes_per_row(50) == 16
+    # lanes x 8 entries cover mean + 2 sqrt(mean) entries
+    assert lanes_per_row(1.2) == 1 and lanes_per_row(4) == 1 and lanes_per_row(4.5) == 2
+    assert lanes_per_row(17) == 4 and lanes_per_row(42) == 8 and lanes_per_row(50) == 16
     assert lanes_per_row(10_000) == 64
 
 
