@@ -292,9 +292,10 @@ typedef struct PdlpBatch {
     const PdlpBatchLP* lps; /* count descriptors                              */
     const int32_t* order;   /* count LP indices, processing order: FP32 part, then FP64 part */
     int32_t* queue;         /* two int32 work counters (FP32, FP64 part), zero before the launch */
-    /* matrices: K by rows, K^T by rows (= K by columns) */
-    const int32_t* row_ptr; const int32_t* col_idx; const float* val;
-    const int32_t* t_col_ptr; const int32_t* t_row_idx; const float* t_val;
+    /* matrices: K by rows, K^T by rows (= K by columns); 16-bit indices within
+       the LP (every LP that fits in LDS has m, n < 32768) */
+    const int32_t* row_ptr; const uint16_t* col_idx; const float* val;
+    const int32_t* t_col_ptr; const uint16_t* t_row_idx; const float* t_val;
     /* per column: c, bounds (+-inf allowed), FP64 c, lower-bound shift, (hi - shift) or +inf */
     const float *c, *lo, *hi;
     const double *c64, *lo_shift, *ub_span;

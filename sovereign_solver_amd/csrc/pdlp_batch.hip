@@ -52,10 +52,10 @@ constexpr int NS = 6;   // KKT sums: rp^2, rd^2, primal obj, dual obj, |dx|^2, |
 struct LPView {
     int m, n;
     const GMEM int32_t* rp;     // m+1, relative to the nnz base
-    const GMEM int32_t* ci;     // relative to the column base
+    const GMEM uint16_t* ci;    // relative to the column base
     const GMEM float* val;
     const GMEM int32_t* cp;     // n+1 column pointers of K^T, relative to the nnz base
-    const GMEM int32_t* ri;     // row index of each K^T entry, relative to the row base
+    const GMEM uint16_t* ri;    // row index of each K^T entry, relative to the row base
     const GMEM float* tval;
     const GMEM float *c, *lo, *hi;
     const GMEM double *c64, *lo_shift, *ub_span;
@@ -146,7 +146,7 @@ __device__ __forceinline__ double group_sum_rt(double v, int lw) {
 // lane's entries of [b_k, e_k) (entries lane, lane + 2^lw, ...).  The first EL
 // entries per lane of both lines are loaded before any is used.
 template <typename T>
-__device__ __forceinline__ void dot2(const GMEM float* val, const GMEM int32_t* idx, const T* vec, int b1, int e1,
+__device__ __forceinline__ void dot2(const GMEM float* val, const GMEM uint16_t* idx, const T* vec, int b1, int e1,
                                      int b2, int e2, int lane, int lw, T& s1, T& a1, T& s2, T& a2) {
     float v1[EL], v2[EL];
     int c1[EL], c2[EL];
@@ -318,7 +318,6 @@ __device__ __forceinline__ Kkt kkt_of(const double (&s)[NS], const LPView& v) {
 template <int NT, typename T>
 __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red) {
     constexpr bool RES = std::is_same<T, double>::value;
-    constexpr int LINES = RES ? 1 : 2;   // lines per group and round (FP64: one, for registers)
     const PdlpBatchLP lp = a.lps[k];
     const LPView v = view_of(a, lp);
     char* base = reinterpret_cast<char*>(lds);
@@ -396,9 +395,9 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
     while (status == PDLP_RUNNING && it < a.max_iters) {
         const T inv_wgt = one / wgt;
         // ---- column pass: aty = K^T y (gathered from LDS), primal step, xbar
-        for (int j = threadIdx.x >> lc; j < v.n; j += LINES * sc) {
+        for (int j = threadIdx.x >> lc; j < v.n; j += 2 * sc) {
             const int j2 = j + sc;
-            const bool two = LINES == 2 && j2 < v.n;
+            const bool two = j2 < v.n;
             const int jb = two ? j2 : j;
             const T xo1 = x_get(j), t1 = (T)v.tau[j], l1 = (T)v.lo[j], h1 = (T)v.hi[j];
             const T xo2 = x_get(jb), t2 = (T)v.tau[jb], l2 = (T)v.lo[jb], h2 = (T)v.hi[jb];
@@ -436,9 +435,9 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
         }
         __syncthreads();
         // ---- row pass: K xbar (gathered from LDS), dual step
-        for (int i = threadIdx.x >> lr; i < v.m; i += LINES * sr) {
+        for (int i = threadIdx.x >> lr; i < v.m; i += 2 * sr) {
             const int i2 = i + sr;
-            const bool two = LINES == 2 && i2 < v.m;
+            const bool two = i2 < v.m;
             const int ib = two ? i2 : i;
             T b1, b2;
             if constexpr (RES) {

@@ -114,6 +114,10 @@ class PackedBatch:
         bad = np.flatnonzero((ms == 0) | (ns == 0))
         if bad.size:
             raise ValueError(f"LP {int(bad[0])}: the batched engine needs m > 0 and n > 0")
+        big = np.flatnonzero(lds_bytes_of(ms, ns) > 160 * 1024)      # also keeps m, n below 2^15
+        if big.size:
+            raise ValueError(f"LP {int(big[0])} (m={int(ms[big[0]])}, n={int(ns[big[0]])}) needs more than a CU's "
+                             f"160 KB of LDS; solve it with the family or single-instance engine")
         self.ms, self.ns, self.nnzs = ms, ns, nnzs
         self.n_matrices = len(stored)
         row_off = np.concatenate([[0], np.cumsum(ms)[:-1]])
@@ -129,7 +133,7 @@ class PackedBatch:
         self.M, self.N, self.Z = M, N, Z
         self.Z_stored = int(s_z.sum())
         self.row_ptr = np.concatenate([K.indptr.astype(np.int32) for K in stored])
-        self.col_idx = np.concatenate([K.indices.astype(np.int32) for K in stored])
+        self.col_idx = np.concatenate([K.indices.astype(np.uint16) for K in stored])
         self.val = np.concatenate([K.data for K in stored])
         # K^T by rows (K by columns), same entry count per matrix
         stored_t = [K.T.tocsr() for K in stored]
@@ -138,7 +142,7 @@ class PackedBatch:
         s_cp = np.concatenate([[0], np.cumsum(s_n + 1)[:-1]])
         self.cp_off = s_cp[mat_of]
         self.t_col_ptr = np.concatenate([Kt.indptr.astype(np.int32) for Kt in stored_t])
-        self.t_row_idx = np.concatenate([Kt.indices.astype(np.int32) for Kt in stored_t])
+        self.t_row_idx = np.concatenate([Kt.indices.astype(np.uint16) for Kt in stored_t])
         self.t_val = np.concatenate([Kt.data for Kt in stored_t])
 
         P = self.problems
@@ -215,10 +219,10 @@ class BatchEngine:
                 "order": dev_tensor(pk.order, i32, dv),
                 "queue": torch.zeros(2, dtype=i32, device=dv),
                 "row_ptr": dev_tensor(pk.row_ptr, i32, dv),
-                "col_idx": dev_tensor(pk.col_idx, i32, dv),
+                "col_idx": dev_tensor(pk.col_idx.view(np.int16), torch.int16, dv),
                 "val": dev_tensor(pk.val, f32, dv),
                 "t_col_ptr": dev_tensor(pk.t_col_ptr, i32, dv),
-                "t_row_idx": dev_tensor(pk.t_row_idx, i32, dv),
+                "t_row_idx": dev_tensor(pk.t_row_idx.view(np.int16), torch.int16, dv),
                 "t_val": dev_tensor(pk.t_val, f32, dv),
                 "c": dev_tensor(pk.c, f32, dv), "lo": dev_tensor(pk.lo, f32, dv), "hi": dev_tensor(pk.hi, f32, dv),
                 "c64": dev_tensor(pk.c, f64, dv), "lo_shift": dev_tensor(pk.lo_shift, f64, dv),

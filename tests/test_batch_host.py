@@ -76,6 +76,13 @@ def test_uneven_family_statistics():
         assert abs(p.A_eq.nnz - round(0.01 * m * n)) <= 1
 
 
+def test_lp_beyond_lds_is_rejected():
+    import pytest
+    big = random_sparse_lp(200, 14_000, 0.001, seed=5)       # 12 n > 160 KB
+    with pytest.raises(ValueError, match="LDS"):
+        PackedBatch([random_sparse_lp(5, 9, 0.3, seed=1), big])
+
+
 def test_lanes_per_row():
     # lanes x 8 entries cover mean + 2 sqrt(mean) entries
     assert lanes_per_row(1.2) == 1 and lanes_per_row(4) == 1 and lanes_per_row(4.5) == 2
