@@ -118,7 +118,7 @@ def cpu_baseline(problem, iters_needed: int, sample_iters: int):
     }
 
 
-def batch_lp_bytes(pk, out, check_every: int):
+def batch_lp_bytes(pk, out, check_every: int, mask=None):
     """Algorithmic bytes the batched kernel moves, summed over LPs.
 
     Per iteration of LP k: K and K^T once each (values and indices, 8 B per
@@ -133,7 +133,8 @@ def batch_lp_bytes(pk, out, check_every: int):
     per_check = 4 * mat + 64 * n + 41 * m
     its = out["iterations"].astype(float)
     checks = out["checks"].astype(float) + 1
-    return float(np.sum(its * per_it + checks * per_check))
+    tot = its * per_it + checks * per_check
+    return float(np.sum(tot if mask is None else tot[mask]))
 
 
 def bench_batch(args):
@@ -159,24 +160,29 @@ def bench_batch(args):
     for s in range(args.steps):
         eng.set_warm_start(None, None, None)
         torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
         t1 = time.perf_counter()
-        e0.record(eng.stream)
-        eng.launch(args.tol, args.max_iters)
-        e1.record(eng.stream)
+        ev[0].record(eng.stream)                    # events on the engine's stream, around each part
+        eng.launch(args.tol, args.max_iters, parts=1)
+        ev[1].record(eng.stream)
+        eng.launch(args.tol, args.max_iters, parts=2)
+        ev[2].record(eng.stream)
         out, xs, ys = eng.results()
-        times.append((time.perf_counter() - t1, e0.elapsed_time(e1) / 1e3))
+        times.append((time.perf_counter() - t1, ev[0].elapsed_time(ev[2]) / 1e3, ev[0].elapsed_time(ev[1]) / 1e3,
+                      ev[1].elapsed_time(ev[2]) / 1e3))
         outs.append(out)
-        log(f"step {s}: wall {times[-1][0]:.3f}s kernel {times[-1][1]:.3f}s optimal "
-            f"{int((out['status'] == 1).sum())}/{len(probs)} iters mean {out['iterations'].mean():.0f} "
-            f"max {out['iterations'].max()}")
+        log(f"step {s}: wall {times[-1][0]:.3f}s kernels {times[-1][1]:.3f}s (FP32 part {times[-1][2]:.3f}s, "
+            f"FP64 part {times[-1][3]:.3f}s) optimal {int((out['status'] == 1).sum())}/{len(probs)} "
+            f"iters mean {out['iterations'].mean():.0f} max {out['iterations'].max()}")
     wall = sum(t[0] for t in times)
-    kern = sum(t[1] for t in times)
     n_ok = sum(int((o["status"] == 1).sum()) for o in outs)
     value = n_ok / wall
     bytes_alg = batch_lp_bytes(pk, outs[-1], 50)
-    gbs = bytes_alg / times[-1][1] / 1e9
+    # dominant kernel: the FP32 part (the large LPs)
+    bytes32 = batch_lp_bytes(pk, outs[-1], 50, mask=~pk.resident)
+    bytes64 = bytes_alg - bytes32
+    gbs = bytes32 / times[-1][2] / 1e9
+    gbs64 = bytes64 / times[-1][3] / 1e9 if times[-1][3] > 0 else None
 
     # plain single-instance GPU PDHG on a random subset, one LP after another
     rng = np.random.default_rng(123)
@@ -222,19 +228,25 @@ def bench_batch(args):
         "metric": "LPs/sec to 1e-4 rel KKT: learned+batched vs plain PDHG, 1 GPU; SpMV HBM GB/s",
         "value": value, "unit": "LPs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32", "data": "synthetic (seeded random sparse LPs, uneven sizes)",
+        "dtype": "fp32 iterates (large LPs), fp64 iterates (LPs within 64 KB of LDS); fp32 matrix, fp64 KKT",
+        "data": "synthetic (seeded random sparse LPs, uneven sizes)",
         "config": {"workload": f"batched restarted PDHG, {len(probs)} LPs, m log-uniform in [50,2000], n=2.5m, "
-                               f"density 1%, one persistent launch, cold start, tol {args.tol:g}",
+                               f"density 1%, two persistent launches (FP32 part, FP64 part), cold start, "
+                               f"tol {args.tol:g}",
                    "sum_m": pk.M, "sum_n": pk.N, "sum_nnz": pk.Z, "block": args.block,
+                   "lps_fp32_part": int(len(probs) - pk.n_resident), "lps_fp64_part": int(pk.n_resident),
+                   "kernel_fp32_part_s": times[-1][2], "kernel_fp64_part_s": times[-1][3],
+                   "fp64_part_achieved_gbs": gbs64, "bytes_per_launch_both_parts": bytes_alg,
                    "iterations_mean": float(outs[-1]["iterations"].mean()),
                    "iterations_max": int(outs[-1]["iterations"].max()),
                    "plain_single_gpu_lps_per_s": plain, "plain_sample": int(len(sub)),
                    "speedup_vs_plain": value / plain if plain > 0 else None,
                    "kernel_s": times[-1][1]},
-        "roofline": {"bound": "hbm", "kernel": "batch_kernel (persistent, one pass over each LP's CSR per iteration)",
+        "roofline": {"bound": "hbm", "kernel": "batch_kernel<512, float> (FP32 part: the LPs above 64 KB of "
+                                               "FP64 LDS; one pass over K and one over K^T per iteration)",
                      "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(f"uneven_{args.batch}/batch_kernel"),
-                     "bytes_per_launch": bytes_alg, "us_per_launch": times[-1][1] * 1e6},
+                     "traffic": measured_traffic(f"uneven_{args.batch}/batch_kernel_fp32"),
+                     "bytes_per_launch": bytes32, "us_per_launch": times[-1][2] * 1e6},
         "cpu_baseline": cpu,
     }
     if args.dump:

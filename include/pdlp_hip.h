@@ -288,10 +288,13 @@ typedef struct PdlpBatch {
     int32_t block_resident; /* FP64 part: threads per workgroup, 256 or 512 (0 = 256) */
     int32_t lds_resident;   /* FP64 part: dynamic LDS, >= 20*n+12*m+8 of its LPs */
     int32_t grid_resident;  /* FP64 part: persistent workgroups (0 = 2 per CU) */
+    int32_t parts;          /* parts to launch: bit 0 FP32 part, bit 1 FP64 part (0 = both) */
+    int32_t reserved;
     double tol;
     const PdlpBatchLP* lps; /* count descriptors                              */
     const int32_t* order;   /* count LP indices, processing order: FP32 part, then FP64 part */
-    int32_t* queue;         /* two int32 work counters (FP32, FP64 part), zero before the launch */
+    int32_t* queue;         /* two int32 work counters (FP32, FP64 part), the launched parts'
+                               zero before the launch */
     /* matrices: K by rows, K^T by rows (= K by columns); 16-bit indices within
        the LP (every LP that fits in LDS has m, n < 32768) */
     const int32_t* row_ptr; const uint16_t* col_idx; const float* val;

@@ -635,6 +635,7 @@ extern "C" int pdlp_batch_solve(const PdlpBatch* a, void* stream) {
     if (a == nullptr) return fail(PDLP_E_ARG, "PdlpBatch is NULL");
     if (a->count < 0) return fail(PDLP_E_ARG, "count must be >= 0");
     if (a->n_resident < 0 || a->n_resident > a->count) return fail(PDLP_E_ARG, "n_resident must be in [0, count]");
+    if (a->parts < 0 || a->parts > 3) return fail(PDLP_E_ARG, "parts must be 0..3");
     if (a->count == 0) return 0;
     if (!a->lps || !a->order || !a->queue || !a->row_ptr || !a->col_idx || !a->val || !a->t_col_ptr ||
         !a->t_row_idx || !a->t_val || !a->c || !a->lo || !a->hi || !a->c64 || !a->lo_shift || !a->ub_span ||
@@ -649,8 +650,14 @@ extern "C" int pdlp_batch_solve(const PdlpBatch* a, void* stream) {
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) cus = v;
     }
     const int n32 = a->count - a->n_resident;
-    int rc = launch_part<float>(a, 0, n32, a->queue, a->block, a->lds_bytes, a->grid, cus, s);
-    if (rc != 0) return rc;
-    return launch_part<double>(a, n32, a->n_resident, a->queue + 1, a->block_resident ? a->block_resident : 256,
-                               a->lds_resident, a->grid_resident, cus, s);
+    const int parts = a->parts ? a->parts : 3;
+    if (parts & 1) {
+        const int rc = launch_part<float>(a, 0, n32, a->queue, a->block, a->lds_bytes, a->grid, cus, s);
+        if (rc != 0) return rc;
+    }
+    if (parts & 2)
+        return launch_part<double>(a, n32, a->n_resident, a->queue + 1,
+                                   a->block_resident ? a->block_resident : 256, a->lds_resident, a->grid_resident,
+                                   cus, s);
+    return 0;
 }

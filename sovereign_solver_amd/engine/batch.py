@@ -276,11 +276,16 @@ class BatchEngine:
             setattr(a, name, ptr(t[name]))
         return a
 
-    def launch(self, tol: float, max_iters: int, check_every: int = 50) -> None:
-        """Enqueue one batched solve on the engine stream (inputs already in place)."""
+    def launch(self, tol: float, max_iters: int, check_every: int = 50, parts: int = 0) -> None:
+        """Enqueue one batched solve on the engine stream (inputs already in place).
+        ``parts``: 1 = only the FP32 part, 2 = only the FP64 part, 0 = both."""
         self._a = self._args(tol, max_iters, check_every)
+        self._a.parts = parts
         with torch.cuda.stream(self.stream):
-            self.t["queue"].zero_()
+            if parts in (0, 3):
+                self.t["queue"].zero_()
+            else:
+                self.t["queue"][parts - 1].zero_()
             L.check(self.lib.pdlp_batch_solve(C.byref(self._a), stream_handle(self.stream)), "pdlp_batch_solve")
 
     def results(self):
