@@ -228,7 +228,7 @@ def bench_batch(args):
         "metric": "LPs/sec to 1e-4 rel KKT: learned+batched vs plain PDHG, 1 GPU; SpMV HBM GB/s",
         "value": value, "unit": "LPs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 iterates (large LPs), fp64 iterates (LPs within 64 KB of LDS); fp32 matrix, fp64 KKT",
+        "dtype": "fp32 iterates (m > ~260), fp64 iterates (LPs within 16 KB of LDS); fp32 matrix, fp64 KKT",
         "data": "synthetic (seeded random sparse LPs, uneven sizes)",
         "config": {"workload": f"batched restarted PDHG, {len(probs)} LPs, m log-uniform in [50,2000], n=2.5m, "
                                f"density 1%, two persistent launches (FP32 part, FP64 part), cold start, "
@@ -242,7 +242,7 @@ def bench_batch(args):
                    "plain_single_gpu_lps_per_s": plain, "plain_sample": int(len(sub)),
                    "speedup_vs_plain": value / plain if plain > 0 else None,
                    "kernel_s": times[-1][1]},
-        "roofline": {"bound": "hbm", "kernel": "batch_kernel<512, float> (FP32 part: the LPs above 64 KB of "
+        "roofline": {"bound": "hbm", "kernel": "batch_kernel<512, float> (FP32 part: the LPs above 16 KB of "
                                                "FP64 LDS; one pass over K and one over K^T per iteration)",
                      "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                      "traffic": measured_traffic(f"uneven_{args.batch}/batch_kernel_fp32"),

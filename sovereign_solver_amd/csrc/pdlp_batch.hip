@@ -392,6 +392,16 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
     const int ce = a.check_every;
     const int sr = NT >> lr, sc = NT >> lc;   // line groups per workgroup
     const T one = 1, two_ = 2, step_num = (T)0.999, floor_ = (T)1e-4;
+    // running sums of the iterates: FP32 partial sums (folded at each check) or the FP64 window sums
+    GMEM T* xps;
+    GMEM T* yps;
+    if constexpr (RES) {
+        xps = v.x_sum;
+        yps = v.y_sum;
+    } else {
+        xps = v.x_psum;
+        yps = v.y_psum;
+    }
     while (status == PDLP_RUNNING && it < a.max_iters) {
         const T inv_wgt = one / wgt;
         // ---- column pass: aty = K^T y (gathered from LDS), primal step, xbar
@@ -401,6 +411,8 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
             const int jb = two ? j2 : j;
             const T xo1 = x_get(j), t1 = (T)v.tau[j], l1 = (T)v.lo[j], h1 = (T)v.hi[j];
             const T xo2 = x_get(jb), t2 = (T)v.tau[jb], l2 = (T)v.lo[jb], h2 = (T)v.hi[jb];
+            // the sums this round adds to, loaded with the other operands (not after the step)
+            const T p1 = lane_c == 0 ? xps[j] : (T)0, p2 = lane_c == 0 ? xps[jb] : (T)0;
             T c1, c2;
             if constexpr (RES) {
                 c1 = v.c64[j];
@@ -420,16 +432,14 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
                 xn = xn > h1 ? h1 : xn;
                 xb[j] = two_ * xn - xo1;
                 x_set(j, xn);
-                if constexpr (RES) v.x_sum[j] += xn;
-                else v.x_psum[j] += xn;
+                xps[j] = p1 + xn;
                 if (two) {
                     T xm = xo2 - (wgt * t2) * (c2 - s2);
                     xm = xm < l2 ? l2 : xm;
                     xm = xm > h2 ? h2 : xm;
                     xb[j2] = two_ * xm - xo2;
                     x_set(j2, xm);
-                    if constexpr (RES) v.x_sum[j2] += xm;
-                    else v.x_psum[j2] += xm;
+                    xps[j2] = p2 + xm;
                 }
             }
         }
@@ -448,6 +458,7 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
                 b2 = v.b[ib];
             }
             const bool le1 = v.kind[i] == PDLP_ROW_LE, le2 = v.kind[ib] == PDLP_ROW_LE;
+            const T q1 = lane_r == 0 ? yps[i] : (T)0, q2 = lane_r == 0 ? yps[ib] : (T)0;
             T d1 = 0, d2 = 0, a1 = 0, a2 = 0;
             dot2<T>(v.val, v.ci, xb, rpl[i], rpl[i + 1], two ? rpl[i2] : 0, two ? rpl[i2 + 1] : 0, lane_r, lr, d1,
                     a1, d2, a2);
@@ -459,14 +470,12 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
                 T yn = yl[i] + inv_wgt * (step_num / (a1 > floor_ ? a1 : floor_)) * (b1 - d1);
                 if (le1 && yn > (T)0) yn = 0;
                 yl[i] = yn;
-                if constexpr (RES) v.y_sum[i] += yn;
-                else v.y_psum[i] += yn;
+                yps[i] = q1 + yn;
                 if (two) {
                     T ym = yl[i2] + inv_wgt * (step_num / (a2 > floor_ ? a2 : floor_)) * (b2 - d2);
                     if (le2 && ym > (T)0) ym = 0;
                     yl[i2] = ym;
-                    if constexpr (RES) v.y_sum[i2] += ym;
-                    else v.y_psum[i2] += ym;
+                    yps[i2] = q2 + ym;
                 }
             }
         }

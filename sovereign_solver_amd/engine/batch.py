@@ -67,9 +67,12 @@ def lds_bytes_resident(m, n):
     return 20 * np.asarray(n) + 12 * np.asarray(m) + 8
 
 
-# LPs whose FP64 part needs at most this much LDS run with FP64 iterates (two
-# workgroups per CU); larger ones with FP32 iterates.
-RESIDENT_LDS = 64 * 1024
+# LPs whose FP64 part needs at most this much LDS (m up to about 260 when
+# n = 2.5m) run with FP64 iterates; larger ones with FP32 iterates on 512
+# threads, which finish a mid-size LP (m ~ 1,000) 1.6x sooner than the FP64
+# part's 256 threads.  On the uneven 4,096-LP batch: 0.600 s at 16 KB,
+# 0.606 s at 24 KB, 0.614 s at 40 KB, 0.711 s at 64 KB.
+RESIDENT_LDS = 16 * 1024
 
 
 def _segment_sums(v: np.ndarray, starts: np.ndarray, lens: np.ndarray) -> np.ndarray:

@@ -53,7 +53,7 @@ def test_packing_offsets_and_contents():
 
 
 def test_fp32_and_fp64_parts():
-    # LPs over 64 KB of FP64-part LDS run in the FP32 part, which comes first in the work order
+    # LPs over 16 KB of FP64-part LDS run in the FP32 part, which comes first in the work order
     probs = [random_sparse_lp(1100, 2750, 0.002, seed=1), random_sparse_lp(60, 150, 0.05, seed=2),
              random_sparse_lp(1300, 3250, 0.002, seed=3), random_sparse_lp(200, 500, 0.02, seed=4)]
     pk = PackedBatch(probs)
@@ -61,7 +61,7 @@ def test_fp32_and_fp64_parts():
     assert list(pk.order) == [2, 0, 3, 1]              # FP32 part, then FP64 part, each largest first
     assert pk.lds_bytes == (int(lds_bytes_of(1300, 3250)) + 15) // 16 * 16
     assert pk.lds_resident == (int(lds_bytes_resident(200, 500)) + 15) // 16 * 16
-    assert pk.lds_bytes <= 160 * 1024 and pk.lds_resident <= 64 * 1024
+    assert pk.lds_bytes <= 160 * 1024 and pk.lds_resident <= 16 * 1024
 
 
 def test_uneven_family_statistics():
