@@ -12,7 +12,7 @@ from scipy.optimize import linprog
 from oracle import yardstick
 from sovereign_solver_amd import solve_lp_pdhg, solve_lp_pdhg_batch
 from sovereign_solver_amd.problem import Problem
-from sovereign_solver_amd.workloads import random_sparse_lp, transportation_lp, uneven_batch
+from sovereign_solver_amd.workloads import random_sparse_lp, transportation_lp, uneven_batch, uneven_sizes
 
 pytestmark = pytest.mark.gpu
 
@@ -85,3 +85,52 @@ def test_batch_larger_lp_in_lds_limit(gpu):
     r = solve_lp_pdhg_batch([p], tol=1e-4, max_iters=100_000)[0]
     assert r.status == "optimal"
     assert yardstick.rel_kkt(p, r.x, r.y)[0] <= 1.02e-4
+
+
+def test_batch_is_deterministic_and_parts_compose(gpu):
+    from sovereign_solver_amd.engine.batch import BatchEngine, PackedBatch
+    probs = uneven_batch(48, m_lo=50, m_hi=700, density=0.02, seed=9)
+    pk = PackedBatch(probs)
+    assert 0 < pk.n_resident < len(probs)                  # both parts present
+    eng = BatchEngine(pk)
+    o1, x1, y1 = eng.solve(1e-4, 50_000)
+    assert np.all(o1["status"] == 1)
+    o2, x2, y2 = eng.solve(1e-4, 50_000)
+    # gathers in a fixed order, no atomics: the same bits every run
+    assert np.array_equal(x1, x2) and np.array_equal(y1, y2) and np.array_equal(o1, o2)
+    # the two parts launched one at a time give the same bits as one call
+    eng.set_warm_start(None, None, None)
+    eng.launch(1e-4, 50_000, parts=1)
+    eng.launch(1e-4, 50_000, parts=2)
+    o3, x3, y3 = eng.results()
+    assert np.array_equal(x1, x3) and np.array_equal(y1, y3) and np.array_equal(o1, o3)
+
+
+def test_batch_fp64_and_fp32_parts_agree(gpu, monkeypatch):
+    import sovereign_solver_amd.engine.batch as B
+    probs = [random_sparse_lp(120, 300, 0.03, seed=s) for s in range(6)]
+    r64 = solve_lp_pdhg_batch(probs, tol=1e-5, engine="persistent")      # small: all in the FP64 part
+    monkeypatch.setattr(B, "RESIDENT_LDS", 0)                             # now all in the FP32 part
+    assert B.PackedBatch(probs).n_resident == 0
+    r32 = solve_lp_pdhg_batch(probs, tol=1e-5, engine="persistent")
+    for p, a, b in zip(probs, r64, r32):
+        assert a.status == b.status == "optimal", (a.status, b.status)
+        ref = _highs(p)
+        for r in (a, b):
+            assert yardstick.rel_kkt(p, r.x, r.y)[0] <= 1.02e-5
+            assert abs(r.objective - ref) <= 1e-4 * (1 + abs(ref)), (r.objective, ref)
+
+
+def test_batch_small_lps_with_empty_columns_converge(gpu):
+    # three LPs of the uneven 4,096-LP batch (m = 50..64, n = 2.5m, 1% dense: most
+    # columns empty) whose FP32 iterates drove the primal weight to its bound
+    # without converging in 200,000 iterations; FP64 iterates finish them
+    ms = uneven_sizes(4096, 50, 2000, 0)
+    probs = [random_sparse_lp(int(ms[k]), int(round(2.5 * ms[k])), 0.01, seed=k, family=2) for k in (655, 2272, 2384)]
+    assert [p.n_constraints for p in probs] == [64, 52, 50]
+    res = solve_lp_pdhg_batch(probs, tol=1e-4, max_iters=50_000, engine="persistent")
+    for p, r in zip(probs, res):
+        assert r.status == "optimal" and r.iterations <= 20_000, (r.status, r.iterations)
+        assert yardstick.rel_kkt(p, r.x, r.y)[0] <= 1.02e-4
+        ref = _highs(p)
+        assert abs(r.objective - ref) <= 1e-3 * (1 + abs(ref))
