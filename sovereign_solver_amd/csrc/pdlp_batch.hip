@@ -21,9 +21,10 @@
 //
 // A group of 2^lw lanes (lw per LP and pass, from the line lengths) owns a
 // line (row or column); each lane takes every 2^lw-th entry, the first EL of
-// them loaded at once.  Each group works on two lines per round, so the loads
-// of both are in flight together.  Line pointers come from LDS, so a round
-// waits for global memory once.
+// them loaded at once.  Each group works on NL lines per round (three in the
+// FP32 part, two in the FP64 part), so the loads of all are in flight
+// together.  Line pointers come from LDS and the operands of the step are
+// loaded with the entries, so a round waits for global memory once.
 //
 // Every check_every iterations the partial sums are folded into the FP64
 // window sums and the reference KKT of the window average (and, for the
@@ -142,40 +143,37 @@ __device__ __forceinline__ double group_sum_rt(double v, int lw) {
     return v;
 }
 
-// Two lines at once: s_k += val[q] * vec[idx[q]] and a_k += |val[q]| over the
-// lane's entries of [b_k, e_k) (entries lane, lane + 2^lw, ...).  The first EL
-// entries per lane of both lines are loaded before any is used.
-template <typename T>
-__device__ __forceinline__ void dot2(const GMEM float* val, const GMEM uint16_t* idx, const T* vec, int b1, int e1,
-                                     int b2, int e2, int lane, int lw, T& s1, T& a1, T& s2, T& a2) {
-    float v1[EL], v2[EL];
-    int c1[EL], c2[EL];
+// NL lines at once: s[k] += val[q] * vec[idx[q]] and a[k] += |val[q]| over the
+// lane's entries of [b[k], e[k]) (entries lane, lane + 2^lw, ...).  The first EL
+// entries per lane of all lines are loaded before any is used.
+template <int NL, typename T>
+__device__ __forceinline__ void dotn(const GMEM float* val, const GMEM uint16_t* idx, const T* vec, const int (&b)[NL],
+                                     const int (&e)[NL], int lane, int lw, T (&s)[NL], T (&a)[NL]) {
+    float v[NL][EL];
+    int c[NL][EL];
 #pragma unroll
-    for (int e = 0; e < EL; ++e) {
-        const int q1 = b1 + lane + (e << lw), q2 = b2 + lane + (e << lw);
-        v1[e] = q1 < e1 ? val[q1] : 0.f;
-        c1[e] = q1 < e1 ? idx[q1] : 0;
-        v2[e] = q2 < e2 ? val[q2] : 0.f;
-        c2[e] = q2 < e2 ? idx[q2] : 0;
-    }
+    for (int k = 0; k < NL; ++k)
 #pragma unroll
-    for (int e = 0; e < EL; ++e) {
-        s1 += (T)v1[e] * vec[c1[e]];
-        a1 += (T)fabsf(v1[e]);
-        s2 += (T)v2[e] * vec[c2[e]];
-        a2 += (T)fabsf(v2[e]);
-    }
+        for (int t = 0; t < EL; ++t) {
+            const int q = b[k] + lane + (t << lw);
+            v[k][t] = q < e[k] ? val[q] : 0.f;
+            c[k][t] = q < e[k] ? idx[q] : 0;
+        }
+#pragma unroll
+    for (int k = 0; k < NL; ++k)
+#pragma unroll
+        for (int t = 0; t < EL; ++t) {
+            s[k] += (T)v[k][t] * vec[c[k][t]];
+            a[k] += (T)fabsf(v[k][t]);
+        }
     // lines longer than EL entries per lane (rare: the lane count covers most lines)
-    for (int q = b1 + lane + (EL << lw); q < e1; q += 1 << lw) {
-        const T kv = (T)val[q];
-        s1 += kv * vec[idx[q]];
-        a1 += fabs(kv);
-    }
-    for (int q = b2 + lane + (EL << lw); q < e2; q += 1 << lw) {
-        const T kv = (T)val[q];
-        s2 += kv * vec[idx[q]];
-        a2 += fabs(kv);
-    }
+#pragma unroll
+    for (int k = 0; k < NL; ++k)
+        for (int q = b[k] + lane + (EL << lw); q < e[k]; q += 1 << lw) {
+            const T kv = (T)val[q];
+            s[k] += kv * vec[idx[q]];
+            a[k] += fabs(kv);
+        }
 }
 
 // Block-wide sum of NV doubles, result broadcast to all threads.
@@ -318,6 +316,8 @@ __device__ __forceinline__ Kkt kkt_of(const double (&s)[NS], const LPView& v) {
 template <int NT, typename T>
 __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red) {
     constexpr bool RES = std::is_same<T, double>::value;
+    // lines per group and round: all their loads are in flight together (FP64: two, for registers)
+    constexpr int NL = RES ? 2 : 3;
     const PdlpBatchLP lp = a.lps[k];
     const LPView v = view_of(a, lp);
     char* base = reinterpret_cast<char*>(lds);
@@ -405,77 +405,89 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
     while (status == PDLP_RUNNING && it < a.max_iters) {
         const T inv_wgt = one / wgt;
         // ---- column pass: aty = K^T y (gathered from LDS), primal step, xbar
-        for (int j = threadIdx.x >> lc; j < v.n; j += 2 * sc) {
-            const int j2 = j + sc;
-            const bool two = j2 < v.n;
-            const int jb = two ? j2 : j;
-            const T xo1 = x_get(j), t1 = (T)v.tau[j], l1 = (T)v.lo[j], h1 = (T)v.hi[j];
-            const T xo2 = x_get(jb), t2 = (T)v.tau[jb], l2 = (T)v.lo[jb], h2 = (T)v.hi[jb];
-            // the sums this round adds to, loaded with the other operands (not after the step)
-            const T p1 = lane_c == 0 ? xps[j] : (T)0, p2 = lane_c == 0 ? xps[jb] : (T)0;
-            T c1, c2;
-            if constexpr (RES) {
-                c1 = v.c64[j];
-                c2 = v.c64[jb];
-            } else {
-                c1 = v.c[j];
-                c2 = v.c[jb];
+        for (int j0 = threadIdx.x >> lc; j0 < v.n; j0 += NL * sc) {
+            int jj[NL], bb[NL], ee[NL];
+            T xo[NL], tt[NL], lo[NL], hi[NL], cc[NL], ps[NL], s[NL], u[NL];
+#pragma unroll
+            for (int k = 0; k < NL; ++k) {
+                const int j = j0 + k * sc;
+                const bool in = j < v.n;
+                jj[k] = in ? j : j0;
+                bb[k] = in ? cpl[j] : 0;
+                ee[k] = in ? cpl[j + 1] : 0;
             }
-            T s1 = 0, s2 = 0, u1 = 0, u2 = 0;
-            dot2<T>(v.tval, v.ri, yl, cpl[j], cpl[j + 1], two ? cpl[j2] : 0, two ? cpl[j2 + 1] : 0, lane_c, lc, s1,
-                    u1, s2, u2);
-            s1 = group_sum_rt(s1, lc);
-            s2 = group_sum_rt(s2, lc);
+#pragma unroll
+            for (int k = 0; k < NL; ++k) {
+                const int j = jj[k];
+                xo[k] = x_get(j);
+                tt[k] = (T)v.tau[j];
+                lo[k] = (T)v.lo[j];
+                hi[k] = (T)v.hi[j];
+                if constexpr (RES) cc[k] = v.c64[j];
+                else cc[k] = v.c[j];
+                // the sums this round adds to, loaded with the other operands (not after the step)
+                ps[k] = lane_c == 0 ? xps[j] : (T)0;
+                s[k] = 0;
+                u[k] = 0;
+            }
+            dotn<NL, T>(v.tval, v.ri, yl, bb, ee, lane_c, lc, s, u);
+#pragma unroll
+            for (int k = 0; k < NL; ++k) s[k] = group_sum_rt(s[k], lc);
             if (lane_c == 0) {
-                T xn = xo1 - (wgt * t1) * (c1 - s1);
-                xn = xn < l1 ? l1 : xn;
-                xn = xn > h1 ? h1 : xn;
-                xb[j] = two_ * xn - xo1;
-                x_set(j, xn);
-                xps[j] = p1 + xn;
-                if (two) {
-                    T xm = xo2 - (wgt * t2) * (c2 - s2);
-                    xm = xm < l2 ? l2 : xm;
-                    xm = xm > h2 ? h2 : xm;
-                    xb[j2] = two_ * xm - xo2;
-                    x_set(j2, xm);
-                    xps[j2] = p2 + xm;
+#pragma unroll
+                for (int k = 0; k < NL; ++k) {
+                    const int j = j0 + k * sc;
+                    if (j < v.n) {
+                        T xn = xo[k] - (wgt * tt[k]) * (cc[k] - s[k]);
+                        xn = xn < lo[k] ? lo[k] : xn;
+                        xn = xn > hi[k] ? hi[k] : xn;
+                        xb[j] = two_ * xn - xo[k];
+                        x_set(j, xn);
+                        xps[j] = ps[k] + xn;
+                    }
                 }
             }
         }
         __syncthreads();
         // ---- row pass: K xbar (gathered from LDS), dual step
-        for (int i = threadIdx.x >> lr; i < v.m; i += 2 * sr) {
-            const int i2 = i + sr;
-            const bool two = i2 < v.m;
-            const int ib = two ? i2 : i;
-            T b1, b2;
-            if constexpr (RES) {
-                b1 = v.b64[i];
-                b2 = v.b64[ib];
-            } else {
-                b1 = v.b[i];
-                b2 = v.b[ib];
+        for (int i0 = threadIdx.x >> lr; i0 < v.m; i0 += NL * sr) {
+            int ii[NL], bb[NL], ee[NL];
+            T bv[NL], qs[NL], d[NL], u[NL];
+            bool le[NL];
+#pragma unroll
+            for (int k = 0; k < NL; ++k) {
+                const int i = i0 + k * sr;
+                const bool in = i < v.m;
+                ii[k] = in ? i : i0;
+                bb[k] = in ? rpl[i] : 0;
+                ee[k] = in ? rpl[i + 1] : 0;
             }
-            const bool le1 = v.kind[i] == PDLP_ROW_LE, le2 = v.kind[ib] == PDLP_ROW_LE;
-            const T q1 = lane_r == 0 ? yps[i] : (T)0, q2 = lane_r == 0 ? yps[ib] : (T)0;
-            T d1 = 0, d2 = 0, a1 = 0, a2 = 0;
-            dot2<T>(v.val, v.ci, xb, rpl[i], rpl[i + 1], two ? rpl[i2] : 0, two ? rpl[i2 + 1] : 0, lane_r, lr, d1,
-                    a1, d2, a2);
-            d1 = group_sum_rt(d1, lr);
-            a1 = group_sum_rt(a1, lr);
-            d2 = group_sum_rt(d2, lr);
-            a2 = group_sum_rt(a2, lr);
+#pragma unroll
+            for (int k = 0; k < NL; ++k) {
+                const int i = ii[k];
+                if constexpr (RES) bv[k] = v.b64[i];
+                else bv[k] = v.b[i];
+                le[k] = v.kind[i] == PDLP_ROW_LE;
+                qs[k] = lane_r == 0 ? yps[i] : (T)0;
+                d[k] = 0;
+                u[k] = 0;
+            }
+            dotn<NL, T>(v.val, v.ci, xb, bb, ee, lane_r, lr, d, u);
+#pragma unroll
+            for (int k = 0; k < NL; ++k) {
+                d[k] = group_sum_rt(d[k], lr);
+                u[k] = group_sum_rt(u[k], lr);
+            }
             if (lane_r == 0) {
-                T yn = yl[i] + inv_wgt * (step_num / (a1 > floor_ ? a1 : floor_)) * (b1 - d1);
-                if (le1 && yn > (T)0) yn = 0;
-                yl[i] = yn;
-                yps[i] = q1 + yn;
-                if (two) {
-                    T ym = yl[i2] + inv_wgt * (step_num / (a2 > floor_ ? a2 : floor_)) * (b2 - d2);
-                    if (le2 && ym > (T)0) ym = 0;
-                    yl[i2] = ym;
-                    yps[i2] = q2 + ym;
+#pragma unroll
+                for (int k = 0; k < NL; ++k) {
+                    const int i = i0 + k * sr;
+                    if (i < v.m) {
+                        T yn = yl[i] + inv_wgt * (step_num / (u[k] > floor_ ? u[k] : floor_)) * (bv[k] - d[k]);
+                        if (le[k] && yn > (T)0) yn = 0;
+                        yl[i] = yn;
+                        yps[i] = qs[k] + yn;
+                    }
                 }
             }
         }
