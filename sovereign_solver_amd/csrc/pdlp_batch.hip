@@ -48,6 +48,7 @@ namespace {
 
 constexpr int EL = 8;   // entries per lane loaded together
 constexpr int KEL = 4;  // entries per lane loaded together in a KKT evaluation (FP64 sums: registers)
+constexpr int KNL = 2;  // lines per group and round in a KKT evaluation
 constexpr int NS = 6;   // KKT sums: rp^2, rd^2, primal obj, dual obj, |dx|^2, |dy|^2
 
 struct LPView {
@@ -144,17 +145,17 @@ __device__ __forceinline__ double group_sum_rt(double v, int lw) {
 }
 
 // NL lines at once: s[k] += val[q] * vec[idx[q]] and a[k] += |val[q]| over the
-// lane's entries of [b[k], e[k]) (entries lane, lane + 2^lw, ...).  The first EL
-// entries per lane of all lines are loaded before any is used.
-template <int NL, typename T>
-__device__ __forceinline__ void dotn(const GMEM float* val, const GMEM uint16_t* idx, const T* vec, const int (&b)[NL],
+// lane's entries of [b[k], e[k]) (entries lane, lane + 2^lw, ...), sums in T.
+// The first E entries per lane of all lines are loaded before any is used.
+template <int NL, int E, typename T, typename V>
+__device__ __forceinline__ void dotn(const GMEM float* val, const GMEM uint16_t* idx, const V* vec, const int (&b)[NL],
                                      const int (&e)[NL], int lane, int lw, T (&s)[NL], T (&a)[NL]) {
-    float v[NL][EL];
-    int c[NL][EL];
+    float v[NL][E];
+    int c[NL][E];
 #pragma unroll
     for (int k = 0; k < NL; ++k)
 #pragma unroll
-        for (int t = 0; t < EL; ++t) {
+        for (int t = 0; t < E; ++t) {
             const int q = b[k] + lane + (t << lw);
             v[k][t] = q < e[k] ? val[q] : 0.f;
             c[k][t] = q < e[k] ? idx[q] : 0;
@@ -162,16 +163,16 @@ __device__ __forceinline__ void dotn(const GMEM float* val, const GMEM uint16_t*
 #pragma unroll
     for (int k = 0; k < NL; ++k)
 #pragma unroll
-        for (int t = 0; t < EL; ++t) {
-            s[k] += (T)v[k][t] * vec[c[k][t]];
+        for (int t = 0; t < E; ++t) {
+            s[k] += (T)v[k][t] * (T)vec[c[k][t]];
             a[k] += (T)fabsf(v[k][t]);
         }
-    // lines longer than EL entries per lane (rare: the lane count covers most lines)
+    // lines longer than E entries per lane (rare: the lane count covers most lines)
 #pragma unroll
     for (int k = 0; k < NL; ++k)
-        for (int q = b[k] + lane + (EL << lw); q < e[k]; q += 1 << lw) {
+        for (int q = b[k] + lane + (E << lw); q < e[k]; q += 1 << lw) {
             const T kv = (T)val[q];
-            s[k] += kv * vec[idx[q]];
+            s[k] += kv * (T)vec[idx[q]];
             a[k] += fabs(kv);
         }
 }
@@ -221,72 +222,85 @@ __device__ __forceinline__ void kkt_eval(const LPView& v, double* X, const T* xl
     __syncthreads();
     // C2: rows — primal residual, dual objective, slack dual residual, dual distance
     {
-        const int lw = v.lr, lane = threadIdx.x & ((1 << lw) - 1);
-        for (int i = threadIdx.x >> lw; i < v.m; i += NT >> lw) {
-            const int beg = rpl[i], end = rpl[i + 1];
-            double d = 0.0, as = 0.0;
-            for (int q0 = beg + lane; q0 < end; q0 += KEL << lw) {
-                float kv[KEL];
-                int cc[KEL];
+        const int lw = v.lr, lane = threadIdx.x & ((1 << lw) - 1), st = NT >> lw;
+        for (int i0 = threadIdx.x >> lw; i0 < v.m; i0 += KNL * st) {
+            int bb[KNL], ee[KNL];
+            double d[KNL], as[KNL];
 #pragma unroll
-                for (int e = 0; e < KEL; ++e) {
-                    const int q = q0 + (e << lw);
-                    kv[e] = q < end ? v.val[q] : 0.f;
-                    cc[e] = q < end ? v.ci[q] : 0;
-                }
-#pragma unroll
-                for (int e = 0; e < KEL; ++e) {
-                    d += (double)kv[e] * X[cc[e]];
-                    as += fabs((double)kv[e]);
-                }
+            for (int k = 0; k < KNL; ++k) {
+                const int i = i0 + k * st;
+                bb[k] = i < v.m ? rpl[i] : 0;
+                ee[k] = i < v.m ? rpl[i + 1] : 0;
+                d[k] = 0.0;
+                as[k] = 0.0;
             }
-            d = group_sum_rt(d, lw);
-            as = group_sum_rt(as, lw);
-            if (lane == 0) {
-                const bool le = v.kind[i] == PDLP_ROW_LE;
-                double res = d - v.b64[i];
-                if (le && res < 0.0) res = 0.0;
-                const double yi = src ? v.y_sum[i] / w : (double)yl[i];
-                acc[0] += res * res;
-                if (le && yi > 0.0) acc[1] += yi * yi;
-                acc[3] += v.b_shift[i] * yi;
-                if (v.adaptive) {
-                    const double dy = yi - (double)v.y_last[i];
-                    acc[5] += dy * dy * (as > 1e-4 ? as : 1e-4) / 0.999;
+            dotn<KNL, KEL, double, double>(v.val, v.ci, X, bb, ee, lane, lw, d, as);
+#pragma unroll
+            for (int k = 0; k < KNL; ++k) {
+                d[k] = group_sum_rt(d[k], lw);
+                as[k] = group_sum_rt(as[k], lw);
+            }
+#pragma unroll
+            for (int k = 0; k < KNL; ++k) {
+                const int i = i0 + k * st;
+                if (lane == 0 && i < v.m) {
+                    const bool le = v.kind[i] == PDLP_ROW_LE;
+                    double res = d[k] - v.b64[i];
+                    if (le && res < 0.0) res = 0.0;
+                    const double yi = src ? v.y_sum[i] / w : (double)yl[i];
+                    acc[0] += res * res;
+                    if (le && yi > 0.0) acc[1] += yi * yi;
+                    acc[3] += v.b_shift[i] * yi;
+                    if (v.adaptive) {
+                        const double dy = yi - (double)v.y_last[i];
+                        acc[5] += dy * dy * (as[k] > 1e-4 ? as[k] : 1e-4) / 0.999;
+                    }
                 }
             }
         }
     }
-    // C3: columns — reduced costs c - K^T y of the point: dual residual / upper-bound-row duals
+    // C3: columns — reduced costs c - K^T y of the point: dual residual / upper-bound-row duals.
+    // The averaged y goes to LDS (in X, free after C2) when it fits (m <= n).
+    const bool staged = src && v.m <= v.n;
+    if (staged) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < v.m; i += NT) X[i] = v.y_sum[i] / w;
+        __syncthreads();
+    }
     {
-        const int lw = v.lc, lane = threadIdx.x & ((1 << lw) - 1);
-        for (int j = threadIdx.x >> lw; j < v.n; j += NT >> lw) {
-            const int beg = cpl[j], end = cpl[j + 1];
-            double s = 0.0;
-            for (int q0 = beg + lane; q0 < end; q0 += KEL << lw) {
-                float kv[KEL];
-                int rr[KEL];
+        const int lw = v.lc, lane = threadIdx.x & ((1 << lw) - 1), st = NT >> lw;
+        for (int j0 = threadIdx.x >> lw; j0 < v.n; j0 += KNL * st) {
+            int bb[KNL], ee[KNL];
+            double s[KNL], u[KNL];
 #pragma unroll
-                for (int e = 0; e < KEL; ++e) {
-                    const int q = q0 + (e << lw);
-                    kv[e] = q < end ? v.tval[q] : 0.f;
-                    rr[e] = q < end ? v.ri[q] : 0;
-                }
-                if (src) {
-#pragma unroll
-                    for (int e = 0; e < KEL; ++e) s += (double)kv[e] * (v.y_sum[rr[e]] / w);
-                } else {
-#pragma unroll
-                    for (int e = 0; e < KEL; ++e) s += (double)kv[e] * (double)yl[rr[e]];
-                }
+            for (int k = 0; k < KNL; ++k) {
+                const int j = j0 + k * st;
+                bb[k] = j < v.n ? cpl[j] : 0;
+                ee[k] = j < v.n ? cpl[j + 1] : 0;
+                s[k] = 0.0;
+                u[k] = 0.0;
             }
-            s = group_sum_rt(s, lw);
-            if (lane == 0) {
-                const double r = v.c64[j] - s;
-                const double rneg = r < 0.0 ? r : 0.0;
-                const double span = v.ub_span[j];
-                if (isfinite(span)) acc[3] += span * rneg;
-                else acc[1] += rneg * rneg;
+            if (!src) {
+                dotn<KNL, KEL, double, T>(v.tval, v.ri, yl, bb, ee, lane, lw, s, u);
+            } else if (staged) {
+                dotn<KNL, KEL, double, double>(v.tval, v.ri, X, bb, ee, lane, lw, s, u);
+            } else {                                // averaged y gathered from global memory
+#pragma unroll
+                for (int k = 0; k < KNL; ++k)
+                    for (int q = bb[k] + lane; q < ee[k]; q += 1 << lw) s[k] += (double)v.tval[q] * (v.y_sum[v.ri[q]] / w);
+            }
+#pragma unroll
+            for (int k = 0; k < KNL; ++k) s[k] = group_sum_rt(s[k], lw);
+#pragma unroll
+            for (int k = 0; k < KNL; ++k) {
+                const int j = j0 + k * st;
+                if (lane == 0 && j < v.n) {
+                    const double r = v.c64[j] - s[k];
+                    const double rneg = r < 0.0 ? r : 0.0;
+                    const double span = v.ub_span[j];
+                    if (isfinite(span)) acc[3] += span * rneg;
+                    else acc[1] += rneg * rneg;
+                }
             }
         }
     }
@@ -430,7 +444,7 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
                 s[k] = 0;
                 u[k] = 0;
             }
-            dotn<NL, T>(v.tval, v.ri, yl, bb, ee, lane_c, lc, s, u);
+            dotn<NL, EL, T, T>(v.tval, v.ri, yl, bb, ee, lane_c, lc, s, u);
 #pragma unroll
             for (int k = 0; k < NL; ++k) s[k] = group_sum_rt(s[k], lc);
             if (lane_c == 0) {
@@ -472,7 +486,7 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
                 d[k] = 0;
                 u[k] = 0;
             }
-            dotn<NL, T>(v.val, v.ci, xb, bb, ee, lane_r, lr, d, u);
+            dotn<NL, EL, T, T>(v.val, v.ci, xb, bb, ee, lane_r, lr, d, u);
 #pragma unroll
             for (int k = 0; k < NL; ++k) {
                 d[k] = group_sum_rt(d[k], lr);
