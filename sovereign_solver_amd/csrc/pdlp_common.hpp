@@ -12,6 +12,9 @@ namespace pdlp {
 
 constexpr int kWave = 64;
 
+// Entries per lane whose loads a row dot product issues together.
+constexpr int kRowBatch = 4;
+
 // The adaptive primal weight stays within [w0 / kWeightDown, w0 * kWeightUp]
 // of its starting value w0.  With FP32 iterates an extreme weight (a tiny tau
 // and a huge sigma, or the reverse) amplifies the rounding of the residual in
@@ -81,10 +84,24 @@ __device__ __forceinline__ void row_dot(const int32_t* __restrict__ rp, const in
             s = add_rn(s, fabs(a));
         }
     } else {
-        for (int k = beg + lane; k < end; k += TPR) {
-            const A a = (A)val[k];
-            d += a * gather(ci[k]);
-            s += fabs(a);
+        // kRowBatch entries per lane loaded before any is used (the same
+        // summation order as one entry at a time)
+        for (int k0 = beg + lane; k0 < end; k0 += TPR * kRowBatch) {
+            A a[kRowBatch];
+            int c[kRowBatch];
+#pragma unroll
+            for (int u = 0; u < kRowBatch; ++u) {
+                const int k = k0 + u * TPR;
+                a[u] = k < end ? (A)val[k] : A(0);
+                c[u] = k < end ? ci[k] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < kRowBatch; ++u) {
+                if (k0 + u * TPR < end) {
+                    d += a[u] * gather(c[u]);
+                    s += fabs(a[u]);
+                }
+            }
         }
         if constexpr (TPR > 1) {
             d = group_sum<TPR>(d);
@@ -105,10 +122,22 @@ __device__ __forceinline__ void row_dot_block(const int32_t* __restrict__ rp, co
     static_assert(TPR > 64 && TPR <= 1024 && TPR % 64 == 0, "block row width");
     const int beg = rp[row], end = rp[row + 1];
     A d = A(0), s = A(0);
-    for (int k = beg + (int)threadIdx.x; k < end; k += TPR) {
-        const A a = (A)val[k];
-        d += a * gather(ci[k]);
-        s += fabs(a);
+    for (int k0 = beg + (int)threadIdx.x; k0 < end; k0 += TPR * kRowBatch) {
+        A a[kRowBatch];
+        int c[kRowBatch];
+#pragma unroll
+        for (int u = 0; u < kRowBatch; ++u) {
+            const int k = k0 + u * TPR;
+            a[u] = k < end ? (A)val[k] : A(0);
+            c[u] = k < end ? ci[k] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kRowBatch; ++u) {
+            if (k0 + u * TPR < end) {
+                d += a[u] * gather(c[u]);
+                s += fabs(a[u]);
+            }
+        }
     }
     d = group_sum<kWave>(d);
     s = group_sum<kWave>(s);
