@@ -1,8 +1,10 @@
-"""Batched PDHG engine: many independent LPs, one persistent HIP launch.
+"""Batched PDHG engine: many independent LPs, persistent HIP launches.
 
-The LPs are packed into one arena of device arrays (CSR rows, per-column and
-per-row vectors, FP64 yardstick data) with a descriptor per LP; the kernel
-(``pdlp_batch_solve``) gives each workgroup one LP at a time from a work queue
+The LPs are packed into one arena of device arrays (each LP's K and K^T by
+rows with 16-bit indices, per-column and per-row vectors, FP64 yardstick
+data) with a descriptor per LP; ``pdlp_batch_solve`` launches two persistent
+kernels, one for the LPs with FP32 iterates and one for the small LPs with
+FP64 iterates, whose workgroups take one LP at a time from a work queue
 ordered largest-first, so the biggest LPs start first and the small ones fill
 in behind them.
 """

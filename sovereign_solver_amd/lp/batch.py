@@ -3,10 +3,11 @@
 The reference has no batch call; its callers loop over ``solve_lp_pdhg``
 (``solver/lp/pdhg.py:152``) or, in branch-and-bound, over simplex node solves
 (``solver/milp/branch_and_bound.py:367-374``).  This call solves all LPs in one
-persistent HIP launch and returns one :class:`SolveResult` per problem with the
-same fields and semantics as :func:`solve_lp_pdhg` (box form, FP32 iterates,
-FP64 KKT, check cadence 50; ``scheme="adaptive"`` (default) or
-``"reference"`` restarts as in :func:`solve_lp_pdhg`).
+batched solve and returns one :class:`SolveResult` per problem with the same
+fields and semantics as :func:`solve_lp_pdhg` (box form, FP32 iterates — FP64
+for the small LPs of the persistent engine —, FP64 KKT, check cadence 50;
+``scheme="adaptive"`` (default) or ``"reference"`` restarts as in
+:func:`solve_lp_pdhg`).
 
 ``engine``: ``"persistent"`` (one workgroup per LP, any mix of matrices; an
 LP needs 12n+8m+8 bytes of LDS), ``"family"`` (all LPs share one constraint
