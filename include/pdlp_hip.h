@@ -289,7 +289,8 @@ typedef struct PdlpBatch {
     int32_t lds_resident;   /* FP64 part: dynamic LDS, >= 20*n+12*m+8 of its LPs */
     int32_t grid_resident;  /* FP64 part: persistent workgroups (0 = 2 per CU) */
     int32_t parts;          /* parts to launch: bit 0 FP32 part, bit 1 FP64 part (0 = both) */
-    int32_t reserved;
+    int32_t x_in_lds;       /* FP32 part: x, its partial sum, tau_hat and c in LDS as well
+                               (then lds_bytes >= 28*n+8*m+8 of its LPs) */
     double tol;
     const PdlpBatchLP* lps; /* count descriptors                              */
     const int32_t* order;   /* count LP indices, processing order: FP32 part, then FP64 part */

@@ -5,7 +5,10 @@
 //   yl  : m — the dual iterate
 //   rpl : m+1 row pointers, cpl : n+1 column pointers (of the transpose)
 // in two parts of the batch, one launch each:
-//   FP32 part: FP32 iterates, x in global memory, 12n + 8m + 8 bytes of LDS;
+//   FP32 part: FP32 iterates, x in global memory, 12n + 8m + 8 bytes of LDS,
+//              or, when every LP of the part fits, x, its partial sum, tau_hat
+//              and c in LDS as well (28n + 8m + 8 bytes: one 512-thread
+//              workgroup per CU has the CU's LDS to itself);
 //   FP64 part: FP64 iterates, x in LDS as well, 20n + 12m + 8 bytes.
 // Small LPs with many empty columns drift in FP32 (their restarts see the
 // rounding of the residuals and shrink the primal weight without end) and
@@ -201,7 +204,7 @@ __device__ __forceinline__ void block_sum_all(double (&acc)[NV], double* red) {
 // Uses X as n doubles.  acc (all threads): rp^2, rd^2, primal and dual
 // objective, and for the adaptive scheme the squared distances of the point
 // from the last restart point in the step-size metric (dx^2/tau_hat, dy^2/sigma_hat).
-template <int NT, typename T>
+template <int NT, typename T, bool XIN>
 __device__ __forceinline__ void kkt_eval(const LPView& v, double* X, const T* xl, const T* yl, const int* rpl,
                                          const int* cpl, int src, double w, double (&acc)[NS], double* red) {
 #pragma unroll
@@ -210,7 +213,7 @@ __device__ __forceinline__ void kkt_eval(const LPView& v, double* X, const T* xl
     for (int j = threadIdx.x; j < v.n; j += NT) {
         double xj;
         if (src) xj = v.x_sum[j] / w;
-        else if constexpr (std::is_same<T, double>::value) xj = xl[j];
+        else if constexpr (XIN) xj = (double)xl[j];
         else xj = (double)v.x[j];
         X[j] = xj;
         acc[2] += v.c64[j] * (xj - v.lo_shift[j]);
@@ -325,30 +328,38 @@ __device__ __forceinline__ Kkt kkt_of(const double (&s)[NS], const LPView& v) {
     return k;
 }
 
-// One LP from start to finish.  T = float: FP32 iterates, x in global memory;
-// T = double: FP64 iterates, x in LDS.
-template <int NT, typename T>
+// One LP from start to finish.  T = float: FP32 iterates, x in global memory,
+// or with XR x, its partial sum, tau_hat and c in LDS as well; T = double:
+// FP64 iterates, x in LDS.
+template <int NT, typename T, bool XR>
 __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red) {
     constexpr bool RES = std::is_same<T, double>::value;
+    constexpr bool XIN = RES || XR;    // x in LDS
+    static_assert(!(RES && XR), "the FP64 part keeps tau_hat and c in global memory");
     // lines per group and round: all their loads are in flight together (FP64: two, for registers)
     constexpr int NL = RES ? 2 : 3;
     const PdlpBatchLP lp = a.lps[k];
     const LPView v = view_of(a, lp);
     char* base = reinterpret_cast<char*>(lds);
-    T* xl = reinterpret_cast<T*>(base);                              // resident x (RES only)
     double* X = reinterpret_cast<double*>(base + (RES ? 8 * v.n : 0));
     T* xb = reinterpret_cast<T*>(X);
     T* yl = reinterpret_cast<T*>(base + (RES ? 16 * v.n : 8 * v.n));
     int* rpl = reinterpret_cast<int*>(yl + v.m);
     int* cpl = rpl + v.m + 1;
+    // x in LDS: at the start (FP64 part) or after the line pointers (FP32 part with XR),
+    // followed there by its partial sum, tau_hat and c
+    T* xl = RES ? reinterpret_cast<T*>(base) : reinterpret_cast<T*>(cpl + v.n + 1);
+    float* pl = reinterpret_cast<float*>(xl) + v.n;
+    float* tl = pl + v.n;
+    float* cl = tl + v.n;
     const int lr = v.lr, lc = v.lc;
     const int lane_r = threadIdx.x & ((1 << lr) - 1), lane_c = threadIdx.x & ((1 << lc) - 1);
     auto x_get = [&](int j) -> T {
-        if constexpr (RES) return xl[j];
+        if constexpr (XIN) return xl[j];
         else return v.x[j];
     };
     auto x_set = [&](int j, T x) {
-        if constexpr (RES) xl[j] = x;
+        if constexpr (XIN) xl[j] = x;
         else v.x[j] = x;
     };
 
@@ -362,7 +373,11 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
         v.y_last[i] = v.y[i];
     }
     for (int j = threadIdx.x; j < v.n; j += NT) {
-        if constexpr (RES) xl[j] = (T)v.x[j];
+        if constexpr (XIN) xl[j] = (T)v.x[j];
+        if constexpr (XR) {
+            pl[j] = 0.f;
+            cl[j] = v.c[j];
+        }
         v.x_sum[j] = 0.0;
         v.x_psum[j] = 0.f;
         v.x_last[j] = v.x[j];
@@ -373,7 +388,11 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
         float s = 0.f;
         for (int q = cpl[j] + lane_c; q < cpl[j + 1]; q += 1 << lc) s += fabsf(v.tval[q]);
         s = group_sum_rt(s, lc);
-        if (lane_c == 0) v.tau[j] = 0.999f / (s > 1e-4f ? s : 1e-4f);
+        if (lane_c == 0) {
+            const float t = 0.999f / (s > 1e-4f ? s : 1e-4f);
+            v.tau[j] = t;
+            if constexpr (XR) tl[j] = t;
+        }
     }
     __syncthreads();
 
@@ -394,7 +413,7 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
     }
 
     double acc[NS], acc2[NS];
-    kkt_eval<NT, T>(v, X, xl, yl, rpl, cpl, 0, 1.0, acc, red);
+    kkt_eval<NT, T, XIN>(v, X, xl, yl, rpl, cpl, 0, 1.0, acc, red);
     Kkt kk = kkt_of(acc, v);
     double ref_err = kk.err, err_prev = INFINITY;
     int status = isfinite(kk.err) ? PDLP_RUNNING : PDLP_NUMERICAL;
@@ -434,13 +453,19 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
             for (int k = 0; k < NL; ++k) {
                 const int j = jj[k];
                 xo[k] = x_get(j);
-                tt[k] = (T)v.tau[j];
                 lo[k] = (T)v.lo[j];
                 hi[k] = (T)v.hi[j];
-                if constexpr (RES) cc[k] = v.c64[j];
-                else cc[k] = v.c[j];
-                // the sums this round adds to, loaded with the other operands (not after the step)
-                ps[k] = lane_c == 0 ? xps[j] : (T)0;
+                if constexpr (XR) {
+                    tt[k] = tl[j];
+                    cc[k] = cl[j];
+                    ps[k] = pl[j];
+                } else {
+                    tt[k] = (T)v.tau[j];
+                    if constexpr (RES) cc[k] = v.c64[j];
+                    else cc[k] = v.c[j];
+                    // the sums this round adds to, loaded with the other operands (not after the step)
+                    ps[k] = lane_c == 0 ? xps[j] : (T)0;
+                }
                 s[k] = 0;
                 u[k] = 0;
             }
@@ -457,7 +482,8 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
                         xn = xn > hi[k] ? hi[k] : xn;
                         xb[j] = two_ * xn - xo[k];
                         x_set(j, xn);
-                        xps[j] = ps[k] + xn;
+                        if constexpr (XR) pl[j] = ps[k] + xn;
+                        else xps[j] = ps[k] + xn;
                     }
                 }
             }
@@ -513,8 +539,13 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
         // ---- check: fold partial sums (FP32 iterates), KKT of the candidate(s), decision
         if constexpr (!RES) {
             for (int j = threadIdx.x; j < v.n; j += NT) {
-                v.x_sum[j] += (double)v.x_psum[j];
-                v.x_psum[j] = 0.f;
+                if constexpr (XR) {
+                    v.x_sum[j] += (double)pl[j];
+                    pl[j] = 0.f;
+                } else {
+                    v.x_sum[j] += (double)v.x_psum[j];
+                    v.x_psum[j] = 0.f;
+                }
             }
             for (int i = threadIdx.x; i < v.m; i += NT) {
                 v.y_sum[i] += (double)v.y_psum[i];
@@ -523,11 +554,11 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
             __syncthreads();
         }
         const double w = (double)window;
-        kkt_eval<NT, T>(v, X, xl, yl, rpl, cpl, 1, w, acc, red);
+        kkt_eval<NT, T, XIN>(v, X, xl, yl, rpl, cpl, 1, w, acc, red);
         Kkt ka = kkt_of(acc, v);
         bool use_avg = true;
         if (v.adaptive) {                       // the better of average and current iterate
-            kkt_eval<NT, T>(v, X, xl, yl, rpl, cpl, 0, 1.0, acc2, red);
+            kkt_eval<NT, T, XIN>(v, X, xl, yl, rpl, cpl, 0, 1.0, acc2, red);
             const Kkt kc = kkt_of(acc2, v);
             if (!isfinite(ka.err) || kc.err < ka.err) {
                 use_avg = false;
@@ -593,7 +624,7 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
         for (int i = threadIdx.x; i < v.m; i += NT) v.y_sum[i] = (double)yl[i];
         if (status == PDLP_RUNNING) done_it = it;
     }
-    if constexpr (RES)
+    if constexpr (XIN)
         for (int j = threadIdx.x; j < v.n; j += NT) v.x[j] = (float)xl[j];
     for (int i = threadIdx.x; i < v.m; i += NT) v.y[i] = (float)yl[i];
     if (threadIdx.x == 0) {
@@ -617,12 +648,13 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
 // Persistent workgroups over one part of the work order: LPs order[first ..
 // first + count), taken through the counter *queue.  T = float: FP32 part,
 // T = double: FP64 part (each instantiation has its own register allocation).
-template <int NT, typename T>
+template <int NT, typename T, bool XR>
 __global__ void __launch_bounds__(NT) batch_kernel(PdlpBatch a, int first, int count, int* queue, int lds_limit) {
     extern __shared__ float lds[];
     __shared__ double red[NS * (NT / kWave)];
     __shared__ int s_q;
-    constexpr long NB = std::is_same<T, double>::value ? 20 : 12, MB = std::is_same<T, double>::value ? 12 : 8;
+    constexpr long NB = std::is_same<T, double>::value ? 20 : (XR ? 28 : 12);
+    constexpr long MB = std::is_same<T, double>::value ? 12 : 8;
     for (;;) {
         if (threadIdx.x == 0) s_q = atomicAdd(queue, 1);
         __syncthreads();
@@ -635,28 +667,28 @@ __global__ void __launch_bounds__(NT) batch_kernel(PdlpBatch a, int first, int c
         const int n = __builtin_amdgcn_readfirstlane(a.lps[k].n);
         // an LP that does not fit the launch's LDS is not run (its output record
         // stays as it was); the packer sizes the LDS for every LP of the part
-        if (NB * n + MB * m + 8 <= lds_limit) solve_one<NT, T>(a, k, lds, red);
+        if (NB * n + MB * m + 8 <= lds_limit) solve_one<NT, T, XR>(a, k, lds, red);
     }
 }
 
-template <int NT, typename T>
+template <int NT, typename T, bool XR>
 int launch(const PdlpBatch* a, int first, int count, int* queue, int lds, int grid, hipStream_t s) {
-    const hipError_t e = hipFuncSetAttribute((const void*)batch_kernel<NT, T>,
+    const hipError_t e = hipFuncSetAttribute((const void*)batch_kernel<NT, T, XR>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
-    batch_kernel<NT, T><<<grid, NT, lds, s>>>(*a, first, count, queue, lds);
+    batch_kernel<NT, T, XR><<<grid, NT, lds, s>>>(*a, first, count, queue, lds);
     return hip_fail(hipGetLastError(), "batch_kernel launch");
 }
 
-template <typename T>
+template <typename T, bool XR>
 int launch_part(const PdlpBatch* a, int first, int count, int* queue, int block, int lds, int grid, int cus,
                 hipStream_t s) {
     if (count == 0) return 0;
     if (lds <= 0 || lds > 160 * 1024) return fail(PDLP_E_LIMIT, "LDS per workgroup must be in (0, 160 KiB]");
     const int g = std::min(count, grid > 0 ? grid : 2 * cus);
     switch (block) {
-        case 256: return launch<256, T>(a, first, count, queue, lds, g, s);
-        case 512: return launch<512, T>(a, first, count, queue, lds, g, s);
+        case 256: return launch<256, T, XR>(a, first, count, queue, lds, g, s);
+        case 512: return launch<512, T, XR>(a, first, count, queue, lds, g, s);
         default: return fail(PDLP_E_TUNE, "block must be 256 or 512");
     }
 }
@@ -687,11 +719,13 @@ extern "C" int pdlp_batch_solve(const PdlpBatch* a, void* stream) {
     const int n32 = a->count - a->n_resident;
     const int parts = a->parts ? a->parts : 3;
     if (parts & 1) {
-        const int rc = launch_part<float>(a, 0, n32, a->queue, a->block, a->lds_bytes, a->grid, cus, s);
+        const int rc = a->x_in_lds
+                           ? launch_part<float, true>(a, 0, n32, a->queue, a->block, a->lds_bytes, a->grid, cus, s)
+                           : launch_part<float, false>(a, 0, n32, a->queue, a->block, a->lds_bytes, a->grid, cus, s);
         if (rc != 0) return rc;
     }
     if (parts & 2)
-        return launch_part<double>(a, n32, a->n_resident, a->queue + 1,
+        return launch_part<double, false>(a, n32, a->n_resident, a->queue + 1,
                                    a->block_resident ? a->block_resident : 256, a->lds_resident, a->grid_resident,
                                    cus, s);
     return 0;

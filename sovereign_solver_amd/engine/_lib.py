@@ -70,7 +70,7 @@ class PdlpBatch(C.Structure):
     _fields_ = [
         ("count", C.c_int32), ("check_every", C.c_int32), ("max_iters", C.c_int32), ("block", C.c_int32),
         ("lds_bytes", C.c_int32), ("grid", C.c_int32), ("n_resident", C.c_int32), ("block_resident", C.c_int32),
-        ("lds_resident", C.c_int32), ("grid_resident", C.c_int32), ("parts", C.c_int32), ("reserved", C.c_int32),
+        ("lds_resident", C.c_int32), ("grid_resident", C.c_int32), ("parts", C.c_int32), ("x_in_lds", C.c_int32),
         ("tol", C.c_double),
         ("lps", C.c_void_p), ("order", C.c_void_p), ("queue", C.c_void_p),
         ("row_ptr", C.c_void_p), ("col_idx", C.c_void_p), ("val", C.c_void_p),

@@ -63,6 +63,12 @@ def lds_bytes_of(m, n):
     return 12 * np.asarray(n) + 8 * np.asarray(m) + 8
 
 
+def lds_bytes_x_in_lds(m, n):
+    """LDS bytes with FP32 iterates and x, its partial sum, tau_hat and c in LDS
+    as well (the FP32 part when every one of its LPs fits)."""
+    return 28 * np.asarray(n) + 8 * np.asarray(m) + 8
+
+
 def lds_bytes_resident(m, n):
     """LDS bytes with FP64 iterates and x in LDS (the FP64 part): x, xbar (n
     doubles each), y (m doubles) and the line pointers."""
@@ -186,7 +192,16 @@ class PackedBatch:
         self.order = np.concatenate([by_size[~self.resident[by_size]], by_size[self.resident[by_size]]]).astype(np.int32)
         self.n_resident = int(self.resident.sum())
         r16 = lambda v: int((int(v) + 15) // 16 * 16)
-        self.lds_bytes = r16(lds_bytes_of(ms, ns)[~self.resident].max()) if self.n_resident < B else 16
+        # FP32 part: x, its partial sum, tau_hat and c in LDS too when every LP of the part
+        # fits (one 512-thread workgroup per CU leaves the CU's whole LDS to it)
+        fp32 = ~self.resident
+        self.x_in_lds = bool(fp32.any() and lds_bytes_x_in_lds(ms, ns)[fp32].max() <= 160 * 1024)
+        if not fp32.any():
+            self.lds_bytes = 16
+        elif self.x_in_lds:
+            self.lds_bytes = r16(lds_bytes_x_in_lds(ms, ns)[fp32].max())
+        else:
+            self.lds_bytes = r16(lds_bytes_of(ms, ns)[fp32].max())
         self.lds_resident = r16(lds_bytes_resident(ms, ns)[self.resident].max()) if self.n_resident else 16
 
     def algorithmic_bytes_per_iteration(self) -> float:
@@ -273,7 +288,8 @@ class BatchEngine:
         t = self.t
         a = PdlpBatch(count=len(self.pk.problems), check_every=check_every, max_iters=max_iters, block=self.block,
                       lds_bytes=self.pk.lds_bytes, grid=self.grid, n_resident=self.pk.n_resident,
-                      block_resident=self.block_resident, lds_resident=self.pk.lds_resident, grid_resident=0, tol=tol)
+                      block_resident=self.block_resident, lds_resident=self.pk.lds_resident, grid_resident=0,
+                      x_in_lds=int(self.pk.x_in_lds), tol=tol)
         for name in ("lps", "order", "queue", "row_ptr", "col_idx", "val", "t_col_ptr", "t_row_idx", "t_val",
                      "c", "lo", "hi", "c64", "lo_shift",
                      "ub_span", "b", "row_kind", "b64", "b_shift", "x", "x_psum", "x_sum", "tau", "y", "y_sum",

@@ -3,7 +3,7 @@
 import numpy as np
 
 from sovereign_solver_amd.engine.batch import (LP_DTYPE, OUT_DTYPE, PackedBatch, lanes_per_row, lds_bytes_of,
-                                               lds_bytes_resident)
+                                               lds_bytes_resident, lds_bytes_x_in_lds)
 from sovereign_solver_amd.lp.batch import pack_warm_start
 from sovereign_solver_amd.workloads import random_sparse_lp, transportation_lp, uneven_batch, uneven_sizes
 
@@ -59,9 +59,13 @@ def test_fp32_and_fp64_parts():
     pk = PackedBatch(probs)
     assert list(pk.resident) == [False, True, False, True] and pk.n_resident == 2
     assert list(pk.order) == [2, 0, 3, 1]              # FP32 part, then FP64 part, each largest first
-    assert pk.lds_bytes == (int(lds_bytes_of(1300, 3250)) + 15) // 16 * 16
+    # the FP32 part fits with x, its partial sum, tau_hat and c in LDS as well
+    assert pk.x_in_lds and pk.lds_bytes == (int(lds_bytes_x_in_lds(1300, 3250)) + 15) // 16 * 16
     assert pk.lds_resident == (int(lds_bytes_resident(200, 500)) + 15) // 16 * 16
     assert pk.lds_bytes <= 160 * 1024 and pk.lds_resident <= 16 * 1024
+    # an FP32-part LP beyond that LDS keeps the whole part on the smaller layout
+    pk2 = PackedBatch(probs + [random_sparse_lp(400, 6000, 0.002, seed=6)])
+    assert not pk2.x_in_lds and pk2.lds_bytes == (int(lds_bytes_of(400, 6000)) + 15) // 16 * 16
 
 
 def test_uneven_family_statistics():

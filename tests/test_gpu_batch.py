@@ -134,3 +134,18 @@ def test_batch_small_lps_with_empty_columns_converge(gpu):
         assert yardstick.rel_kkt(p, r.x, r.y)[0] <= 1.02e-4
         ref = _highs(p)
         assert abs(r.objective - ref) <= 1e-3 * (1 + abs(ref))
+
+
+def test_batch_fp32_part_layouts_agree_bitwise(gpu):
+    # x, its partial sum, tau_hat and c in LDS or in global memory: same arithmetic, same bits
+    from sovereign_solver_amd.engine.batch import BatchEngine, PackedBatch, lds_bytes_of
+    probs = uneven_batch(24, m_lo=300, m_hi=900, density=0.02, seed=4)
+    pk = PackedBatch(probs)
+    assert pk.n_resident < len(probs) and pk.x_in_lds
+    o1, x1, y1 = BatchEngine(pk).solve(1e-4, 50_000)
+    pk.x_in_lds = False
+    fp32 = ~pk.resident
+    pk.lds_bytes = int((lds_bytes_of(pk.ms, pk.ns)[fp32].max() + 15) // 16 * 16)
+    o2, x2, y2 = BatchEngine(pk).solve(1e-4, 50_000)
+    assert np.all(o1["status"] == 1)
+    assert np.array_equal(x1, x2) and np.array_equal(y1, y2) and np.array_equal(o1, o2)
