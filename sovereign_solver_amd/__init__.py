@@ -14,6 +14,7 @@ from .lp.pdhg import solve_lp_pdhg
 from .lp.pdhg_gpu import get_available_device, solve_lp_pdhg_gpu
 from .lp.batch import solve_lp_pdhg_batch
 from .lp.branching import branch_strong_batched
+from .milp import MILPSolveResult, solve_milp_batched
 
 __all__ = ["Problem", "SolveResult", "read_mps", "solve_lp_pdhg", "solve_lp_pdhg_gpu", "get_available_device",
-           "solve_lp_pdhg_batch", "branch_strong_batched"]
+           "solve_lp_pdhg_batch", "branch_strong_batched", "MILPSolveResult", "solve_milp_batched"]

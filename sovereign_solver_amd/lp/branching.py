@@ -37,19 +37,26 @@ def fractional_vars(x: np.ndarray, mask: np.ndarray) -> np.ndarray:
     return iv[frac > INTEGER_TOL]
 
 
+def node_problem(problem: Problem, lb: np.ndarray, ub: np.ndarray) -> Problem:
+    """The LP relaxation of a branch-and-bound node: the problem with bounds
+    ``lb``/``ub`` and no integer restrictions (branch_and_bound.py:95-113).  It
+    keeps the problem's matrix objects, so the relaxations of one tree form a
+    shared-matrix batch."""
+    return Problem(c=problem.c, A_ub=problem.A_ub, b_ub=problem.b_ub, A_eq=problem.A_eq, b_eq=problem.b_eq,
+                   lb=np.asarray(lb, float), ub=np.asarray(ub, float),
+                   integer_mask=np.zeros(problem.n_vars, dtype=bool), sense=problem.sense, name=problem.name)
+
+
 def child_problem(problem: Problem, lb_extra: np.ndarray, ub_extra: np.ndarray, var: int, direction: str,
                   x_val: float) -> Problem:
-    """The LP relaxation of one child (branching.py:105-128); it keeps the parent's
-    matrix objects, so all children form one shared-matrix batch."""
+    """The LP relaxation of one child (branching.py:105-128)."""
     lb = np.maximum(problem.lb, lb_extra).astype(float)
     ub = np.minimum(problem.ub, ub_extra).astype(float)
     if direction == "down":
         ub[var] = min(ub[var], np.floor(x_val))
     else:
         lb[var] = max(lb[var], np.ceil(x_val))
-    return Problem(c=problem.c, A_ub=problem.A_ub, b_ub=problem.b_ub, A_eq=problem.A_eq, b_eq=problem.b_eq,
-                   lb=lb, ub=ub, integer_mask=np.zeros(problem.n_vars, dtype=bool), sense=problem.sense,
-                   name=problem.name)
+    return node_problem(problem, lb, ub)
 
 
 def _gain(r, parent_obj: float) -> float:
