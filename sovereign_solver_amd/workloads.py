@@ -239,3 +239,17 @@ class SetCoverNodes:
             if np.all(self.A @ ub >= 1.0):
                 return self.node_from(lb, ub, name=f"setcover_node_{k}")
         raise RuntimeError("could not draw a feasible node")
+
+
+def unit_cost_set_cover(m: int = 120, n: int = 300, density: float = 0.04, seed: int = 0) -> Problem:
+    """A set-covering MILP that branch-and-bound has to search: min c'x,
+    A x >= 1 (as -A x <= -1), x binary, A random 0/1 with the given density
+    (every row has at least one 1), costs 1 + U[0, 0.01] (near-ties: many
+    optimal-looking subsets).  The covering structure of
+    benchmarks/miplib_benchmark.py:133-157 with near-unit costs."""
+    rng = _rng(5, seed)
+    A = (rng.random((m, n)) < density).astype(float)
+    A[np.arange(m), rng.integers(0, n, m)] = 1.0
+    return Problem(c=1.0 + 0.01 * rng.random(n), A_ub=sp.csr_matrix(-A), b_ub=-np.ones(m), A_eq=_empty(n),
+                   b_eq=np.zeros(0), lb=np.zeros(n), ub=np.ones(n), integer_mask=np.ones(n, bool),
+                   name=f"setcover_unit_{m}x{n}_{seed}")
