@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PDLP_ABI_VERSION 2
+#define PDLP_ABI_VERSION 3
 
 /* precision of iterates, matrix values and problem vectors */
 #define PDLP_F32 0
@@ -149,6 +149,13 @@ typedef struct PdlpDev {
     void *sell_part;          /* prec, 64 * sell_slices * ceil(sell_max_width / 8): partial
                                  dot products of the split SELL step (PdlpTune.sell >= 8) */
     const void *sell_asum;    /* prec, 64 * sell_slices: sum_j |K_ij| of each lane's row */
+    /* Diagonal scaling of a preconditioned LP: the kernels iterate on K = D_r K0 D_c
+       (b = D_r b0, c = D_c c0, bounds / D_c) while the KKT yardstick is evaluated on the
+       original LP: residual rows (K x - b)_i / D_r[i], duals D_r[i] y_i, reduced costs
+       r_j / D_c[j] (objectives, b_shift . y and ub_span . min(r, 0) are invariant).
+       The distances of the restart criteria stay in the scaled space.  NULL = unscaled. */
+    const double *row_scale;  /* m: D_r */
+    const double *col_scale;  /* n: D_c */
 } PdlpDev;
 
 /* Launch shape, chosen by the autotuner (sovereign_solver_amd.engine.tune). */

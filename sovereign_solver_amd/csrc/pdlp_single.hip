@@ -205,8 +205,15 @@ __global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src) {
             double res = ax - d.b64[i];
             if (le && res < 0.0) res = 0.0;
             const double yi = src ? avg_of(d.y_sum, d.y_psum, i, w) : (double)y[i];
+            if (d.row_scale != nullptr) {              // residual and dual of the original row
+                const double dr = d.row_scale[i];
+                res /= dr;
+                const double yo = yi * dr;
+                if (le && yo > 0.0) acc[1] += yo * yo;
+            } else if (le && yi > 0.0) {
+                acc[1] += yi * yi;                      // slack column reduced cost -y_i
+            }
             acc[0] += res * res;
-            if (le && yi > 0.0) acc[1] += yi * yi;      // slack column reduced cost -y_i
             acc[3] += d.b_shift[i] * yi;
             if (ylast != nullptr) {
                 const double dy = yi - (double)ylast[i];
@@ -243,8 +250,12 @@ __global__ void __launch_bounds__(256) kkt_cols_kernel(PdlpDev d, int src) {
             const double r = d.c64[j] - aty;
             const double span = d.ub_span != nullptr ? d.ub_span[j] : INFINITY;
             const double rneg = r < 0.0 ? r : 0.0;
-            if (isfinite(span)) acc[3] += span * rneg;   // upper-bound row dual w_j = min(r_j, 0)
-            else acc[1] += rneg * rneg;
+            if (isfinite(span)) {
+                acc[3] += span * rneg;                   // upper-bound row dual w_j = min(r_j, 0)
+            } else {
+                const double ro = d.col_scale != nullptr ? rneg / d.col_scale[j] : rneg;
+                acc[1] += ro * ro;                       // reduced cost of the original column
+            }
             const double xj = src ? avg_of(d.x_sum, d.x_psum, j, w) : (double)x[j];
             const double sh = d.lo_shift != nullptr ? d.lo_shift[j] : 0.0;
             acc[2] += d.c64[j] * (xj - sh);
@@ -486,6 +497,8 @@ int check_dev(const PdlpDev* d) {
         return fail(PDLP_E_ARG, "a required PdlpDev pointer is NULL");
     if ((d->x_psum == nullptr) != (d->y_psum == nullptr))
         return fail(PDLP_E_ARG, "x_psum and y_psum must both be set or both be NULL");
+    if ((d->row_scale == nullptr) != (d->col_scale == nullptr))
+        return fail(PDLP_E_ARG, "row_scale and col_scale must both be set or both be NULL");
     if (d->x_psum != nullptr && d->prec != PDLP_F32)
         return fail(PDLP_E_ARG, "partial sums are for PDLP_F32 only");
     if (d->sell_slices < 0 || d->n_csr_rows < 0 || d->n_csr_rows > d->m)

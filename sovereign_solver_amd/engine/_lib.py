@@ -13,7 +13,7 @@ import threading
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "_lib", "libpdlp_hip.so")
 
-PDLP_ABI_VERSION = 2
+PDLP_ABI_VERSION = 3
 PDLP_F32, PDLP_F64 = 0, 1
 PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL, PDLP_INFEASIBLE = 0, 1, 3, 4
 PDLP_ROW_EQ, PDLP_ROW_LE = 0, 1
@@ -55,6 +55,7 @@ class PdlpDev(C.Structure):
         ("sell_slices", C.c_int32), ("n_csr_rows", C.c_int32), ("sell_ptr", _p), ("sell_rows", _p),
         ("sell_col", _p), ("sell_val", _p), ("csr_rows", _p), ("sell_max_width", C.c_int32),
         ("sell_reserved", C.c_int32), ("sell_part", _p), ("sell_asum", _p),
+        ("row_scale", _p), ("col_scale", _p),
     ]
 
 

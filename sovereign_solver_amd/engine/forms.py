@@ -27,6 +27,12 @@ upper bound, lb_f = finite lower bound else 0, r = c − Kᵀy):
   ``(b − K lb_f)ᵀy + Σ_F (u_j − lb_f,j) w_j``;
 * normalisers ``max(1, ‖b_ref‖)`` and ``max(1, ‖c‖)`` with
   ``b_ref = [b − K lb_f ; (u − lb_f)_F]``.
+
+A box form may be *scaled* (``box_form(p, scaling=...)``, methods in
+:mod:`.scaling`): K, b, c and the bounds are those of ``D_r K D_c``, and
+``row_scale`` / ``col_scale`` carry ``D_r`` / ``D_c`` so that the kernels
+evaluate the yardstick on the original LP; ``norm_b`` and ``norm_c`` are the
+original LP's.  :func:`extract` maps the solution back.
 """
 
 from __future__ import annotations
@@ -39,6 +45,7 @@ import scipy.sparse as sp
 
 from ..config import FEASIBILITY_TOL
 from ..problem import Problem
+from . import scaling as S
 
 ROW_EQ, ROW_LE = 0, 1
 
@@ -65,6 +72,9 @@ class HostForm:
     lb_f: Optional[np.ndarray] = None
     row_sign: Optional[np.ndarray] = None
     n_ub_rows: int = 0
+    # scaled box form only: K = diag(row_scale) K0 diag(col_scale)
+    row_scale: Optional[np.ndarray] = None
+    col_scale: Optional[np.ndarray] = None
     meta: dict = field(default_factory=dict)
 
     @property
@@ -95,7 +105,7 @@ def _transpose(A: sp.csr_matrix) -> sp.csr_matrix:
     return _canonical(A.T.tocsr())
 
 
-def box_form(p: Problem) -> HostForm:
+def box_form(p: Problem, scaling: str = "none") -> HostForm:
     n = p.n_vars
     m_ub, m_eq = p.n_ineq, p.n_eq
     K = _canonical(sp.vstack([sp.csr_matrix(p.A_ub), sp.csr_matrix(p.A_eq)], format="csr")
@@ -110,6 +120,15 @@ def box_form(p: Problem) -> HostForm:
     span = np.where(fin_ub, ub - lb_f, np.inf)
     norm_b = max(1.0, float(np.sqrt(b_shift @ b_shift + np.sum(span[fin_ub] ** 2))))
     c = np.asarray(p.c, float)
+    norm_c = max(1.0, float(np.linalg.norm(c)))
+    Dr = Dc = None
+    if scaling != "none":
+        Dr, Dc = S.scales(K, scaling)
+        K = _canonical(S.scale_matrix(K, Dr, Dc))
+        b, b_shift, c = b * Dr, b_shift * Dr, c * Dc
+        lb, ub, lb_f, span = lb / Dc, ub / Dc, lb_f / Dc, span / Dc
+    elif scaling not in S.METHODS:
+        raise ValueError(f"scaling must be one of {S.METHODS}, got {scaling!r}")
     trivial_box = bool(np.all(lb == 0.0) and not fin_ub.any())
     return HostForm(
         kind="box", K=K, KT=_transpose(K), c=c, b=b,
@@ -117,8 +136,8 @@ def box_form(p: Problem) -> HostForm:
         row_kind=kind if m_ub else None,
         b_shift=b_shift, lo_shift=None if not lb_f.any() else lb_f,
         ub_span=span if fin_ub.any() else None,
-        norm_b=norm_b, norm_c=max(1.0, float(np.linalg.norm(c))),
-        n_orig=n, m_ub=m_ub, m_eq=m_eq, lb_f=lb_f,
+        norm_b=norm_b, norm_c=norm_c,
+        n_orig=n, m_ub=m_ub, m_eq=m_eq, lb_f=lb_f, row_scale=Dr, col_scale=Dc,
     )
 
 
@@ -158,10 +177,12 @@ def reference_form(p: Problem) -> HostForm:
     )
 
 
-def make_form(p: Problem, formulation: str) -> HostForm:
+def make_form(p: Problem, formulation: str, scaling: str = "none") -> HostForm:
     if formulation == "box":
-        return box_form(p)
+        return box_form(p, scaling)
     if formulation == "reference":
+        if scaling != "none":
+            raise ValueError("scaling applies to formulation='box' (the reference form is the parity mode)")
         return reference_form(p)
     raise ValueError(f"formulation must be 'box' or 'reference', got {formulation!r}")
 
@@ -175,6 +196,9 @@ def extract(form: HostForm, p: Problem, x_dev: np.ndarray, y_dev: np.ndarray):
     if form.kind == "box":
         x = np.asarray(x_dev[: form.n_orig], float)
         y = np.asarray(y_dev, float)
+        if form.col_scale is not None:
+            x = x * form.col_scale
+            y = y * form.row_scale
     else:
         x = np.asarray(x_dev[: form.n_orig], float) + form.lb_f
         ys = np.asarray(y_dev, float) * form.row_sign

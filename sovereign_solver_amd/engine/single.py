@@ -73,6 +73,9 @@ class SingleEngine:
             self.b_shift = dev_tensor(form.b_shift, f64, dv)
             self.lo_shift = None if form.lo_shift is None else dev_tensor(form.lo_shift, f64, dv)
             self.ub_span = None if form.ub_span is None else dev_tensor(form.ub_span, f64, dv)
+            # scaled form: D_r, D_c for the KKT yardstick of the original LP
+            self.row_scale = None if form.row_scale is None else dev_tensor(form.row_scale, f64, dv)
+            self.col_scale = None if form.col_scale is None else dev_tensor(form.col_scale, f64, dv)
             self.x = torch.zeros(self.n, dtype=T, device=dv)
             self.xbar = torch.zeros(self.n, dtype=T, device=dv)
             self.y = torch.zeros(self.m, dtype=T, device=dv)
@@ -125,7 +128,7 @@ class SingleEngine:
             sell_slices=self.sell_slices, n_csr_rows=self.n_csr_rows, sell_ptr=ptr(self.sell_ptr),
             sell_rows=ptr(self.sell_rows), sell_col=ptr(self.sell_col), sell_val=ptr(self.sell_val),
             csr_rows=ptr(self.csr_rows), sell_max_width=self.sell_max_width, sell_part=ptr(self.sell_part),
-            sell_asum=ptr(self.sell_asum))
+            sell_asum=ptr(self.sell_asum), row_scale=ptr(self.row_scale), col_scale=ptr(self.col_scale))
         self._graphs = {}
         self._graph_tune = None
 

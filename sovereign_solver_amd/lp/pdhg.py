@@ -39,6 +39,13 @@ Keyword-only extensions (defaults keep the reference's contract):
 ``check_every``, ``tune``
     KKT/restart cadence (50 = reference) and launch shape (default: the
     autotuner's heuristic).
+``scaling``
+    Diagonal preconditioning of the box form (``"none"`` default,
+    ``"ruiz"``, ``"geometric"``, ``"pock_chambolle"``, ``"ruiz_pc"``; the
+    reference's ``solver/utils/scaling.py`` scalers, which its PDHG never
+    calls).  PDHG iterates on the scaled LP; the tolerance applies to the
+    KKT yardstick of the original LP, and x, y come back unscaled.  The
+    returned ``step_scale`` is the primal weight of the scaled LP.
 
 The reference returns no duals and no residuals; the result additionally
 carries ``y``, ``rel_kkt``, ``rp``, ``rd``, ``gap`` and ``restarts``.
@@ -88,6 +95,9 @@ def _warm_start(form, p: Problem, x0, y0):
         y = np.asarray(y0, float).copy()
         if form.m_ub:
             y[: form.m_ub] = np.minimum(y[: form.m_ub], 0.0)
+    if form.col_scale is not None:                      # into the scaled LP's space
+        x = None if x is None else x / form.col_scale
+        y = None if y is None else y / form.row_scale
     return x, y
 
 
@@ -106,8 +116,9 @@ def solve_lp_pdhg(
     check_every: int = 50,
     tune: Optional[Tune] = None,
     use_graphs: bool = True,
+    scaling: str = "none",
 ) -> SolveResult:
-    form = make_form(problem, formulation)
+    form = make_form(problem, formulation, scaling)
     if form.m == 0 or form.n == 0:                      # pdhg.py:171-172
         return SolveResult("optimal", np.zeros(problem.n_vars), 0.0, 0)
     prec = _precision(precision, tol)

@@ -42,7 +42,8 @@ def test_every_declared_symbol_exported_and_bound():
 def test_struct_sizes_match_header():
     # PdlpCtrl: 8 int32 + 12 double + (double, 2 int32, double); PdlpDev: 4 int32 + 27 pointers; PdlpTune: 8 int32
     assert C.sizeof(L.PdlpCtrl) == 8 * 4 + 13 * 8 + 2 * 4 + 8
-    assert C.sizeof(L.PdlpDev) == 4 * 4 + 27 * 8 + 2 * 4 + 5 * 8 + 2 * 4 + 2 * 8
+    # ... then the SELL-64 block (2 int32, 5 pointers, 2 int32, 2 pointers) and the two scale vectors
+    assert C.sizeof(L.PdlpDev) == 4 * 4 + 27 * 8 + 2 * 4 + 5 * 8 + 2 * 4 + 2 * 8 + 2 * 8
     assert C.sizeof(L.PdlpTune) == 10 * 4
 
 
