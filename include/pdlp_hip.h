@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PDLP_ABI_VERSION 3
+#define PDLP_ABI_VERSION 4
 
 /* precision of iterates, matrix values and problem vectors */
 #define PDLP_F32 0
@@ -43,9 +43,10 @@ extern "C" {
 #define PDLP_RUNNING    0
 #define PDLP_OPTIMAL    1
 #define PDLP_NUMERICAL  3    /* NaN/Inf in a KKT evaluation */
-#define PDLP_INFEASIBLE 4    /* primal infeasible (family engine): the dual iterate is a
-                                Farkas certificate, b'y - max_{lo<=x<=hi} (K'y)'x > 0 with
-                                y <= 0 on PDLP_ROW_LE rows */
+#define PDLP_INFEASIBLE 4    /* primal infeasible (single-instance box form and family
+                                engine): the dual iterate is a Farkas certificate,
+                                b'y - max_{lo<=x<=hi} (K'y)'x > 0 with y <= 0 on
+                                PDLP_ROW_LE rows */
 
 /* row kinds of the box form */
 #define PDLP_ROW_EQ 0        /* K_i x  = b_i, dual free     */
@@ -66,9 +67,9 @@ extern "C" {
                                          with (x0, y0) the starting point */
 
 /* Number of doubles of PdlpDev.partials the engine needs:
- * 2 candidates x 2 passes x PDLP_PARTIAL_BLOCKS blocks x 6 sums. */
+ * 2 candidates x 2 passes x PDLP_PARTIAL_BLOCKS blocks x 10 sums. */
 #define PDLP_PARTIAL_BLOCKS 1024
-#define PDLP_PARTIALS_LEN   (4 * PDLP_PARTIAL_BLOCKS * 6)
+#define PDLP_PARTIALS_LEN   (4 * PDLP_PARTIAL_BLOCKS * 10)
 
 /*
  * Device control block, one per instance, device-resident.  The engine
@@ -79,7 +80,7 @@ extern "C" {
 typedef struct PdlpCtrl {
     int32_t iter;            /* iterations executed                          */
     int32_t window;          /* iterations in the current averaging window   */
-    int32_t status;          /* PDLP_RUNNING / PDLP_OPTIMAL / PDLP_NUMERICAL */
+    int32_t status;          /* PDLP_RUNNING / PDLP_OPTIMAL / PDLP_NUMERICAL / PDLP_INFEASIBLE */
     int32_t restarts;
     int32_t done_iter;       /* iteration at which status left RUNNING       */
     int32_t restart_pending; /* set by the check, consumed by the apply step */

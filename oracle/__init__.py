@@ -21,6 +21,9 @@ Contents and what pins each part:
   and against brute-force vertex enumeration on tiny LPs.
 * ``mlp_ref``   — FP64 NumPy MLP forward/backward.  Pinned by central finite
   differences.
+* ``farkas``    — the Farkas value of a dual vector (> 0 proves primal
+  infeasibility).  Pinned by a hand-made certificate and by F <= 0 for random
+  duals of feasible LPs.
 * ``generators`` are not here: the workload generators are product code
   (``sovereign_solver_amd.workloads``) because users build the same families.
 """

@@ -170,9 +170,12 @@ __global__ void __launch_bounds__(256) dual_sell_kernel(PdlpDev d) {
 // 0 = rows, 1 = columns; k = 0 rp^2, 1 rd^2, 2 primal objective, 3 dual
 // objective, 4 |dx|^2 and 5 |dy|^2: the distance of the candidate from the
 // last restart point in the step-size metric (sum dx_j^2 / tau_hat_j and
-// sum dy_i^2 / sigma_hat_i), used by the adaptive primal weight.
+// sum dy_i^2 / sigma_hat_i), used by the adaptive primal weight; then the
+// Farkas test of the candidate's y (box form): 6 b_shift'y, 7 the maximum of
+// (K'y)'(x - lo_shift) over the shifted box, 8 sum |b_shift_i y_i| and 9 the
+// sum of the |terms| of 7.
 
-constexpr int NP = 6;
+constexpr int NP = 10;
 
 __device__ __forceinline__ double* part_ptr(const PdlpDev& d, int src, int part, int blk) {
     return d.partials + ((size_t)(src * 2 + part) * PDLP_PARTIAL_BLOCKS + blk) * NP;
@@ -180,6 +183,15 @@ __device__ __forceinline__ double* part_ptr(const PdlpDev& d, int src, int part,
 
 __device__ __forceinline__ double avg_of(const double* s, const float* ps, int k, double w) {
     return ps != nullptr ? (s[k] + (double)ps[k]) / w : s[k] / w;
+}
+
+// Largest value of g * z over z in [lo - lo_shift, span]: z >= 0 up to span when the
+// lower bound is finite, z <= span without one; +inf when unbounded (then y
+// certifies nothing).
+__device__ __forceinline__ double farkas_support(double g, double span, bool lo_inf) {
+    if (g > 0.0) return isfinite(span) ? g * span : INFINITY;
+    if (g < 0.0 && lo_inf) return INFINITY;
+    return 0.0;
 }
 
 template <typename T, int TPR>
@@ -193,7 +205,7 @@ __global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src) {
     const T* __restrict__ y = (const T*)d.y;
     const T* ylast = (const T*)d.y_last;
     const int gpb = blockDim.x / TPR, lane = threadIdx.x % TPR;
-    double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double acc[NP] = {};
     for (int i = blockIdx.x * gpb + threadIdx.x / TPR; i < d.m; i += gridDim.x * gpb) {
         double ax, asum;
         if (src) row_dot<TPR, false, double>(d.row_ptr, d.col_idx, v, i, lane,
@@ -214,7 +226,10 @@ __global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src) {
                 acc[1] += yi * yi;                      // slack column reduced cost -y_i
             }
             acc[0] += res * res;
-            acc[3] += d.b_shift[i] * yi;
+            const double by = d.b_shift[i] * yi;
+            acc[3] += by;
+            acc[6] += by;
+            acc[8] += fabs(by);
             if (ylast != nullptr) {
                 const double dy = yi - (double)ylast[i];
                 acc[5] += dy * dy * (asum > 1e-4 ? asum : 1e-4) / 0.999;
@@ -239,7 +254,7 @@ __global__ void __launch_bounds__(256) kkt_cols_kernel(PdlpDev d, int src) {
     const T* __restrict__ y = (const T*)d.y;
     const T* xlast = (const T*)d.x_last;
     const int gpb = blockDim.x / TPR, lane = threadIdx.x % TPR;
-    double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double acc[NP] = {};
     for (int j = blockIdx.x * gpb + threadIdx.x / TPR; j < d.n; j += gridDim.x * gpb) {
         double aty, asum;
         if (src) row_dot<TPR, false, double>(d.t_row_ptr, d.t_col_idx, tv, j, lane,
@@ -256,6 +271,11 @@ __global__ void __launch_bounds__(256) kkt_cols_kernel(PdlpDev d, int src) {
                 const double ro = d.col_scale != nullptr ? rneg / d.col_scale[j] : rneg;
                 acc[1] += ro * ro;                       // reduced cost of the original column
             }
+            // Farkas test: max over x - lo_shift in [lo - lo_shift, span] of (K'y)_j (x - lo_shift)_j
+            const bool lo_inf = d.lo != nullptr && isinf((double)((const T*)d.lo)[j]);
+            const double u = farkas_support(aty, span, lo_inf);
+            acc[7] += u;
+            acc[9] += fabs(u);
             const double xj = src ? avg_of(d.x_sum, d.x_psum, j, w) : (double)x[j];
             const double sh = d.lo_shift != nullptr ? d.lo_shift[j] : 0.0;
             acc[2] += d.c64[j] * (xj - sh);
@@ -274,6 +294,7 @@ __global__ void __launch_bounds__(256) kkt_cols_kernel(PdlpDev d, int src) {
 
 struct KktVal {
     double rp, rd, gap, err, po, dobj, dx, dy;
+    bool infeasible;     // the candidate's y is a Farkas certificate of primal infeasibility
 };
 
 __device__ KktVal kkt_value(const double (&s)[NP], const PdlpCtrl* ctrl) {
@@ -287,6 +308,12 @@ __device__ KktVal kkt_value(const double (&s)[NP], const PdlpCtrl* ctrl) {
     if (!isfinite(k.po) || !isfinite(k.dobj)) k.err = NAN;
     k.dx = sqrt(s[4]);
     k.dy = sqrt(s[5]);
+    // F(y) = b_shift'y - max_box (K'y)'(x - lo_shift) > 0 proves that no x in the box satisfies
+    // the rows (y <= 0 on <= rows holds by projection, for the average too).  The margin 1e-6
+    // of the summed magnitudes exceeds the FP32 rounding of K's values (6e-8) and the FP64
+    // rounding of the sums, so a feasible LP is never flagged.
+    const double F = s[6] - s[7], scale = s[8] + s[9];
+    k.infeasible = isfinite(F) && F > 0.0 && F > 1e-6 * scale;
     return k;
 }
 
@@ -302,7 +329,7 @@ __global__ void __launch_bounds__(256) kkt_finalize_kernel(PdlpDev d, int nb_row
     }
     double sums[2][NP];
     for (int c = 0; c < 2; ++c) {
-        double acc[NP] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        double acc[NP] = {};
         if (cands & (1 << c)) {
             for (int b = threadIdx.x; b < nb_rows; b += blockDim.x) {
                 const double* p = part_ptr(d, c, 0, b);
@@ -330,6 +357,18 @@ __global__ void __launch_bounds__(256) kkt_finalize_kernel(PdlpDev d, int nb_row
     ctrl->d_obj = k.dobj;
     ctrl->last_err = k.err;
     ctrl->restart_pending = 0;
+    // a candidate whose y is a Farkas certificate ends the solve (not with the reference
+    // scheme: the reference reports such LPs at the iteration limit)
+    const bool inf0 = (cands & 1) && k0.infeasible, inf1 = (cands & 2) && k1.infeasible;
+    if (scheme != PDLP_SCHEME_REFERENCE && (inf0 || inf1)) {
+        ctrl->status = PDLP_INFEASIBLE;
+        ctrl->done_iter = ctrl->iter;
+        if (mode != 0) ctrl->checks += 1;
+        ctrl->cand = inf0 ? 0 : 1;
+        ctrl->restart_pending = inf0 ? 0 : 1;   // finish at the average when it holds the certificate
+        ctrl->apply_window = ctrl->window;
+        return;
+    }
     if (!isfinite(k.err)) {
         ctrl->status = PDLP_NUMERICAL;
         ctrl->done_iter = ctrl->iter;

@@ -23,7 +23,8 @@ from .device import dev_tensor, ptr, require_gpu, stream_handle, struct_from_byt
 from .forms import HostForm
 from .tune import Tune, heuristic_tune
 
-_STATUS = {L.PDLP_RUNNING: "iteration_limit", L.PDLP_OPTIMAL: "optimal", L.PDLP_NUMERICAL: "numerical_error"}
+_STATUS = {L.PDLP_RUNNING: "iteration_limit", L.PDLP_OPTIMAL: "optimal", L.PDLP_NUMERICAL: "numerical_error",
+           L.PDLP_INFEASIBLE: "infeasible"}
 
 # Largest graph (in check blocks).  Small graphs bound the surplus of no-op
 # launches after convergence (at most two graphs are in flight).

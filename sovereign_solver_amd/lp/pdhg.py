@@ -49,6 +49,16 @@ Keyword-only extensions (defaults keep the reference's contract):
 
 The reference returns no duals and no residuals; the result additionally
 carries ``y``, ``rel_kkt``, ``rp``, ``rd``, ``gap`` and ``restarts``.
+
+Status ``"infeasible"`` (not with the reference scheme, which keeps the
+reference's behaviour of running to the iteration limit): the dual ``y``
+(the current iterate or the window average, whichever passed) is a Farkas
+certificate, ``b_shift'y - max over the box of (K'y)'(x - lb_f) > 0`` with
+``y <= 0`` on the ``<=`` rows, checked in FP64 with a relative margin of
+1e-6 so a feasible LP is never flagged.  A certificate that needs ``K'y = 0``
+exactly on a column unbounded in the needed direction (free columns) is
+not found: such LPs end at the iteration limit.  ``"numerical_error"``: a
+NaN or Inf in a KKT evaluation.
 """
 
 from __future__ import annotations

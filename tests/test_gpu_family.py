@@ -11,6 +11,7 @@ import pytest
 from scipy.optimize import linprog
 
 from oracle import yardstick
+from oracle.farkas import farkas_value as _farkas_value
 from sovereign_solver_amd import solve_lp_pdhg_batch
 from sovereign_solver_amd.engine.family import FamilyBatch, FamilyEngine
 from sovereign_solver_amd.workloads import DispatchFamily, SetCoverNodes
@@ -148,22 +149,6 @@ def test_family_xcd_affinity_and_long_split_agree_bitwise(gpu):
     assert np.array_equal(c[1], d[1]) and np.array_equal(c[0]["iterations"], d[0]["iterations"])
     r = solve_lp_pdhg_batch(probs, tol=1e-4, engine="family")
     _check(probs, r, n_highs=3)
-
-
-def _farkas_value(p, y):
-    """F(y) = b_shift'y - max over the shifted box of (K'y)'x, on the host in FP64
-    with the original data: F > 0 (and y <= 0 on <= rows) proves infeasibility."""
-    import scipy.sparse as sp
-    K = sp.vstack([sp.csr_matrix(p.A_ub), sp.csr_matrix(p.A_eq)]).tocsr()
-    b = np.concatenate([p.b_ub, p.b_eq])
-    y = np.asarray(y, float).copy()
-    y[: p.n_ineq] = np.minimum(y[: p.n_ineq], 0.0)
-    lbf = np.where(np.isfinite(p.lb), p.lb, 0.0)
-    g = K.T @ y
-    span = p.ub - lbf
-    sup = np.where(g > 0, np.where(np.isfinite(span), g * span, np.inf),
-                   np.where((g < 0) & ~np.isfinite(p.lb), np.inf, 0.0))
-    return float((b - K @ lbf) @ y - sup.sum())
 
 
 def test_family_detects_infeasible_nodes_with_certificates(gpu):

@@ -123,3 +123,26 @@ def test_yardstick_zero_at_highs_optimum():
     y = r.eqlin.marginals          # HiGHS: d obj / d b
     err = yardstick.rel_kkt(p, r.x, y)[0]
     assert err < 1e-8
+
+
+def test_farkas_hand_made_certificate():
+    # x >= 0, x1 + x2 <= 1, x1 + x2 >= 2 (as -x1 - x2 <= -2): y = (-1, -1) gives
+    # K'y = 0 and b'y = -1 + 2 = 1 > 0
+    from oracle.farkas import farkas_value
+    p = Problem(c=np.ones(2), A_ub=sp.csr_matrix(np.array([[1.0, 1.0], [-1.0, -1.0]])), b_ub=np.array([1.0, -2.0]),
+                A_eq=sp.csr_matrix((0, 2)), b_eq=np.zeros(0), lb=np.zeros(2), ub=np.full(2, np.inf),
+                integer_mask=np.zeros(2, bool))
+    assert farkas_value(p, [-1.0, -1.0]) == pytest.approx(1.0)
+    assert farkas_value(p, [-1.0, 0.0]) == pytest.approx(-1.0)   # K'y = (-1, -1) < 0 costs nothing with lb = 0 ...
+    q = Problem(c=p.c, A_ub=p.A_ub, b_ub=p.b_ub, A_eq=p.A_eq, b_eq=p.b_eq, lb=np.zeros(2), ub=np.full(2, 3.0),
+                integer_mask=p.integer_mask)
+    assert farkas_value(q, [0.0, -1.0]) == pytest.approx(2.0 - 6.0)   # ... but g > 0 over a span of 3 costs 3 g
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_farkas_never_positive_on_feasible_lps(seed):
+    from oracle.farkas import farkas_value
+    p = random_sparse_lp(30, 60, 0.1, seed=seed, family=2)
+    rng = np.random.default_rng(seed)
+    for _ in range(200):
+        assert farkas_value(p, rng.standard_normal(p.n_constraints) * 10.0 ** rng.uniform(-3, 3)) <= 1e-9
