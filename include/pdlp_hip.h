@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PDLP_ABI_VERSION 4
+#define PDLP_ABI_VERSION 5
 
 /* precision of iterates, matrix values and problem vectors */
 #define PDLP_F32 0
@@ -442,6 +442,8 @@ int pdlp_family_graph_capture_step(const PdlpFamily* f, int32_t which, int32_t r
 /* ----------------------------------------------------------------------------
  * Learned warm start: a 3-layer MLP (d_in -> d_h -> d_h -> d_out, ReLU) whose
  * GEMMs run on BF16 MFMA (v_mfma_f32_32x32x16_bf16) with FP32 accumulation.
+ * pdlp_mlp_bwd_bf16 uses dout as the split-K work space of dz2 = dout w3^T
+ * once the bias gradient has been taken from it.
  * The reference has no learned component (its PDHG always starts from 0,
  * pdhg.py:175-176); these kernels are new.  BF16 values are passed as
  * uint16_t bit patterns.
@@ -452,20 +454,28 @@ int pdlp_family_graph_capture_step(const PdlpFamily* f, int32_t which, int32_t r
  * op(B)(k,n) = trans_b ? B[n*ldb + k] : B[k*ldb + n]   (BF16)
  * epilogue, in order: + bias[n] (FP32, may be NULL); ReLU if relu;
  * zero where mask[m*ldc + n] <= 0 (BF16, may be NULL; the ReLU derivative);
- * stores to C (FP32, may be NULL) and/or Cb (BF16, may be NULL), both ldc. */
+ * stores to C (FP32, may be NULL) and/or Cb (BF16, may be NULL), both ldc.
+ * Tiles of 128 x 128 per 256-thread workgroup, K in stages of 64; 16-byte
+ * operand loads when lda and ldb are multiples of 8 and A, B are 16-byte
+ * aligned, 2-byte loads otherwise.
+ * split_k > 1 (a small M x N with a long K): the K range is cut into split_k
+ * parts whose FP32 partial products go to work (split_k * M * N floats,
+ * caller-owned), then summed in a fixed order (the same bits on every run)
+ * before the epilogue; split_k <= 1 and work unused otherwise. */
 typedef struct PdlpGemm {
     int32_t M, N, K;
     int32_t lda, ldb, ldc;
     int32_t trans_a, trans_b;
     int32_t relu, accumulate;
     float alpha;
-    int32_t pad;
+    int32_t split_k;
     const uint16_t* A;
     const uint16_t* B;
     const float* bias;
     const uint16_t* mask;
     float* C;
     uint16_t* Cb;
+    float* work;
 } PdlpGemm;
 
 int pdlp_gemm_bf16(const PdlpGemm* g, void* stream);

@@ -1,13 +1,26 @@
 // MLP kernels of the learned warm start: BF16 MFMA GEMM with fused epilogues,
 // loss gradient, bias gradients, Adam, casts.
 //
-// GEMM tiling: a 256-thread workgroup (4 wavefronts) computes a 64x64 tile of
-// C; each wavefront owns a 32x32 sub-tile held in 16 FP32 accumulators per
-// lane and issues v_mfma_f32_32x32x16_bf16 over K in steps of 16.  Operand
-// tiles (64 x 32 of A, 32 x 64 of B) are staged in LDS with the K dimension
-// contiguous, so each lane fetches its 8 consecutive K values of a row (A) or
-// column (B) with one 16-byte LDS read.  Global loads map consecutive threads
-// to the contiguous dimension of each operand (row- or column-major).
+// GEMM tiling: a 256-thread workgroup (4 wavefronts, 2 x 2) computes a
+// 128 x 128 tile of C; each wavefront owns a 64 x 64 sub-tile as 2 x 2 blocks
+// of 32 x 32, each held in 16 FP32 accumulators per lane and updated by
+// v_mfma_f32_32x32x16_bf16 over K in steps of 16.  K advances in stages of 64:
+// the next stage's operand tiles are loaded from global memory into registers
+// while the MFMAs of the current stage run out of LDS (a second register set,
+// two stages ahead, measured no faster on the MLP's shapes).  In LDS both operand
+// tiles keep K contiguous (As[m][k], Bs[n][k]), so each lane fetches its 8
+// consecutive K values of a row (A) or column (B) with one 16-byte LDS read.
+// Global loads move 8 BF16 values (16 bytes) per thread along the operand's
+// contiguous dimension when the leading dimensions are multiples of 8 and the
+// bases 16-byte aligned; an operand whose contiguous dimension is M or N is
+// transposed on its way into LDS.  Otherwise consecutive threads load
+// consecutive 2-byte values.
+//
+// A long K with few output tiles (the MLP's dz2 = dout w3^T: M = batch,
+// N = d_h, K = d_out) is split: workgroup z of split_k computes the partial
+// product of its K range into work[z], and a second kernel sums the parts in
+// order of z and applies the epilogue, so the result does not depend on
+// scheduling.
 //
 // MFMA operand/result layout (32x32x16): lane l supplies A[i][k] for
 // i = l % 32, k = 8*(l/32) .. 8*(l/32)+7 and B[k][j] for j = l % 32 and the
@@ -34,69 +47,202 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
     return (uint16_t)(u >> 16);
 }
 
-constexpr int BM = 64, BN = 64, BK = 32, PAD = 8;
+constexpr int BM = 128, BN = 128, BK = 64, PAD = 8, LDT = BK + PAD;
+constexpr int GEMM_THREADS = 256;
 
-__global__ void __launch_bounds__(256) gemm_kernel(PdlpGemm g) {
-    __shared__ __align__(16) uint16_t As[BM][BK + PAD];
-    __shared__ __align__(16) uint16_t Bs[BN][BK + PAD];
-    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    f32x16 acc;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+// One operand tile of R rows (M or N) by BK values of K, held in registers
+// between its global load and its LDS store.  TR: the operand's contiguous
+// dimension is R (op(A) = A^T / op(B) = B stored by K rows) instead of K.
+// Transposed 16-byte path: a unit is two consecutive K rows (k, k+1) of 8
+// consecutive R values; in LDS the pair goes out as 8 dword stores
+// (S[r+u][k], S[r+u][k+1]).  A wavefront's 64 units are 32 K pairs of two
+// R vectors 8 rows apart, whose dwords fall in 64 different banks (row
+// stride 36 dwords).
+template <bool VEC, bool TR, int R>
+struct TileLoader {
+    static constexpr int VPT = R * BK / 8 / GEMM_THREADS;       // 8-value vectors per thread
+    static constexpr int EPT = R * BK / GEMM_THREADS;           // 2-byte values per thread
+    static constexpr int NREG = VEC ? VPT : EPT;
+    typedef unsigned short u16x8v __attribute__((ext_vector_type(8)));
+    u16x8v v[VEC ? VPT : 1];
+    uint16_t e[VEC ? 1 : EPT];
 
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
-        // A tile -> As[m][k]
+    // element (r, k) of the tile at (r0, k0): X[(r0+r)*ld + k0+k] (K contiguous) or X[(k0+k)*ld + r0+r]
+    __device__ __forceinline__ void load(const uint16_t* __restrict__ X, int ld, int r0, int rmax, int k0, int kend) {
+        const int t = threadIdx.x;
+        if constexpr (VEC) {
 #pragma unroll
-        for (int e0 = 0; e0 < BM * BK; e0 += 256) {
-            const int e = e0 + tid;
-            int mm, kk;
-            if (g.trans_a) { mm = e % BM; kk = e / BM; }     // A[k*lda + m]: m contiguous
-            else { kk = e % BK; mm = e / BK; }               // A[m*lda + k]: k contiguous
-            const int gm = m0 + mm, gk = k0 + kk;
-            uint16_t v = 0;
-            if (gm < g.M && gk < g.K) v = g.trans_a ? g.A[(size_t)gk * g.lda + gm] : g.A[(size_t)gm * g.lda + gk];
-            As[mm][kk] = v;
-        }
-        // B tile -> Bs[n][k]
+            for (int q = 0; q < VPT; ++q) {
+                int r, k;
+                if constexpr (TR) {
+                    const int w = t + (q >> 1) * GEMM_THREADS;       // unit: K pair w % 32, R vector w / 32
+                    k = 2 * (w % (BK / 2)) + (q & 1);
+                    r = (w / (BK / 2)) * 8;
+                } else {
+                    const int w = t + q * GEMM_THREADS;
+                    r = w / (BK / 8);
+                    k = (w % (BK / 8)) * 8;
+                }
+                const int gr = r0 + r, gk = k0 + k;
+                u16x8v x;
+                if constexpr (TR) {
+                    if (gk < kend && gr + 7 < rmax) {
+                        x = *reinterpret_cast<const u16x8v*>(X + (size_t)gk * ld + gr);
+                    } else {
 #pragma unroll
-        for (int e0 = 0; e0 < BN * BK; e0 += 256) {
-            const int e = e0 + tid;
-            int nn, kk;
-            if (g.trans_b) { kk = e % BK; nn = e / BK; }     // B[n*ldb + k]: k contiguous
-            else { nn = e % BN; kk = e / BN; }               // B[k*ldb + n]: n contiguous
-            const int gn = n0 + nn, gk = k0 + kk;
-            uint16_t v = 0;
-            if (gn < g.N && gk < g.K) v = g.trans_b ? g.B[(size_t)gn * g.ldb + gk] : g.B[(size_t)gk * g.ldb + gn];
-            Bs[nn][kk] = v;
-        }
-        __syncthreads();
+                        for (int u = 0; u < 8; ++u)
+                            x[u] = (gk < kend && gr + u < rmax) ? X[(size_t)gk * ld + gr + u] : (uint16_t)0;
+                    }
+                } else {
+                    if (gr < rmax && gk + 7 < kend) {
+                        x = *reinterpret_cast<const u16x8v*>(X + (size_t)gr * ld + gk);
+                    } else {
 #pragma unroll
-        for (int kk = 0; kk < BK; kk += 16) {
-            const u16x8 a = *reinterpret_cast<const u16x8*>(&As[wm + (lane & 31)][kk + 8 * (lane >> 5)]);
-            const u16x8 b = *reinterpret_cast<const u16x8*>(&Bs[wn + (lane & 31)][kk + 8 * (lane >> 5)]);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
-                                                          acc, 0, 0, 0);
+                        for (int u = 0; u < 8; ++u)
+                            x[u] = (gr < rmax && gk + u < kend) ? X[(size_t)gr * ld + gk + u] : (uint16_t)0;
+                    }
+                }
+                v[q] = x;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+                const int w = t + q * GEMM_THREADS;
+                int r, k;
+                if constexpr (TR) { k = w / R; r = w % R; }
+                else { r = w / BK; k = w % BK; }
+                const int gr = r0 + r, gk = k0 + k;
+                e[q] = (gr < rmax && gk < kend) ? (TR ? X[(size_t)gk * ld + gr] : X[(size_t)gr * ld + gk]) : (uint16_t)0;
+            }
         }
-        __syncthreads();
     }
 
-    const int col = n0 + wn + (lane & 31);
-    if (col >= g.N) return;
-    const float bias = g.bias != nullptr ? g.bias[col] : 0.f;
+    __device__ __forceinline__ void store(uint16_t (*S)[LDT]) const {
+        const int t = threadIdx.x;
+        if constexpr (VEC) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int row = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        if (row >= g.M) continue;
-        const size_t o = (size_t)row * g.ldc + col;
-        float v = g.alpha * acc[r];
-        if (g.accumulate && g.C != nullptr) v += g.C[o];
-        v += bias;
-        if (g.relu) v = v > 0.f ? v : 0.f;
-        if (g.mask != nullptr && !(bf2f(g.mask[o]) > 0.f)) v = 0.f;
-        if (g.C != nullptr) g.C[o] = v;
-        if (g.Cb != nullptr) g.Cb[o] = f2bf(v);
+            for (int q = 0; q < VPT; ++q) {
+                if constexpr (TR) {
+                    if (q & 1) continue;
+                    const int w = t + (q >> 1) * GEMM_THREADS;
+                    const int k = 2 * (w % (BK / 2)), r = (w / (BK / 2)) * 8;
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        *reinterpret_cast<uint32_t*>(&S[r + u][k]) = (uint32_t)v[q][u] | ((uint32_t)v[q + 1][u] << 16);
+                } else {
+                    const int w = t + q * GEMM_THREADS;
+                    const int r = w / (BK / 8), k = (w % (BK / 8)) * 8;
+                    *reinterpret_cast<u16x8v*>(&S[r][k]) = v[q];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < EPT; ++q) {
+                const int w = t + q * GEMM_THREADS;
+                int r, k;
+                if constexpr (TR) { k = w / R; r = w % R; }
+                else { r = w / BK; k = w % BK; }
+                S[r][k] = e[q];
+            }
+        }
+    }
+};
+
+// alpha * acc (+ C) + bias, ReLU, mask; stores C and/or Cb
+__device__ __forceinline__ void epilogue(const PdlpGemm& g, int row, int col, float acc) {
+    const size_t o = (size_t)row * g.ldc + col;
+    float v = g.alpha * acc;
+    if (g.accumulate && g.C != nullptr) v += g.C[o];
+    if (g.bias != nullptr) v += g.bias[col];
+    if (g.relu) v = v > 0.f ? v : 0.f;
+    if (g.mask != nullptr && !(bf2f(g.mask[o]) > 0.f)) v = 0.f;
+    if (g.C != nullptr) g.C[o] = v;
+    if (g.Cb != nullptr) g.Cb[o] = f2bf(v);
+}
+
+// blockIdx.x: N tile, blockIdx.y: M tile, blockIdx.z: K part (kc values of K each, a multiple of BK)
+template <bool VEC, bool TA, bool TB>
+__global__ void __launch_bounds__(GEMM_THREADS) gemm_kernel(PdlpGemm g, int kc) {
+    __shared__ __align__(16) uint16_t As[BM][LDT];
+    __shared__ __align__(16) uint16_t Bs[BN][LDT];
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int kbeg = blockIdx.z * kc, kend = min(g.K, kbeg + kc);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+    // the MFMAs of one stage out of LDS
+    auto compute = [&]() {
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            u16x8 a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                a[i] = *reinterpret_cast<const u16x8*>(&As[wm + 32 * i + (lane & 31)][kk + 8 * (lane >> 5)]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                b[j] = *reinterpret_cast<const u16x8*>(&Bs[wn + 32 * j + (lane & 31)][kk + 8 * (lane >> 5)]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                        __builtin_bit_cast(bf16x8, a[i]), __builtin_bit_cast(bf16x8, b[j]), acc[i][j], 0, 0, 0);
+        }
+    };
+    typedef TileLoader<VEC, TA, BM> LA;
+    typedef TileLoader<VEC, !TB, BN> LB;  // op(B)(k, n): B[n*ldb + k] has K contiguous
+    // one stage: the registers of stage k0 into LDS, stage knext into the same registers, MFMAs
+    auto stage = [&](LA& la, LB& lb, int knext) {
+        __syncthreads();                  // the previous stage's MFMAs are done with LDS
+        la.store(As);
+        lb.store(Bs);
+        __syncthreads();
+        if (knext < kend) {               // in flight during this stage's MFMAs
+            la.load(g.A, g.lda, m0, g.M, knext, kend);
+            lb.load(g.B, g.ldb, n0, g.N, knext, kend);
+        }
+        compute();
+    };
+    LA la;
+    LB lb;
+    if (kbeg < kend) {
+        la.load(g.A, g.lda, m0, g.M, kbeg, kend);
+        lb.load(g.B, g.ldb, n0, g.N, kbeg, kend);
+    }
+    for (int k0 = kbeg; k0 < kend; k0 += BK) stage(la, lb, k0 + BK);
+
+    const bool split = gridDim.z > 1;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int col = n0 + wn + 32 * j + (lane & 31);
+        if (col >= g.N) continue;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = m0 + wm + 32 * i + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (row >= g.M) continue;
+                if (split) g.work[((size_t)blockIdx.z * g.M + row) * g.N + col] = acc[i][j][r];
+                else epilogue(g, row, col, acc[i][j][r]);
+            }
+        }
+    }
+}
+
+// sum of the split_k partial products in order of z, then the epilogue
+__global__ void __launch_bounds__(256) gemm_split_finish_kernel(PdlpGemm g, int parts) {
+    const int64_t total = (int64_t)g.M * g.N;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (int z = 0; z < parts; ++z) s += g.work[(size_t)z * total + e];
+        epilogue(g, (int)(e / g.N), (int)(e % g.N), s);
     }
 }
 
@@ -104,32 +250,60 @@ __global__ void zero1_kernel(float* p) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0.f;
 }
 
-// dout = 2 w (out - t) / (B N); loss += w (out - t)^2 / (B N)
+// dout = 2 w (out - t) / (B N); loss += w (out - t)^2 / (B N).  blockIdx.x
+// strides over the columns, blockIdx.y over the rows (no index division);
+// one atomic per workgroup for the loss.
 __global__ void __launch_bounds__(256) loss_grad_kernel(const float* out, const float* t, const float* w, float* dout,
-                                                        uint16_t* dout_b, float* loss, int64_t total, int N,
-                                                        float inv) {
+                                                        uint16_t* dout_b, float* loss, int B, int N, float inv) {
     __shared__ double lds[16 * 4];
     double part = 0.0;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const float wj = w != nullptr ? w[e % N] : 1.f;
-        const float d = out[e] - t[e];
-        const float gr = 2.f * wj * d * inv;
-        dout[e] = gr;
-        dout_b[e] = f2bf(gr);
-        part += (double)(wj * d * d) * inv;
+    for (int b = blockIdx.y; b < B; b += gridDim.y) {
+        const size_t row = (size_t)b * N;
+        for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < N; j += gridDim.x * blockDim.x) {
+            const float wj = w != nullptr ? w[j] : 1.f;
+            const float d = out[row + j] - t[row + j];
+            const float gr = 2.f * wj * d * inv;
+            dout[row + j] = gr;
+            dout_b[row + j] = f2bf(gr);
+            part += (double)(wj * d * d) * inv;
+        }
     }
     double acc[4] = {part, 0.0, 0.0, 0.0};
     block_sum<4>(acc, lds);
     if (threadIdx.x == 0) atomicAdd(loss, (float)acc[0]);
 }
 
-// db[j] = sum_b dz[b][j]; dz FP32 (dzf) or BF16 (dzb)
-__global__ void __launch_bounds__(256) colsum_kernel(const float* dzf, const uint16_t* dzb, float* db, int B, int N) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= N) return;
+// db[j] = sum_b dz[b][j]; dz FP32 (dzf) or BF16 (dzb).  A workgroup takes 64
+// columns; its 16 wavefronts sum every 16th row (8 rows' loads in flight per
+// lane), then add their sums in LDS in a fixed order.
+constexpr int CS_COLS = 64, CS_GROUPS = 16, CS_UNROLL = 8;
+
+__global__ void __launch_bounds__(CS_COLS * CS_GROUPS) colsum_kernel(const float* dzf, const uint16_t* dzb, float* db,
+                                                                   int B, int N) {
+    __shared__ float part[CS_GROUPS][CS_COLS];
+    const int c = threadIdx.x % CS_COLS, grp = threadIdx.x / CS_COLS;
+    const int j = blockIdx.x * CS_COLS + c;
     float s = 0.f;
-    for (int b = 0; b < B; ++b) s += dzf != nullptr ? dzf[(size_t)b * N + j] : bf2f(dzb[(size_t)b * N + j]);
-    db[j] = s;
+    if (j < N) {
+        for (int b0 = grp; b0 < B; b0 += CS_GROUPS * CS_UNROLL) {
+            float v[CS_UNROLL];
+#pragma unroll
+            for (int u = 0; u < CS_UNROLL; ++u) {
+                const int b = b0 + u * CS_GROUPS;
+                v[u] = b < B ? (dzf != nullptr ? dzf[(size_t)b * N + j] : bf2f(dzb[(size_t)b * N + j])) : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < CS_UNROLL; ++u) s += v[u];
+        }
+    }
+    part[grp][c] = s;
+    __syncthreads();
+    if (grp == 0 && j < N) {
+        float t = part[0][c];
+#pragma unroll
+        for (int g = 1; g < CS_GROUPS; ++g) t += part[g][c];
+        db[j] = t;
+    }
 }
 
 __global__ void __launch_bounds__(256) adam_kernel(float* w, const float* g, float* m, float* v, uint16_t* wb,
@@ -152,14 +326,44 @@ __global__ void __launch_bounds__(256) cast_kernel(const float* x, uint16_t* y, 
         y[i] = f2bf(x[i]);
 }
 
+template <bool VEC, bool TA, bool TB>
+void launch_gemm(const PdlpGemm& g, dim3 grid, int kc, hipStream_t s) {
+    gemm_kernel<VEC, TA, TB><<<grid, GEMM_THREADS, 0, s>>>(g, kc);
+}
+
+unsigned grid1(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)); }
+
 int gemm(const PdlpGemm& g, hipStream_t s) {
     if (g.M < 0 || g.N < 0 || g.K < 0) return fail(PDLP_E_SHAPE, "gemm sizes must be >= 0");
     if (g.M == 0 || g.N == 0) return 0;
     if (!g.A || !g.B || (!g.C && !g.Cb)) return fail(PDLP_E_ARG, "gemm: A, B and an output are required");
     if (g.accumulate && !g.C) return fail(PDLP_E_ARG, "gemm: accumulate needs C");
-    dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM);
-    gemm_kernel<<<grid, 256, 0, s>>>(g);
-    return hip_fail(hipGetLastError(), "gemm_kernel launch");
+    if (g.split_k > 1 && !g.work) return fail(PDLP_E_ARG, "gemm: split_k > 1 needs work");
+    if (g.split_k > 1024) return fail(PDLP_E_LIMIT, "gemm: split_k at most 1024");
+    const int stages = (g.K + BK - 1) / BK;
+    const int parts = std::max(1, std::min(g.split_k, stages));
+    const int kc = std::max(1, (stages + parts - 1) / parts) * BK;   // K values per part, a multiple of BK
+    const int used = std::max(1, (g.K + kc - 1) / kc);               // parts that hold any K
+    dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, used);
+    const bool vec = g.lda % 8 == 0 && g.ldb % 8 == 0 && (reinterpret_cast<uintptr_t>(g.A) & 15) == 0 &&
+                     (reinterpret_cast<uintptr_t>(g.B) & 15) == 0;
+    const int sel = (vec ? 4 : 0) | (g.trans_a ? 2 : 0) | (g.trans_b ? 1 : 0);
+    switch (sel) {
+        case 0: launch_gemm<false, false, false>(g, grid, kc, s); break;
+        case 1: launch_gemm<false, false, true>(g, grid, kc, s); break;
+        case 2: launch_gemm<false, true, false>(g, grid, kc, s); break;
+        case 3: launch_gemm<false, true, true>(g, grid, kc, s); break;
+        case 4: launch_gemm<true, false, false>(g, grid, kc, s); break;
+        case 5: launch_gemm<true, false, true>(g, grid, kc, s); break;
+        case 6: launch_gemm<true, true, false>(g, grid, kc, s); break;
+        default: launch_gemm<true, true, true>(g, grid, kc, s); break;
+    }
+    int rc = hip_fail(hipGetLastError(), "gemm_kernel launch");
+    if (!rc && used > 1) {
+        gemm_split_finish_kernel<<<grid1((int64_t)g.M * g.N), 256, 0, s>>>(g, used);
+        rc = hip_fail(hipGetLastError(), "gemm_split_finish_kernel launch");
+    }
+    return rc;
 }
 
 PdlpGemm make(int M, int N, int K, const uint16_t* A, int lda, int ta, const uint16_t* B, int ldb, int tb,
@@ -172,8 +376,6 @@ PdlpGemm make(int M, int N, int K, const uint16_t* A, int lda, int ta, const uin
     g.A = A; g.B = B; g.bias = bias; g.mask = mask; g.C = C; g.Cb = Cb;
     return g;
 }
-
-unsigned grid1(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192)); }
 
 }  // namespace
 }  // namespace pdlp
@@ -206,22 +408,32 @@ int pdlp_mlp_bwd_bf16(const PdlpMlp* p, void* stream) {
     const int B = p->batch, di = p->d_in, dh = p->d_h, dout = p->d_out;
     const int64_t total = (int64_t)B * dout;
     zero1_kernel<<<1, 64, 0, s>>>(p->loss);
-    loss_grad_kernel<<<grid1(total), 256, 0, s>>>(p->out, p->target, p->out_weight, p->dout, p->dout_b, p->loss,
-                                                 total, dout, 1.f / (float)total);
-    colsum_kernel<<<(dout + 255) / 256, 256, 0, s>>>(p->dout, nullptr, p->gb3, B, dout);
+    const dim3 lg((unsigned)std::min((dout + 255) / 256, 64), (unsigned)std::min(B, 16));
+    loss_grad_kernel<<<lg, 256, 0, s>>>(p->out, p->target, p->out_weight, p->dout, p->dout_b, p->loss, B, dout,
+                                        1.f / (float)total);
+    colsum_kernel<<<(dout + CS_COLS - 1) / CS_COLS, CS_COLS * CS_GROUPS, 0, s>>>(p->dout, nullptr, p->gb3, B, dout);
     int rc = hip_fail(hipGetLastError(), "mlp loss/bias kernels");
     // gw3 = h2^T dout ; dz2 = (dout w3^T) * [h2 > 0]
     if (!rc) rc = gemm(make(dh, dout, B, p->h2, dh, 1, p->dout_b, dout, 0, nullptr, 0, nullptr, p->gw3, nullptr, dout), s);
-    if (!rc) rc = gemm(make(B, dh, dout, p->dout_b, dout, 0, p->w3, dout, 1, nullptr, 0, p->h2, nullptr, p->dz2, dh), s);
     if (!rc) {
-        colsum_kernel<<<(dh + 255) / 256, 256, 0, s>>>(nullptr, p->dz2, p->gb2, B, dh);
+        // K = d_out is long and the output (batch x d_h) has few tiles: split K, with dout
+        // (free once gb3 is taken from it) as the work space of split_k * batch * d_h floats
+        PdlpGemm g = make(B, dh, dout, p->dout_b, dout, 0, p->w3, dout, 1, nullptr, 0, p->h2, nullptr, p->dz2, dh);
+        const int tiles = ((B + BM - 1) / BM) * ((dh + BN - 1) / BN);
+        const int stages = (dout + BK - 1) / BK;
+        g.split_k = std::max(1, std::min({dout / std::max(dh, 1), 256 / tiles, stages / 4}));
+        g.work = p->dout;
+        rc = gemm(g, s);
+    }
+    if (!rc) {
+        colsum_kernel<<<(dh + CS_COLS - 1) / CS_COLS, CS_COLS * CS_GROUPS, 0, s>>>(nullptr, p->dz2, p->gb2, B, dh);
         rc = hip_fail(hipGetLastError(), "colsum");
     }
     // gw2 = h1^T dz2 ; dz1 = (dz2 w2^T) * [h1 > 0]
     if (!rc) rc = gemm(make(dh, dh, B, p->h1, dh, 1, p->dz2, dh, 0, nullptr, 0, nullptr, p->gw2, nullptr, dh), s);
     if (!rc) rc = gemm(make(B, dh, dh, p->dz2, dh, 0, p->w2, dh, 1, nullptr, 0, p->h1, nullptr, p->dz1, dh), s);
     if (!rc) {
-        colsum_kernel<<<(dh + 255) / 256, 256, 0, s>>>(nullptr, p->dz1, p->gb1, B, dh);
+        colsum_kernel<<<(dh + CS_COLS - 1) / CS_COLS, CS_COLS * CS_GROUPS, 0, s>>>(nullptr, p->dz1, p->gb1, B, dh);
         rc = hip_fail(hipGetLastError(), "colsum");
     }
     // gw1 = x^T dz1

@@ -13,7 +13,7 @@ import threading
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "_lib", "libpdlp_hip.so")
 
-PDLP_ABI_VERSION = 4
+PDLP_ABI_VERSION = 5
 PDLP_F32, PDLP_F64 = 0, 1
 PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL, PDLP_INFEASIBLE = 0, 1, 3, 4
 PDLP_ROW_EQ, PDLP_ROW_LE = 0, 1
@@ -115,8 +115,8 @@ class PdlpGemm(C.Structure):
         ("M", C.c_int32), ("N", C.c_int32), ("K", C.c_int32),
         ("lda", C.c_int32), ("ldb", C.c_int32), ("ldc", C.c_int32),
         ("trans_a", C.c_int32), ("trans_b", C.c_int32), ("relu", C.c_int32), ("accumulate", C.c_int32),
-        ("alpha", C.c_float), ("pad", C.c_int32),
-        ("A", _p), ("B", _p), ("bias", _p), ("mask", _p), ("C", _p), ("Cb", _p),
+        ("alpha", C.c_float), ("split_k", C.c_int32),
+        ("A", _p), ("B", _p), ("bias", _p), ("mask", _p), ("C", _p), ("Cb", _p), ("work", _p),
     ]
 
 

@@ -130,6 +130,11 @@ def output_blocks(fam):
     return [slice(0, fam.n), slice(fam.n, fam.n + fam.m), slice(fam.n + fam.m, fam.n + fam.m + 1)]
 
 
+def padded_width(d: int) -> int:
+    """Output columns of the network for d targets: d rounded up to a multiple of 8."""
+    return -(-d // 8) * 8
+
+
 def train(feats: np.ndarray, targets: np.ndarray, d_h: int = 512, epochs: int = 20, batch: int = 512,
           lr: float = 1e-3, seed: int = 0, step_weight: float = 200.0, log=None, out_blocks=None,
           normalise_inputs: bool = True):
@@ -138,11 +143,16 @@ def train(feats: np.ndarray, targets: np.ndarray, d_h: int = 512, epochs: int = 
     nin = Normaliser(feats, identity=not normalise_inputs)
     nout = Normaliser(targets, blocks=out_blocks) if out_blocks is not None else Normaliser(targets)
     X = torch.from_numpy(nin.fwd(feats).astype(np.float32)).to(dev).to(torch.bfloat16)
-    T = torch.from_numpy(nout.fwd(targets).astype(np.float32)).to(dev)
-    w = np.ones(targets.shape[1], np.float32)
-    w[-1] = step_weight                      # one scalar among many outputs
+    # the output layer is padded to a multiple of 8 columns (zero targets, zero
+    # loss weight): rows of 16 bytes keep every GEMM operand on 16-byte loads
+    d_t = targets.shape[1]
+    d_net = padded_width(d_t)
+    T = torch.from_numpy(np.pad(nout.fwd(targets).astype(np.float32), ((0, 0), (0, d_net - d_t)))).to(dev)
+    w = np.zeros(d_net, np.float32)
+    w[:d_t] = 1.0
+    w[d_t - 1] = step_weight                 # one scalar among many outputs
     W = torch.from_numpy(w).to(dev)
-    mlp = WarmStartMLP(feats.shape[1], d_h, targets.shape[1], seed=seed, device=dev)
+    mlp = WarmStartMLP(feats.shape[1], d_h, d_net, seed=seed, device=dev)
     rng = np.random.default_rng(seed)
     N = feats.shape[0]
     losses = []
@@ -174,7 +184,7 @@ def predict(mlp: WarmStartMLP, nin: Normaliser, nout: Normaliser, feats: np.ndar
         X = torch.from_numpy(nin.fwd(feats).astype(np.float32)).to(mlp.device).to(torch.bfloat16).contiguous()
     out = mlp.forward(X)
     mlp.stream.synchronize()
-    return nout.inv(out.cpu().numpy().astype(np.float64))
+    return nout.inv(out.cpu().numpy()[:, : nout.mean.size].astype(np.float64))
 
 
 def warm_starts(pred: np.ndarray, n: int, m: int, use_x: bool = True, use_y: bool = True,
@@ -194,8 +204,9 @@ class LearnedStart:
         dv = mlp.device
         self.in_mean = torch.from_numpy(nin.mean.astype(np.float32)).to(dv)
         self.in_inv_std = torch.from_numpy((1.0 / nin.std).astype(np.float32)).to(dv)
-        self.out_scale = torch.from_numpy(nout.std.astype(np.float32)).to(dv)
-        self.out_shift = torch.from_numpy(nout.mean.astype(np.float32)).to(dv)
+        pad = mlp.d_out - nout.mean.size        # padded output columns (unused by the engine)
+        self.out_scale = torch.from_numpy(np.pad(nout.std, (0, pad), constant_values=1.0).astype(np.float32)).to(dv)
+        self.out_shift = torch.from_numpy(np.pad(nout.mean, (0, pad)).astype(np.float32)).to(dv)
 
     def forward(self, feats: np.ndarray):
         """Network output [B, d_out] (normalised units) on the device, and an event

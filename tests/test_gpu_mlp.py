@@ -17,8 +17,10 @@ def _bf(a, dev):
     return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(dev).to(torch.bfloat16).contiguous()
 
 
+# (64, 64, 32), (256, 512, 224), (200, 136, 72): leading dimensions multiples of 8
+# (16-byte loads; 200 and 136 leave partial tiles); the others 2-byte loads
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
-@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (70, 131, 45), (5, 300, 17)])
+@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (70, 131, 45), (5, 300, 17), (256, 512, 224), (200, 136, 72)])
 def test_gemm_matches_fp64(gpu, ta, tb, M, N, K):
     rng = np.random.default_rng(M * 7 + N + K)
     A = mlp_ref.bf16_round(rng.standard_normal((M, K)))
@@ -39,19 +41,82 @@ def test_gemm_matches_fp64(gpu, ta, tb, M, N, K):
     assert np.all(np.abs(got - want) <= bound), np.max(np.abs(got - want) / bound)
 
 
-def test_gemm_identity_pins_lane_layout(gpu):
-    """A x I = A and I x B = B: a transposed or shifted operand/result layout breaks these."""
+@pytest.mark.parametrize("n", [64, 63, 136])
+def test_gemm_identity_pins_lane_layout(gpu, n):
+    """A x I = A and I x B = B: a transposed or shifted operand/result layout breaks
+    these (n = 64, 136: 16-byte loads; 63: 2-byte loads)."""
     rng = np.random.default_rng(3)
-    A = mlp_ref.bf16_round(rng.standard_normal((64, 64)))
-    I = np.eye(64)
+    A = mlp_ref.bf16_round(rng.standard_normal((n, n)))
+    I = np.eye(n)
     for left, right in ((A, I), (I, A)):
         At, Bt = _bf(left, gpu), _bf(right, gpu)
-        Ct = torch.zeros(64, 64, dtype=torch.float32, device=gpu)
-        g = L.PdlpGemm(M=64, N=64, K=64, lda=64, ldb=64, ldc=64, alpha=1.0, A=At.data_ptr(), B=Bt.data_ptr(),
+        Ct = torch.zeros(n, n, dtype=torch.float32, device=gpu)
+        g = L.PdlpGemm(M=n, N=n, K=n, lda=n, ldb=n, ldc=n, alpha=1.0, A=At.data_ptr(), B=Bt.data_ptr(),
                        C=Ct.data_ptr())
         L.check(L.lib().pdlp_gemm_bf16(C.byref(g), None), "gemm")
         torch.cuda.synchronize()
         assert np.array_equal(Ct.cpu().numpy(), A)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 1)])
+def test_gemm_padded_leading_dims(gpu, ta, tb):
+    """lda, ldb multiples of 8 with K = 45 and M, N = 70, 131 inside them: 16-byte
+    loads with partial vectors at the edges."""
+    M, N, K = 70, 131, 45
+    rng = np.random.default_rng(11)
+    A = mlp_ref.bf16_round(rng.standard_normal((M, K)))
+    B = mlp_ref.bf16_round(rng.standard_normal((K, N)))
+    As = np.zeros((K, 72) if ta else (M, 48))
+    Bs = np.zeros((N, 48) if tb else (K, 136))
+    if ta:
+        As[:, :M] = A.T
+    else:
+        As[:, :K] = A
+    if tb:
+        Bs[:, :K] = B.T
+    else:
+        Bs[:, :N] = B
+    At, Bt = _bf(As, gpu), _bf(Bs, gpu)
+    Ct = torch.zeros(M, N, dtype=torch.float32, device=gpu)
+    g = L.PdlpGemm(M=M, N=N, K=K, lda=As.shape[1], ldb=Bs.shape[1], ldc=N, trans_a=ta, trans_b=tb, alpha=1.0,
+                   A=At.data_ptr(), B=Bt.data_ptr(), C=Ct.data_ptr())
+    L.check(L.lib().pdlp_gemm_bf16(C.byref(g), None), "gemm")
+    torch.cuda.synchronize()
+    want = A @ B
+    bound = (K + 2) * 2.0**-23 * (np.abs(A) @ np.abs(B)) + 1e-6
+    assert np.all(np.abs(Ct.cpu().numpy() - want) <= bound)
+
+
+@pytest.mark.parametrize("split,K", [(16, 4000), (27, 14025), (3, 130)])
+def test_gemm_split_k(gpu, split, K):
+    """Split K (dz2's shape: B^T operand, ReLU-derivative mask, BF16 output): within
+    the FP32 bound of the FP64 product, and the same bits on a second run."""
+    M, N = 256, 512
+    rng = np.random.default_rng(K)
+    A = mlp_ref.bf16_round(rng.standard_normal((M, K)))
+    B = mlp_ref.bf16_round(rng.standard_normal((K, N)))
+    mask = (rng.random((M, N)) > 0.3).astype(np.float64)
+    At, Bt, Mt = _bf(A, gpu), _bf(B.T, gpu), _bf(mask, gpu)
+    work = torch.zeros(split * M * N, dtype=torch.float32, device=gpu)
+    outs = []
+    for _ in range(2):
+        Ct = torch.zeros(M, N, dtype=torch.float32, device=gpu)
+        Cb = torch.zeros(M, N, dtype=torch.bfloat16, device=gpu)
+        g = L.PdlpGemm(M=M, N=N, K=K, lda=K, ldb=K, ldc=N, trans_a=0, trans_b=1, alpha=1.0, split_k=split,
+                       A=At.data_ptr(), B=Bt.data_ptr(), mask=Mt.data_ptr(), C=Ct.data_ptr(), Cb=Cb.data_ptr(),
+                       work=work.data_ptr())
+        L.check(L.lib().pdlp_gemm_bf16(C.byref(g), None), "gemm")
+        torch.cuda.synchronize()
+        outs.append((Ct.cpu().numpy(), Cb.float().cpu().numpy()))
+    want = (A @ B) * mask
+    bound = (K + 2) * 2.0**-23 * (np.abs(A) @ np.abs(B)) + 1e-6
+    assert np.all(np.abs(outs[0][0] - want) <= bound)
+    np.testing.assert_allclose(outs[0][1], outs[0][0], rtol=2.0**-8)      # BF16 copy of the same values
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    # split_k without a work space is refused
+    g.work = 0
+    with pytest.raises(L.PdlpError):
+        L.check(L.lib().pdlp_gemm_bf16(C.byref(g), None), "gemm")
 
 
 def test_mlp_forward_backward_vs_oracle(gpu):
