@@ -342,6 +342,29 @@ def family_rate(res, secs):
     return {"lps_per_s": ok / secs, "s": secs, "ok": ok, "iters_mean": float(its.mean()), "iters_max": int(its.max())}
 
 
+def mlp_forward_rate(mlp, batch: int, reps: int = 20):
+    """MFMA rate of the network's forward pass on ``batch`` instances: the three
+    GEMMs (2 B (d_in d_h + d_h d_h + d_h d_out) FLOP) timed with HIP events on the
+    network's stream over ``reps`` back-to-back passes."""
+    import torch
+    with torch.cuda.stream(mlp.stream):
+        x = torch.randn(batch, mlp.d_in, device=mlp.device).to(torch.bfloat16).contiguous()
+    for _ in range(3):
+        mlp.forward(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(mlp.stream):
+        e0.record(mlp.stream)
+        for _ in range(reps):
+            mlp.forward(x)
+        e1.record(mlp.stream)
+    mlp.stream.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / reps
+    flop = 2.0 * batch * (mlp.d_in * mlp.d_h + mlp.d_h * mlp.d_h + mlp.d_h * mlp.d_out)
+    tflops = flop / (us * 1e-6) / 1e12
+    return {"batch": batch, "gflop": flop / 1e9, "us": us, "tflops": tflops, "peak_tflops": 2500.0,
+            "frac": tflops / 2500.0, "bound": "mfma (dense BF16)"}
+
+
 def bench_learned(args):
     """Learned warm start on the economic-dispatch family (200 generators x 24 periods)."""
     import torch
@@ -365,6 +388,9 @@ def bench_learned(args):
     t_train = time.perf_counter() - t1
     log(f"trained in {t_train:.1f}s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
     ls = W.LearnedStart(mlp, nin, nout)
+    mlp_rate = mlp_forward_rate(mlp, len(eval_ids))
+    log(f"network forward on {len(eval_ids)} instances: {mlp_rate['gflop']:.1f} GFLOP in {mlp_rate['us']:.1f} us "
+        f"({mlp_rate['tflops']:.0f} TFLOP/s, {100 * mlp_rate['frac']:.1f}% of dense BF16 peak)")
     probs = [fam.problem(int(k)) for k in eval_ids]
     roof = family_roofline(probs, args.tol, f"dispatch_{len(probs)}")
 
@@ -403,6 +429,7 @@ def bench_learned(args):
                                f"shared-matrix batched engine; value = learned (x + primal weight) start, "
                                f"network inference and host<->device transfers included",
                    "train_label_s": t_label, "train_s": t_train, "epochs": args.epochs,
+                   "mlp_forward_mfma": mlp_rate,
                    "loss_first_last": [losses[0], losses[-1]],
                    "cold_batched": cold, "learned_x_weight": learned, "learned_x_y_weight": learned_full,
                    "solution_as_start": perfect, "persistent_engine_cold": persistent,
