@@ -174,7 +174,12 @@ typedef struct PdlpTune {
                              0 = same as block */
     int32_t scheme;       /* PDLP_SCHEME_* restart scheme */
     int32_t sell;         /* dual step on the SELL-64 slices of PdlpDev (if any): 0 off;
-                             1 one wavefront per slice; k >= 8 (multiple of 8) a wavefront
+                             1 one wavefront per slice; 2..7 slices and the CSR rows in
+                             one launch, a workgroup per CSR row and per slice piece
+                             (threads / rows of a piece: 2: 1024/64, 3: 512/64,
+                             4: 1024/16, 5: 1024/8, 6: 1024/32, 7: 512/16), the piece's
+                             entries split among its wavefronts and their sums added
+                             in a fixed order; k >= 8 (multiple of 8) a wavefront
                              per k entries of a slice, partial sums added in order */
     int32_t reserved;
 } PdlpTune;

@@ -163,6 +163,72 @@ __global__ void __launch_bounds__(256) dual_sell_kernel(PdlpDev d) {
     dual_update<T, false>(d, sigma_scale, i, ax, asum);
 }
 
+// SELL-64 slices and CSR rows in one launch, one workgroup of NT threads per
+// item.  The first slices * (64 / R) workgroups each take R consecutive rows
+// of a slice: a wavefront's lane l works on row l % R of them at entry phase
+// l / R, so one access reads entry k .. k + 64/R - 1 of the R rows as 64/R
+// contiguous pieces of R values (and their gathers of xbar share cache
+// lines); the slice's entries are split into NT / 64 contiguous ranges, one
+// per wavefront.  The partial sums meet in LDS and are added in a fixed order
+// (wavefront, then phase).  The remaining workgroups each take one row of
+// csr_rows with all NT threads.  Block 0 counts the iteration.
+template <typename T, int NT, int R>
+__global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d) {
+    constexpr int NW = NT / 64, P = 64 / R;
+    static_assert(R >= 1 && R <= 64 && 64 % R == 0, "rows per workgroup divide the slice");
+    __shared__ T part[2][NW][64];
+    __shared__ T red[2 * 16];
+    PdlpCtrl* ctrl = d.ctrl;
+    if (ctrl->status != PDLP_RUNNING) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {   // single writer; read only by later kernels
+        ctrl->iter += 1;
+        ctrl->window += 1;
+    }
+    const T sigma_scale = (T)ctrl->sigma_scale;
+    const T* __restrict__ xbar = (const T*)d.xbar;
+    const int b = blockIdx.x;
+    if (b < d.sell_slices * P) {
+        const int sl = b / P, rb = (b % P) * R;
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        const int r = lane % R, ph = lane / R;
+        const T* __restrict__ val = (const T*)d.sell_val;
+        const int32_t* __restrict__ col = d.sell_col;
+        const int beg = d.sell_ptr[sl], width = (d.sell_ptr[sl + 1] - beg) >> 6;
+        const int kc = (width + NW - 1) / NW;
+        const int k0 = min(width, w * kc), k1 = min(width, k0 + kc);
+        T ax = T(0), asum = T(0);
+        int q = beg + (k0 + ph) * 64 + rb + r;
+#pragma unroll 8
+        for (int k = k0 + ph; k < k1; k += P, q += 64 * P) {
+            const T a = val[q];
+            ax += a * xbar[col[q]];
+            asum += fabs(a);
+        }
+        part[0][w][lane] = ax;
+        part[1][w][lane] = asum;
+        __syncthreads();
+        if (threadIdx.x < R) {
+            T tx = T(0), ta = T(0);
+#pragma unroll
+            for (int v = 0; v < NW; ++v)
+#pragma unroll
+                for (int f = 0; f < P; ++f) {
+                    tx += part[0][v][f * R + threadIdx.x];
+                    ta += part[1][v][f * R + threadIdx.x];
+                }
+            const int i = d.sell_rows[sl * 64 + rb + threadIdx.x];
+            if (i >= 0) dual_update<T, false>(d, sigma_scale, i, tx, ta);
+        }
+        return;
+    }
+    const int rr = b - d.sell_slices * P;
+    if (rr >= d.n_csr_rows) return;
+    const int i = d.csr_rows[rr];
+    T ax, asum;
+    row_dot_block<NT, T>(d.row_ptr, d.col_idx, (const T*)d.val, i, [&](int k) { return xbar[k]; }, ax, asum, red);
+    if (threadIdx.x == 0) dual_update<T, false>(d, sigma_scale, i, ax, asum);
+}
+
 // ------------------------------------------------------------- KKT partials
 // A check evaluates one or two candidates: src 0 = current iterate, src 1 =
 // window average ((FP64 sum + FP32 partial sum) / window).  Each workgroup
@@ -562,8 +628,8 @@ int check_tune(const PdlpTune* t) {
         return fail(PDLP_E_TUNE, "exact mode needs tpr_primal = tpr_dual = 1");
     if (t->scheme < PDLP_SCHEME_REFERENCE || t->scheme > PDLP_SCHEME_ADAPTIVE_START)
         return fail(PDLP_E_TUNE, "scheme must be one of the PDLP_SCHEME_* values");
-    if (t->sell < 0 || (t->sell > 1 && (t->sell < 8 || (t->sell & 7))))
-        return fail(PDLP_E_TUNE, "sell must be 0, 1 or a multiple of 8");
+    if (t->sell < 0 || (t->sell > 7 && (t->sell & 7)))
+        return fail(PDLP_E_TUNE, "sell must be 0..7 or a multiple of 8");
     if (t->exact && t->scheme != PDLP_SCHEME_REFERENCE)
         return fail(PDLP_E_TUNE, "exact mode reproduces the reference scheme only");
     return 0;
@@ -662,6 +728,20 @@ int launch_dual_t(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
     if constexpr (EXACT) {
         dual_kernel<T, 1, true><<<g, nt, 0, s>>>(d, remap, 0);
     } else {
+        if (sell && t.sell >= 2 && t.sell <= 7) {
+            // slices and CSR rows in one launch, a workgroup per item (it counts the iteration)
+            static constexpr int kRows[8] = {0, 0, 64, 64, 16, 8, 32, 16};
+            const unsigned items = (unsigned)(d.sell_slices * (64 / kRows[t.sell]) + d.n_csr_rows);
+            switch (t.sell) {
+                case 2: dual_sell_wg_kernel<T, 1024, 64><<<items, 1024, 0, s>>>(d); break;
+                case 3: dual_sell_wg_kernel<T, 512, 64><<<items, 512, 0, s>>>(d); break;
+                case 4: dual_sell_wg_kernel<T, 1024, 16><<<items, 1024, 0, s>>>(d); break;
+                case 5: dual_sell_wg_kernel<T, 1024, 8><<<items, 1024, 0, s>>>(d); break;
+                case 6: dual_sell_wg_kernel<T, 1024, 32><<<items, 1024, 0, s>>>(d); break;
+                default: dual_sell_wg_kernel<T, 512, 16><<<items, 512, 0, s>>>(d); break;
+            }
+            return hip_fail(hipGetLastError(), "dual_sell_wg_kernel launch");
+        }
         if (sell && t.sell == 1) {
             dual_sell_kernel<T><<<(unsigned)((d.sell_slices + 3) / 4), 256, 0, s>>>(d);
         } else if (sell) {

@@ -34,7 +34,9 @@ class Tune:
     exact: int = 0
     block_dual: int = 0
     scheme: int = 0             # 0 reference restart rule, 1 adaptive (see include/pdlp_hip.h)
-    sell: int = 0               # SELL-64 dual step: 0 off, 1 a wavefront per slice, k >= 8 per k entries
+    sell: int = 0               # SELL-64 dual step: 0 off, 1 a wavefront per slice, 2..7 one launch with a
+                                # workgroup per CSR row and per slice piece (threads/rows: 2 1024/64, 3 512/64,
+                                # 4 1024/16, 5 1024/8, 6 1024/32, 7 512/16), k >= 8 a wavefront per k entries
 
     def to_c(self):
         from ._lib import PdlpTune
@@ -148,7 +150,7 @@ def autotune(engine, reps: int = 20, use_cache: bool = True) -> Tune:
     sell = 0
     if getattr(engine, "sell_slices", 0) > 0:
         # the SELL slices take the coalescing rows; the CSR kernel keeps the rest
-        for sv in (1, 16, 32, 64, 128):
+        for sv in (1, 2, 3, 4, 5, 6, 7, 16, 32, 64, 128):
             for td in _neighbours(base.tpr_dual):
                 for remap in (0, 1):
                     t = Tune(best_p[0], td, 256, base.check_every, remap << 1, 0, 0, 0, sv)

@@ -224,6 +224,31 @@ def test_sell_dual_step_matches_csr_bitwise(gpu):
     assert c["status"] == "optimal" and abs(c["iterations"] - a["iterations"]) <= 500
 
 
+@pytest.mark.parametrize("mode", [2, 3, 4, 5, 6, 7])
+def test_workgroup_sell_dual_step_agrees(gpu, mode):
+    """Slices and CSR rows in one launch, a workgroup per slice (entries split among
+    its wavefronts) or per row: the same optimum as the CSR step, every run the
+    same bits, and the iteration counted once per step."""
+    from sovereign_solver_amd.engine.forms import box_form
+    from sovereign_solver_amd.engine.single import SingleEngine
+    from sovereign_solver_amd.engine.tune import Tune
+    p = transportation_lp(60, 200, seed=2)
+    f = box_form(p)
+    base = Tune(tpr_primal=1, tpr_dual=8, block=256, check_every=50)
+    a = SingleEngine(f, "fp32", tune=base).solve(1e-4, 50_000)
+    eng = SingleEngine(f, "fp32", tune=base.with_(sell=mode))
+    assert eng.sell_slices >= 1 and eng.n_csr_rows > 0
+    b = eng.solve(1e-4, 50_000)
+    c = eng.solve(1e-4, 50_000)
+    assert a["status"] == b["status"] == "optimal"
+    assert abs(a["iterations"] - b["iterations"]) <= 500
+    assert b["iterations"] == c["iterations"] and np.array_equal(b["x"], c["x"]) and np.array_equal(b["y"], c["y"])
+    assert b["ctrl"].iter == b["iterations"] or b["ctrl"].done_iter == b["iterations"]
+    assert yardstick.rel_kkt(p, b["x"], b["y"])[0] <= 1.02e-4
+    ref = _highs_obj(p)
+    assert abs(float(p.c @ b["x"]) - ref) <= 2e-3 * (1 + abs(ref))
+
+
 @pytest.mark.parametrize("kc", [8, 32])
 def test_split_sell_dual_step_agrees(gpu, kc):
     """The split SELL step (partial sums per chunk of kc entries) agrees with the
