@@ -45,6 +45,8 @@ def test_struct_sizes_match_header():
     # ... then the SELL-64 block (2 int32, 5 pointers, 2 int32, 2 pointers) and the two scale vectors
     assert C.sizeof(L.PdlpDev) == 4 * 4 + 27 * 8 + 2 * 4 + 5 * 8 + 2 * 4 + 2 * 8 + 2 * 8
     assert C.sizeof(L.PdlpTune) == 10 * 4
+    # PdlpGemm: 10 int32 (M..accumulate), alpha, split_k, 7 pointers (A, B, bias, mask, C, Cb, work)
+    assert C.sizeof(L.PdlpGemm) == 10 * 4 + 4 + 4 + 7 * 8
 
 
 def test_argument_errors_without_device():
