@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PDLP_ABI_VERSION 5
+#define PDLP_ABI_VERSION 6
 
 /* precision of iterates, matrix values and problem vectors */
 #define PDLP_F32 0
@@ -325,6 +325,11 @@ typedef struct PdlpBatch {
     PdlpBatchOut* out;
     float* x_last; float* y_last;   /* per column / row: last restart point (scratch) */
     float* y_psum;                  /* per row: sum of y since the last check (scratch) */
+    /* Diagonal scaling of the stored LPs (K = D_r K0 D_c, b, c and bounds scaled to
+       match): the checks evaluate the KKT yardstick of the original LPs, as PdlpDev's
+       row_scale / col_scale do.  NULL = unscaled. */
+    const double* row_scale;        /* per row: D_r */
+    const double* col_scale;        /* per column: D_c */
 } PdlpBatch;
 
 /* Solve every LP of the batch (one launch). */

@@ -72,6 +72,7 @@ struct LPView {
     GMEM float *y, *y_psum;
     GMEM double* y_sum;
     GMEM float *x_last, *y_last;   // last restart point (adaptive scheme)
+    const GMEM double *rs, *cs;    // row / column scales of a scaled LP (rs == nullptr: unscaled)
     double norm_b, norm_c;
     float weight;             // starting primal weight w: tau = w tau_hat, sigma = sigma_hat / w
     int lr, lc;               // log2 lanes per row / per column
@@ -113,6 +114,8 @@ __device__ __forceinline__ LPView view_of(const PdlpBatch& a, const PdlpBatchLP&
     v.y_psum = (decltype(v.y_psum))(a.y_psum + lp.row_off);
     v.x_last = (decltype(v.x_last))(a.x_last + lp.col_off);
     v.y_last = (decltype(v.y_last))(a.y_last + lp.row_off);
+    v.rs = a.row_scale != nullptr ? (decltype(v.rs))(a.row_scale + lp.row_off) : nullptr;
+    v.cs = a.col_scale != nullptr ? (decltype(v.cs))(a.col_scale + lp.col_off) : nullptr;
     v.norm_b = lp.norm_b;
     v.norm_c = lp.norm_c;
     v.weight = (float)lp.tau_scale;
@@ -204,7 +207,7 @@ __device__ __forceinline__ void block_sum_all(double (&acc)[NV], double* red) {
 // Uses X as n doubles.  acc (all threads): rp^2, rd^2, primal and dual
 // objective, and for the adaptive scheme the squared distances of the point
 // from the last restart point in the step-size metric (dx^2/tau_hat, dy^2/sigma_hat).
-template <int NT, typename T, bool XIN>
+template <int NT, typename T, bool XIN, bool SC>
 __device__ __forceinline__ void kkt_eval(const LPView& v, double* X, const T* xl, const T* yl, const int* rpl,
                                          const int* cpl, int src, double w, double (&acc)[NS], double* red) {
 #pragma unroll
@@ -251,8 +254,13 @@ __device__ __forceinline__ void kkt_eval(const LPView& v, double* X, const T* xl
                     double res = d[k] - v.b64[i];
                     if (le && res < 0.0) res = 0.0;
                     const double yi = src ? v.y_sum[i] / w : (double)yl[i];
+                    double yo = yi;               // residual and dual of the original row
+                    if constexpr (SC) {
+                        res /= v.rs[i];
+                        yo = yi * v.rs[i];
+                    }
                     acc[0] += res * res;
-                    if (le && yi > 0.0) acc[1] += yi * yi;
+                    if (le && yo > 0.0) acc[1] += yo * yo;
                     acc[3] += v.b_shift[i] * yi;
                     if (v.adaptive) {
                         const double dy = yi - (double)v.y_last[i];
@@ -301,8 +309,13 @@ __device__ __forceinline__ void kkt_eval(const LPView& v, double* X, const T* xl
                     const double r = v.c64[j] - s[k];
                     const double rneg = r < 0.0 ? r : 0.0;
                     const double span = v.ub_span[j];
-                    if (isfinite(span)) acc[3] += span * rneg;
-                    else acc[1] += rneg * rneg;
+                    if (isfinite(span)) {
+                        acc[3] += span * rneg;
+                    } else {                      // reduced cost of the original column
+                        double ro = rneg;
+                        if constexpr (SC) ro /= v.cs[j];
+                        acc[1] += ro * ro;
+                    }
                 }
             }
         }
@@ -331,7 +344,7 @@ __device__ __forceinline__ Kkt kkt_of(const double (&s)[NS], const LPView& v) {
 // One LP from start to finish.  T = float: FP32 iterates, x in global memory,
 // or with XR x, its partial sum, tau_hat and c in LDS as well; T = double:
 // FP64 iterates, x in LDS.
-template <int NT, typename T, bool XR>
+template <int NT, typename T, bool XR, bool SC>
 __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red) {
     constexpr bool RES = std::is_same<T, double>::value;
     constexpr bool XIN = RES || XR;    // x in LDS
@@ -413,7 +426,7 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
     }
 
     double acc[NS], acc2[NS];
-    kkt_eval<NT, T, XIN>(v, X, xl, yl, rpl, cpl, 0, 1.0, acc, red);
+    kkt_eval<NT, T, XIN, SC>(v, X, xl, yl, rpl, cpl, 0, 1.0, acc, red);
     Kkt kk = kkt_of(acc, v);
     double ref_err = kk.err, err_prev = INFINITY;
     int status = isfinite(kk.err) ? PDLP_RUNNING : PDLP_NUMERICAL;
@@ -554,11 +567,11 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
             __syncthreads();
         }
         const double w = (double)window;
-        kkt_eval<NT, T, XIN>(v, X, xl, yl, rpl, cpl, 1, w, acc, red);
+        kkt_eval<NT, T, XIN, SC>(v, X, xl, yl, rpl, cpl, 1, w, acc, red);
         Kkt ka = kkt_of(acc, v);
         bool use_avg = true;
         if (v.adaptive) {                       // the better of average and current iterate
-            kkt_eval<NT, T, XIN>(v, X, xl, yl, rpl, cpl, 0, 1.0, acc2, red);
+            kkt_eval<NT, T, XIN, SC>(v, X, xl, yl, rpl, cpl, 0, 1.0, acc2, red);
             const Kkt kc = kkt_of(acc2, v);
             if (!isfinite(ka.err) || kc.err < ka.err) {
                 use_avg = false;
@@ -648,7 +661,7 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
 // Persistent workgroups over one part of the work order: LPs order[first ..
 // first + count), taken through the counter *queue.  T = float: FP32 part,
 // T = double: FP64 part (each instantiation has its own register allocation).
-template <int NT, typename T, bool XR>
+template <int NT, typename T, bool XR, bool SC>
 __global__ void __launch_bounds__(NT) batch_kernel(PdlpBatch a, int first, int count, int* queue, int lds_limit) {
     extern __shared__ float lds[];
     __shared__ double red[NS * (NT / kWave)];
@@ -667,16 +680,16 @@ __global__ void __launch_bounds__(NT) batch_kernel(PdlpBatch a, int first, int c
         const int n = __builtin_amdgcn_readfirstlane(a.lps[k].n);
         // an LP that does not fit the launch's LDS is not run (its output record
         // stays as it was); the packer sizes the LDS for every LP of the part
-        if (NB * n + MB * m + 8 <= lds_limit) solve_one<NT, T, XR>(a, k, lds, red);
+        if (NB * n + MB * m + 8 <= lds_limit) solve_one<NT, T, XR, SC>(a, k, lds, red);
     }
 }
 
-template <int NT, typename T, bool XR>
+template <int NT, typename T, bool XR, bool SC>
 int launch(const PdlpBatch* a, int first, int count, int* queue, int lds, int grid, hipStream_t s) {
-    const hipError_t e = hipFuncSetAttribute((const void*)batch_kernel<NT, T, XR>,
+    const hipError_t e = hipFuncSetAttribute((const void*)batch_kernel<NT, T, XR, SC>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
-    batch_kernel<NT, T, XR><<<grid, NT, lds, s>>>(*a, first, count, queue, lds);
+    batch_kernel<NT, T, XR, SC><<<grid, NT, lds, s>>>(*a, first, count, queue, lds);
     return hip_fail(hipGetLastError(), "batch_kernel launch");
 }
 
@@ -687,8 +700,11 @@ int launch_part(const PdlpBatch* a, int first, int count, int* queue, int block,
     if (lds <= 0 || lds > 160 * 1024) return fail(PDLP_E_LIMIT, "LDS per workgroup must be in (0, 160 KiB]");
     const int g = std::min(count, grid > 0 ? grid : 2 * cus);
     switch (block) {
-        case 256: return launch<256, T, XR>(a, first, count, queue, lds, g, s);
-        case 512: return launch<512, T, XR>(a, first, count, queue, lds, g, s);
+        // scaled LPs (row_scale set) run the instantiation whose checks unscale
+        case 256: return a->row_scale ? launch<256, T, XR, true>(a, first, count, queue, lds, g, s)
+                                      : launch<256, T, XR, false>(a, first, count, queue, lds, g, s);
+        case 512: return a->row_scale ? launch<512, T, XR, true>(a, first, count, queue, lds, g, s)
+                                      : launch<512, T, XR, false>(a, first, count, queue, lds, g, s);
         default: return fail(PDLP_E_TUNE, "block must be 256 or 512");
     }
 }
@@ -709,6 +725,8 @@ extern "C" int pdlp_batch_solve(const PdlpBatch* a, void* stream) {
         !a->b || !a->row_kind || !a->b64 || !a->b_shift || !a->x || !a->x_psum || !a->x_sum || !a->tau || !a->y ||
         !a->y_sum || !a->out || !a->x_last || !a->y_last || !a->y_psum)
         return fail(PDLP_E_ARG, "a required PdlpBatch pointer is NULL");
+    if ((a->row_scale == nullptr) != (a->col_scale == nullptr))
+        return fail(PDLP_E_ARG, "row_scale and col_scale must both be set or both be NULL");
     if (a->check_every <= 0 || a->max_iters < 0) return fail(PDLP_E_ARG, "check_every > 0 and max_iters >= 0");
     hipStream_t s = as_stream(stream);
     int dev = 0, cus = 256;

@@ -13,7 +13,7 @@ import threading
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "_lib", "libpdlp_hip.so")
 
-PDLP_ABI_VERSION = 5
+PDLP_ABI_VERSION = 6
 PDLP_F32, PDLP_F64 = 0, 1
 PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL, PDLP_INFEASIBLE = 0, 1, 3, 4
 PDLP_ROW_EQ, PDLP_ROW_LE = 0, 1
@@ -82,6 +82,7 @@ class PdlpBatch(C.Structure):
         ("x", C.c_void_p), ("x_psum", C.c_void_p), ("x_sum", C.c_void_p), ("tau", C.c_void_p),
         ("y", C.c_void_p), ("y_sum", C.c_void_p), ("out", C.c_void_p),
         ("x_last", C.c_void_p), ("y_last", C.c_void_p), ("y_psum", C.c_void_p),
+        ("row_scale", C.c_void_p), ("col_scale", C.c_void_p),
     ]
 
 

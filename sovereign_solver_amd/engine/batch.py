@@ -19,6 +19,7 @@ import scipy.sparse as sp
 import torch
 
 from . import _lib as L
+from . import scaling as S
 from ._lib import PdlpBatch
 from .device import dev_tensor, ptr, require_gpu, stream_handle
 from .forms import ROW_EQ, ROW_LE
@@ -98,10 +99,19 @@ def _segment_sums(v: np.ndarray, starts: np.ndarray, lens: np.ndarray) -> np.nda
 
 
 class PackedBatch:
-    """Host-side arena of B box-form LPs (same layout as the device arena)."""
+    """Host-side arena of B box-form LPs (same layout as the device arena).
 
-    def __init__(self, problems: Sequence[Problem]):
+    ``scaling`` (a method of :mod:`.scaling`): each stored matrix, and with it
+    its LPs' b, c and bounds, is scaled (K = D_r K0 D_c); ``row_scale`` and
+    ``col_scale`` hold D_r and D_c per row and column of the arena, the
+    normalisers stay the original LPs', and the engine maps starting points
+    and solutions between the two spaces."""
+
+    def __init__(self, problems: Sequence[Problem], scaling: str = "none"):
+        if scaling not in S.METHODS:
+            raise ValueError(f"scaling must be one of {S.METHODS}, got {scaling!r}")
         self.problems = list(problems)
+        self.scaling = scaling
         B = len(self.problems)
         # LPs whose (A_ub, A_eq) are the same objects share one stored CSR
         mat_of = np.empty(B, np.int64)
@@ -143,6 +153,12 @@ class PackedBatch:
             raise ValueError("batch beyond int32 indexing of the arena")
         self.M, self.N, self.Z = M, N, Z
         self.Z_stored = int(s_z.sum())
+        # scaled copies of the stored matrices (same pattern); b_shift below uses the originals
+        if scaling != "none":
+            sc = [S.scales(K, scaling) for K in stored]
+            stored_orig, stored = stored, [S.scale_matrix(K, dr, dc) for K, (dr, dc) in zip(stored, sc)]
+        else:
+            stored_orig = stored
         self.row_ptr = np.concatenate([K.indptr.astype(np.int32) for K in stored])
         self.col_idx = np.concatenate([K.indices.astype(np.uint16) for K in stored])
         self.val = np.concatenate([K.data for K in stored])
@@ -167,7 +183,7 @@ class PackedBatch:
         lbf = np.where(np.isfinite(lo), lo, 0.0)
         b_shift = b.copy()
         for k in np.flatnonzero(_segment_sums(np.abs(lbf), col_off, ns) > 0):   # only LPs with a lower shift
-            K = stored[mat_of[k]]
+            K = stored_orig[mat_of[k]]
             b_shift[row_off[k]: row_off[k] + ms[k]] -= K @ lbf[col_off[k]: col_off[k] + ns[k]]
         fin = np.isfinite(hi)
         span2 = np.where(fin, (hi - lbf) ** 2, 0.0)
@@ -183,6 +199,13 @@ class PackedBatch:
         desc["tau_scale"], desc["sigma_scale"] = 1.0, 1.0
         lo_shift = lbf
         ub_span = np.where(fin, hi - lo_shift, np.inf)
+        self.row_scale = self.col_scale = None
+        if scaling != "none":                    # the LPs of the scaled matrices (normalisers above stay)
+            rs = np.concatenate([sc[mat_of[k]][0] for k in range(B)])
+            cs = np.concatenate([sc[mat_of[k]][1] for k in range(B)])
+            c, lo, hi, b = c * cs, lo / cs, hi / cs, b * rs
+            b_shift, lo_shift, ub_span = b_shift * rs, lo_shift / cs, ub_span / cs
+            self.row_scale, self.col_scale = rs, cs
         self.c, self.lo, self.hi, self.b, self.kind = c, lo, hi, b, kind
         self.lo_shift, self.ub_span, self.b_shift = lo_shift, ub_span, b_shift
         self.desc = desc
@@ -256,13 +279,19 @@ class BatchEngine:
                 "x_last": torch.zeros(pk.N, dtype=f32, device=dv), "y_last": torch.zeros(pk.M, dtype=f32, device=dv),
                 "y_psum": torch.zeros(pk.M, dtype=f32, device=dv),
             }
+            if pk.row_scale is not None:
+                self.t["row_scale"] = dev_tensor(pk.row_scale, f64, dv)
+                self.t["col_scale"] = dev_tensor(pk.col_scale, f64, dv)
         self.stream.synchronize()
 
     def set_warm_start(self, x0: Optional[np.ndarray] = None, y0: Optional[np.ndarray] = None,
                        step_scale: Optional[np.ndarray] = None) -> None:
-        """Packed starting point (length N / M arrays; None = cold start) and per-LP
-        starting primal weights (None = 1 for the reference scheme, the norm-ratio
-        estimate for the adaptive one)."""
+        """Packed starting point (length N / M arrays in the LPs' own space; None =
+        cold start) and per-LP starting primal weights (None = 1 for the
+        reference scheme, the norm-ratio estimate for the adaptive one)."""
+        if self.pk.col_scale is not None:        # into the scaled LPs' space
+            x0 = None if x0 is None else np.asarray(x0, float) / self.pk.col_scale
+            y0 = None if y0 is None else np.asarray(y0, float) / self.pk.row_scale
         with torch.cuda.stream(self.stream):
             if x0 is None:          # cold start: the origin projected onto the box (x = lb when lb >= 0)
                 self.t["x"].copy_(torch.from_numpy(np.clip(0.0, self.pk.lo, self.pk.hi).astype(np.float32))
@@ -295,6 +324,8 @@ class BatchEngine:
                      "ub_span", "b", "row_kind", "b64", "b_shift", "x", "x_psum", "x_sum", "tau", "y", "y_sum",
                      "out", "x_last", "y_last", "y_psum"):
             setattr(a, name, ptr(t[name]))
+        if "row_scale" in t:
+            a.row_scale, a.col_scale = ptr(t["row_scale"]), ptr(t["col_scale"])
         return a
 
     def launch(self, tol: float, max_iters: int, check_every: int = 50, parts: int = 0) -> None:
@@ -314,6 +345,8 @@ class BatchEngine:
         out = np.frombuffer(self.t["out"].cpu().numpy().tobytes(), dtype=OUT_DTYPE)
         xs = self.t["x_sum"].cpu().numpy()
         ys = self.t["y_sum"].cpu().numpy()
+        if self.pk.col_scale is not None:        # back to the LPs' own space
+            xs, ys = xs * self.pk.col_scale, ys * self.pk.row_scale
         return out, xs, ys
 
     def solve(self, tol: float, max_iters: int, check_every: int = 50, x0=None, y0=None, step_scale=None):

@@ -16,6 +16,11 @@ matrix and step in lockstep, 64 LPs per wavefront; no size limit) or
 objects and there are at least 16 of them, or when an LP is too large for
 LDS; else persistent).
 
+``scaling`` (persistent engine): diagonal preconditioning of every LP
+(``"ruiz"``, ``"geometric"``, ``"pock_chambolle"``, ``"ruiz_pc"``; see
+:func:`~sovereign_solver_amd.lp.pdhg.solve_lp_pdhg`); the tolerance applies to
+the original LPs' KKT and the results come back unscaled.
+
 ``warm_start`` is an optional list (one entry per problem, ``None`` allowed)
 of ``(x0, y0)`` or ``(x0, y0, step_scale)``: x0 in the original variable
 space, y0 the duals of ``[A_ub; A_eq]`` (y ≤ 0 on the ``<=`` rows), and a
@@ -103,9 +108,12 @@ def solve_lp_pdhg_batch(
     block: int = 0,
     scheme: str = "adaptive",
     engine: str = "auto",
+    scaling: str = "none",
 ) -> list:
     if engine not in ("auto", "persistent", "family"):
         raise ValueError("engine must be 'auto', 'persistent' or 'family'")
+    if scaling != "none" and engine == "family":
+        raise ValueError("scaling is supported by the persistent engine (engine='persistent' or 'auto')")
     problems = list(problems)
     if not problems:
         return []
@@ -118,7 +126,7 @@ def solve_lp_pdhg_batch(
         ps = [problems[k] for k in live]
         same = all(p.A_ub is ps[0].A_ub and p.A_eq is ps[0].A_eq for p in ps)
         too_big = max(lds_bytes_of(p.n_constraints, p.n_vars) for p in ps) > 160 * 1024
-        engine = "family" if (same and len(ps) >= 16) or (too_big and same) else "persistent"
+        engine = "family" if scaling == "none" and ((same and len(ps) >= 16) or (too_big and same)) else "persistent"
     if live and engine == "family":
         fb = FamilyBatch([problems[k] for k in live])
         eng = FamilyEngine(fb, scheme=scheme, check_every=check_every)
@@ -130,7 +138,7 @@ def solve_lp_pdhg_batch(
         for k, r in zip(live, results_of(eng, o, xs, ys)):
             out[k] = r
     elif live:
-        pk = PackedBatch([problems[k] for k in live])
+        pk = PackedBatch([problems[k] for k in live], scaling=scaling)
         eng = BatchEngine(pk, block=block, scheme=scheme)
         ws = None if warm_start is None else [warm_start[k] for k in live]
         x0, y0, w = pack_warm_start(pk, ws)

@@ -1,6 +1,8 @@
 """Host-side packing of the batched engine (no GPU)."""
 
 import numpy as np
+import pytest
+import scipy.sparse as sp
 
 from sovereign_solver_amd.engine.batch import (LP_DTYPE, OUT_DTYPE, PackedBatch, lanes_per_row, lds_bytes_of,
                                                lds_bytes_resident, lds_bytes_x_in_lds)
@@ -103,3 +105,39 @@ def test_warm_start_packing_clips_and_projects():
     assert np.all(x0[6:] == 0.0)                        # clipped to lb = 0
     assert np.all(y0[4:6] == 0.0) and np.all(y0[6:] == 1.0)   # <= rows projected to y <= 0
     assert w[0] == 2.0 and np.isnan(w[1])                # NaN = the engine default
+
+
+def test_scaled_batch_is_the_same_lps():
+    """A scaled arena holds D_r K D_c and the matching b, c and bounds; the
+    normalisers are the original LPs'; the transposes are of the scaled matrices."""
+    from sovereign_solver_amd.engine import scaling as S
+    probs = [random_sparse_lp(30 + 7 * k, 80 + 11 * k, 0.08, seed=k, family=2) for k in range(5)]
+    plain = PackedBatch(probs)
+    sc = PackedBatch(probs, scaling="ruiz_pc")
+    np.testing.assert_array_equal(plain.desc["norm_b"], sc.desc["norm_b"])
+    np.testing.assert_array_equal(plain.desc["norm_c"], sc.desc["norm_c"])
+    for k, p in enumerate(probs):
+        r0, c0 = sc.row_off[k], sc.col_off[k]
+        m, n = sc.ms[k], sc.ns[k]
+        dr, dc = sc.row_scale[r0: r0 + m], sc.col_scale[c0: c0 + n]
+        K0 = sp.vstack([p.A_ub, p.A_eq]).tocsr()
+        Dr, Dc = S.scales(K0, "ruiz_pc")
+        np.testing.assert_allclose(dr, Dr)
+        np.testing.assert_allclose(dc, Dc)
+        z0, rp0 = sc.nnz_off[k], int(sc.desc["rp_off"][k])
+        K = sp.csr_matrix((sc.val[z0: z0 + K0.nnz], sc.col_idx[z0: z0 + K0.nnz].astype(np.int64),
+                           sc.row_ptr[rp0: rp0 + m + 1]), shape=(m, n))
+        np.testing.assert_allclose(K.toarray(), (sp.diags(Dr) @ K0 @ sp.diags(Dc)).toarray(), rtol=1e-12)
+        cp0 = sc.cp_off[k]
+        KT = sp.csr_matrix((sc.t_val[z0: z0 + K0.nnz], sc.t_row_idx[z0: z0 + K0.nnz].astype(np.int64),
+                            sc.t_col_ptr[cp0: cp0 + n + 1]), shape=(n, m))
+        np.testing.assert_allclose(KT.toarray(), K.toarray().T, rtol=1e-12)
+        np.testing.assert_allclose(sc.c[c0: c0 + n], p.c * Dc)
+        np.testing.assert_allclose(sc.lo[c0: c0 + n], p.lb / Dc)
+        np.testing.assert_allclose(sc.hi[c0: c0 + n], p.ub / Dc)
+        b = np.concatenate([p.b_ub, p.b_eq])
+        np.testing.assert_allclose(sc.b[r0: r0 + m], b * Dr)
+        np.testing.assert_allclose(sc.b_shift[r0: r0 + m], plain.b_shift[r0: r0 + m] * Dr)
+    assert plain.row_scale is None and plain.col_scale is None
+    with pytest.raises(ValueError):
+        PackedBatch(probs, scaling="bogus")

@@ -149,3 +149,29 @@ def test_batch_fp32_part_layouts_agree_bitwise(gpu):
     o2, x2, y2 = BatchEngine(pk).solve(1e-4, 50_000)
     assert np.all(o1["status"] == 1)
     assert np.array_equal(x1, x2) and np.array_equal(y1, y2) and np.array_equal(o1, o2)
+
+
+@pytest.mark.parametrize("method", ["ruiz", "ruiz_pc"])
+def test_scaled_batch(gpu, method):
+    """Scaled persistent batch: every LP stops at the requested KKT of the
+    original LP (host yardstick on the unscaled x, y), and three small LPs of
+    the uneven batch that the unscaled scheme needs 6,000-15,000 iterations for
+    finish in a fraction of that.  Those three are ill-conditioned (86 of the
+    170 columns of m = 68 empty, finite bounds): a relative KKT of 7e-5 (primal
+    residual 0.006 against a normaliser of 90) leaves the objective 4% from
+    HiGHS's, so only the other LPs are compared with HiGHS."""
+    slow = [(94, 3721), (68, 3579), (68, 1498)]               # (m, index in the 4,096-LP uneven batch)
+    probs = [random_sparse_lp(m, round(2.5 * m), 0.01, seed=s, family=2) for m, s in slow]
+    probs += [random_sparse_lp(m, round(2.5 * m), 0.01, seed=100 + k, family=2)
+              for k, m in enumerate((60, 150, 300, 500, 800))]
+    plain = solve_lp_pdhg_batch(probs, tol=1e-4, max_iters=100_000, engine="persistent")
+    scaled = solve_lp_pdhg_batch(probs, tol=1e-4, max_iters=100_000, engine="persistent", scaling=method)
+    for k, (p, r) in enumerate(zip(probs, scaled)):
+        assert r.status == "optimal", (p.name, r.status, r.iterations)
+        err = yardstick.rel_kkt(p, r.x, r.y)[0]
+        assert err <= 1.05e-4, (p.name, err, r.rel_kkt)
+        assert abs(err - r.rel_kkt) <= 1e-3 * err + 1e-9     # the device measured the original LP too
+        if k >= len(slow):
+            ref = _highs(p)
+            assert abs(r.objective - ref) <= 1e-3 * (1 + abs(ref)), (p.name, r.objective, ref)
+    assert sum(r.iterations for r in scaled[:3]) < 0.3 * sum(r.iterations for r in plain[:3])
