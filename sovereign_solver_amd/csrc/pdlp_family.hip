@@ -71,7 +71,19 @@ __device__ __forceinline__ float gather_dot(const int32_t* __restrict__ idx, con
 #pragma unroll
         for (int u = 0; u < 8; ++u) d += w[u] * g[u];
     }
-    for (; q < end; ++q) d += val[q] * v[at(tile, len, idx[q], lane)];
+    // the rest (fewer than 8): loads issued together, added in the same order
+    if (q < end) {
+        float g[8], w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const bool ok = q + u < end;
+            w[u] = ok ? val[q + u] : 0.f;
+            g[u] = ok ? v[at(tile, len, idx[q + u], lane)] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (q + u < end) d += w[u] * g[u];
+    }
     return d;
 }
 
@@ -154,10 +166,24 @@ __global__ void __launch_bounds__(NT) primal_kernel(PdlpFamily f, int P) {
     const float wgt = act ? f.lp[k].weight : 1.f;
     const int p0 = (pl.chunk * NWV + w) * f.items_per_wave;
     const int p1 = min(p0 + f.items_per_wave, f.n - f.long_cols);
+    const bool shc = f.shared & PDLP_FAM_SHARED_C, shl = f.shared & PDLP_FAM_SHARED_LO,
+               shh = f.shared & PDLP_FAM_SHARED_HI;
     for (int p = p0; p < p1; ++p) {
         const int j = p;
+        // the column's own data first, so its loads are in flight with the gather's
+        const size_t ix = at(tile, f.n, j, lane);
+        const float xo = f.x[ix], th = f.tau_hat[j], cj = ldv(f.c, shc, tile, f.n, j, lane);
+        const float lo = ldv(f.lo, shl, tile, f.n, j, lane), hi = ldv(f.hi, shh, tile, f.n, j, lane);
+        const float ps = f.x_psum[ix];
         const float aty = gather_dot(f.t_col_idx, f.t_val, f.t_row_ptr[j], f.t_row_ptr[j + 1], f.y, tile, f.m, lane);
-        column_update(f, tile, lane, act, wgt, j, aty);
+        float xn = xo - (wgt * th) * (cj - aty);
+        xn = xn < lo ? lo : xn;
+        xn = xn > hi ? hi : xn;
+        if (act) {
+            f.x[ix] = xn;
+            f.xbar[ix] = 2.f * xn - xo;
+            f.x_psum[ix] = ps + xn;
+        }
     }
 }
 
@@ -174,10 +200,21 @@ __global__ void __launch_bounds__(NT) dual_kernel(PdlpFamily f, int P) {
     const float inv_wgt = act ? 1.f / f.lp[k].weight : 1.f;
     const int p0 = (pl.chunk * NWV + w) * f.items_per_wave;
     const int p1 = min(p0 + f.items_per_wave, f.m - f.long_rows);
+    const bool shb = f.shared & PDLP_FAM_SHARED_B;
     for (int p = p0; p < p1; ++p) {
         const int i = p;
+        // the row's own data first, so its loads are in flight with the gather's
+        const size_t ix = at(tile, f.m, i, lane);
+        const float yo = f.y[ix], sh = f.sigma_hat[i], bi = ldv(f.b, shb, tile, f.m, i, lane);
+        const bool le = f.row_kind[i] == PDLP_ROW_LE;
+        const float ps = f.y_psum[ix];
         const float d = gather_dot(f.col_idx, f.val, f.row_ptr[i], f.row_ptr[i + 1], f.xbar, tile, f.n, lane);
-        row_update(f, tile, lane, act, inv_wgt, i, d);
+        float yn = yo + (inv_wgt * sh) * (bi - d);
+        if (le && yn > 0.f) yn = 0.f;
+        if (act) {
+            f.y[ix] = yn;
+            f.y_psum[ix] = ps + yn;
+        }
     }
 }
 
