@@ -65,28 +65,36 @@ __global__ void __launch_bounds__(1024) primal_kernel(PdlpDev d, int remap) {
     int j, lane;
     group_of<TPR>(remap, j, lane);
     if (j >= d.n) return;                               // whole groups leave together
+    const T* c = (const T*)d.c;
+    const T* lo = (const T*)d.lo;
+    const T* hi = (const T*)d.hi;
+    T* x = (T*)d.x;
+    T* xbar = (T*)d.xbar;
+    // the column's own data first (group lane 0), so its loads are in flight with the dot product's
+    const bool own = lane == 0;
+    const T xj = own ? x[j] : T(0), cj = own ? c[j] : T(0);
+    const T lj = (own && lo != nullptr) ? lo[j] : T(0), hj = (own && hi != nullptr) ? hi[j] : T(0);
+    const float psf = (own && d.x_psum != nullptr) ? d.x_psum[j] : 0.f;
+    const double psd = (own && d.x_psum == nullptr) ? d.x_sum[j] : 0.0;
     T aty, asum;
     any_row_dot<TPR, EXACT, T>(d.t_row_ptr, d.t_col_idx, (const T*)d.t_val, j, lane,
                                [&](int k) { return y[k]; }, aty, asum);
     if (lane != 0) return;
-    const T* c = (const T*)d.c;
-    T* x = (T*)d.x;
-    T* xbar = (T*)d.xbar;
     const T colsum = asum > T(1e-4) ? asum : T(1e-4);
     T tau = T(0.999) / colsum;
     if (tau_scale != T(1)) tau *= tau_scale;
-    const T xj = x[j];
     T xn;
-    if constexpr (EXACT) {
-        xn = clamp_box(add_rn(xj, -mul_rn(tau, add_rn(c[j], -aty))), (const T*)d.lo, (const T*)d.hi, j);
-        xbar[j] = add_rn(mul_rn(T(2), xn), -xj);
-    } else {
-        xn = clamp_box(xj - tau * (c[j] - aty), (const T*)d.lo, (const T*)d.hi, j);
-        xbar[j] = T(2) * xn - xj;
-    }
+    if constexpr (EXACT) xn = add_rn(xj, -mul_rn(tau, add_rn(cj, -aty)));
+    else xn = xj - tau * (cj - aty);
+    // the box projection of clamp_box with the preloaded bounds
+    if (lo != nullptr) xn = xn < lj ? lj : xn;
+    else xn = xn < T(0) ? T(0) : xn;
+    if (hi != nullptr) xn = xn > hj ? hj : xn;
+    if constexpr (EXACT) xbar[j] = add_rn(mul_rn(T(2), xn), -xj);
+    else xbar[j] = T(2) * xn - xj;
     x[j] = xn;
-    if (d.x_psum != nullptr) d.x_psum[j] += (float)xn;
-    else d.x_sum[j] += (double)xn;
+    if (d.x_psum != nullptr) d.x_psum[j] = psf + (float)xn;
+    else d.x_sum[j] = psd + (double)xn;
 }
 
 // ------------------------------------------------------------------ dual step
@@ -127,11 +135,26 @@ __global__ void __launch_bounds__(1024) dual_kernel(PdlpDev d, int remap, int us
     } else if (i >= d.m) {
         return;
     }
+    // the row's own data first (group lane 0), so its loads are in flight with the dot product's
+    const bool own = lane == 0;
+    const T yi = own ? ((const T*)d.y)[i] : T(0), bi = own ? ((const T*)d.b)[i] : T(0);
+    const bool le = own && d.row_kind != nullptr && d.row_kind[i] == PDLP_ROW_LE;
+    const float psf = (own && d.y_psum != nullptr) ? d.y_psum[i] : 0.f;
+    const double psd = (own && d.y_psum == nullptr) ? d.y_sum[i] : 0.0;
     T ax, asum;
     any_row_dot<TPR, EXACT, T>(d.row_ptr, d.col_idx, (const T*)d.val, i, lane,
                                [&](int k) { return xbar[k]; }, ax, asum);
     if (lane != 0) return;
-    dual_update<T, EXACT>(d, sigma_scale, i, ax, asum);
+    const T rowsum = asum > T(1e-4) ? asum : T(1e-4);
+    T sigma = T(0.999) / rowsum;
+    if (sigma_scale != T(1)) sigma *= sigma_scale;
+    T yn;
+    if constexpr (EXACT) yn = add_rn(yi, mul_rn(sigma, add_rn(bi, -ax)));
+    else yn = yi + sigma * (bi - ax);
+    if (le && yn > T(0)) yn = T(0);
+    ((T*)d.y)[i] = yn;
+    if (d.y_psum != nullptr) d.y_psum[i] = psf + (float)yn;
+    else d.y_sum[i] = psd + (double)yn;
 }
 
 // SELL-64 dual step: one wavefront per slice, one row per lane; entry k of
