@@ -115,6 +115,44 @@ __device__ __forceinline__ void dual_update(const PdlpDev& d, T sigma_scale, int
     else d.y_sum[i] += (double)yn;
 }
 
+// A row's own data for its dual update, loaded ahead of the row's dot
+// product so that the loads overlap (only the thread that applies the update
+// loads: own).
+template <typename T>
+struct RowOwn {
+    T y, b;
+    bool le;
+    float psf;
+    double psd;
+};
+
+template <typename T>
+__device__ __forceinline__ RowOwn<T> row_own(const PdlpDev& d, int i, bool own) {
+    RowOwn<T> o;
+    o.y = own ? ((const T*)d.y)[i] : T(0);
+    o.b = own ? ((const T*)d.b)[i] : T(0);
+    o.le = own && d.row_kind != nullptr && d.row_kind[i] == PDLP_ROW_LE;
+    o.psf = (own && d.y_psum != nullptr) ? d.y_psum[i] : 0.f;
+    o.psd = (own && d.y_psum == nullptr) ? d.y_sum[i] : 0.0;
+    return o;
+}
+
+// dual_update with the row's own data already loaded (same arithmetic)
+template <typename T, bool EXACT>
+__device__ __forceinline__ void dual_update_own(const PdlpDev& d, T sigma_scale, int i, T ax, T asum,
+                                                const RowOwn<T>& o) {
+    const T rowsum = asum > T(1e-4) ? asum : T(1e-4);
+    T sigma = T(0.999) / rowsum;
+    if (sigma_scale != T(1)) sigma *= sigma_scale;
+    T yn;
+    if constexpr (EXACT) yn = add_rn(o.y, mul_rn(sigma, add_rn(o.b, -ax)));
+    else yn = o.y + sigma * (o.b - ax);
+    if (o.le && yn > T(0)) yn = T(0);
+    ((T*)d.y)[i] = yn;
+    if (d.y_psum != nullptr) d.y_psum[i] = o.psf + (float)yn;
+    else d.y_sum[i] = o.psd + (double)yn;
+}
+
 // CSR dual step over all rows (use_list = 0) or over csr_rows (the rows the
 // SELL slices leave).  Block 0 counts the iteration.
 template <typename T, int TPR, bool EXACT>
@@ -136,25 +174,12 @@ __global__ void __launch_bounds__(1024) dual_kernel(PdlpDev d, int remap, int us
         return;
     }
     // the row's own data first (group lane 0), so its loads are in flight with the dot product's
-    const bool own = lane == 0;
-    const T yi = own ? ((const T*)d.y)[i] : T(0), bi = own ? ((const T*)d.b)[i] : T(0);
-    const bool le = own && d.row_kind != nullptr && d.row_kind[i] == PDLP_ROW_LE;
-    const float psf = (own && d.y_psum != nullptr) ? d.y_psum[i] : 0.f;
-    const double psd = (own && d.y_psum == nullptr) ? d.y_sum[i] : 0.0;
+    const RowOwn<T> o = row_own<T>(d, i, lane == 0);
     T ax, asum;
     any_row_dot<TPR, EXACT, T>(d.row_ptr, d.col_idx, (const T*)d.val, i, lane,
                                [&](int k) { return xbar[k]; }, ax, asum);
     if (lane != 0) return;
-    const T rowsum = asum > T(1e-4) ? asum : T(1e-4);
-    T sigma = T(0.999) / rowsum;
-    if (sigma_scale != T(1)) sigma *= sigma_scale;
-    T yn;
-    if constexpr (EXACT) yn = add_rn(yi, mul_rn(sigma, add_rn(bi, -ax)));
-    else yn = yi + sigma * (bi - ax);
-    if (le && yn > T(0)) yn = T(0);
-    ((T*)d.y)[i] = yn;
-    if (d.y_psum != nullptr) d.y_psum[i] = psf + (float)yn;
-    else d.y_sum[i] = psd + (double)yn;
+    dual_update_own<T, EXACT>(d, sigma_scale, i, ax, asum, o);
 }
 
 // SELL-64 dual step: one wavefront per slice, one row per lane; entry k of
@@ -217,6 +242,9 @@ __global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d) {
         const T* __restrict__ val = (const T*)d.sell_val;
         const int32_t* __restrict__ col = d.sell_col;
         const int beg = d.sell_ptr[sl], width = (d.sell_ptr[sl + 1] - beg) >> 6;
+        // the rows' own data, by the threads that apply the updates
+        const int ir = threadIdx.x < R ? d.sell_rows[sl * 64 + rb + threadIdx.x] : -1;
+        const RowOwn<T> o = row_own<T>(d, ir, ir >= 0);
         const int kc = (width + NW - 1) / NW;
         const int k0 = min(width, w * kc), k1 = min(width, k0 + kc);
         T ax = T(0), asum = T(0);
@@ -239,17 +267,17 @@ __global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d) {
                     tx += part[0][v][f * R + threadIdx.x];
                     ta += part[1][v][f * R + threadIdx.x];
                 }
-            const int i = d.sell_rows[sl * 64 + rb + threadIdx.x];
-            if (i >= 0) dual_update<T, false>(d, sigma_scale, i, tx, ta);
+            if (ir >= 0) dual_update_own<T, false>(d, sigma_scale, ir, tx, ta, o);
         }
         return;
     }
     const int rr = b - d.sell_slices * P;
     if (rr >= d.n_csr_rows) return;
     const int i = d.csr_rows[rr];
+    const RowOwn<T> o = row_own<T>(d, i, threadIdx.x == 0);
     T ax, asum;
     row_dot_block<NT, T>(d.row_ptr, d.col_idx, (const T*)d.val, i, [&](int k) { return xbar[k]; }, ax, asum, red);
-    if (threadIdx.x == 0) dual_update<T, false>(d, sigma_scale, i, ax, asum);
+    if (threadIdx.x == 0) dual_update_own<T, false>(d, sigma_scale, i, ax, asum, o);
 }
 
 // ------------------------------------------------------------- KKT partials
