@@ -61,6 +61,17 @@ __device__ __forceinline__ float gather_dot(const int32_t* __restrict__ idx, con
                                             int end, const float* __restrict__ v, int tile, int len, int lane) {
     float d = 0.f;
     int q = beg;
+    // 16 entries' loads in flight at a time, then 8, then the rest (same summation order)
+    for (; q + 16 <= end; q += 16) {
+        float g[16], w[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            w[u] = val[q + u];
+            g[u] = v[at(tile, len, idx[q + u], lane)];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) d += w[u] * g[u];
+    }
     for (; q + 8 <= end; q += 8) {
         float g[8], w[8];
 #pragma unroll
