@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PDLP_ABI_VERSION 6
+#define PDLP_ABI_VERSION 7
 
 /* precision of iterates, matrix values and problem vectors */
 #define PDLP_F32 0
@@ -47,6 +47,11 @@ extern "C" {
                                 engine): the dual iterate is a Farkas certificate,
                                 b'y - max_{lo<=x<=hi} (K'y)'x > 0 with y <= 0 on
                                 PDLP_ROW_LE rows */
+#define PDLP_UNBOUNDED  5    /* dual infeasible (single-instance box form, adaptive
+                                schemes): d = x - x_sum / apply_window is a ray with
+                                c'd < 0 whose row and bound violations pass the
+                                engine's relative test; the LP is unbounded when it
+                                is feasible */
 
 /* row kinds of the box form */
 #define PDLP_ROW_EQ 0        /* K_i x  = b_i, dual free     */
@@ -67,9 +72,9 @@ extern "C" {
                                          with (x0, y0) the starting point */
 
 /* Number of doubles of PdlpDev.partials the engine needs:
- * 2 candidates x 2 passes x PDLP_PARTIAL_BLOCKS blocks x 10 sums. */
+ * 2 candidates x 2 passes x PDLP_PARTIAL_BLOCKS blocks x 16 sums. */
 #define PDLP_PARTIAL_BLOCKS 1024
-#define PDLP_PARTIALS_LEN   (4 * PDLP_PARTIAL_BLOCKS * 10)
+#define PDLP_PARTIALS_LEN   (4 * PDLP_PARTIAL_BLOCKS * 16)
 
 /*
  * Device control block, one per instance, device-resident.  The engine
@@ -80,7 +85,8 @@ extern "C" {
 typedef struct PdlpCtrl {
     int32_t iter;            /* iterations executed                          */
     int32_t window;          /* iterations in the current averaging window   */
-    int32_t status;          /* PDLP_RUNNING / PDLP_OPTIMAL / PDLP_NUMERICAL / PDLP_INFEASIBLE */
+    int32_t status;          /* PDLP_RUNNING / PDLP_OPTIMAL / PDLP_NUMERICAL / PDLP_INFEASIBLE /
+                                PDLP_UNBOUNDED */
     int32_t restarts;
     int32_t done_iter;       /* iteration at which status left RUNNING       */
     int32_t restart_pending; /* set by the check, consumed by the apply step */

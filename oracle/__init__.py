@@ -22,8 +22,10 @@ Contents and what pins each part:
 * ``mlp_ref``   — FP64 NumPy MLP forward/backward.  Pinned by central finite
   differences.
 * ``farkas``    — the Farkas value of a dual vector (> 0 proves primal
-  infeasibility).  Pinned by a hand-made certificate and by F <= 0 for random
-  duals of feasible LPs.
+  infeasibility), and the violations of a ray of decrease (dual
+  infeasibility).  Pinned by hand-made certificates and rays, by F <= 0 for
+  random duals of feasible LPs, and by the bound c'd >= -|y||viol(Kd)| -
+  |r||viol_bounds(d)| for HiGHS's duals of bounded LPs.
 * ``generators`` are not here: the workload generators are product code
   (``sovereign_solver_amd.workloads``) because users build the same families.
 """

@@ -290,9 +290,14 @@ __global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d) {
 // sum dy_i^2 / sigma_hat_i), used by the adaptive primal weight; then the
 // Farkas test of the candidate's y (box form): 6 b_shift'y, 7 the maximum of
 // (K'y)'(x - lo_shift) over the shifted box, 8 sum |b_shift_i y_i| and 9 the
-// sum of the |terms| of 7.
+// sum of the |terms| of 7.  Then the unboundedness test of the ray estimate
+// d = x - (window average), in the average's launches when `ray` is set
+// (zero otherwise), all in the original LP's units: 10 the squared violations
+// of K d against the rows' recession cone (= 0, <= 0), 11 the squared row sums
+// of |K_ij d_j|, 12 c'd, 13 |y|^2 of the average, 14 the squared violations of
+// d against the bounds' recession cone, 15 |c - K'y|^2 of the average.
 
-constexpr int NP = 10;
+constexpr int NP = 16;
 
 __device__ __forceinline__ double* part_ptr(const PdlpDev& d, int src, int part, int blk) {
     return d.partials + ((size_t)(src * 2 + part) * PDLP_PARTIAL_BLOCKS + blk) * NP;
@@ -311,8 +316,27 @@ __device__ __forceinline__ double farkas_support(double g, double span, bool lo_
     return 0.0;
 }
 
+// (K d)_i and sum_j |K_ij d_j| of one row, by TPR lanes (result in every lane).
+template <int TPR, typename V, typename G>
+__device__ __forceinline__ void row_ray(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                        const V* __restrict__ val, int row, int lane, G dir, double& kd,
+                                        double& kabs) {
+    double s = 0.0, a = 0.0;
+    for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
+        const double t = (double)val[k] * dir(ci[k]);
+        s += t;
+        a += fabs(t);
+    }
+    if constexpr (TPR > 1) {
+        s = group_sum<TPR>(s);
+        a = group_sum<TPR>(a);
+    }
+    kd = s;
+    kabs = a;
+}
+
 template <typename T, int TPR>
-__global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src) {
+__global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src, int ray) {
     __shared__ double lds[16 * NP];
     const PdlpCtrl* ctrl = d.ctrl;
     if (ctrl->status != PDLP_RUNNING) return;
@@ -329,6 +353,9 @@ __global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src) {
                                              [&](int k) { return avg_of(d.x_sum, d.x_psum, k, w); }, ax, asum);
         else row_dot<TPR, false, double>(d.row_ptr, d.col_idx, v, i, lane,
                                          [&](int k) { return (double)x[k]; }, ax, asum);
+        double kd = 0.0, kabs = 0.0;
+        if (ray) row_ray<TPR>(d.row_ptr, d.col_idx, v, i, lane,
+                              [&](int k) { return (double)x[k] - avg_of(d.x_sum, d.x_psum, k, w); }, kd, kabs);
         if (lane == 0) {
             const bool le = d.row_kind != nullptr && d.row_kind[i] == PDLP_ROW_LE;
             double res = ax - d.b64[i];
@@ -343,6 +370,13 @@ __global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src) {
                 acc[1] += yi * yi;                      // slack column reduced cost -y_i
             }
             acc[0] += res * res;
+            if (ray) {
+                const double dr = d.row_scale != nullptr ? d.row_scale[i] : 1.0;
+                const double viol = (le && kd < 0.0 ? 0.0 : kd) / dr, ka = kabs / dr, yo = yi * dr;
+                acc[10] += viol * viol;
+                acc[11] += ka * ka;
+                acc[13] += yo * yo;
+            }
             const double by = d.b_shift[i] * yi;
             acc[3] += by;
             acc[6] += by;
@@ -361,7 +395,7 @@ __global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src) {
 }
 
 template <typename T, int TPR>
-__global__ void __launch_bounds__(256) kkt_cols_kernel(PdlpDev d, int src) {
+__global__ void __launch_bounds__(256) kkt_cols_kernel(PdlpDev d, int src, int ray) {
     __shared__ double lds[16 * NP];
     const PdlpCtrl* ctrl = d.ctrl;
     if (ctrl->status != PDLP_RUNNING) return;
@@ -396,6 +430,16 @@ __global__ void __launch_bounds__(256) kkt_cols_kernel(PdlpDev d, int src) {
             const double xj = src ? avg_of(d.x_sum, d.x_psum, j, w) : (double)x[j];
             const double sh = d.lo_shift != nullptr ? d.lo_shift[j] : 0.0;
             acc[2] += d.c64[j] * (xj - sh);
+            if (ray) {
+                const double dj = (double)x[j] - xj, dc = d.col_scale != nullptr ? d.col_scale[j] : 1.0;
+                const double dor = dj * dc, ro = r / dc;
+                double bv = 0.0;                         // d must keep finite bounds' sides
+                if (!lo_inf && dor < 0.0) bv = dor;
+                if (isfinite(span) && dor > 0.0) bv = dor;
+                acc[12] += d.c64[j] * dj;
+                acc[14] += bv * bv;
+                acc[15] += ro * ro;
+            }
             if (xlast != nullptr) {
                 const double dx = xj - (double)xlast[j];
                 acc[4] += dx * dx * (asum > 1e-4 ? asum : 1e-4) / 0.999;
@@ -412,6 +456,7 @@ __global__ void __launch_bounds__(256) kkt_cols_kernel(PdlpDev d, int src) {
 struct KktVal {
     double rp, rd, gap, err, po, dobj, dx, dy;
     bool infeasible;     // the candidate's y is a Farkas certificate of primal infeasibility
+    bool unbounded;      // the ray estimate d passes the dual infeasibility test
 };
 
 __device__ KktVal kkt_value(const double (&s)[NP], const PdlpCtrl* ctrl) {
@@ -431,6 +476,17 @@ __device__ KktVal kkt_value(const double (&s)[NP], const PdlpCtrl* ctrl) {
     // rounding of the sums, so a feasible LP is never flagged.
     const double F = s[6] - s[7], scale = s[8] + s[9];
     k.infeasible = isfinite(F) && F > 0.0 && F > 1e-6 * scale;
+    // d is a ray of decrease (dual infeasibility) when c'd < 0 and its violations are
+    // negligible against any dual solution: for dual-feasible (y, r = c - K'y),
+    // c'd >= -|y| |viol(K d)| - |r| |viol_bounds(d)|, so a bounded LP fails
+    // |viol(Kd)| (1 + |y|) + |viol_bounds(d)| (1 + |r|) <= M^-1 (-c'd) while the
+    // candidate's (y, r) is within a factor M of a dual solution in norm.  M = 1e3 when
+    // the rows' violation is also small against their terms, |viol(K d)| <= 1e-3 |(|K||d|)|
+    // (K d cancels, as along a ray and unlike the early iterations' moves), else 1e6: a
+    // ray on columns that no row holds has no terms to cancel.
+    const double cd = s[12], vr = sqrt(s[10]), vb = sqrt(s[14]);
+    const double lhs = vr * (1.0 + sqrt(s[13])) + vb * (1.0 + sqrt(s[15]));
+    k.unbounded = isfinite(cd) && cd < 0.0 && lhs <= (vr <= 1e-3 * sqrt(s[11]) ? 1e-3 : 1e-6) * -cd;
     return k;
 }
 
@@ -483,6 +539,17 @@ __global__ void __launch_bounds__(256) kkt_finalize_kernel(PdlpDev d, int nb_row
         if (mode != 0) ctrl->checks += 1;
         ctrl->cand = inf0 ? 0 : 1;
         ctrl->restart_pending = inf0 ? 0 : 1;   // finish at the average when it holds the certificate
+        ctrl->apply_window = ctrl->window;
+        return;
+    }
+    // the same for a ray of decrease: finish at the current iterate with the window
+    // sums kept (apply folds them), so the host forms d = x - x_sum / apply_window
+    if (scheme != PDLP_SCHEME_REFERENCE && (cands & 2) && k1.unbounded) {
+        ctrl->status = PDLP_UNBOUNDED;
+        ctrl->done_iter = ctrl->iter;
+        if (mode != 0) ctrl->checks += 1;
+        ctrl->cand = 0;
+        ctrl->restart_pending = 0;
         ctrl->apply_window = ctrl->window;
         return;
     }
@@ -825,13 +892,15 @@ int launch_kkt_t(const PdlpDev& d, const PdlpTune& t, int mode, hipStream_t s) {
     const int cands = mode == 0 ? 1 : (t.scheme != PDLP_SCHEME_REFERENCE ? 3 : 2);
     for (int src = 0; src < 2; ++src) {
         if (!(cands & (1 << src))) continue;
+        // the unboundedness test rides on the average's launches of an adaptive check
+        const int ray = src == 1 && mode == 1 && t.scheme != PDLP_SCHEME_REFERENCE;
         int rc = with_tpr(td, [&](auto k) {
-            if constexpr (decltype(k)::value <= 64) kkt_rows_kernel<T, decltype(k)::value><<<nr, B, 0, s>>>(d, src);
+            if constexpr (decltype(k)::value <= 64) kkt_rows_kernel<T, decltype(k)::value><<<nr, B, 0, s>>>(d, src, ray);
             return 0;
         });
         if (rc) return rc;
         rc = with_tpr(tp, [&](auto k) {
-            if constexpr (decltype(k)::value <= 64) kkt_cols_kernel<T, decltype(k)::value><<<nc, B, 0, s>>>(d, src);
+            if constexpr (decltype(k)::value <= 64) kkt_cols_kernel<T, decltype(k)::value><<<nc, B, 0, s>>>(d, src, ray);
             return 0;
         });
         if (rc) return rc;

@@ -13,12 +13,12 @@ import threading
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "_lib", "libpdlp_hip.so")
 
-PDLP_ABI_VERSION = 6
+PDLP_ABI_VERSION = 7
 PDLP_F32, PDLP_F64 = 0, 1
-PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL, PDLP_INFEASIBLE = 0, 1, 3, 4
+PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL, PDLP_INFEASIBLE, PDLP_UNBOUNDED = 0, 1, 3, 4, 5
 PDLP_ROW_EQ, PDLP_ROW_LE = 0, 1
 PDLP_PARTIAL_BLOCKS = 1024
-PDLP_PARTIALS_LEN = 4 * PDLP_PARTIAL_BLOCKS * 10
+PDLP_PARTIALS_LEN = 4 * PDLP_PARTIAL_BLOCKS * 16
 PDLP_SCHEME_REFERENCE, PDLP_SCHEME_ADAPTIVE = 0, 1
 PDLP_FAM_TILE = 64
 PDLP_FAM_SHARED = {"c": 1, "b": 2, "lo": 4, "hi": 8, "c64": 16, "b64": 32, "b_shift": 64, "lo_shift": 128,

@@ -24,7 +24,7 @@ from .forms import HostForm
 from .tune import Tune, heuristic_tune
 
 _STATUS = {L.PDLP_RUNNING: "iteration_limit", L.PDLP_OPTIMAL: "optimal", L.PDLP_NUMERICAL: "numerical_error",
-           L.PDLP_INFEASIBLE: "infeasible"}
+           L.PDLP_INFEASIBLE: "infeasible", L.PDLP_UNBOUNDED: "unbounded"}
 
 # Largest graph (in check blocks).  Small graphs bound the surplus of no-op
 # launches after convergence (at most two graphs are in flight).
@@ -276,6 +276,15 @@ class SingleEngine:
         self.stream.synchronize()
         return x.numpy(), y.numpy()
 
+    def ray(self, st: L.PdlpCtrl) -> np.ndarray:
+        """The ray estimate of an unbounded solve, d = x - (window average), in the
+        engine's (scaled) space: the kernels left the current iterate in x and the
+        window's FP64 sums in x_sum."""
+        with torch.cuda.stream(self.stream):
+            d = (self.x.to(torch.float64) - self.x_sum / float(st.apply_window)).cpu()
+        self.stream.synchronize()
+        return d.numpy()
+
     def solve(self, tol: float, max_iters: int, x0=None, y0=None, step_scale: Optional[float] = None):
         """One solve.  ``step_scale`` is the starting primal weight w (tau = w tau_hat,
         sigma = sigma_hat / w); None = 1 for the reference scheme, the norm ratio of
@@ -286,8 +295,9 @@ class SingleEngine:
         st = self.run_iterations(max_iters)
         x, y = self.solution(st)
         iters = st.done_iter if st.status != L.PDLP_RUNNING else st.iter
+        ray = self.ray(st) if st.status == L.PDLP_UNBOUNDED else None
         return dict(status=_STATUS[st.status], x=x, y=y, iterations=int(iters), rel_kkt=st.last_err,
-                    rp=st.rp, rd=st.rd, gap=st.gap, restarts=int(st.restarts), ctrl=st)
+                    rp=st.rp, rd=st.rd, gap=st.gap, restarts=int(st.restarts), ctrl=st, ray=ray)
 
     # --------------------------------------------------------------- timing
     def time_kernel(self, which: str, t: Tune, reps: int = 20) -> float:

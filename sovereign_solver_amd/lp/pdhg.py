@@ -50,6 +50,19 @@ Keyword-only extensions (defaults keep the reference's contract):
 The reference returns no duals and no residuals; the result additionally
 carries ``y``, ``rel_kkt``, ``rp``, ``rd``, ``gap`` and ``restarts``.
 
+Status ``"unbounded"`` (not with the reference scheme): the LP has a ray of
+decrease, i.e. it is dual infeasible — unbounded when it is feasible.  The
+estimate d = x − (window average) is tested at every check of the adaptive
+schemes: c'd < 0 and ``|viol(Kd)|(1+|y|) + |viol_bounds(d)|(1+|c−K'y|) <=
+(−c'd)/M`` with the violations of K d against the rows' recession cone
+(``=`` rows: 0, ``<=`` rows: ≤ 0) and of d against the bounds'.  A dual
+solution (y, r) bounds c'd below by ``−|y||viol(Kd)| − |r||viol_bounds(d)|``,
+so a bounded LP fails the test while the iterate's duals are within a factor
+M of a dual solution in norm: M = 1e3 when the rows' violation is below 1e-3
+of the row sums of |K_ij d_j| (K d cancels), else 1e6.  The result then has ``x=None``, objective
+``-inf`` (the reference simplex's convention) and ``ray`` = d in the original
+space with ``max |d_j| = 1``.
+
 Status ``"infeasible"`` (not with the reference scheme, which keeps the
 reference's behaviour of running to the iteration limit): the dual ``y``
 (the current iterate or the window average, whichever passed) is a Farkas
@@ -144,6 +157,16 @@ def solve_lp_pdhg(
         r = eng.solve(tol, max_iters, xs, ys, step_scale=step_scale)
     finally:
         eng.drop_graphs()
+    if r["status"] == "unbounded":
+        d = np.asarray(r["ray"][: problem.n_vars], float)
+        if form.col_scale is not None:
+            d = d * form.col_scale
+        d = d / np.abs(d).max()
+        # c'x (the minimised objective) decreases along d without bound: -inf, or +inf in the
+        # reported sense of a 'max' problem, whose c is already negated
+        return SolveResult("unbounded", None, -np.inf if problem.sense != "max" else np.inf, r["iterations"],
+                           y=None, rel_kkt=r["rel_kkt"], rp=r["rp"], rd=r["rd"], gap=r["gap"],
+                           restarts=r["restarts"], step_scale=r["ctrl"].tau_scale, ray=d)
     x, obj, y = extract(form, problem, r["x"], r["y"])
     return SolveResult(r["status"], x, obj, r["iterations"], y=y, rel_kkt=r["rel_kkt"], rp=r["rp"],
                        rd=r["rd"], gap=r["gap"], restarts=r["restarts"], step_scale=r["ctrl"].tau_scale)

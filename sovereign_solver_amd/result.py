@@ -7,8 +7,10 @@ space), ``objective`` (original sense), ``iterations``, ``message``,
 the reference never returned: the dual vector ``y`` (rows ordered
 ``[A_ub; A_eq]``), the relative KKT error and its parts measured by the
 reference formula (``solver/lp/pdhg.py:184-196``) on the reference-form lift,
-the restart count and the primal weight at exit.  Callers that only use the reference fields see no
-difference.
+the restart count and the primal weight at exit, and for status ``"unbounded"``
+the ray of decrease that proves it (``x`` is then None and the objective
+``-inf``, as the reference's simplex reports it).  Callers that only use the
+reference fields see no difference.
 """
 
 from __future__ import annotations
@@ -36,6 +38,7 @@ class SolveResult:
         gap: Optional[float] = None,
         restarts: int = 0,
         step_scale: Optional[float] = None,
+        ray: Optional[np.ndarray] = None,
     ):
         self.status = status
         self.x = x
@@ -51,6 +54,7 @@ class SolveResult:
         self.gap = gap
         self.restarts = restarts
         self.step_scale = step_scale    # primal weight at exit: pass it back to resume a related solve
+        self.ray = ray                  # "unbounded": direction d, max |d_j| = 1, with c'd < 0
 
     def __repr__(self) -> str:
         obj = f"{self.objective:.6g}" if self.objective is not None else "N/A"

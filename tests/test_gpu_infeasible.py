@@ -11,7 +11,7 @@ import pytest
 import scipy.sparse as sp
 from scipy.optimize import linprog
 
-from oracle.farkas import farkas_value
+from oracle.farkas import farkas_value, ray_violations
 from sovereign_solver_amd import solve_lp_pdhg
 from sovereign_solver_amd.problem import Problem
 from sovereign_solver_amd.workloads import SetCoverNodes, random_sparse_lp
@@ -73,13 +73,16 @@ def test_infeasible_lp_returns_certificate(gpu, make, precision):
 
 
 def test_free_columns_conflict_is_not_certified(gpu):
-    # with free columns a certificate needs K'y = 0 there exactly, which PDHG's
-    # iterates only approach, so the exact test never fires: the solve runs to
-    # the iteration limit (and is never reported optimal)
+    # with free columns a Farkas certificate needs K'y = 0 there exactly, which PDHG's
+    # iterates only approach, so the exact test never fires.  This LP (40 free columns,
+    # 22 rows) is dual infeasible as well: c'x has a ray of decrease, which the
+    # unboundedness test reports (status "unbounded" = dual infeasible)
     p = _conflict(-np.inf, np.inf)
     assert _highs_status(p) == 2
     r = solve_lp_pdhg(p, max_iters=5_000, tol=1e-4)
-    assert r.status == "iteration_limit"
+    assert r.status == "unbounded"
+    cd, vr, vb = ray_violations(p, r.ray)
+    assert cd < 0.0 and vr <= 2e-3 * -cd and vb == 0.0
 
 
 def test_infeasible_lp_with_scaling(gpu):

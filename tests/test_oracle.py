@@ -146,3 +146,41 @@ def test_farkas_never_positive_on_feasible_lps(seed):
     rng = np.random.default_rng(seed)
     for _ in range(200):
         assert farkas_value(p, rng.standard_normal(p.n_constraints) * 10.0 ** rng.uniform(-3, 3)) <= 1e-9
+
+
+def test_ray_violations_hand_made():
+    # min -x1 over x >= 0 with x1 - x2 = 0 and -x1 <= 0: d = (1, 1) is a ray of decrease
+    from oracle.farkas import ray_violations
+    p = Problem(c=np.array([-1.0, 0.0]), A_ub=sp.csr_matrix(np.array([[-1.0, 0.0]])), b_ub=np.zeros(1),
+                A_eq=sp.csr_matrix(np.array([[1.0, -1.0]])), b_eq=np.zeros(1), lb=np.zeros(2),
+                ub=np.full(2, np.inf), integer_mask=np.zeros(2, bool))
+    assert ray_violations(p, [1.0, 1.0]) == (-1.0, 0.0, 0.0)
+    assert ray_violations(p, [1.0, 0.0]) == (-1.0, 1.0, 0.0)          # breaks the '=' row
+    assert ray_violations(p, [-1.0, -1.0]) == pytest.approx((1.0, 1.0, np.sqrt(2.0)))   # '<=' row and both bounds
+    q = Problem(c=p.c, A_ub=p.A_ub, b_ub=p.b_ub, A_eq=p.A_eq, b_eq=p.b_eq, lb=np.zeros(2), ub=np.array([np.inf, 4.0]),
+                integer_mask=p.integer_mask)
+    assert ray_violations(q, [1.0, 1.0]) == (-1.0, 0.0, 1.0)          # a finite upper bound stops the ray
+
+
+@pytest.mark.parametrize("seed", [0, 2, 5])       # draws HiGHS solves (1, 3, 4: unbounded)
+def test_ray_test_bound_holds_on_bounded_lps(seed):
+    # the unboundedness test's lemma: for a dual solution (y, r = c - K'y) of a bounded LP,
+    # c'd >= -|y| |viol(Kd)| - |r| |viol_bounds(d)| for every direction d
+    from oracle.farkas import ray_violations
+    p = random_sparse_lp(30, 60, 0.1, seed=seed, family=2)
+    lb = np.where(np.arange(60) % 3 == 0, -np.inf, p.lb)           # some columns free below ...
+    ub = np.where(np.arange(60) % 2 == 0, np.inf, p.ub)            # ... or above
+    p = Problem(c=p.c, A_ub=p.A_ub, b_ub=p.b_ub, A_eq=p.A_eq, b_eq=p.b_eq, lb=lb, ub=ub, integer_mask=p.integer_mask)
+    r = linprog(p.c, A_ub=p.A_ub if p.n_ineq else None, b_ub=p.b_ub if p.n_ineq else None,
+                A_eq=p.A_eq if p.n_eq else None, b_eq=p.b_eq if p.n_eq else None,
+                bounds=list(zip([None if np.isinf(v) else v for v in lb], [None if np.isinf(v) else v for v in ub])),
+                method="highs")
+    assert r.status == 0
+    y = np.concatenate([r.ineqlin.marginals if p.n_ineq else [], r.eqlin.marginals if p.n_eq else []])
+    K = sp.vstack([sp.csr_matrix(p.A_ub), sp.csr_matrix(p.A_eq)]).tocsr()
+    red = p.c - K.T @ y
+    rng = np.random.default_rng(seed)
+    for _ in range(500):
+        d = rng.standard_normal(60) * (rng.random(60) < 0.3)
+        cd, vr, vb = ray_violations(p, d)
+        assert cd >= -np.linalg.norm(y) * vr - np.linalg.norm(red) * vb - 1e-7 * (1 + np.abs(p.c) @ np.abs(d))
