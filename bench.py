@@ -723,12 +723,32 @@ def main():
     from sovereign_solver_amd.engine import tune as T
     from sovereign_solver_amd.workloads import transportation_lp
 
+    from sovereign_solver_amd.engine.device import csr_transpose
     t0 = time.perf_counter()
     p = transportation_lp(args.sources, args.sinks, seed=0)
+    t1 = time.perf_counter()
     form = box_form(p)
+    t2 = time.perf_counter()
     log(f"built transport LP {args.sources}x{args.sinks}: m={form.m} n={form.n} nnz={form.nnz} "
-        f"({time.perf_counter() - t0:.1f}s)")
+        f"(generator {t1 - t0:.2f}s, box form {t2 - t1:.2f}s)")
     eng = SingleEngine(form, "fp32")
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    # ingestion: K^T on the device (second call timed with events) vs on the host
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    for _ in range(2):
+        ev[0].record(eng.stream)
+        csr_transpose(eng.row_ptr, eng.col_idx, eng.val, eng.n, eng.stream)
+        ev[1].record(eng.stream)
+        eng.stream.synchronize()
+    dev_transpose_ms = ev[0].elapsed_time(ev[1])
+    th = time.perf_counter()
+    _ = form.KT
+    host_transpose_ms = (time.perf_counter() - th) * 1e3
+    ingest = {"box_form_s": t2 - t1, "engine_setup_s": t3 - t2, "device_transpose_ms": dev_transpose_ms,
+              "host_transpose_ms": host_transpose_ms}
+    log(f"ingest: box form {t2 - t1:.3f}s, engine setup {t3 - t2:.3f}s; K^T on the device "
+        f"{dev_transpose_ms:.2f} ms vs SciPy on the host {host_transpose_ms:.1f} ms")
     if not args.no_autotune:
         eng.set_tune(T.autotune(eng))
     log(f"tune {eng.tune}")
@@ -809,7 +829,7 @@ def main():
                    "objective": obj, "objective_highs_ds": obj_ref,
                    "objective_rel_err": None if obj_ref is None else abs(obj - obj_ref) / (1 + abs(obj_ref)),
                    "tune": eng.tune.__dict__, "iterate_dtype": "fp32", "kkt_dtype": "fp64",
-                   "reference_form_iterations": ref_iters},
+                   "reference_form_iterations": ref_iters, "ingest": ingest},
         "roofline": {"bound": "hbm", "kernel": "primal_kernel (fused K^T y + step + projection + sums)",
                      "achieved": gbs_p, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs_p / HBM_PEAK_GBS,
                      "traffic": measured_traffic(f"transport_{args.sources}x{args.sinks}/primal_kernel"),

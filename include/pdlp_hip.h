@@ -193,6 +193,19 @@ typedef struct PdlpTune {
 int         pdlp_abi_version(void);
 const char* pdlp_last_error(void);
 
+/* Device CSR transpose for problem ingestion: K^T (cols x rows) from K
+ * (rows x cols, CSR with ascending column indices per row), values of `prec`.
+ * Replaces the host transposes `AT = A.T.tocsr()` of pdhg.py:179 and
+ * pdhg_gpu.py:137,161,186.  The result equals SciPy's canonical transpose bit
+ * for bit: rows of K^T in column order, entries in ascending row order.
+ * t_row_ptr: cols + 1 entries; t_col_idx, t_val: nnz.  `work` holds at least
+ * pdlp_csr_transpose_work_bytes(rows, cols, nnz) bytes of device memory (may
+ * be NULL when nnz = 0).  Column indices must lie in [0, cols). */
+int pdlp_csr_transpose_work_bytes(int32_t rows, int32_t cols, int32_t nnz, size_t* bytes);
+int pdlp_csr_transpose(int32_t prec, int32_t rows, int32_t cols, int32_t nnz, const int32_t* row_ptr,
+                       const int32_t* col_idx, const void* val, int32_t* t_row_ptr, int32_t* t_col_idx,
+                       void* t_val, void* work, size_t work_bytes, void* stream);
+
 /* y = A x, A in CSR (prec values), x,y prec.  The SpMV of pdhg.py:211,217
  * (SciPy csr_matvec) and pdhg_gpu.py:215,222,245,252 (cuSPARSE / torch.mv). */
 int pdlp_spmv_csr(int32_t prec, const int32_t* row_ptr, const int32_t* col_idx,

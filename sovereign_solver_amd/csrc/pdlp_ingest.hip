@@ -1,0 +1,150 @@
+// Problem ingestion on the device: the CSR transpose K -> K^T that the
+// single-instance engine's primal step gathers over.  The host restatement is
+// SciPy's `(K.T).tocsr()` after `sort_indices()` (the canonical transpose that
+// `solver/lp/pdhg.py:204-217` multiplies by through `A.T @ y`): rows of K^T in
+// column order of K, entries of each in ascending row order.
+//
+// Method: a stable LSD radix sort (hipCUB) of the entries' column indices,
+// carrying the entry ids, leaves the entries grouped by column and, within a
+// column, in row order (the input is row-major and the sort is stable).  Then
+// one kernel gathers row index and value per entry, and one writes the row
+// pointers of K^T from the boundaries of the sorted keys.  Every step is
+// deterministic, so the result equals SciPy's bit for bit.
+
+#include "pdlp_common.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+namespace pdlp {
+namespace {
+
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+int key_bits(int32_t cols) {
+    int b = 1;
+    while (b < 31 && (int64_t(1) << b) < (int64_t)cols) ++b;
+    return b;
+}
+
+// Entry e of row r (found by binary search over the row pointers): its row
+// index and its id, the value the sort carries.
+__global__ void __launch_bounds__(256) entry_rows_kernel(const int32_t* __restrict__ rp, int32_t rows, int32_t nnz,
+                                                         int32_t* __restrict__ row_of, int32_t* __restrict__ ids) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    int lo = 0, hi = rows - 1;                  // last row with rp[row] <= e
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (rp[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    row_of[e] = lo;
+    ids[e] = e;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256) gather_kernel(const int32_t* __restrict__ sorted_ids,
+                                                     const int32_t* __restrict__ row_of,
+                                                     const V* __restrict__ val, int32_t nnz,
+                                                     int32_t* __restrict__ t_col_idx, V* __restrict__ t_val) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const int e = sorted_ids[k];
+    t_col_idx[k] = row_of[e];
+    t_val[k] = val[e];
+}
+
+// t_row_ptr[c] = first position of a key >= c.  Position k writes the pointers
+// of the columns between the previous key and its own; position nnz closes the
+// columns after the last key.  Indices are kept within [0, cols].
+__global__ void __launch_bounds__(256) pointers_kernel(const int32_t* __restrict__ keys, int32_t nnz, int32_t cols,
+                                                       int32_t* __restrict__ t_row_ptr) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nnz) return;
+    const int prev = k == 0 ? -1 : keys[k - 1];
+    const int cur = k == nnz ? cols : min(keys[k], cols);
+    for (int c = max(prev + 1, 0); c <= cur && c <= cols; ++c) t_row_ptr[c] = k;
+}
+
+struct Layout {
+    size_t sorted_keys, ids, sorted_ids, row_of, temp, temp_bytes, total;
+};
+
+int layout(int32_t cols, int32_t nnz, Layout& l) {
+    size_t temp = 0;
+    if (nnz > 0) {
+        const hipError_t e = hipcub::DeviceRadixSort::SortPairs(
+            nullptr, temp, (const int32_t*)nullptr, (int32_t*)nullptr, (const int32_t*)nullptr, (int32_t*)nullptr,
+            nnz, 0, key_bits(cols));
+        if (e != hipSuccess) return hip_fail(e, "radix sort workspace query");
+    }
+    const size_t v = align_up((size_t)nnz * sizeof(int32_t));
+    l.sorted_keys = 0;
+    l.ids = v;
+    l.sorted_ids = 2 * v;
+    l.row_of = 3 * v;
+    l.temp = 4 * v;
+    l.temp_bytes = temp;
+    l.total = 4 * v + align_up(temp);
+    return 0;
+}
+
+}  // namespace
+}  // namespace pdlp
+
+using namespace pdlp;
+
+extern "C" {
+
+int pdlp_csr_transpose_work_bytes(int32_t rows, int32_t cols, int32_t nnz, size_t* bytes) {
+    if (bytes == nullptr) return fail(PDLP_E_ARG, "bytes is NULL");
+    if (rows < 0 || cols < 0 || nnz < 0) return fail(PDLP_E_SHAPE, "negative size");
+    Layout l;
+    if (int rc = layout(cols, nnz, l)) return rc;
+    *bytes = l.total;
+    return 0;
+}
+
+int pdlp_csr_transpose(int32_t prec, int32_t rows, int32_t cols, int32_t nnz, const int32_t* row_ptr,
+                       const int32_t* col_idx, const void* val, int32_t* t_row_ptr, int32_t* t_col_idx,
+                       void* t_val, void* work, size_t work_bytes, void* stream) {
+    if (prec != PDLP_F32 && prec != PDLP_F64) return fail(PDLP_E_ARG, "prec must be PDLP_F32 or PDLP_F64");
+    if (rows < 0 || cols < 0 || nnz < 0) return fail(PDLP_E_SHAPE, "negative size");
+    if (nnz > 0 && rows == 0) return fail(PDLP_E_SHAPE, "entries without rows");
+    if (t_row_ptr == nullptr || (nnz > 0 && (row_ptr == nullptr || col_idx == nullptr || val == nullptr ||
+                                             t_col_idx == nullptr || t_val == nullptr)))
+        return fail(PDLP_E_ARG, "NULL array");
+    Layout l;
+    if (int rc = layout(cols, nnz, l)) return rc;
+    if (nnz > 0 && (work == nullptr || work_bytes < l.total))
+        return fail(PDLP_E_ARG, "work is smaller than pdlp_csr_transpose_work_bytes");
+    const hipStream_t s = (hipStream_t)stream;
+    constexpr int B = 256;
+    char* w = (char*)work;
+    int32_t* sorted_keys = (int32_t*)(w + l.sorted_keys);
+    int32_t* ids = (int32_t*)(w + l.ids);
+    int32_t* sorted_ids = (int32_t*)(w + l.sorted_ids);
+    int32_t* row_of = (int32_t*)(w + l.row_of);
+    if (nnz > 0) {
+        const int g = (nnz + B - 1) / B;
+        entry_rows_kernel<<<g, B, 0, s>>>(row_ptr, rows, nnz, row_of, ids);
+        if (int rc = hip_fail(hipGetLastError(), "entry_rows_kernel launch")) return rc;
+        size_t temp = l.temp_bytes;
+        const hipError_t e = hipcub::DeviceRadixSort::SortPairs(w + l.temp, temp, col_idx, sorted_keys, ids,
+                                                                sorted_ids, nnz, 0, key_bits(cols), s);
+        if (e != hipSuccess) return hip_fail(e, "radix sort");
+        if (prec == PDLP_F32)
+            gather_kernel<float><<<g, B, 0, s>>>(sorted_ids, row_of, (const float*)val, nnz, t_col_idx,
+                                                 (float*)t_val);
+        else
+            gather_kernel<double><<<g, B, 0, s>>>(sorted_ids, row_of, (const double*)val, nnz, t_col_idx,
+                                                  (double*)t_val);
+        if (int rc = hip_fail(hipGetLastError(), "gather_kernel launch")) return rc;
+    }
+    pointers_kernel<<<(nnz + 1 + B - 1) / B, B, 0, s>>>(sorted_keys, nnz, cols, t_row_ptr);
+    return hip_fail(hipGetLastError(), "pointers_kernel launch");
+}
+
+}  // extern "C"

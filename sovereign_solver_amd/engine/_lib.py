@@ -142,6 +142,9 @@ _lib = None
 _SIGS = {
     "pdlp_abi_version": (C.c_int, []),
     "pdlp_last_error": (C.c_char_p, []),
+    "pdlp_csr_transpose_work_bytes": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_size_t)]),
+    "pdlp_csr_transpose": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, _p, _p, _p, _p, _p, _p, _p,
+                                     C.c_size_t, _p]),
     "pdlp_spmv_csr": (C.c_int, [C.c_int32, _p, _p, _p, _p, _p, C.c_int32, C.c_int32, _p]),
     "pdlp_spmv_csr_f64acc": (C.c_int, [_p, _p, _p, _p, _p, C.c_int32, C.c_int32, _p]),
     "pdlp_init": (C.c_int, [C.POINTER(PdlpDev), C.POINTER(PdlpTune), _p]),

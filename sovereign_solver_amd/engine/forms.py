@@ -37,6 +37,7 @@ original LP's.  :func:`extract` maps the solution back.
 
 from __future__ import annotations
 
+import functools
 from dataclasses import dataclass, field
 from typing import Optional
 
@@ -54,7 +55,6 @@ ROW_EQ, ROW_LE = 0, 1
 class HostForm:
     kind: str                      # "box" | "reference"
     K: sp.csr_matrix               # m x n, canonical CSR (sorted, no duplicates), int32 indices
-    KT: sp.csr_matrix              # n x m
     c: np.ndarray                  # n
     b: np.ndarray                  # m
     lo: Optional[np.ndarray]       # n or None (= 0)
@@ -80,6 +80,11 @@ class HostForm:
     @property
     def m(self) -> int:
         return self.K.shape[0]
+
+    @functools.cached_property
+    def KT(self) -> sp.csr_matrix:
+        """n x m, built on the host on first use (the engines build K^T on the device)."""
+        return _transpose(self.K)
 
     @property
     def n(self) -> int:
@@ -131,7 +136,7 @@ def box_form(p: Problem, scaling: str = "none") -> HostForm:
         raise ValueError(f"scaling must be one of {S.METHODS}, got {scaling!r}")
     trivial_box = bool(np.all(lb == 0.0) and not fin_ub.any())
     return HostForm(
-        kind="box", K=K, KT=_transpose(K), c=c, b=b,
+        kind="box", K=K, c=c, b=b,
         lo=None if trivial_box else lb, hi=None if trivial_box else ub,
         row_kind=kind if m_ub else None,
         b_shift=b_shift, lo_shift=None if not lb_f.any() else lb_f,
@@ -170,7 +175,7 @@ def reference_form(p: Problem) -> HostForm:
     K = _canonical(K)
     c = np.concatenate([np.asarray(p.c, float), np.zeros(m_le)])
     return HostForm(
-        kind="reference", K=K, KT=_transpose(K), c=c, b=b, lo=None, hi=None, row_kind=None,
+        kind="reference", K=K, c=c, b=b, lo=None, hi=None, row_kind=None,
         b_shift=b.copy(), lo_shift=None, ub_span=None,
         norm_b=max(1.0, float(np.linalg.norm(b))), norm_c=max(1.0, float(np.linalg.norm(c))),
         n_orig=n, m_ub=p.n_ineq, m_eq=m_eq, lb_f=lb_f, row_sign=sign, n_ub_rows=nF,

@@ -19,7 +19,7 @@ import torch
 
 from . import _lib as L
 from . import sell
-from .device import dev_tensor, ptr, require_gpu, stream_handle, struct_from_bytes
+from .device import csr_transpose, dev_tensor, ptr, require_gpu, stream_handle, struct_from_bytes
 from .forms import HostForm
 from .tune import Tune, heuristic_tune
 
@@ -57,13 +57,13 @@ class SingleEngine:
         f64 = torch.float64
         i32 = torch.int32
         with torch.cuda.stream(self.stream):
-            K, KT = form.K, form.KT
+            K = form.K
             self.row_ptr = dev_tensor(K.indptr, i32, dv)
             self.col_idx = dev_tensor(K.indices, i32, dv)
             self.val = dev_tensor(K.data, T, dv)
-            self.t_row_ptr = dev_tensor(KT.indptr, i32, dv)
-            self.t_col_idx = dev_tensor(KT.indices, i32, dv)
-            self.t_val = dev_tensor(KT.data, T, dv)
+            # K^T for the primal step, built on the device from the uploaded K
+            self.t_row_ptr, self.t_col_idx, self.t_val = csr_transpose(self.row_ptr, self.col_idx, self.val,
+                                                                       self.n, self.stream)
             self.c = dev_tensor(form.c, T, dv)
             self.lo = None if form.lo is None else dev_tensor(form.lo, T, dv)
             self.hi = None if form.hi is None else dev_tensor(form.hi, T, dv)
