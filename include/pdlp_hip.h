@@ -351,7 +351,11 @@ typedef struct PdlpBatch {
     const double* col_scale;        /* per column: D_c */
 } PdlpBatch;
 
-/* Solve every LP of the batch (one launch). */
+/* Solve every LP of the batch: one persistent launch per part.  With both
+ * parts the FP64 part runs concurrently on a second stream of the library's
+ * (default priority, one per device, kept until the process ends), forked
+ * from and joined back into `stream` through events, so work enqueued on
+ * `stream` afterwards sees both parts finished. */
 int pdlp_batch_solve(const PdlpBatch* batch, void* stream);
 
 /* ----------------------------------------------------------------------------

@@ -38,6 +38,8 @@
 #include "pdlp_common.hpp"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 namespace pdlp {
@@ -709,6 +711,29 @@ int launch_part(const PdlpBatch* a, int first, int count, int* queue, int block,
     }
 }
 
+// A second stream per device (default priority, created on first use and kept
+// until the process ends) on which the FP64 part runs beside the FP32 part:
+// the FP64 part's time is set by a few small LPs' long, latency-bound chains,
+// which overlap the FP32 part's streaming.  Fork and join through events, so
+// the caller's stream still orders everything after the call.
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+std::mutex g_side_mu;
+std::map<int, SideStream> g_side;
+
+int side_stream(int dev, SideStream& out) {
+    SideStream& ss = g_side[dev];
+    if (ss.s == nullptr) {
+        if (int rc = hip_fail(hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking), "hipStreamCreate")) return rc;
+        if (int rc = hip_fail(hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming), "hipEventCreate")) return rc;
+        if (int rc = hip_fail(hipEventCreateWithFlags(&ss.join, hipEventDisableTiming), "hipEventCreate")) return rc;
+    }
+    out = ss;
+    return 0;
+}
+
 }  // namespace
 }  // namespace pdlp
 
@@ -736,15 +761,31 @@ extern "C" int pdlp_batch_solve(const PdlpBatch* a, void* stream) {
     }
     const int n32 = a->count - a->n_resident;
     const int parts = a->parts ? a->parts : 3;
-    if (parts & 1) {
-        const int rc = a->x_in_lds
-                           ? launch_part<float, true>(a, 0, n32, a->queue, a->block, a->lds_bytes, a->grid, cus, s)
-                           : launch_part<float, false>(a, 0, n32, a->queue, a->block, a->lds_bytes, a->grid, cus, s);
-        if (rc != 0) return rc;
-    }
-    if (parts & 2)
+    auto part32 = [&](hipStream_t st) {
+        return a->x_in_lds ? launch_part<float, true>(a, 0, n32, a->queue, a->block, a->lds_bytes, a->grid, cus, st)
+                           : launch_part<float, false>(a, 0, n32, a->queue, a->block, a->lds_bytes, a->grid, cus, st);
+    };
+    auto part64 = [&](hipStream_t st) {
         return launch_part<double, false>(a, n32, a->n_resident, a->queue + 1,
-                                   a->block_resident ? a->block_resident : 256, a->lds_resident, a->grid_resident,
-                                   cus, s);
+                                          a->block_resident ? a->block_resident : 256, a->lds_resident,
+                                          a->grid_resident, cus, st);
+    };
+    if (parts == 3 && n32 > 0 && a->n_resident > 0) {
+        // both parts: the FP64 part on the side stream, concurrently.  The lock keeps
+        // each call's record/wait pairs together when host threads share a device.
+        std::lock_guard<std::mutex> lock(g_side_mu);
+        SideStream ss;
+        if (int rc = side_stream(dev, ss)) return rc;
+        if (int rc = hip_fail(hipEventRecord(ss.fork, s), "hipEventRecord")) return rc;
+        if (int rc = hip_fail(hipStreamWaitEvent(ss.s, ss.fork, 0), "hipStreamWaitEvent")) return rc;
+        if (int rc = part64(ss.s)) return rc;
+        if (int rc = hip_fail(hipEventRecord(ss.join, ss.s), "hipEventRecord")) return rc;
+        if (int rc = part32(s)) return rc;
+        return hip_fail(hipStreamWaitEvent(s, ss.join, 0), "hipStreamWaitEvent");
+    }
+    if (parts & 1) {
+        if (int rc = part32(s)) return rc;
+    }
+    if (parts & 2) return part64(s);
     return 0;
 }
