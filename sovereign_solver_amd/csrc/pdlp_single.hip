@@ -298,6 +298,7 @@ __global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d) {
 // d against the bounds' recession cone, 15 |c - K'y|^2 of the average.
 
 constexpr int NP = 16;
+constexpr int NP_BASE = 10;   // the sums of a check without the unboundedness test
 
 __device__ __forceinline__ double* part_ptr(const PdlpDev& d, int src, int part, int blk) {
     return d.partials + ((size_t)(src * 2 + part) * PDLP_PARTIAL_BLOCKS + blk) * NP;
@@ -335,9 +336,10 @@ __device__ __forceinline__ void row_ray(const int32_t* __restrict__ rp, const in
     kabs = a;
 }
 
-template <typename T, int TPR>
-__global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src, int ray) {
-    __shared__ double lds[16 * NP];
+template <typename T, int TPR, bool RAY>
+__global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src) {
+    constexpr int NS = RAY ? NP : NP_BASE;
+    __shared__ double lds[16 * NS];
     const PdlpCtrl* ctrl = d.ctrl;
     if (ctrl->status != PDLP_RUNNING) return;
     const double w = src ? (double)ctrl->window : 1.0;
@@ -346,7 +348,7 @@ __global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src, int r
     const T* __restrict__ y = (const T*)d.y;
     const T* ylast = (const T*)d.y_last;
     const int gpb = blockDim.x / TPR, lane = threadIdx.x % TPR;
-    double acc[NP] = {};
+    double acc[NS] = {};
     for (int i = blockIdx.x * gpb + threadIdx.x / TPR; i < d.m; i += gridDim.x * gpb) {
         double ax, asum;
         if (src) row_dot<TPR, false, double>(d.row_ptr, d.col_idx, v, i, lane,
@@ -354,7 +356,7 @@ __global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src, int r
         else row_dot<TPR, false, double>(d.row_ptr, d.col_idx, v, i, lane,
                                          [&](int k) { return (double)x[k]; }, ax, asum);
         double kd = 0.0, kabs = 0.0;
-        if (ray) row_ray<TPR>(d.row_ptr, d.col_idx, v, i, lane,
+        if constexpr (RAY) row_ray<TPR>(d.row_ptr, d.col_idx, v, i, lane,
                               [&](int k) { return (double)x[k] - avg_of(d.x_sum, d.x_psum, k, w); }, kd, kabs);
         if (lane == 0) {
             const bool le = d.row_kind != nullptr && d.row_kind[i] == PDLP_ROW_LE;
@@ -370,7 +372,7 @@ __global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src, int r
                 acc[1] += yi * yi;                      // slack column reduced cost -y_i
             }
             acc[0] += res * res;
-            if (ray) {
+            if constexpr (RAY) {
                 const double dr = d.row_scale != nullptr ? d.row_scale[i] : 1.0;
                 const double viol = (le && kd < 0.0 ? 0.0 : kd) / dr, ka = kabs / dr, yo = yi * dr;
                 acc[10] += viol * viol;
@@ -387,16 +389,17 @@ __global__ void __launch_bounds__(256) kkt_rows_kernel(PdlpDev d, int src, int r
             }
         }
     }
-    block_sum<NP>(acc, lds);
+    block_sum<NS>(acc, lds);
     if (threadIdx.x == 0) {
         double* p = part_ptr(d, src, 0, blockIdx.x);
-        for (int k = 0; k < NP; ++k) p[k] = acc[k];
+        for (int k = 0; k < NP; ++k) p[k] = k < NS ? acc[k] : 0.0;
     }
 }
 
-template <typename T, int TPR>
-__global__ void __launch_bounds__(256) kkt_cols_kernel(PdlpDev d, int src, int ray) {
-    __shared__ double lds[16 * NP];
+template <typename T, int TPR, bool RAY>
+__global__ void __launch_bounds__(256) kkt_cols_kernel(PdlpDev d, int src) {
+    constexpr int NS = RAY ? NP : NP_BASE;
+    __shared__ double lds[16 * NS];
     const PdlpCtrl* ctrl = d.ctrl;
     if (ctrl->status != PDLP_RUNNING) return;
     const double w = src ? (double)ctrl->window : 1.0;
@@ -405,7 +408,7 @@ __global__ void __launch_bounds__(256) kkt_cols_kernel(PdlpDev d, int src, int r
     const T* __restrict__ y = (const T*)d.y;
     const T* xlast = (const T*)d.x_last;
     const int gpb = blockDim.x / TPR, lane = threadIdx.x % TPR;
-    double acc[NP] = {};
+    double acc[NS] = {};
     for (int j = blockIdx.x * gpb + threadIdx.x / TPR; j < d.n; j += gridDim.x * gpb) {
         double aty, asum;
         if (src) row_dot<TPR, false, double>(d.t_row_ptr, d.t_col_idx, tv, j, lane,
@@ -430,7 +433,7 @@ __global__ void __launch_bounds__(256) kkt_cols_kernel(PdlpDev d, int src, int r
             const double xj = src ? avg_of(d.x_sum, d.x_psum, j, w) : (double)x[j];
             const double sh = d.lo_shift != nullptr ? d.lo_shift[j] : 0.0;
             acc[2] += d.c64[j] * (xj - sh);
-            if (ray) {
+            if constexpr (RAY) {
                 const double dj = (double)x[j] - xj, dc = d.col_scale != nullptr ? d.col_scale[j] : 1.0;
                 const double dor = dj * dc, ro = r / dc;
                 double bv = 0.0;                         // d must keep finite bounds' sides
@@ -446,10 +449,10 @@ __global__ void __launch_bounds__(256) kkt_cols_kernel(PdlpDev d, int src, int r
             }
         }
     }
-    block_sum<NP>(acc, lds);
+    block_sum<NS>(acc, lds);
     if (threadIdx.x == 0) {
         double* p = part_ptr(d, src, 1, blockIdx.x);
-        for (int k = 0; k < NP; ++k) p[k] = acc[k];
+        for (int k = 0; k < NP; ++k) p[k] = k < NS ? acc[k] : 0.0;
     }
 }
 
@@ -492,9 +495,11 @@ __device__ KktVal kkt_value(const double (&s)[NP], const PdlpCtrl* ctrl) {
 
 // mode 0: initial point (candidate 0 only; sets the reference KKT).
 // mode 1: check; `cands` = bit mask of the evaluated candidates (bit c = src c).
+template <bool RAY>
 __global__ void __launch_bounds__(256) kkt_finalize_kernel(PdlpDev d, int nb_rows, int nb_cols, int mode,
                                                            int scheme, int cands) {
-    __shared__ double lds[16 * NP];
+    constexpr int NS = RAY ? NP : NP_BASE;      // sums beyond NS are zero
+    __shared__ double lds[16 * NS];
     PdlpCtrl* ctrl = d.ctrl;
     if (ctrl->status != PDLP_RUNNING) {
         if (threadIdx.x == 0) ctrl->restart_pending = 0;
@@ -502,19 +507,19 @@ __global__ void __launch_bounds__(256) kkt_finalize_kernel(PdlpDev d, int nb_row
     }
     double sums[2][NP];
     for (int c = 0; c < 2; ++c) {
-        double acc[NP] = {};
+        double acc[NS] = {};
         if (cands & (1 << c)) {
             for (int b = threadIdx.x; b < nb_rows; b += blockDim.x) {
                 const double* p = part_ptr(d, c, 0, b);
-                for (int k = 0; k < NP; ++k) acc[k] += p[k];
+                for (int k = 0; k < NS; ++k) acc[k] += p[k];
             }
             for (int b = threadIdx.x; b < nb_cols; b += blockDim.x) {
                 const double* p = part_ptr(d, c, 1, b);
-                for (int k = 0; k < NP; ++k) acc[k] += p[k];
+                for (int k = 0; k < NS; ++k) acc[k] += p[k];
             }
         }
-        block_sum<NP>(acc, lds);
-        for (int k = 0; k < NP; ++k) sums[c][k] = acc[k];
+        block_sum<NS>(acc, lds);
+        for (int k = 0; k < NP; ++k) sums[c][k] = k < NS ? acc[k] : 0.0;
     }
     if (threadIdx.x != 0) return;
     const KktVal k0 = kkt_value(sums[0], ctrl), k1 = kkt_value(sums[1], ctrl);
@@ -890,22 +895,32 @@ int launch_kkt_t(const PdlpDev& d, const PdlpTune& t, int mode, hipStream_t s) {
     // candidates: the initial point is "current"; a reference check uses the
     // average only; an adaptive check evaluates both
     const int cands = mode == 0 ? 1 : (t.scheme != PDLP_SCHEME_REFERENCE ? 3 : 2);
+    const bool check_ray = mode == 1 && t.scheme != PDLP_SCHEME_REFERENCE;
     for (int src = 0; src < 2; ++src) {
         if (!(cands & (1 << src))) continue;
         // the unboundedness test rides on the average's launches of an adaptive check
-        const int ray = src == 1 && mode == 1 && t.scheme != PDLP_SCHEME_REFERENCE;
+        const bool ray = src == 1 && check_ray;
         int rc = with_tpr(td, [&](auto k) {
-            if constexpr (decltype(k)::value <= 64) kkt_rows_kernel<T, decltype(k)::value><<<nr, B, 0, s>>>(d, src, ray);
+            constexpr int P = decltype(k)::value;
+            if constexpr (P <= 64) {
+                if (ray) kkt_rows_kernel<T, P, true><<<nr, B, 0, s>>>(d, src);
+                else kkt_rows_kernel<T, P, false><<<nr, B, 0, s>>>(d, src);
+            }
             return 0;
         });
         if (rc) return rc;
         rc = with_tpr(tp, [&](auto k) {
-            if constexpr (decltype(k)::value <= 64) kkt_cols_kernel<T, decltype(k)::value><<<nc, B, 0, s>>>(d, src, ray);
+            constexpr int P = decltype(k)::value;
+            if constexpr (P <= 64) {
+                if (ray) kkt_cols_kernel<T, P, true><<<nc, B, 0, s>>>(d, src);
+                else kkt_cols_kernel<T, P, false><<<nc, B, 0, s>>>(d, src);
+            }
             return 0;
         });
         if (rc) return rc;
     }
-    kkt_finalize_kernel<<<1, B, 0, s>>>(d, nr, nc, mode, t.scheme, cands);
+    if (check_ray) kkt_finalize_kernel<true><<<1, B, 0, s>>>(d, nr, nc, mode, t.scheme, cands);
+    else kkt_finalize_kernel<false><<<1, B, 0, s>>>(d, nr, nc, mode, t.scheme, cands);
     if (mode == 1) {
         const int work = std::max(d.n, d.m);
         const int g = std::min((work + B - 1) / B, 4096);
