@@ -60,3 +60,26 @@ def test_argument_errors_without_device():
     assert rc in (-1, -2, -3)
     with pytest.raises(L.PdlpError):
         L.check(rc, "primal")
+
+
+def test_header_constants_match_binding():
+    txt = open(os.path.join(ROOT, "include", "pdlp_hip.h")).read()
+    defs = dict(re.findall(r"#define\s+(PDLP_\w+)\s+([^/\n]+)", txt))
+    env = {}
+    for k, v in defs.items():
+        try:
+            env[k] = eval(v.strip(), {}, dict(env))
+        except (NameError, SyntaxError):
+            pass
+    for name in ("PDLP_ABI_VERSION", "PDLP_RUNNING", "PDLP_OPTIMAL", "PDLP_NUMERICAL", "PDLP_INFEASIBLE",
+                 "PDLP_UNBOUNDED", "PDLP_PARTIAL_BLOCKS", "PDLP_PARTIALS_LEN"):
+        assert env[name] == getattr(L, name), name
+
+
+def test_transpose_argument_errors_without_device():
+    lib = L.lib()
+    b = C.c_size_t(0)
+    assert lib.pdlp_csr_transpose_work_bytes(-1, 2, 3, C.byref(b)) == -3          # PDLP_E_SHAPE
+    assert lib.pdlp_csr_transpose_work_bytes(1, 2, 3, None) == -1                 # PDLP_E_ARG
+    assert lib.pdlp_csr_transpose(7, 1, 1, 0, None, None, None, None, None, None, None, 0, None) == -1
+    assert b"prec" in lib.pdlp_last_error()
