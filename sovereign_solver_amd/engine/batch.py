@@ -80,7 +80,9 @@ def lds_bytes_resident(m, n):
 # n = 2.5m) run with FP64 iterates; larger ones with FP32 iterates on 512
 # threads, which finish a mid-size LP (m ~ 1,000) 1.6x sooner than the FP64
 # part's 256 threads.  On the uneven 4,096-LP batch: 0.600 s at 16 KB,
-# 0.606 s at 24 KB, 0.614 s at 40 KB, 0.711 s at 64 KB.
+# 0.606 s at 24 KB, 0.614 s at 40 KB, 0.711 s at 64 KB (parts back to back,
+# earlier kernels); with the parts concurrent, 0.481 s at 16 KB, 0.482 s at
+# 24 KB, 0.484 s at 32 KB, 0.494 s at 40 KB.
 RESIDENT_LDS = 16 * 1024
 
 
