@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PDLP_ABI_VERSION 7
+#define PDLP_ABI_VERSION 8
 
 /* precision of iterates, matrix values and problem vectors */
 #define PDLP_F32 0
@@ -47,11 +47,15 @@ extern "C" {
                                 engine): the dual iterate is a Farkas certificate,
                                 b'y - max_{lo<=x<=hi} (K'y)'x > 0 with y <= 0 on
                                 PDLP_ROW_LE rows */
+#define PDLP_SKIPPED    6    /* persistent batch engine: the LP needs more LDS than its
+                                part's launch has (descriptors inconsistent with
+                                lds_need); not run */
 #define PDLP_UNBOUNDED  5    /* dual infeasible (single-instance box form, adaptive
                                 schemes): d = x - x_sum / apply_window is a ray with
                                 c'd < 0 whose row and bound violations pass the
-                                engine's relative test; the LP is unbounded when it
-                                is feasible */
+                                engine's relative tests, among them |viol(K d)| +
+                                |viol_bounds(d)| <= 1e-6 (-c'd); the LP is unbounded
+                                when it is feasible */
 
 /* row kinds of the box form */
 #define PDLP_ROW_EQ 0        /* K_i x  = b_i, dual free     */
@@ -187,7 +191,8 @@ typedef struct PdlpTune {
                              entries split among its wavefronts and their sums added
                              in a fixed order; k >= 8 (multiple of 8) a wavefront
                              per k entries of a slice, partial sums added in order */
-    int32_t reserved;
+    int32_t no_ray;       /* 1: skip the unboundedness (ray of decrease) test of the
+                             adaptive checks; 0: test at every check */
 } PdlpTune;
 
 int         pdlp_abi_version(void);
@@ -314,7 +319,8 @@ typedef struct PdlpBatch {
     int32_t max_iters;
     int32_t block;          /* FP32 part: threads per workgroup, 256 or 512   */
     int32_t lds_bytes;      /* FP32 part: dynamic LDS per workgroup, >= 12*n+8*m+8 of its LPs
-                               (an LP that does not fit is not run, its output left as it was) */
+                               (checked against lds_need; an LP whose own need still exceeds
+                               it is not run and reports PDLP_SKIPPED) */
     int32_t grid;           /* FP32 part: persistent workgroups (0 = 2 per CU) */
     int32_t n_resident;     /* the last n_resident LPs of `order` run with FP64 iterates */
     int32_t block_resident; /* FP64 part: threads per workgroup, 256 or 512 (0 = 256) */
@@ -323,6 +329,11 @@ typedef struct PdlpBatch {
     int32_t parts;          /* parts to launch: bit 0 FP32 part, bit 1 FP64 part (0 = both) */
     int32_t x_in_lds;       /* FP32 part: x, its partial sum, tau_hat and c in LDS as well
                                (then lds_bytes >= 28*n+8*m+8 of its LPs) */
+    int32_t lds_need;       /* FP32 part: the largest LDS need of its LPs (12*n+8*m+8, or
+                               28*n+8*m+8 with x_in_lds); the call returns PDLP_E_LIMIT
+                               when it exceeds lds_bytes */
+    int32_t lds_need_resident; /* FP64 part: the largest 20*n+12*m+8 of its LPs; PDLP_E_LIMIT
+                               when it exceeds lds_resident */
     double tol;
     const PdlpBatchLP* lps; /* count descriptors                              */
     const int32_t* order;   /* count LP indices, processing order: FP32 part, then FP64 part */
@@ -353,9 +364,11 @@ typedef struct PdlpBatch {
 
 /* Solve every LP of the batch: one persistent launch per part.  With both
  * parts the FP64 part runs concurrently on a second stream of the library's
- * (default priority, one per device, kept until the process ends), forked
- * from and joined back into `stream` through events, so work enqueued on
- * `stream` afterwards sees both parts finished. */
+ * (default priority, one per device and caller stream, kept until the process
+ * ends), forked from and joined back into `stream` through events, so work
+ * enqueued on `stream` afterwards sees both parts finished — also when the call
+ * returns an error after the fork.  Returns PDLP_E_LIMIT when a part's lds_need
+ * exceeds its LDS (the reference raises on bad input, pdhg_gpu.py:92). */
 int pdlp_batch_solve(const PdlpBatch* batch, void* stream);
 
 /* ----------------------------------------------------------------------------
@@ -401,6 +414,16 @@ typedef struct PdlpFamilyLP {
     double ref_err, err_prev, w_div;
     float weight;           /* primal weight: tau = w tau_hat, sigma = sigma_hat / w */
     float weight0;          /* starting weight; the adaptive weight stays within [w0/1e2, w0*1e4] */
+    float weight_cold;      /* weight of a cold start, taken when the warm start is dropped */
+    int32_t warm;           /* host: bit 0 = the LP starts from a warm start (x, y or weight set by
+                               the caller or pdlp_family_set_start), bit 1 = the start sets y, so
+                               its KKT (whose gap term is near 1 for any x with y = 0) is compared
+                               with the cold start's at init; the engine clears it once judged */
+    int32_t iter0;          /* iteration of the last cold restart by the failover (0 otherwise): the
+                               adaptive scheme's window-length rule counts from it */
+    int32_t reserved;
+    double start_err;       /* KKT of the warm start (set at pdlp_family_init) */
+    double cold_err;        /* KKT of the cold start clip(0, lo, hi), y = 0 (set at init for warm LPs) */
 } PdlpFamilyLP;
 
 typedef struct PdlpFamily {
@@ -417,7 +440,16 @@ typedef struct PdlpFamily {
     int32_t xcd_affinity;   /* 1: pin each tile's workgroups to its own XCD(s), so a tile's
                                gathered vectors stay in one L2 (workgroups are dispatched
                                round-robin over the 8 XCDs); 0: 2-D grid (chunk, tile) */
-    int32_t reserved;
+    int32_t failover_iters; /* warm-start failover (0 = off; the reference's
+                               WARMSTART_FAILOVER_ITERS = 200, solver/config.py:75): for each LP
+                               with lp.warm set, pdlp_family_init evaluates the KKT of the warm
+                               start and of the cold start, and drops a start that sets y (warm
+                               bit 1) when the cold start is better (or the warm KKT is NaN); at
+                               the first check at or after
+                               failover_iters iterations, a warm-started LP whose KKT has not
+                               fallen below its starting value and is at least a tenth of the
+                               cold start's restarts from the cold start (weight_cold, sums
+                               cleared; iter0 = that iteration) */
     double tol;
     /* K and K^T, CSR, FP32 values */
     const int32_t* row_ptr; const int32_t* col_idx; const float* val;

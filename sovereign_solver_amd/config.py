@@ -12,7 +12,10 @@ PDHG_TOL: float = 1e-6            # relative KKT tolerance of solve_lp_pdhg
 MAX_PDHG_ITERS: int = 100_000
 PDHG_RESTART_PERIOD: int = 50     # KKT of the window average is evaluated every 50 iterations
 PDHG_OVER_RELAXATION: float = 1.6  # unused, as in the reference
-WARMSTART_FAILOVER_ITERS: int = 200  # a warm start that has not restarted by then is dropped
+WARMSTART_FAILOVER_ITERS: int = 200  # a warm start no better than the cold start is dropped at once;
+                                     # one whose KKT has not fallen, and is still at least a tenth of
+                                     # the cold start's, by the first check at or after this many
+                                     # iterations restarts cold (family and single engines)
 
 # check cadence the reference derives from the restart period: max(10, min(P, 100))
 CHECK_INTERVAL: int = max(10, min(PDHG_RESTART_PERIOD, 100))

@@ -39,6 +39,7 @@
 
 #include <algorithm>
 #include <map>
+#include <string>
 #include <mutex>
 #include <type_traits>
 
@@ -680,9 +681,16 @@ __global__ void __launch_bounds__(NT) batch_kernel(PdlpBatch a, int first, int c
         const int k = __builtin_amdgcn_readfirstlane(a.order[first + q]);
         const int m = __builtin_amdgcn_readfirstlane(a.lps[k].m);
         const int n = __builtin_amdgcn_readfirstlane(a.lps[k].n);
-        // an LP that does not fit the launch's LDS is not run (its output record
-        // stays as it was); the packer sizes the LDS for every LP of the part
-        if (NB * n + MB * m + 8 <= lds_limit) solve_one<NT, T, XR, SC>(a, k, lds, red);
+        // an LP that does not fit the launch's LDS is not run and says so (the host
+        // checked the part's largest need, so only inconsistent descriptors get here)
+        if (NB * n + MB * m + 8 <= lds_limit) {
+            solve_one<NT, T, XR, SC>(a, k, lds, red);
+        } else if (threadIdx.x == 0) {
+            PdlpBatchOut o = {};
+            o.status = PDLP_SKIPPED;
+            o.rel_kkt = NAN;
+            a.out[k] = o;
+        }
     }
 }
 
@@ -721,10 +729,12 @@ struct SideStream {
     hipEvent_t fork = nullptr, join = nullptr;
 };
 std::mutex g_side_mu;
-std::map<int, SideStream> g_side;
+std::map<std::pair<int, hipStream_t>, SideStream> g_side;
 
-int side_stream(int dev, SideStream& out) {
-    SideStream& ss = g_side[dev];
+// One side stream per (device, caller stream): independent callers' FP64 parts do
+// not queue behind each other.
+int side_stream(int dev, hipStream_t caller, SideStream& out) {
+    SideStream& ss = g_side[{dev, caller}];
     if (ss.s == nullptr) {
         if (int rc = hip_fail(hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking), "hipStreamCreate")) return rc;
         if (int rc = hip_fail(hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming), "hipEventCreate")) return rc;
@@ -753,14 +763,27 @@ extern "C" int pdlp_batch_solve(const PdlpBatch* a, void* stream) {
     if ((a->row_scale == nullptr) != (a->col_scale == nullptr))
         return fail(PDLP_E_ARG, "row_scale and col_scale must both be set or both be NULL");
     if (a->check_every <= 0 || a->max_iters < 0) return fail(PDLP_E_ARG, "check_every > 0 and max_iters >= 0");
+    const int n32 = a->count - a->n_resident;
+    const int parts = a->parts ? a->parts : 3;
+    if ((parts & 1) && n32 > 0) {
+        if (a->lds_need <= 0) return fail(PDLP_E_ARG, "lds_need must be set for the FP32 part");
+        if (a->lds_need > a->lds_bytes)
+            return fail(PDLP_E_LIMIT, "an LP of the FP32 part needs " + std::to_string(a->lds_need) +
+                                          " bytes of LDS, more than lds_bytes = " + std::to_string(a->lds_bytes));
+    }
+    if ((parts & 2) && a->n_resident > 0) {
+        if (a->lds_need_resident <= 0) return fail(PDLP_E_ARG, "lds_need_resident must be set for the FP64 part");
+        if (a->lds_need_resident > a->lds_resident)
+            return fail(PDLP_E_LIMIT, "an LP of the FP64 part needs " + std::to_string(a->lds_need_resident) +
+                                          " bytes of LDS, more than lds_resident = " +
+                                          std::to_string(a->lds_resident));
+    }
     hipStream_t s = as_stream(stream);
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
         int v = 0;
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0) cus = v;
     }
-    const int n32 = a->count - a->n_resident;
-    const int parts = a->parts ? a->parts : 3;
     auto part32 = [&](hipStream_t st) {
         return a->x_in_lds ? launch_part<float, true>(a, 0, n32, a->queue, a->block, a->lds_bytes, a->grid, cus, st)
                            : launch_part<float, false>(a, 0, n32, a->queue, a->block, a->lds_bytes, a->grid, cus, st);
@@ -775,13 +798,21 @@ extern "C" int pdlp_batch_solve(const PdlpBatch* a, void* stream) {
         // each call's record/wait pairs together when host threads share a device.
         std::lock_guard<std::mutex> lock(g_side_mu);
         SideStream ss;
-        if (int rc = side_stream(dev, ss)) return rc;
+        if (int rc = side_stream(dev, s, ss)) return rc;
         if (int rc = hip_fail(hipEventRecord(ss.fork, s), "hipEventRecord")) return rc;
         if (int rc = hip_fail(hipStreamWaitEvent(ss.s, ss.fork, 0), "hipStreamWaitEvent")) return rc;
-        if (int rc = part64(ss.s)) return rc;
-        if (int rc = hip_fail(hipEventRecord(ss.join, ss.s), "hipEventRecord")) return rc;
-        if (int rc = part32(s)) return rc;
-        return hip_fail(hipStreamWaitEvent(s, ss.join, 0), "hipStreamWaitEvent");
+        // once forked, the caller's stream always waits for the side stream, also after an
+        // error, so the caller never frees buffers that a running FP64 part still reads
+        int rc = part64(ss.s);
+        const int rc_rec = hip_fail(hipEventRecord(ss.join, ss.s), "hipEventRecord");
+        if (rc == 0 && rc_rec == 0) rc = part32(s);
+        if (rc_rec != 0) {
+            // no join event: wait here for the side stream's work instead
+            (void)hipStreamSynchronize(ss.s);
+            return rc ? rc : rc_rec;
+        }
+        const int rc_wait = hip_fail(hipStreamWaitEvent(s, ss.join, 0), "hipStreamWaitEvent");
+        return rc ? rc : rc_wait;
     }
     if (parts & 1) {
         if (int rc = part32(s)) return rc;

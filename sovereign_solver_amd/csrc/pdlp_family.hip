@@ -38,7 +38,13 @@ constexpr int KCH = PDLP_FAM_KKT_CHUNKS;
 
 static_assert(T == kWave, "one tile per wavefront");
 
-enum Action : int32_t { ACT_NONE = 0, ACT_RESTART_AVG = 1, ACT_RESTART_CUR = 2, ACT_DONE_AVG = 3, ACT_DONE_CUR = 4 };
+// ACT_COLD: restart from the cold start (x = clip(0, lo, hi), y = 0), window sums cleared.
+enum Action : int32_t {
+    ACT_NONE = 0, ACT_RESTART_AVG = 1, ACT_RESTART_CUR = 2, ACT_DONE_AVG = 3, ACT_DONE_CUR = 4, ACT_COLD = 5
+};
+
+// The cold start's x_j = clip(0, lo_j, hi_j) for the lane's LP.
+__device__ __forceinline__ float cold_x(const PdlpFamily& f, int tile, int j, int lane);
 
 __device__ __forceinline__ size_t at(int tile, int len, int d, int lane) {
     return ((size_t)tile * len + d) * T + lane;
@@ -47,6 +53,13 @@ __device__ __forceinline__ size_t at(int tile, int len, int d, int lane) {
 template <typename V>
 __device__ __forceinline__ V ldv(const V* p, bool shared, int tile, int len, int d, int lane) {
     return shared ? p[d] : p[at(tile, len, d, lane)];
+}
+
+__device__ __forceinline__ float cold_x(const PdlpFamily& f, int tile, int j, int lane) {
+    const float lo = ldv(f.lo, f.shared & PDLP_FAM_SHARED_LO, tile, f.n, j, lane);
+    const float hi = ldv(f.hi, f.shared & PDLP_FAM_SHARED_HI, tile, f.n, j, lane);
+    float v = 0.f < lo ? lo : 0.f;
+    return v > hi ? hi : v;
 }
 
 // wavefront index inside the workgroup, as a uniform value
@@ -564,11 +577,29 @@ __global__ void __launch_bounds__(NSC * T) decide_kernel(PdlpFamily f, int mode,
             const Kkt kc = kkt_of(sum + NSC, s.norm_b, s.norm_c);
             s.action = ACT_NONE;
             if (mode == 0) {
-                s.ref_err = kc.err;
+                Kkt k0 = kc;
+                bool cold = false;
+                if (s.warm && f.failover_iters > 0) {
+                    // the average slot held the cold start (init_vec_kernel): keep the warm start
+                    // unless the cold point is better (a NaN KKT loses; a tie, e.g. a start that
+                    // only sets the primal weight, keeps it)
+                    s.start_err = kc.err;
+                    s.cold_err = ka.err;
+                    cold = (s.warm & 2) && !(kc.err <= ka.err);
+                    if (cold) {
+                        k0 = ka;
+                        s.warm = 0;
+                        s.weight = s.weight_cold;
+                        s.weight0 = s.weight_cold;
+                    }
+                    s.window = 0;
+                    s.action = cold ? ACT_COLD : ACT_RESTART_CUR;   // clear the sums either way
+                }
+                s.ref_err = k0.err;
                 s.err_prev = INFINITY;
-                s.status = kc.infeasible ? PDLP_INFEASIBLE : isfinite(kc.err) ? PDLP_RUNNING : PDLP_NUMERICAL;
-                if (s.status != PDLP_RUNNING) s.action = ACT_DONE_CUR;
-                write_out(&f.out[k], s, kc, 0);
+                s.status = k0.infeasible ? PDLP_INFEASIBLE : isfinite(k0.err) ? PDLP_RUNNING : PDLP_NUMERICAL;
+                if (s.status != PDLP_RUNNING) s.action = cold ? ACT_COLD : ACT_DONE_CUR;
+                write_out(&f.out[k], s, k0, 0);
             } else {
                 const int it = s.iter + f.check_every;
                 s.iter = it;
@@ -600,7 +631,7 @@ __global__ void __launch_bounds__(NSC * T) decide_kernel(PdlpFamily f, int mode,
                         restart = ka.err < 0.5 * s.ref_err;                  // pdhg.py:238-244
                     } else {
                         restart = ka.err <= 0.2 * s.ref_err || (ka.err <= 0.8 * s.ref_err && ka.err > s.err_prev) ||
-                                  (double)(it - s.last_restart) >= 0.36 * (double)it;
+                                  (double)(it - s.last_restart) >= 0.36 * (double)(it - s.iter0);
                     }
                     if (restart) {
                         if (adaptive && !fixed_weight && ka.dx > 1e-10 && ka.dy > 1e-10) {
@@ -616,6 +647,23 @@ __global__ void __launch_bounds__(NSC * T) decide_kernel(PdlpFamily f, int mode,
                         s.action = use_avg ? ACT_RESTART_AVG : ACT_RESTART_CUR;
                     } else {
                         s.err_prev = ka.err;
+                    }
+                    // warm-start failover, judged once at the first check past failover_iters: a
+                    // start that has made no progress and is still far out (KKT at least a tenth of
+                    // the cold start's; one stalled near the tolerance is kept) solves cold from here
+                    if (s.warm && f.failover_iters > 0 && it >= f.failover_iters) {
+                        if (!(ka.err < s.start_err) && !(ka.err < 0.1 * s.cold_err)) {
+                            s.weight = s.weight_cold;
+                            s.weight0 = s.weight_cold;
+                            s.ref_err = s.cold_err;
+                            s.err_prev = INFINITY;
+                            s.last_restart = it;
+                            s.iter0 = it;
+                            ++s.restarts;
+                            s.window = 0;
+                            s.action = ACT_COLD;
+                        }
+                        s.warm = 0;
                     }
                 }
                 write_out(&f.out[k], s, ka, s.status == PDLP_RUNNING ? it : s.done_iter);
@@ -665,6 +713,14 @@ __global__ void __launch_bounds__(NT) apply_kernel(PdlpFamily f) {
             case ACT_DONE_CUR:
                 sum[ix] = (double)v[ix];
                 break;
+            case ACT_COLD: {
+                const float nv = col ? cold_x(f, tile, d, lane) : 0.f;
+                v[ix] = nv;
+                last[ix] = nv;
+                // a start that ends the solve at once (status set at init) keeps it as solution
+                sum[ix] = f.lp[k].status == PDLP_RUNNING ? 0.0 : (double)nv;
+                break;
+            }
             default:
                 break;
         }
@@ -692,10 +748,14 @@ __global__ void __launch_bounds__(NT) init_vec_kernel(PdlpFamily f) {
     const int per = 16;
     const int d0 = (blockIdx.x * NWV + w) * per;
     const int d1 = min(d0 + per, f.n + f.m);
+    // LPs with a warm start and the failover on hold the cold start in their sums with a
+    // window of 1 (init_lp_kernel), so the initial check evaluates it as the second candidate
+    const int k = tile * T + lane;
+    const bool judge = f.failover_iters > 0 && k < f.count && f.lp[k].warm;
     for (int d = d0; d < d1; ++d) {
         if (d < f.n) {
             const size_t ix = at(tile, f.n, d, lane);
-            f.x_sum[ix] = 0.0;
+            f.x_sum[ix] = judge ? (double)cold_x(f, tile, d, lane) : 0.0;
             f.x_psum[ix] = 0.f;
             f.x_last[ix] = f.x[ix];
             f.xbar[ix] = f.x[ix];
@@ -715,7 +775,8 @@ __global__ void __launch_bounds__(T) init_lp_kernel(PdlpFamily f) {
     if (valid) {
         PdlpFamilyLP s = f.lp[k];
         s.status = PDLP_RUNNING;
-        s.iter = s.window = s.restarts = s.checks = s.done_iter = s.last_restart = 0;
+        s.iter = s.window = s.restarts = s.checks = s.done_iter = s.last_restart = s.iter0 = 0;
+        if (f.failover_iters > 0 && s.warm) s.window = 1;   // the cold start in the sums (init_vec_kernel)
         s.action = ACT_NONE;
         s.ref_err = INFINITY;
         s.err_prev = INFINITY;

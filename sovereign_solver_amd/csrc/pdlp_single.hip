@@ -299,6 +299,7 @@ __global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d) {
 
 constexpr int NP = 16;
 constexpr int NP_BASE = 10;   // the sums of a check without the unboundedness test
+constexpr double kRayEps = 1e-6;   // normalised violation a ray of decrease must reach
 
 __device__ __forceinline__ double* part_ptr(const PdlpDev& d, int src, int part, int blk) {
     return d.partials + ((size_t)(src * 2 + part) * PDLP_PARTIAL_BLOCKS + blk) * NP;
@@ -487,9 +488,15 @@ __device__ KktVal kkt_value(const double (&s)[NP], const PdlpCtrl* ctrl) {
     // the rows' violation is also small against their terms, |viol(K d)| <= 1e-3 |(|K||d|)|
     // (K d cancels, as along a ray and unlike the early iterations' moves), else 1e6: a
     // ray on columns that no row holds has no terms to cancel.
+    // Second, and independent of the iterate's duals: the violations of d itself must be
+    // below kRayEps of the decrease, (|viol(K d)| + |viol_bounds(d)|) <= kRayEps (-c'd).  By
+    // the same bound a bounded LP passes this only when every dual solution has
+    // |y*| + |r*| >= 1 / kRayEps, so LPs whose optimum is merely far away (small row
+    // coefficients, large but finite duals) keep running while their iterates travel.
     const double cd = s[12], vr = sqrt(s[10]), vb = sqrt(s[14]);
     const double lhs = vr * (1.0 + sqrt(s[13])) + vb * (1.0 + sqrt(s[15]));
-    k.unbounded = isfinite(cd) && cd < 0.0 && lhs <= (vr <= 1e-3 * sqrt(s[11]) ? 1e-3 : 1e-6) * -cd;
+    k.unbounded = isfinite(cd) && cd < 0.0 && lhs <= (vr <= 1e-3 * sqrt(s[11]) ? 1e-3 : 1e-6) * -cd &&
+                  vr + vb <= kRayEps * -cd;
     return k;
 }
 
@@ -895,7 +902,7 @@ int launch_kkt_t(const PdlpDev& d, const PdlpTune& t, int mode, hipStream_t s) {
     // candidates: the initial point is "current"; a reference check uses the
     // average only; an adaptive check evaluates both
     const int cands = mode == 0 ? 1 : (t.scheme != PDLP_SCHEME_REFERENCE ? 3 : 2);
-    const bool check_ray = mode == 1 && t.scheme != PDLP_SCHEME_REFERENCE;
+    const bool check_ray = mode == 1 && t.scheme != PDLP_SCHEME_REFERENCE && !t.no_ray;
     for (int src = 0; src < 2; ++src) {
         if (!(cands & (1 << src))) continue;
         // the unboundedness test rides on the average's launches of an adaptive check

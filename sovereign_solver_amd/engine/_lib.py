@@ -13,9 +13,10 @@ import threading
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "_lib", "libpdlp_hip.so")
 
-PDLP_ABI_VERSION = 7
+PDLP_ABI_VERSION = 8
 PDLP_F32, PDLP_F64 = 0, 1
-PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL, PDLP_INFEASIBLE, PDLP_UNBOUNDED = 0, 1, 3, 4, 5
+PDLP_E_ARG, PDLP_E_TUNE, PDLP_E_SHAPE, PDLP_E_LIMIT = -1, -2, -3, -4
+PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL, PDLP_INFEASIBLE, PDLP_UNBOUNDED, PDLP_SKIPPED = 0, 1, 3, 4, 5, 6
 PDLP_ROW_EQ, PDLP_ROW_LE = 0, 1
 PDLP_PARTIAL_BLOCKS = 1024
 PDLP_PARTIALS_LEN = 4 * PDLP_PARTIAL_BLOCKS * 16
@@ -63,7 +64,7 @@ class PdlpTune(C.Structure):
     _fields_ = [
         ("tpr_primal", C.c_int32), ("tpr_dual", C.c_int32), ("block", C.c_int32),
         ("check_every", C.c_int32), ("xcd_remap", C.c_int32), ("exact", C.c_int32), ("block_dual", C.c_int32),
-        ("scheme", C.c_int32), ("sell", C.c_int32), ("reserved", C.c_int32),
+        ("scheme", C.c_int32), ("sell", C.c_int32), ("no_ray", C.c_int32),
     ]
 
 
@@ -72,6 +73,7 @@ class PdlpBatch(C.Structure):
         ("count", C.c_int32), ("check_every", C.c_int32), ("max_iters", C.c_int32), ("block", C.c_int32),
         ("lds_bytes", C.c_int32), ("grid", C.c_int32), ("n_resident", C.c_int32), ("block_resident", C.c_int32),
         ("lds_resident", C.c_int32), ("grid_resident", C.c_int32), ("parts", C.c_int32), ("x_in_lds", C.c_int32),
+        ("lds_need", C.c_int32), ("lds_need_resident", C.c_int32),
         ("tol", C.c_double),
         ("lps", C.c_void_p), ("order", C.c_void_p), ("queue", C.c_void_p),
         ("row_ptr", C.c_void_p), ("col_idx", C.c_void_p), ("val", C.c_void_p),
@@ -92,6 +94,8 @@ class PdlpFamilyLP(C.Structure):
         ("checks", C.c_int32), ("done_iter", C.c_int32), ("last_restart", C.c_int32), ("action", C.c_int32),
         ("norm_b", C.c_double), ("norm_c", C.c_double), ("ref_err", C.c_double), ("err_prev", C.c_double),
         ("w_div", C.c_double), ("weight", C.c_float), ("weight0", C.c_float),
+        ("weight_cold", C.c_float), ("warm", C.c_int32), ("iter0", C.c_int32), ("reserved", C.c_int32),
+        ("start_err", C.c_double), ("cold_err", C.c_double),
     ]
 
 
@@ -99,7 +103,8 @@ class PdlpFamily(C.Structure):
     _fields_ = [
         ("m", C.c_int32), ("n", C.c_int32), ("count", C.c_int32), ("tiles", C.c_int32),
         ("check_every", C.c_int32), ("flags", C.c_int32), ("shared", C.c_int32), ("items_per_wave", C.c_int32),
-        ("long_rows", C.c_int32), ("long_cols", C.c_int32), ("xcd_affinity", C.c_int32), ("reserved", C.c_int32),
+        ("long_rows", C.c_int32), ("long_cols", C.c_int32), ("xcd_affinity", C.c_int32),
+        ("failover_iters", C.c_int32),
         ("tol", C.c_double),
         ("row_ptr", _p), ("col_idx", _p), ("val", _p), ("t_row_ptr", _p), ("t_col_idx", _p), ("t_val", _p),
         ("row_kind", _p), ("tau_hat", _p), ("sigma_hat", _p), ("row_metric", _p), ("row_orig", _p),

@@ -40,7 +40,7 @@ OUT_DTYPE = np.dtype([
 
 
 _STATUS = {L.PDLP_RUNNING: "iteration_limit", L.PDLP_OPTIMAL: "optimal", L.PDLP_NUMERICAL: "numerical_error",
-           L.PDLP_INFEASIBLE: "infeasible"}
+           L.PDLP_INFEASIBLE: "infeasible", L.PDLP_SKIPPED: "skipped"}
 
 
 ENTRIES_PER_LANE = 8     # entries each lane loads at once in a pass (EL in pdlp_batch.hip)
@@ -220,14 +220,21 @@ class PackedBatch:
         # FP32 part: x, its partial sum, tau_hat and c in LDS too when every LP of the part
         # fits (one 512-thread workgroup per CU leaves the CU's whole LDS to it)
         fp32 = ~self.resident
-        self.x_in_lds = bool(fp32.any() and lds_bytes_x_in_lds(ms, ns)[fp32].max() <= 160 * 1024)
-        if not fp32.any():
-            self.lds_bytes = 16
-        elif self.x_in_lds:
-            self.lds_bytes = r16(lds_bytes_x_in_lds(ms, ns)[fp32].max())
-        else:
-            self.lds_bytes = r16(lds_bytes_of(ms, ns)[fp32].max())
+        self.set_x_in_lds(bool(fp32.any() and lds_bytes_x_in_lds(ms, ns)[fp32].max() <= 160 * 1024))
         self.lds_resident = r16(lds_bytes_resident(ms, ns)[self.resident].max()) if self.n_resident else 16
+        # the largest need of the FP64 part, checked by pdlp_batch_solve against its LDS
+        self.lds_need_resident = int(lds_bytes_resident(ms, ns)[self.resident].max()) if self.n_resident else 0
+
+    def set_x_in_lds(self, on: bool) -> None:
+        """FP32 part layout: x, its partial sum, tau_hat and c in LDS (``on``) or in global
+        memory; sets the part's LDS per workgroup and its largest need to match."""
+        fp32 = ~self.resident
+        need = (lds_bytes_x_in_lds if on else lds_bytes_of)(self.ms, self.ns)
+        if on and fp32.any() and need[fp32].max() > 160 * 1024:
+            raise ValueError("an LP of the FP32 part does not fit in 160 KiB with x in LDS")
+        self.x_in_lds = bool(on)
+        self.lds_need = int(need[fp32].max()) if fp32.any() else 0
+        self.lds_bytes = int((self.lds_need + 15) // 16 * 16) if fp32.any() else 16
 
     def algorithmic_bytes_per_iteration(self) -> float:
         """Bytes one iteration of every LP moves at minimum: K and K^T once each
@@ -320,7 +327,8 @@ class BatchEngine:
         a = PdlpBatch(count=len(self.pk.problems), check_every=check_every, max_iters=max_iters, block=self.block,
                       lds_bytes=self.pk.lds_bytes, grid=self.grid, n_resident=self.pk.n_resident,
                       block_resident=self.block_resident, lds_resident=self.pk.lds_resident, grid_resident=0,
-                      x_in_lds=int(self.pk.x_in_lds), tol=tol)
+                      x_in_lds=int(self.pk.x_in_lds), lds_need=self.pk.lds_need,
+                      lds_need_resident=self.pk.lds_need_resident, tol=tol)
         for name in ("lps", "order", "queue", "row_ptr", "col_idx", "val", "t_col_ptr", "t_row_idx", "t_val",
                      "c", "lo", "hi", "c64", "lo_shift",
                      "ub_span", "b", "row_kind", "b64", "b_shift", "x", "x_psum", "x_sum", "tau", "y", "y_sum",

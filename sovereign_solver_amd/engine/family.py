@@ -22,6 +22,7 @@ from . import _lib as L
 from .batch import OUT_DTYPE, _STATUS
 from .device import dev_tensor, ptr, require_gpu, stream_handle
 from .forms import ROW_EQ, ROW_LE
+from ..config import WARMSTART_FAILOVER_ITERS
 from ..problem import Problem
 
 TILE = L.PDLP_FAM_TILE
@@ -203,7 +204,7 @@ class FamilyEngine:
 
     def __init__(self, fb: FamilyBatch, scheme: str = "adaptive", check_every: int = 50,
                  items_per_wave: int = 1, use_graphs: bool = True, compaction: bool = True,
-                 xcd_affinity: bool = False):
+                 xcd_affinity: bool = False, failover_iters: int = WARMSTART_FAILOVER_ITERS):
         flags = {"reference": 0, "adaptive": 1, "adaptive_fixed": 3, "adaptive_start": 5}
         if scheme not in flags:
             raise ValueError("scheme must be 'adaptive', 'adaptive_fixed', 'adaptive_start' or 'reference'")
@@ -261,7 +262,8 @@ class FamilyEngine:
         self.compactions = 0
         self.args = L.PdlpFamily(m=m, n=n, count=B, tiles=fb.tiles, check_every=check_every, flags=self.flags,
                                  shared=fb.shared, items_per_wave=items_per_wave, long_rows=fb.long_rows,
-                                 long_cols=fb.long_cols, xcd_affinity=int(xcd_affinity), tol=0.0)
+                                 long_cols=fb.long_cols, xcd_affinity=int(xcd_affinity),
+                                 failover_iters=int(failover_iters), tol=0.0)
         for name, _ in L.PdlpFamily._fields_[13:]:
             setattr(self.args, name, ptr(t[name]))
         self._graphs = {}
@@ -318,10 +320,14 @@ class FamilyEngine:
     LP_DTYPE = np.dtype([(n_, "<i4") for n_ in ("status", "iter", "window", "restarts", "checks", "done_iter",
                                                 "last_restart", "action")] +
                         [(n_, "<f8") for n_ in ("norm_b", "norm_c", "ref_err", "err_prev", "w_div")] +
-                        [("weight", "<f4"), ("weight0", "<f4")])
+                        [("weight", "<f4"), ("weight0", "<f4"), ("weight_cold", "<f4"), ("warm", "<i4"),
+                         ("iter0", "<i4"), ("reserved", "<i4"), ("start_err", "<f8"), ("cold_err", "<f8")])
 
-    def _reset(self, tol: float, weight: Optional[np.ndarray]) -> None:
-        """Slot bookkeeping and per-LP records for a new solve (original slot order)."""
+    def _reset(self, tol: float, weight: Optional[np.ndarray], warm=False) -> None:
+        """Slot bookkeeping and per-LP records for a new solve (original slot order).
+        ``warm`` (one value, or one per LP): bit 0 the LP starts from a caller's point or
+        weight, which the warm-start failover judges; bit 1 that start sets y and is compared
+        with the cold start at init."""
         fb = self.fb
         self.set_tol(tol)
         if self.compactions or self.args.tiles != fb.tiles:
@@ -331,13 +337,16 @@ class FamilyEngine:
             self.drop_graphs()
         self.slot_lp = np.concatenate([np.arange(fb.count), np.full(fb.tiles * TILE - fb.count, -1)])
         self.saved = np.zeros(fb.count, bool)
-        w = fb.init_weight.copy() if self.flags & 1 else np.ones(fb.count)
+        w_cold = fb.init_weight.copy() if self.flags & 1 else np.ones(fb.count)
+        w = w_cold
         if weight is not None:
             ww = np.asarray(weight, float)
-            w = np.where(np.isfinite(ww), ww, w)
+            w = np.where(np.isfinite(ww) & (ww > 0), ww, w_cold)
         lp = np.zeros(fb.count, dtype=self.LP_DTYPE)
         assert lp.dtype.itemsize == C.sizeof(L.PdlpFamilyLP)
         lp["norm_b"], lp["norm_c"], lp["weight"], lp["weight0"] = fb.norm_b, fb.norm_c, w, w
+        lp["weight_cold"] = w_cold
+        lp["warm"] = np.broadcast_to(np.asarray(warm, np.int32), (fb.count,))
         with torch.cuda.stream(self.stream):
             self.t["lp"].copy_(torch.from_numpy(lp.view(np.uint8).copy()).to(self.device))
 
@@ -347,7 +356,18 @@ class FamilyEngine:
         y = 0) and per-LP primal weights (None or NaN entries: 1 for the reference
         scheme, the norm ratio for the adaptive ones)."""
         fb = self.fb
-        self._reset(tol, weight)
+        # an LP starts warm when the caller gives it a point other than the cold start or a
+        # weight (bit 0), and is judged against the cold start at init when it sets y (bit 1)
+        warm = np.zeros(fb.count, np.int32)
+        if weight is not None:
+            warm |= np.isfinite(np.asarray(weight, float)).astype(np.int32)
+        if x0 is not None:
+            xd = fb.to_device_cols(np.asarray(x0, float).reshape(fb.count, fb.n))
+            warm |= (np.clip(xd, fb.rows_of("lo"), fb.rows_of("hi")) != np.clip(0.0, fb.rows_of("lo"),
+                                                                                fb.rows_of("hi"))).any(axis=1)
+        if y0 is not None:
+            warm |= 3 * (np.asarray(y0, float).reshape(fb.count, fb.m) != 0.0).any(axis=1).astype(np.int32)
+        self._reset(tol, weight, warm=warm)
         dv = self.device
         if x0 is None and y0 is None:           # cold start written on the device
             L.check(self.lib.pdlp_family_set_start(C.byref(self.args), None, 0, -1, -1, -1, None, None, self.sh),
@@ -382,7 +402,7 @@ class FamilyEngine:
         for t_ in (scale, shift):
             if t_ is not None and (t_.dtype != torch.float32 or t_.numel() != d or not t_.is_contiguous()):
                 raise ValueError("scale / shift must be contiguous float32 vectors of length d")
-        self._reset(tol, None)
+        self._reset(tol, None, warm=(1 if x_off >= 0 or y_off >= 0 or w_col >= 0 else 0) | (2 if y_off >= 0 else 0))
         if ready is not None:
             self.stream.wait_event(ready)
         else:

@@ -20,6 +20,7 @@ import torch
 from . import _lib as L
 from . import sell
 from .device import csr_transpose, dev_tensor, ptr, require_gpu, stream_handle, struct_from_bytes
+from ..config import WARMSTART_FAILOVER_ITERS
 from .forms import HostForm
 from .tune import Tune, heuristic_tune
 
@@ -179,6 +180,16 @@ class SingleEngine:
 
     # ------------------------------------------------------------------ solve
     def start(self, tol: float, x0=None, y0=None, tau_scale: float = 1.0, sigma_scale: float = 1.0) -> None:
+        """State for a solve from (x0, y0) (None: the cold start); non-finite entries of a
+        start count as the cold start's (x = clip(0, lo, hi), y = 0)."""
+        if x0 is not None:
+            x0 = np.asarray(x0, float)
+            if not np.all(np.isfinite(x0)):
+                cold = np.zeros(self.n) if self.lo is None else np.clip(0.0, self.form.lo, self.form.hi)
+                x0 = np.where(np.isfinite(x0), x0, cold)
+        if y0 is not None:
+            y0 = np.asarray(y0, float)
+            y0 = np.where(np.isfinite(y0), y0, 0.0)
         with torch.cuda.stream(self.stream):
             if x0 is None and self.lo is None:
                 self.x.zero_()
@@ -285,19 +296,76 @@ class SingleEngine:
         self.stream.synchronize()
         return d.numpy()
 
-    def solve(self, tol: float, max_iters: int, x0=None, y0=None, step_scale: Optional[float] = None):
-        """One solve.  ``step_scale`` is the starting primal weight w (tau = w tau_hat,
-        sigma = sigma_hat / w); None = 1 for the reference scheme, the norm ratio of
-        :meth:`initial_weight` for the adaptive one."""
-        if step_scale is None:
-            step_scale = self.initial_weight() if self.tune.scheme != L.PDLP_SCHEME_REFERENCE else 1.0
-        self.start(tol, x0, y0, float(step_scale), 1.0 / float(step_scale))
-        st = self.run_iterations(max_iters)
+    def _result(self, st: L.PdlpCtrl) -> dict:
         x, y = self.solution(st)
-        iters = st.done_iter if st.status != L.PDLP_RUNNING else st.iter
+        iters = (st.done_iter if st.status != L.PDLP_RUNNING else st.iter) + getattr(self, "_offset", 0)
         ray = self.ray(st) if st.status == L.PDLP_UNBOUNDED else None
         return dict(status=_STATUS[st.status], x=x, y=y, iterations=int(iters), rel_kkt=st.last_err,
                     rp=st.rp, rd=st.rd, gap=st.gap, restarts=int(st.restarts), ctrl=st, ray=ray)
+
+    def solve(self, tol: float, max_iters: int, x0=None, y0=None, step_scale: Optional[float] = None,
+              failover_iters: int = WARMSTART_FAILOVER_ITERS):
+        """One solve.  ``step_scale`` is the starting primal weight w (tau = w tau_hat,
+        sigma = sigma_hat / w); None = 1 for the reference scheme, the norm ratio of
+        :meth:`initial_weight` for the adaptive one.
+
+        Warm-start failover (``failover_iters`` > 0, the reference's
+        WARMSTART_FAILOVER_ITERS): a start given by ``x0``, ``y0`` or ``step_scale`` is
+        kept only when its KKT is no worse than the cold start's (when it sets y), and a
+        solve whose KKT
+        has not fallen below the warm start's by the first check at or after
+        ``failover_iters`` iterations, and is at least a tenth of the cold start's, starts
+        again from the cold start; the iterations spent before count in the result."""
+        w_cold = self.initial_weight() if self.tune.scheme != L.PDLP_SCHEME_REFERENCE else 1.0
+        w = w_cold if step_scale is None or not np.isfinite(step_scale) or step_scale <= 0 else float(step_scale)
+        self.max_iters = int(max_iters)
+        self._offset = 0
+        judge = failover_iters > 0 and (x0 is not None or y0 is not None or step_scale is not None)
+        if not judge:
+            self.start(tol, x0, y0, w, 1.0 / w)
+            return self._result(self.run_iterations(max_iters))
+        self.start(tol, None, None, w_cold, 1.0 / w_cold)
+        cold_err = self.read_ctrl().ref_err
+        self.start(tol, x0, y0, w, 1.0 / w)
+        st0 = self.read_ctrl()
+        start_err = st0.ref_err
+        # a start that sets y competes with the cold point at once (with y = 0 the KKT's gap term
+        # is near 1 for any x, so x-only starts are judged by their progress only); a start whose
+        # KKT is not finite never runs
+        if st0.status != L.PDLP_RUNNING or (y0 is not None and not start_err <= cold_err):
+            self.start(tol, None, None, w_cold, 1.0 / w_cold)
+            return self._result(self.run_iterations(max_iters))
+        ce = self.tune.check_every
+        first = min(int(max_iters), -(-int(failover_iters) // ce) * ce)
+        st = self.run_iterations(first)
+        if st.status == L.PDLP_RUNNING and first < max_iters:
+            # no progress from the warm start and still far out (one stalled near the tolerance is
+            # kept): fail over
+            if not st.last_err < start_err and not st.last_err < 0.1 * cold_err:
+                self._offset = int(st.iter)
+                self.start(tol, None, None, w_cold, 1.0 / w_cold)
+                left = int(max_iters) - self._offset
+            else:                                        # keep going from where the run stands
+                left = int(max_iters) - int(st.iter)
+            st = self.run_iterations(left)
+        return self._result(st)
+
+    def resume_without_ray_test(self) -> dict:
+        """Continue a solve that stopped ``unbounded`` as if its ray had not passed:
+        status back to running, the unboundedness test off for the rest of the run, and
+        iterations up to the solve's ``max_iters``.  The kernels froze the state when the
+        status left running (iterate, window sums, restart point), so the run picks up
+        exactly where it stopped."""
+        st = self.read_ctrl()
+        if st.status != L.PDLP_UNBOUNDED:
+            raise RuntimeError("resume_without_ray_test: the last solve did not stop unbounded")
+        st.status = L.PDLP_RUNNING
+        st.done_iter = 0
+        self.write_ctrl(st)
+        self.set_tune(self.tune.with_(no_ray=1))
+        left = self.max_iters - getattr(self, "_offset", 0) - int(st.iter)
+        st = self.run_iterations(left) if left > 0 else self.read_ctrl()
+        return self._result(st)
 
     # --------------------------------------------------------------- timing
     def time_kernel(self, which: str, t: Tune, reps: int = 20) -> float:

@@ -37,11 +37,12 @@ class Tune:
     sell: int = 0               # SELL-64 dual step: 0 off, 1 a wavefront per slice, 2..7 one launch with a
                                 # workgroup per CSR row and per slice piece (threads/rows: 2 1024/64, 3 512/64,
                                 # 4 1024/16, 5 1024/8, 6 1024/32, 7 512/16), k >= 8 a wavefront per k entries
+    no_ray: int = 0             # 1: the adaptive checks skip the unboundedness test
 
     def to_c(self):
         from ._lib import PdlpTune
         return PdlpTune(self.tpr_primal, self.tpr_dual, self.block, self.check_every, self.xcd_remap, self.exact,
-                        self.block_dual, self.scheme, self.sell, 0)
+                        self.block_dual, self.scheme, self.sell, self.no_ray)
 
     def with_(self, **kw) -> "Tune":
         d = asdict(self)

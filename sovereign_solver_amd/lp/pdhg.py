@@ -59,7 +59,15 @@ schemes: c'd < 0 and ``|viol(Kd)|(1+|y|) + |viol_bounds(d)|(1+|c−K'y|) <=
 solution (y, r) bounds c'd below by ``−|y||viol(Kd)| − |r||viol_bounds(d)|``,
 so a bounded LP fails the test while the iterate's duals are within a factor
 M of a dual solution in norm: M = 1e3 when the rows' violation is below 1e-3
-of the row sums of |K_ij d_j| (K d cancels), else 1e6.  The result then has ``x=None``, objective
+of the row sums of |K_ij d_j| (K d cancels), else 1e6.  The same bound with a
+dual *solution* gives the second test, which does not depend on the iterate:
+``|viol(Kd)| + |viol_bounds(d)| <= 1e-6 (−c'd)`` — a bounded LP passes it only
+when every dual solution has ``|y*| + |r*| >= 1e6``, so an LP whose optimum is
+merely far away (a row with coefficients of 1e-5, say) keeps iterating.  The
+host then re-checks the ray in FP64 on the caller's data (:func:`ray_is_certified`,
+``2e-6``); a ray that fails there (the device tested it against the iterate
+precision's copy of K) is dropped, and the solve resumes from where it stopped
+with the unboundedness test off.  The result then has ``x=None``, objective
 ``-inf`` (the reference simplex's convention) and ``ray`` = d in the original
 space with ``max |d_j| = 1``.
 
@@ -124,6 +132,42 @@ def _warm_start(form, p: Problem, x0, y0):
     return x, y
 
 
+RAY_EPS_HOST = 2e-6
+
+
+def _ray_in_original_space(form, p: Problem, ray) -> np.ndarray:
+    d = np.asarray(ray[: p.n_vars], float)
+    if form.col_scale is not None:
+        d = d * form.col_scale
+    m = np.abs(d).max() if d.size else 0.0
+    return d / m if m > 0 else d
+
+
+def ray_quality(p: Problem, d: np.ndarray):
+    """``(c'd, |viol(K d)|_2, |viol_bounds(d)|_2)`` in FP64 on the caller's data: the
+    rows' violations are ``A_eq d`` and ``max(A_ub d, 0)``, the bounds' ``min(d_j, 0)``
+    where lb_j is finite and ``max(d_j, 0)`` where ub_j is finite."""
+    d = np.asarray(d, float)
+    viol = []
+    if p.n_ineq:
+        viol.append(np.maximum(p.A_ub @ d, 0.0))
+    if p.n_eq:
+        viol.append(p.A_eq @ d)
+    vr = float(np.linalg.norm(np.concatenate(viol))) if viol else 0.0
+    lb, ub = np.asarray(p.lb, float), np.asarray(p.ub, float)
+    bnd = np.where(np.isfinite(lb), np.minimum(d, 0.0), 0.0) + np.where(np.isfinite(ub), np.maximum(d, 0.0), 0.0)
+    return float(np.asarray(p.c, float) @ d), vr, float(np.linalg.norm(bnd))
+
+
+def ray_is_certified(p: Problem, d: np.ndarray, eps: float = RAY_EPS_HOST) -> bool:
+    """A ray of decrease to within ``eps``: ``c'd < 0`` and ``|viol(K d)| +
+    |viol_bounds(d)| <= eps (-c'd)``.  For a bounded LP with a dual solution (y*, r*)
+    every d has ``c'd >= -|y*| |viol(K d)| - |r*| |viol_bounds(d)|``, so a bounded LP
+    passes only when ``|y*| + |r*| >= 1 / eps``."""
+    cd, vr, vb = ray_quality(p, d)
+    return bool(np.isfinite(cd) and cd < 0.0 and vr + vb <= eps * -cd)
+
+
 def solve_lp_pdhg(
     problem: Problem,
     max_iters: int = MAX_PDHG_ITERS,
@@ -152,16 +196,19 @@ def solve_lp_pdhg(
         tune = heuristic_tune(form.m, form.n, form.nnz, check_every)
     eng = SingleEngine(form, prec, tune=tune, check_every=check_every, exact=exact, use_graphs=use_graphs,
                        scheme=sch)
+    d = None
     try:
         xs, ys = _warm_start(form, problem, x0, y0)
         r = eng.solve(tol, max_iters, xs, ys, step_scale=step_scale)
+        if r["status"] == "unbounded":
+            d = _ray_in_original_space(form, problem, r["ray"])
+            if not ray_is_certified(problem, d):
+                # the device's test ran on the iterate precision's copy of K; the ray does not
+                # hold in FP64 on the caller's data, so the LP is treated as not unbounded
+                r = eng.resume_without_ray_test()
     finally:
         eng.drop_graphs()
     if r["status"] == "unbounded":
-        d = np.asarray(r["ray"][: problem.n_vars], float)
-        if form.col_scale is not None:
-            d = d * form.col_scale
-        d = d / np.abs(d).max()
         # c'x (the minimised objective) decreases along d without bound: -inf, or +inf in the
         # reported sense of a 'max' problem, whose c is already negated
         return SolveResult("unbounded", None, -np.inf if problem.sense != "max" else np.inf, r["iterations"],

@@ -17,11 +17,20 @@ heuristics and pseudocost/strong branching inside the loop (strong branching
 in batches is :func:`~sovereign_solver_amd.lp.branching.branch_strong_batched`).
 
 PDHG solves each node LP to a relative KKT tolerance (``lp_tol``), so node
-bounds are approximate: a node is pruned when its LP objective is within
+bounds are approximate: a node is pruned when its LP objective, less a margin
+of ``2 lp_tol (1 + |obj|)`` for the objective's own error, is within
 ``rel_gap`` (relative, default 1e-4) of the incumbent, and the result is
 optimal to that gap.  Integer-feasible candidates are rounded and checked
 against the original rows and bounds before they become the incumbent, so
 the returned x is always feasible to ``FEASIBILITY_TOL``-scaled checks.
+
+A node whose LP ends unsolved (iteration limit or a numerical error) is never
+dropped silently: its rounded point is still offered as an incumbent, a
+finite fractional point is branched on, and otherwise (an integral or
+non-finite point says nothing about the subtree) the node is solved again from
+a cold start with four times the iteration limit, twice at most.  A node still
+unsolved after that leaves its subtree unexplored, and the search then ends
+``node_limit`` rather than ``optimal`` or ``infeasible``.
 """
 
 from __future__ import annotations
@@ -62,6 +71,7 @@ class _Node:
     bound: float                # parent's LP objective (min form)
     depth: int
     warm: Optional[tuple]       # parent's (x, y, primal weight)
+    tries: int = 0              # earlier solves of this node that ended unsolved
 
 
 def _min_obj(p: Problem, x: np.ndarray) -> float:
@@ -134,7 +144,7 @@ def solve_milp_batched(
     nodes = 1
     if root.status == "infeasible":
         return result("infeasible", None, np.inf, nodes, np.inf, np.inf)
-    root_bound = _min_obj(p, root.x)
+    root_bound = _min_obj(p, root.x) if root.x is not None and np.all(np.isfinite(root.x)) else -np.inf
     incumbent_x, incumbent = None, np.inf
 
     def consider(x: np.ndarray):
@@ -175,24 +185,39 @@ def solve_milp_batched(
             if np.all(lbc <= ubc + FEASIBILITY_TOL):
                 push(_Node(lbc, ubc, obj, depth + 1, (x, y, w)))
 
-    def settle(node_lb, node_ub, r, depth, parent_bound):
+    unresolved = 0                                      # subtrees left unexplored (unsolved node LPs)
+
+    def settle(node_lb, node_ub, r, depth, parent_bound, tries=0):
         """Process one solved node relaxation."""
+        nonlocal unresolved
         if r.status == "infeasible":
             return
         x = r.x
-        if r.status == "optimal":
+        finite = x is not None and bool(np.all(np.isfinite(x)))
+        if r.status == "optimal" and finite:
             obj = _min_obj(p, x)
             consider(x)
-            if obj >= cutoff():
+            if obj - 2.0 * lp_tol * (1.0 + abs(obj)) >= cutoff():
                 return                                  # cannot beat the incumbent
-        else:                                           # not solved: keep the parent's bound, branch anyway
-            obj = parent_bound
-        frac = np.abs(x[ivars] - np.round(x[ivars]))
-        if r.status == "optimal" and (frac.size == 0 or frac.max() <= int_tol):
-            return                                      # integral: handled by consider()
-        branch(node_lb, node_ub, x, r.y, r.step_scale, obj, depth)
+            frac = np.abs(x[ivars] - np.round(x[ivars]))
+            if frac.size == 0 or frac.max() <= int_tol:
+                return                                  # integral: handled by consider()
+            branch(node_lb, node_ub, x, r.y, r.step_scale, obj, depth)
+            return
+        # not solved: keep the parent's bound; a finite fractional point still splits the node
+        if finite:
+            consider(x)
+            frac = np.abs(x[ivars] - np.round(x[ivars]))
+            if frac.size and frac.max() > int_tol:
+                branch(node_lb, node_ub, x, r.y, r.step_scale, parent_bound, depth)
+                return
+        if tries < 2:                                   # solve it again, cold, with more iterations
+            push(_Node(node_lb, node_ub, parent_bound, depth, None, tries + 1))
+        else:
+            unresolved += 1
 
-    settle(lb0, ub0, root, 0, root_bound)
+    # an unsolved root bounds nothing: its subtree keeps the trivial bound
+    settle(lb0, ub0, root, 0, root_bound if root.status == "optimal" else -np.inf)
     while heap and nodes < node_limit:
         if time.monotonic() - t0 > time_limit:
             best = heap[0][0] if heap else incumbent
@@ -204,13 +229,17 @@ def solve_milp_batched(
                 take.append(node)
         if not take:
             break
-        children = [node_problem(p, nd.lb, nd.ub) for nd in take]
-        warm = [(np.clip(nd.warm[0], c.lb, c.ub), nd.warm[1], nd.warm[2]) if nd.warm else None
-                for nd, c in zip(take, children)]
-        res = solve_lp_pdhg_batch(children, warm_start=warm, tol=lp_tol, max_iters=max_lp_iters, engine="family")
-        nodes += len(take)
-        for nd, r in zip(take, res):
-            settle(nd.lb, nd.ub, r, nd.depth, nd.bound)
+        # nodes solved again after an unsolved attempt get four times the iterations per attempt
+        for tries in sorted({nd.tries for nd in take}):
+            group = [nd for nd in take if nd.tries == tries]
+            children = [node_problem(p, nd.lb, nd.ub) for nd in group]
+            warm = [(np.clip(nd.warm[0], c.lb, c.ub), nd.warm[1], nd.warm[2]) if nd.warm else None
+                    for nd, c in zip(group, children)]
+            res = solve_lp_pdhg_batch(children, warm_start=warm if any(warm) else None, tol=lp_tol,
+                                      max_iters=max_lp_iters * 4 ** tries, engine="family")
+            nodes += len(group)
+            for nd, r in zip(group, res):
+                settle(nd.lb, nd.ub, r, nd.depth, nd.bound, nd.tries)
         if verbose:
             best = heap[0][0] if heap else incumbent
             print(f"  nodes {nodes} open {len(heap)} incumbent {sign * incumbent:.6f} bound {sign * best:.6f}")
@@ -218,6 +247,8 @@ def solve_milp_batched(
     live = [e[0] for e in heap if e[0] < cutoff()]       # open nodes that could still improve
     if live:
         return result("node_limit", incumbent_x, incumbent, nodes, root_bound, min(min(live), incumbent))
+    if unresolved:                                       # a subtree was left unexplored: no proof
+        return result("node_limit", incumbent_x, incumbent, nodes, root_bound, -np.inf)
     if incumbent_x is None:
         return result("infeasible", None, np.inf, nodes, root_bound, np.inf)
     return result("optimal", incumbent_x, incumbent, nodes, root_bound, incumbent)

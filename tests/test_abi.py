@@ -72,7 +72,7 @@ def test_header_constants_match_binding():
         except (NameError, SyntaxError):
             pass
     for name in ("PDLP_ABI_VERSION", "PDLP_RUNNING", "PDLP_OPTIMAL", "PDLP_NUMERICAL", "PDLP_INFEASIBLE",
-                 "PDLP_UNBOUNDED", "PDLP_PARTIAL_BLOCKS", "PDLP_PARTIALS_LEN"):
+                 "PDLP_UNBOUNDED", "PDLP_SKIPPED", "PDLP_PARTIAL_BLOCKS", "PDLP_PARTIALS_LEN"):
         assert env[name] == getattr(L, name), name
 
 
@@ -83,3 +83,41 @@ def test_transpose_argument_errors_without_device():
     assert lib.pdlp_csr_transpose_work_bytes(1, 2, 3, None) == -1                 # PDLP_E_ARG
     assert lib.pdlp_csr_transpose(7, 1, 1, 0, None, None, None, None, None, None, None, 0, None) == -1
     assert b"prec" in lib.pdlp_last_error()
+
+
+def _batch_args(count=1, n_resident=0):
+    """A PdlpBatch whose pointers are non-NULL host addresses: the argument checks run
+    before anything touches them (no launch follows a rejected call)."""
+    buf = (C.c_char * 64)()
+    a = L.PdlpBatch(count=count, check_every=50, max_iters=100, block=512, lds_bytes=1024, n_resident=n_resident,
+                    block_resident=256, lds_resident=1024, tol=1e-4)
+    for name, typ in L.PdlpBatch._fields_:
+        if typ is C.c_void_p:
+            setattr(a, name, C.addressof(buf))
+    return a, buf
+
+
+def test_batch_rejects_lps_beyond_lds_without_device():
+    lib = L.lib()
+    a, _buf = _batch_args()
+    a.lds_need = 2048                                   # an LP of the FP32 part needs more than lds_bytes
+    assert lib.pdlp_batch_solve(C.byref(a), None) == -4                    # PDLP_E_LIMIT
+    assert b"lds_bytes" in lib.pdlp_last_error()
+    a, _buf = _batch_args(count=2, n_resident=1)
+    a.lds_need, a.lds_need_resident = 512, 4096         # the FP64 part's LP does not fit
+    assert lib.pdlp_batch_solve(C.byref(a), None) == -4
+    assert b"lds_resident" in lib.pdlp_last_error()
+    a, _buf = _batch_args()
+    a.lds_need = 0                                      # the need must be stated
+    assert lib.pdlp_batch_solve(C.byref(a), None) == -1                    # PDLP_E_ARG
+
+
+def test_packer_states_each_parts_lds_need():
+    from sovereign_solver_amd.engine.batch import (PackedBatch, lds_bytes_of, lds_bytes_resident,
+                                                   lds_bytes_x_in_lds)
+    from sovereign_solver_amd.workloads import uneven_batch
+    pk = PackedBatch(uneven_batch(40, seed=3))
+    fp32 = ~pk.resident
+    need = (lds_bytes_x_in_lds if pk.x_in_lds else lds_bytes_of)(pk.ms, pk.ns)
+    assert pk.lds_need == int(need[fp32].max()) <= pk.lds_bytes
+    assert pk.lds_need_resident == int(lds_bytes_resident(pk.ms, pk.ns)[pk.resident].max()) <= pk.lds_resident

@@ -77,7 +77,8 @@ def test_equal_but_distinct_matrix_objects_accepted():
 
 def test_struct_layouts():
     import ctypes as C
-    assert C.sizeof(L.PdlpFamilyLP) == 8 * 4 + 5 * 8 + 4 + 4
+    # 8 int32, 5 doubles, weight, weight0, weight_cold, warm, iter0, reserved, start_err, cold_err
+    assert C.sizeof(L.PdlpFamilyLP) == 8 * 4 + 5 * 8 + 4 * 6 + 8 + 8
     assert C.sizeof(L.PdlpFamily) == 12 * 4 + 8 + 35 * 8
     lib = L.lib()
     assert lib.pdlp_family_init(None, None) == -1                               # PDLP_E_ARG

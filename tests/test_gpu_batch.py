@@ -143,9 +143,8 @@ def test_batch_fp32_part_layouts_agree_bitwise(gpu):
     pk = PackedBatch(probs)
     assert pk.n_resident < len(probs) and pk.x_in_lds
     o1, x1, y1 = BatchEngine(pk).solve(1e-4, 50_000)
-    pk.x_in_lds = False
-    fp32 = ~pk.resident
-    pk.lds_bytes = int((lds_bytes_of(pk.ms, pk.ns)[fp32].max() + 15) // 16 * 16)
+    pk.set_x_in_lds(False)
+    assert pk.lds_bytes >= int(lds_bytes_of(pk.ms, pk.ns)[~pk.resident].max())
     o2, x2, y2 = BatchEngine(pk).solve(1e-4, 50_000)
     assert np.all(o1["status"] == 1)
     assert np.array_equal(x1, x2) and np.array_equal(y1, y2) and np.array_equal(o1, o2)

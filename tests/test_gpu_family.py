@@ -115,11 +115,12 @@ def test_family_device_start_matches_host_start(gpu):
     shift = rng.standard_normal(n + m + 1).astype(np.float32)
     val = raw.astype(np.float64) * scale + shift
     val32 = (raw * scale + shift).astype(np.float64)       # the kernel's FP32 arithmetic
-    a_eng = FamilyEngine(fb)
+    # failover off: this compares the two ways of setting the same start
+    a_eng = FamilyEngine(fb, failover_iters=0)
     a_eng.start(1e-4, val32[:, :n], val32[:, n:n + m], np.exp(np.clip(val32[:, -1], -8, 8)))
     a_eng.run(100_000)
     a = a_eng.results()
-    b_eng = FamilyEngine(fb)
+    b_eng = FamilyEngine(fb, failover_iters=0)
     dv = b_eng.device
     src = torch.from_numpy(raw).to(dv)
     b_eng.start_from_device(1e-4, src, 0, n, n + m, torch.from_numpy(scale).to(dv), torch.from_numpy(shift).to(dv))
@@ -129,6 +130,50 @@ def test_family_device_start_matches_host_start(gpu):
     # the host path rounds x0 / y0 / weight through FP64 first: agree to a few checks
     assert np.all(np.abs(a[0]["iterations"] - b[0]["iterations"]) <= 100)
     assert np.allclose(val, val32, rtol=1e-5, atol=1e-5)
+
+
+def test_family_garbage_and_nan_device_starts_fail_over(gpu):
+    """Network outputs of garbage (large random values, a wild primal weight) and NaN
+    through pdlp_family_set_start: the engine compares each start's KKT with the cold
+    start's and drops the worse, and restarts cold a start that has made no progress
+    by 200 iterations, so every LP still ends optimal within the cold solve's
+    iterations plus the failover's 200 (and a check)."""
+    import torch
+    fam = DispatchFamily(G=20, T=6)
+    probs = [fam.problem(k) for k in range(96)]
+    fb = FamilyBatch(probs)
+    n, m = fb.n, fb.m
+    cold = FamilyEngine(fb).solve(1e-4, 100_000)[0]
+    assert np.all(cold["status"] == 1)
+    rng = np.random.default_rng(7)
+    garbage = (100.0 * rng.standard_normal((96, n + m + 1))).astype(np.float32)
+    nan = np.full((96, n + m + 1), np.nan, np.float32)
+    mixed = np.where(rng.random((96, 1)) < 0.5, garbage, nan).astype(np.float32)
+    mixed[:, -1] = 8.0                                   # weight e^8 for every LP
+    for src in (garbage, nan, mixed):
+        eng = FamilyEngine(fb)
+        t = torch.from_numpy(np.ascontiguousarray(src)).to(eng.device)
+        eng.start_from_device(1e-4, t, 0, n, n + m)
+        eng.run(100_000)
+        out, X, Y = eng.results()
+        assert np.all(out["status"] == 1)
+        assert np.all(out["iterations"] <= cold["iterations"] + 250), (out["iterations"] - cold["iterations"]).max()
+        for k in range(0, 96, 12):
+            assert yardstick.rel_kkt(probs[k], X[k], Y[k])[0] <= 1.02e-4
+
+
+def test_family_good_warm_start_survives_the_failover(gpu):
+    # solutions as starts: better than the cold start at init, so they are kept
+    fam = DispatchFamily(G=20, T=6)
+    fb = FamilyBatch([fam.problem(k) for k in range(64)])
+    out, X, Y = FamilyEngine(fb).solve(1e-4, 100_000)
+    X, Y = X.copy(), Y.copy()
+    warm = FamilyEngine(fb).solve(1e-4, 100_000, X, Y, out["weight"])[0]
+    assert np.all(warm["status"] == 1)
+    # FP32 iterates restarted at the FP64 solution need a few checks for some LPs; none is slower
+    # than its cold solve and the mean is a fraction of the cold one
+    assert np.all(warm["iterations"] <= out["iterations"])
+    assert warm["iterations"].mean() <= 0.35 * out["iterations"].mean()
 
 
 def test_family_xcd_affinity_and_long_split_agree_bitwise(gpu):

@@ -164,6 +164,6 @@ def test_training_reduces_loss(gpu):
     X = rng.standard_normal((2048, 16))
     Wt = rng.standard_normal((16, 40))
     T = np.tanh(X @ Wt) + 0.5 * (X[:, :1] ** 2)
-    from sovereign_solver_amd.learned.dispatch import train
+    from sovereign_solver_amd.learned.warmstart import train
     _, _, _, losses = train(X, T, d_h=64, epochs=15, batch=256, lr=3e-3, step_weight=1.0)
     assert losses[-1] < 0.25 * losses[0], losses

@@ -190,13 +190,36 @@ def test_reference_test_lps(gpu, c, A, b, expected):
     assert np.all(r.x >= -1e-12)
 
 
-def test_device_resolution(gpu):
+def test_device_resolution(gpu, caplog):
+    """tests/test_pdhg_gpu.py:148-161 of the reference: an explicit device the engine
+    does not have warns and still returns a result (here computed on the HIP GPU);
+    an unknown device string raises ValueError."""
     assert get_available_device() == "cuda"
     p = _make([-1.0, -2.0], [[1.0, 1.0]], [4.0])
-    with pytest.raises(RuntimeError):
-        solve_lp_pdhg_gpu(p, device="cpu")
+    for dev in ("cpu", "mps", "cuda", "auto"):
+        with caplog.at_level("WARNING"):
+            r = solve_lp_pdhg_gpu(p, device=dev)
+        assert r.status == "optimal" and abs(r.objective + 8.0) <= 1e-4 * 9, (dev, r.status, r.objective)
+        if dev in ("cpu", "mps"):
+            assert "HIP GPU only" in caplog.text
     with pytest.raises(ValueError):
         solve_lp_pdhg_gpu(p, device="tpu")
+
+
+def test_garbage_and_nan_warm_starts_fail_over(gpu):
+    """A warm start worse than the cold start is dropped at once, so a garbage or NaN
+    start costs no more than the failover's 200 iterations over the cold solve."""
+    p = random_sparse_lp(100, 250, 0.03, seed=3)
+    cold = solve_lp_pdhg(p, tol=1e-4, precision="fp32")
+    rng = np.random.default_rng(1)
+    starts = [(1e3 * rng.standard_normal(p.n_vars), 1e3 * rng.standard_normal(p.n_constraints), 5e3),
+              (np.full(p.n_vars, np.nan), np.full(p.n_constraints, np.nan), float("nan")),
+              (rng.uniform(0, 1, p.n_vars) * p.ub, None, None)]
+    for x0, y0, w in starts:
+        r = solve_lp_pdhg(p, tol=1e-4, precision="fp32", x0=x0, y0=y0, step_scale=w)
+        assert r.status == "optimal", r.status
+        assert yardstick.rel_kkt(p, r.x, r.y)[0] <= 1.02e-4
+        assert r.iterations <= cold.iterations + 250, (r.iterations, cold.iterations)
 
 
 def test_empty_problem_returns_optimal_zeros(gpu):
