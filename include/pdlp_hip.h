@@ -249,6 +249,23 @@ int pdlp_kkt_check(const PdlpDev* dev, const PdlpTune* tune, void* stream);
 int pdlp_run(const PdlpDev* dev, const PdlpTune* tune, int32_t iters,
              int32_t check, void* stream);
 
+/* pdlp_run without workspace: the same iterations and check as pdlp_run, but the
+ * scratch of its kernels — the check's reduction partials (dev->partials) and the
+ * split SELL dual step's partial dot products (dev->sell_part) — comes from one
+ * static device array of the library, PDLP_NOALLOC_SCRATCH_BYTES per GPU (the
+ * partials take PDLP_PARTIALS_LEN doubles of it, the SELL partials the rest), held
+ * by each GPU the library runs on until the process ends.  dev->partials and
+ * dev->sell_part are ignored and may be NULL; every other pointer is the caller's, as
+ * for pdlp_run.  The call allocates nothing (no hipMalloc, no library allocation): it
+ * is for callers that cannot allocate device memory, such as code under HIP graph
+ * capture and serving loops.  Returns PDLP_E_LIMIT when the split SELL step
+ * (tune->sell >= 8) needs more than the array's remaining bytes.  All calls on one
+ * GPU share its array and it keeps the last call's contents (each call writes what
+ * it reads), so calls that overlap in time on one GPU race: serialise them (one
+ * stream, or events).  No counterpart in the reference. */
+#define PDLP_NOALLOC_SCRATCH_BYTES (4u << 20)
+int pdlp_run_noalloc(const PdlpDev* dev, const PdlpTune* tune, int32_t iters, int32_t check, void* stream);
+
 /* Capture `blocks` x pdlp_run(dev, tune, tune->check_every, 1) in a HIP graph.
  * *exec receives an opaque handle for pdlp_graph_launch / pdlp_graph_destroy. */
 int pdlp_graph_capture(const PdlpDev* dev, const PdlpTune* tune, int32_t blocks,

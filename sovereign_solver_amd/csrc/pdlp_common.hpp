@@ -33,6 +33,11 @@ __host__ __device__ inline double clamp_weight(double w, double w0) {
     return fmin(fmax(w, w0 / kWeightDown), w0 * kWeightUp);
 }
 
+// The single-instance engine's argument checks and iteration blocks (pdlp_single.hip),
+// shared with the library's other entry points.
+int single_check_all(const PdlpDev* d, const PdlpTune* t);
+int single_run(const PdlpDev& d, const PdlpTune& t, int iters, int check, hipStream_t s);
+
 // Thread-local message behind pdlp_last_error().
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);

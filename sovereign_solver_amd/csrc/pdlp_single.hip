@@ -964,6 +964,12 @@ int run(const PdlpDev& d, const PdlpTune& t, int iters, int check, hipStream_t s
 
 }  // namespace
 
+// Library-internal entry points for the other sources of libpdlp_hip.so.
+int single_check_all(const PdlpDev* d, const PdlpTune* t) { return check_all(d, t); }
+int single_run(const PdlpDev& d, const PdlpTune& t, int iters, int check, hipStream_t s) {
+    return run(d, t, iters, check, s);
+}
+
 }  // namespace pdlp
 
 using namespace pdlp;

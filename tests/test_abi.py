@@ -121,3 +121,13 @@ def test_packer_states_each_parts_lds_need():
     need = (lds_bytes_x_in_lds if pk.x_in_lds else lds_bytes_of)(pk.ms, pk.ns)
     assert pk.lds_need == int(need[fp32].max()) <= pk.lds_bytes
     assert pk.lds_need_resident == int(lds_bytes_resident(pk.ms, pk.ns)[pk.resident].max()) <= pk.lds_resident
+
+
+def test_noalloc_rejects_oversized_sell_scratch_without_device():
+    # the size check comes before any device call
+    lib = L.lib()
+    d = L.PdlpDev(m=1, n=1, nnz=1, prec=L.PDLP_F32, sell_slices=4, sell_max_width=1 << 20)
+    t = L.PdlpTune(1, 1, 256, 50, 0, 0, 0, 1, 8, 0)
+    assert lib.pdlp_run_noalloc(C.byref(d), C.byref(t), 1, 1, None) == L.PDLP_E_LIMIT
+    assert b"scratch" in lib.pdlp_last_error()
+    assert lib.pdlp_run_noalloc(None, C.byref(t), 1, 1, None) == L.PDLP_E_ARG
