@@ -491,6 +491,23 @@ typedef struct PdlpFamily {
     double* partials;             /* 2 * PDLP_FAM_KKT_CHUNKS * 12 * tiles * 64 */
 } PdlpFamily;
 
+/* Per-LP problem data from LP-major FP64 arrays already on the device, in the
+ * problems' own row and column order: c [count][n], b [count][m] (rows
+ * [A_ub; A_eq]), lo and hi [count][n] (both or neither).  A NULL source keeps
+ * that vector shared: its single copy in f is used as it stands, and f->shared
+ * carries the matching bits (c: C and C64; b: B and B64; lo / hi: LO, HI,
+ * LOSHIFT and UBSPAN; BSHIFT exactly when b and lo are both shared).  Writes
+ * the tiled per-LP vectors of f in device order (through row_orig / col_orig):
+ * c, c64, b, b64, lo, hi, lo_shift = lo where finite else 0, ub_span = hi -
+ * lo_shift where hi is finite else +inf, b_shift = b - K lo_shift (FP64), and
+ * norms[3k .. 3k+2] = (norm_b, norm_c, cold weight) of LP k: the reference-form
+ * normalisers max(1, ||(b_shift, finite ub_span)||) and max(1, ||c||)
+ * (pdhg.py:181-182 on the lifted form) and ||sqrt(sigma_hat) b|| /
+ * ||sqrt(tau_hat) c|| clamped to [1e-6, 1e6] (1 when a norm vanishes).  This
+ * replaces the host-side construction of each LP's standard form
+ * (pdhg.py:62-125) for a batch of LPs sharing K. */
+int pdlp_family_load(const PdlpFamily* f, const double* c, const double* b, const double* lo, const double* hi,
+                     double* norms, void* stream);
 /* Set up every LP from (x, y), lp[k].norm_b/norm_c/weight as filled by the
  * host: sums cleared, restart point = start, KKT of the start. */
 int pdlp_family_init(const PdlpFamily* f, void* stream);

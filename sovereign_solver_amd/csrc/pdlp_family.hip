@@ -866,6 +866,100 @@ __global__ void __launch_bounds__(T) set_weight_kernel(PdlpFamily f, const float
     }
 }
 
+// ------------------------------------------------------------ problem data
+// Per-LP vectors from LP-major FP64 sources in the problems' own order (one row
+// per LP), written tiled in device order.  Lane = LP; each wavefront takes
+// LOAD_PER consecutive device indices, so consecutive loads of a lane fall in
+// the same source cache lines.  Columns (which = 0): c -> c, c64; lo, hi -> lo,
+// hi, lo_shift, ub_span.  Rows (which = 1): b -> b, b64.
+constexpr int LOAD_PER = 32;
+
+__global__ void __launch_bounds__(NT) load_vec_kernel(PdlpFamily f, int which, const double* __restrict__ sc,
+                                                      const double* __restrict__ slo, const double* __restrict__ shi,
+                                                      const double* __restrict__ sb) {
+    const int tile = blockIdx.y, lane = lane_id(), w = wave_id();
+    const int k = tile * T + lane;
+    if (k >= f.count) return;                                // no barrier in this kernel
+    const int L = which == 0 ? f.n : f.m;
+    const int d0 = (blockIdx.x * NWV + w) * LOAD_PER, d1 = min(d0 + LOAD_PER, L);
+    for (int d = d0; d < d1; ++d) {
+        const size_t ix = at(tile, L, d, lane);
+        if (which == 0) {
+            const size_t src = (size_t)k * f.n + f.col_orig[d];
+            if (sc) {
+                const double c = sc[src];
+                const_cast<float*>(f.c)[ix] = (float)c;
+                const_cast<double*>(f.c64)[ix] = c;
+            }
+            if (slo) {
+                const double lo = slo[src], hi = shi[src];
+                const double ls = isfinite(lo) ? lo : 0.0;
+                const_cast<float*>(f.lo)[ix] = (float)lo;
+                const_cast<float*>(f.hi)[ix] = (float)hi;
+                const_cast<double*>(f.lo_shift)[ix] = ls;
+                const_cast<double*>(f.ub_span)[ix] = isfinite(hi) ? hi - ls : INFINITY;
+            }
+        } else {
+            const double b = sb[(size_t)k * f.m + f.row_orig[d]];
+            const_cast<float*>(f.b)[ix] = (float)b;
+            const_cast<double*>(f.b64)[ix] = b;
+        }
+    }
+}
+
+// b_shift = b - K lo_shift (FP64), one row per wavefront, lane = LP.
+__global__ void __launch_bounds__(NT) load_shift_kernel(PdlpFamily f) {
+    const int tile = blockIdx.y, lane = lane_id(), w = wave_id();
+    const int i = blockIdx.x * NWV + w;
+    const int k = tile * T + lane;
+    if (i >= f.m || k >= f.count) return;
+    const bool shl = f.shared & PDLP_FAM_SHARED_LOSHIFT, shb = f.shared & PDLP_FAM_SHARED_B64;
+    double a = 0.0;
+    for (int q = f.row_ptr[i]; q < f.row_ptr[i + 1]; ++q)
+        a += (double)f.val[q] * ldv(f.lo_shift, shl, tile, f.n, f.col_idx[q], lane);
+    const_cast<double*>(f.b_shift)[at(tile, f.m, i, lane)] = ldv(f.b64, shb, tile, f.m, i, lane) - a;
+}
+
+// Per LP: norm_b = max(1, ||(b_shift, finite ub_span)||), norm_c = max(1, ||c||) (the
+// reference-form normalisers, pdhg.py:181-182 on the lifted form) and the cold
+// weight ||sqrt(sigma_hat) b|| / ||sqrt(tau_hat) c|| clamped to [1e-6, 1e6] (1 when a
+// norm vanishes).  One workgroup of NWN wavefronts per tile: wavefront w sums the
+// indices w, w + NWN, ...; the partial sums meet in a fixed order.
+constexpr int NWN = 16;
+
+__global__ void __launch_bounds__(NWN * T) load_norm_kernel(PdlpFamily f, double* __restrict__ norms) {
+    __shared__ double red[NWN][4][T];
+    const int tile = blockIdx.x, lane = lane_id(), w = wave_id();
+    const int k = tile * T + lane;
+    const bool shc = f.shared & PDLP_FAM_SHARED_C64, shu = f.shared & PDLP_FAM_SHARED_UBSPAN,
+               shb = f.shared & PDLP_FAM_SHARED_B64, shs = f.shared & PDLP_FAM_SHARED_BSHIFT;
+    double bs2 = 0.0, c2 = 0.0, bw2 = 0.0, cw2 = 0.0;
+    for (int d = w; d < f.n; d += NWN) {
+        const double c = ldv(f.c64, shc, tile, f.n, d, lane), sp = ldv(f.ub_span, shu, tile, f.n, d, lane);
+        if (isfinite(sp)) bs2 += sp * sp;
+        c2 += c * c;
+        cw2 += c * c * (double)f.tau_hat[d];
+    }
+    for (int i = w; i < f.m; i += NWN) {
+        const double b = ldv(f.b64, shb, tile, f.m, i, lane), bs = ldv(f.b_shift, shs, tile, f.m, i, lane);
+        bs2 += bs * bs;
+        bw2 += b * b / f.row_metric[i];
+    }
+    red[w][0][lane] = bs2;
+    red[w][1][lane] = c2;
+    red[w][2][lane] = bw2;
+    red[w][3][lane] = cw2;
+    __syncthreads();
+    if (w != 0 || k >= f.count) return;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int v = 0; v < NWN; ++v)
+        for (int q = 0; q < 4; ++q) s[q] += red[v][q][lane];
+    const double nb = sqrt(s[2]), nc = sqrt(s[3]);
+    norms[3 * (size_t)k + 0] = fmax(1.0, sqrt(s[0]));
+    norms[3 * (size_t)k + 1] = fmax(1.0, sqrt(s[1]));
+    norms[3 * (size_t)k + 2] = (nb > 1e-12 && nc > 1e-12) ? fmin(fmax(nb / nc, 1e-6), 1e6) : 1.0;
+}
+
 __global__ void __launch_bounds__(T) clear_mask_kernel(PdlpFamily f) {
     if (threadIdx.x == 0) f.run_mask[blockIdx.x] = 0;
 }
@@ -988,6 +1082,38 @@ extern "C" int pdlp_family_set_start(const PdlpFamily* f, const float* src, int6
     set_start_kernel<<<g, NT, 0, s>>>(*f, src, ld, x_off, y_off, scale, shift);
     if (w_col >= 0) set_weight_kernel<<<f->tiles, T, 0, s>>>(*f, src, ld, w_col, scale, shift);
     return hip_fail(hipGetLastError(), "pdlp_family_set_start launch");
+}
+
+extern "C" int pdlp_family_load(const PdlpFamily* f, const double* c, const double* b, const double* lo,
+                                const double* hi, double* norms, void* stream) {
+    if (int e = check_family(f)) return e;
+    if (norms == nullptr) return fail(PDLP_E_ARG, "norms is NULL");
+    if ((lo == nullptr) != (hi == nullptr)) return fail(PDLP_E_ARG, "lo and hi are both given or both NULL");
+    const int sh = f->shared;
+    auto both = [&](int a, int b2) { return (sh & a) && (sh & b2); };
+    auto none = [&](int a, int b2) { return !(sh & a) && !(sh & b2); };
+    if (c ? !none(PDLP_FAM_SHARED_C, PDLP_FAM_SHARED_C64) : !both(PDLP_FAM_SHARED_C, PDLP_FAM_SHARED_C64))
+        return fail(PDLP_E_ARG, "c is given exactly when c / c64 are not shared");
+    if (b ? !none(PDLP_FAM_SHARED_B, PDLP_FAM_SHARED_B64) : !both(PDLP_FAM_SHARED_B, PDLP_FAM_SHARED_B64))
+        return fail(PDLP_E_ARG, "b is given exactly when b / b64 are not shared");
+    const int lo_bits = PDLP_FAM_SHARED_LO | PDLP_FAM_SHARED_HI | PDLP_FAM_SHARED_LOSHIFT | PDLP_FAM_SHARED_UBSPAN;
+    if (lo ? (sh & lo_bits) != 0 : (sh & lo_bits) != lo_bits)
+        return fail(PDLP_E_ARG, "lo / hi are given exactly when lo, hi, lo_shift and ub_span are not shared");
+    const bool bshift_shared = (sh & PDLP_FAM_SHARED_BSHIFT) != 0;
+    if (bshift_shared != (b == nullptr && lo == nullptr))
+        return fail(PDLP_E_ARG, "b_shift is shared exactly when b and lo are");
+    hipStream_t s = as_stream(stream);
+    const int per_block = NWV * LOAD_PER;
+    if (c || lo)
+        load_vec_kernel<<<dim3((unsigned)((f->n + per_block - 1) / per_block), (unsigned)f->tiles), NT, 0, s>>>(
+            *f, 0, c, lo, hi, nullptr);
+    if (b)
+        load_vec_kernel<<<dim3((unsigned)((f->m + per_block - 1) / per_block), (unsigned)f->tiles), NT, 0, s>>>(
+            *f, 1, nullptr, nullptr, nullptr, b);
+    if (!bshift_shared)
+        load_shift_kernel<<<dim3((unsigned)((f->m + NWV - 1) / NWV), (unsigned)f->tiles), NT, 0, s>>>(*f);
+    load_norm_kernel<<<f->tiles, NWN * T, 0, s>>>(*f, norms);
+    return hip_fail(hipGetLastError(), "pdlp_family_load launch");
 }
 
 extern "C" int pdlp_family_finish(const PdlpFamily* f, int32_t tail, void* stream) {

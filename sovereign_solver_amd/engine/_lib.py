@@ -166,6 +166,7 @@ _SIGS = {
                                           C.POINTER(C.c_void_p)]),
     "pdlp_batch_solve": (C.c_int, [C.POINTER(PdlpBatch), _p]),
     "pdlp_family_init": (C.c_int, [C.POINTER(PdlpFamily), _p]),
+    "pdlp_family_load": (C.c_int, [C.POINTER(PdlpFamily), _p, _p, _p, _p, _p, _p]),
     "pdlp_family_run": (C.c_int, [C.POINTER(PdlpFamily), C.c_int32, _p]),
     "pdlp_family_iterate": (C.c_int, [C.POINTER(PdlpFamily), C.c_int32, _p]),
     "pdlp_family_finish": (C.c_int, [C.POINTER(PdlpFamily), C.c_int32, _p]),

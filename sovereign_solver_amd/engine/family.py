@@ -11,6 +11,7 @@ and termination, exactly as in the persistent batched engine.
 
 from __future__ import annotations
 
+import concurrent.futures as cf
 import ctypes as C
 from typing import Optional, Sequence
 
@@ -62,28 +63,33 @@ def from_tiles(flat: np.ndarray, tiles: int, Ln: int, B: int) -> np.ndarray:
     return flat.reshape(tiles, Ln, TILE).transpose(0, 2, 1).reshape(tiles * TILE, Ln)[:B]
 
 
-def _rows(vectors, perm) -> np.ndarray:
-    """[B, L] array of the vectors (permuted to device order), or [1, L] when they
-    all hold the same values (same object, or equal entries)."""
+def _same_vectors(vectors) -> bool:
+    """Whether every vector holds the first one's values (the same object, or equal
+    entries); an unequal pair usually shows at once."""
     first = vectors[0]
-    if all(v is first for v in vectors):
-        return np.asarray(first, float)[perm][None, :]
-    last = np.asarray(vectors[-1], float)
-    f0 = np.asarray(first, float)
-    if np.array_equal(last, f0):                   # maybe all equal: check every one
-        if all(np.array_equal(np.asarray(v, float), f0) for v in vectors):
-            return f0[perm][None, :]
-    return np.stack([np.asarray(v, float)[perm] for v in vectors])
+    f0 = None
+    for v in vectors:
+        if v is first:
+            continue
+        if f0 is None:
+            f0 = np.asarray(first, float)
+        a = np.asarray(v, float)
+        if a.shape != f0.shape or not np.array_equal(a, f0):
+            return False
+    return True
 
 
 class FamilyBatch:
-    """Host data of B box-form LPs that share K = [A_ub; A_eq].
+    """Host description of B box-form LPs that share K = [A_ub; A_eq].
 
-    The device numbers rows and columns so that those with more than
-    ``PDLP_FAM_LONG`` entries (one workgroup each) come last:
+    Holds the matrix analysis (device numbering, K and K^T, row kinds, the
+    diagonal step sizes) and which problem vectors every LP shares (stored
+    once on the device) — c, b = [b_ub; b_eq], the bounds; the per-LP vectors
+    are packed on the device by :meth:`FamilyEngine.load`
+    (``pdlp_family_load``).  The device numbers rows and columns so that those
+    with more than ``PDLP_FAM_LONG`` entries (one workgroup each) come last:
     ``row_perm[i]`` / ``col_perm[j]`` is the problem's index of device row i /
-    column j.  Every ``*_dev`` array and every vector in ``vec`` is in device
-    order; :meth:`to_device_cols` / :meth:`from_device_cols` convert."""
+    column j; :meth:`to_device_cols` / :meth:`from_device_cols` convert."""
 
     def __init__(self, problems: Sequence[Problem], long_threshold: int = L.PDLP_FAM_LONG):
         self.problems = list(problems)
@@ -91,19 +97,15 @@ class FamilyBatch:
             raise ValueError("empty family")
         self.long_threshold = long_threshold
         p0 = self.problems[0]
+        self.A_ub, self.A_eq = p0.A_ub, p0.A_eq
         self.m_ub, self.m_eq = p0.n_ineq, p0.n_eq
-        for k, p in enumerate(self.problems):
-            if (p.n_ineq, p.n_eq, p.n_vars) != (self.m_ub, self.m_eq, p0.n_vars):
-                raise ValueError(f"LP {k}: shape differs from LP 0")
-            if not (_same_matrix(p.A_ub, p0.A_ub) and _same_matrix(p.A_eq, p0.A_eq)):
-                raise ValueError(f"LP {k}: constraint matrix differs from LP 0 (use the persistent batch engine)")
+        self._check_matrices(self.problems)
         K0 = _canonical(sp.vstack([sp.csr_matrix(p0.A_ub), sp.csr_matrix(p0.A_eq)], format="csr"))
         self.K_orig = K0
         m, n = K0.shape
         if m == 0 or n == 0:
             raise ValueError("the family engine needs m > 0 and n > 0")
-        self.m, self.n, self.count = m, n, len(self.problems)
-        self.tiles = (self.count + TILE - 1) // TILE
+        self.m, self.n = m, n
         rl, cl = np.diff(K0.indptr), np.diff(K0.tocsc().indptr)
         lt = long_threshold
         self.row_perm = np.concatenate([np.flatnonzero(rl <= lt), np.flatnonzero(rl > lt)])
@@ -120,35 +122,80 @@ class FamilyBatch:
         self.tau_hat = (0.999 / np.maximum(colsum, 1e-4)).astype(np.float32)
         self.sigma_hat = (0.999 / np.maximum(rowsum, 1e-4)).astype(np.float32)
         self.row_metric = np.maximum(rowsum, 1e-4) / 0.999
+        self._analyse_sharing()
 
-        P = self.problems
-        cp, rp = self.col_perm, self.row_perm
-        B = self.count
-        # per-LP vectors as [B, L] arrays, or [1, L] when every LP has the same values
-        c = _rows([p.c for p in P], cp)
-        b = _rows([np.concatenate([np.asarray(p.b_ub, float), np.asarray(p.b_eq, float)]) for p in P], rp)
-        lo = _rows([p.lb for p in P], cp)
-        hi = _rows([p.ub for p in P], cp)
-        lbf = np.where(np.isfinite(lo), lo, 0.0)
-        b_shift = b if not lbf.any() else b - (K @ lbf.T).T
-        span = np.where(np.isfinite(hi), hi - lbf, np.inf)
-        fin_span = np.where(np.isfinite(span), span, 0.0)
-        per_lp = lambda v: np.broadcast_to(v, (B,)).copy()
-        self.norm_b = per_lp(np.maximum(1.0, np.sqrt(np.sum(b_shift * b_shift, axis=1) +
-                                                     np.sum(fin_span * fin_span, axis=1))))
-        self.norm_c = per_lp(np.maximum(1.0, np.linalg.norm(c, axis=1)))
-        # starting primal weight of the adaptive scheme: ||sqrt(sigma_hat) b|| / ||sqrt(tau_hat) c||
-        nb = per_lp(np.sqrt(np.sum(b * b * (0.999 / np.maximum(rowsum, 1e-4)), axis=1)))
-        nc = per_lp(np.sqrt(np.sum(c * c * (0.999 / np.maximum(colsum, 1e-4)), axis=1)))
-        ok = (nb > 1e-12) & (nc > 1e-12)
-        self.init_weight = np.where(ok, np.clip(nb / np.where(ok, nc, 1.0), 1e-6, 1e6), 1.0)
-        self.vec = {"c": c, "b": b, "lo": lo, "hi": hi, "c64": c, "b64": b, "b_shift": b_shift,
-                    "lo_shift": lbf, "ub_span": span}
-        self.shared = 0
-        for name, v in self.vec.items():
-            if v.shape[0] == 1:
-                self.shared |= L.PDLP_FAM_SHARED[name]
-        self.lo, self.hi = lo, hi
+    @property
+    def count(self) -> int:
+        return len(self.problems)
+
+    @property
+    def tiles(self) -> int:
+        return (self.count + TILE - 1) // TILE
+
+    def _check_matrices(self, problems) -> None:
+        p0 = self.problems[0]
+        for k, p in enumerate(problems):
+            if (p.n_ineq, p.n_eq, p.n_vars) != (self.m_ub, self.m_eq, p0.n_vars):
+                raise ValueError(f"LP {k}: shape differs from LP 0")
+            if not (_same_matrix(p.A_ub, self.A_ub) and _same_matrix(p.A_eq, self.A_eq)):
+                raise ValueError(f"LP {k}: constraint matrix differs from LP 0 (use the persistent batch engine)")
+
+    def _pattern(self, problems) -> dict:
+        """Which of c, b, bounds every problem shares."""
+        return {"c": _same_vectors([p.c for p in problems]),
+                "b": _same_vectors([p.b_ub for p in problems]) and _same_vectors([p.b_eq for p in problems]),
+                "bounds": _same_vectors([p.lb for p in problems]) and _same_vectors([p.ub for p in problems])}
+
+    def _analyse_sharing(self) -> None:
+        pat = self._pattern(self.problems)
+        self.pattern = pat
+        F = L.PDLP_FAM_SHARED
+        bits = 0
+        if pat["c"]:
+            bits |= F["c"] | F["c64"]
+        if pat["b"]:
+            bits |= F["b"] | F["b64"]
+        if pat["bounds"]:
+            bits |= F["lo"] | F["hi"] | F["lo_shift"] | F["ub_span"]
+        if pat["b"] and pat["bounds"]:
+            bits |= F["b_shift"]
+        self.shared = bits
+        # the single copies of the shared vectors, in device order
+        p0, cp, rp = self.problems[0], self.col_perm, self.row_perm
+        sv = {}
+        if pat["c"]:
+            sv["c"] = sv["c64"] = np.asarray(p0.c, float)[cp]
+        if pat["bounds"]:
+            lo, hi = np.asarray(p0.lb, float)[cp], np.asarray(p0.ub, float)[cp]
+            lbf = np.where(np.isfinite(lo), lo, 0.0)
+            sv["lo"], sv["hi"], sv["lo_shift"] = lo, hi, lbf
+            sv["ub_span"] = np.where(np.isfinite(hi), hi - lbf, np.inf)
+        if pat["b"]:
+            sv["b"] = sv["b64"] = np.concatenate([np.asarray(p0.b_ub, float), np.asarray(p0.b_eq, float)])[rp]
+        if pat["b"] and pat["bounds"]:
+            sv["b_shift"] = sv["b"] - self.K @ sv["lo_shift"]
+        self.shared_vec = sv
+
+    def with_problems(self, problems: Sequence[Problem]) -> "FamilyBatch":
+        """The same family (matrix analysis, shared vectors) with other LPs: they must
+        hold the same matrices and share exactly what this batch shares, with equal
+        values."""
+        problems = list(problems)
+        if len(problems) != self.count:
+            raise ValueError(f"{len(problems)} LPs for an engine of {self.count}")
+        self._check_matrices(problems)
+        pat = self._pattern(problems)
+        if pat != self.pattern:
+            raise ValueError(f"the LPs share {pat}, the engine's batch {self.pattern}: build a new engine")
+        q, p0 = problems[0], self.problems[0]
+        eq = lambda a, b: a is b or np.array_equal(np.asarray(a, float), np.asarray(b, float))
+        if ((pat["c"] and not eq(q.c, p0.c)) or (pat["b"] and not (eq(q.b_ub, p0.b_ub) and eq(q.b_eq, p0.b_eq)))
+                or (pat["bounds"] and not (eq(q.lb, p0.lb) and eq(q.ub, p0.ub)))):
+            raise ValueError("a vector the engine stores once differs in the new LPs: build a new engine")
+        fb = object.__new__(FamilyBatch)
+        fb.__dict__.update(self.__dict__)
+        fb.problems = problems
+        return fb
 
     def to_device_cols(self, X: np.ndarray) -> np.ndarray:
         return X[:, self.col_perm]
@@ -169,13 +216,10 @@ class FamilyBatch:
     def is_shared(self, name: str) -> bool:
         return bool(self.shared & L.PDLP_FAM_SHARED[name])
 
-    def device_vector(self, name: str) -> np.ndarray:
-        a = self.vec[name]
-        return a[0] if self.is_shared(name) else to_tiles(a, self.tiles)
-
-    def rows_of(self, name: str) -> np.ndarray:
-        """[B, L] view of a per-LP vector in device order (shared ones broadcast)."""
-        return np.broadcast_to(self.vec[name], (self.count, self.vec[name].shape[1]))
+    def bounds(self):
+        """[B, n] lower and upper bounds in the problems' own column order."""
+        return (np.stack([np.asarray(p.lb, float) for p in self.problems]),
+                np.stack([np.asarray(p.ub, float) for p in self.problems]))
 
     def column_pass_bytes(self) -> float:
         """Algorithmic bytes of one column pass over all B LPs: K^T's CSR once
@@ -248,9 +292,26 @@ class FamilyEngine:
             t["Y_out"] = torch.zeros((B, m), dtype=f64, device=dv)
             t["out_final"] = torch.zeros(B * OUT_DTYPE.itemsize, dtype=torch.uint8, device=dv)
             t["partials"] = torch.zeros(2 * L.PDLP_FAM_KKT_CHUNKS * L.PDLP_FAM_NSUM * Tn, dtype=f64, device=dv)
+            # problem vectors: one copy of the shared ones, tiled per-LP arrays (filled by load)
+            # with a copy each that restores the slot order after compaction
+            self._orig = {}
+            for name, dt in self._DATA.items():
+                ln = n if name in self._COLS else m
+                if fb.is_shared(name):
+                    t[name] = dev_tensor(fb.shared_vec[name], dt, dv)
+                else:
+                    t[name] = torch.zeros(Tn * ln, dtype=dt, device=dv)
+                    self._orig[name] = torch.zeros(Tn * ln, dtype=dt, device=dv)
+            # LP-major FP64 staging of the per-LP sources (pinned host copy, device copy)
+            self._src = {}
+            for name, ln, per_lp in (("c", n, not fb.is_shared("c")), ("b", m, not fb.is_shared("b")),
+                                     ("lo", n, not fb.is_shared("lo")), ("hi", n, not fb.is_shared("hi"))):
+                if per_lp:
+                    self._src[name] = (torch.empty((B, ln), dtype=f64).pin_memory(),
+                                       torch.empty((B, ln), dtype=f64, device=dv))
+            t["norms"] = torch.zeros(3 * B, dtype=f64, device=dv)
+            self.norms_host = torch.zeros(3 * B, dtype=f64).pin_memory()
         self.t = t
-        self._orig = {}
-        self._upload_data()
         self.stream.synchronize()
         # per-LP tiled arrays that compaction moves (window partial sums and xbar
         # are zero / rewritten at a check boundary and need not move)
@@ -268,24 +329,74 @@ class FamilyEngine:
             setattr(self.args, name, ptr(t[name]))
         self._graphs = {}
         self._host = None
+        self.load()
 
     _DATA = {"c": torch.float32, "lo": torch.float32, "hi": torch.float32, "b": torch.float32,
              "c64": torch.float64, "lo_shift": torch.float64, "ub_span": torch.float64, "b64": torch.float64,
              "b_shift": torch.float64}
+    _COLS = ("c", "lo", "hi", "c64", "lo_shift", "ub_span")
 
-    def _upload_data(self) -> None:
-        """Problem vectors in the original slot order.  Compaction permutes the
-        working copies of the per-LP ones; device originals restore them."""
+    def _gather(self, problems) -> None:
+        """Per-LP sources into the pinned LP-major staging, in parallel threads."""
+        src = self._src
+        if not src:
+            return
+        B = len(problems)
+        m_ub = self.fb.m_ub
+
+        def fill(lo_k, hi_k):
+            for k in range(lo_k, hi_k):
+                p = problems[k]
+                if "c" in src:
+                    src["c"][0][k].numpy()[:] = p.c
+                if "b" in src:
+                    row = src["b"][0][k].numpy()
+                    row[:m_ub] = p.b_ub
+                    row[m_ub:] = p.b_eq
+                if "lo" in src:
+                    src["lo"][0][k].numpy()[:] = p.lb
+                    src["hi"][0][k].numpy()[:] = p.ub
+
+        workers = max(1, min(8, B // 64))
+        if workers == 1:
+            fill(0, B)
+            return
+        step = (B + workers - 1) // workers
+        with cf.ThreadPoolExecutor(workers) as ex:
+            list(ex.map(lambda a: fill(a, min(B, a + step)), range(0, B, step)))
+
+    def load(self, problems: Optional[Sequence[Problem]] = None) -> None:
+        """Problem data of the batch's LPs (or of ``problems``: the same family, count and
+        sharing, see :meth:`FamilyBatch.with_problems`) onto the device: per-LP
+        vectors gathered into pinned LP-major staging, copied over, and tiled with
+        their derived vectors and normalisers by ``pdlp_family_load``."""
+        if problems is not None:
+            self.fb = self.fb.with_problems(problems)
+        fb = self.fb
+        self._gather(fb.problems)
+        src = self._src
         with torch.cuda.stream(self.stream):
-            for name, dt in self._DATA.items():
-                if name in self.t:
-                    if name in self._orig:
-                        self.t[name].copy_(self._orig[name])
-                    continue
-                a = torch.from_numpy(np.ascontiguousarray(self.fb.device_vector(name))).to(self.device, dt)
-                self.t[name] = a
-                if not self.fb.is_shared(name):
-                    self._orig[name] = a.clone()
+            for h, d in src.values():
+                d.copy_(h, non_blocking=True)
+            dp = lambda name: src[name][1].data_ptr() if name in src else None
+            if self.compactions or self.args.tiles != fb.tiles:
+                self.args.tiles, self.args.count = fb.tiles, fb.count
+                self.compactions = 0
+                self.drop_graphs()
+            L.check(self.lib.pdlp_family_load(C.byref(self.args), dp("c"), dp("b"), dp("lo"), dp("hi"),
+                                              self.t["norms"].data_ptr(), self.sh), "pdlp_family_load")
+            for name, orig in self._orig.items():
+                orig.copy_(self.t[name])
+            self.norms_host.copy_(self.t["norms"], non_blocking=True)
+        self.stream.synchronize()
+        nv = self.norms_host.numpy().reshape(-1, 3)
+        self.norm_b, self.norm_c, self.init_weight = nv[:, 0].copy(), nv[:, 1].copy(), nv[:, 2].copy()
+
+    def _restore_data(self) -> None:
+        """Per-LP problem vectors back in the original slot order (after compaction)."""
+        with torch.cuda.stream(self.stream):
+            for name, orig in self._orig.items():
+                self.t[name].copy_(orig)
 
     @property
     def sh(self):
@@ -331,20 +442,20 @@ class FamilyEngine:
         fb = self.fb
         self.set_tol(tol)
         if self.compactions or self.args.tiles != fb.tiles:
-            self._upload_data()
+            self._restore_data()
             self.args.tiles, self.args.count = fb.tiles, fb.count
             self.compactions = 0
             self.drop_graphs()
         self.slot_lp = np.concatenate([np.arange(fb.count), np.full(fb.tiles * TILE - fb.count, -1)])
         self.saved = np.zeros(fb.count, bool)
-        w_cold = fb.init_weight.copy() if self.flags & 1 else np.ones(fb.count)
+        w_cold = self.init_weight.copy() if self.flags & 1 else np.ones(fb.count)
         w = w_cold
         if weight is not None:
             ww = np.asarray(weight, float)
             w = np.where(np.isfinite(ww) & (ww > 0), ww, w_cold)
         lp = np.zeros(fb.count, dtype=self.LP_DTYPE)
         assert lp.dtype.itemsize == C.sizeof(L.PdlpFamilyLP)
-        lp["norm_b"], lp["norm_c"], lp["weight"], lp["weight0"] = fb.norm_b, fb.norm_c, w, w
+        lp["norm_b"], lp["norm_c"], lp["weight"], lp["weight0"] = self.norm_b, self.norm_c, w, w
         lp["weight_cold"] = w_cold
         lp["warm"] = np.broadcast_to(np.asarray(warm, np.int32), (fb.count,))
         with torch.cuda.stream(self.stream):
@@ -361,10 +472,11 @@ class FamilyEngine:
         warm = np.zeros(fb.count, np.int32)
         if weight is not None:
             warm |= np.isfinite(np.asarray(weight, float)).astype(np.int32)
+        LB = UB = None
         if x0 is not None:
-            xd = fb.to_device_cols(np.asarray(x0, float).reshape(fb.count, fb.n))
-            warm |= (np.clip(xd, fb.rows_of("lo"), fb.rows_of("hi")) != np.clip(0.0, fb.rows_of("lo"),
-                                                                                fb.rows_of("hi"))).any(axis=1)
+            LB, UB = fb.bounds()
+            x0 = np.clip(np.asarray(x0, float).reshape(fb.count, fb.n), LB, UB)
+            warm |= (x0 != np.clip(0.0, LB, UB)).any(axis=1).astype(np.int32)
         if y0 is not None:
             warm |= 3 * (np.asarray(y0, float).reshape(fb.count, fb.m) != 0.0).any(axis=1).astype(np.int32)
         self._reset(tol, weight, warm=warm)
@@ -373,9 +485,10 @@ class FamilyEngine:
             L.check(self.lib.pdlp_family_set_start(C.byref(self.args), None, 0, -1, -1, -1, None, None, self.sh),
                     "pdlp_family_set_start")
         else:
-            lo, hi = fb.rows_of("lo"), fb.rows_of("hi")
-            x = (np.clip(0.0, lo, hi) if x0 is None else
-                 np.clip(fb.to_device_cols(np.asarray(x0, float).reshape(fb.count, fb.n)), lo, hi))
+            if x0 is None:
+                LB, UB = fb.bounds()
+                x0 = np.clip(0.0, LB, UB)
+            x = fb.to_device_cols(x0)
             if y0 is None:
                 y = np.zeros((fb.count, fb.m))
             else:

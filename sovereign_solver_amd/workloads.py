@@ -127,6 +127,9 @@ class DispatchFamily:
         self.A.sort_indices()
         self.m, self.n = m, n
         self.no_eq = sp.csr_matrix((0, n))
+        # vectors equal for every instance are one object each (the batched engine stores them once)
+        self.lb, self.ub = np.zeros(n), np.tile(self.pmax, T)
+        self.b_eq = np.zeros(0)
 
     def params(self, k: int):
         """(demand, cost) of instance k."""
@@ -159,8 +162,15 @@ class DispatchFamily:
         n = self.n
         # every instance references the same matrix objects (the batched engine stores them once)
         return Problem(c=self.objective(c), A_ub=self.A, b_ub=self.rhs(d), A_eq=self.no_eq,
-                       b_eq=np.zeros(0), lb=np.zeros(n), ub=np.tile(self.pmax, self.T),
-                       integer_mask=np.zeros(n, dtype=bool), name=name)
+                       b_eq=self.b_eq, lb=self.lb, ub=self.ub, integer_mask=np.zeros(n, dtype=bool), name=name)
+
+    def features_from(self, problems) -> np.ndarray:
+        """Network inputs [B, T + G] read back from instances' own vectors (demand from the
+        first T rows, -b_ub[:T]; costs from the first period's columns, c[:G])."""
+        T, G = self.T, self.G
+        d = -np.stack([np.asarray(p.b_ub[:T], float) for p in problems])
+        c = np.stack([np.asarray(p.c[:G], float) for p in problems])
+        return np.concatenate([d / self.demand - 1.0, c / self.cost - 1.0], axis=1)
 
 
 class SetCoverNodes:

@@ -42,23 +42,36 @@ def test_setcover_nodes_share_costs_not_bounds():
     assert not fb.is_shared("lo") and not fb.is_shared("hi") and not fb.is_shared("b_shift")
 
 
-def test_normalisers_match_single_box_form_and_yardstick():
-    """Per-LP norm_b / norm_c equal the single engine's box form, whose KKT data
-    the yardstick pins; step sizes are the Pock-Chambolle diagonal ones."""
+def test_shared_copies_match_the_problems():
+    """The single copies of shared vectors are the problems' own vectors in device
+    order, with the lifted-form shift; step sizes are the Pock-Chambolle diagonal ones."""
     sc = SetCoverNodes(rows=60, cols=300, density=0.03)
     probs = [sc.node(k) for k in range(3)]
     fb = FamilyBatch(probs)
-    for k, p in enumerate(probs):
-        f = box_form(p)
-        assert fb.norm_b[k] == pytest.approx(f.norm_b, rel=1e-14)
-        assert fb.norm_c[k] == pytest.approx(f.norm_c, rel=1e-14)
-        np.testing.assert_allclose(fb.from_device_rows(fb.rows_of("b_shift"))[k], f.b_shift, rtol=0, atol=1e-12)
-        # the yardstick normaliser is the same quantity
-        x = np.clip(0.0, p.lb, p.ub)
-        y = np.zeros(p.n_constraints)
-        assert np.isfinite(yardstick.rel_kkt(p, x, y)[0])
+    np.testing.assert_array_equal(fb.from_device_cols(fb.shared_vec["c"][None])[0], probs[0].c)
+    np.testing.assert_array_equal(fb.from_device_rows(fb.shared_vec["b"][None])[0], probs[0].b_ub)
+    assert "lo" not in fb.shared_vec and "b_shift" not in fb.shared_vec
+    fam = DispatchFamily(G=10, T=4)
+    fd = FamilyBatch([fam.problem(k) for k in range(5)])
+    f = box_form(fam.problem(0))
+    np.testing.assert_array_equal(fd.shared_vec["lo_shift"], np.zeros(fd.n))
+    np.testing.assert_array_equal(fd.from_device_cols(fd.shared_vec["ub_span"][None])[0], f.ub_span)
     colsum = np.asarray(abs(fb.K).sum(axis=0)).ravel()
     np.testing.assert_allclose(fb.tau_hat, 0.999 / np.maximum(colsum, 1e-4), rtol=1e-7)
+
+
+def test_with_problems_checks_family_and_sharing():
+    fam = DispatchFamily(G=10, T=4)
+    fb = FamilyBatch([fam.problem(k) for k in range(5)])
+    fb2 = fb.with_problems([fam.problem(k) for k in range(5, 10)])
+    assert fb2.count == 5 and fb2.K is fb.K and fb2.problems[0].name == "dispatch_5"
+    with pytest.raises(ValueError, match="LPs for an engine"):
+        fb.with_problems([fam.problem(k) for k in range(4)])
+    other = fam.problem(3)
+    other = Problem(c=other.c, A_ub=other.A_ub, b_ub=other.b_ub, A_eq=other.A_eq, b_eq=other.b_eq,
+                    lb=other.lb, ub=other.ub * 2, integer_mask=other.integer_mask)
+    with pytest.raises(ValueError, match="share"):
+        fb.with_problems([fam.problem(k) for k in range(4)] + [other])
 
 
 def test_rejects_different_matrices():
@@ -119,7 +132,5 @@ def test_device_numbering_puts_long_rows_last():
     np.testing.assert_array_equal(fb.from_device_cols(fb.to_device_cols(X)), X)
     Y = rng.standard_normal((3, fb.m))
     np.testing.assert_array_equal(fb.from_device_rows(fb.to_device_rows(Y)), Y)
-    # device-order problem data matches the problems' own data
-    p = fam.problem(1)
-    np.testing.assert_array_equal(fb.from_device_cols(fb.rows_of("c"))[1], p.c)
-    np.testing.assert_array_equal(fb.from_device_rows(fb.rows_of("b"))[1], p.b_ub)
+    # device-order shared data matches the problems' own data
+    np.testing.assert_array_equal(fb.from_device_cols(fb.shared_vec["hi"][None])[0], fam.problem(1).ub)

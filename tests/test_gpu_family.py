@@ -257,3 +257,49 @@ def test_family_mixed_rows_and_infinite_upper_bounds(gpu):
     _check(probs, res, n_highs=4, obj_tol=2e-3)
     fb = FamilyBatch(probs)
     assert fb.is_shared("ub_span") and not fb.is_shared("b")
+
+
+def _device_rows(eng, name, ln):
+    """[B, L] of a per-LP (tiled) or shared device vector, in the problems' own order."""
+    from sovereign_solver_amd.engine.family import from_tiles
+    fb = eng.fb
+    a = eng.t[name].cpu().numpy().astype(np.float64)
+    a = np.broadcast_to(a, (fb.count, ln)) if fb.is_shared(name) else from_tiles(a, fb.tiles, ln, fb.count)
+    return fb.from_device_cols(np.array(a)) if ln == fb.n else fb.from_device_rows(np.array(a))
+
+
+def test_family_device_load_matches_the_box_form(gpu):
+    """pdlp_family_load tiles the problems' vectors and forms the lifted-form data and
+    normalisers: per-LP bounds (set-cover nodes, shared c and b) and per-LP c and b
+    (dispatch, shared bounds) against each LP's own box form."""
+    from sovereign_solver_amd.engine.forms import box_form
+    sc = SetCoverNodes(rows=60, cols=300, density=0.03)
+    fam = DispatchFamily(G=12, T=5)
+    for probs in ([sc.node(k) for k in range(70)], [fam.problem(k) for k in range(70)]):
+        eng = FamilyEngine(FamilyBatch(probs))
+        n, m = eng.fb.n, eng.fb.m
+        c64, b64, bs = _device_rows(eng, "c64", n), _device_rows(eng, "b64", m), _device_rows(eng, "b_shift", m)
+        span, c32 = _device_rows(eng, "ub_span", n), _device_rows(eng, "c", n)
+        for k in (0, 33, 69):
+            f = box_form(probs[k])
+            assert eng.norm_b[k] == pytest.approx(f.norm_b, rel=1e-13)
+            assert eng.norm_c[k] == pytest.approx(f.norm_c, rel=1e-13)
+            np.testing.assert_array_equal(c64[k], f.c)
+            np.testing.assert_array_equal(c32[k], f.c.astype(np.float32))
+            np.testing.assert_array_equal(b64[k], f.b)
+            np.testing.assert_allclose(bs[k], f.b_shift, rtol=0, atol=1e-12)
+            np.testing.assert_array_equal(span[k], f.ub_span if f.ub_span is not None else np.full(n, np.inf))
+
+
+def test_family_reload_matches_a_fresh_engine(gpu):
+    # a resident engine loaded with other LPs of the family solves them as a new engine does
+    fam = DispatchFamily(G=12, T=5)
+    a, b = [fam.problem(k) for k in range(96)], [fam.problem(1000 + k) for k in range(96)]
+    eng = FamilyEngine(FamilyBatch(a))
+    eng.solve(1e-4, 100_000)                               # compacts on the way
+    eng.load(b)
+    o1, x1, y1 = eng.solve(1e-4, 100_000)
+    o1, x1, y1 = o1.copy(), x1.copy(), y1.copy()
+    o2, x2, y2 = FamilyEngine(FamilyBatch(b)).solve(1e-4, 100_000)
+    assert np.all(o1["status"] == 1)
+    assert np.array_equal(o1, o2) and np.array_equal(x1, x2) and np.array_equal(y1, y2)
