@@ -1,37 +1,50 @@
-"""Benchmark: LPs/s to 1e-4 relative KKT on one MI355X, with the SpMV roofline.
+"""Benchmark: LPs/s to 1e-4 relative KKT on one MI355X — learned + batched vs
+plain PDHG — with the roofline of the dominant kernel and the CPU baseline.
 
-Default workload: the single large transportation LP (1,000 sources ×
-2,000 sinks: n = 2,000,000 arcs, m = 3,000 rows, nnz = 4,000,000), plain
-restarted PDHG from a cold start (x = y = 0), FP32 iterates and SpMV, FP64
-KKT.  A *step* is one complete solve to relative KKT ≤ 1e-4 (reference
-formula on the reference-form lift) with the LP already resident in HBM.
+Default run (``python bench.py [--steps K --warmup W]``): the headline,
+config 3 at its full size — economic dispatch, 200 generators x 24 periods
+(m = 9,224 rows, n = 4,800 columns), a 3-layer width-512 MLP trained on
+50,000 PDHG-labelled instances, 1,024 held-out instances solved in one
+shared-matrix batch warm-started by the network (x and primal weight), every
+solve finished and judged by PDHG at relative KKT <= 1e-4 (reference formula
+on the reference-form lift, FP64).  A *step* is one such solve of all 1,024
+LPs with their data resident in HBM: network inference, device-side start,
+the batched solve and the results gathered in HBM.  ``value`` is LPs/s over
+the K timed steps.  The same run measures, on the same held-out LPs:
 
-The JSON line also carries:
+* the host-to-host rate (problems on the host -> features, device packing,
+  inference, solve, results and objectives on the host);
+* the cold-started batched engine, and plain single-instance GPU PDHG (one
+  LP after another, cold start) both with its LP resident and through the
+  drop-in call ``solve_lp_pdhg`` (``speedup_vs_plain``);
+* the objective error of 64 LPs against HiGHS dual simplex (FP64);
+* ``cpu_baseline``: the FP64 CPU restatement of the reference's PDHG on a
+  pool of the host's cores (and on one core), on LPs of the same set.
 
-* ``roofline`` for the dominant kernel (the fused primal step: Kᵀy gather,
-  projection, extrapolation, ergodic sum).  ``achieved`` = its algorithmic
-  bytes per launch ÷ its mean launch time, timed with HIP events on the
-  engine's stream over a run of back-to-back launches at the solve's state.
-* ``cpu_baseline``: the FP64 CPU oracle (restatement of the reference's
-  solver/lp/pdhg.py) on 1 core, timed per iteration over a bounded sample of
-  the same LP and scaled by the iteration count the reference algorithm needs
-  (measured on the GPU in the reference form, FP64).
+``config.subrecords`` carry the other configs, each with its clock: C1 the
+transportation LP (1,000 x 2,000), C2 the 4,096 uneven LPs (two persistent
+launches), C4 256 set-cover node relaxations, and ``beyond_mall`` the
+standalone SpMV and fused steps on a transportation LP whose working set
+exceeds the 256 MB Infinity Cache.  ``--workload`` runs one part alone.
 """
 
 from __future__ import annotations
 
 import argparse
 import json
-
-import numpy as np
+import multiprocessing as mp
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+METRIC = "LPs/sec to 1e-4 rel KKT: learned+batched vs plain PDHG, 1 GPU; SpMV HBM GB/s"
 
 
 def log(msg: str) -> None:
@@ -50,6 +63,154 @@ def measured_traffic(key: str):
     return None if rec is None else float(rec["bytes"])
 
 
+# ------------------------------------------------------------------ references
+def highs_objective(p, method: str = "highs-ds", time_limit: float = 0.0):
+    """FP64 optimum of p by HiGHS (scipy; dual simplex, or interior point with
+    crossover), or None when HiGHS does not finish (within ``time_limit`` s if > 0)."""
+    from scipy.optimize import linprog
+    opts = {"time_limit": float(time_limit)} if time_limit > 0 else {}
+    r = linprog(p.c, A_ub=p.A_ub if p.n_ineq else None, b_ub=p.b_ub if p.n_ineq else None,
+                A_eq=p.A_eq if p.n_eq else None, b_eq=p.b_eq if p.n_eq else None,
+                bounds=list(zip([None if np.isinf(v) else v for v in p.lb],
+                                [None if np.isinf(v) else v for v in p.ub])), method=method, options=opts)
+    if r.status != 0:
+        return None
+    return -float(r.fun) if p.sense == "max" else float(r.fun)
+
+
+def objective_errors(probs, objs, k: int, method: str = "highs-ds", time_limit: float = 0.0,
+                     total_cap_s: float = 120.0):
+    """|obj - obj*| / (1 + |obj*|) on the first k LPs, obj* by HiGHS (FP64); LPs that
+    HiGHS does not finish within its per-LP limit, or after the total cap, are counted
+    as unchecked."""
+    t0 = time.perf_counter()
+    errs, unchecked = [], 0
+    for p, o in list(zip(probs, objs))[:k]:
+        if time.perf_counter() - t0 > total_cap_s:
+            unchecked += 1
+            continue
+        ref = highs_objective(p, method, time_limit)
+        if ref is None:
+            unchecked += 1
+        else:
+            errs.append(abs(o - ref) / (1.0 + abs(ref)))
+    e = np.array(errs) if errs else np.array([np.nan])
+    return {"lps": len(errs), "unchecked": unchecked, "max": float(e.max()), "median": float(np.median(e)),
+            "reference": f"HiGHS {method} (scipy.optimize.linprog, FP64)"
+                         + (f", {time_limit:g} s per LP" if time_limit > 0 else ""),
+            "s": time.perf_counter() - t0}
+
+
+# ---------------------------------------------------------------- CPU baseline
+_CPU_CACHE = {}
+
+
+def _cpu_init():
+    os.environ["OMP_NUM_THREADS"] = "1"
+    os.environ["OPENBLAS_NUM_THREADS"] = "1"
+    try:
+        from threadpoolctl import threadpool_limits
+        threadpool_limits(1)
+    except Exception:                                   # pragma: no cover
+        pass
+
+
+def _cpu_problem(spec):
+    """The LP of a workload spec, rebuilt from its seeds (no data crosses processes)."""
+    from sovereign_solver_amd.workloads import DispatchFamily, SetCoverNodes, random_sparse_lp, uneven_sizes
+    kind, k = spec
+    if kind == "dispatch":
+        fam = _CPU_CACHE.get(kind) or _CPU_CACHE.setdefault(kind, DispatchFamily())
+        return fam.problem(k)
+    if kind.startswith("uneven/"):                       # "uneven/<batch size>"
+        ms = _CPU_CACHE.get(kind)
+        if ms is None:
+            ms = _CPU_CACHE.setdefault(kind, uneven_sizes(int(kind.split("/")[1]), 50, 2000, 0))
+        m = int(ms[k])
+        return random_sparse_lp(m, int(round(2.5 * m)), 0.01, seed=k, family=2)
+    if kind == "setcover":
+        sc = _CPU_CACHE.get(kind) or _CPU_CACHE.setdefault(kind, SetCoverNodes())
+        return sc.node(k)
+    raise ValueError(kind)
+
+
+def _cpu_noop(_):
+    return os.getpid()
+
+
+def _cpu_solve(job):
+    from oracle import pdhg_ref
+    spec, tol = job
+    p = _cpu_problem(spec)
+    t0 = time.perf_counter()
+    r = pdhg_ref.solve(p, max_iters=100_000, tol=tol)
+    return time.perf_counter() - t0, r.status
+
+
+def cpu_workers() -> int:
+    """The host cores this process may use, at most 16 (one GPU's share of a box)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:                              # pragma: no cover
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _pool_rate(jobs, workers: int, cap_s: float):
+    """LPs finished per second by ``workers`` single-threaded processes in cap_s seconds
+    of wall time (LPs still running at the deadline count for nothing)."""
+    ctx = mp.get_context("spawn")
+    pool = ctx.Pool(workers, initializer=_cpu_init)
+    try:
+        pool.map(_cpu_noop, range(workers))              # interpreters started before the clock
+        t0 = time.perf_counter()
+        deadline = t0 + cap_s
+        done = 0
+        it = pool.imap_unordered(_cpu_solve, jobs)
+        while True:
+            left = deadline - time.perf_counter()
+            if left <= 0:
+                break
+            try:
+                it.next(timeout=left)
+            except (StopIteration, mp.TimeoutError):
+                break
+            done += 1
+        wall = min(time.perf_counter(), deadline) - t0
+    finally:
+        pool.terminate()
+        pool.join()
+    return done, wall
+
+
+def cpu_baseline(kind: str, ids, tol: float, cap_s: float, what: str):
+    """The FP64 CPU restatement of the reference's PDHG (``oracle/pdhg_ref.py``) on
+    LPs of the workload, in a random order: a pool of single-threaded processes over
+    the host cores for cap_s seconds of wall time, and one process for cap_s / 2."""
+    _cpu_init()
+    workers = cpu_workers()
+    jobs = [((kind, int(k)), tol) for k in np.random.default_rng(11).permutation(np.asarray(ids))]
+    done, wall = _pool_rate(jobs, workers, cap_s)
+    pool_rate = done / wall
+    one, wall1 = _pool_rate(jobs, 1, cap_s / 2)
+    one_rate = one / wall1
+    last, t0 = wall, 0.0
+    try:
+        model = next(l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name"))
+    except (OSError, StopIteration):                    # pragma: no cover
+        model = "unknown"
+    log(f"cpu baseline ({what}): {pool_rate:.3f} LPs/s on {workers} processes ({done} LPs), "
+        f"{one_rate:.3f} LPs/s on 1 core ({one} LPs)")
+    return {"value": pool_rate, "unit": "LPs/s", "cores": workers, "kind": "port",
+            "sample": (f"oracle FP64 reference-form PDHG (restatement of solver/lp/pdhg.py, SciPy/NumPy) on {what}, "
+                       f"to {tol:g}, in a seeded random order: a pool of {workers} single-threaded processes "
+                       f"finished {done} LPs in {last - t0:.1f}s of wall time (LPs unfinished at the deadline "
+                       f"count for nothing); one process alone {one_rate:.3f} LPs/s ({one} LPs); "
+                       f"OMP/BLAS threads 1; host {model}, cpu_count {os.cpu_count()}"),
+            "one_core_lps_per_s": one_rate}
+
+
+# --------------------------------------------------------------------- timing
 def primal_bytes(form, elem: int) -> float:
     """Algorithmic bytes of one fused primal-step launch."""
     n, m, nnz = form.n, form.m, form.nnz
@@ -73,7 +234,7 @@ def dual_bytes(form, elem: int) -> float:
     return float(b)
 
 
-def spmv_rates(eng, reps: int = 50):
+def spmv_rates(eng, reps: int = 50, kx_tpr: int = 64):
     """Standalone FP32 SpMV (``pdlp_spmv_csr``) on the engine's K and K^T: a HIP
     graph of ``reps`` back-to-back launches timed with events on its stream.
     Algorithmic bytes per product: 8 nnz + 4 (rows + 1) + 4 cols + 4 rows (FP32
@@ -86,7 +247,7 @@ def spmv_rates(eng, reps: int = 50):
     s = torch.cuda.Stream(device=eng.device)
     for name, rp, ci, val, x, rows, cols, tpr in (
             ("KT_y", eng.t_row_ptr, eng.t_col_idx, eng.t_val, eng.y, eng.n, eng.m, 1),
-            ("K_x", eng.row_ptr, eng.col_idx, eng.val, eng.x, eng.m, eng.n, 64)):
+            ("K_x", eng.row_ptr, eng.col_idx, eng.val, eng.x, eng.m, eng.n, kx_tpr)):
         with torch.cuda.stream(s):
             y = torch.empty(rows, dtype=torch.float32, device=eng.device)
 
@@ -117,265 +278,17 @@ def spmv_rates(eng, reps: int = 50):
     return out
 
 
-def cpu_baseline(problem, iters_needed: int, sample_iters: int):
-    """Time the CPU oracle per iteration on a bounded sample; scale to a full solve."""
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
-    from oracle import pdhg_ref
-    try:
-        from threadpoolctl import threadpool_limits
-        limiter = threadpool_limits(1)
-    except Exception:                        # pragma: no cover
-        limiter = None
-    t0 = time.perf_counter()
-    sf = pdhg_ref.to_standard_form(problem)
-    tau, sigma = pdhg_ref.step_sizes(sf.A)
-    AT = sf.A.T.tocsr()
-    t_setup = time.perf_counter() - t0
-    import numpy as np
-    A, b, c = sf.A, sf.b, sf.c
-    x = np.zeros(A.shape[1]); y = np.zeros(A.shape[0])
-    xs = np.zeros_like(x); ys = np.zeros_like(y)
-    t1 = time.perf_counter()
-    for _ in range(sample_iters):            # the loop body of pdhg.py:209-222
-        xn = np.maximum(x - tau * (c - AT @ y), 0.0)
-        xb = 2.0 * xn - x
-        y = y + sigma * (b - A @ xb)
-        x = xn
-        xs += x
-        ys += y
-    t_iter = (time.perf_counter() - t1) / sample_iters
-    # one KKT evaluation per 50 iterations (pdhg.py:225-229)
-    t2 = time.perf_counter()
-    norm_b = max(1.0, float(np.linalg.norm(b))); norm_c = max(1.0, float(np.linalg.norm(c)))
-    pdhg_ref.kkt(A, AT, b, c, norm_b, norm_c, xs / sample_iters, ys / sample_iters)
-    t_kkt = time.perf_counter() - t2
-    if limiter is not None:
-        limiter.unregister() if hasattr(limiter, "unregister") else None
-    total = t_setup + iters_needed * (t_iter + t_kkt / 50.0)
-    return {
-        "value": 1.0 / total, "unit": "LPs/s", "cores": 1, "kind": "port",
-        "sample": (f"oracle FP64 reference-form PDHG on the same LP: {sample_iters} iterations timed "
-                   f"({t_iter * 1e3:.1f} ms/iter, KKT {t_kkt * 1e3:.1f} ms per 50 it, setup {t_setup:.1f} s), "
-                   f"scaled to the {iters_needed} iterations the reference algorithm needs to 1e-4 "
-                   f"(counted on the GPU in the reference form, FP64); OMP/BLAS threads 1, "
-                   f"host cpu_count {os.cpu_count()}"),
-    }
-
-
-def batch_lp_bytes(pk, out, check_every: int, mask=None):
-    """Algorithmic bytes the batched kernel moves, summed over LPs.
-
-    Per iteration of LP k: K and K^T once each (values and indices, 8 B per
-    entry each; the line pointers stay in LDS), 32 B per column (c, tau, lo,
-    hi, x r/w, x partial sum r/w), 5 B per row (b, row kind).  Per check (two
-    candidates): K and K^T twice more, 64 B per column and 41 B per row of
-    FP64 sums and data."""
-    import numpy as np
-    nnz, m, n = pk.nnzs.astype(float), pk.ms.astype(float), pk.ns.astype(float)
-    mat = 8 * nnz
-    per_it = 2 * mat + 32 * n + 5 * m
-    per_check = 4 * mat + 64 * n + 41 * m
-    its = out["iterations"].astype(float)
-    checks = out["checks"].astype(float) + 1
-    tot = its * per_it + checks * per_check
-    return float(np.sum(tot if mask is None else tot[mask]))
-
-
-def bench_batch(args):
-    import numpy as np
-    import torch
-    from sovereign_solver_amd.engine.batch import BatchEngine, PackedBatch
-    from sovereign_solver_amd.engine.forms import box_form
-    from sovereign_solver_amd.engine.single import SingleEngine
-    from sovereign_solver_amd.workloads import uneven_batch
-
-    t0 = time.perf_counter()
-    probs = uneven_batch(args.batch, seed=0)
-    pk = PackedBatch(probs)
-    eng = BatchEngine(pk, block=args.block)
-    log(f"built {len(probs)} LPs (sum m={pk.M}, n={pk.N}, nnz={pk.Z}; LDS {pk.lds_bytes} B) "
-        f"in {time.perf_counter() - t0:.1f}s")
-    for w in range(args.warmup):
-        out, _, _ = eng.solve(args.tol, args.max_iters)
-        log(f"warmup {w}: optimal {int((out['status'] == 1).sum())}/{len(probs)}")
-    eng.set_warm_start(None, None, None)
-    torch.cuda.synchronize()
-    times, outs = [], []
-    for s in range(args.steps):
-        eng.set_warm_start(None, None, None)
-        torch.cuda.synchronize()
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-        t1 = time.perf_counter()
-        ev[0].record(eng.stream)                    # both parts, the FP64 part on the library's side stream
-        eng.launch(args.tol, args.max_iters)
-        ev[1].record(eng.stream)
-        out, xs, ys = eng.results()
-        times.append((time.perf_counter() - t1, ev[0].elapsed_time(ev[1]) / 1e3))
-        outs.append(out)
-        log(f"step {s}: wall {times[-1][0]:.3f}s kernels {times[-1][1]:.3f}s (parts concurrent) "
-            f"optimal {int((out['status'] == 1).sum())}/{len(probs)} "
-            f"iters mean {out['iterations'].mean():.0f} max {out['iterations'].max()}")
-    # each part alone, untimed by the step clock: the FP32 part's launch duration for the roofline
-    eng.set_warm_start(None, None, None)
-    torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-    ev[0].record(eng.stream)
-    eng.launch(args.tol, args.max_iters, parts=1)
-    ev[1].record(eng.stream)
-    eng.launch(args.tol, args.max_iters, parts=2)
-    ev[2].record(eng.stream)
-    eng.results()
-    t_part32, t_part64 = ev[0].elapsed_time(ev[1]) / 1e3, ev[1].elapsed_time(ev[2]) / 1e3
-    log(f"parts alone: FP32 part {t_part32:.3f}s, FP64 part {t_part64:.3f}s (sum {t_part32 + t_part64:.3f}s)")
-    wall = sum(t[0] for t in times)
-    n_ok = sum(int((o["status"] == 1).sum()) for o in outs)
-    value = n_ok / wall
-    bytes_alg = batch_lp_bytes(pk, outs[-1], 50)
-    # dominant kernel: the FP32 part (the large LPs)
-    bytes32 = batch_lp_bytes(pk, outs[-1], 50, mask=~pk.resident)
-    bytes64 = bytes_alg - bytes32
-    gbs = bytes32 / t_part32 / 1e9
-    gbs64 = bytes64 / t_part64 / 1e9 if t_part64 > 0 else None
-
-    # plain single-instance GPU PDHG on a random subset, one LP after another
-    rng = np.random.default_rng(123)
-    sub = rng.choice(len(probs), size=min(args.plain_sample, len(probs)), replace=False)
-    t_plain = 0.0
-    ok_plain = 0
-    for k in sub:
-        f = box_form(probs[k])
-        se = SingleEngine(f, "fp32")
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        r = se.solve(args.tol, args.max_iters)
-        t_plain += time.perf_counter() - t2
-        ok_plain += r["status"] == "optimal"
-        se.drop_graphs()
-        del se
-    plain = ok_plain / t_plain if t_plain > 0 else 0.0
-    log(f"plain single-instance GPU PDHG: {ok_plain}/{len(sub)} optimal, {plain:.2f} LPs/s")
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        from oracle import pdhg_ref
-        try:
-            from threadpoolctl import threadpool_limits
-            threadpool_limits(1)
-        except Exception:
-            pass
-        t3 = time.perf_counter()
-        done = 0
-        for k in rng.permutation(len(probs)):
-            pdhg_ref.solve(probs[k], max_iters=args.max_iters, tol=args.tol)
-            done += 1
-            if time.perf_counter() - t3 > args.cpu_cap_s:
-                break
-        tc = time.perf_counter() - t3
-        cpu = {"value": done / tc, "unit": "LPs/s", "cores": 1, "kind": "port",
-               "sample": (f"oracle FP64 reference-form PDHG (restatement of solver/lp/pdhg.py) on {done} LPs "
-                          f"drawn uniformly from the batch, solved to {args.tol:g} one after another in "
-                          f"{tc:.1f}s on 1 core (BLAS threads 1)")}
-        log(f"cpu baseline {cpu['value']:.3f} LPs/s over {done} LPs")
-
-    out = {
-        "metric": "LPs/sec to 1e-4 rel KKT: learned+batched vs plain PDHG, 1 GPU; SpMV HBM GB/s",
-        "value": value, "unit": "LPs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 iterates (m > ~260), fp64 iterates (LPs within 16 KB of LDS); fp32 matrix, fp64 KKT",
-        "data": "synthetic (seeded random sparse LPs, uneven sizes)",
-        "config": {"workload": f"batched restarted PDHG, {len(probs)} LPs, m log-uniform in [50,2000], n=2.5m, "
-                               f"density 1%, two concurrent persistent launches (FP32 part, FP64 part), cold start, "
-                               f"tol {args.tol:g}",
-                   "sum_m": pk.M, "sum_n": pk.N, "sum_nnz": pk.Z, "block": args.block,
-                   "lps_fp32_part": int(len(probs) - pk.n_resident), "lps_fp64_part": int(pk.n_resident),
-                   "kernel_fp32_part_s": t_part32, "kernel_fp64_part_s": t_part64,
-                   "fp64_part_achieved_gbs": gbs64, "bytes_per_launch_both_parts": bytes_alg,
-                   "iterations_mean": float(outs[-1]["iterations"].mean()),
-                   "iterations_max": int(outs[-1]["iterations"].max()),
-                   "plain_single_gpu_lps_per_s": plain, "plain_sample": int(len(sub)),
-                   "speedup_vs_plain": value / plain if plain > 0 else None,
-                   "kernel_s": times[-1][1]},
-        "roofline": {"bound": "hbm", "kernel": "batch_kernel<512, float> (FP32 part: the LPs above 16 KB of "
-                                               "FP64 LDS; one pass over K and one over K^T per iteration)",
-                     "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(f"uneven_{args.batch}/batch_kernel_fp32"),
-                     "bytes_per_launch": bytes32, "us_per_launch": t_part32 * 1e6},
-        "cpu_baseline": cpu,
-    }
-    if args.dump:
-        np.savez(args.dump, m=pk.ms, n=pk.ns, nnz=pk.nnzs, iterations=outs[-1]["iterations"],
-                 status=outs[-1]["status"], restarts=outs[-1]["restarts"])
-    calib_copy(args)
-    print(json.dumps(out), flush=True)
-
-
-def objective_errors(probs, results, k_max: int = 8):
-    """max |obj - obj*| / (1 + |obj*|) over the first k_max LPs, obj* by HiGHS dual simplex (FP64)."""
-    from scipy.optimize import linprog
-    errs = []
-    for p, r in list(zip(probs, results))[:k_max]:
-        h = linprog(p.c, A_ub=p.A_ub if p.n_ineq else None, b_ub=p.b_ub if p.n_ineq else None,
-                    A_eq=p.A_eq if p.n_eq else None, b_eq=p.b_eq if p.n_eq else None,
-                    bounds=list(zip(p.lb, [None if np.isinf(u) else u for u in p.ub])), method="highs-ds")
-        if h.status == 0:
-            errs.append(abs(r.objective - h.fun) / (1 + abs(h.fun)))
-    return max(errs) if errs else None
-
-
-def plain_rate(probs, tol, max_iters):
-    """Plain single-instance GPU PDHG (cold start, one LP after another): LPs/s."""
-    import torch
-    from sovereign_solver_amd.engine.forms import box_form
-    from sovereign_solver_amd.engine.single import SingleEngine
-    t, ok = 0.0, 0
-    for p in probs:
-        se = SingleEngine(box_form(p), "fp32")
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        r = se.solve(tol, max_iters)
-        t += time.perf_counter() - t0
-        ok += r["status"] == "optimal"
-        se.drop_graphs()
-    return ok / t if t > 0 else 0.0
-
-
-def cpu_rate(probs, tol, cap_s):
-    """The FP64 CPU oracle (reference-form PDHG, 1 core) over LPs of the sample until cap_s."""
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
-    from oracle import pdhg_ref
-    try:
-        from threadpoolctl import threadpool_limits
-        threadpool_limits(1)
-    except Exception:                        # pragma: no cover
-        pass
-    t0 = time.perf_counter()
-    done = 0
-    for p in probs:
-        pdhg_ref.solve(p, max_iters=100_000, tol=tol)
-        done += 1
-        if time.perf_counter() - t0 > cap_s:
-            break
-    tc = time.perf_counter() - t0
-    return {"value": done / tc, "unit": "LPs/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle FP64 reference-form PDHG (restatement of solver/lp/pdhg.py) on {done} LPs of the "
-                       f"same workload, solved to {tol:g} one after another in {tc:.1f}s on 1 core (BLAS threads 1)")}
-
-
-def family_roofline(probs, tol, traffic_key=None):
-    """Graph-timed column and row passes of the family engine over all LPs at
-    the cold-start state (every LP running): algorithmic bytes / mean time."""
-    from sovereign_solver_amd.engine.family import FamilyBatch, FamilyEngine
-    fb = FamilyBatch(probs)
-    eng = FamilyEngine(fb)
-    shape = eng.autotune(tol)
+def family_passes(eng, tol, traffic_key=None, reps: int = 20):
+    """Graph-timed column and row passes of a family engine over all its LPs at the
+    cold-start state (every LP running): algorithmic bytes / mean time."""
+    fb = eng.fb
     eng.start(tol)
-    us = {w: eng.time_step(w, reps=20) for w in ("primal", "dual")}
-    del eng
+    us = {w: eng.time_step(w, reps=reps) for w in ("primal", "dual")}
     b = {"primal": fb.column_pass_bytes(), "dual": fb.row_pass_bytes()}
     gbs = {w: b[w] / (us[w] * 1e-6) / 1e9 for w in us}
     dom = max(us, key=us.get)
-    names = {"primal": "column pass (primal_kernel + primal_long_kernel)",
-             "dual": "row pass (dual_kernel + dual_long_kernel)"}
+    names = {"primal": "family column pass (primal_kernel + primal_long_kernel)",
+             "dual": "family row pass (dual_kernel + dual_long_kernel)"}
     log(f"family passes: column {us['primal']:.1f} us ({b['primal'] / 1e6:.1f} MB, {gbs['primal']:.0f} GB/s), "
         f"row {us['dual']:.1f} us ({b['dual'] / 1e6:.1f} MB, {gbs['dual']:.0f} GB/s)")
     other = "dual" if dom == "primal" else "primal"
@@ -384,15 +297,11 @@ def family_roofline(probs, tol, traffic_key=None):
             "traffic": measured_traffic(f"{traffic_key}/{'column' if dom == 'primal' else 'row'}_pass")
             if traffic_key else None,
             "bytes_per_launch": b[dom], "us_per_launch": us[dom],
-            "launch_shape": shape,
+            "launch_shape": {"items_per_wave": eng.args.items_per_wave, "xcd_affinity": eng.args.xcd_affinity},
             "other_pass": {"kernel": names[other], "achieved": gbs[other], "bytes_per_launch": b[other],
-                           "us_per_launch": us[other]}}
-
-
-def family_rate(res, secs):
-    ok = sum(r.status == "optimal" for r in res)
-    its = np.array([r.iterations for r in res])
-    return {"lps_per_s": ok / secs, "s": secs, "ok": ok, "iters_mean": float(its.mean()), "iters_max": int(its.max())}
+                           "us_per_launch": us[other],
+                           "traffic": measured_traffic(f"{traffic_key}/{'row' if other == 'dual' else 'column'}"
+                                                       f"_pass") if traffic_key else None}}
 
 
 def mlp_forward_rate(mlp, batch: int, reps: int = 20):
@@ -414,181 +323,40 @@ def mlp_forward_rate(mlp, batch: int, reps: int = 20):
     us = e0.elapsed_time(e1) * 1e3 / reps
     flop = 2.0 * batch * (mlp.d_in * mlp.d_h + mlp.d_h * mlp.d_h + mlp.d_h * mlp.d_out)
     tflops = flop / (us * 1e-6) / 1e12
-    return {"batch": batch, "gflop": flop / 1e9, "us": us, "tflops": tflops, "peak_tflops": 2500.0,
-            "frac": tflops / 2500.0, "bound": "mfma (dense BF16)"}
+    return {"batch": batch, "gflop": flop / 1e9, "us": us, "tflops": tflops, "peak_tflops": BF16_PEAK_TFLOPS,
+            "frac": tflops / BF16_PEAK_TFLOPS, "bound": "mfma (dense BF16)"}
 
 
-def bench_learned(args):
-    """Learned warm start on the economic-dispatch family (200 generators x 24 periods)."""
-    import torch
-    from sovereign_solver_amd.engine.batch import BatchEngine, PackedBatch
-    from sovereign_solver_amd.learned import warmstart as W
-    from sovereign_solver_amd.workloads import DispatchFamily
-
-    fam = DispatchFamily()
-    log(f"dispatch family: m={fam.m} n={fam.n} nnz={fam.A.nnz}")
-    train_ids = np.arange(args.train)
-    eval_ids = 10**6 + np.arange(args.eval)
-    t0 = time.perf_counter()
-    lab = W.label(fam, train_ids, tol=args.tol, max_iters=args.max_iters)
-    t_label = time.perf_counter() - t0
-    keep = lab.status == "optimal"
-    log(f"labelled {len(train_ids)} instances in {t_label:.1f}s (engine {lab.seconds:.1f}s), optimal {int(keep.sum())}, "
-        f"iterations mean {lab.iterations.mean():.0f}")
-    t1 = time.perf_counter()
-    mlp, nin, nout, losses = W.train(fam.features_of(train_ids)[keep], W.targets_of(lab)[keep], d_h=512,
-                                     epochs=args.epochs, batch=256, lr=1e-3, out_blocks=W.output_blocks(fam))
-    t_train = time.perf_counter() - t1
-    log(f"trained in {t_train:.1f}s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
-    ls = W.LearnedStart(mlp, nin, nout)
-    mlp_rate = mlp_forward_rate(mlp, len(eval_ids))
-    log(f"network forward on {len(eval_ids)} instances: {mlp_rate['gflop']:.1f} GFLOP in {mlp_rate['us']:.1f} us "
-        f"({mlp_rate['tflops']:.0f} TFLOP/s, {100 * mlp_rate['frac']:.1f}% of dense BF16 peak)")
-    probs = [fam.problem(int(k)) for k in eval_ids]
-    roof = family_roofline(probs, args.tol, f"dispatch_{len(probs)}")
-
-    W.solve_family(fam, eval_ids[:64], args.tol, args.max_iters)                      # warm-up (build, graphs)
-    cold = family_rate(*W.solve_family(fam, eval_ids, args.tol, args.max_iters))
-    res_x, t_x = W.solve_learned(fam, eval_ids, ls, args.tol, args.max_iters, use_y=False)
-    learned = family_rate(res_x, t_x)
-    learned_full = family_rate(*W.solve_learned(fam, eval_ids, ls, args.tol, args.max_iters))
-    lab_ev = W.label(fam, eval_ids, tol=args.tol, max_iters=args.max_iters)
-    perfect = family_rate(*W.solve_family(fam, eval_ids, args.tol, args.max_iters,
-                                          warm=[(lab_ev.x[k], lab_ev.y[k], float(np.exp(lab_ev.log_step[k])))
-                                                for k in range(len(eval_ids))]))
-    log(f"cold {cold}; learned x+weight {learned}; learned x+y+weight {learned_full}; solution as start {perfect}")
-    pk = PackedBatch(probs)
-    be = BatchEngine(pk)
-    be.set_warm_start(None, None, None)
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    be.launch(args.tol, args.max_iters)
-    o, _, _ = be.results()
-    persistent = {"lps_per_s": int((o["status"] == 1).sum()) / (time.perf_counter() - t2),
-                  "iters_mean": float(o["iterations"].mean()), "iters_max": int(o["iterations"].max())}
-    del be
-    plain = plain_rate(probs[: args.plain_sample], args.tol, args.max_iters)
-    obj_err = objective_errors(probs, res_x)
-    log(f"persistent cold {persistent}; plain single-instance {plain:.2f} LPs/s; objective rel err vs HiGHS {obj_err}")
-    cpu = None if args.no_cpu_baseline else cpu_rate(probs, args.tol, args.cpu_cap_s)
-    value = learned["lps_per_s"]
-    out = {
-        "metric": "LPs/sec to 1e-4 rel KKT: learned+batched vs plain PDHG, 1 GPU; SpMV HBM GB/s",
-        "value": value, "unit": "LPs/s", "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": learned["s"] * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 iterates, fp64 KKT (MLP bf16 x bf16 -> fp32)", "data": "synthetic dispatch family",
-        "config": {"workload": f"learned warm start, economic dispatch 200x24 (m={fam.m}, n={fam.n}), "
-                               f"{args.train} PDHG-labelled training instances, {len(eval_ids)} held-out, "
-                               f"shared-matrix batched engine; value = learned (x + primal weight) start, "
-                               f"network inference and host<->device transfers included",
-                   "train_label_s": t_label, "train_s": t_train, "epochs": args.epochs,
-                   "mlp_forward_mfma": mlp_rate,
-                   "loss_first_last": [losses[0], losses[-1]],
-                   "cold_batched": cold, "learned_x_weight": learned, "learned_x_y_weight": learned_full,
-                   "solution_as_start": perfect, "persistent_engine_cold": persistent,
-                   "plain_single_gpu_lps_per_s": plain, "plain_sample": args.plain_sample,
-                   "speedup_vs_plain": value / plain if plain > 0 else None,
-                   "learned_vs_cold_batched": value / cold["lps_per_s"],
-                   "objective_rel_err_vs_highs_max": obj_err},
-        "roofline": roof, "cpu_baseline": cpu,
-    }
-    calib_copy(args)
-    print(json.dumps(out), flush=True)
-
-
-def bench_nodes(args):
-    """Branch-and-bound node relaxations of a set-covering MILP (1000 x 10000), batched."""
+def plain_rates(probs, tol, max_iters):
+    """Plain single-instance GPU PDHG (cold start), one LP after another: LPs/s with
+    the LP resident in HBM (engine built before the clock, solve only) and through the
+    drop-in call ``solve_lp_pdhg`` (host problem to host result)."""
     import torch
     from sovereign_solver_amd import solve_lp_pdhg
-    from sovereign_solver_amd.engine.batch import BatchEngine, PackedBatch
-    from sovereign_solver_amd.learned import warmstart as W
-    from sovereign_solver_amd.workloads import SetCoverNodes
-
-    sc = SetCoverNodes()
-    log(f"set cover: rows={sc.rows} cols={sc.cols} nnz={sc.A.nnz}")
-    root = solve_lp_pdhg(sc.root(), tol=args.tol, precision="fp32")
-    log(f"root LP: {root.status} it={root.iterations} obj={root.objective:.4f} weight={root.step_scale:.3g}")
-    node_ids = np.arange(args.nodes)
-    train_ids = 10**6 + np.arange(args.train)
-    probs = [sc.node(int(k)) for k in node_ids]
-    roof = family_roofline(probs, args.tol, f"setcover_{len(probs)}")
-    W.solve_family(sc, node_ids[:64], args.tol, args.max_iters)                       # warm-up
-    cold = family_rate(*W.solve_family(sc, node_ids, args.tol, args.max_iters))
-    parent_ws = [(np.clip(root.x, p.lb, p.ub), root.y, root.step_scale) for p in probs]
-    res_p, t_p = W.solve_family(sc, node_ids, args.tol, args.max_iters, warm=parent_ws)
-    parent = family_rate(res_p, t_p)
-    log(f"cold {cold}; parent warm start {parent}")
-    learned = learned_full = None
-    t_label = t_train = 0.0
-    losses = [None]
-    if args.train > 0:
+    from sovereign_solver_amd.engine.forms import box_form
+    from sovereign_solver_amd.engine.single import SingleEngine
+    from sovereign_solver_amd.engine._lib import PDLP_SCHEME_ADAPTIVE
+    t_res, ok_res, t_call, ok_call = 0.0, 0, 0.0, 0
+    its = []
+    for p in probs:
+        # the drop-in's own default for the box form: adaptive restarts and primal weight
+        se = SingleEngine(box_form(p), "fp32", scheme=PDLP_SCHEME_ADAPTIVE)
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
-        lab = W.label(sc, train_ids, tol=args.tol, max_iters=args.max_iters)
-        t_label = time.perf_counter() - t0
-        keep = lab.status == "optimal"
+        r = se.solve(tol, max_iters)
+        t_res += time.perf_counter() - t0
+        ok_res += r["status"] == "optimal"
+        its.append(r["iterations"])
+        se.drop_graphs()
+        del se
+        torch.cuda.synchronize()
         t1 = time.perf_counter()
-        mlp, nin, nout, losses = W.train(sc.features_of(train_ids)[keep], W.targets_of(lab)[keep], d_h=512,
-                                         epochs=args.epochs, batch=256, lr=1e-3, out_blocks=W.output_blocks(sc),
-                                         normalise_inputs=False)
-        t_train = time.perf_counter() - t1
-        log(f"labelled {len(train_ids)} nodes in {t_label:.1f}s, trained in {t_train:.1f}s, "
-            f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
-        ls = W.LearnedStart(mlp, nin, nout)
-        learned = family_rate(*W.solve_learned(sc, node_ids, ls, args.tol, args.max_iters, use_y=False))
-        learned_full = family_rate(*W.solve_learned(sc, node_ids, ls, args.tol, args.max_iters))
-        log(f"learned x+weight {learned}; learned x+y+weight {learned_full}")
-    pk = PackedBatch(probs)
-    be = BatchEngine(pk)
-    be.set_warm_start(None, None, None)
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    be.launch(args.tol, args.max_iters)
-    o, _, _ = be.results()
-    persistent = {"lps_per_s": int((o["status"] == 1).sum()) / (time.perf_counter() - t2),
-                  "iters_mean": float(o["iterations"].mean()), "iters_max": int(o["iterations"].max())}
-    del be
-    plain = plain_rate(probs[: args.plain_sample], args.tol, args.max_iters)
-    obj_err = objective_errors(probs, res_p)
-    # strong branching at the root: 2 x 20 child LPs in one batch, warm-started from the root
-    from sovereign_solver_amd.lp.branching import branch_strong_batched, child_problem, fractional_vars
-    mask = np.ones(sc.cols, bool)
-    rp = sc.root()
-    branch_strong_batched(root.x, mask, rp, rp.lb, rp.ub, root.objective, y=root.y, step_scale=root.step_scale)
-    torch.cuda.synchronize()
-    t4 = time.perf_counter()
-    var, g_down, g_up = branch_strong_batched(root.x, mask, rp, rp.lb, rp.ub, root.objective, y=root.y,
-                                              step_scale=root.step_scale)
-    t_sb = time.perf_counter() - t4
-    n_frac = int(fractional_vars(root.x, mask).size)
-    kids = [child_problem(rp, rp.lb, rp.ub, int(j), d, float(root.x[j]))
-            for j in fractional_vars(root.x, mask)[:2] for d in ("down", "up")]
-    seq = plain_rate(kids, args.tol, args.max_iters)
-    strong = {"fractional_vars": n_frac, "children": 2 * min(20, n_frac), "batched_s": t_sb,
-              "children_per_s_batched": 2 * min(20, n_frac) / t_sb, "children_per_s_single_cold": seq,
-              "branch_var": var, "down_gain": g_down, "up_gain": g_up}
-    log(f"strong branching at the root: {strong}")
-    log(f"persistent cold {persistent}; plain single-instance {plain:.2f} LPs/s; objective rel err vs HiGHS {obj_err}")
-    cpu = None if args.no_cpu_baseline else cpu_rate(probs, args.tol, args.cpu_cap_s)
-    value = parent["lps_per_s"]
-    out = {
-        "metric": "LPs/sec to 1e-4 rel KKT: learned+batched vs plain PDHG, 1 GPU; SpMV HBM GB/s",
-        "value": value, "unit": "LPs/s", "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": parent["s"] * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 iterates, fp64 KKT", "data": "synthetic set-cover MILP nodes",
-        "config": {"workload": f"branch-and-bound node relaxations, set cover {sc.rows}x{sc.cols} "
-                               f"(nnz {sc.A.nnz}), {len(probs)} nodes with 1-50 fixings, shared-matrix batched "
-                               f"engine warm-started from the parent (root) solution",
-                   "root_iterations": root.iterations, "cold_batched": cold, "parent_warm": parent,
-                   "learned_x_weight": learned, "learned_x_y_weight": learned_full,
-                   "train_nodes": args.train, "train_label_s": t_label, "train_s": t_train,
-                   "loss_first_last": [losses[0], losses[-1]], "persistent_engine_cold": persistent,
-                   "plain_single_gpu_lps_per_s": plain, "plain_sample": args.plain_sample,
-                   "speedup_vs_plain": value / plain if plain > 0 else None,
-                   "objective_rel_err_vs_highs_max": obj_err, "strong_branching_root": strong},
-        "roofline": roof, "cpu_baseline": cpu,
-    }
-    calib_copy(args)
-    print(json.dumps(out), flush=True)
+        rc = solve_lp_pdhg(p, max_iters=max_iters, tol=tol, precision="fp32")
+        t_call += time.perf_counter() - t1
+        ok_call += rc.status == "optimal"
+    return {"lps": len(probs), "resident_lps_per_s": ok_res / t_res if t_res else 0.0,
+            "drop_in_call_lps_per_s": ok_call / t_call if t_call else 0.0,
+            "optimal": ok_res, "iters_mean": float(np.mean(its)), "iters_max": int(np.max(its))}
 
 
 def calib_copy(args):
@@ -605,11 +373,520 @@ def calib_copy(args):
     log("calibration copies: 3 x 1 GiB read + 1 GiB written")
 
 
+# -------------------------------------------------------------- C3: headline
+def run_headline(args):
+    """Config 3: learned warm start + shared-matrix batched PDHG on the economic
+    dispatch family.  Returns (JSON line fields, roofline, cpu baseline)."""
+    import torch
+    from sovereign_solver_amd.engine.family import FamilyBatch, FamilyEngine
+    from sovereign_solver_amd.learned import warmstart as W
+    from sovereign_solver_amd.workloads import DispatchFamily
+
+    tol, mi = args.tol, args.max_iters
+    fam = DispatchFamily()
+    n, m = fam.n, fam.m
+    log(f"dispatch family 200x24: m={m} n={n} nnz={fam.A.nnz}; training on {args.train}, {args.eval} held out")
+    train_ids = np.arange(args.train)
+    eval_ids = 10**6 + np.arange(args.eval)
+    t0 = time.perf_counter()
+    lab = W.label(fam, train_ids, tol=tol, max_iters=mi)
+    t_label = time.perf_counter() - t0
+    keep = lab.status == "optimal"
+    log(f"labelled {len(train_ids)} in {t_label:.1f}s (engine {lab.seconds:.1f}s), optimal {int(keep.sum())}, "
+        f"iterations mean {lab.iterations.mean():.0f}")
+    t1 = time.perf_counter()
+    mlp, nin, nout, losses = W.train(fam.features_of(train_ids)[keep], W.targets_of(lab)[keep], d_h=512,
+                                     epochs=args.epochs, batch=256, lr=1e-3, out_blocks=W.output_blocks(fam))
+    t_train = time.perf_counter() - t1
+    log(f"trained {args.epochs} epochs in {t_train:.1f}s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    ls = W.LearnedStart(mlp, nin, nout)
+    mlp_rate = mlp_forward_rate(mlp, len(eval_ids))
+    log(f"network forward on {len(eval_ids)}: {mlp_rate['gflop']:.1f} GFLOP in {mlp_rate['us']:.1f} us "
+        f"({mlp_rate['tflops']:.0f} TFLOP/s, {100 * mlp_rate['frac']:.1f}% of dense BF16 peak)")
+
+    probs = [fam.problem(int(k)) for k in eval_ids]
+    t2 = time.perf_counter()
+    eng = FamilyEngine(FamilyBatch(probs))
+    shape = eng.autotune(tol)
+    t_setup = time.perf_counter() - t2
+    log(f"engine for {len(probs)} LPs built in {t_setup:.2f}s (once per family and batch size), shape {shape}")
+    with torch.cuda.stream(mlp.stream):
+        feats_dev = torch.from_numpy(fam.features_from(probs).astype(np.float32)).to(mlp.device)
+    mlp.stream.synchronize()
+    out_host = torch.empty(eng.t["out_final"].shape, dtype=torch.uint8).pin_memory()
+    from sovereign_solver_amd.engine.batch import OUT_DTYPE
+
+    def records():
+        return np.frombuffer(out_host.numpy().tobytes(), dtype=OUT_DTYPE).copy()
+
+    def learned_resident():
+        out, ev = ls.forward_device(feats_dev)
+        eng.start_from_device(tol, out, x_off=0, y_off=-1, w_col=n + m, scale=ls.out_scale, shift=ls.out_shift,
+                              ready=ev)
+        eng.run(mi)
+        eng.collect()
+        with torch.cuda.stream(eng.stream):
+            out_host.copy_(eng.t["out_final"], non_blocking=True)
+
+    def cold_resident():
+        eng.start(tol)
+        eng.run(mi)
+        eng.collect()
+        with torch.cuda.stream(eng.stream):
+            out_host.copy_(eng.t["out_final"], non_blocking=True)
+
+    def timed(fn, steps):
+        torch.cuda.synchronize()
+        ok, secs, last = 0, 0.0, None
+        for _ in range(steps):
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            secs += time.perf_counter() - t
+            last = records()
+            ok += int((last["status"] == 1).sum())
+        return ok, secs, last
+
+    for _ in range(args.warmup):
+        learned_resident()
+    torch.cuda.synchronize()
+    ok, secs, rec = timed(learned_resident, args.steps)
+    value = ok / secs
+    log(f"learned + batched, resident: {args.steps} steps x {len(probs)} LPs, {ok} optimal in {secs:.3f}s -> "
+        f"{value:.0f} LPs/s (iterations mean {rec['iterations'].mean():.0f}, max {rec['iterations'].max()}, "
+        f"device rel KKT max {rec['rel_kkt'].max():.2e})")
+    learned = {"lps_per_s": value, "ms_per_step": secs * 1e3 / args.steps, "optimal_per_step": ok / args.steps,
+               "iters_mean": float(rec["iterations"].mean()), "iters_max": int(rec["iterations"].max()),
+               "device_rel_kkt_max": float(rec["rel_kkt"].max())}
+    cold_resident()
+    ok_c, secs_c, rec_c = timed(cold_resident, max(2, min(args.steps, 5)))
+    cold = {"lps_per_s": ok_c / secs_c, "iters_mean": float(rec_c["iterations"].mean()),
+            "iters_max": int(rec_c["iterations"].max())}
+    log(f"cold batched, resident: {cold['lps_per_s']:.0f} LPs/s (iterations mean {cold['iters_mean']:.0f}, "
+        f"max {cold['iters_max']})")
+
+    # host to host: problems on the host -> features, packing, inference, solve -> results and objectives
+    def learned_h2h():
+        feats = fam.features_from(probs)
+        eng.load(probs)
+        out, ev = ls.forward(feats)
+        eng.start_from_device(tol, out, x_off=0, y_off=-1, w_col=n + m, scale=ls.out_scale, shift=ls.out_shift,
+                              ready=ev)
+        eng.run(mi)
+        o, X, Y = eng.results()
+        return o, eng.objectives(X)
+
+    def cold_h2h():
+        eng.load(probs)
+        eng.start(tol)
+        eng.run(mi)
+        o, X, Y = eng.results()
+        return o, eng.objectives(X)
+
+    h2h = {}
+    for name, fn in (("learned", learned_h2h), ("cold", cold_h2h)):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            o, obj = fn()
+        dt = (time.perf_counter() - t) / reps
+        h2h[name] = {"lps_per_s": int((o["status"] == 1).sum()) / dt, "ms_per_batch": dt * 1e3}
+        if name == "learned":
+            learned_obj = obj.copy()
+    log(f"host to host: learned {h2h['learned']['lps_per_s']:.0f} LPs/s ({h2h['learned']['ms_per_batch']:.1f} ms "
+        f"per batch), cold {h2h['cold']['lps_per_s']:.0f} LPs/s")
+    roof = family_passes(eng, tol, f"dispatch_{len(probs)}")
+    del eng
+    plain = plain_rates(probs[: args.plain_sample], tol, mi)
+    log(f"plain single-instance GPU PDHG on {plain['lps']} of the held-out LPs: resident "
+        f"{plain['resident_lps_per_s']:.2f} LPs/s, drop-in call {plain['drop_in_call_lps_per_s']:.2f} LPs/s")
+    oe = objective_errors(probs, learned_obj, args.obj_sample)
+    log(f"objective vs HiGHS on {oe['lps']} LPs: max {oe['max']:.2e}, median {oe['median']:.2e} ({oe['s']:.1f}s)")
+    cpu = None if args.no_cpu_baseline else cpu_baseline("dispatch", eval_ids, tol, args.cpu_cap_s,
+                                                         "held-out dispatch LPs")
+    fields = {
+        "value": value, "ms_per_step": learned["ms_per_step"],
+        "dtype": "fp32 iterates and SpMV, fp64 KKT and objectives; network bf16 x bf16 -> fp32",
+        "data": "synthetic: seeded economic-dispatch family (random-init network trained on PDHG's own solutions)",
+        "config": {
+            "workload": (f"C3 learned warm start + shared-matrix batched restarted PDHG, economic dispatch 200x24 "
+                         f"(m={m}, n={n}), {len(probs)} held-out LPs per step, tol {tol:g}; step = network "
+                         f"inference + device-side start + batched solve, inputs and results in HBM"),
+            "lps_per_step": len(probs), "train_instances": args.train, "epochs": args.epochs, "width": 512,
+            "train_label_s": t_label, "train_s": t_train, "loss_first_last": [losses[0], losses[-1]],
+            "engine_setup_s": t_setup, "launch_shape": shape, "mlp_forward_mfma": mlp_rate,
+            "learned_batched_resident": learned, "cold_batched_resident": cold,
+            "host_to_host": h2h, "plain_single_gpu": plain,
+            "speedup_vs_plain": value / plain["resident_lps_per_s"] if plain["resident_lps_per_s"] else None,
+            "speedup_vs_plain_host_to_host": (h2h["learned"]["lps_per_s"] / plain["drop_in_call_lps_per_s"]
+                                              if plain["drop_in_call_lps_per_s"] else None),
+            "learned_vs_cold_batched": value / cold["lps_per_s"],
+            "objective_rel_err_vs_fp64": oe,
+        },
+    }
+    return fields, roof, cpu
+
+
+# ---------------------------------------------------------- C1: transport LP
+def run_transport(args, sources=1000, sinks=2000, cpu=True):
+    """Config 1: one transportation LP, plain restarted PDHG (cold start)."""
+    import torch
+    from sovereign_solver_amd.engine.forms import box_form, extract, reference_form
+    from sovereign_solver_amd.engine.single import SingleEngine
+    from sovereign_solver_amd.engine import tune as T
+    from sovereign_solver_amd.workloads import transportation_lp
+
+    tol, mi = args.tol, args.max_iters
+    p = transportation_lp(sources, sinks, seed=0)
+    form = box_form(p)
+    eng = SingleEngine(form, "fp32")
+    tune = T.autotune(eng) if not args.no_autotune else eng.tune
+    eng.set_tune(tune)
+    del eng
+    torch.cuda.synchronize()
+    # host to host: the box form, the engine (upload, K^T and SELL build), the solve, the extraction
+    t0 = time.perf_counter()
+    f2 = box_form(p)
+    t1 = time.perf_counter()
+    eng = SingleEngine(f2, "fp32", tune=tune)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    r = eng.solve(tol, mi)
+    x, obj, y = extract(f2, p, r["x"], r["y"])
+    t3 = time.perf_counter()
+    h2h = {"s": t3 - t0, "lps_per_s": (r["status"] == "optimal") / (t3 - t0), "box_form_s": t1 - t0,
+           "engine_setup_s": t2 - t1, "solve_and_extract_s": t3 - t2}
+    log(f"transport {sources}x{sinks} host to host {h2h['s']:.3f}s (box form {t1 - t0:.3f}, engine {t2 - t1:.3f}, "
+        f"solve + extract {t3 - t2:.3f})")
+    steps = args.c1_steps
+    torch.cuda.synchronize()
+    t4 = time.perf_counter()
+    results = [eng.solve(tol, mi) for _ in range(steps)]
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t4
+    ok = sum(rr["status"] == "optimal" for rr in results)
+    log(f"transport resident: {ok}/{steps} optimal, {secs / steps * 1e3:.0f} ms per solve, "
+        f"iterations {results[-1]['iterations']}, rel KKT {results[-1]['rel_kkt']:.2e}")
+    obj_ref = highs_objective(p) if not args.no_highs else None
+    us_p = eng.time_kernel("primal", eng.tune, reps=50)
+    us_d = eng.time_kernel("dual", eng.tune, reps=50)
+    bp, bd = primal_bytes(form, 4), dual_bytes(form, 4)
+    gp, gd = bp / (us_p * 1e-6) / 1e9, bd / (us_d * 1e-6) / 1e9
+    spmv = spmv_rates(eng)
+    log(f"primal step {us_p:.2f} us ({gp:.0f} GB/s), dual step {us_d:.2f} us ({gd:.0f} GB/s); SpMV K^T y "
+        f"{spmv['KT_y']['achieved_gbs']:.0f}, K x {spmv['K_x']['achieved_gbs']:.0f} GB/s")
+    key = f"transport_{sources}x{sinks}"
+    rec = {
+        "workload": f"C1 single transportation LP {sources}x{sinks} (m={form.m}, n={form.n}, nnz={form.nnz}), "
+                    f"plain restarted PDHG from a cold start, tol {tol:g}",
+        "clock": "resident: LP in HBM, one solve to tolerance per step; host_to_host: box form, engine set-up "
+                 "(upload, device K^T, SELL slices), solve and extraction",
+        "value": ok / secs, "unit": "LPs/s", "steps": steps, "ms_per_step": secs * 1e3 / steps,
+        "iterations": results[-1]["iterations"], "restarts": results[-1]["restarts"],
+        "rel_kkt": results[-1]["rel_kkt"], "objective": float(obj),
+        "objective_rel_err_vs_fp64": None if obj_ref is None else abs(obj - obj_ref) / (1 + abs(obj_ref)),
+        "host_to_host": h2h, "tune": tune.__dict__,
+        "roofline": {"bound": "hbm", "kernel": "primal_kernel (fused K^T y + step + projection + sums)",
+                     "achieved": gp, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gp / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(f"{key}/primal_kernel"), "bytes_per_launch": bp,
+                     "us_per_launch": us_p,
+                     "dual_kernel": {"achieved": gd, "frac": gd / HBM_PEAK_GBS, "bytes_per_launch": bd,
+                                     "us_per_launch": us_d, "traffic": measured_traffic(f"{key}/dual_kernel")},
+                     "spmv_standalone": spmv},
+    }
+    if cpu and not args.no_cpu_baseline:
+        rf = reference_form(p)
+        reng = SingleEngine(rf, "fp64")
+        rr = reng.solve(tol, mi)
+        ref_iters = rr["iterations"] if rr["status"] == "optimal" else None
+        del reng
+        rec["reference_form_iterations"] = ref_iters
+        rec["cpu_baseline"] = transport_cpu(p, ref_iters or results[-1]["iterations"], args.cpu_sample_iters)
+    del eng
+    return rec
+
+
+def transport_cpu(problem, iters_needed: int, sample_iters: int):
+    """The CPU oracle per iteration on a bounded sample of the transportation LP,
+    scaled to the iterations the reference algorithm needs (a full run takes minutes)."""
+    _cpu_init()
+    from oracle import pdhg_ref
+    t0 = time.perf_counter()
+    sf = pdhg_ref.to_standard_form(problem)
+    tau, sigma = pdhg_ref.step_sizes(sf.A)
+    AT = sf.A.T.tocsr()
+    t_setup = time.perf_counter() - t0
+    A, b, c = sf.A, sf.b, sf.c
+    x = np.zeros(A.shape[1]); y = np.zeros(A.shape[0])
+    xs = np.zeros_like(x); ys = np.zeros_like(y)
+    t1 = time.perf_counter()
+    for _ in range(sample_iters):            # the loop body of pdhg.py:209-222
+        xn = np.maximum(x - tau * (c - AT @ y), 0.0)
+        xb = 2.0 * xn - x
+        y = y + sigma * (b - A @ xb)
+        x = xn
+        xs += x
+        ys += y
+    t_iter = (time.perf_counter() - t1) / sample_iters
+    t2 = time.perf_counter()
+    norm_b = max(1.0, float(np.linalg.norm(b))); norm_c = max(1.0, float(np.linalg.norm(c)))
+    pdhg_ref.kkt(A, AT, b, c, norm_b, norm_c, xs / sample_iters, ys / sample_iters)
+    t_kkt = time.perf_counter() - t2
+    total = t_setup + iters_needed * (t_iter + t_kkt / 50.0)
+    return {"value": 1.0 / total, "unit": "LPs/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle FP64 reference-form PDHG on the same LP: {sample_iters} iterations timed "
+                       f"({t_iter * 1e3:.1f} ms/iter, KKT {t_kkt * 1e3:.1f} ms per 50 it, setup {t_setup:.1f} s), "
+                       f"scaled to the {iters_needed} iterations the reference algorithm needs to 1e-4 "
+                       f"(counted on the GPU in the reference form, FP64); 1 core, BLAS threads 1")}
+
+
+# ---------------------------------------------------- C2: uneven batched LPs
+def batch_lp_bytes(pk, out, mask=None):
+    """Bytes the persistent kernel moves at minimum, summed over LPs.
+
+    Per iteration of LP k: K and K^T once each at 6 B per entry (FP32 value, 16-bit
+    index; the line pointers stay in LDS), 32 B per column (c, tau, lo, hi, x r/w, x
+    partial sum r/w), 5 B per row (b, row kind).  Per check (two candidates): K and
+    K^T twice more, 64 B per column and 41 B per row of FP64 sums and data."""
+    nnz, m, n = pk.nnzs.astype(float), pk.ms.astype(float), pk.ns.astype(float)
+    mat = 6 * nnz
+    per_it = 2 * mat + 32 * n + 5 * m
+    per_check = 4 * mat + 64 * n + 41 * m
+    its = out["iterations"].astype(float)
+    checks = out["checks"].astype(float) + 1
+    tot = its * per_it + checks * per_check
+    return float(np.sum(tot if mask is None else tot[mask]))
+
+
+def run_batch(args, cpu=True):
+    """Config 2: 4,096 uneven random LPs, two persistent launches."""
+    import torch
+    from sovereign_solver_amd.engine.batch import BatchEngine, PackedBatch
+    from sovereign_solver_amd.lp.batch import results_of
+    from sovereign_solver_amd.workloads import uneven_batch
+
+    tol, mi = args.tol, args.max_iters
+    probs = uneven_batch(args.batch, seed=0)
+    # host to host: packing (K, K^T with 16-bit indices, descriptors), upload, solve, results
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pk = PackedBatch(probs)
+    t1 = time.perf_counter()
+    eng = BatchEngine(pk)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    out, xs, ys = eng.solve(tol, mi)
+    res = results_of(eng, out, xs, ys)
+    t3 = time.perf_counter()
+    h2h = {"s": t3 - t0, "lps_per_s": sum(r.status == "optimal" for r in res) / (t3 - t0), "pack_s": t1 - t0,
+           "upload_s": t2 - t1, "solve_and_results_s": t3 - t2}
+    log(f"uneven batch host to host {h2h['s']:.2f}s (pack {t1 - t0:.2f}, upload {t2 - t1:.2f}, "
+        f"solve + results {t3 - t2:.2f}) -> {h2h['lps_per_s']:.0f} LPs/s")
+    steps = args.c2_steps
+    times, outs = [], []
+    for _ in range(steps):
+        eng.set_warm_start(None, None, None)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        t = time.perf_counter()
+        ev[0].record(eng.stream)
+        eng.launch(tol, mi)
+        ev[1].record(eng.stream)
+        eng.stream.synchronize()
+        times.append((time.perf_counter() - t, ev[0].elapsed_time(ev[1]) / 1e3))
+        outs.append(np.frombuffer(eng.t["out"].cpu().numpy().tobytes(), dtype=out.dtype).copy())
+    wall = sum(t[0] for t in times)
+    n_ok = sum(int((o["status"] == 1).sum()) for o in outs)
+    value = n_ok / wall
+    log(f"uneven batch resident: {n_ok}/{steps * len(probs)} optimal, {wall / steps:.3f}s per batch -> "
+        f"{value:.0f} LPs/s")
+    # each part alone: the FP32 part's launch for the roofline
+    eng.set_warm_start(None, None, None)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record(eng.stream)
+    eng.launch(tol, mi, parts=1)
+    ev[1].record(eng.stream)
+    eng.launch(tol, mi, parts=2)
+    ev[2].record(eng.stream)
+    eng.results()
+    t32, t64 = ev[0].elapsed_time(ev[1]) / 1e3, ev[1].elapsed_time(ev[2]) / 1e3
+    b32 = batch_lp_bytes(pk, outs[-1], mask=~pk.resident)
+    b64 = batch_lp_bytes(pk, outs[-1], mask=pk.resident)
+    g32 = b32 / t32 / 1e9
+    traffic = measured_traffic(f"uneven_{args.batch}/batch_kernel_fp32")
+    log(f"FP32 part {t32:.3f}s ({b32 / 1e12:.3f} TB -> {g32:.0f} GB/s), FP64 part {t64:.3f}s")
+    rng = np.random.default_rng(123)
+    sub = rng.choice(len(probs), size=min(args.c2_plain, len(probs)), replace=False)
+    plain = plain_rates([probs[k] for k in sub], tol, mi)
+    log(f"plain single-instance on {plain['lps']} LPs: resident {plain['resident_lps_per_s']:.1f} LPs/s, "
+        f"drop-in call {plain['drop_in_call_lps_per_s']:.1f} LPs/s")
+    osub = rng.choice(len(probs), size=min(args.obj_sample, len(probs)), replace=False)
+    # random LPs of this family can take HiGHS's simplex minutes: interior point, 5 s per LP
+    oe = objective_errors([probs[k] for k in osub], [res[k].objective for k in osub], len(osub),
+                          method="highs-ipm", time_limit=5.0, total_cap_s=60.0)
+    log(f"objective vs HiGHS on {oe['lps']} LPs: max {oe['max']:.2e}, median {oe['median']:.2e}")
+    rec = {
+        "workload": f"C2 batched restarted PDHG, {len(probs)} LPs, m log-uniform in [50,2000], n=2.5m, density 1%, "
+                    f"two concurrent persistent launches (FP32 part, FP64 part), cold start, tol {tol:g}",
+        "clock": "resident: packed batch in HBM, both parts launched and finished per step; host_to_host: "
+                 "packing, upload, solve and SolveResults",
+        "value": value, "unit": "LPs/s", "steps": steps, "ms_per_step": wall * 1e3 / steps,
+        "sum_m": pk.M, "sum_n": pk.N, "sum_nnz": pk.Z, "lps_fp32_part": int(len(probs) - pk.n_resident),
+        "lps_fp64_part": int(pk.n_resident), "kernel_s_both_parts": times[-1][1],
+        "iterations_mean": float(outs[-1]["iterations"].mean()), "iterations_max": int(outs[-1]["iterations"].max()),
+        "host_to_host": h2h, "plain_single_gpu": plain,
+        "speedup_vs_plain": value / plain["resident_lps_per_s"] if plain["resident_lps_per_s"] else None,
+        "objective_rel_err_vs_fp64": oe,
+        "roofline": {"bound": "hbm", "kernel": "batch_kernel<512, float> (FP32 part: one pass over K and one over "
+                                               "K^T per iteration, FP32 values and 16-bit indices)",
+                     "achieved": g32, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": g32 / HBM_PEAK_GBS,
+                     "traffic": traffic, "bytes_per_launch": b32, "us_per_launch": t32 * 1e6,
+                     "traffic_rate_gbs": None if traffic is None else traffic / t32 / 1e9,
+                     "fp64_part": {"s": t64, "bytes": b64, "achieved": b64 / t64 / 1e9 if t64 > 0 else None}},
+    }
+    if cpu and not args.no_cpu_baseline:
+        rec["cpu_baseline"] = cpu_baseline(f"uneven/{len(probs)}", np.arange(len(probs)), tol, args.cpu_cap_s / 2,
+                                           "LPs drawn uniformly from the uneven batch")
+    del eng
+    return rec
+
+
+# ---------------------------------------------------- C4: set-cover node LPs
+def run_nodes(args, cpu=True):
+    """Config 4: 256 branch-and-bound node relaxations of a set-covering MILP,
+    one shared-matrix batch warm-started from the parent (root) solution."""
+    import torch
+    from sovereign_solver_amd import solve_lp_pdhg
+    from sovereign_solver_amd.engine.family import FamilyBatch, FamilyEngine
+    from sovereign_solver_amd.lp.batch import family_warm_start
+    from sovereign_solver_amd.workloads import SetCoverNodes
+
+    tol, mi = args.tol, args.max_iters
+    sc = SetCoverNodes()
+    root = solve_lp_pdhg(sc.root(), tol=tol, precision="fp32")
+    log(f"set cover {sc.rows}x{sc.cols}: root {root.status} it={root.iterations} obj={root.objective:.4f}")
+    node_ids = np.arange(args.nodes)
+    probs = [sc.node(int(k)) for k in node_ids]
+    eng = FamilyEngine(FamilyBatch(probs))
+    shape = eng.autotune(tol)
+    ws = [(np.clip(root.x, p.lb, p.ub), root.y, root.step_scale) for p in probs]
+    X0, Y0, w0 = family_warm_start(eng.fb, ws)
+    out_host = torch.empty(eng.t["out_final"].shape, dtype=torch.uint8).pin_memory()
+    from sovereign_solver_amd.engine.batch import OUT_DTYPE
+
+    def solve_parent():
+        eng.start(tol, X0, Y0, w0)
+        eng.run(mi)
+        eng.collect()
+        with torch.cuda.stream(eng.stream):
+            out_host.copy_(eng.t["out_final"], non_blocking=True)
+        eng.stream.synchronize()
+        return np.frombuffer(out_host.numpy().tobytes(), dtype=OUT_DTYPE).copy()
+
+    def solve_cold():
+        eng.start(tol)
+        eng.run(mi)
+        eng.collect()
+        with torch.cuda.stream(eng.stream):
+            out_host.copy_(eng.t["out_final"], non_blocking=True)
+        eng.stream.synchronize()
+        return np.frombuffer(out_host.numpy().tobytes(), dtype=OUT_DTYPE).copy()
+
+    rates = {}
+    for name, fn in (("parent_warm", solve_parent), ("cold", solve_cold)):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        steps = args.c4_steps
+        ok = 0
+        for _ in range(steps):
+            o = fn()
+            ok += int((o["status"] == 1).sum())
+        dt = time.perf_counter() - t
+        rates[name] = {"lps_per_s": ok / dt, "ms_per_batch": dt * 1e3 / steps,
+                       "iters_mean": float(o["iterations"].mean()), "iters_max": int(o["iterations"].max())}
+    log(f"nodes: parent warm {rates['parent_warm']}, cold {rates['cold']}")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    eng.load(probs)
+    eng.start(tol, X0, Y0, w0)
+    eng.run(mi)
+    o, X, Y = eng.results()
+    obj = eng.objectives(X)
+    dt = time.perf_counter() - t
+    h2h = {"lps_per_s": int((o["status"] == 1).sum()) / dt, "ms_per_batch": dt * 1e3}
+    roof = family_passes(eng, tol, f"setcover_{len(probs)}")
+    del eng
+    plain = plain_rates(probs[: args.c4_plain], tol, mi)
+    oe = objective_errors(probs, obj, args.c4_obj_sample, total_cap_s=60.0)
+    log(f"nodes plain {plain}; objective vs HiGHS on {oe['lps']}: max {oe['max']:.2e} ({oe['s']:.1f}s)")
+    rec = {
+        "workload": f"C4 branch-and-bound node relaxations, set cover {sc.rows}x{sc.cols} (nnz {sc.A.nnz}), "
+                    f"{len(probs)} nodes with 1-50 fixings, shared-matrix batch warm-started from the parent (root) "
+                    f"solution, tol {tol:g}",
+        "clock": "resident: node data in HBM, start from the parent point (host arrays), solve, results in HBM; "
+                 "host_to_host adds the node data's packing and the results' copy and objectives",
+        "value": rates["parent_warm"]["lps_per_s"], "unit": "LPs/s", "steps": args.c4_steps,
+        "root_iterations": root.iterations, "parent_warm": rates["parent_warm"], "cold": rates["cold"],
+        "host_to_host": h2h, "launch_shape": shape, "plain_single_gpu": plain,
+        "speedup_vs_plain": (rates["parent_warm"]["lps_per_s"] / plain["resident_lps_per_s"]
+                             if plain["resident_lps_per_s"] else None),
+        "objective_rel_err_vs_fp64": oe, "roofline": roof,
+    }
+    if cpu and not args.no_cpu_baseline:
+        rec["cpu_baseline"] = cpu_baseline("setcover", node_ids, tol, args.cpu_cap_s / 2, "the set-cover nodes")
+    return rec
+
+
+# ------------------------------------------------ beyond the Infinity Cache
+def run_beyond_mall(args, sources=4000, sinks=8000):
+    """Standalone SpMV and the fused steps on a transportation LP whose K and K^T
+    (64M entries, 0.5 GB each in FP32 with int32 indices) exceed the 256 MB MALL."""
+    import torch
+    from sovereign_solver_amd.engine.forms import box_form
+    from sovereign_solver_amd.engine.single import SingleEngine
+    from sovereign_solver_amd.engine import tune as T
+    from sovereign_solver_amd.workloads import transportation_lp
+
+    t0 = time.perf_counter()
+    p = transportation_lp(sources, sinks, seed=0)
+    form = box_form(p)
+    del p
+    eng = SingleEngine(form, "fp32")
+    tune = T.autotune(eng, reps=10) if not args.no_autotune else eng.tune
+    eng.set_tune(tune)
+    log(f"transport {sources}x{sinks} (n={form.n}, nnz={form.nnz}) built and tuned in {time.perf_counter() - t0:.1f}s")
+    us_p = eng.time_kernel("primal", eng.tune, reps=10)
+    us_d = eng.time_kernel("dual", eng.tune, reps=10)
+    bp, bd = primal_bytes(form, 4), dual_bytes(form, 4)
+    gp, gd = bp / (us_p * 1e-6) / 1e9, bd / (us_d * 1e-6) / 1e9
+    spmv = spmv_rates(eng, reps=10, kx_tpr=64)
+    key = f"transport_{sources}x{sinks}"
+    log(f"beyond MALL: primal step {us_p:.1f} us ({gp:.0f} GB/s), dual step {us_d:.1f} us ({gd:.0f} GB/s), "
+        f"SpMV K^T y {spmv['KT_y']['achieved_gbs']:.0f} GB/s, K x {spmv['K_x']['achieved_gbs']:.0f} GB/s")
+    rec = {
+        "workload": f"transportation LP {sources}x{sinks} (m={form.m}, n={form.n}, nnz={form.nnz}): K and K^T "
+                    f"0.5 GB each, working set of a step ~{bp / 1e9:.2f} GB > 256 MB Infinity Cache",
+        "clock": "HIP graphs of back-to-back launches at the cold-start state, events on the engine stream",
+        "tune": tune.__dict__,
+        "primal_kernel": {"achieved": gp, "frac": gp / HBM_PEAK_GBS, "bytes_per_launch": bp, "us_per_launch": us_p,
+                          "traffic": measured_traffic(f"{key}/primal_kernel")},
+        "dual_kernel": {"achieved": gd, "frac": gd / HBM_PEAK_GBS, "bytes_per_launch": bd, "us_per_launch": us_d,
+                        "traffic": measured_traffic(f"{key}/dual_kernel")},
+        "spmv_standalone": spmv,
+    }
+    del eng
+    torch.cuda.empty_cache()
+    return rec
+
+
+# ---------------------------------------------------- branch-and-bound demo
 def bench_bnb(args):
     """Branch-and-bound of a set-covering MILP with batched node relaxations."""
     from scipy.optimize import Bounds, LinearConstraint, milp
-    from oracle import pdhg_ref
-    from sovereign_solver_amd.lp.branching import node_problem
     from sovereign_solver_amd.milp import solve_milp_batched
     from sovereign_solver_amd.workloads import unit_cost_set_cover
 
@@ -632,38 +909,9 @@ def bench_bnb(args):
     t_s = time.perf_counter() - t0
     seq_rate = seq.nodes / t_s
     log(f"one node at a time (batch 1, {t_s:.0f}s sample): {seq.nodes} nodes -> {seq_rate:.0f} node LPs/s")
-    # node relaxations of the same tree shape for the roofline of the family passes
-    rng = np.random.default_rng(7)
-    probs = []
-    for _ in range(256):
-        lb, ub = p.lb.copy(), p.ub.copy()
-        fix = rng.choice(p.n_vars, 8, replace=False)
-        v = rng.integers(0, 2, 8).astype(float)
-        lb[fix], ub[fix] = v, v
-        probs.append(node_problem(p, lb, ub))
-    roof = family_roofline(probs, args.tol)
-    cpu = None
-    if not args.no_cpu_baseline:
-        try:
-            from threadpoolctl import threadpool_limits
-            threadpool_limits(1)
-        except Exception:
-            pass
-        t0 = time.perf_counter()
-        done = 0
-        while time.perf_counter() - t0 < args.cpu_cap_s:
-            pdhg_ref.solve(probs[done % len(probs)], max_iters=args.max_iters, tol=args.tol)
-            done += 1
-        tc = time.perf_counter() - t0
-        cpu = {"value": done / tc, "unit": "LPs/s", "cores": 1, "kind": "port",
-               "sample": (f"oracle FP64 reference-form PDHG (restatement of solver/lp/pdhg.py) on {done} node "
-                          f"relaxations of this MILP (8 random fixings each), to {args.tol:g}, one after another "
-                          f"in {tc:.1f}s on 1 core")}
-        log(f"cpu baseline {cpu['value']:.2f} LPs/s")
     out = {
-        "metric": "LPs/sec to 1e-4 rel KKT: learned+batched vs plain PDHG, 1 GPU; SpMV HBM GB/s",
-        "value": rate, "unit": "LPs/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": t_b * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "metric": METRIC, "value": rate, "unit": "LPs/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": t_b * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32 iterates, fp64 KKT", "data": "synthetic set-covering MILP",
         "config": {"workload": f"branch-and-bound, set cover {p.n_constraints}x{p.n_vars} with near-unit costs, "
                                f"best-bound rounds of {args.bnb_batch} node relaxations solved as one shared-matrix "
@@ -674,181 +922,97 @@ def bench_bnb(args):
                    "objective_rel_err": abs(res.objective - ref.fun) / (1 + abs(ref.fun)),
                    "one_node_at_a_time_node_lps_per_s": seq_rate,
                    "speedup_vs_one_at_a_time": rate / seq_rate if seq_rate > 0 else None},
-        "roofline": roof,
-        "cpu_baseline": cpu,
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
+
+
+# ----------------------------------------------------------------------- main
+def line(fields, roof, cpu, args, extra_config=None):
+    cfg = dict(fields["config"])
+    if extra_config:
+        cfg.update(extra_config)
+    return {"metric": METRIC, "value": fields["value"], "unit": "LPs/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": fields["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": fields["dtype"], "data": fields["data"],
+            "config": cfg, "roofline": roof, "cpu_baseline": cpu}
+
+
+def sub_as_line(rec, args):
+    """A sub-record as the JSON line of a single-workload run."""
+    return {"metric": METRIC, "value": rec["value"], "unit": "LPs/s", "n_gpus": 1, "steps": rec.get("steps", 1),
+            "warmup": 1, "ms_per_step": rec.get("ms_per_step"), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32 iterates and SpMV, fp64 KKT", "data": "synthetic",
+            "config": {k: v for k, v in rec.items() if k not in ("roofline", "cpu_baseline")},
+            "roofline": rec.get("roofline"), "cpu_baseline": rec.get("cpu_baseline")}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="transport", choices=["transport", "batch", "learned", "nodes", "bnb"])
+    ap.add_argument("--steps", type=int, default=10, help="timed steps of the headline workload")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "learned", "transport", "batch", "nodes", "beyond_mall", "bnb"])
+    ap.add_argument("--no-subrecords", action="store_true", help="headline only (profiling runs)")
+    ap.add_argument("--skip", default="", help="comma list of sub-records to leave out: c1,c2,c4,beyond_mall")
+    ap.add_argument("--train", type=int, default=50_000)
+    ap.add_argument("--eval", type=int, default=1024)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--plain-sample", type=int, default=64)
+    ap.add_argument("--obj-sample", type=int, default=64)
+    ap.add_argument("--cpu-cap-s", type=float, default=12.0)
+    ap.add_argument("--c1-steps", type=int, default=3)
+    ap.add_argument("--c2-steps", type=int, default=2)
+    ap.add_argument("--c2-plain", type=int, default=32)
+    ap.add_argument("--c4-steps", type=int, default=3)
+    ap.add_argument("--c4-plain", type=int, default=16)
+    ap.add_argument("--c4-obj-sample", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--nodes", type=int, default=256)
     ap.add_argument("--bnb-batch", type=int, default=256, help="bnb: node relaxations per round")
     ap.add_argument("--bnb-seq-s", type=float, default=15.0, help="bnb: seconds of the one-node-at-a-time sample")
-    ap.add_argument("--nodes", type=int, default=256)
-    ap.add_argument("--train", type=int, default=4096)
-    ap.add_argument("--eval", type=int, default=1024)
-    ap.add_argument("--epochs", type=int, default=30)
-    ap.add_argument("--no-step", action="store_true", help="learned: do not use the predicted step")
-    ap.add_argument("--batch", type=int, default=4096)
-    ap.add_argument("--block", type=int, default=0, help="batch workgroup size (0 = by LDS need)")
-    ap.add_argument("--plain-sample", type=int, default=32)
-    ap.add_argument("--cpu-cap-s", type=float, default=20.0)
-    ap.add_argument("--sources", type=int, default=1000)
-    ap.add_argument("--sinks", type=int, default=2000)
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--max-iters", type=int, default=100_000, help="per LP (the reference's MAX_PDHG_ITERS)")
     ap.add_argument("--cpu-sample-iters", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-ref-iters", action="store_true",
-                    help="skip the reference-form GPU run that counts the reference algorithm's iterations")
     ap.add_argument("--no-autotune", action="store_true", help="use the heuristic launch shape")
-    ap.add_argument("--no-highs", action="store_true", help="skip the HiGHS objective reference")
+    ap.add_argument("--no-highs", action="store_true", help="skip the HiGHS objective reference of C1")
     ap.add_argument("--calib-copy", action="store_true",
                     help="end with 1 GiB device copies (calibration of the HBM byte counters)")
-    ap.add_argument("--dump", default="", help="batch workload: write per-LP sizes/iterations (.npz)")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); every box here has one GPU, "
               "and multi-GPU sharding of this path is design-only.", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "batch":
-        return bench_batch(args)
-    if args.workload == "learned":
-        return bench_learned(args)
-    if args.workload == "nodes":
-        return bench_nodes(args)
     if args.workload == "bnb":
         return bench_bnb(args)
-
-    import numpy as np
-    import torch
-    from sovereign_solver_amd.engine.forms import box_form, reference_form
-    from sovereign_solver_amd.engine.single import SingleEngine
-    from sovereign_solver_amd.engine import tune as T
-    from sovereign_solver_amd.workloads import transportation_lp
-
-    from sovereign_solver_amd.engine.device import csr_transpose
+    single = {"transport": run_transport, "batch": run_batch, "nodes": run_nodes, "beyond_mall": run_beyond_mall}
+    if args.workload in single:
+        rec = single[args.workload](args)
+        calib_copy(args)
+        print(json.dumps(sub_as_line(rec, args) if "value" in rec else rec), flush=True)
+        return
     t0 = time.perf_counter()
-    p = transportation_lp(args.sources, args.sinks, seed=0)
-    t1 = time.perf_counter()
-    form = box_form(p)
-    t2 = time.perf_counter()
-    log(f"built transport LP {args.sources}x{args.sinks}: m={form.m} n={form.n} nnz={form.nnz} "
-        f"(generator {t1 - t0:.2f}s, box form {t2 - t1:.2f}s)")
-    eng = SingleEngine(form, "fp32")
-    torch.cuda.synchronize()
-    t3 = time.perf_counter()
-    # ingestion: K^T on the device (second call timed with events) vs on the host
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    for _ in range(2):
-        ev[0].record(eng.stream)
-        csr_transpose(eng.row_ptr, eng.col_idx, eng.val, eng.n, eng.stream)
-        ev[1].record(eng.stream)
-        eng.stream.synchronize()
-    dev_transpose_ms = ev[0].elapsed_time(ev[1])
-    th = time.perf_counter()
-    _ = form.KT
-    host_transpose_ms = (time.perf_counter() - th) * 1e3
-    ingest = {"box_form_s": t2 - t1, "engine_setup_s": t3 - t2, "device_transpose_ms": dev_transpose_ms,
-              "host_transpose_ms": host_transpose_ms}
-    log(f"ingest: box form {t2 - t1:.3f}s, engine setup {t3 - t2:.3f}s; K^T on the device "
-        f"{dev_transpose_ms:.2f} ms vs SciPy on the host {host_transpose_ms:.1f} ms")
-    if not args.no_autotune:
-        eng.set_tune(T.autotune(eng))
-    log(f"tune {eng.tune}")
-
-    for w in range(args.warmup):
-        r = eng.solve(args.tol, args.max_iters)
-        log(f"warmup {w}: {r['status']} it={r['iterations']} rel_kkt={r['rel_kkt']:.3e}")
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    iters = []
-    results = []
-    for s in range(args.steps):
-        r = eng.solve(args.tol, args.max_iters)
-        iters.append(r["iterations"])
-        results.append(r)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    ok = sum(1 for r in results if r["status"] == "optimal")
-    for s, r in enumerate(results):
-        log(f"step {s}: {r['status']} it={r['iterations']} restarts={r['restarts']} rel_kkt={r['rel_kkt']:.3e}")
-    value = ok / elapsed
-    ms_per_step = elapsed * 1e3 / args.steps
-
-    # objective error vs HiGHS dual simplex (FP64) on the same LP
-    x, y = results[-1]["x"], results[-1]["y"]
-    obj = float(p.c @ x)
-    obj_ref = None
-    if args.no_highs:
-        pass
-    else:
-      try:
-        from scipy.optimize import linprog
-        t1 = time.perf_counter()
-        lr = linprog(p.c, A_ub=p.A_ub, b_ub=p.b_ub, A_eq=p.A_eq, b_eq=p.b_eq, bounds=(0, None), method="highs-ds")
-        if lr.status == 0:
-            obj_ref = float(lr.fun)
-        log(f"HiGHS-ds objective {obj_ref} ({time.perf_counter() - t1:.1f}s)")
-      except Exception as e:                       # pragma: no cover
-        log(f"HiGHS reference unavailable: {e}")
-
-    # roofline of the dominant kernel: event-timed back-to-back launches
-    elem = 4
-    us_p = eng.time_kernel("primal", eng.tune, reps=50)
-    us_d = eng.time_kernel("dual", eng.tune, reps=50)
-    bp, bd = primal_bytes(form, elem), dual_bytes(form, elem)
-    gbs_p = bp / (us_p * 1e-6) / 1e9
-    gbs_d = bd / (us_d * 1e-6) / 1e9
-    log(f"primal step {us_p:.2f} us ({bp / 1e6:.1f} MB -> {gbs_p:.0f} GB/s); "
-        f"dual step {us_d:.2f} us ({bd / 1e6:.1f} MB -> {gbs_d:.0f} GB/s)")
-    spmv = spmv_rates(eng)
-    for k, v in spmv.items():
-        log(f"standalone SpMV {k}: {v['us']:.2f} us ({v['bytes'] / 1e6:.1f} MB -> {v['achieved_gbs']:.0f} GB/s, "
-            f"{100 * v['frac']:.0f}% of peak)")
-
-    cpu = None
-    ref_iters = None
-    if not args.no_ref_iters:
-        rf = reference_form(p)
-        reng = SingleEngine(rf, "fp64")
-        rr = reng.solve(args.tol, args.max_iters)
-        ref_iters = rr["iterations"] if rr["status"] == "optimal" else None
-        log(f"reference form FP64 on GPU: {rr['status']} it={rr['iterations']} rel_kkt={rr['rel_kkt']:.3e}")
-        del reng
-    if not args.no_cpu_baseline:
-        n_it = ref_iters if ref_iters else int(np.mean(iters))
-        cpu = cpu_baseline(p, n_it, args.cpu_sample_iters)
-        log(f"cpu baseline {cpu['value']:.4g} LPs/s ({cpu['sample']})")
-
-    out = {
-        "metric": "LPs/sec to 1e-4 rel KKT: learned+batched vs plain PDHG, 1 GPU; SpMV HBM GB/s",
-        "value": value, "unit": "LPs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32", "data": "synthetic (seeded transportation LP, 95% of supply demanded)",
-        "config": {"workload": f"single transportation LP {args.sources}x{args.sinks}, plain restarted PDHG "
-                               f"(cold start), tol {args.tol:g} rel KKT",
-                   "m": form.m, "n": form.n, "nnz": form.nnz, "iterations": iters,
-                   "restarts": [r["restarts"] for r in results], "rel_kkt": results[-1]["rel_kkt"],
-                   "objective": obj, "objective_highs_ds": obj_ref,
-                   "objective_rel_err": None if obj_ref is None else abs(obj - obj_ref) / (1 + abs(obj_ref)),
-                   "tune": eng.tune.__dict__, "iterate_dtype": "fp32", "kkt_dtype": "fp64",
-                   "reference_form_iterations": ref_iters, "ingest": ingest},
-        "roofline": {"bound": "hbm", "kernel": "primal_kernel (fused K^T y + step + projection + sums)",
-                     "achieved": gbs_p, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs_p / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(f"transport_{args.sources}x{args.sinks}/primal_kernel"),
-                     "bytes_per_launch": bp, "us_per_launch": us_p,
-                     "dual_kernel": {"achieved": gbs_d, "bytes_per_launch": bd, "us_per_launch": us_d},
-                     "spmv_standalone": spmv},
-        "cpu_baseline": cpu,
-    }
+    fields, roof, cpu = run_headline(args)
+    subs = {}
+    skip = set(s.strip() for s in args.skip.split(",") if s.strip())
+    if not args.no_subrecords and args.workload == "headline":
+        for name, fn in (("c1", run_transport), ("c2", run_batch), ("c4", run_nodes),
+                         ("beyond_mall", run_beyond_mall)):
+            if name in skip:
+                continue
+            t = time.perf_counter()
+            try:
+                subs[name] = fn(args)
+            except Exception as e:                       # a sub-record must not lose the headline
+                log(f"sub-record {name} failed: {type(e).__name__}: {e}")
+                subs[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"sub-record {name} done in {time.perf_counter() - t:.1f}s")
+    log(f"bench done in {time.perf_counter() - t0:.1f}s")
     calib_copy(args)
-    print(json.dumps(out), flush=True)
+    print(json.dumps(line(fields, roof, cpu, args, {"subrecords": subs} if subs else None)), flush=True)
 
 
 if __name__ == "__main__":

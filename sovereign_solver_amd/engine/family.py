@@ -649,14 +649,19 @@ class FamilyEngine:
         self.compactions += 1
         self.drop_graphs()
 
-    def results(self):
-        """(out records, X [B, n] FP64 solutions, Y [B, m]) by LP, in the problems'
-        own row and column order.  X and Y are views of pinned host buffers that
-        the next solve on this engine overwrites: copy what must outlive it."""
+    def collect(self) -> None:
+        """Gather every LP's result into the LP-major device outputs (``X_out``, ``Y_out``,
+        ``out_final``; problems' own order), leaving them in HBM."""
         self.stream.synchronize()
         cnt = self.args.count
         live = np.arange(cnt)[~self.saved[self.slot_lp[:cnt]]]
         self._save(live)
+
+    def results(self):
+        """(out records, X [B, n] FP64 solutions, Y [B, m]) by LP, in the problems'
+        own row and column order.  X and Y are views of pinned host buffers that
+        the next solve on this engine overwrites: copy what must outlive it."""
+        self.collect()
         if self._host is None:                  # pinned staging, allocated once per engine
             self._host = {k: torch.empty(self.t[k].shape, dtype=self.t[k].dtype).pin_memory()
                           for k in ("X_out", "Y_out", "out_final")}
@@ -666,6 +671,17 @@ class FamilyEngine:
         self.stream.synchronize()
         out = np.frombuffer(self._host["out_final"].numpy().tobytes(), dtype=OUT_DTYPE).copy()
         return out, self._host["X_out"].numpy(), self._host["Y_out"].numpy()
+
+    def objectives(self, X: np.ndarray) -> np.ndarray:
+        """Objective of each LP at X [B, n] (problems' own column order) in each problem's
+        own sense, from the engine's copy of c (the staging of per-LP costs, or the
+        shared vector)."""
+        if "c" in self._src:
+            obj = np.einsum("ij,ij->i", self._src["c"][0].numpy(), X)
+        else:
+            obj = X @ np.asarray(self.fb.problems[0].c, float)
+        sign = np.array([-1.0 if p.sense == "max" else 1.0 for p in self.fb.problems])
+        return obj * sign
 
     def solve(self, tol: float, max_iters: int, x0=None, y0=None, weight=None):
         self.start(tol, x0, y0, weight)

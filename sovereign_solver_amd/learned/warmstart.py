@@ -63,24 +63,34 @@ class Labels:
     seconds: float
 
 
+def _engine_for(engines: dict, probs, tol: float, scheme: str) -> FamilyEngine:
+    """A family engine for ``probs``: one built earlier for as many LPs is loaded with
+    them (the matrix and the device buffers stay), else a new one."""
+    eng = engines.get(len(probs))
+    if eng is not None:
+        eng.load(probs)
+        return eng
+    eng = engines[len(probs)] = FamilyEngine(FamilyBatch(probs), scheme=scheme)
+    if len(probs) >= 256:
+        eng.autotune(tol)
+    return eng
+
+
 def solve_family(fam, ids, tol: float, max_iters: int, warm=None, chunk: int = 2048, scheme: str = "adaptive"):
     """Batched solves of family instances (shared-matrix engine); returns
     SolveResults and the engine seconds (launch to results on the host)."""
     res = []
     secs = 0.0
+    engines = {}
     for s in range(0, len(ids), chunk):
         part = ids[s: s + chunk]
-        fb = FamilyBatch([fam.problem(int(k)) for k in part])
-        eng = FamilyEngine(fb, scheme=scheme)
-        if fb.count >= 256:
-            eng.autotune(tol)
-        X, Y, w = family_warm_start(fb, None if warm is None else warm[s: s + chunk])
+        eng = _engine_for(engines, [fam.problem(int(k)) for k in part], tol, scheme)
+        X, Y, w = family_warm_start(eng.fb, None if warm is None else warm[s: s + chunk])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         out, xs, ys = eng.solve(tol, max_iters, X, Y, w)
         secs += time.perf_counter() - t0
         res += results_of(eng, out, xs, ys)
-        del eng
     return res, secs
 
 
@@ -214,6 +224,13 @@ class LearnedStart:
         mlp = self.mlp
         with torch.cuda.stream(mlp.stream):
             f = torch.from_numpy(np.ascontiguousarray(feats, np.float32)).to(mlp.device, non_blocking=True)
+        return self.forward_device(f)
+
+    def forward_device(self, f: torch.Tensor):
+        """As :meth:`forward` from features already in HBM ([B, d_in] FP32, allocated on
+        or synchronised with the network's stream)."""
+        mlp = self.mlp
+        with torch.cuda.stream(mlp.stream):
             xb = ((f - self.in_mean) * self.in_inv_std).to(torch.bfloat16).contiguous()
         out = mlp.forward(xb)
         ev = torch.cuda.Event()
@@ -228,12 +245,10 @@ def solve_learned(fam, ids, ls: LearnedStart, tol: float, max_iters: int, use_x:
     res = []
     secs = 0.0
     n, m = fam.n, fam.m
+    engines = {}
     for s in range(0, len(ids), chunk):
         part = ids[s: s + chunk]
-        fb = FamilyBatch([fam.problem(int(k)) for k in part])
-        eng = FamilyEngine(fb, scheme=scheme)
-        if fb.count >= 256:
-            eng.autotune(tol)
+        eng = _engine_for(engines, [fam.problem(int(k)) for k in part], tol, scheme)
         feats = fam.features_of(part)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -244,5 +259,4 @@ def solve_learned(fam, ids, ls: LearnedStart, tol: float, max_iters: int, use_x:
         o, xs, ys = eng.results()
         secs += time.perf_counter() - t0
         res += results_of(eng, o, xs, ys)
-        del eng
     return res, secs
