@@ -234,7 +234,7 @@ def dual_bytes(form, elem: int) -> float:
     return float(b)
 
 
-def spmv_rates(eng, reps: int = 50, kx_tpr: int = 64):
+def spmv_rates(eng, reps: int = 50, kx_tpr: int = 64, traffic_key=None):
     """Standalone FP32 SpMV (``pdlp_spmv_csr``) on the engine's K and K^T: a HIP
     graph of ``reps`` back-to-back launches timed with events on its stream.
     Algorithmic bytes per product: 8 nnz + 4 (rows + 1) + 4 cols + 4 rows (FP32
@@ -273,7 +273,8 @@ def spmv_rates(eng, reps: int = 50, kx_tpr: int = 64):
         nbytes = 8 * nnz + 4 * (rows + 1) + 4 * cols + 4 * rows
         gbs = nbytes / (us * 1e-6) / 1e9
         out[name] = {"rows": rows, "nnz": nnz, "lanes_per_row": tpr, "bytes": nbytes, "us": us,
-                     "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS}
+                     "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(f"{traffic_key}/spmv_{name}") if traffic_key else None}
         del g
     return out
 
@@ -863,8 +864,8 @@ def run_beyond_mall(args, sources=4000, sinks=8000):
     us_d = eng.time_kernel("dual", eng.tune, reps=10)
     bp, bd = primal_bytes(form, 4), dual_bytes(form, 4)
     gp, gd = bp / (us_p * 1e-6) / 1e9, bd / (us_d * 1e-6) / 1e9
-    spmv = spmv_rates(eng, reps=10, kx_tpr=64)
     key = f"transport_{sources}x{sinks}"
+    spmv = spmv_rates(eng, reps=10, kx_tpr=64, traffic_key=key)
     log(f"beyond MALL: primal step {us_p:.1f} us ({gp:.0f} GB/s), dual step {us_d:.1f} us ({gd:.0f} GB/s), "
         f"SpMV K^T y {spmv['KT_y']['achieved_gbs']:.0f} GB/s, K x {spmv['K_x']['achieved_gbs']:.0f} GB/s")
     rec = {
