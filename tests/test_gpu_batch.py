@@ -13,6 +13,7 @@ from oracle import yardstick
 from sovereign_solver_amd import solve_lp_pdhg, solve_lp_pdhg_batch
 from sovereign_solver_amd.problem import Problem
 from sovereign_solver_amd.workloads import random_sparse_lp, transportation_lp, uneven_batch, uneven_sizes
+from tests.parity import assert_objective_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -33,8 +34,7 @@ def test_batch_uneven_small(gpu, block):
         assert r.status == "optimal", (p.name, r.status, r.iterations)
         err = yardstick.rel_kkt(p, r.x, r.y)[0]
         assert err <= 1.02e-4, (p.name, err, r.rel_kkt)
-        ref = _highs(p)
-        assert abs(r.objective - ref) <= 1e-3 * (1 + abs(ref)), (p.name, r.objective, ref)
+        assert_objective_parity(p, r.objective, ref=_highs(p), x=r.x, y=r.y)
 
 
 def test_batch_mixed_row_kinds_and_bounds(gpu):
@@ -45,8 +45,7 @@ def test_batch_mixed_row_kinds_and_bounds(gpu):
         assert r.status == "optimal"
         assert yardstick.rel_kkt(p, r.x, r.y)[0] <= 1.02e-4
         assert np.all(r.x >= p.lb - 1e-9) and np.all(r.x <= p.ub + 1e-9)
-        ref = _highs(p)
-        assert abs(r.objective - ref) <= 2e-3 * (1 + abs(ref))
+        assert_objective_parity(p, r.objective, ref=_highs(p), x=r.x, y=r.y)
 
 
 def test_batch_agrees_with_single_engine(gpu):
@@ -55,7 +54,9 @@ def test_batch_agrees_with_single_engine(gpu):
     for p, b in zip(probs, rb):
         s = solve_lp_pdhg(p, tol=1e-4, precision="fp32")
         assert s.status == b.status == "optimal"
-        assert abs(s.objective - b.objective) <= 1e-3 * (1 + abs(s.objective))
+        ref = _highs(p)
+        assert_objective_parity(p, s.objective, ref=ref, x=s.x, y=s.y)
+        assert_objective_parity(p, b.objective, ref=ref, x=b.x, y=b.y)
         # same algorithm: iteration counts agree to a few checks (summation order differs)
         assert abs(s.iterations - b.iterations) <= max(200, 0.2 * s.iterations)
 
@@ -132,8 +133,7 @@ def test_batch_small_lps_with_empty_columns_converge(gpu):
     for p, r in zip(probs, res):
         assert r.status == "optimal" and r.iterations <= 20_000, (r.status, r.iterations)
         assert yardstick.rel_kkt(p, r.x, r.y)[0] <= 1.02e-4
-        ref = _highs(p)
-        assert abs(r.objective - ref) <= 1e-3 * (1 + abs(ref))
+        assert_objective_parity(p, r.objective, ref=_highs(p), x=r.x, y=r.y)
 
 
 def test_batch_fp32_part_layouts_agree_bitwise(gpu):
@@ -158,7 +158,8 @@ def test_scaled_batch(gpu, method):
     finish in a fraction of that.  Those three are ill-conditioned (86 of the
     170 columns of m = 68 empty, finite bounds): a relative KKT of 7e-5 (primal
     residual 0.006 against a normaliser of 90) leaves the objective 4% from
-    HiGHS's, so only the other LPs are compared with HiGHS."""
+    HiGHS's — where the reference's own PDHG at the same tolerance lands as far
+    (4.0% on LP 3579): each LP is held to the objective bar of tests/parity.py."""
     slow = [(94, 3721), (68, 3579), (68, 1498)]               # (m, index in the 4,096-LP uneven batch)
     probs = [random_sparse_lp(m, round(2.5 * m), 0.01, seed=s, family=2) for m, s in slow]
     probs += [random_sparse_lp(m, round(2.5 * m), 0.01, seed=100 + k, family=2)
@@ -170,7 +171,5 @@ def test_scaled_batch(gpu, method):
         err = yardstick.rel_kkt(p, r.x, r.y)[0]
         assert err <= 1.05e-4, (p.name, err, r.rel_kkt)
         assert abs(err - r.rel_kkt) <= 1e-3 * err + 1e-9     # the device measured the original LP too
-        if k >= len(slow):
-            ref = _highs(p)
-            assert abs(r.objective - ref) <= 1e-3 * (1 + abs(ref)), (p.name, r.objective, ref)
+        assert_objective_parity(p, r.objective, ref=_highs(p), x=r.x, y=r.y)
     assert sum(r.iterations for r in scaled[:3]) < 0.3 * sum(r.iterations for r in plain[:3])

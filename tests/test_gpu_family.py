@@ -15,6 +15,7 @@ from oracle.farkas import farkas_value as _farkas_value
 from sovereign_solver_amd import solve_lp_pdhg_batch
 from sovereign_solver_amd.engine.family import FamilyBatch, FamilyEngine
 from sovereign_solver_amd.workloads import DispatchFamily, SetCoverNodes
+from tests.parity import assert_objective_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -27,15 +28,14 @@ def _highs(p):
     return r.fun
 
 
-def _check(probs, res, tol=1e-4, n_highs=6, obj_tol=1e-3):
+def _check(probs, res, tol=1e-4, n_highs=6):
     for p, r in zip(probs, res):
         assert r.status == "optimal", (p.name, r.status, r.iterations)
         err = yardstick.rel_kkt(p, r.x, r.y)[0]
         assert err <= 1.02 * tol, (p.name, err, r.rel_kkt)
         assert np.all(r.x >= p.lb - 1e-9) and np.all(r.x <= p.ub + 1e-9)
     for p, r in list(zip(probs, res))[:n_highs]:
-        ref = _highs(p)
-        assert abs(r.objective - ref) <= obj_tol * (1 + abs(ref)), (p.name, r.objective, ref)
+        assert_objective_parity(p, r.objective, tol, ref=_highs(p), x=r.x, y=r.y)
 
 
 @pytest.mark.parametrize("scheme", ["adaptive", "reference"])
@@ -58,9 +58,11 @@ def test_family_agrees_with_persistent_engine(gpu):
     probs = [fam.problem(k) for k in range(20)]
     rf = solve_lp_pdhg_batch(probs, tol=1e-4, engine="family")
     rp = solve_lp_pdhg_batch(probs, tol=1e-4, engine="persistent")
-    for f, p in zip(rf, rp):
+    for k, (f, p) in enumerate(zip(rf, rp)):
         assert f.status == p.status == "optimal"
-        assert abs(f.objective - p.objective) <= 1e-3 * (1 + abs(p.objective))
+        ref = _highs(probs[k])
+        assert_objective_parity(probs[k], f.objective, ref=ref, x=f.x, y=f.y)
+        assert_objective_parity(probs[k], p.objective, ref=ref, x=p.x, y=p.y)
         assert abs(f.iterations - p.iterations) <= max(200, 0.25 * p.iterations), (f.iterations, p.iterations)
 
 
@@ -254,7 +256,7 @@ def test_family_mixed_rows_and_infinite_upper_bounds(gpu):
                              b_eq=base.b_eq, lb=base.lb, ub=base.ub, integer_mask=base.integer_mask,
                              name=f"transport_v{k}"))
     res = solve_lp_pdhg_batch(probs, tol=1e-4, max_iters=200_000, engine="family")
-    _check(probs, res, n_highs=4, obj_tol=2e-3)
+    _check(probs, res, n_highs=4)
     fb = FamilyBatch(probs)
     assert fb.is_shared("ub_span") and not fb.is_shared("b")
 

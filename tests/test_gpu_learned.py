@@ -13,6 +13,8 @@ from oracle import yardstick
 from sovereign_solver_amd.learned import warmstart as W
 from sovereign_solver_amd.workloads import DispatchFamily, SetCoverNodes
 
+from tests.parity import assert_objective_parity, highs
+
 pytestmark = pytest.mark.gpu
 
 
@@ -46,8 +48,11 @@ def test_trained_network_gains_over_cold_start(gpu):
     print(f"held-out iterations: cold mean {ci.mean():.0f} max {ci.max()}, learned mean {wi.mean():.0f} max {wi.max()}")
     assert wi.mean() < ci.mean()
     assert wi.max() <= ci.max()          # the slowest LP sets the time of a lockstep batch
-    for c, w in zip(cold, warm):
-        assert abs(c.objective - w.objective) <= 2e-3 * (1 + abs(c.objective))
+    for k in range(6):
+        p = fam.problem(int(eval_ids[k]))
+        ref = highs(p)
+        assert_objective_parity(p, cold[k].objective, ref=ref, x=cold[k].x, y=cold[k].y)
+        assert_objective_parity(p, warm[k].objective, ref=ref, x=warm[k].x, y=warm[k].y)
 
 
 def test_parent_warm_start_gains_on_nodes(gpu):

@@ -184,3 +184,34 @@ def test_ray_test_bound_holds_on_bounded_lps(seed):
         d = rng.standard_normal(60) * (rng.random(60) < 0.3)
         cd, vr, vb = ray_violations(p, d)
         assert cd >= -np.linalg.norm(y) * vr - np.linalg.norm(red) * vb - 1e-7 * (1 + np.abs(p.c) @ np.abs(d))
+
+
+def test_reference_pdhg_at_1e4_does_not_pin_the_objective():
+    """Why the objective bar falls back to parity with the reference PDHG: on LP 3579 of
+    the uneven batch (m = 68, 86 of 170 columns empty) the reference's own PDHG stops at
+    relative KKT 1e-4 with its objective 4% from the FP64 optimum."""
+    from tests.parity import highs, reference_pdhg_error
+    from sovereign_solver_amd.workloads import random_sparse_lp, uneven_sizes
+    m = int(uneven_sizes(4096, 50, 2000, 0)[3579])
+    p = random_sparse_lp(m, int(round(2.5 * m)), 0.01, seed=3579, family=2)
+    assert m == 68
+    assert 0.03 < reference_pdhg_error(p, highs(p)) < 0.05
+
+
+def test_certified_objective_interval_holds_for_any_box_point():
+    """tests/parity.py's interval: for every x in the box and every y, c'x - c'x* lies in
+    [-|y*|'|Az - b|, (c'z - b'y) + (A'y - c)_+' z*] on the lift; at HiGHS's own pair it is
+    a point (0 within rounding)."""
+    from tests.parity import certified_interval, _linprog
+    from sovereign_solver_amd.workloads import random_sparse_lp
+    rng = np.random.default_rng(5)
+    for seed in range(4):
+        p = random_sparse_lp(30, 75, 0.08, seed=seed, family=2)
+        for _ in range(20):
+            x = rng.uniform(0.0, 1.0, p.n_vars) * p.ub
+            y = rng.standard_normal(p.n_constraints)
+            lo, hi, d = certified_interval(p, x, y)
+            assert lo - 1e-9 <= d <= hi + 1e-9, (seed, lo, d, hi)
+        h = _linprog(p)
+        lo, hi, d = certified_interval(p, h.x, h.eqlin.marginals)
+        assert abs(lo) <= 1e-6 and abs(hi) <= 1e-6 * (1 + abs(h.fun)) and abs(d) <= 1e-9
