@@ -305,27 +305,31 @@ def family_passes(eng, tol, traffic_key=None, reps: int = 20):
                                                        f"_pass") if traffic_key else None}}
 
 
-def mlp_forward_rate(mlp, batch: int, reps: int = 20):
-    """MFMA rate of the network's forward pass on ``batch`` instances: the three
-    GEMMs (2 B (d_in d_h + d_h d_h + d_h d_out) FLOP) timed with HIP events on the
-    network's stream over ``reps`` back-to-back passes."""
-    import torch
-    with torch.cuda.stream(mlp.stream):
-        x = torch.randn(batch, mlp.d_in, device=mlp.device).to(torch.bfloat16).contiguous()
-    for _ in range(3):
-        mlp.forward(x)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    with torch.cuda.stream(mlp.stream):
-        e0.record(mlp.stream)
-        for _ in range(reps):
-            mlp.forward(x)
-        e1.record(mlp.stream)
-    mlp.stream.synchronize()
+def node_forward_rate(model, V_dev, reps: int = 10):
+    """MFMA rate of the node network on the instances V_dev: every stage's gather, three
+    GEMMs (2 rows (d_in d_h + d_h d_h + d_h d_out) FLOP each) and the assembly, timed
+    with HIP events on the network's stream over ``reps`` back-to-back passes."""
+    from sovereign_solver_amd.learned import nodes as N
+    st = model.mlp.stream
+    for _ in range(2):
+        model.starts(V_dev)
+    e0, e1 = torch_events()
+    e0.record(st)
+    for _ in range(reps):
+        model.starts(V_dev)
+    e1.record(st)
+    st.synchronize()
     us = e0.elapsed_time(e1) * 1e3 / reps
-    flop = 2.0 * batch * (mlp.d_in * mlp.d_h + mlp.d_h * mlp.d_h + mlp.d_h * mlp.d_out)
+    rows = V_dev.shape[0] * model.spec.nodes
+    flop = N.forward_flop(model, rows)
     tflops = flop / (us * 1e-6) / 1e12
-    return {"batch": batch, "gflop": flop / 1e9, "us": us, "tflops": tflops, "peak_tflops": BF16_PEAK_TFLOPS,
-            "frac": tflops / BF16_PEAK_TFLOPS, "bound": "mfma (dense BF16)"}
+    return {"instances": V_dev.shape[0], "rows": rows, "gflop": flop / 1e9, "us": us, "tflops": tflops,
+            "peak_tflops": BF16_PEAK_TFLOPS, "frac": tflops / BF16_PEAK_TFLOPS, "bound": "mfma (dense BF16)"}
+
+
+def torch_events():
+    import torch
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
 
 def plain_rates(probs, tol, max_iters):
@@ -380,6 +384,7 @@ def run_headline(args):
     dispatch family.  Returns (JSON line fields, roofline, cpu baseline)."""
     import torch
     from sovereign_solver_amd.engine.family import FamilyBatch, FamilyEngine
+    from sovereign_solver_amd.learned import nodes as N
     from sovereign_solver_amd.learned import warmstart as W
     from sovereign_solver_amd.workloads import DispatchFamily
 
@@ -396,14 +401,18 @@ def run_headline(args):
     log(f"labelled {len(train_ids)} in {t_label:.1f}s (engine {lab.seconds:.1f}s), optimal {int(keep.sum())}, "
         f"iterations mean {lab.iterations.mean():.0f}")
     t1 = time.perf_counter()
-    mlp, nin, nout, losses = W.train(fam.features_of(train_ids)[keep], W.targets_of(lab)[keep], d_h=512,
-                                     epochs=args.epochs, batch=256, lr=1e-3, out_blocks=W.output_blocks(fam))
+    # the weight is calibrated on 256 training instances (one scalar for the family)
+    model = N.train_node_model(fam, lab, stages=args.stages, steps=args.node_steps, rows=args.node_rows,
+                               calib_ids=train_ids[-256:], tol=tol, max_iters=mi, log=log)
     t_train = time.perf_counter() - t1
-    log(f"trained {args.epochs} epochs in {t_train:.1f}s, loss {losses[0]:.4f} -> {losses[-1]:.4f}")
-    ls = W.LearnedStart(mlp, nin, nout)
-    mlp_rate = mlp_forward_rate(mlp, len(eval_ids))
-    log(f"network forward on {len(eval_ids)}: {mlp_rate['gflop']:.1f} GFLOP in {mlp_rate['us']:.1f} us "
-        f"({mlp_rate['tflops']:.0f} TFLOP/s, {100 * mlp_rate['frac']:.1f}% of dense BF16 peak)")
+    stages = N.stage_list(model)
+    train_flop = sum(s.train_flop for s in stages)
+    train_gpu_s = sum(s.train_gpu_seconds for s in stages)
+    train_rate = {"tflops": train_flop / train_gpu_s / 1e12, "frac": train_flop / train_gpu_s / 1e12 / BF16_PEAK_TFLOPS,
+                  "gpu_s": train_gpu_s, "rows_per_step": args.node_rows, "steps_per_stage": args.node_steps}
+    log(f"trained {len(stages)} node-model stages in {t_train:.1f}s (label excluded; device step loop "
+        f"{train_gpu_s:.1f}s at {train_rate['tflops']:.0f} TFLOP/s, {100 * train_rate['frac']:.1f}% of dense BF16), "
+        f"primal weight {'default' if model.log_weight is None else f'{np.exp(model.log_weight):.2f}'}")
 
     probs = [fam.problem(int(k)) for k in eval_ids]
     t2 = time.perf_counter()
@@ -411,9 +420,14 @@ def run_headline(args):
     shape = eng.autotune(tol)
     t_setup = time.perf_counter() - t2
     log(f"engine for {len(probs)} LPs built in {t_setup:.2f}s (once per family and batch size), shape {shape}")
-    with torch.cuda.stream(mlp.stream):
-        feats_dev = torch.from_numpy(fam.features_from(probs).astype(np.float32)).to(mlp.device)
-    mlp.stream.synchronize()
+    with torch.cuda.stream(model.mlp.stream):
+        V_dev = torch.from_numpy(fam.node_inputs_from(probs)).to(model.device)
+    model.mlp.stream.synchronize()
+    net_rate = node_forward_rate(model, V_dev)
+    log(f"node network (both stages + gathers + assembly) on {len(probs)} LPs ({net_rate['rows']} rows per stage): "
+        f"{net_rate['gflop']:.0f} GFLOP in {net_rate['us']:.0f} us ({net_rate['tflops']:.0f} TFLOP/s, "
+        f"{100 * net_rate['frac']:.1f}% of dense BF16 peak)")
+    w_col = n + m if model.log_weight is not None else -1
     out_host = torch.empty(eng.t["out_final"].shape, dtype=torch.uint8).pin_memory()
     from sovereign_solver_amd.engine.batch import OUT_DTYPE
 
@@ -421,9 +435,8 @@ def run_headline(args):
         return np.frombuffer(out_host.numpy().tobytes(), dtype=OUT_DTYPE).copy()
 
     def learned_resident():
-        out, ev = ls.forward_device(feats_dev)
-        eng.start_from_device(tol, out, x_off=0, y_off=-1, w_col=n + m, scale=ls.out_scale, shift=ls.out_shift,
-                              ready=ev)
+        out, ev = model.starts(V_dev)
+        eng.start_from_device(tol, out, x_off=0, y_off=n, w_col=w_col, ready=ev)
         eng.run(mi)
         eng.collect()
         with torch.cuda.stream(eng.stream):
@@ -468,11 +481,9 @@ def run_headline(args):
 
     # host to host: problems on the host -> features, packing, inference, solve -> results and objectives
     def learned_h2h():
-        feats = fam.features_from(probs)
+        V = fam.node_inputs_from(probs)
         eng.load(probs)
-        out, ev = ls.forward(feats)
-        eng.start_from_device(tol, out, x_off=0, y_off=-1, w_col=n + m, scale=ls.out_scale, shift=ls.out_shift,
-                              ready=ev)
+        model.start_engine(eng, tol, V)
         eng.run(mi)
         o, X, Y = eng.results()
         return o, eng.objectives(X)
@@ -515,9 +526,12 @@ def run_headline(args):
             "workload": (f"C3 learned warm start + shared-matrix batched restarted PDHG, economic dispatch 200x24 "
                          f"(m={m}, n={n}), {len(probs)} held-out LPs per step, tol {tol:g}; step = network "
                          f"inference + device-side start + batched solve, inputs and results in HBM"),
-            "lps_per_step": len(probs), "train_instances": args.train, "epochs": args.epochs, "width": 512,
-            "train_label_s": t_label, "train_s": t_train, "loss_first_last": [losses[0], losses[-1]],
-            "engine_setup_s": t_setup, "launch_shape": shape, "mlp_forward_mfma": mlp_rate,
+            "lps_per_step": len(probs), "train_instances": args.train,
+            "network": (f"node model: {len(stages)} stages of a 3-layer width-512 MLP shared by all 4,800 "
+                        f"variables (second stage reads the first's outputs at t-2..t+2)"),
+            "train_label_s": t_label, "train_s": t_train, "train_mfma": train_rate,
+            "primal_weight": None if model.log_weight is None else float(np.exp(model.log_weight)),
+            "engine_setup_s": t_setup, "launch_shape": shape, "network_forward_mfma": net_rate,
             "learned_batched_resident": learned, "cold_batched_resident": cold,
             "host_to_host": h2h, "plain_single_gpu": plain,
             "speedup_vs_plain": value / plain["resident_lps_per_s"] if plain["resident_lps_per_s"] else None,
@@ -959,7 +973,9 @@ def main():
     ap.add_argument("--skip", default="", help="comma list of sub-records to leave out: c1,c2,c4,beyond_mall")
     ap.add_argument("--train", type=int, default=50_000)
     ap.add_argument("--eval", type=int, default=1024)
-    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--stages", type=int, default=3, help="node-model stages")
+    ap.add_argument("--node-steps", type=int, default=60_000, help="training steps per stage")
+    ap.add_argument("--node-rows", type=int, default=32768, help="node rows per training step")
     ap.add_argument("--plain-sample", type=int, default=64)
     ap.add_argument("--obj-sample", type=int, default=64)
     ap.add_argument("--cpu-cap-s", type=float, default=12.0)

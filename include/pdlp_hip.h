@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PDLP_ABI_VERSION 8
+#define PDLP_ABI_VERSION 9
 
 /* precision of iterates, matrix values and problem vectors */
 #define PDLP_F32 0
@@ -595,6 +595,10 @@ typedef struct PdlpMlp {
     float *gw1, *gw2, *gw3;            /* weight gradients (FP32, same layout as w)  */
     float *gb1, *gb2, *gb3;            /* bias gradients (FP32)                      */
     float* loss;                       /* 1 float: sum_b,j w_j (out - target)^2 / (batch*d_out) */
+    float* work;                       /* backward: NULL, or work_floats FP32 of scratch for the
+                                          K-split weight-gradient GEMMs (K = batch) and column sums,
+                                          so a long batch with few output tiles still fills the chip */
+    int64_t work_floats;
 } PdlpMlp;
 
 /* h1 = relu(x w1 + b1), h2 = relu(h1 w2 + b2), out = h2 w3 + b3. */
@@ -607,6 +611,62 @@ int pdlp_adam_step(float* w, const float* g, float* m, float* v, uint16_t* wb, i
                    float beta1, float beta2, float eps, int32_t step, void* stream);
 /* y = bf16(x), round to nearest even. */
 int pdlp_cast_bf16(const float* x, uint16_t* y, int64_t n, void* stream);
+
+/* ---------------------------------------------------------------- node model
+ * A network applied to every node (variable) of every LP of a family with one
+ * set of weights.  Its input rows are gathered from compact per-instance
+ * vectors through a fixed table, and its outputs are mapped back to a start
+ * [x, y, log primal weight] by a sparse assembly map.  No counterpart in the
+ * reference, whose PDHG always starts at 0 (pdhg.py:175-176). */
+
+/* Row r of the output holds node q = rows[r] (q = b * nodes + j: node j of
+ * instance b); with rows NULL, q = hash(seed, r) mod (batch * nodes) when
+ * seed != 0 (uniform sampling with replacement, the same rows for the same
+ * seed and r), else q = r.  Entry f < width of the row is
+ *   v = (g * scale[j][f] - mean[f]) * inv_std[f],
+ *   g = 0 for table[j][f] < 0, src[b][t] for t = table[j][f] < src_len,
+ *       shared[t - src_len] for t < src_len + shared_len, else
+ *       src2[b][t - src_len - shared_len] (t < src_len + shared_len + src2_len:
+ *       a second per-instance source, e.g. an earlier network's outputs);
+ * entries width .. ld_out-1 are 0.  scale, mean, inv_std may be NULL (1, 0, 1).
+ * Exactly one of out_bf16 / out_f32 is written. */
+typedef struct PdlpNodeGather {
+    int32_t batch, nodes, width, ld_out;
+    int32_t src_len, shared_len;
+    int64_t src2_len;
+    int64_t count;                 /* output rows (rows != NULL or seed != 0), else batch * nodes */
+    uint64_t seed;
+    const float* src;              /* [batch][src_len]                  */
+    const float* shared;           /* [shared_len] (NULL if 0)          */
+    const float* src2;             /* [batch][src2_len] (NULL if 0)     */
+    const int32_t* table;          /* [nodes][width]                    */
+    const float* scale;            /* [nodes][width]                    */
+    const float* mean;             /* [width]                           */
+    const float* inv_std;          /* [width]                           */
+    const int64_t* rows;           /* [count]                           */
+    uint16_t* out_bf16;            /* [count][ld_out]                   */
+    float* out_f32;                /* [count][ld_out]                   */
+} PdlpNodeGather;
+
+int pdlp_node_gather(const PdlpNodeGather* g, void* stream);
+
+/* out[b][i] = bias[i] + sum_{e in indptr[i] .. indptr[i+1]-1} weight[e] *
+ * net[(b * nodes + node[e]) * ld_net + chan[e]] for i < len (columns len ..
+ * ld_out-1 untouched): net is the node network's FP32 output for `batch`
+ * instances in order, out the start matrix read by pdlp_family_set_start. */
+typedef struct PdlpNodeAssembly {
+    int32_t batch, nodes, ld_net;
+    int32_t len, ld_out, nnz;
+    const int32_t* indptr;         /* [len + 1]   */
+    const int32_t* node;           /* [nnz]       */
+    const int32_t* chan;           /* [nnz]       */
+    const float* weight;           /* [nnz]       */
+    const float* bias;             /* [len] or NULL */
+    const float* net;              /* [batch * nodes][ld_net] */
+    float* out;                    /* [batch][ld_out]         */
+} PdlpNodeAssembly;
+
+int pdlp_node_assemble(const PdlpNodeAssembly* a, void* stream);
 
 #ifdef __cplusplus
 }

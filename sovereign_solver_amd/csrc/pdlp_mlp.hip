@@ -236,37 +236,62 @@ __global__ void __launch_bounds__(GEMM_THREADS) gemm_kernel(PdlpGemm g, int kc) 
     }
 }
 
-// sum of the split_k partial products in order of z, then the epilogue
-__global__ void __launch_bounds__(256) gemm_split_finish_kernel(PdlpGemm g, int parts) {
-    const int64_t total = (int64_t)g.M * g.N;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+// Sum of `parts` partial arrays part[z][0 .. total) in a fixed order (the same
+// bits every run), then the GEMM epilogue (EPI) or a plain store.  A workgroup
+// takes 64 consecutive elements; its 4 wavefronts sum every 4th part (four
+// loads in flight per lane), then the 4 sums are added in LDS in wavefront order.
+template <bool EPI>
+__global__ void __launch_bounds__(256) sum_parts_kernel(const float* __restrict__ part, int64_t total, int parts,
+                                                        float* out, PdlpGemm g) {
+    __shared__ float red[4][64];
+    const int c = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    for (int64_t base = blockIdx.x * 64LL; base < total; base += gridDim.x * 64LL) {
+        const int64_t e = base + c;
         float s = 0.f;
-        for (int z = 0; z < parts; ++z) s += g.work[(size_t)z * total + e];
-        epilogue(g, (int)(e / g.N), (int)(e % g.N), s);
+        if (e < total) {
+            int z = grp;
+            for (; z + 12 < parts; z += 16) {
+                const float v0 = part[(size_t)z * total + e], v1 = part[(size_t)(z + 4) * total + e];
+                const float v2 = part[(size_t)(z + 8) * total + e], v3 = part[(size_t)(z + 12) * total + e];
+                s += v0;
+                s += v1;
+                s += v2;
+                s += v3;
+            }
+            for (; z < parts; z += 4) s += part[(size_t)z * total + e];
+        }
+        red[grp][c] = s;
+        __syncthreads();
+        if (grp == 0 && e < total) {
+            const float t = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
+            if constexpr (EPI) epilogue(g, (int)(e / g.N), (int)(e % g.N), t);
+            else out[e] = t;
+        }
+        __syncthreads();
     }
 }
+
+unsigned parts_grid(int64_t total) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((total + 63) / 64, 8192)); }
 
 __global__ void zero1_kernel(float* p) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0.f;
 }
 
-// dout = 2 w (out - t) / (B N); loss += w (out - t)^2 / (B N).  blockIdx.x
-// strides over the columns, blockIdx.y over the rows (no index division);
-// one atomic per workgroup for the loss.
+// dout = 2 w (out - t) / (B N); loss += w (out - t)^2 / (B N), over the
+// flattened B x N elements (any N, from one column to thousands); one atomic
+// per workgroup for the loss.
 __global__ void __launch_bounds__(256) loss_grad_kernel(const float* out, const float* t, const float* w, float* dout,
-                                                        uint16_t* dout_b, float* loss, int B, int N, float inv) {
+                                                        uint16_t* dout_b, float* loss, int64_t total, int N,
+                                                        float inv) {
     __shared__ double lds[16 * 4];
     double part = 0.0;
-    for (int b = blockIdx.y; b < B; b += gridDim.y) {
-        const size_t row = (size_t)b * N;
-        for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < N; j += gridDim.x * blockDim.x) {
-            const float wj = w != nullptr ? w[j] : 1.f;
-            const float d = out[row + j] - t[row + j];
-            const float gr = 2.f * wj * d * inv;
-            dout[row + j] = gr;
-            dout_b[row + j] = f2bf(gr);
-            part += (double)(wj * d * d) * inv;
-        }
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const float wj = w != nullptr ? w[total < INT32_MAX ? (int)e % N : (int)(e % N)] : 1.f;
+        const float d = out[e] - t[e];
+        const float gr = 2.f * wj * d * inv;
+        dout[e] = gr;
+        dout_b[e] = f2bf(gr);
+        part += (double)(wj * d * d) * inv;
     }
     double acc[4] = {part, 0.0, 0.0, 0.0};
     block_sum<4>(acc, lds);
@@ -274,23 +299,26 @@ __global__ void __launch_bounds__(256) loss_grad_kernel(const float* out, const 
 }
 
 // db[j] = sum_b dz[b][j]; dz FP32 (dzf) or BF16 (dzb).  A workgroup takes 64
-// columns; its 16 wavefronts sum every 16th row (8 rows' loads in flight per
-// lane), then add their sums in LDS in a fixed order.
+// columns of one chunk of rows (blockIdx.y; rows_per rows each); its 16
+// wavefronts sum every 16th row (8 rows' loads in flight per lane), then add
+// their sums in LDS in a fixed order.  With more than one chunk the sums go
+// to part[chunk][j] and sum_parts_kernel adds them in chunk order.
 constexpr int CS_COLS = 64, CS_GROUPS = 16, CS_UNROLL = 8;
 
 __global__ void __launch_bounds__(CS_COLS * CS_GROUPS) colsum_kernel(const float* dzf, const uint16_t* dzb, float* db,
-                                                                   int B, int N) {
+                                                                   float* part_out, int B, int N, int rows_per) {
     __shared__ float part[CS_GROUPS][CS_COLS];
     const int c = threadIdx.x % CS_COLS, grp = threadIdx.x / CS_COLS;
     const int j = blockIdx.x * CS_COLS + c;
+    const int rbeg = blockIdx.y * rows_per, rend = min(B, rbeg + rows_per);
     float s = 0.f;
     if (j < N) {
-        for (int b0 = grp; b0 < B; b0 += CS_GROUPS * CS_UNROLL) {
+        for (int b0 = rbeg + grp; b0 < rend; b0 += CS_GROUPS * CS_UNROLL) {
             float v[CS_UNROLL];
 #pragma unroll
             for (int u = 0; u < CS_UNROLL; ++u) {
                 const int b = b0 + u * CS_GROUPS;
-                v[u] = b < B ? (dzf != nullptr ? dzf[(size_t)b * N + j] : bf2f(dzb[(size_t)b * N + j])) : 0.f;
+                v[u] = b < rend ? (dzf != nullptr ? dzf[(size_t)b * N + j] : bf2f(dzb[(size_t)b * N + j])) : 0.f;
             }
 #pragma unroll
             for (int u = 0; u < CS_UNROLL; ++u) s += v[u];
@@ -302,7 +330,8 @@ __global__ void __launch_bounds__(CS_COLS * CS_GROUPS) colsum_kernel(const float
         float t = part[0][c];
 #pragma unroll
         for (int g = 1; g < CS_GROUPS; ++g) t += part[g][c];
-        db[j] = t;
+        if (gridDim.y > 1) part_out[(size_t)blockIdx.y * N + j] = t;
+        else db[j] = t;
     }
 }
 
@@ -360,10 +389,38 @@ int gemm(const PdlpGemm& g, hipStream_t s) {
     }
     int rc = hip_fail(hipGetLastError(), "gemm_kernel launch");
     if (!rc && used > 1) {
-        gemm_split_finish_kernel<<<grid1((int64_t)g.M * g.N), 256, 0, s>>>(g, used);
-        rc = hip_fail(hipGetLastError(), "gemm_split_finish_kernel launch");
+        const int64_t total = (int64_t)g.M * g.N;
+        sum_parts_kernel<true><<<parts_grid(total), 256, 0, s>>>(g.work, total, used, nullptr, g);
+        rc = hip_fail(hipGetLastError(), "sum_parts_kernel launch");
     }
     return rc;
+}
+
+// Column sums of a B x N gradient: with work, the rows are cut into chunks of at
+// least 256 rows so the launch has about 1024 workgroups (a long batch with few columns).
+int colsum(const float* dzf, const uint16_t* dzb, float* db, int B, int N, float* work, int64_t work_floats,
+           hipStream_t s) {
+    const int cblocks = (N + CS_COLS - 1) / CS_COLS;
+    int chunks = 1;
+    if (work != nullptr && N > 0) {
+        const int64_t cap = std::min<int64_t>(work_floats / N, 1024);
+        chunks = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)(1024 + cblocks - 1) / cblocks,
+                                                                (int64_t)(B + 255) / 256, cap}));
+    }
+    const int rows_per = (B + chunks - 1) / chunks;
+    colsum_kernel<<<dim3(cblocks, chunks), CS_COLS * CS_GROUPS, 0, s>>>(dzf, dzb, db, work, B, N, rows_per);
+    if (chunks > 1) sum_parts_kernel<false><<<parts_grid(N), 256, 0, s>>>(work, N, chunks, db, PdlpGemm{});
+    return hip_fail(hipGetLastError(), "colsum");
+}
+
+// K parts of a weight-gradient GEMM (M x N output, K = batch): about 512
+// workgroups, at least 4 K stages per part, within the work space
+int weight_split(int M, int N, int K, int64_t work_floats) {
+    if (work_floats <= 0) return 1;
+    const int64_t tiles = (int64_t)((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+    const int64_t stages = (K + BK - 1) / BK;
+    const int64_t fit = work_floats / std::max<int64_t>(1, (int64_t)M * N);
+    return (int)std::max<int64_t>(1, std::min<int64_t>({(512 + tiles - 1) / tiles, stages / 4, fit, 1024}));
 }
 
 PdlpGemm make(int M, int N, int K, const uint16_t* A, int lda, int ta, const uint16_t* B, int ldb, int tb,
@@ -404,20 +461,29 @@ int pdlp_mlp_bwd_bf16(const PdlpMlp* p, void* stream) {
     if (p == nullptr || !p->target || !p->dout || !p->dout_b || !p->dz2 || !p->dz1 || !p->gw1 || !p->gw2 ||
         !p->gw3 || !p->gb1 || !p->gb2 || !p->gb3 || !p->loss)
         return fail(PDLP_E_ARG, "pdlp_mlp_bwd_bf16: NULL pointer");
+    if (p->work_floats < 0 || (p->work_floats > 0 && p->work == nullptr))
+        return fail(PDLP_E_ARG, "pdlp_mlp_bwd_bf16: work_floats > 0 needs work");
     hipStream_t s = as_stream(stream);
     const int B = p->batch, di = p->d_in, dh = p->d_h, dout = p->d_out;
     const int64_t total = (int64_t)B * dout;
+    float* work = p->work_floats > 0 ? p->work : nullptr;
+    const int64_t wf = work != nullptr ? p->work_floats : 0;
     zero1_kernel<<<1, 64, 0, s>>>(p->loss);
-    const dim3 lg((unsigned)std::min((dout + 255) / 256, 64), (unsigned)std::min(B, 16));
-    loss_grad_kernel<<<lg, 256, 0, s>>>(p->out, p->target, p->out_weight, p->dout, p->dout_b, p->loss, B, dout,
+    const unsigned lg = (unsigned)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 2048));
+    loss_grad_kernel<<<lg, 256, 0, s>>>(p->out, p->target, p->out_weight, p->dout, p->dout_b, p->loss, total, dout,
                                         1.f / (float)total);
-    colsum_kernel<<<(dout + CS_COLS - 1) / CS_COLS, CS_COLS * CS_GROUPS, 0, s>>>(p->dout, nullptr, p->gb3, B, dout);
-    int rc = hip_fail(hipGetLastError(), "mlp loss/bias kernels");
+    int rc = hip_fail(hipGetLastError(), "mlp loss kernel");
+    if (!rc) rc = colsum(p->dout, nullptr, p->gb3, B, dout, work, wf, s);
     // gw3 = h2^T dout ; dz2 = (dout w3^T) * [h2 > 0]
-    if (!rc) rc = gemm(make(dh, dout, B, p->h2, dh, 1, p->dout_b, dout, 0, nullptr, 0, nullptr, p->gw3, nullptr, dout), s);
+    auto wgrad = [&](PdlpGemm g) {
+        g.split_k = weight_split(g.M, g.N, g.K, wf);
+        g.work = work;
+        return gemm(g, s);
+    };
+    if (!rc) rc = wgrad(make(dh, dout, B, p->h2, dh, 1, p->dout_b, dout, 0, nullptr, 0, nullptr, p->gw3, nullptr, dout));
     if (!rc) {
         // K = d_out is long and the output (batch x d_h) has few tiles: split K, with dout
-        // (free once gb3 is taken from it) as the work space of split_k * batch * d_h floats
+        // (free once gb3 and gw3 are taken from it) as the work space of split_k * batch * d_h floats
         PdlpGemm g = make(B, dh, dout, p->dout_b, dout, 0, p->w3, dout, 1, nullptr, 0, p->h2, nullptr, p->dz2, dh);
         const int tiles = ((B + BM - 1) / BM) * ((dh + BN - 1) / BN);
         const int stages = (dout + BK - 1) / BK;
@@ -425,19 +491,13 @@ int pdlp_mlp_bwd_bf16(const PdlpMlp* p, void* stream) {
         g.work = p->dout;
         rc = gemm(g, s);
     }
-    if (!rc) {
-        colsum_kernel<<<(dh + CS_COLS - 1) / CS_COLS, CS_COLS * CS_GROUPS, 0, s>>>(nullptr, p->dz2, p->gb2, B, dh);
-        rc = hip_fail(hipGetLastError(), "colsum");
-    }
+    if (!rc) rc = colsum(nullptr, p->dz2, p->gb2, B, dh, work, wf, s);
     // gw2 = h1^T dz2 ; dz1 = (dz2 w2^T) * [h1 > 0]
-    if (!rc) rc = gemm(make(dh, dh, B, p->h1, dh, 1, p->dz2, dh, 0, nullptr, 0, nullptr, p->gw2, nullptr, dh), s);
+    if (!rc) rc = wgrad(make(dh, dh, B, p->h1, dh, 1, p->dz2, dh, 0, nullptr, 0, nullptr, p->gw2, nullptr, dh));
     if (!rc) rc = gemm(make(B, dh, dh, p->dz2, dh, 0, p->w2, dh, 1, nullptr, 0, p->h1, nullptr, p->dz1, dh), s);
-    if (!rc) {
-        colsum_kernel<<<(dh + CS_COLS - 1) / CS_COLS, CS_COLS * CS_GROUPS, 0, s>>>(nullptr, p->dz1, p->gb1, B, dh);
-        rc = hip_fail(hipGetLastError(), "colsum");
-    }
+    if (!rc) rc = colsum(nullptr, p->dz1, p->gb1, B, dh, work, wf, s);
     // gw1 = x^T dz1
-    if (!rc) rc = gemm(make(di, dh, B, p->x, di, 1, p->dz1, dh, 0, nullptr, 0, nullptr, p->gw1, nullptr, dh), s);
+    if (!rc) rc = wgrad(make(di, dh, B, p->x, di, 1, p->dz1, dh, 0, nullptr, 0, nullptr, p->gw1, nullptr, dh));
     return rc;
 }
 

@@ -1,0 +1,128 @@
+// Node-model plumbing of the learned warm start: input/target rows gathered
+// from compact per-instance vectors, and the sparse map from per-node network
+// outputs back to a start [x, y, log weight] per LP.
+//
+// Gather: a group of `lanes` lanes per output row (the power of two covering the
+// row, at most 64: one row per wavefront for 48 inputs, eight rows for 8
+// outputs), lane f writing entries f, f + lanes, ... of the row, so a row's
+// values go out as one contiguous store and the table row is one coalesced
+// load; the instance's source vector is re-read by the rows of the same
+// instance and stays in L2.
+//
+// Assembly: one thread per (instance, start entry); the entries of one
+// instance read that instance's block of network rows.
+
+#include "pdlp_common.hpp"
+
+#include <algorithm>
+
+namespace pdlp {
+namespace {
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// splitmix64 finaliser: a bijective mix of (seed, r)
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+constexpr int GATHER_THREADS = 256;
+
+__global__ void __launch_bounds__(GATHER_THREADS) node_gather_kernel(PdlpNodeGather g, int64_t count, int lanes) {
+    const int lane = threadIdx.x & (lanes - 1);
+    const int per_block = GATHER_THREADS / lanes;
+    const int64_t groups = (int64_t)gridDim.x * per_block;
+    const int64_t total = (int64_t)g.batch * g.nodes;
+    for (int64_t r = blockIdx.x * (int64_t)per_block + threadIdx.x / lanes; r < count; r += groups) {
+        int64_t q;
+        if (g.rows != nullptr) q = g.rows[r];
+        else if (g.seed != 0) q = (int64_t)(mix64(g.seed ^ mix64((uint64_t)r)) % (uint64_t)total);
+        else q = r;
+        const int64_t b = q / g.nodes;
+        const int j = (int)(q - b * g.nodes);
+        const float* src = g.src + b * g.src_len;
+        const float* src2 = g.src2 + b * g.src2_len;
+        const int base2 = g.src_len + g.shared_len;
+        for (int f = lane; f < g.ld_out; f += lanes) {
+            float v = 0.f;
+            if (f < g.width) {
+                const size_t e = (size_t)j * g.width + f;
+                const int t = g.table[e];
+                float x = t < 0 ? 0.f : t < g.src_len ? src[t] : t < base2 ? g.shared[t - g.src_len] : src2[t - base2];
+                if (g.scale != nullptr) x *= g.scale[e];
+                if (g.mean != nullptr) x -= g.mean[f];
+                if (g.inv_std != nullptr) x *= g.inv_std[f];
+                v = x;
+            }
+            const size_t o = (size_t)r * g.ld_out + f;
+            if (g.out_bf16 != nullptr) g.out_bf16[o] = to_bf16(v);
+            else g.out_f32[o] = v;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) node_assemble_kernel(PdlpNodeAssembly a) {
+    const int b = blockIdx.y;
+    const float* net = a.net + (size_t)b * a.nodes * a.ld_net;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.len; i += gridDim.x * blockDim.x) {
+        float s = a.bias != nullptr ? a.bias[i] : 0.f;
+        for (int e = a.indptr[i]; e < a.indptr[i + 1]; ++e)
+            s += a.weight[e] * net[(size_t)a.node[e] * a.ld_net + a.chan[e]];
+        a.out[(size_t)b * a.ld_out + i] = s;
+    }
+}
+
+}  // namespace
+}  // namespace pdlp
+
+using namespace pdlp;
+
+extern "C" {
+
+int pdlp_node_gather(const PdlpNodeGather* g, void* stream) {
+    if (g == nullptr) return fail(PDLP_E_ARG, "PdlpNodeGather is NULL");
+    if (g->batch < 0 || g->nodes < 1 || g->width < 0 || g->ld_out < g->width || g->src_len < 0 ||
+        g->shared_len < 0 || g->src2_len < 0 || g->count < 0)
+        return fail(PDLP_E_SHAPE, "pdlp_node_gather: batch >= 0, nodes >= 1, ld_out >= width >= 0, lengths >= 0");
+    if ((g->out_bf16 == nullptr) == (g->out_f32 == nullptr))
+        return fail(PDLP_E_ARG, "pdlp_node_gather: exactly one of out_bf16 / out_f32");
+    if (g->width > 0 && g->table == nullptr) return fail(PDLP_E_ARG, "pdlp_node_gather: table is NULL");
+    if (g->src_len > 0 && g->batch > 0 && g->src == nullptr) return fail(PDLP_E_ARG, "pdlp_node_gather: src is NULL");
+    if (g->shared_len > 0 && g->shared == nullptr) return fail(PDLP_E_ARG, "pdlp_node_gather: shared is NULL");
+    if (g->src2_len > 0 && g->batch > 0 && g->src2 == nullptr)
+        return fail(PDLP_E_ARG, "pdlp_node_gather: src2 is NULL");
+    const int64_t total = (int64_t)g->batch * g->nodes;
+    const bool listed = g->rows != nullptr || g->seed != 0;
+    const int64_t count = listed ? g->count : total;
+    if (listed && count > 0 && total == 0) return fail(PDLP_E_SHAPE, "pdlp_node_gather: rows drawn from an empty batch");
+    if (count == 0 || g->ld_out == 0) return 0;
+    int lanes = 1;
+    while (lanes < 64 && lanes < g->ld_out) lanes *= 2;
+    const int64_t rpb = GATHER_THREADS / lanes;
+    const unsigned blocks = (unsigned)std::min<int64_t>((count + rpb - 1) / rpb, 65536);
+    node_gather_kernel<<<blocks, GATHER_THREADS, 0, as_stream(stream)>>>(*g, count, lanes);
+    return hip_fail(hipGetLastError(), "node_gather_kernel launch");
+}
+
+int pdlp_node_assemble(const PdlpNodeAssembly* a, void* stream) {
+    if (a == nullptr) return fail(PDLP_E_ARG, "PdlpNodeAssembly is NULL");
+    if (a->batch < 0 || a->nodes < 1 || a->ld_net < 1 || a->len < 0 || a->ld_out < a->len || a->nnz < 0)
+        return fail(PDLP_E_SHAPE, "pdlp_node_assemble: batch >= 0, nodes >= 1, ld_net >= 1, ld_out >= len >= 0");
+    if (a->batch > 65535) return fail(PDLP_E_LIMIT, "pdlp_node_assemble: at most 65535 instances per call");
+    if (a->batch == 0 || a->len == 0) return 0;
+    if (!a->indptr || !a->net || !a->out || (a->nnz > 0 && (!a->node || !a->chan || !a->weight)))
+        return fail(PDLP_E_ARG, "pdlp_node_assemble: NULL pointer");
+    dim3 grid((unsigned)std::min((a->len + 255) / 256, 1024), (unsigned)a->batch);
+    node_assemble_kernel<<<grid, 256, 0, as_stream(stream)>>>(*a);
+    return hip_fail(hipGetLastError(), "node_assemble_kernel launch");
+}
+
+}  // extern "C"

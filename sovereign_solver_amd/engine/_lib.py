@@ -13,7 +13,7 @@ import threading
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "_lib", "libpdlp_hip.so")
 
-PDLP_ABI_VERSION = 8
+PDLP_ABI_VERSION = 9
 PDLP_F32, PDLP_F64 = 0, 1
 PDLP_E_ARG, PDLP_E_TUNE, PDLP_E_SHAPE, PDLP_E_LIMIT = -1, -2, -3, -4
 PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL, PDLP_INFEASIBLE, PDLP_UNBOUNDED, PDLP_SKIPPED = 0, 1, 3, 4, 5, 6
@@ -133,6 +133,24 @@ class PdlpMlp(C.Structure):
         ("h1", _p), ("h2", _p), ("out", _p),
         ("target", _p), ("out_weight", _p), ("dout", _p), ("dout_b", _p), ("dz2", _p), ("dz1", _p),
         ("gw1", _p), ("gw2", _p), ("gw3", _p), ("gb1", _p), ("gb2", _p), ("gb3", _p), ("loss", _p),
+        ("work", _p), ("work_floats", C.c_int64),
+    ]
+
+
+class PdlpNodeGather(C.Structure):
+    _fields_ = [
+        ("batch", C.c_int32), ("nodes", C.c_int32), ("width", C.c_int32), ("ld_out", C.c_int32),
+        ("src_len", C.c_int32), ("shared_len", C.c_int32), ("src2_len", C.c_int64), ("count", C.c_int64),
+        ("seed", C.c_uint64), ("src", _p), ("shared", _p), ("src2", _p), ("table", _p), ("scale", _p), ("mean", _p), ("inv_std", _p), ("rows", _p),
+        ("out_bf16", _p), ("out_f32", _p),
+    ]
+
+
+class PdlpNodeAssembly(C.Structure):
+    _fields_ = [
+        ("batch", C.c_int32), ("nodes", C.c_int32), ("ld_net", C.c_int32),
+        ("len", C.c_int32), ("ld_out", C.c_int32), ("nnz", C.c_int32),
+        ("indptr", _p), ("node", _p), ("chan", _p), ("weight", _p), ("bias", _p), ("net", _p), ("out", _p),
     ]
 
 
@@ -181,6 +199,8 @@ _SIGS = {
     "pdlp_adam_step": (C.c_int, [_p, _p, _p, _p, _p, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float,
                                  C.c_int32, _p]),
     "pdlp_cast_bf16": (C.c_int, [_p, _p, C.c_int64, _p]),
+    "pdlp_node_gather": (C.c_int, [C.POINTER(PdlpNodeGather), _p]),
+    "pdlp_node_assemble": (C.c_int, [C.POINTER(PdlpNodeAssembly), _p]),
 }
 
 

@@ -23,7 +23,8 @@ from ..engine.device import require_gpu, stream_handle
 
 
 class WarmStartMLP:
-    def __init__(self, d_in: int, d_h: int, d_out: int, seed: int = 0, device=None):
+    def __init__(self, d_in: int, d_h: int, d_out: int, seed: int = 0, device=None,
+                 stream: Optional[torch.cuda.Stream] = None):
         self.lib = L.lib()
         self.device = device or require_gpu()
         self.d_in, self.d_h, self.d_out = d_in, d_h, d_out
@@ -42,7 +43,7 @@ class WarmStartMLP:
                 host[o: o + int(np.prod(shp))] = (rng.standard_normal(int(np.prod(shp))) *
                                                   np.sqrt(2.0 / shp[0])).astype(np.float32)
         dv = self.device
-        self.stream = torch.cuda.Stream(device=dv)
+        self.stream = stream if stream is not None else torch.cuda.Stream(device=dv)
         with torch.cuda.stream(self.stream):
             self.w = torch.from_numpy(host).to(dv)
             self.wb = torch.empty(off, dtype=torch.bfloat16, device=dv)
@@ -85,19 +86,29 @@ class WarmStartMLP:
         g = self.g.cpu().numpy()
         return {k: g[o: o + int(np.prod(s))].reshape(s).astype(np.float64) for k, (o, s) in self.offsets.items()}
 
-    def _buffers(self, B: int):
+    # batches from this size on get a work space for the K-split weight gradients
+    SPLIT_BATCH = 2048
+
+    def _buffers(self, B: int, backward: bool = False):
+        """Activation buffers of a B-row batch; the backward scratch is allocated on
+        the first backward pass of that size (inference never holds it)."""
+        dv, bf, f32 = self.device, torch.bfloat16, torch.float32
         b = self._bufs.get(B)
         if b is None:
-            dv, bf, f32 = self.device, torch.bfloat16, torch.float32
             with torch.cuda.stream(self.stream):
                 b = dict(h1=torch.empty(B, self.d_h, dtype=bf, device=dv),
                          h2=torch.empty(B, self.d_h, dtype=bf, device=dv),
-                         out=torch.empty(B, self.d_out, dtype=f32, device=dv),
-                         dout=torch.empty(B, self.d_out, dtype=f32, device=dv),
-                         dout_b=torch.empty(B, self.d_out, dtype=bf, device=dv),
-                         dz2=torch.empty(B, self.d_h, dtype=bf, device=dv),
-                         dz1=torch.empty(B, self.d_h, dtype=bf, device=dv))
+                         out=torch.empty(B, self.d_out, dtype=f32, device=dv))
             self._bufs[B] = b
+        if backward and "dout" not in b:
+            dh, di, do = self.d_h, self.d_in, self.d_out
+            wf = 32 * max(dh * dh, di * dh, dh * do) if B >= self.SPLIT_BATCH else 0
+            with torch.cuda.stream(self.stream):
+                b.update(dout=torch.empty(B, do, dtype=f32, device=dv),
+                         dout_b=torch.empty(B, do, dtype=bf, device=dv),
+                         dz2=torch.empty(B, dh, dtype=bf, device=dv),
+                         dz1=torch.empty(B, dh, dtype=bf, device=dv),
+                         work=torch.empty(max(wf, 1), dtype=f32, device=dv), work_floats=wf)
         return b
 
     def _desc(self, xb: torch.Tensor, target: Optional[torch.Tensor] = None,
@@ -105,7 +116,7 @@ class WarmStartMLP:
         B = xb.shape[0]
         if xb.dtype != torch.bfloat16 or xb.shape[1] != self.d_in or not xb.is_contiguous():
             raise ValueError("input must be a contiguous [B, d_in] bfloat16 device tensor")
-        bu = self._buffers(B)
+        bu = self._buffers(B, backward=target is not None)
         d = L.PdlpMlp(batch=B, d_in=self.d_in, d_h=self.d_h, d_out=self.d_out, x=xb.data_ptr(),
                       w1=self._p("W1", self.wb), w2=self._p("W2", self.wb), w3=self._p("W3", self.wb),
                       b1=self._p("b1", self.w), b2=self._p("b2", self.w), b3=self._p("b3", self.w),
@@ -120,6 +131,8 @@ class WarmStartMLP:
             d.gw1, d.gw2, d.gw3 = self._p("W1", self.g), self._p("W2", self.g), self._p("W3", self.g)
             d.gb1, d.gb2, d.gb3 = self._p("b1", self.g), self._p("b2", self.g), self._p("b3", self.g)
             d.loss = self.loss.data_ptr()
+            if bu["work_floats"]:
+                d.work, d.work_floats = bu["work"].data_ptr(), bu["work_floats"]
         return d
 
     # ------------------------------------------------------------------- API

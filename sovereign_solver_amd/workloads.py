@@ -164,13 +164,104 @@ class DispatchFamily:
         return Problem(c=self.objective(c), A_ub=self.A, b_ub=self.rhs(d), A_eq=self.no_eq,
                        b_eq=self.b_eq, lb=self.lb, ub=self.ub, integer_mask=np.zeros(n, dtype=bool), name=name)
 
-    def features_from(self, problems) -> np.ndarray:
-        """Network inputs [B, T + G] read back from instances' own vectors (demand from the
-        first T rows, -b_ub[:T]; costs from the first period's columns, c[:G])."""
+    def params_from(self, problems):
+        """(demand [B, T], cost [B, G]) read back from instances' own vectors (demand
+        from the first T rows, -b_ub[:T]; costs from the first period's columns, c[:G])."""
         T, G = self.T, self.G
         d = -np.stack([np.asarray(p.b_ub[:T], float) for p in problems])
         c = np.stack([np.asarray(p.c[:G], float) for p in problems])
+        return d, c
+
+    def features_from(self, problems) -> np.ndarray:
+        """Network inputs [B, T + G] read back from instances' own vectors."""
+        d, c = self.params_from(problems)
         return np.concatenate([d / self.demand - 1.0, c / self.cost - 1.0], axis=1)
+
+    # ------------------------------------------------------------ node model
+    # One node per variable p_gt (node j = t*G + g, the column index).  Its input row
+    # gathers, from a per-instance vector and a shared one:
+    #   demand 4 d_{t+k} / sum(Pmax), k = -3..3 (clamped to the horizon);
+    #   cost c_g / 60, its perturbation c_g / cost_g - 1, Pmax_g / 400, and its
+    #   merit-order position: 2 (capacity strictly cheaper than g) / sum(Pmax) and
+    #   2 (that + Pmax_g) / sum(Pmax);
+    #   the merit-order dispatch that ignores ramps, clip(d_t' - cheaper_g, 0, Pmax_g) / Pmax_g,
+    #   at t' = t-2..t+2, and its marginal price / 60 at t-1..t+1;
+    #   the period t / (T-1), first / last period flags;
+    #   the whole demand profile d / demand - 1.
+    # Targets (4 channels): p_gt / Pmax_g, the ramp-up and ramp-down duals of (t, g) / 10
+    # (rows of t >= 1), and the period's demand dual / 60.
+    NODE_WINDOW_D, NODE_WINDOW_X, NODE_WINDOW_P = 3, 2, 1
+
+    def _node_offsets(self):
+        T, G = self.T, self.G
+        o = dict(D=0, C=T, U=T + G, CC=T + 2 * G, CC2=T + 3 * G, P=T + 4 * G, M=2 * T + 4 * G, X=3 * T + 4 * G)
+        return o, 3 * T + 4 * G + T * G
+
+    def node_spec(self):
+        """Index table, shared vector and target map of the node model (see above)."""
+        from .learned.nodes import NodeSpec
+        T, G = self.T, self.G
+        o, L = self._node_offsets()
+        shared = np.concatenate([self.pmax / 400.0, np.arange(T) / max(T - 1, 1), (np.arange(T) == 0),
+                                 (np.arange(T) == T - 1)]).astype(np.float32)
+        sPM, sT, sF, sL = L, L + G, L + G + T, L + G + 2 * T
+        t = np.repeat(np.arange(T), G)
+        g = np.tile(np.arange(G), T)
+        cl = lambda a: np.clip(a, 0, T - 1)
+        cols = [o["D"] + cl(t + k) for k in range(-self.NODE_WINDOW_D, self.NODE_WINDOW_D + 1)]
+        cols += [o["C"] + g, o["U"] + g, o["CC"] + g, o["CC2"] + g, sPM + g]
+        cols += [o["X"] + cl(t + k) * G + g for k in range(-self.NODE_WINDOW_X, self.NODE_WINDOW_X + 1)]
+        cols += [o["P"] + cl(t + k) for k in range(-self.NODE_WINDOW_P, self.NODE_WINDOW_P + 1)]
+        cols += [sT + t, sF + t, sL + t]
+        cols += [np.full(T * G, o["M"] + k) for k in range(T)]
+        table = np.stack(cols, axis=1).astype(np.int32)
+        n = self.n
+        up = np.where(t >= 1, n + T + 2 * ((t - 1) * G + g), -1)
+        tidx = np.stack([np.arange(n), up, np.where(t >= 1, up + 1, -1), n + t], axis=1).astype(np.int32)
+        tscale = np.stack([1.0 / np.tile(self.pmax, T), np.full(n, 0.1), np.full(n, 0.1), np.full(n, 1.0 / 60)],
+                          axis=1).astype(np.float32)
+        return NodeSpec(nodes=n, table=table, shared=shared, src_len=L, target_index=tidx, target_scale=tscale)
+
+    def node_neighbours(self, reach: int = 2) -> np.ndarray:
+        """[n, 2 reach + 1] nodes of the same generator at periods t-reach..t+reach
+        (clamped to the horizon): the inputs a second node-model stage reads."""
+        T, G = self.T, self.G
+        t = np.repeat(np.arange(T), G)
+        g = np.tile(np.arange(G), T)
+        return np.stack([np.clip(t + k, 0, T - 1) * G + g for k in range(-reach, reach + 1)], axis=1).astype(np.int32)
+
+    def node_inputs_params(self, d: np.ndarray, c: np.ndarray) -> np.ndarray:
+        """Per-instance vectors [B, L] (float32) of demand [B, T] and cost [B, G]."""
+        T, G = self.T, self.G
+        d, c = np.atleast_2d(d), np.atleast_2d(c)
+        B = d.shape[0]
+        o, L = self._node_offsets()
+        S = self.pmax.sum()
+        order = np.argsort(c, axis=1, kind="stable")
+        pm = self.pmax[order]
+        cum = np.cumsum(pm, axis=1)
+        cheaper = np.empty_like(c)
+        np.put_along_axis(cheaper, order, cum - pm, axis=1)
+        marg = np.minimum((cum[:, None, :] < d[:, :, None]).sum(axis=2), G - 1)       # [B, T]
+        price = np.take_along_axis(np.take_along_axis(c, order, axis=1), marg, axis=1)
+        xm = np.clip(d[:, :, None] - cheaper[:, None, :], 0.0, self.pmax) / self.pmax  # [B, T, G]
+        v = np.empty((B, L), np.float32)
+        v[:, o["D"]:o["D"] + T] = 4.0 * d / S
+        v[:, o["C"]:o["C"] + G] = c / 60.0
+        v[:, o["U"]:o["U"] + G] = c / self.cost - 1.0
+        v[:, o["CC"]:o["CC"] + G] = 2.0 * cheaper / S
+        v[:, o["CC2"]:o["CC2"] + G] = 2.0 * (cheaper + self.pmax) / S
+        v[:, o["P"]:o["P"] + T] = price / 60.0
+        v[:, o["M"]:o["M"] + T] = d / self.demand - 1.0
+        v[:, o["X"]:] = xm.reshape(B, T * G)
+        return v
+
+    def node_inputs(self, ids) -> np.ndarray:
+        P = [self.params(int(k)) for k in ids]
+        return self.node_inputs_params(np.stack([p[0] for p in P]), np.stack([p[1] for p in P]))
+
+    def node_inputs_from(self, problems) -> np.ndarray:
+        return self.node_inputs_params(*self.params_from(problems))
 
 
 class SetCoverNodes:

@@ -47,6 +47,28 @@ def test_struct_sizes_match_header():
     assert C.sizeof(L.PdlpTune) == 10 * 4
     # PdlpGemm: 10 int32 (M..accumulate), alpha, split_k, 7 pointers (A, B, bias, mask, C, Cb, work)
     assert C.sizeof(L.PdlpGemm) == 10 * 4 + 4 + 4 + 7 * 8
+    # PdlpMlp: 4 int32, 23 pointers, then the backward work space (pointer, int64)
+    assert C.sizeof(L.PdlpMlp) == 4 * 4 + 23 * 8 + 8 + 8
+    # PdlpNodeGather: 6 int32, src2_len, count, seed, 10 pointers; PdlpNodeAssembly: 6 int32, 7 pointers
+    assert C.sizeof(L.PdlpNodeGather) == 6 * 4 + 3 * 8 + 10 * 8
+    assert C.sizeof(L.PdlpNodeAssembly) == 6 * 4 + 7 * 8
+
+
+def test_node_kernels_reject_bad_descriptors_without_device():
+    lib = L.lib()
+    assert lib.pdlp_node_gather(None, None) == L.PDLP_E_ARG
+    g = L.PdlpNodeGather(batch=2, nodes=3, width=4, ld_out=2)          # ld_out < width
+    assert lib.pdlp_node_gather(C.byref(g), None) == L.PDLP_E_SHAPE
+    g = L.PdlpNodeGather(batch=2, nodes=3, width=4, ld_out=8)          # no output
+    assert lib.pdlp_node_gather(C.byref(g), None) == L.PDLP_E_ARG
+    a = L.PdlpNodeAssembly(batch=70000, nodes=1, ld_net=8, len=4, ld_out=4)
+    assert lib.pdlp_node_assemble(C.byref(a), None) == L.PDLP_E_LIMIT
+    a = L.PdlpNodeAssembly(batch=1, nodes=1, ld_net=8, len=4, ld_out=3)
+    assert lib.pdlp_node_assemble(C.byref(a), None) == L.PDLP_E_SHAPE
+    m = L.PdlpMlp(batch=4, d_in=8, d_h=8, d_out=8, work_floats=16)     # work_floats without work
+    for f in ("target", "dout", "dout_b", "dz2", "dz1", "gw1", "gw2", "gw3", "gb1", "gb2", "gb3", "loss"):
+        setattr(m, f, 16)                                                # any non-NULL value: rejected first
+    assert lib.pdlp_mlp_bwd_bf16(C.byref(m), None) == L.PDLP_E_ARG
 
 
 def test_argument_errors_without_device():

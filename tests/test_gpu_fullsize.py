@@ -7,7 +7,8 @@ tests/parity.py (1e-4, or parity with the reference PDHG at the same tolerance).
 
 * C1: the transportation LP, 1,000 sources x 2,000 sinks (n = 2,000,000);
 * C2: all 4,096 uneven random LPs in one batch (two persistent launches);
-* C3: the dispatch family at 200 generators x 24 periods, learned warm start;
+* C3: the dispatch family at 200 generators x 24 periods, learned warm start
+  (the benchmark's node model, trained on 50,000 instances), >= 1.5x fewer iterations;
 * C4: 256 node relaxations of the 1,000 x 10,000 set-covering MILP, parent start.
 """
 
@@ -16,6 +17,7 @@ import pytest
 
 from oracle import yardstick
 from sovereign_solver_amd import solve_lp_pdhg, solve_lp_pdhg_batch
+from sovereign_solver_amd.learned import nodes as N
 from sovereign_solver_amd.learned import warmstart as W
 from sovereign_solver_amd.workloads import DispatchFamily, SetCoverNodes, transportation_lp, uneven_batch
 from tests.parity import OBJ_TOL, assert_objective_parity, highs, rel_err
@@ -58,25 +60,29 @@ def test_c2_all_4096_uneven_lps(gpu):
         assert_objective_parity(probs[k], res[k].objective, name=f"uneven_{k}", x=res[k].x, y=res[k].y)
 
 
+@pytest.mark.timeout(600)
 def test_c3_dispatch_full_size_learned(gpu):
+    """The benchmark's learned configuration: 50,000 training instances labelled by
+    PDHG, a three-stage node model (3-layer width-512 MLP per stage), the primal
+    weight calibrated on 256 training instances; 1,024 held-out LPs solved cold and
+    from the network's starts.  The learned start takes at least 1.5x fewer
+    iterations on average."""
     fam = DispatchFamily()                               # 200 generators x 24 periods
     assert (fam.m, fam.n) == (9224, 4800)
-    train_ids, eval_ids = np.arange(4096), 10**6 + np.arange(256)
+    train_ids, eval_ids = np.arange(50_000), 10**6 + np.arange(1024)
     lab = W.label(fam, train_ids)
     assert np.all(lab.status == "optimal")
-    mlp, nin, nout, losses = W.train(fam.features_of(train_ids), W.targets_of(lab), d_h=512, epochs=10,
-                                     batch=256, lr=1e-3, out_blocks=W.output_blocks(fam))
-    assert losses[-1] < 0.5 * losses[0], losses
+    model = N.train_node_model(fam, lab, stages=3, steps=60_000, rows=32768, calib_ids=train_ids[-256:])
     cold, _ = W.solve_family(fam, eval_ids, TOL, 100_000)
-    warm, _ = W.solve_learned(fam, eval_ids, W.LearnedStart(mlp, nin, nout), TOL, 100_000, use_y=False)
+    warm, _ = N.solve_nodes(fam, eval_ids, model, TOL, 100_000)
     probs = [fam.problem(int(k)) for k in eval_ids]
     _kkt_all(probs, cold)
     _kkt_all(probs, warm)
     ci = np.array([r.iterations for r in cold])
     wi = np.array([r.iterations for r in warm])
-    print(f"C3: held-out iterations cold mean {ci.mean():.0f} max {ci.max()}, learned mean {wi.mean():.0f} "
-          f"max {wi.max()}")
-    assert wi.mean() < ci.mean()
+    print(f"C3: held-out iterations cold mean {ci.mean():.1f} max {ci.max()}, learned mean {wi.mean():.1f} "
+          f"max {wi.max()}: {ci.mean() / wi.mean():.3f}x")
+    assert ci.mean() >= 1.5 * wi.mean()
     for k in range(16):
         ref = highs(probs[k])
         assert_objective_parity(probs[k], warm[k].objective, ref=ref, x=warm[k].x, y=warm[k].y)
