@@ -461,6 +461,19 @@ def run_headline(args):
             ok += int((last["status"] == 1).sum())
         return ok, secs, last
 
+    # restart-check cadence by time to tolerance of the learned step itself (autotuner's search)
+    from sovereign_solver_amd.engine import tune as T
+    cadence = {"check_every": eng.check_every, "trials": {}}
+    if not args.no_autotune:
+        def measure(cfg):
+            eng.set_check_every(cfg["check_every"])
+            learned_resident()
+            o, s, r = timed(learned_resident, 1)
+            return {"seconds": s, "ok": o == len(probs), "iters_mean": float(r["iterations"].mean())}
+        best, trials = T.coordinate_search({"check_every": (25, 50, 100)}, {"check_every": 50}, measure)
+        eng.set_check_every(best["check_every"])
+        cadence = {"check_every": best["check_every"], "trials": trials}
+        log(f"check cadence: {best['check_every']} ({ {k: round(v['seconds'] * 1e3, 1) for k, v in trials.items()} } ms)")
     for _ in range(args.warmup):
         learned_resident()
     torch.cuda.synchronize()
@@ -531,7 +544,8 @@ def run_headline(args):
                         f"variables (second stage reads the first's outputs at t-2..t+2)"),
             "train_label_s": t_label, "train_s": t_train, "train_mfma": train_rate,
             "primal_weight": None if model.log_weight is None else float(np.exp(model.log_weight)),
-            "engine_setup_s": t_setup, "launch_shape": shape, "network_forward_mfma": net_rate,
+            "engine_setup_s": t_setup, "launch_shape": shape, "check_cadence": cadence,
+            "network_forward_mfma": net_rate,
             "learned_batched_resident": learned, "cold_batched_resident": cold,
             "host_to_host": h2h, "plain_single_gpu": plain,
             "speedup_vs_plain": value / plain["resident_lps_per_s"] if plain["resident_lps_per_s"] else None,
@@ -559,6 +573,13 @@ def run_transport(args, sources=1000, sinks=2000, cpu=True):
     eng = SingleEngine(form, "fp32")
     tune = T.autotune(eng) if not args.no_autotune else eng.tune
     eng.set_tune(tune)
+    cadence = None
+    if not args.no_autotune:                 # check cadence by time to tolerance (one solve per candidate)
+        tune = T.tune_cadence(eng, tol, mi, use_cache=False)
+        cadence = T._load_cache().get(T._bucket(eng.m, eng.n, eng.nnz, eng.precision), {}).get("cadence")
+        if cadence:
+            log(f"check cadence {tune.check_every}: " + ", ".join(
+                f"{k}: {v['seconds']:.3f}s/{v['iterations']} it" for k, v in cadence["trials"].items()))
     del eng
     torch.cuda.synchronize()
     # host to host: the box form, the engine (upload, K^T and SELL build), the solve, the extraction
@@ -602,7 +623,7 @@ def run_transport(args, sources=1000, sinks=2000, cpu=True):
         "iterations": results[-1]["iterations"], "restarts": results[-1]["restarts"],
         "rel_kkt": results[-1]["rel_kkt"], "objective": float(obj),
         "objective_rel_err_vs_fp64": None if obj_ref is None else abs(obj - obj_ref) / (1 + abs(obj_ref)),
-        "host_to_host": h2h, "tune": tune.__dict__,
+        "host_to_host": h2h, "tune": tune.__dict__, "cadence": cadence,
         "roofline": {"bound": "hbm", "kernel": "primal_kernel (fused K^T y + step + projection + sums)",
                      "achieved": gp, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gp / HBM_PEAK_GBS,
                      "traffic": measured_traffic(f"{key}/primal_kernel"), "bytes_per_launch": bp,
@@ -699,6 +720,14 @@ def run_batch(args, cpu=True):
            "upload_s": t2 - t1, "solve_and_results_s": t3 - t2}
     log(f"uneven batch host to host {h2h['s']:.2f}s (pack {t1 - t0:.2f}, upload {t2 - t1:.2f}, "
         f"solve + results {t3 - t2:.2f}) -> {h2h['lps_per_s']:.0f} LPs/s")
+    tuned = None
+    if not args.no_autotune:
+        from sovereign_solver_amd.engine import tune as T
+        t = time.perf_counter()
+        tuned = T.tune_batch(eng, tol, mi, use_cache=False)
+        tuned["s"] = time.perf_counter() - t
+        log(f"persistent engine tuned in {tuned['s']:.1f}s: {tuned['config']} "
+            f"(trials: { {k: round(v['seconds'], 3) for k, v in tuned['trials'].items()} })")
     steps = args.c2_steps
     times, outs = [], []
     for _ in range(steps):
@@ -752,10 +781,10 @@ def run_batch(args, cpu=True):
         "sum_m": pk.M, "sum_n": pk.N, "sum_nnz": pk.Z, "lps_fp32_part": int(len(probs) - pk.n_resident),
         "lps_fp64_part": int(pk.n_resident), "kernel_s_both_parts": times[-1][1],
         "iterations_mean": float(outs[-1]["iterations"].mean()), "iterations_max": int(outs[-1]["iterations"].max()),
-        "host_to_host": h2h, "plain_single_gpu": plain,
+        "host_to_host": h2h, "plain_single_gpu": plain, "tuned": tuned,
         "speedup_vs_plain": value / plain["resident_lps_per_s"] if plain["resident_lps_per_s"] else None,
         "objective_rel_err_vs_fp64": oe,
-        "roofline": {"bound": "hbm", "kernel": "batch_kernel<512, float> (FP32 part: one pass over K and one over "
+        "roofline": {"bound": "hbm", "kernel": f"batch_kernel<{eng.block}, float> (FP32 part: one pass over K and one over "
                                                "K^T per iteration, FP32 values and 16-bit indices)",
                      "achieved": g32, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": g32 / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": b32, "us_per_launch": t32 * 1e6,

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PDLP_ABI_VERSION 9
+#define PDLP_ABI_VERSION 10
 
 /* precision of iterates, matrix values and problem vectors */
 #define PDLP_F32 0
@@ -351,6 +351,8 @@ typedef struct PdlpBatch {
                                when it exceeds lds_bytes */
     int32_t lds_need_resident; /* FP64 part: the largest 20*n+12*m+8 of its LPs; PDLP_E_LIMIT
                                when it exceeds lds_resident */
+    int32_t lines;          /* FP32 part: lines (rows or columns) per lane group and round,
+                               whose loads are in flight together: 2, 3 or 4 (0 = 3) */
     double tol;
     const PdlpBatchLP* lps; /* count descriptors                              */
     const int32_t* order;   /* count LP indices, processing order: FP32 part, then FP64 part */

@@ -24,7 +24,7 @@
 //
 // A group of 2^lw lanes (lw per LP and pass, from the line lengths) owns a
 // line (row or column); each lane takes every 2^lw-th entry, the first EL of
-// them loaded at once.  Each group works on NL lines per round (three in the
+// them loaded at once.  Each group works on NL lines per round (2-4 in the
 // FP32 part, two in the FP64 part), so the loads of all are in flight
 // together.  Line pointers come from LDS and the operands of the step are
 // loaded with the entries, so a round waits for global memory once.
@@ -346,14 +346,14 @@ __device__ __forceinline__ Kkt kkt_of(const double (&s)[NS], const LPView& v) {
 
 // One LP from start to finish.  T = float: FP32 iterates, x in global memory,
 // or with XR x, its partial sum, tau_hat and c in LDS as well; T = double:
-// FP64 iterates, x in LDS.
-template <int NT, typename T, bool XR, bool SC>
+// FP64 iterates, x in LDS.  NLF: lines per group and round of the FP32 part.
+template <int NT, typename T, bool XR, bool SC, int NLF>
 __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds, double* red) {
     constexpr bool RES = std::is_same<T, double>::value;
     constexpr bool XIN = RES || XR;    // x in LDS
     static_assert(!(RES && XR), "the FP64 part keeps tau_hat and c in global memory");
     // lines per group and round: all their loads are in flight together (FP64: two, for registers)
-    constexpr int NL = RES ? 2 : 3;
+    constexpr int NL = RES ? 2 : NLF;
     const PdlpBatchLP lp = a.lps[k];
     const LPView v = view_of(a, lp);
     char* base = reinterpret_cast<char*>(lds);
@@ -664,7 +664,7 @@ __device__ __forceinline__ void solve_one(const PdlpBatch& a, int k, float* lds,
 // Persistent workgroups over one part of the work order: LPs order[first ..
 // first + count), taken through the counter *queue.  T = float: FP32 part,
 // T = double: FP64 part (each instantiation has its own register allocation).
-template <int NT, typename T, bool XR, bool SC>
+template <int NT, typename T, bool XR, bool SC, int NLF>
 __global__ void __launch_bounds__(NT) batch_kernel(PdlpBatch a, int first, int count, int* queue, int lds_limit) {
     extern __shared__ float lds[];
     __shared__ double red[NS * (NT / kWave)];
@@ -684,7 +684,7 @@ __global__ void __launch_bounds__(NT) batch_kernel(PdlpBatch a, int first, int c
         // an LP that does not fit the launch's LDS is not run and says so (the host
         // checked the part's largest need, so only inconsistent descriptors get here)
         if (NB * n + MB * m + 8 <= lds_limit) {
-            solve_one<NT, T, XR, SC>(a, k, lds, red);
+            solve_one<NT, T, XR, SC, NLF>(a, k, lds, red);
         } else if (threadIdx.x == 0) {
             PdlpBatchOut o = {};
             o.status = PDLP_SKIPPED;
@@ -694,13 +694,28 @@ __global__ void __launch_bounds__(NT) batch_kernel(PdlpBatch a, int first, int c
     }
 }
 
-template <int NT, typename T, bool XR, bool SC>
+template <int NT, typename T, bool XR, bool SC, int NLF>
 int launch(const PdlpBatch* a, int first, int count, int* queue, int lds, int grid, hipStream_t s) {
-    const hipError_t e = hipFuncSetAttribute((const void*)batch_kernel<NT, T, XR, SC>,
+    const hipError_t e = hipFuncSetAttribute((const void*)batch_kernel<NT, T, XR, SC, NLF>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
-    batch_kernel<NT, T, XR, SC><<<grid, NT, lds, s>>>(*a, first, count, queue, lds);
+    batch_kernel<NT, T, XR, SC, NLF><<<grid, NT, lds, s>>>(*a, first, count, queue, lds);
     return hip_fail(hipGetLastError(), "batch_kernel launch");
+}
+
+// The FP32 part's lines per round (2, 3, 4); the FP64 part always takes two.
+template <int NT, typename T, bool XR, bool SC>
+int launch_lines(const PdlpBatch* a, int first, int count, int* queue, int lds, int grid, hipStream_t s) {
+    if constexpr (std::is_same<T, double>::value) {
+        return launch<NT, T, XR, SC, 2>(a, first, count, queue, lds, grid, s);
+    } else {
+        switch (a->lines ? a->lines : 3) {
+            case 2: return launch<NT, T, XR, SC, 2>(a, first, count, queue, lds, grid, s);
+            case 3: return launch<NT, T, XR, SC, 3>(a, first, count, queue, lds, grid, s);
+            case 4: return launch<NT, T, XR, SC, 4>(a, first, count, queue, lds, grid, s);
+            default: return fail(PDLP_E_TUNE, "lines must be 2, 3 or 4");
+        }
+    }
 }
 
 template <typename T, bool XR>
@@ -711,10 +726,10 @@ int launch_part(const PdlpBatch* a, int first, int count, int* queue, int block,
     const int g = std::min(count, grid > 0 ? grid : 2 * cus);
     switch (block) {
         // scaled LPs (row_scale set) run the instantiation whose checks unscale
-        case 256: return a->row_scale ? launch<256, T, XR, true>(a, first, count, queue, lds, g, s)
-                                      : launch<256, T, XR, false>(a, first, count, queue, lds, g, s);
-        case 512: return a->row_scale ? launch<512, T, XR, true>(a, first, count, queue, lds, g, s)
-                                      : launch<512, T, XR, false>(a, first, count, queue, lds, g, s);
+        case 256: return a->row_scale ? launch_lines<256, T, XR, true>(a, first, count, queue, lds, g, s)
+                                      : launch_lines<256, T, XR, false>(a, first, count, queue, lds, g, s);
+        case 512: return a->row_scale ? launch_lines<512, T, XR, true>(a, first, count, queue, lds, g, s)
+                                      : launch_lines<512, T, XR, false>(a, first, count, queue, lds, g, s);
         default: return fail(PDLP_E_TUNE, "block must be 256 or 512");
     }
 }
@@ -763,6 +778,7 @@ extern "C" int pdlp_batch_solve(const PdlpBatch* a, void* stream) {
     if ((a->row_scale == nullptr) != (a->col_scale == nullptr))
         return fail(PDLP_E_ARG, "row_scale and col_scale must both be set or both be NULL");
     if (a->check_every <= 0 || a->max_iters < 0) return fail(PDLP_E_ARG, "check_every > 0 and max_iters >= 0");
+    if (a->lines != 0 && (a->lines < 2 || a->lines > 4)) return fail(PDLP_E_TUNE, "lines must be 2, 3 or 4 (0 = 3)");
     const int n32 = a->count - a->n_resident;
     const int parts = a->parts ? a->parts : 3;
     if ((parts & 1) && n32 > 0) {

@@ -13,7 +13,7 @@ import threading
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "_lib", "libpdlp_hip.so")
 
-PDLP_ABI_VERSION = 9
+PDLP_ABI_VERSION = 10
 PDLP_F32, PDLP_F64 = 0, 1
 PDLP_E_ARG, PDLP_E_TUNE, PDLP_E_SHAPE, PDLP_E_LIMIT = -1, -2, -3, -4
 PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL, PDLP_INFEASIBLE, PDLP_UNBOUNDED, PDLP_SKIPPED = 0, 1, 3, 4, 5, 6
@@ -73,7 +73,7 @@ class PdlpBatch(C.Structure):
         ("count", C.c_int32), ("check_every", C.c_int32), ("max_iters", C.c_int32), ("block", C.c_int32),
         ("lds_bytes", C.c_int32), ("grid", C.c_int32), ("n_resident", C.c_int32), ("block_resident", C.c_int32),
         ("lds_resident", C.c_int32), ("grid_resident", C.c_int32), ("parts", C.c_int32), ("x_in_lds", C.c_int32),
-        ("lds_need", C.c_int32), ("lds_need_resident", C.c_int32),
+        ("lds_need", C.c_int32), ("lds_need_resident", C.c_int32), ("lines", C.c_int32),
         ("tol", C.c_double),
         ("lps", C.c_void_p), ("order", C.c_void_p), ("queue", C.c_void_p),
         ("row_ptr", C.c_void_p), ("col_idx", C.c_void_p), ("val", C.c_void_p),

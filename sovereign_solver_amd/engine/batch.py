@@ -211,8 +211,15 @@ class PackedBatch:
         self.c, self.lo, self.hi, self.b, self.kind = c, lo, hi, b, kind
         self.lo_shift, self.ub_span, self.b_shift = lo_shift, ub_span, b_shift
         self.desc = desc
-        # FP32 part (largest first), then the FP64 part (largest first)
-        self.resident = lds_bytes_resident(ms, ns) <= RESIDENT_LDS
+        self.set_split(RESIDENT_LDS)
+
+    def set_split(self, resident_lds: int) -> None:
+        """The FP32/FP64 split: LPs whose FP64-part layout needs at most ``resident_lds``
+        bytes of LDS run with FP64 iterates.  Sets the work order (FP32 part largest
+        first, then the FP64 part largest first) and both parts' LDS sizes."""
+        ms, ns, nnzs = self.ms, self.ns, self.nnzs
+        self.split = int(resident_lds)
+        self.resident = lds_bytes_resident(ms, ns) <= self.split
         by_size = np.argsort(-nnzs, kind="stable")
         self.order = np.concatenate([by_size[~self.resident[by_size]], by_size[self.resident[by_size]]]).astype(np.int32)
         self.n_resident = int(self.resident.sum())
@@ -248,7 +255,7 @@ class BatchEngine:
     or "reference" restarts for every LP (see ``include/pdlp_hip.h``)."""
 
     def __init__(self, packed: PackedBatch, block: int = 0, grid: int = 0, scheme: str = "adaptive",
-                 block_resident: int = 256):
+                 block_resident: int = 256, check_every: int = 50, lines: int = 3):
         if scheme not in ("adaptive", "reference", "adaptive_fixed", "adaptive_start"):
             raise ValueError("scheme must be 'adaptive', 'adaptive_fixed', 'adaptive_start' or 'reference'")
         self.lib = L.lib()
@@ -261,6 +268,8 @@ class BatchEngine:
         self.block = block or 512
         self.block_resident = block_resident
         self.grid = grid
+        self.check_every = int(check_every)
+        self.lines = int(lines)
         self.stream = torch.cuda.Stream(device=self.device)
         dv = self.device
         f32, f64, i32 = torch.float32, torch.float64, torch.int32
@@ -292,6 +301,36 @@ class BatchEngine:
                 self.t["row_scale"] = dev_tensor(pk.row_scale, f64, dv)
                 self.t["col_scale"] = dev_tensor(pk.col_scale, f64, dv)
         self.stream.synchronize()
+
+    def configure(self, check_every: Optional[int] = None, block: Optional[int] = None,
+                  lines: Optional[int] = None, split: Optional[int] = None,
+                  block_resident: Optional[int] = None) -> None:
+        """Launch configuration (None keeps a value): check cadence, threads per
+        workgroup of the FP32 part (256, 512), its lines per lane group and round (2-4),
+        the FP32/FP64 split (:meth:`PackedBatch.set_split`) and the FP64 part's
+        threads per workgroup (256, 512).  None of these changes an LP's arithmetic
+        except the cadence (when the checks run) and the split (its iterate precision)."""
+        if check_every is not None:
+            if int(check_every) <= 0:
+                raise ValueError("check_every must be positive")
+            self.check_every = int(check_every)
+        if block is not None:
+            if int(block) not in (256, 512):
+                raise ValueError("block must be 256 or 512")
+            self.block = int(block)
+        if lines is not None:
+            if int(lines) not in (2, 3, 4):
+                raise ValueError("lines must be 2, 3 or 4")
+            self.lines = int(lines)
+        if block_resident is not None:
+            if int(block_resident) not in (256, 512):
+                raise ValueError("block_resident must be 256 or 512")
+            self.block_resident = int(block_resident)
+        if split is not None and int(split) != self.pk.split:
+            self.pk.set_split(int(split))
+            with torch.cuda.stream(self.stream):
+                self.t["order"].copy_(torch.from_numpy(self.pk.order).to(self.device))
+            self.stream.synchronize()
 
     def set_warm_start(self, x0: Optional[np.ndarray] = None, y0: Optional[np.ndarray] = None,
                        step_scale: Optional[np.ndarray] = None) -> None:
@@ -328,7 +367,7 @@ class BatchEngine:
                       lds_bytes=self.pk.lds_bytes, grid=self.grid, n_resident=self.pk.n_resident,
                       block_resident=self.block_resident, lds_resident=self.pk.lds_resident, grid_resident=0,
                       x_in_lds=int(self.pk.x_in_lds), lds_need=self.pk.lds_need,
-                      lds_need_resident=self.pk.lds_need_resident, tol=tol)
+                      lds_need_resident=self.pk.lds_need_resident, lines=self.lines, tol=tol)
         for name in ("lps", "order", "queue", "row_ptr", "col_idx", "val", "t_col_ptr", "t_row_idx", "t_val",
                      "c", "lo", "hi", "c64", "lo_shift",
                      "ub_span", "b", "row_kind", "b64", "b_shift", "x", "x_psum", "x_sum", "tau", "y", "y_sum",
@@ -338,10 +377,11 @@ class BatchEngine:
             a.row_scale, a.col_scale = ptr(t["row_scale"]), ptr(t["col_scale"])
         return a
 
-    def launch(self, tol: float, max_iters: int, check_every: int = 50, parts: int = 0) -> None:
+    def launch(self, tol: float, max_iters: int, check_every: Optional[int] = None, parts: int = 0) -> None:
         """Enqueue one batched solve on the engine stream (inputs already in place).
-        ``parts``: 1 = only the FP32 part, 2 = only the FP64 part, 0 = both."""
-        self._a = self._args(tol, max_iters, check_every)
+        ``parts``: 1 = only the FP32 part, 2 = only the FP64 part, 0 = both;
+        ``check_every`` None = the engine's cadence."""
+        self._a = self._args(tol, max_iters, self.check_every if check_every is None else check_every)
         self._a.parts = parts
         with torch.cuda.stream(self.stream):
             if parts in (0, 3):
@@ -359,7 +399,8 @@ class BatchEngine:
             xs, ys = xs * self.pk.col_scale, ys * self.pk.row_scale
         return out, xs, ys
 
-    def solve(self, tol: float, max_iters: int, check_every: int = 50, x0=None, y0=None, step_scale=None):
+    def solve(self, tol: float, max_iters: int, check_every: Optional[int] = None, x0=None, y0=None,
+              step_scale=None):
         """Cold start (x = y = 0) unless a packed warm start is given."""
         self.set_warm_start(x0, y0, step_scale)
         self.launch(tol, max_iters, check_every)

@@ -692,6 +692,14 @@ class FamilyEngine:
         return [(_STATUS[int(out["status"][k])], X[k], Y[k], out[k]) for k in range(self.fb.count)]
 
     # ---------------------------------------------------------------- tuning
+    def set_check_every(self, ce: int) -> None:
+        """Iterations between restart checks (the captured graphs are rebuilt)."""
+        if int(ce) <= 0:
+            raise ValueError("check_every must be positive")
+        self.check_every = int(ce)
+        self.args.check_every = int(ce)
+        self.drop_graphs()
+
     def autotune(self, tol: float = 1e-4, reps: int = 10) -> dict:
         """Pick rows / columns per wavefront (1, 2) and XCD affinity (off, on) by
         timing the column and row passes (HIP graphs, events) at the cold-start

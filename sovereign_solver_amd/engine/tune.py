@@ -1,6 +1,6 @@
-"""Launch-shape selection for the PDHG kernels.
+"""Launch-shape and check-cadence selection for the PDHG engines.
 
-Two layers:
+Layers:
 
 * :func:`heuristic_tune` — lanes per row from the mean row length of K (dual
   step) and Kᵀ (primal step): the largest power of two not above the mean,
@@ -10,6 +10,21 @@ Two layers:
   heuristic on the actual problem with HIP events on the engine's stream and
   keeps the fastest; results are cached per (arch, shape bucket) in a JSON
   file (``SOVEREIGN_PDLP_TUNE_CACHE`` or ``~/.cache/sovereign_pdlp/tune.json``).
+* :func:`tune_cadence` — the restart-check cadence of the single-instance
+  engine by measured time to tolerance: the LP is solved once per candidate
+  cadence (a check every 25, 50, 100 or 200 iterations) and the fastest
+  solve that ends optimal wins.  The cadence trades the check's cost (two
+  FP64 passes over K per candidate point) against how late a solve notices
+  convergence and how often the restart rule looks, so only a whole solve
+  measures it.
+* :func:`tune_family` and :func:`tune_batch` — the same for the
+  shared-matrix family engine (cadence) and the persistent batch engine
+  (cadence, workgroup size of each part, lines per lane group and round of
+  the FP32 part, and the FP32/FP64 split), by a coordinate search over
+  whole batched solves.
+
+Every choice is cached per shape bucket (:func:`_bucket`, :func:`batch_bucket`),
+with the timings that decided it.
 """
 
 from __future__ import annotations
@@ -18,10 +33,13 @@ import json
 import math
 import os
 import threading
+import time
 from dataclasses import asdict, dataclass
+from typing import Callable, Optional
 
 ARCH = "gfx950"
 TPR_CHOICES = (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024)
+CADENCES = (25, 50, 100, 200)          # iterations between restart checks
 
 
 @dataclass(frozen=True)
@@ -89,9 +107,13 @@ def _load_cache() -> dict:
 
 
 def _store_cache(key: str, t: Tune, timings: dict) -> None:
+    _store_entry(key, {"tune": asdict(t), "timings_us": timings})
+
+
+def _store_entry(key: str, entry: dict) -> None:
     with _cache_lock:
         data = _load_cache()
-        data[key] = {"tune": asdict(t), "timings_us": timings}
+        data[key] = entry
         path = _cache_path()
         try:
             os.makedirs(os.path.dirname(path), exist_ok=True)
@@ -103,12 +125,15 @@ def _store_cache(key: str, t: Tune, timings: dict) -> None:
             pass
 
 
-def cached_tune(m: int, n: int, nnz: int, prec: str, check_every: int = 50):
+def cached_tune(m: int, n: int, nnz: int, prec: str, check_every: Optional[int] = 50):
+    """The cached launch shape of the bucket; ``check_every=None`` keeps the cached
+    cadence (the one :func:`tune_cadence` chose, when it ran), else the given one."""
     rec = _load_cache().get(_bucket(m, n, nnz, prec))
     if not rec:
         return None
     d = dict(rec["tune"])
-    d["check_every"] = check_every
+    if check_every is not None:
+        d["check_every"] = check_every
     return Tune(**d)
 
 
@@ -162,3 +187,153 @@ def autotune(engine, reps: int = 20, use_cache: bool = True) -> Tune:
     t = Tune(best_p[0], best_d[0], 256, base.check_every, best_p[1] | (best_d[1] << 1), 0, best_d[2], 0, sell)
     _store_cache(key, t, timings)
     return t.with_(scheme=engine.tune.scheme)
+
+
+# ------------------------------------------------------------------ cadence
+def _fastest(trials: dict) -> Optional[object]:
+    """Key of the fastest trial that ended with every LP optimal (None if none did)."""
+    ok = {k: v for k, v in trials.items() if v["ok"]}
+    return min(ok, key=lambda k: ok[k]["seconds"]) if ok else None
+
+
+def tune_cadence(engine, tol: float, max_iters: int, candidates=CADENCES, use_cache: bool = True,
+                 x0=None, y0=None, step_scale=None) -> Tune:
+    """Check cadence of a :class:`~.single.SingleEngine` by time to tolerance.
+
+    Solves the engine's LP once per candidate cadence (from the same start) and
+    keeps the fastest solve that ends ``optimal``; ties and failures keep the
+    current cadence.  The choice and the timings are stored with the bucket's
+    launch shape.  Returns the tuning with the chosen cadence (also set on the
+    engine)."""
+    key = _bucket(engine.m, engine.n, engine.nnz, engine.precision)
+    rec = _load_cache().get(key) if use_cache else None
+    if rec and "cadence" in rec:
+        t = engine.tune.with_(check_every=int(rec["cadence"]["check_every"]))
+        engine.set_tune(t)
+        return t
+    base = engine.tune
+    trials = {}
+    for ce in candidates:
+        engine.set_tune(base.with_(check_every=int(ce)))
+        engine.stream.synchronize()
+        t0 = time.perf_counter()
+        r = engine.solve(tol, max_iters, x0, y0, step_scale=step_scale)
+        engine.stream.synchronize()
+        trials[int(ce)] = {"seconds": time.perf_counter() - t0, "ok": r["status"] == "optimal",
+                           "iterations": int(r["iterations"])}
+    best = _fastest(trials)
+    ce = best if best is not None else base.check_every
+    t = base.with_(check_every=int(ce))
+    engine.set_tune(t)
+    entry = dict(rec or _load_cache().get(key) or {"tune": asdict(base), "timings_us": {}})
+    entry["cadence"] = {"check_every": int(ce), "tol": tol,
+                        "trials": {str(k): v for k, v in trials.items()}}
+    _store_entry(key, entry)
+    return t
+
+
+def coordinate_search(space: dict, start: dict, measure: Callable[[dict], dict]):
+    """One pass of coordinate descent: for each parameter in ``space`` order, try
+    each of its values with the others at the best so far, keep the fastest
+    trial whose LPs all ended optimal.  ``measure(config)`` returns
+    ``{"seconds": s, "ok": bool, ...}``.  Returns (best config, trials)."""
+    best = dict(start)
+    trials = {}
+
+    def key(cfg):
+        return ",".join(f"{k}={cfg[k]}" for k in sorted(cfg))
+
+    best_rec = None
+    for name, values in space.items():
+        for v in values:
+            cfg = dict(best)
+            cfg[name] = v
+            kk = key(cfg)
+            if kk not in trials:
+                trials[kk] = measure(cfg)
+            rec = trials[kk]
+            if rec["ok"] and (best_rec is None or rec["seconds"] < best_rec["seconds"]):
+                best, best_rec = cfg, rec
+    return best, trials
+
+
+def _lg(v) -> int:
+    return int(round(math.log2(max(float(v), 1.0))))
+
+
+def batch_bucket(kind: str, count: int, ms, nnzs, extra: str = "") -> str:
+    """Shape bucket of a batch: engine kind, log2 of the LP count, of the median and
+    largest row count and of the total entries."""
+    import numpy as np
+    ms = np.asarray(ms)
+    return (f"{ARCH}/{kind}/B{_lg(count)}/m{_lg(np.median(ms))}/M{_lg(ms.max())}/z{_lg(np.sum(nnzs))}"
+            + (f"/{extra}" if extra else ""))
+
+
+def tune_family(engine, tol: float, max_iters: int, X=None, Y=None, w=None, candidates=(25, 50, 100),
+                use_cache: bool = True) -> dict:
+    """Check cadence of a :class:`~.family.FamilyEngine` by time to tolerance of the
+    whole batch (from the given starts); sets it on the engine and returns the
+    cached record ``{"check_every": ..., "trials": ...}``."""
+    fb = engine.fb
+    key = batch_bucket("family", fb.count, [fb.m], [fb.K.nnz], f"n{_lg(fb.n)}")
+    rec = _load_cache().get(key) if use_cache else None
+    if rec is None:
+        def measure(cfg):
+            engine.set_check_every(cfg["check_every"])
+            engine.stream.synchronize()
+            t0 = time.perf_counter()
+            out, _, _ = engine.solve(tol, max_iters, X, Y, w)
+            return {"seconds": time.perf_counter() - t0, "ok": bool((out["status"] == 1).all()),
+                    "mean_iterations": float(out["iterations"].mean())}
+        best, trials = coordinate_search({"check_every": tuple(candidates)},
+                                         {"check_every": engine.check_every}, measure)
+        rec = {"check_every": int(best["check_every"]), "tol": tol, "trials": trials}
+        _store_entry(key, rec)
+    engine.set_check_every(int(rec["check_every"]))
+    return rec
+
+
+BATCH_SPACE = {
+    "check_every": (25, 50, 100),
+    "block": (256, 512),                # FP32 part: threads per workgroup
+    "lines": (2, 3, 4),                 # FP32 part: lines per lane group and round
+    "split": (8 * 1024, 16 * 1024, 32 * 1024),   # FP64 part: LPs needing at most this much LDS
+    "block_resident": (256, 512),       # FP64 part: threads per workgroup
+}
+BATCH_DEFAULT = {"check_every": 50, "block": 512, "lines": 3, "split": 16 * 1024, "block_resident": 256}
+
+
+def tune_batch(engine, tol: float, max_iters: int, x0=None, y0=None, w=None, space=None,
+               use_cache: bool = True) -> dict:
+    """Launch configuration of a :class:`~.batch.BatchEngine` by time to tolerance of
+    the whole batch: a coordinate search over ``space`` (default
+    :data:`BATCH_SPACE`) from :data:`BATCH_DEFAULT`, each trial one batched solve
+    from the given start, timed with HIP events on the engine stream.  The best
+    configuration is set on the engine; returns ``{"config": ..., "trials": ...}``."""
+    pk = engine.pk
+    key = batch_bucket("persistent", len(pk.problems), pk.ms, pk.nnzs,
+                       "scaled" if pk.row_scale is not None else "")
+    rec = _load_cache().get(key) if use_cache else None
+    if rec is None:
+        import torch
+
+        engine.set_warm_start(x0, y0, w)              # kernels loaded before the first timed trial
+        engine.launch(tol, min(max_iters, 1), engine.check_every)
+        engine.stream.synchronize()
+
+        def measure(cfg):
+            engine.configure(**cfg)
+            engine.set_warm_start(x0, y0, w)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(engine.stream)
+            engine.launch(tol, max_iters, engine.check_every)
+            e1.record(engine.stream)
+            out, _, _ = engine.results()
+            return {"seconds": e0.elapsed_time(e1) * 1e-3, "ok": bool((out["status"] == 1).all()),
+                    "max_iterations": int(out["iterations"].max())}
+        best, trials = coordinate_search(space or BATCH_SPACE, BATCH_DEFAULT, measure)
+        rec = {"config": best, "tol": tol, "trials": trials}
+        _store_entry(key, rec)
+    engine.configure(**rec["config"])
+    return rec

@@ -21,6 +21,15 @@ LDS; else persistent).
 :func:`~sovereign_solver_amd.lp.pdhg.solve_lp_pdhg`); the tolerance applies to
 the original LPs' KKT and the results come back unscaled.
 
+``tune=True``: the launch configuration comes from the autotuner
+(:mod:`~sovereign_solver_amd.engine.tune`): for the persistent engine the
+check cadence, each part's workgroup size, the FP32 part's lines per round and
+the FP32/FP64 split (:func:`~sovereign_solver_amd.engine.tune.tune_batch`);
+for the family engine the launch shape and the check cadence
+(:func:`~sovereign_solver_amd.engine.tune.tune_family`).  A shape bucket seen
+before takes its cached choice; a new one is tuned by solving this batch once
+per candidate (from the given warm start).
+
 ``warm_start`` is an optional list (one entry per problem, ``None`` allowed)
 of ``(x0, y0)`` or ``(x0, y0, step_scale)``: x0 in the original variable
 space, y0 the duals of ``[A_ub; A_eq]`` (y ≤ 0 on the ``<=`` rows), and a
@@ -34,6 +43,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 from ..config import MAX_PDHG_ITERS
+from ..engine import tune as T
 from ..engine.batch import BatchEngine, PackedBatch, lds_bytes_of
 from ..engine.family import FamilyBatch, FamilyEngine
 from ..problem import Problem
@@ -109,6 +119,7 @@ def solve_lp_pdhg_batch(
     scheme: str = "adaptive",
     engine: str = "auto",
     scaling: str = "none",
+    tune: bool = False,
 ) -> list:
     if engine not in ("auto", "persistent", "family"):
         raise ValueError("engine must be 'auto', 'persistent' or 'family'")
@@ -130,19 +141,23 @@ def solve_lp_pdhg_batch(
     if live and engine == "family":
         fb = FamilyBatch([problems[k] for k in live])
         eng = FamilyEngine(fb, scheme=scheme, check_every=check_every)
-        if fb.count >= 256:                      # launch shape by timing; a few ms per engine
+        if fb.count >= 256 or tune:              # launch shape by timing; a few ms per engine
             eng.autotune(tol)
         ws = None if warm_start is None else [warm_start[k] for k in live]
         X, Y, w = family_warm_start(fb, ws)
+        if tune:
+            T.tune_family(eng, tol, max_iters, X, Y, w)
         o, xs, ys = eng.solve(tol, max_iters, X, Y, w)
         for k, r in zip(live, results_of(eng, o, xs, ys)):
             out[k] = r
     elif live:
         pk = PackedBatch([problems[k] for k in live], scaling=scaling)
-        eng = BatchEngine(pk, block=block, scheme=scheme)
+        eng = BatchEngine(pk, block=block, scheme=scheme, check_every=check_every)
         ws = None if warm_start is None else [warm_start[k] for k in live]
         x0, y0, w = pack_warm_start(pk, ws)
-        o, xs, ys = eng.solve(tol, max_iters, check_every, x0, y0, w)
+        if tune:
+            T.tune_batch(eng, tol, max_iters, x0, y0, w)
+        o, xs, ys = eng.solve(tol, max_iters, None, x0, y0, w)
         for k, r in zip(live, results_of(eng, o, xs, ys)):
             out[k] = r
     return out

@@ -38,7 +38,11 @@ Keyword-only extensions (defaults keep the reference's contract):
     estimate for the adaptive one).
 ``check_every``, ``tune``
     KKT/restart cadence (50 = reference) and launch shape (default: the
-    autotuner's heuristic).
+    autotuner's heuristic).  ``tune="auto"``: launch shape and check cadence
+    from the autotuner (:func:`~sovereign_solver_amd.engine.tune.autotune`
+    times the kernels; :func:`~sovereign_solver_amd.engine.tune.tune_cadence`
+    solves this LP once per candidate cadence and keeps the fastest optimal
+    solve), cached per shape bucket, so the cost is paid once per bucket.
 ``scaling``
     Diagonal preconditioning of the box form (``"none"`` default,
     ``"ruiz"``, ``"geometric"``, ``"pock_chambolle"``, ``"ruiz_pc"``; the
@@ -91,7 +95,8 @@ import numpy as np
 from ..config import MAX_PDHG_ITERS, PDHG_TOL
 from ..engine.forms import extract, make_form
 from ..engine.single import SingleEngine
-from ..engine.tune import Tune, heuristic_tune
+from ..engine.tune import Tune, heuristic_tune, tune_cadence
+from ..engine.tune import autotune as autotune_shape
 from ..problem import Problem
 from ..result import SolveResult
 
@@ -181,7 +186,7 @@ def solve_lp_pdhg(
     step_scale: Optional[float] = None,
     scheme: str = "auto",
     check_every: int = 50,
-    tune: Optional[Tune] = None,
+    tune=None,
     use_graphs: bool = True,
     scaling: str = "none",
 ) -> SolveResult:
@@ -190,15 +195,21 @@ def solve_lp_pdhg(
         return SolveResult("optimal", np.zeros(problem.n_vars), 0.0, 0)
     prec = _precision(precision, tol)
     sch = _scheme(scheme, formulation, exact)
+    auto = isinstance(tune, str)
+    if auto and tune != "auto":
+        raise ValueError("tune must be a Tune, None or 'auto'")
     if exact:
         tune = Tune(1, 1, 256, check_every, 0, 1)
-    elif tune is None:
+    elif tune is None or auto:
         tune = heuristic_tune(form.m, form.n, form.nnz, check_every)
     eng = SingleEngine(form, prec, tune=tune, check_every=check_every, exact=exact, use_graphs=use_graphs,
                        scheme=sch)
     d = None
     try:
         xs, ys = _warm_start(form, problem, x0, y0)
+        if auto and not exact:
+            eng.set_tune(autotune_shape(eng))
+            tune_cadence(eng, tol, max_iters, x0=xs, y0=ys, step_scale=step_scale)
         r = eng.solve(tol, max_iters, xs, ys, step_scale=step_scale)
         if r["status"] == "unbounded":
             d = _ray_in_original_space(form, problem, r["ray"])
