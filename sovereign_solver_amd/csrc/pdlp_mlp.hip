@@ -355,6 +355,206 @@ __global__ void __launch_bounds__(256) cast_kernel(const float* x, uint16_t* y, 
         y[i] = f2bf(x[i]);
 }
 
+// --------------------------------------------------------- fused inference
+// The whole 3-layer forward pass of FI_ROWS input rows in one workgroup, for
+// d_h = 512: out = relu(relu(x W1 + b1) W2 + b2) W3 + b3 with the hidden
+// activations rounded to BF16 as the layer-by-layer path rounds them, but
+// never written to memory.  Each of the 8 wavefronts owns 64 hidden units
+// and all FI_ROWS rows; every product is computed transposed (units x rows),
+// D = W^T X^T, so that:
+//   layer 1: A = W1^T (global, [unit][k]), B = x rows (global); the result,
+//            after bias and ReLU, goes to LDS as H1[row][unit] in BF16;
+//   layer 2: A = W2^T (global, [unit][k]: each lane reads 64 contiguous bytes
+//            of its unit's row per group of 64 k), B = H1 rows from LDS; a
+//            64-k group is four MFMA steps whose k slots map to
+//            k = kg + 32 h + 8 s + t (half h of the wavefront, step s, element
+//            t), the same for both operands;
+//   layer 3: B = the layer-2 accumulators themselves after bias, ReLU and
+//            BF16 rounding: accumulator r of a 32x32 block holds unit
+//            8 (r/4) + 4 h + r % 4 of the lane's row, so accumulators 8q..8q+7
+//            are the B operand of one MFMA whose k slot (h, t) is unit
+//            16 q + 8 (t/4) + 4 h + t % 4; A = W3 gathered into that order
+//            (w3p, built by pdlp_mlp_infer_pack).
+// The wavefronts' layer-3 partial sums (over their 64 units) meet in LDS (in
+// the space of H1) and are added in wavefront order, so the result does not
+// depend on scheduling.
+constexpr int FI_ROWS = 128, FI_DH = 512, FI_WAVES = 8, FI_THREADS = FI_WAVES * 64;
+constexpr int FI_LDH = FI_DH + 8;               // H1 row stride (BF16): a 16-byte pad per row
+constexpr int FI_MAX_DIN = 128, FI_MAX_DOUT = 32;
+constexpr size_t FI_LDS = (size_t)FI_ROWS * FI_LDH * 2;   // >= FI_WAVES * 32 * FI_ROWS floats
+
+__device__ __forceinline__ u16x8 ld16(const uint16_t* p) { return *reinterpret_cast<const u16x8*>(p); }
+
+__device__ __forceinline__ f32x16 mfma(const u16x8& a, const u16x8& b, const f32x16& c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+}
+
+__global__ void __launch_bounds__(FI_THREADS) mlp_infer_kernel(PdlpMlpInfer p, int k1) {
+    extern __shared__ __align__(16) uint16_t fi_lds[];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 31, lh = lane >> 5;
+    const int r0 = blockIdx.x * FI_ROWS;
+    const int u0 = w * 64;
+    f32x16 acc[2][4];
+#pragma unroll
+    for (int ub = 0; ub < 2; ++ub)
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[ub][rb][r] = 0.f;
+
+    // ---- layer 1: D[unit][row] = sum_k W1^T[unit][k] x[row][k]
+    for (int k0 = 0; k0 < k1; k0 += 16) {
+        const int kk = k0 + 8 * lh;
+        u16x8 a[2], b[4];
+#pragma unroll
+        for (int ub = 0; ub < 2; ++ub) a[ub] = ld16(p.w1t + (size_t)(u0 + 32 * ub + li) * k1 + kk);
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            const int row = r0 + 32 * rb + li;
+            if (row < p.rows && kk < p.d_in) {
+                b[rb] = ld16(p.x + (size_t)row * p.d_in + kk);
+            } else {
+#pragma unroll
+                for (int t = 0; t < 8; ++t) b[rb][t] = 0;
+            }
+        }
+#pragma unroll
+        for (int ub = 0; ub < 2; ++ub)
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) acc[ub][rb] = mfma(a[ub], b[rb], acc[ub][rb]);
+    }
+    // bias, ReLU, BF16 -> H1[row][unit]: accumulators 4g..4g+3 are 4 consecutive units
+#pragma unroll
+    for (int ub = 0; ub < 2; ++ub)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int unit = u0 + 32 * ub + 8 * g + 4 * lh;
+            const float4 bias = *reinterpret_cast<const float4*>(p.b1 + unit);
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) {
+                const float v0 = fmaxf(acc[ub][rb][4 * g + 0] + bias.x, 0.f);
+                const float v1 = fmaxf(acc[ub][rb][4 * g + 1] + bias.y, 0.f);
+                const float v2 = fmaxf(acc[ub][rb][4 * g + 2] + bias.z, 0.f);
+                const float v3 = fmaxf(acc[ub][rb][4 * g + 3] + bias.w, 0.f);
+                uint2 pk;
+                pk.x = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
+                pk.y = (uint32_t)f2bf(v2) | ((uint32_t)f2bf(v3) << 16);
+                *reinterpret_cast<uint2*>(fi_lds + (size_t)(32 * rb + li) * FI_LDH + unit) = pk;
+            }
+        }
+    __syncthreads();
+
+    // ---- layer 2: D[unit][row] = sum_k W2^T[unit][k] H1[row][k], k in groups of 64
+#pragma unroll
+    for (int ub = 0; ub < 2; ++ub)
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[ub][rb][r] = 0.f;
+    const uint16_t* wrow0 = p.w2t + (size_t)(u0 + li) * FI_DH + 32 * lh;
+    const uint16_t* wrow1 = wrow0 + (size_t)32 * FI_DH;
+    const uint16_t* hrow = fi_lds + (size_t)li * FI_LDH + 32 * lh;
+    u16x8 an[2][4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        an[0][s] = ld16(wrow0 + 8 * s);
+        an[1][s] = ld16(wrow1 + 8 * s);
+    }
+    for (int kg = 0; kg < FI_DH; kg += 64) {
+        u16x8 a[2][4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            a[0][s] = an[0][s];
+            a[1][s] = an[1][s];
+        }
+        if (kg + 64 < FI_DH) {                      // the next group's A in flight during this group
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                an[0][s] = ld16(wrow0 + kg + 64 + 8 * s);
+                an[1][s] = ld16(wrow1 + kg + 64 + 8 * s);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            u16x8 b[4];
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) b[rb] = *reinterpret_cast<const u16x8*>(hrow + (size_t)32 * rb * FI_LDH + kg + 8 * s);
+#pragma unroll
+            for (int ub = 0; ub < 2; ++ub)
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb) acc[ub][rb] = mfma(a[ub][s], b[rb], acc[ub][rb]);
+        }
+    }
+
+    // ---- layer 3 from the accumulators: D3[channel][row] over this wavefront's units
+    f32x16 acc3[4];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc3[rb][r] = 0.f;
+#pragma unroll
+    for (int ub = 0; ub < 2; ++ub)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int blk = (u0 >> 4) + 2 * ub + q;
+            const u16x8 a3 = ld16(p.w3p + ((size_t)(blk * 32 + li) * 2 + lh) * 8);
+            const int ubase = u0 + 32 * ub + 16 * q + 4 * lh;
+            const float4 bl = *reinterpret_cast<const float4*>(p.b2 + ubase);
+            const float4 bh = *reinterpret_cast<const float4*>(p.b2 + ubase + 8);
+            const float bias[8] = {bl.x, bl.y, bl.z, bl.w, bh.x, bh.y, bh.z, bh.w};
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) {
+                u16x8 hv;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) hv[t] = f2bf(fmaxf(acc[ub][rb][8 * q + t] + bias[t], 0.f));
+                acc3[rb] = mfma(a3, hv, acc3[rb]);
+            }
+        }
+    __syncthreads();                                // every wavefront is done with H1
+    float* red = reinterpret_cast<float*>(fi_lds);  // [wave][channel 32][row]
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int ch = 8 * (r >> 2) + 4 * lh + (r & 3);
+            red[((size_t)w * 32 + ch) * FI_ROWS + 32 * rb + li] = acc3[rb][r];
+        }
+    __syncthreads();
+    for (int e = threadIdx.x; e < FI_ROWS * p.d_out; e += FI_THREADS) {
+        const int row = e / p.d_out, ch = e - row * p.d_out;
+        float s = 0.f;
+#pragma unroll
+        for (int v = 0; v < FI_WAVES; ++v) s += red[((size_t)v * 32 + ch) * FI_ROWS + row];
+        if (r0 + row < p.rows) p.out[(size_t)(r0 + row) * p.d_out + ch] = s + p.b3[ch];
+    }
+}
+
+// w1t[unit][k] = W1[k][unit] (zero for k >= d_in, k < k1), w2t[unit][k] = W2[k][unit],
+// w3p in the layer-3 operand order of mlp_infer_kernel (zero for channels >= d_out).
+__global__ void __launch_bounds__(256) mlp_infer_pack_kernel(const uint16_t* w1, const uint16_t* w2,
+                                                             const uint16_t* w3, int d_in, int k1, int d_out,
+                                                             uint16_t* w1t, uint16_t* w2t, uint16_t* w3p) {
+    const int64_t n1 = (int64_t)FI_DH * k1, n2 = (int64_t)FI_DH * FI_DH, n3 = (int64_t)(FI_DH / 16) * 32 * 16;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n1 + n2 + n3;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        if (e < n1) {
+            const int unit = (int)(e / k1), k = (int)(e % k1);
+            w1t[e] = k < d_in ? w1[(size_t)k * FI_DH + unit] : (uint16_t)0;
+        } else if (e < n1 + n2) {
+            const int64_t f = e - n1;
+            const int unit = (int)(f / FI_DH), k = (int)(f % FI_DH);
+            w2t[f] = w2[(size_t)k * FI_DH + unit];
+        } else {
+            const int64_t f = e - n1 - n2;                  // ((blk * 32 + ch) * 2 + h) * 8 + t
+            const int t = (int)(f & 7), h = (int)((f >> 3) & 1), ch = (int)((f >> 4) & 31), blk = (int)(f >> 9);
+            const int unit = 16 * blk + 8 * (t >> 2) + 4 * h + (t & 3);
+            w3p[f] = ch < d_out ? w3[(size_t)unit * d_out + ch] : (uint16_t)0;
+        }
+    }
+}
+
 template <bool VEC, bool TA, bool TB>
 void launch_gemm(const PdlpGemm& g, dim3 grid, int kc, hipStream_t s) {
     gemm_kernel<VEC, TA, TB><<<grid, GEMM_THREADS, 0, s>>>(g, kc);
@@ -510,6 +710,42 @@ int pdlp_adam_step(float* w, const float* g, float* m, float* v, uint16_t* wb, i
     const float c2 = 1.f / (1.f - powf(beta2, (float)step));
     adam_kernel<<<grid1(n), 256, 0, as_stream(stream)>>>(w, g, m, v, wb, n, lr, beta1, beta2, eps, c1, c2);
     return hip_fail(hipGetLastError(), "adam_kernel launch");
+}
+
+int pdlp_mlp_infer_pack(const uint16_t* w1, const uint16_t* w2, const uint16_t* w3, int32_t d_in, int32_t d_h,
+                        int32_t d_out, uint16_t* w1t, uint16_t* w2t, uint16_t* w3p, void* stream) {
+    if (d_h != FI_DH) return fail(PDLP_E_SHAPE, "mlp infer: d_h must be 512");
+    if (d_in <= 0 || d_in > FI_MAX_DIN || d_in % 8 != 0) return fail(PDLP_E_SHAPE, "mlp infer: d_in in 8..128, multiple of 8");
+    if (d_out <= 0 || d_out > FI_MAX_DOUT) return fail(PDLP_E_SHAPE, "mlp infer: d_out in 1..32");
+    if (!w1 || !w2 || !w3 || !w1t || !w2t || !w3p) return fail(PDLP_E_ARG, "mlp infer pack: NULL pointer");
+    const int k1 = (d_in + 15) / 16 * 16;
+    const int64_t total = (int64_t)FI_DH * k1 + (int64_t)FI_DH * FI_DH + (int64_t)(FI_DH / 16) * 32 * 16;
+    mlp_infer_pack_kernel<<<grid1(total), 256, 0, as_stream(stream)>>>(w1, w2, w3, d_in, k1, d_out, w1t, w2t, w3p);
+    return hip_fail(hipGetLastError(), "mlp_infer_pack_kernel launch");
+}
+
+int pdlp_mlp_infer_bf16(const PdlpMlpInfer* p, void* stream) {
+    if (p == nullptr) return fail(PDLP_E_ARG, "PdlpMlpInfer is NULL");
+    if (p->d_h != FI_DH) return fail(PDLP_E_SHAPE, "mlp infer: d_h must be 512");
+    if (p->d_in <= 0 || p->d_in > FI_MAX_DIN || p->d_in % 8 != 0)
+        return fail(PDLP_E_SHAPE, "mlp infer: d_in in 8..128, multiple of 8");
+    if (p->d_out <= 0 || p->d_out > FI_MAX_DOUT) return fail(PDLP_E_SHAPE, "mlp infer: d_out in 1..32");
+    if (p->rows < 0) return fail(PDLP_E_SHAPE, "mlp infer: rows >= 0");
+    if (p->rows == 0) return 0;
+    if (!p->x || !p->w1t || !p->w2t || !p->w3p || !p->b1 || !p->b2 || !p->b3 || !p->out)
+        return fail(PDLP_E_ARG, "mlp infer: NULL pointer");
+    // 16-byte operand loads: x rows (d_in a multiple of 8) and the packed weights
+    if ((reinterpret_cast<uintptr_t>(p->x) | reinterpret_cast<uintptr_t>(p->w1t) |
+         reinterpret_cast<uintptr_t>(p->w2t) | reinterpret_cast<uintptr_t>(p->w3p) |
+         reinterpret_cast<uintptr_t>(p->b1) | reinterpret_cast<uintptr_t>(p->b2)) & 15)
+        return fail(PDLP_E_ARG, "mlp infer: x, the packed weights, b1 and b2 must be 16-byte aligned");
+    const hipError_t e = hipFuncSetAttribute((const void*)mlp_infer_kernel,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)FI_LDS);
+    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
+    const int k1 = (p->d_in + 15) / 16 * 16;
+    const unsigned grid = (unsigned)((p->rows + FI_ROWS - 1) / FI_ROWS);
+    mlp_infer_kernel<<<grid, FI_THREADS, FI_LDS, as_stream(stream)>>>(*p, k1);
+    return hip_fail(hipGetLastError(), "mlp_infer_kernel launch");
 }
 
 int pdlp_cast_bf16(const float* x, uint16_t* y, int64_t n, void* stream) {

@@ -137,6 +137,13 @@ class PdlpMlp(C.Structure):
     ]
 
 
+class PdlpMlpInfer(C.Structure):
+    _fields_ = [
+        ("rows", C.c_int32), ("d_in", C.c_int32), ("d_h", C.c_int32), ("d_out", C.c_int32),
+        ("x", _p), ("w1t", _p), ("w2t", _p), ("w3p", _p), ("b1", _p), ("b2", _p), ("b3", _p), ("out", _p),
+    ]
+
+
 class PdlpNodeGather(C.Structure):
     _fields_ = [
         ("batch", C.c_int32), ("nodes", C.c_int32), ("width", C.c_int32), ("ld_out", C.c_int32),
@@ -196,6 +203,8 @@ _SIGS = {
     "pdlp_gemm_bf16": (C.c_int, [C.POINTER(PdlpGemm), _p]),
     "pdlp_mlp_fwd_bf16": (C.c_int, [C.POINTER(PdlpMlp), _p]),
     "pdlp_mlp_bwd_bf16": (C.c_int, [C.POINTER(PdlpMlp), _p]),
+    "pdlp_mlp_infer_pack": (C.c_int, [_p, _p, _p, C.c_int32, C.c_int32, C.c_int32, _p, _p, _p, _p]),
+    "pdlp_mlp_infer_bf16": (C.c_int, [C.POINTER(PdlpMlpInfer), _p]),
     "pdlp_adam_step": (C.c_int, [_p, _p, _p, _p, _p, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float,
                                  C.c_int32, _p]),
     "pdlp_cast_bf16": (C.c_int, [_p, _p, C.c_int64, _p]),

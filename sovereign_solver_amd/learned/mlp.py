@@ -53,6 +53,7 @@ class WarmStartMLP:
             self.loss = torch.zeros(1, dtype=torch.float32, device=dv)
         self.step = 0
         self._bufs = {}
+        self._packed = None          # fused-inference weights, packed from the current BF16 copy
         self.refresh_bf16()
 
     # ---------------------------------------------------------------- helpers
@@ -67,6 +68,7 @@ class WarmStartMLP:
     def refresh_bf16(self):
         with torch.cuda.stream(self.stream):
             L.check(self.lib.pdlp_cast_bf16(self.w.data_ptr(), self.wb.data_ptr(), self.n_params, self.sh), "cast")
+        self._packed = None
 
     def params_numpy(self) -> dict:
         self.stream.synchronize()
@@ -143,6 +145,49 @@ class WarmStartMLP:
             L.check(self.lib.pdlp_mlp_fwd_bf16(C.byref(d), self.sh), "mlp fwd")
         return self._buffers(xb.shape[0])["out"]
 
+    # fused inference (pdlp_mlp_infer_bf16) limits
+    INFER_D_H, INFER_MAX_D_IN, INFER_MAX_D_OUT = 512, 128, 32
+
+    @property
+    def can_infer_fused(self) -> bool:
+        return (self.d_h == self.INFER_D_H and self.d_in <= self.INFER_MAX_D_IN and self.d_in % 8 == 0
+                and self.d_out <= self.INFER_MAX_D_OUT)
+
+    def _pack_infer(self):
+        """W1^T, W2^T and the layer-3 operand order of the BF16 weights (once per update)."""
+        if self._packed is None:
+            k1 = -(-self.d_in // 16) * 16
+            with torch.cuda.stream(self.stream):
+                pk = dict(w1t=torch.empty(self.d_h * k1, dtype=torch.bfloat16, device=self.device),
+                          w2t=torch.empty(self.d_h * self.d_h, dtype=torch.bfloat16, device=self.device),
+                          w3p=torch.empty(32 * 32 * 16, dtype=torch.bfloat16, device=self.device))
+                L.check(self.lib.pdlp_mlp_infer_pack(self._p("W1", self.wb), self._p("W2", self.wb),
+                                                     self._p("W3", self.wb), self.d_in, self.d_h, self.d_out,
+                                                     pk["w1t"].data_ptr(), pk["w2t"].data_ptr(),
+                                                     pk["w3p"].data_ptr(), self.sh), "mlp infer pack")
+            self._packed = pk
+        return self._packed
+
+    def infer(self, xb: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """[B, d_out] FP32 output of the fused inference kernel (the three layers in one
+        launch; the same BF16 rounding points as :meth:`forward`, no activations kept).
+        Falls back to :meth:`forward` for shapes outside the fused kernel's limits."""
+        if xb.dtype != torch.bfloat16 or xb.dim() != 2 or xb.shape[1] != self.d_in or not xb.is_contiguous():
+            raise ValueError("input must be a contiguous [B, d_in] bfloat16 device tensor")
+        if not self.can_infer_fused:
+            return self.forward(xb)
+        B = xb.shape[0]
+        pk = self._pack_infer()
+        with torch.cuda.stream(self.stream):
+            if out is None:
+                out = torch.empty(B, self.d_out, dtype=torch.float32, device=self.device)
+            d = L.PdlpMlpInfer(rows=B, d_in=self.d_in, d_h=self.d_h, d_out=self.d_out, x=xb.data_ptr(),
+                               w1t=pk["w1t"].data_ptr(), w2t=pk["w2t"].data_ptr(), w3p=pk["w3p"].data_ptr(),
+                               b1=self._p("b1", self.w), b2=self._p("b2", self.w), b3=self._p("b3", self.w),
+                               out=out.data_ptr())
+            L.check(self.lib.pdlp_mlp_infer_bf16(C.byref(d), self.sh), "mlp infer")
+        return out
+
     def loss_and_grad(self, xb: torch.Tensor, target: torch.Tensor, out_weight=None) -> torch.Tensor:
         d = self._desc(xb, target, out_weight)
         with torch.cuda.stream(self.stream):
@@ -156,6 +201,7 @@ class WarmStartMLP:
             L.check(self.lib.pdlp_adam_step(self.w.data_ptr(), self.g.data_ptr(), self.m.data_ptr(),
                                             self.v.data_ptr(), self.wb.data_ptr(), self.n_params, lr, beta1, beta2,
                                             eps, self.step, self.sh), "adam")
+        self._packed = None
 
     def save(self, path: str, extra: Optional[dict] = None):
         """Weights (and extra arrays, e.g. target normalisation) as safetensors."""

@@ -198,7 +198,7 @@ class NodeModel:
         with torch.cuda.stream(self.mlp.stream):
             xb = torch.empty(rows, self.d_in, dtype=torch.bfloat16, device=self.device)
         self._gather_inputs(Vc, xb, P=P)
-        return self.mlp.forward(xb)
+        return self.mlp.infer(xb)
 
     def node_outputs(self, Vd: torch.Tensor, chunk_rows: int = 1 << 20) -> torch.Tensor:
         """Normalised outputs of every node of every instance, [B, nodes * d_out] FP32

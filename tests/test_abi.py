@@ -52,6 +52,21 @@ def test_struct_sizes_match_header():
     # PdlpNodeGather: 6 int32, src2_len, count, seed, 10 pointers; PdlpNodeAssembly: 6 int32, 7 pointers
     assert C.sizeof(L.PdlpNodeGather) == 6 * 4 + 3 * 8 + 10 * 8
     assert C.sizeof(L.PdlpNodeAssembly) == 6 * 4 + 7 * 8
+    # PdlpMlpInfer: 4 int32, 8 pointers
+    assert C.sizeof(L.PdlpMlpInfer) == 4 * 4 + 8 * 8
+
+
+def test_fused_inference_rejects_shapes_without_device():
+    lib = L.lib()
+    d = L.PdlpMlpInfer(rows=4, d_in=48, d_h=256, d_out=8)                # d_h must be 512
+    assert lib.pdlp_mlp_infer_bf16(C.byref(d), None) == L.PDLP_E_SHAPE
+    for d_in, d_out in ((44, 8), (136, 8), (48, 33), (48, 0)):
+        d = L.PdlpMlpInfer(rows=4, d_in=d_in, d_h=512, d_out=d_out)
+        assert lib.pdlp_mlp_infer_bf16(C.byref(d), None) == L.PDLP_E_SHAPE
+    d = L.PdlpMlpInfer(rows=4, d_in=48, d_h=512, d_out=8)                 # NULL pointers
+    assert lib.pdlp_mlp_infer_bf16(C.byref(d), None) == L.PDLP_E_ARG
+    assert lib.pdlp_mlp_infer_pack(None, None, None, 48, 512, 8, None, None, None, None) == L.PDLP_E_ARG
+    assert lib.pdlp_mlp_infer_pack(None, None, None, 48, 500, 8, None, None, None, None) == L.PDLP_E_SHAPE
 
 
 def test_node_kernels_reject_bad_descriptors_without_device():

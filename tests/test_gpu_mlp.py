@@ -167,3 +167,86 @@ def test_training_reduces_loss(gpu):
     from sovereign_solver_amd.learned.warmstart import train
     _, _, _, losses = train(X, T, d_h=64, epochs=15, batch=256, lr=3e-3, step_weight=1.0)
     assert losses[-1] < 0.25 * losses[0], losses
+
+
+def _integer_net(rng, d_in, d_out, d_h=512):
+    """Small-integer inputs, weights and biases: every product and partial sum of the
+    three layers is an integer below 2^24 (exact in FP32 in any order) and the hidden
+    activations' BF16 rounding is the same for any summation order, so the fused
+    kernel must give the oracle's values bit for bit."""
+    P = {"W1": rng.integers(-1, 2, (d_in, d_h)) * (rng.random((d_in, d_h)) < 0.5),
+         "b1": rng.integers(-3, 4, d_h),
+         "W2": rng.integers(-1, 2, (d_h, d_h)) * (rng.random((d_h, d_h)) < 0.1),
+         "b2": rng.integers(-3, 4, d_h),
+         "W3": rng.integers(-1, 2, (d_h, d_out)) * (rng.random((d_h, d_out)) < 0.5),
+         "b3": rng.integers(-3, 4, d_out)}
+    return {k: v.astype(np.float64) for k, v in P.items()}
+
+
+@pytest.mark.parametrize("rows,d_in,d_out", [(128, 48, 8), (300, 88, 8), (77, 8, 1), (1000, 128, 32), (129, 40, 5)])
+def test_fused_inference_exact_on_integers(gpu, rows, d_in, d_out):
+    """pdlp_mlp_infer_bf16 against the FP64 oracle with the kernels' BF16 rounding points,
+    bit for bit on integer data: pins the packed operand orders (W1^T, W2^T, the layer-3
+    gather), the permuted k slots of layer 2, the LDS row layout, the wavefront
+    reduction and the row tail (rows not a multiple of 128)."""
+    rng = np.random.default_rng(rows * 31 + d_in + d_out)
+    mlp = WarmStartMLP(d_in, 512, d_out, seed=3)
+    P = _integer_net(rng, d_in, d_out)
+    mlp.set_params(P)
+    x = rng.integers(-2, 3, (rows, d_in)).astype(np.float64)
+    out = mlp.infer(_bf(x, gpu))
+    mlp.stream.synchronize()
+    _, _, want = mlp_ref.forward(x, **P, rnd=mlp_ref.bf16_round)
+    assert np.array_equal(out.cpu().numpy().astype(np.float64), want)
+
+
+def test_fused_inference_matches_layer_by_layer(gpu):
+    """Random data: the fused kernel and the three-GEMM forward round the hidden
+    activations at the same points; they differ only where an FP32 summation-order
+    difference moves a value across a BF16 rounding boundary (one BF16 ulp, 2^-8
+    relative, of that activation)."""
+    rng = np.random.default_rng(11)
+    d_in, d_out, rows = 88, 8, 4096
+    mlp = WarmStartMLP(d_in, 512, d_out, seed=5)
+    P = mlp.params_numpy()
+    P["b1"] = rng.standard_normal(512) * 0.1
+    P["b2"] = rng.standard_normal(512) * 0.1
+    P["b3"] = rng.standard_normal(d_out) * 0.1
+    mlp.set_params(P)
+    x = rng.standard_normal((rows, d_in))
+    xb = _bf(mlp_ref.bf16_round(x), gpu)
+    fused_t = mlp.infer(xb)
+    mlp.stream.synchronize()                 # outputs live on the network's stream
+    fused = fused_t.cpu().numpy().astype(np.float64)
+    layered_t = mlp.forward(xb)
+    mlp.stream.synchronize()
+    layered = layered_t.cpu().numpy().astype(np.float64)
+    P32 = {k: v.astype(np.float32).astype(np.float64) for k, v in mlp.params_numpy().items()}
+    h1, h2, want = mlp_ref.forward(x, **P32, rnd=mlp_ref.bf16_round)
+    W2b, W3b = mlp_ref.bf16_round(P32["W2"]), mlp_ref.bf16_round(P32["W3"])
+    # every hidden activation off by at most one BF16 ulp, propagated (ReLU is 1-Lipschitz), plus FP32 sums
+    dh1 = 2.0**-8 * np.abs(h1)
+    dh2 = 2.0**-8 * np.abs(h2) + (dh1 @ np.abs(W2b)) * (1 + 2.0**-8)
+    bound = dh2 @ np.abs(W3b) + 600 * 2.0**-24 * (np.abs(h2) @ np.abs(W3b) + np.abs(P32["b3"])) + 1e-6
+    for got in (fused, layered):
+        assert np.all(np.abs(got - want) <= bound), np.max(np.abs(got - want) / bound)
+    # and in aggregate the two GPU paths agree far inside that bound
+    assert np.linalg.norm(fused - layered) <= 1e-3 * np.linalg.norm(layered)
+
+
+def test_fused_inference_refreshes_after_weight_update(gpu):
+    rng = np.random.default_rng(2)
+    mlp = WarmStartMLP(16, 512, 4, seed=1)
+    P = _integer_net(rng, 16, 4)
+    mlp.set_params(P)
+    x = rng.integers(-2, 3, (64, 16)).astype(np.float64)
+    xb = _bf(x, gpu)
+    a_t = mlp.infer(xb)
+    mlp.stream.synchronize()
+    a = a_t.cpu().numpy()
+    P2 = dict(P, W3=-P["W3"], b3=-P["b3"])
+    mlp.set_params(P2)                       # the packed weights must follow
+    b_t = mlp.infer(xb)
+    mlp.stream.synchronize()
+    b = b_t.cpu().numpy()
+    assert np.array_equal(a, -b) and np.any(a != 0)
