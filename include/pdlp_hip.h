@@ -612,21 +612,22 @@ int pdlp_mlp_bwd_bf16(const PdlpMlp* mlp, void* stream);
  * rows per workgroup in one kernel, the hidden activations (rounded to BF16
  * at the same points as pdlp_mlp_fwd_bf16) kept in LDS and registers instead
  * of HBM.  The weights are first packed by pdlp_mlp_infer_pack (once per
- * weight update): w1t = W1^T [512][k1] with k1 = d_in rounded up to 16 (zero
- * columns beyond d_in), w2t = W2^T [512][512], w3p = W3 in the kernel's
- * layer-3 operand order, 32 * 32 * 16 values (zero for channels >= d_out).
+ * weight update) into the kernel's MFMA operand-fragment order, so that every
+ * operand load reads whole cache lines: w1p holds 512 * k1 values with k1 =
+ * d_in rounded up to 16 (zeros beyond d_in), w2p 512 * 512, w3p 32 * 32 * 16
+ * (zeros for channels >= d_out).
  * Replaces the layer-by-layer forward of the reference's cold start
  * (solver/lp/pdhg.py:175-176 starts from zeros; this predicts the start).
  * Limits: d_in a multiple of 8 up to 128, d_out up to 32 (PDLP_E_SHAPE). */
 typedef struct PdlpMlpInfer {
     int32_t rows, d_in, d_h, d_out;
     const uint16_t* x;                 /* rows x d_in BF16, 16-byte aligned        */
-    const uint16_t *w1t, *w2t, *w3p;   /* packed BF16 weights, 16-byte aligned     */
+    const uint16_t *w1p, *w2p, *w3p;   /* packed BF16 weights, 16-byte aligned     */
     const float *b1, *b2, *b3;         /* FP32 biases (b1, b2 16-byte aligned)     */
     float* out;                        /* rows x d_out FP32                         */
 } PdlpMlpInfer;
 int pdlp_mlp_infer_pack(const uint16_t* w1, const uint16_t* w2, const uint16_t* w3, int32_t d_in, int32_t d_h,
-                        int32_t d_out, uint16_t* w1t, uint16_t* w2t, uint16_t* w3p, void* stream);
+                        int32_t d_out, uint16_t* w1p, uint16_t* w2p, uint16_t* w3p, void* stream);
 int pdlp_mlp_infer_bf16(const PdlpMlpInfer* p, void* stream);
 /* Adam step on FP32 master weights w (n values) with gradient g, moments m, v;
  * refreshes the BF16 copy wb (may be NULL).  step is 1-based. */

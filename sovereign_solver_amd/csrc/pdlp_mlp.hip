@@ -362,90 +362,131 @@ __global__ void __launch_bounds__(256) cast_kernel(const float* x, uint16_t* y, 
 // never written to memory.  Each of the 8 wavefronts owns 64 hidden units
 // and all FI_ROWS rows; every product is computed transposed (units x rows),
 // D = W^T X^T, so that:
-//   layer 1: A = W1^T (global, [unit][k]), B = x rows (global); the result,
-//            after bias and ReLU, goes to LDS as H1[row][unit] in BF16;
-//   layer 2: A = W2^T (global, [unit][k]: each lane reads 64 contiguous bytes
-//            of its unit's row per group of 64 k), B = H1 rows from LDS; a
-//            64-k group is four MFMA steps whose k slots map to
-//            k = kg + 32 h + 8 s + t (half h of the wavefront, step s, element
-//            t), the same for both operands;
+//   layer 1: A = W1^T (global), B = x rows (global); the result, after bias
+//            and ReLU, goes to LDS as H1[row][unit] in BF16;
+//   layer 2: A = W2^T (global), B = H1 rows from LDS; a 64-k group is four
+//            MFMA steps whose k slots map to k = kg + 32 h + 8 s + t (half h
+//            of the wavefront, step s, element t), the same for both operands;
 //   layer 3: B = the layer-2 accumulators themselves after bias, ReLU and
 //            BF16 rounding: accumulator r of a 32x32 block holds unit
 //            8 (r/4) + 4 h + r % 4 of the lane's row, so accumulators 8q..8q+7
 //            are the B operand of one MFMA whose k slot (h, t) is unit
 //            16 q + 8 (t/4) + 4 h + t % 4; A = W3 gathered into that order
-//            (w3p, built by pdlp_mlp_infer_pack).
+//            (w3p).
+// The weights come packed by pdlp_mlp_infer_pack in MFMA fragment order (one
+// fragment = the 64 lanes' 16-byte operands, 1 KB of consecutive memory), so
+// each A load instruction reads 8 whole cache lines instead of touching 32.
 // The wavefronts' layer-3 partial sums (over their 64 units) meet in LDS (in
 // the space of H1) and are added in wavefront order, so the result does not
 // depend on scheduling.
 constexpr int FI_ROWS = 128, FI_DH = 512, FI_WAVES = 8, FI_THREADS = FI_WAVES * 64;
 constexpr int FI_LDH = FI_DH + 8;               // H1 row stride (BF16): a 16-byte pad per row
 constexpr int FI_MAX_DIN = 128, FI_MAX_DOUT = 32;
-constexpr size_t FI_LDS = (size_t)FI_ROWS * FI_LDH * 2;   // >= FI_WAVES * 32 * FI_ROWS floats
+constexpr size_t FI_H1 = (size_t)FI_ROWS * FI_LDH * 2;    // H1; later >= FI_WAVES * 32 * FI_ROWS floats
+constexpr size_t FI_LDS = FI_H1 + 2 * FI_DH * 4;            // + b1 and b2 (FP32)
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ u16x8 ld16(const uint16_t* p) { return *reinterpret_cast<const u16x8*>(p); }
+
+// FP32 -> BF16, round to nearest even (the hardware conversion of gfx950; finite
+// values as f2bf)
+__device__ __forceinline__ uint16_t bf16_rne(float v) { return __builtin_bit_cast(uint16_t, (__bf16)v); }
 
 __device__ __forceinline__ f32x16 mfma(const u16x8& a, const u16x8& b, const f32x16& c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
                                                    0, 0, 0);
 }
 
-__global__ void __launch_bounds__(FI_THREADS) mlp_infer_kernel(PdlpMlpInfer p, int k1) {
+// NK: layer-1 MFMA steps (k1 = 16 NK).  Layer 1 runs one block of 32 rows at a
+// time with every W1^T operand loaded once up front and the next block's x
+// rows in flight during the current block's MFMAs; each block goes to LDS
+// (bias, ReLU, BF16) as soon as it is done, so only its two accumulator blocks
+// are live.  The first layer-2 operand group is issued before layer 1.
+template <int NK>
+__global__ void __launch_bounds__(FI_THREADS) mlp_infer_kernel(PdlpMlpInfer p) {
+    constexpr int k1 = 16 * NK;
     extern __shared__ __align__(16) uint16_t fi_lds[];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int li = lane & 31, lh = lane >> 5;
     const int r0 = blockIdx.x * FI_ROWS;
     const int u0 = w * 64;
-    f32x16 acc[2][4];
+    float* bl1 = reinterpret_cast<float*>(reinterpret_cast<char*>(fi_lds) + FI_H1);   // b1 then b2 in LDS
+    float* bl2 = bl1 + FI_DH;
+    bl1[threadIdx.x] = p.b1[threadIdx.x];
+    bl2[threadIdx.x] = p.b2[threadIdx.x];
+    __syncthreads();
+
+    // The packed weights are in fragment order: the 64 lanes' operands of one MFMA
+    // are 1 KB of consecutive memory, so every load instruction reads whole lines.
+    // W2: fragment (wave, k group, unit block, step) at ((w * 8 + kg/64) * 2 + ub) * 4 + s.
+    const uint16_t* w2w = p.w2p + (size_t)w * (FI_DH / 64) * 2 * 4 * 512 + lane * 8;
+    // layer 2's first A group (no dependence on layer 1): in flight during layer 1
+    u16x8 an[2][4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        an[0][s] = ld16(w2w + (size_t)(0 * 4 + s) * 512);
+        an[1][s] = ld16(w2w + (size_t)(1 * 4 + s) * 512);
+    }
+
+    // ---- layer 1: D[unit][row] = sum_k W1^T[unit][k] x[row][k], one row block at a time
+    // W1: fragment (wave, unit block, step) at (w * 2 + ub) * NK + s
+    u16x8 a1[2][NK], bx[NK], bn[NK];
 #pragma unroll
     for (int ub = 0; ub < 2; ++ub)
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
+        for (int s = 0; s < NK; ++s) a1[ub][s] = ld16(p.w1p + ((size_t)((w * 2 + ub) * NK + s) * 64 + lane) * 8);
+    // branch-free: every lane loads from a valid address (row and k clamped) and
+    // zeroes what lies beyond the rows or d_in, so the loads stay in flight together
+    auto load_x = [&](int rb, u16x8 (&dst)[NK]) {
+        const int row = r0 + 32 * rb + li;
+        const uint16_t* xr = p.x + (size_t)min(row, p.rows - 1) * p.d_in;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) acc[ub][rb][r] = 0.f;
-
-    // ---- layer 1: D[unit][row] = sum_k W1^T[unit][k] x[row][k]
-    for (int k0 = 0; k0 < k1; k0 += 16) {
-        const int kk = k0 + 8 * lh;
-        u16x8 a[2], b[4];
+        for (int s = 0; s < NK; ++s) {
+            const int kk = 16 * s + 8 * lh;
+            const u16x8 v = ld16(xr + min(kk, p.d_in - 8));
+            const bool ok = row < p.rows && kk < p.d_in;
 #pragma unroll
-        for (int ub = 0; ub < 2; ++ub) a[ub] = ld16(p.w1t + (size_t)(u0 + 32 * ub + li) * k1 + kk);
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb) {
-            const int row = r0 + 32 * rb + li;
-            if (row < p.rows && kk < p.d_in) {
-                b[rb] = ld16(p.x + (size_t)row * p.d_in + kk);
-            } else {
-#pragma unroll
-                for (int t = 0; t < 8; ++t) b[rb][t] = 0;
-            }
+            for (int t = 0; t < 8; ++t) dst[s][t] = ok ? v[t] : (unsigned short)0;
         }
+    };
+    load_x(0, bx);
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+        if (rb < 3) load_x(rb + 1, bn);
+        f32x16 c[2];
 #pragma unroll
         for (int ub = 0; ub < 2; ++ub)
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb) acc[ub][rb] = mfma(a[ub], b[rb], acc[ub][rb]);
-    }
-    // bias, ReLU, BF16 -> H1[row][unit]: accumulators 4g..4g+3 are 4 consecutive units
+            for (int r = 0; r < 16; ++r) c[ub][r] = 0.f;
 #pragma unroll
-    for (int ub = 0; ub < 2; ++ub)
+        for (int s = 0; s < NK; ++s)
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const int unit = u0 + 32 * ub + 8 * g + 4 * lh;
-            const float4 bias = *reinterpret_cast<const float4*>(p.b1 + unit);
+            for (int ub = 0; ub < 2; ++ub) c[ub] = mfma(a1[ub][s], bx[s], c[ub]);
+        // bias, ReLU, BF16 -> H1[row][unit]: accumulators 4g..4g+3 are 4 consecutive units
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb) {
-                const float v0 = fmaxf(acc[ub][rb][4 * g + 0] + bias.x, 0.f);
-                const float v1 = fmaxf(acc[ub][rb][4 * g + 1] + bias.y, 0.f);
-                const float v2 = fmaxf(acc[ub][rb][4 * g + 2] + bias.z, 0.f);
-                const float v3 = fmaxf(acc[ub][rb][4 * g + 3] + bias.w, 0.f);
+        for (int ub = 0; ub < 2; ++ub)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int unit = u0 + 32 * ub + 8 * g + 4 * lh;
+                const f32x4 bias = *reinterpret_cast<const f32x4*>(bl1 + unit);
+                const float v0 = fmaxf(c[ub][4 * g + 0] + bias[0], 0.f);
+                const float v1 = fmaxf(c[ub][4 * g + 1] + bias[1], 0.f);
+                const float v2 = fmaxf(c[ub][4 * g + 2] + bias[2], 0.f);
+                const float v3 = fmaxf(c[ub][4 * g + 3] + bias[3], 0.f);
                 uint2 pk;
-                pk.x = (uint32_t)f2bf(v0) | ((uint32_t)f2bf(v1) << 16);
-                pk.y = (uint32_t)f2bf(v2) | ((uint32_t)f2bf(v3) << 16);
+                pk.x = (uint32_t)bf16_rne(v0) | ((uint32_t)bf16_rne(v1) << 16);
+                pk.y = (uint32_t)bf16_rne(v2) | ((uint32_t)bf16_rne(v3) << 16);
                 *reinterpret_cast<uint2*>(fi_lds + (size_t)(32 * rb + li) * FI_LDH + unit) = pk;
             }
+        if (rb < 3) {
+#pragma unroll
+            for (int s = 0; s < NK; ++s) bx[s] = bn[s];
         }
+    }
     __syncthreads();
 
+    f32x16 acc[2][4];
     // ---- layer 2: D[unit][row] = sum_k W2^T[unit][k] H1[row][k], k in groups of 64
 #pragma unroll
     for (int ub = 0; ub < 2; ++ub)
@@ -453,39 +494,39 @@ __global__ void __launch_bounds__(FI_THREADS) mlp_infer_kernel(PdlpMlpInfer p, i
         for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[ub][rb][r] = 0.f;
-    const uint16_t* wrow0 = p.w2t + (size_t)(u0 + li) * FI_DH + 32 * lh;
-    const uint16_t* wrow1 = wrow0 + (size_t)32 * FI_DH;
     const uint16_t* hrow = fi_lds + (size_t)li * FI_LDH + 32 * lh;
-    u16x8 an[2][4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        an[0][s] = ld16(wrow0 + 8 * s);
-        an[1][s] = ld16(wrow1 + 8 * s);
-    }
-    for (int kg = 0; kg < FI_DH; kg += 64) {
-        u16x8 a[2][4];
+    // two register sets of A in turn: the loads of one group are issued a whole group
+    // (32 MFMAs) before its MFMAs; the loop is kept rolled so the schedule keeps them there
+    auto group_load = [&](int gk, u16x8 (&dst)[2][4]) {
+        const uint16_t* g = w2w + (size_t)gk * 2 * 4 * 512;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            a[0][s] = an[0][s];
-            a[1][s] = an[1][s];
+            dst[0][s] = ld16(g + (size_t)(0 * 4 + s) * 512);
+            dst[1][s] = ld16(g + (size_t)(1 * 4 + s) * 512);
         }
-        if (kg + 64 < FI_DH) {                      // the next group's A in flight during this group
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                an[0][s] = ld16(wrow0 + kg + 64 + 8 * s);
-                an[1][s] = ld16(wrow1 + kg + 64 + 8 * s);
-            }
-        }
+    };
+    auto group_mfma = [&](int kg, const u16x8 (&a)[2][4]) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             u16x8 b[4];
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb) b[rb] = *reinterpret_cast<const u16x8*>(hrow + (size_t)32 * rb * FI_LDH + kg + 8 * s);
+            for (int rb = 0; rb < 4; ++rb)
+                b[rb] = *reinterpret_cast<const u16x8*>(hrow + (size_t)32 * rb * FI_LDH + kg + 8 * s);
 #pragma unroll
             for (int ub = 0; ub < 2; ++ub)
 #pragma unroll
                 for (int rb = 0; rb < 4; ++rb) acc[ub][rb] = mfma(a[ub][s], b[rb], acc[ub][rb]);
         }
+    };
+    u16x8 ab[2][4];
+#pragma unroll 1
+    for (int kg = 0; kg < FI_DH; kg += 128) {
+        group_load(kg / 64 + 1, ab);
+        __builtin_amdgcn_sched_barrier(0);          // the loads stay ahead of the group's MFMAs
+        group_mfma(kg, an);
+        if (kg + 128 < FI_DH) group_load(kg / 64 + 2, an);
+        __builtin_amdgcn_sched_barrier(0);
+        group_mfma(kg + 64, ab);
     }
 
     // ---- layer 3 from the accumulators: D3[channel][row] over this wavefront's units
@@ -501,14 +542,14 @@ __global__ void __launch_bounds__(FI_THREADS) mlp_infer_kernel(PdlpMlpInfer p, i
             const int blk = (u0 >> 4) + 2 * ub + q;
             const u16x8 a3 = ld16(p.w3p + ((size_t)(blk * 32 + li) * 2 + lh) * 8);
             const int ubase = u0 + 32 * ub + 16 * q + 4 * lh;
-            const float4 bl = *reinterpret_cast<const float4*>(p.b2 + ubase);
-            const float4 bh = *reinterpret_cast<const float4*>(p.b2 + ubase + 8);
-            const float bias[8] = {bl.x, bl.y, bl.z, bl.w, bh.x, bh.y, bh.z, bh.w};
+            const f32x4 bl = *reinterpret_cast<const f32x4*>(bl2 + ubase);
+            const f32x4 bh = *reinterpret_cast<const f32x4*>(bl2 + ubase + 8);
+            const float bias[8] = {bl[0], bl[1], bl[2], bl[3], bh[0], bh[1], bh[2], bh[3]};
 #pragma unroll
             for (int rb = 0; rb < 4; ++rb) {
                 u16x8 hv;
 #pragma unroll
-                for (int t = 0; t < 8; ++t) hv[t] = f2bf(fmaxf(acc[ub][rb][8 * q + t] + bias[t], 0.f));
+                for (int t = 0; t < 8; ++t) hv[t] = bf16_rne(fmaxf(acc[ub][rb][8 * q + t] + bias[t], 0.f));
                 acc3[rb] = mfma(a3, hv, acc3[rb]);
             }
         }
@@ -531,21 +572,38 @@ __global__ void __launch_bounds__(FI_THREADS) mlp_infer_kernel(PdlpMlpInfer p, i
     }
 }
 
-// w1t[unit][k] = W1[k][unit] (zero for k >= d_in, k < k1), w2t[unit][k] = W2[k][unit],
-// w3p in the layer-3 operand order of mlp_infer_kernel (zero for channels >= d_out).
+template <int NK>
+int launch_infer(const PdlpMlpInfer& p, unsigned grid, hipStream_t s) {
+    const hipError_t e = hipFuncSetAttribute((const void*)mlp_infer_kernel<NK>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)FI_LDS);
+    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
+    mlp_infer_kernel<NK><<<grid, FI_THREADS, FI_LDS, s>>>(p);
+    return hip_fail(hipGetLastError(), "mlp_infer_kernel launch");
+}
+
+// The weights in the fragment order of mlp_infer_kernel: element t of lane l's
+// A operand (unit 64 w + 32 ub + l % 32, half h = l / 32) of
+//   w1p: step s (k = 16 s + 8 h + t; zero for k >= d_in) at (((w*2 + ub)*NK + s)*64 + l)*8 + t,
+//   w2p: k group g, step s (k = 64 g + 32 h + 8 s + t) at (((((w*8 + g)*2 + ub)*4 + s)*64 + l)*8 + t,
+//   w3p: (channel ch, unit block blk) at ((blk*32 + ch)*2 + h)*8 + t, zero for ch >= d_out.
 __global__ void __launch_bounds__(256) mlp_infer_pack_kernel(const uint16_t* w1, const uint16_t* w2,
                                                              const uint16_t* w3, int d_in, int k1, int d_out,
-                                                             uint16_t* w1t, uint16_t* w2t, uint16_t* w3p) {
+                                                             uint16_t* w1p, uint16_t* w2p, uint16_t* w3p) {
     const int64_t n1 = (int64_t)FI_DH * k1, n2 = (int64_t)FI_DH * FI_DH, n3 = (int64_t)(FI_DH / 16) * 32 * 16;
+    const int nk = k1 / 16;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n1 + n2 + n3;
          e += (int64_t)gridDim.x * blockDim.x) {
         if (e < n1) {
-            const int unit = (int)(e / k1), k = (int)(e % k1);
-            w1t[e] = k < d_in ? w1[(size_t)k * FI_DH + unit] : (uint16_t)0;
+            const int t = (int)(e & 7), l = (int)((e >> 3) & 63);
+            const int rest = (int)(e >> 9), s = rest % nk, ub = (rest / nk) & 1, w = (rest / nk) >> 1;
+            const int unit = 64 * w + 32 * ub + (l & 31), k = 16 * s + 8 * (l >> 5) + t;
+            w1p[e] = k < d_in ? w1[(size_t)k * FI_DH + unit] : (uint16_t)0;
         } else if (e < n1 + n2) {
             const int64_t f = e - n1;
-            const int unit = (int)(f / FI_DH), k = (int)(f % FI_DH);
-            w2t[f] = w2[(size_t)k * FI_DH + unit];
+            const int t = (int)(f & 7), l = (int)((f >> 3) & 63), s = (int)((f >> 9) & 3);
+            const int ub = (int)((f >> 11) & 1), g = (int)((f >> 12) & 7), w = (int)(f >> 15);
+            const int unit = 64 * w + 32 * ub + (l & 31), k = 64 * g + 32 * (l >> 5) + 8 * s + t;
+            w2p[f] = w2[(size_t)k * FI_DH + unit];
         } else {
             const int64_t f = e - n1 - n2;                  // ((blk * 32 + ch) * 2 + h) * 8 + t
             const int t = (int)(f & 7), h = (int)((f >> 3) & 1), ch = (int)((f >> 4) & 31), blk = (int)(f >> 9);
@@ -713,14 +771,14 @@ int pdlp_adam_step(float* w, const float* g, float* m, float* v, uint16_t* wb, i
 }
 
 int pdlp_mlp_infer_pack(const uint16_t* w1, const uint16_t* w2, const uint16_t* w3, int32_t d_in, int32_t d_h,
-                        int32_t d_out, uint16_t* w1t, uint16_t* w2t, uint16_t* w3p, void* stream) {
+                        int32_t d_out, uint16_t* w1p, uint16_t* w2p, uint16_t* w3p, void* stream) {
     if (d_h != FI_DH) return fail(PDLP_E_SHAPE, "mlp infer: d_h must be 512");
     if (d_in <= 0 || d_in > FI_MAX_DIN || d_in % 8 != 0) return fail(PDLP_E_SHAPE, "mlp infer: d_in in 8..128, multiple of 8");
     if (d_out <= 0 || d_out > FI_MAX_DOUT) return fail(PDLP_E_SHAPE, "mlp infer: d_out in 1..32");
-    if (!w1 || !w2 || !w3 || !w1t || !w2t || !w3p) return fail(PDLP_E_ARG, "mlp infer pack: NULL pointer");
+    if (!w1 || !w2 || !w3 || !w1p || !w2p || !w3p) return fail(PDLP_E_ARG, "mlp infer pack: NULL pointer");
     const int k1 = (d_in + 15) / 16 * 16;
     const int64_t total = (int64_t)FI_DH * k1 + (int64_t)FI_DH * FI_DH + (int64_t)(FI_DH / 16) * 32 * 16;
-    mlp_infer_pack_kernel<<<grid1(total), 256, 0, as_stream(stream)>>>(w1, w2, w3, d_in, k1, d_out, w1t, w2t, w3p);
+    mlp_infer_pack_kernel<<<grid1(total), 256, 0, as_stream(stream)>>>(w1, w2, w3, d_in, k1, d_out, w1p, w2p, w3p);
     return hip_fail(hipGetLastError(), "mlp_infer_pack_kernel launch");
 }
 
@@ -732,20 +790,24 @@ int pdlp_mlp_infer_bf16(const PdlpMlpInfer* p, void* stream) {
     if (p->d_out <= 0 || p->d_out > FI_MAX_DOUT) return fail(PDLP_E_SHAPE, "mlp infer: d_out in 1..32");
     if (p->rows < 0) return fail(PDLP_E_SHAPE, "mlp infer: rows >= 0");
     if (p->rows == 0) return 0;
-    if (!p->x || !p->w1t || !p->w2t || !p->w3p || !p->b1 || !p->b2 || !p->b3 || !p->out)
+    if (!p->x || !p->w1p || !p->w2p || !p->w3p || !p->b1 || !p->b2 || !p->b3 || !p->out)
         return fail(PDLP_E_ARG, "mlp infer: NULL pointer");
     // 16-byte operand loads: x rows (d_in a multiple of 8) and the packed weights
-    if ((reinterpret_cast<uintptr_t>(p->x) | reinterpret_cast<uintptr_t>(p->w1t) |
-         reinterpret_cast<uintptr_t>(p->w2t) | reinterpret_cast<uintptr_t>(p->w3p) |
+    if ((reinterpret_cast<uintptr_t>(p->x) | reinterpret_cast<uintptr_t>(p->w1p) |
+         reinterpret_cast<uintptr_t>(p->w2p) | reinterpret_cast<uintptr_t>(p->w3p) |
          reinterpret_cast<uintptr_t>(p->b1) | reinterpret_cast<uintptr_t>(p->b2)) & 15)
         return fail(PDLP_E_ARG, "mlp infer: x, the packed weights, b1 and b2 must be 16-byte aligned");
-    const hipError_t e = hipFuncSetAttribute((const void*)mlp_infer_kernel,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)FI_LDS);
-    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
-    const int k1 = (p->d_in + 15) / 16 * 16;
     const unsigned grid = (unsigned)((p->rows + FI_ROWS - 1) / FI_ROWS);
-    mlp_infer_kernel<<<grid, FI_THREADS, FI_LDS, as_stream(stream)>>>(*p, k1);
-    return hip_fail(hipGetLastError(), "mlp_infer_kernel launch");
+    switch ((p->d_in + 15) / 16) {
+        case 1: return launch_infer<1>(*p, grid, as_stream(stream));
+        case 2: return launch_infer<2>(*p, grid, as_stream(stream));
+        case 3: return launch_infer<3>(*p, grid, as_stream(stream));
+        case 4: return launch_infer<4>(*p, grid, as_stream(stream));
+        case 5: return launch_infer<5>(*p, grid, as_stream(stream));
+        case 6: return launch_infer<6>(*p, grid, as_stream(stream));
+        case 7: return launch_infer<7>(*p, grid, as_stream(stream));
+        default: return launch_infer<8>(*p, grid, as_stream(stream));
+    }
 }
 
 int pdlp_cast_bf16(const float* x, uint16_t* y, int64_t n, void* stream) {

@@ -140,7 +140,7 @@ class PdlpMlp(C.Structure):
 class PdlpMlpInfer(C.Structure):
     _fields_ = [
         ("rows", C.c_int32), ("d_in", C.c_int32), ("d_h", C.c_int32), ("d_out", C.c_int32),
-        ("x", _p), ("w1t", _p), ("w2t", _p), ("w3p", _p), ("b1", _p), ("b2", _p), ("b3", _p), ("out", _p),
+        ("x", _p), ("w1p", _p), ("w2p", _p), ("w3p", _p), ("b1", _p), ("b2", _p), ("b3", _p), ("out", _p),
     ]
 
 

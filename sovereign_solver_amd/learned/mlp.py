@@ -154,16 +154,16 @@ class WarmStartMLP:
                 and self.d_out <= self.INFER_MAX_D_OUT)
 
     def _pack_infer(self):
-        """W1^T, W2^T and the layer-3 operand order of the BF16 weights (once per update)."""
+        """The BF16 weights in the fused kernel's operand-fragment order (once per update)."""
         if self._packed is None:
             k1 = -(-self.d_in // 16) * 16
             with torch.cuda.stream(self.stream):
-                pk = dict(w1t=torch.empty(self.d_h * k1, dtype=torch.bfloat16, device=self.device),
-                          w2t=torch.empty(self.d_h * self.d_h, dtype=torch.bfloat16, device=self.device),
+                pk = dict(w1p=torch.empty(self.d_h * k1, dtype=torch.bfloat16, device=self.device),
+                          w2p=torch.empty(self.d_h * self.d_h, dtype=torch.bfloat16, device=self.device),
                           w3p=torch.empty(32 * 32 * 16, dtype=torch.bfloat16, device=self.device))
                 L.check(self.lib.pdlp_mlp_infer_pack(self._p("W1", self.wb), self._p("W2", self.wb),
                                                      self._p("W3", self.wb), self.d_in, self.d_h, self.d_out,
-                                                     pk["w1t"].data_ptr(), pk["w2t"].data_ptr(),
+                                                     pk["w1p"].data_ptr(), pk["w2p"].data_ptr(),
                                                      pk["w3p"].data_ptr(), self.sh), "mlp infer pack")
             self._packed = pk
         return self._packed
@@ -182,7 +182,7 @@ class WarmStartMLP:
             if out is None:
                 out = torch.empty(B, self.d_out, dtype=torch.float32, device=self.device)
             d = L.PdlpMlpInfer(rows=B, d_in=self.d_in, d_h=self.d_h, d_out=self.d_out, x=xb.data_ptr(),
-                               w1t=pk["w1t"].data_ptr(), w2t=pk["w2t"].data_ptr(), w3p=pk["w3p"].data_ptr(),
+                               w1p=pk["w1p"].data_ptr(), w2p=pk["w2p"].data_ptr(), w3p=pk["w3p"].data_ptr(),
                                b1=self._p("b1", self.w), b2=self._p("b2", self.w), b3=self._p("b3", self.w),
                                out=out.data_ptr())
             L.check(self.lib.pdlp_mlp_infer_bf16(C.byref(d), self.sh), "mlp infer")
