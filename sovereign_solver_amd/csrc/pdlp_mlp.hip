@@ -383,7 +383,7 @@ constexpr int FI_ROWS = 128, FI_DH = 512, FI_WAVES = 8, FI_THREADS = FI_WAVES * 
 constexpr int FI_LDH = FI_DH + 8;               // H1 row stride (BF16): a 16-byte pad per row
 constexpr int FI_MAX_DIN = 128, FI_MAX_DOUT = 32;
 constexpr size_t FI_H1 = (size_t)FI_ROWS * FI_LDH * 2;    // H1; later >= FI_WAVES * 32 * FI_ROWS floats
-constexpr size_t FI_LDS = FI_H1 + 2 * FI_DH * 4;            // + b1 and b2 (FP32)
+constexpr size_t FI_LDS = FI_H1 + FI_DH * 4;                // + b2 (FP32)
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -411,11 +411,15 @@ __global__ void __launch_bounds__(FI_THREADS) mlp_infer_kernel(PdlpMlpInfer p) {
     const int li = lane & 31, lh = lane >> 5;
     const int r0 = blockIdx.x * FI_ROWS;
     const int u0 = w * 64;
-    float* bl1 = reinterpret_cast<float*>(reinterpret_cast<char*>(fi_lds) + FI_H1);   // b1 then b2 in LDS
-    float* bl2 = bl1 + FI_DH;
-    bl1[threadIdx.x] = p.b1[threadIdx.x];
+    // b2 to LDS for the layer-3 epilogue (the barrier after layer 1 orders it); this
+    // lane's b1 values (units 8 g + 4 h .. + 3 of each block) in registers
+    float* bl2 = reinterpret_cast<float*>(reinterpret_cast<char*>(fi_lds) + FI_H1);
     bl2[threadIdx.x] = p.b2[threadIdx.x];
-    __syncthreads();
+    f32x4 bias1[2][4];
+#pragma unroll
+    for (int ub = 0; ub < 2; ++ub)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) bias1[ub][g] = *reinterpret_cast<const f32x4*>(p.b1 + u0 + 32 * ub + 8 * g + 4 * lh);
 
     // The packed weights are in fragment order: the 64 lanes' operands of one MFMA
     // are 1 KB of consecutive memory, so every load instruction reads whole lines.
@@ -451,6 +455,7 @@ __global__ void __launch_bounds__(FI_THREADS) mlp_infer_kernel(PdlpMlpInfer p) {
         }
     };
     load_x(0, bx);
+    __builtin_amdgcn_sched_barrier(0);              // every load above is issued before any wait
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) {
         if (rb < 3) load_x(rb + 1, bn);
@@ -469,7 +474,7 @@ __global__ void __launch_bounds__(FI_THREADS) mlp_infer_kernel(PdlpMlpInfer p) {
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const int unit = u0 + 32 * ub + 8 * g + 4 * lh;
-                const f32x4 bias = *reinterpret_cast<const f32x4*>(bl1 + unit);
+                const f32x4 bias = bias1[ub][g];
                 const float v0 = fmaxf(c[ub][4 * g + 0] + bias[0], 0.f);
                 const float v1 = fmaxf(c[ub][4 * g + 1] + bias[1], 0.f);
                 const float v2 = fmaxf(c[ub][4 * g + 2] + bias[2], 0.f);
@@ -560,7 +565,7 @@ __global__ void __launch_bounds__(FI_THREADS) mlp_infer_kernel(PdlpMlpInfer p) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
             const int ch = 8 * (r >> 2) + 4 * lh + (r & 3);
-            red[((size_t)w * 32 + ch) * FI_ROWS + 32 * rb + li] = acc3[rb][r];
+            if (ch < p.d_out) red[((size_t)w * 32 + ch) * FI_ROWS + 32 * rb + li] = acc3[rb][r];
         }
     __syncthreads();
     for (int e = threadIdx.x; e < FI_ROWS * p.d_out; e += FI_THREADS) {
