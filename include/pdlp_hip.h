@@ -625,6 +625,9 @@ typedef struct PdlpMlpInfer {
     const uint16_t *w1p, *w2p, *w3p;   /* packed BF16 weights, 16-byte aligned     */
     const float *b1, *b2, *b3;         /* FP32 biases (b1, b2 16-byte aligned)     */
     float* out;                        /* rows x d_out FP32                         */
+    uint16_t *h1, *h2;                 /* NULL, or rows x 512 BF16 each (16-byte aligned):
+                                          the hidden activations, stored for a backward
+                                          pass (pdlp_mlp_bwd_bf16 reads them)          */
 } PdlpMlpInfer;
 int pdlp_mlp_infer_pack(const uint16_t* w1, const uint16_t* w2, const uint16_t* w3, int32_t d_in, int32_t d_h,
                         int32_t d_out, uint16_t* w1p, uint16_t* w2p, uint16_t* w3p, void* stream);

@@ -403,7 +403,8 @@ __device__ __forceinline__ f32x16 mfma(const u16x8& a, const u16x8& b, const f32
 // rows in flight during the current block's MFMAs; each block goes to LDS
 // (bias, ReLU, BF16) as soon as it is done, so only its two accumulator blocks
 // are live.  The first layer-2 operand group is issued before layer 1.
-template <int NK>
+// ST: also store the hidden activations h1, h2 (BF16, [rows][512]) for a backward pass.
+template <int NK, bool ST>
 __global__ void __launch_bounds__(FI_THREADS) mlp_infer_kernel(PdlpMlpInfer p) {
     constexpr int k1 = 16 * NK;
     extern __shared__ __align__(16) uint16_t fi_lds[];
@@ -490,6 +491,14 @@ __global__ void __launch_bounds__(FI_THREADS) mlp_infer_kernel(PdlpMlpInfer p) {
         }
     }
     __syncthreads();
+    if constexpr (ST) {                             // H1 rows to h1, 16 bytes a thread at a time
+        for (int e = threadIdx.x; e < FI_ROWS * (FI_DH / 8); e += FI_THREADS) {
+            const int row = e / (FI_DH / 8), c8 = (e % (FI_DH / 8)) * 8;
+            if (r0 + row < p.rows)
+                *reinterpret_cast<u16x8*>(p.h1 + (size_t)(r0 + row) * FI_DH + c8) =
+                    *reinterpret_cast<const u16x8*>(fi_lds + (size_t)row * FI_LDH + c8);
+        }
+    }
 
     f32x16 acc[2][4];
     // ---- layer 2: D[unit][row] = sum_k W2^T[unit][k] H1[row][k], k in groups of 64
@@ -555,6 +564,16 @@ __global__ void __launch_bounds__(FI_THREADS) mlp_infer_kernel(PdlpMlpInfer p) {
                 u16x8 hv;
 #pragma unroll
                 for (int t = 0; t < 8; ++t) hv[t] = bf16_rne(fmaxf(acc[ub][rb][8 * q + t] + bias[t], 0.f));
+                if constexpr (ST) {                 // units ubase + 0..3 and ubase + 8..11 of this row
+                    const int row = r0 + 32 * rb + li;
+                    if (row < p.rows) {
+                        uint16_t* h = p.h2 + (size_t)row * FI_DH + ubase;
+                        *reinterpret_cast<uint2*>(h) = make_uint2((uint32_t)hv[0] | ((uint32_t)hv[1] << 16),
+                                                                  (uint32_t)hv[2] | ((uint32_t)hv[3] << 16));
+                        *reinterpret_cast<uint2*>(h + 8) = make_uint2((uint32_t)hv[4] | ((uint32_t)hv[5] << 16),
+                                                                      (uint32_t)hv[6] | ((uint32_t)hv[7] << 16));
+                    }
+                }
                 acc3[rb] = mfma(a3, hv, acc3[rb]);
             }
         }
@@ -577,13 +596,18 @@ __global__ void __launch_bounds__(FI_THREADS) mlp_infer_kernel(PdlpMlpInfer p) {
     }
 }
 
-template <int NK>
-int launch_infer(const PdlpMlpInfer& p, unsigned grid, hipStream_t s) {
-    const hipError_t e = hipFuncSetAttribute((const void*)mlp_infer_kernel<NK>,
+template <int NK, bool ST>
+int launch_infer_st(const PdlpMlpInfer& p, unsigned grid, hipStream_t s) {
+    const hipError_t e = hipFuncSetAttribute((const void*)mlp_infer_kernel<NK, ST>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)FI_LDS);
     if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
-    mlp_infer_kernel<NK><<<grid, FI_THREADS, FI_LDS, s>>>(p);
+    mlp_infer_kernel<NK, ST><<<grid, FI_THREADS, FI_LDS, s>>>(p);
     return hip_fail(hipGetLastError(), "mlp_infer_kernel launch");
+}
+
+template <int NK>
+int launch_infer(const PdlpMlpInfer& p, unsigned grid, hipStream_t s) {
+    return p.h1 != nullptr ? launch_infer_st<NK, true>(p, grid, s) : launch_infer_st<NK, false>(p, grid, s);
 }
 
 // The weights in the fragment order of mlp_infer_kernel: element t of lane l's
@@ -797,6 +821,9 @@ int pdlp_mlp_infer_bf16(const PdlpMlpInfer* p, void* stream) {
     if (p->rows == 0) return 0;
     if (!p->x || !p->w1p || !p->w2p || !p->w3p || !p->b1 || !p->b2 || !p->b3 || !p->out)
         return fail(PDLP_E_ARG, "mlp infer: NULL pointer");
+    if ((p->h1 == nullptr) != (p->h2 == nullptr)) return fail(PDLP_E_ARG, "mlp infer: h1 and h2 go together");
+    if ((reinterpret_cast<uintptr_t>(p->h1) | reinterpret_cast<uintptr_t>(p->h2)) & 15)
+        return fail(PDLP_E_ARG, "mlp infer: h1 and h2 must be 16-byte aligned");
     // 16-byte operand loads: x rows (d_in a multiple of 8) and the packed weights
     if ((reinterpret_cast<uintptr_t>(p->x) | reinterpret_cast<uintptr_t>(p->w1p) |
          reinterpret_cast<uintptr_t>(p->w2p) | reinterpret_cast<uintptr_t>(p->w3p) |

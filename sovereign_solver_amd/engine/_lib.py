@@ -141,6 +141,7 @@ class PdlpMlpInfer(C.Structure):
     _fields_ = [
         ("rows", C.c_int32), ("d_in", C.c_int32), ("d_h", C.c_int32), ("d_out", C.c_int32),
         ("x", _p), ("w1p", _p), ("w2p", _p), ("w3p", _p), ("b1", _p), ("b2", _p), ("b3", _p), ("out", _p),
+        ("h1", _p), ("h2", _p),
     ]
 
 

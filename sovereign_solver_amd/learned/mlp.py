@@ -139,7 +139,27 @@ class WarmStartMLP:
 
     # ------------------------------------------------------------------- API
     def forward(self, xb: torch.Tensor) -> torch.Tensor:
-        """[B, d_out] FP32 output (a view of an internal buffer, valid until the next call)."""
+        """[B, d_out] FP32 output (a view of an internal buffer, valid until the next call);
+        the hidden activations stay in the internal buffers for a backward pass."""
+        d = self._desc(xb)
+        self._forward_desc(d)
+        return self._buffers(xb.shape[0])["out"]
+
+    def _forward_desc(self, d: L.PdlpMlp) -> None:
+        """Forward pass of a descriptor: the fused kernel (storing h1 and h2) where its
+        shape limits allow, else layer by layer."""
+        with torch.cuda.stream(self.stream):
+            if self.can_infer_fused:
+                pk = self._pack_infer()
+                fi = L.PdlpMlpInfer(rows=d.batch, d_in=self.d_in, d_h=self.d_h, d_out=self.d_out, x=d.x,
+                                    w1p=pk["w1p"].data_ptr(), w2p=pk["w2p"].data_ptr(), w3p=pk["w3p"].data_ptr(),
+                                    b1=d.b1, b2=d.b2, b3=d.b3, out=d.out, h1=d.h1, h2=d.h2)
+                L.check(self.lib.pdlp_mlp_infer_bf16(C.byref(fi), self.sh), "mlp fwd (fused)")
+            else:
+                L.check(self.lib.pdlp_mlp_fwd_bf16(C.byref(d), self.sh), "mlp fwd")
+
+    def forward_layered(self, xb: torch.Tensor) -> torch.Tensor:
+        """The forward pass layer by layer (three GEMM launches), whatever the shape."""
         d = self._desc(xb)
         with torch.cuda.stream(self.stream):
             L.check(self.lib.pdlp_mlp_fwd_bf16(C.byref(d), self.sh), "mlp fwd")
@@ -190,8 +210,8 @@ class WarmStartMLP:
 
     def loss_and_grad(self, xb: torch.Tensor, target: torch.Tensor, out_weight=None) -> torch.Tensor:
         d = self._desc(xb, target, out_weight)
+        self._forward_desc(d)
         with torch.cuda.stream(self.stream):
-            L.check(self.lib.pdlp_mlp_fwd_bf16(C.byref(d), self.sh), "mlp fwd")
             L.check(self.lib.pdlp_mlp_bwd_bf16(C.byref(d), self.sh), "mlp bwd")
         return self.loss
 

@@ -52,8 +52,8 @@ def test_struct_sizes_match_header():
     # PdlpNodeGather: 6 int32, src2_len, count, seed, 10 pointers; PdlpNodeAssembly: 6 int32, 7 pointers
     assert C.sizeof(L.PdlpNodeGather) == 6 * 4 + 3 * 8 + 10 * 8
     assert C.sizeof(L.PdlpNodeAssembly) == 6 * 4 + 7 * 8
-    # PdlpMlpInfer: 4 int32, 8 pointers
-    assert C.sizeof(L.PdlpMlpInfer) == 4 * 4 + 8 * 8
+    # PdlpMlpInfer: 4 int32, 10 pointers
+    assert C.sizeof(L.PdlpMlpInfer) == 4 * 4 + 10 * 8
 
 
 def test_fused_inference_rejects_shapes_without_device():

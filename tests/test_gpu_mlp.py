@@ -133,15 +133,20 @@ def test_mlp_forward_backward_vs_oracle(gpu):
     xb = _bf(mlp_ref.bf16_round(x), gpu)
     tt = torch.from_numpy(t.astype(np.float32)).to(gpu)
     wt = torch.from_numpy(w.astype(np.float32)).to(gpu)
-    loss = float(mlp.loss_and_grad(xb, tt, wt).item())
+    loss_t = mlp.loss_and_grad(xb, tt, wt)
+    mlp.stream.synchronize()                 # the loss is written on the network's stream
+    loss = float(loss_t.item())
     G = mlp.grads_numpy()
     P32 = {k: v.astype(np.float32).astype(np.float64) for k, v in mlp.params_numpy().items()}
     ref_loss, ref_g, ref_out = mlp_ref.loss_and_grads(x, t.astype(np.float32).astype(np.float64), **P32, w=w,
                                                       rnd=mlp_ref.bf16_round)
     assert abs(loss - ref_loss) <= 1e-3 * ref_loss
-    for k in ref_g:
-        err = np.linalg.norm(G[k] - ref_g[k]) / max(np.linalg.norm(ref_g[k]), 1e-30)
-        assert err < 2e-2, (k, err)
+    errs = {k: np.linalg.norm(G[k] - ref_g[k]) / max(np.linalg.norm(ref_g[k]), 1e-30) for k in ref_g}
+    print("relative gradient errors:", {k: f"{v:.2e}" for k, v in errs.items()})
+    # measured at most 8e-8: at these sizes no activation crosses a BF16 rounding boundary
+    # between the FP32 and FP64 sums, so what is left is FP32 summation order
+    for k, e in errs.items():
+        assert e < 1e-6, (k, e)
 
 
 def test_adam_step_matches_formula(gpu):
@@ -218,7 +223,7 @@ def test_fused_inference_matches_layer_by_layer(gpu):
     fused_t = mlp.infer(xb)
     mlp.stream.synchronize()                 # outputs live on the network's stream
     fused = fused_t.cpu().numpy().astype(np.float64)
-    layered_t = mlp.forward(xb)
+    layered_t = mlp.forward_layered(xb)
     mlp.stream.synchronize()
     layered = layered_t.cpu().numpy().astype(np.float64)
     P32 = {k: v.astype(np.float32).astype(np.float64) for k, v in mlp.params_numpy().items()}
@@ -250,3 +255,57 @@ def test_fused_inference_refreshes_after_weight_update(gpu):
     mlp.stream.synchronize()
     b = b_t.cpu().numpy()
     assert np.array_equal(a, -b) and np.any(a != 0)
+
+
+@pytest.mark.parametrize("rows,d_in", [(300, 88), (4096, 48)])
+def test_fused_training_forward_stores_hidden_layers_exactly(gpu, rows, d_in):
+    """The training forward of a d_h = 512 network runs the fused kernel and stores h1 and
+    h2 for the backward pass: on integer data every stored activation and output equals
+    the oracle's bit for bit (the row tail included)."""
+    rng = np.random.default_rng(rows + d_in)
+    mlp = WarmStartMLP(d_in, 512, 8, seed=4)
+    assert mlp.can_infer_fused
+    P = _integer_net(rng, d_in, 8)
+    mlp.set_params(P)
+    x = rng.integers(-2, 3, (rows, d_in)).astype(np.float64)
+    out_t = mlp.forward(_bf(x, gpu))
+    mlp.stream.synchronize()
+    bu = mlp._buffers(rows)
+    h1, h2, want = mlp_ref.forward(x, **P, rnd=mlp_ref.bf16_round)
+    assert np.array_equal(out_t.cpu().numpy().astype(np.float64), want)
+    assert np.array_equal(bu["h1"].float().cpu().numpy().astype(np.float64), h1)
+    assert np.array_equal(bu["h2"].float().cpu().numpy().astype(np.float64), h2)
+
+
+def test_node_shape_gradients_vs_oracle(gpu):
+    """A training step at the node model's shapes (d_in 88, d_h 512, d_out 8, 4,096 rows:
+    fused forward, K-split weight gradients) against the FP64 oracle with the kernels'
+    BF16 rounding points."""
+    rng = np.random.default_rng(8)
+    B, di, dh, do = 4096, 88, 512, 8
+    mlp = WarmStartMLP(di, dh, do, seed=2)
+    P = mlp.params_numpy()
+    P["b1"] = rng.standard_normal(dh) * 0.1
+    P["b2"] = rng.standard_normal(dh) * 0.1
+    P["b3"] = rng.standard_normal(do) * 0.1
+    mlp.set_params(P)
+    x = rng.standard_normal((B, di))
+    t = rng.standard_normal((B, do))
+    w = np.ones(do)
+    xb = _bf(mlp_ref.bf16_round(x), gpu)
+    tt = torch.from_numpy(t.astype(np.float32)).to(gpu)
+    wt = torch.from_numpy(w.astype(np.float32)).to(gpu)
+    loss_t = mlp.loss_and_grad(xb, tt, wt)
+    mlp.stream.synchronize()                 # the loss is written on the network's stream
+    loss = float(loss_t.item())
+    G = mlp.grads_numpy()
+    P32 = {k: v.astype(np.float32).astype(np.float64) for k, v in mlp.params_numpy().items()}
+    ref_loss, ref_g, _ = mlp_ref.loss_and_grads(x, t.astype(np.float32).astype(np.float64), **P32, w=w,
+                                                rnd=mlp_ref.bf16_round)
+    assert abs(loss - ref_loss) <= 1e-3 * ref_loss
+    errs = {k: np.linalg.norm(G[k] - ref_g[k]) / max(np.linalg.norm(ref_g[k]), 1e-30) for k in ref_g}
+    print("relative gradient errors:", {k: f"{v:.2e}" for k, v in errs.items()})
+    # measured at most 8e-5 (W1): FP32 summation order and rare BF16 rounding flips of
+    # the activations; a wrong operand, index or rounding point gives O(1)
+    for k, e in errs.items():
+        assert e < 1e-3, (k, e)
