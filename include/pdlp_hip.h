@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PDLP_ABI_VERSION 10
+#define PDLP_ABI_VERSION 11
 
 /* precision of iterates, matrix values and problem vectors */
 #define PDLP_F32 0
@@ -601,6 +601,11 @@ typedef struct PdlpMlp {
                                           K-split weight-gradient GEMMs (K = batch) and column sums,
                                           so a long batch with few output tiles still fills the chip */
     int64_t work_floats;
+    /* backward: NULL, or W2 and W3 packed by pdlp_mlp_dz_pack (16-byte aligned); with
+       d_h = 512 and d_out a multiple of 8 up to 32 the two masked products
+       dz2 = (dout w3^T) * [h2 > 0] and dz1 = (dz2 w2^T) * [h1 > 0] then run as one fused
+       kernel (same BF16 rounding points) */
+    const uint16_t *w2b, *w3b;
 } PdlpMlp;
 
 /* h1 = relu(x w1 + b1), h2 = relu(h1 w2 + b2), out = h2 w3 + b3. */
@@ -629,6 +634,10 @@ typedef struct PdlpMlpInfer {
                                           the hidden activations, stored for a backward
                                           pass (pdlp_mlp_bwd_bf16 reads them)          */
 } PdlpMlpInfer;
+/* W2 (512 x 512) and W3 (512 x d_out) of a PdlpMlp in the fused backward kernel's
+ * operand-fragment order: w2b 512 * 512 values, w3b 512 * 16 * ceil(d_out / 16). */
+int pdlp_mlp_dz_pack(const uint16_t* w2, const uint16_t* w3, int32_t d_h, int32_t d_out, uint16_t* w2b,
+                     uint16_t* w3b, void* stream);
 int pdlp_mlp_infer_pack(const uint16_t* w1, const uint16_t* w2, const uint16_t* w3, int32_t d_in, int32_t d_h,
                         int32_t d_out, uint16_t* w1p, uint16_t* w2p, uint16_t* w3p, void* stream);
 int pdlp_mlp_infer_bf16(const PdlpMlpInfer* p, void* stream);

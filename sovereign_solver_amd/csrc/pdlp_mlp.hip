@@ -398,6 +398,47 @@ __device__ __forceinline__ f32x16 mfma(const u16x8& a, const u16x8& b, const f32
                                                    0, 0, 0);
 }
 
+// acc[ub][rb] += the 512-k product of this wavefront's A fragments (w2w: its first
+// fragment of the packed 512 x 512 weight, fragment order (k group, unit block,
+// step)) and the FI_ROWS x 512 BF16 rows in LDS (hrow: row li, column 32 h);
+// `an` holds k group 0's fragments on entry.  Two register sets of A in turn: the
+// loads of a group are issued a whole group (32 MFMAs) before its MFMAs; the loop
+// is kept rolled and scheduling barriers keep the loads there.
+__device__ __forceinline__ void hidden_product(const uint16_t* w2w, const uint16_t* hrow, u16x8 (&an)[2][4],
+                                               f32x16 (&acc)[2][4]) {
+    auto group_load = [&](int gk, u16x8 (&dst)[2][4]) {
+        const uint16_t* g = w2w + (size_t)gk * 2 * 4 * 512;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            dst[0][s] = ld16(g + (size_t)(0 * 4 + s) * 512);
+            dst[1][s] = ld16(g + (size_t)(1 * 4 + s) * 512);
+        }
+    };
+    auto group_mfma = [&](int kg, const u16x8 (&a)[2][4]) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            u16x8 b[4];
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+                b[rb] = *reinterpret_cast<const u16x8*>(hrow + (size_t)32 * rb * FI_LDH + kg + 8 * s);
+#pragma unroll
+            for (int ub = 0; ub < 2; ++ub)
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb) acc[ub][rb] = mfma(a[ub][s], b[rb], acc[ub][rb]);
+        }
+    };
+    u16x8 ab[2][4];
+#pragma unroll 1
+    for (int kg = 0; kg < FI_DH; kg += 128) {
+        group_load(kg / 64 + 1, ab);
+        __builtin_amdgcn_sched_barrier(0);          // the loads stay ahead of the group's MFMAs
+        group_mfma(kg, an);
+        if (kg + 128 < FI_DH) group_load(kg / 64 + 2, an);
+        __builtin_amdgcn_sched_barrier(0);
+        group_mfma(kg + 64, ab);
+    }
+}
+
 // NK: layer-1 MFMA steps (k1 = 16 NK).  Layer 1 runs one block of 32 rows at a
 // time with every W1^T operand loaded once up front and the next block's x
 // rows in flight during the current block's MFMAs; each block goes to LDS
@@ -508,40 +549,7 @@ __global__ void __launch_bounds__(FI_THREADS) mlp_infer_kernel(PdlpMlpInfer p) {
         for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[ub][rb][r] = 0.f;
-    const uint16_t* hrow = fi_lds + (size_t)li * FI_LDH + 32 * lh;
-    // two register sets of A in turn: the loads of one group are issued a whole group
-    // (32 MFMAs) before its MFMAs; the loop is kept rolled so the schedule keeps them there
-    auto group_load = [&](int gk, u16x8 (&dst)[2][4]) {
-        const uint16_t* g = w2w + (size_t)gk * 2 * 4 * 512;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            dst[0][s] = ld16(g + (size_t)(0 * 4 + s) * 512);
-            dst[1][s] = ld16(g + (size_t)(1 * 4 + s) * 512);
-        }
-    };
-    auto group_mfma = [&](int kg, const u16x8 (&a)[2][4]) {
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            u16x8 b[4];
-#pragma unroll
-            for (int rb = 0; rb < 4; ++rb)
-                b[rb] = *reinterpret_cast<const u16x8*>(hrow + (size_t)32 * rb * FI_LDH + kg + 8 * s);
-#pragma unroll
-            for (int ub = 0; ub < 2; ++ub)
-#pragma unroll
-                for (int rb = 0; rb < 4; ++rb) acc[ub][rb] = mfma(a[ub][s], b[rb], acc[ub][rb]);
-        }
-    };
-    u16x8 ab[2][4];
-#pragma unroll 1
-    for (int kg = 0; kg < FI_DH; kg += 128) {
-        group_load(kg / 64 + 1, ab);
-        __builtin_amdgcn_sched_barrier(0);          // the loads stay ahead of the group's MFMAs
-        group_mfma(kg, an);
-        if (kg + 128 < FI_DH) group_load(kg / 64 + 2, an);
-        __builtin_amdgcn_sched_barrier(0);
-        group_mfma(kg + 64, ab);
-    }
+    hidden_product(w2w, fi_lds + (size_t)li * FI_LDH + 32 * lh, an, acc);
 
     // ---- layer 3 from the accumulators: D3[channel][row] over this wavefront's units
     f32x16 acc3[4];
@@ -594,6 +602,158 @@ __global__ void __launch_bounds__(FI_THREADS) mlp_infer_kernel(PdlpMlpInfer p) {
         for (int v = 0; v < FI_WAVES; ++v) s += red[((size_t)v * 32 + ch) * FI_ROWS + row];
         if (r0 + row < p.rows) p.out[(size_t)(r0 + row) * p.d_out + ch] = s + p.b3[ch];
     }
+}
+
+// ------------------------------------------------------- fused backward dz
+// dz2 = BF16((dout W3^T) * [h2 > 0]) and dz1 = BF16((dz2 W2^T) * [h1 > 0]) of FI_ROWS
+// rows per workgroup in one kernel (the two masked products of pdlp_mlp_bwd_bf16,
+// same rounding points), transposed as in mlp_infer_kernel: each of the 8
+// wavefronts owns 64 hidden units.  h2 is staged into LDS; the dz2 tile replaces
+// it in place (each element read and written by the same lane), goes to global
+// memory for the weight and bias gradients, and is the B operand of the second
+// product (A = W2 rows in fragment order, w2b).  NK3: MFMA steps over d_out.
+template <int NK3>
+__global__ void __launch_bounds__(FI_THREADS) mlp_dz_kernel(PdlpMlp p) {
+    extern __shared__ __align__(16) uint16_t fi_lds[];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 31, lh = lane >> 5;
+    const int r0 = blockIdx.x * FI_ROWS, R = p.batch;
+    const int u0 = w * 64;
+    // the second product's first A group, in flight during the first product
+    const uint16_t* w2w = p.w2b + (size_t)w * (FI_DH / 64) * 2 * 4 * 512 + lane * 8;
+    u16x8 an[2][4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        an[0][s] = ld16(w2w + (size_t)(0 * 4 + s) * 512);
+        an[1][s] = ld16(w2w + (size_t)(1 * 4 + s) * 512);
+    }
+    // first product's operands: W3 fragments ((w*2 + ub)*NK3 + s) and dout rows (zero beyond d_out)
+    u16x8 a3[2][NK3], bd[4][NK3];
+#pragma unroll
+    for (int ub = 0; ub < 2; ++ub)
+#pragma unroll
+        for (int s = 0; s < NK3; ++s) a3[ub][s] = ld16(p.w3b + ((size_t)((w * 2 + ub) * NK3 + s) * 64 + lane) * 8);
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+        const int row = r0 + 32 * rb + li;
+        const uint16_t* dr = p.dout_b + (size_t)min(row, R - 1) * p.d_out;
+#pragma unroll
+        for (int s = 0; s < NK3; ++s) {
+            const int kk = 16 * s + 8 * lh;
+            const u16x8 v = ld16(dr + min(kk, p.d_out - 8));
+            const bool ok = row < R && kk < p.d_out;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) bd[rb][s][t] = ok ? v[t] : (unsigned short)0;
+        }
+    }
+    // h2 tile -> LDS [row][unit] (zeros beyond the batch: their dz2 is 0)
+    for (int e = threadIdx.x; e < FI_ROWS * (FI_DH / 8); e += FI_THREADS) {
+        const int row = e / (FI_DH / 8), c8 = (e % (FI_DH / 8)) * 8;
+        u16x8 v = {};
+        if (r0 + row < R) v = ld16(p.h2 + (size_t)(r0 + row) * FI_DH + c8);
+        *reinterpret_cast<u16x8*>(fi_lds + (size_t)row * FI_LDH + c8) = v;
+    }
+    __syncthreads();
+    // ---- dz2[unit][row] = [h2 > 0] sum_c W3[unit][c] dout[row][c], in place of h2
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+        f32x16 c[2];
+#pragma unroll
+        for (int ub = 0; ub < 2; ++ub) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) c[ub][r] = 0.f;
+#pragma unroll
+            for (int s = 0; s < NK3; ++s) c[ub] = mfma(a3[ub][s], bd[rb][s], c[ub]);
+        }
+#pragma unroll
+        for (int ub = 0; ub < 2; ++ub)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                uint2* cell = reinterpret_cast<uint2*>(fi_lds + (size_t)(32 * rb + li) * FI_LDH + u0 + 32 * ub + 8 * g +
+                                                      4 * lh);
+                const uint2 m = *cell;
+                const float h[4] = {bf2f((uint16_t)(m.x & 0xffff)), bf2f((uint16_t)(m.x >> 16)),
+                                    bf2f((uint16_t)(m.y & 0xffff)), bf2f((uint16_t)(m.y >> 16))};
+                uint16_t v[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v[e] = bf16_rne(h[e] > 0.f ? c[ub][4 * g + e] : 0.f);
+                *cell = make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+            }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < FI_ROWS * (FI_DH / 8); e += FI_THREADS) {     // dz2 to global
+        const int row = e / (FI_DH / 8), c8 = (e % (FI_DH / 8)) * 8;
+        if (r0 + row < R)
+            *reinterpret_cast<u16x8*>(p.dz2 + (size_t)(r0 + row) * FI_DH + c8) =
+                *reinterpret_cast<const u16x8*>(fi_lds + (size_t)row * FI_LDH + c8);
+    }
+    // ---- dz1[unit][row] = [h1 > 0] sum_j W2[unit][j] dz2[row][j]
+    f32x16 acc[2][4];
+#pragma unroll
+    for (int ub = 0; ub < 2; ++ub)
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[ub][rb][r] = 0.f;
+    hidden_product(w2w, fi_lds + (size_t)li * FI_LDH + 32 * lh, an, acc);
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+        const int row = r0 + 32 * rb + li;
+        const uint16_t* hr = p.h1 + (size_t)min(row, R - 1) * FI_DH;
+        uint2 m[2][4];
+#pragma unroll
+        for (int ub = 0; ub < 2; ++ub)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) m[ub][g] = *reinterpret_cast<const uint2*>(hr + u0 + 32 * ub + 8 * g + 4 * lh);
+        if (row < R) {
+#pragma unroll
+            for (int ub = 0; ub < 2; ++ub)
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const float h[4] = {bf2f((uint16_t)(m[ub][g].x & 0xffff)), bf2f((uint16_t)(m[ub][g].x >> 16)),
+                                        bf2f((uint16_t)(m[ub][g].y & 0xffff)), bf2f((uint16_t)(m[ub][g].y >> 16))};
+                    uint16_t v[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) v[e] = bf16_rne(h[e] > 0.f ? acc[ub][rb][4 * g + e] : 0.f);
+                    *reinterpret_cast<uint2*>(p.dz1 + (size_t)row * FI_DH + u0 + 32 * ub + 8 * g + 4 * lh) =
+                        make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+                }
+        }
+    }
+}
+
+// Backward operands in fragment order (element t of lane l, unit 64 w + 32 ub + l % 32,
+// half h = l / 32):  w2b: W2 rows (k = 64 g + 32 h + 8 s + t) at
+// (((((w*8 + g)*2 + ub)*4 + s)*64 + l)*8 + t;  w3b: W3 rows (k = 16 s + 8 h + t, zero
+// for k >= d_out) at (((w*2 + ub)*NK3 + s)*64 + l)*8 + t.
+__global__ void __launch_bounds__(256) mlp_dz_pack_kernel(const uint16_t* w2, const uint16_t* w3, int d_out, int nk3,
+                                                          uint16_t* w2b, uint16_t* w3b) {
+    const int64_t n2 = (int64_t)FI_DH * FI_DH, n3 = (int64_t)FI_DH * 16 * nk3;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n2 + n3;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        if (e < n2) {
+            const int t = (int)(e & 7), l = (int)((e >> 3) & 63), s = (int)((e >> 9) & 3);
+            const int ub = (int)((e >> 11) & 1), g = (int)((e >> 12) & 7), w = (int)(e >> 15);
+            const int unit = 64 * w + 32 * ub + (l & 31), k = 64 * g + 32 * (l >> 5) + 8 * s + t;
+            w2b[e] = w2[(size_t)unit * FI_DH + k];
+        } else {
+            const int64_t f = e - n2;
+            const int t = (int)(f & 7), l = (int)((f >> 3) & 63);
+            const int rest = (int)(f >> 9), s = rest % nk3, ub = (rest / nk3) & 1, w = (rest / nk3) >> 1;
+            const int unit = 64 * w + 32 * ub + (l & 31), k = 16 * s + 8 * (l >> 5) + t;
+            w3b[f] = k < d_out ? w3[(size_t)unit * d_out + k] : (uint16_t)0;
+        }
+    }
+}
+
+template <int NK3>
+int launch_dz(const PdlpMlp& p, hipStream_t s) {
+    const hipError_t e = hipFuncSetAttribute((const void*)mlp_dz_kernel<NK3>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)FI_H1);
+    if (e != hipSuccess) return hip_fail(e, "hipFuncSetAttribute");
+    const unsigned grid = (unsigned)((p.batch + FI_ROWS - 1) / FI_ROWS);
+    mlp_dz_kernel<NK3><<<grid, FI_THREADS, FI_H1, s>>>(p);
+    return hip_fail(hipGetLastError(), "mlp_dz_kernel launch");
 }
 
 template <int NK, bool ST>
@@ -768,7 +928,17 @@ int pdlp_mlp_bwd_bf16(const PdlpMlp* p, void* stream) {
         return gemm(g, s);
     };
     if (!rc) rc = wgrad(make(dh, dout, B, p->h2, dh, 1, p->dout_b, dout, 0, nullptr, 0, nullptr, p->gw3, nullptr, dout));
-    if (!rc) {
+    // the fused kernel for both masked products where it applies (d_h = 512, packed weights given)
+    const bool fused = p->w2b != nullptr && p->w3b != nullptr && dh == FI_DH && dout % 8 == 0 && dout <= 32 &&
+                       !((reinterpret_cast<uintptr_t>(p->dout_b) | reinterpret_cast<uintptr_t>(p->h1) |
+                          reinterpret_cast<uintptr_t>(p->h2) | reinterpret_cast<uintptr_t>(p->dz1) |
+                          reinterpret_cast<uintptr_t>(p->dz2) | reinterpret_cast<uintptr_t>(p->w2b) |
+                          reinterpret_cast<uintptr_t>(p->w3b)) & 15);
+    if (!rc && fused && B > 0) {
+        rc = dout <= 16 ? launch_dz<1>(*p, s) : launch_dz<2>(*p, s);
+        if (!rc) rc = colsum(nullptr, p->dz2, p->gb2, B, dh, work, wf, s);
+        if (!rc) rc = wgrad(make(dh, dh, B, p->h1, dh, 1, p->dz2, dh, 0, nullptr, 0, nullptr, p->gw2, nullptr, dh));
+    } else if (!rc) {
         // K = d_out is long and the output (batch x d_h) has few tiles: split K, with dout
         // (free once gb3 and gw3 are taken from it) as the work space of split_k * batch * d_h floats
         PdlpGemm g = make(B, dh, dout, p->dout_b, dout, 0, p->w3, dout, 1, nullptr, 0, p->h2, nullptr, p->dz2, dh);
@@ -777,11 +947,11 @@ int pdlp_mlp_bwd_bf16(const PdlpMlp* p, void* stream) {
         g.split_k = std::max(1, std::min({dout / std::max(dh, 1), 256 / tiles, stages / 4}));
         g.work = p->dout;
         rc = gemm(g, s);
+        if (!rc) rc = colsum(nullptr, p->dz2, p->gb2, B, dh, work, wf, s);
+        // gw2 = h1^T dz2 ; dz1 = (dz2 w2^T) * [h1 > 0]
+        if (!rc) rc = wgrad(make(dh, dh, B, p->h1, dh, 1, p->dz2, dh, 0, nullptr, 0, nullptr, p->gw2, nullptr, dh));
+        if (!rc) rc = gemm(make(B, dh, dh, p->dz2, dh, 0, p->w2, dh, 1, nullptr, 0, p->h1, nullptr, p->dz1, dh), s);
     }
-    if (!rc) rc = colsum(nullptr, p->dz2, p->gb2, B, dh, work, wf, s);
-    // gw2 = h1^T dz2 ; dz1 = (dz2 w2^T) * [h1 > 0]
-    if (!rc) rc = wgrad(make(dh, dh, B, p->h1, dh, 1, p->dz2, dh, 0, nullptr, 0, nullptr, p->gw2, nullptr, dh));
-    if (!rc) rc = gemm(make(B, dh, dh, p->dz2, dh, 0, p->w2, dh, 1, nullptr, 0, p->h1, nullptr, p->dz1, dh), s);
     if (!rc) rc = colsum(nullptr, p->dz1, p->gb1, B, dh, work, wf, s);
     // gw1 = x^T dz1
     if (!rc) rc = wgrad(make(di, dh, B, p->x, di, 1, p->dz1, dh, 0, nullptr, 0, nullptr, p->gw1, nullptr, dh));
@@ -809,6 +979,17 @@ int pdlp_mlp_infer_pack(const uint16_t* w1, const uint16_t* w2, const uint16_t* 
     const int64_t total = (int64_t)FI_DH * k1 + (int64_t)FI_DH * FI_DH + (int64_t)(FI_DH / 16) * 32 * 16;
     mlp_infer_pack_kernel<<<grid1(total), 256, 0, as_stream(stream)>>>(w1, w2, w3, d_in, k1, d_out, w1p, w2p, w3p);
     return hip_fail(hipGetLastError(), "mlp_infer_pack_kernel launch");
+}
+
+int pdlp_mlp_dz_pack(const uint16_t* w2, const uint16_t* w3, int32_t d_h, int32_t d_out, uint16_t* w2b,
+                     uint16_t* w3b, void* stream) {
+    if (d_h != FI_DH) return fail(PDLP_E_SHAPE, "mlp dz pack: d_h must be 512");
+    if (d_out <= 0 || d_out > 32 || d_out % 8 != 0) return fail(PDLP_E_SHAPE, "mlp dz pack: d_out in 8..32, multiple of 8");
+    if (!w2 || !w3 || !w2b || !w3b) return fail(PDLP_E_ARG, "mlp dz pack: NULL pointer");
+    const int nk3 = (d_out + 15) / 16;
+    const int64_t total = (int64_t)FI_DH * FI_DH + (int64_t)FI_DH * 16 * nk3;
+    mlp_dz_pack_kernel<<<grid1(total), 256, 0, as_stream(stream)>>>(w2, w3, d_out, nk3, w2b, w3b);
+    return hip_fail(hipGetLastError(), "mlp_dz_pack_kernel launch");
 }
 
 int pdlp_mlp_infer_bf16(const PdlpMlpInfer* p, void* stream) {

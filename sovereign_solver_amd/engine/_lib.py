@@ -13,7 +13,7 @@ import threading
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "_lib", "libpdlp_hip.so")
 
-PDLP_ABI_VERSION = 10
+PDLP_ABI_VERSION = 11
 PDLP_F32, PDLP_F64 = 0, 1
 PDLP_E_ARG, PDLP_E_TUNE, PDLP_E_SHAPE, PDLP_E_LIMIT = -1, -2, -3, -4
 PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL, PDLP_INFEASIBLE, PDLP_UNBOUNDED, PDLP_SKIPPED = 0, 1, 3, 4, 5, 6
@@ -133,7 +133,7 @@ class PdlpMlp(C.Structure):
         ("h1", _p), ("h2", _p), ("out", _p),
         ("target", _p), ("out_weight", _p), ("dout", _p), ("dout_b", _p), ("dz2", _p), ("dz1", _p),
         ("gw1", _p), ("gw2", _p), ("gw3", _p), ("gb1", _p), ("gb2", _p), ("gb3", _p), ("loss", _p),
-        ("work", _p), ("work_floats", C.c_int64),
+        ("work", _p), ("work_floats", C.c_int64), ("w2b", _p), ("w3b", _p),
     ]
 
 
@@ -205,6 +205,7 @@ _SIGS = {
     "pdlp_mlp_fwd_bf16": (C.c_int, [C.POINTER(PdlpMlp), _p]),
     "pdlp_mlp_bwd_bf16": (C.c_int, [C.POINTER(PdlpMlp), _p]),
     "pdlp_mlp_infer_pack": (C.c_int, [_p, _p, _p, C.c_int32, C.c_int32, C.c_int32, _p, _p, _p, _p]),
+    "pdlp_mlp_dz_pack": (C.c_int, [_p, _p, C.c_int32, C.c_int32, _p, _p, _p]),
     "pdlp_mlp_infer_bf16": (C.c_int, [C.POINTER(PdlpMlpInfer), _p]),
     "pdlp_adam_step": (C.c_int, [_p, _p, _p, _p, _p, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float,
                                  C.c_int32, _p]),

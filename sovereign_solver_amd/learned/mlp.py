@@ -54,6 +54,8 @@ class WarmStartMLP:
         self.step = 0
         self._bufs = {}
         self._packed = None          # fused-inference weights, packed from the current BF16 copy
+        self._packed_dz = None       # fused-backward weights, likewise
+        self.fused = True            # fused kernels where the shapes allow (False: layer by layer)
         self.refresh_bf16()
 
     # ---------------------------------------------------------------- helpers
@@ -68,7 +70,7 @@ class WarmStartMLP:
     def refresh_bf16(self):
         with torch.cuda.stream(self.stream):
             L.check(self.lib.pdlp_cast_bf16(self.w.data_ptr(), self.wb.data_ptr(), self.n_params, self.sh), "cast")
-        self._packed = None
+        self._packed = self._packed_dz = None
 
     def params_numpy(self) -> dict:
         self.stream.synchronize()
@@ -135,6 +137,9 @@ class WarmStartMLP:
             d.loss = self.loss.data_ptr()
             if bu["work_floats"]:
                 d.work, d.work_floats = bu["work"].data_ptr(), bu["work_floats"]
+            if self.fused and self.can_dz_fused:
+                pb = self._pack_dz()
+                d.w2b, d.w3b = pb["w2b"].data_ptr(), pb["w3b"].data_ptr()
         return d
 
     # ------------------------------------------------------------------- API
@@ -149,7 +154,7 @@ class WarmStartMLP:
         """Forward pass of a descriptor: the fused kernel (storing h1 and h2) where its
         shape limits allow, else layer by layer."""
         with torch.cuda.stream(self.stream):
-            if self.can_infer_fused:
+            if self.fused and self.can_infer_fused:
                 pk = self._pack_infer()
                 fi = L.PdlpMlpInfer(rows=d.batch, d_in=self.d_in, d_h=self.d_h, d_out=self.d_out, x=d.x,
                                     w1p=pk["w1p"].data_ptr(), w2p=pk["w2p"].data_ptr(), w3p=pk["w3p"].data_ptr(),
@@ -188,6 +193,23 @@ class WarmStartMLP:
             self._packed = pk
         return self._packed
 
+    @property
+    def can_dz_fused(self) -> bool:
+        return self.d_h == self.INFER_D_H and self.d_out % 8 == 0 and self.d_out <= self.INFER_MAX_D_OUT
+
+    def _pack_dz(self):
+        """W2 and W3 in the fused backward kernel's operand-fragment order (once per update)."""
+        if self._packed_dz is None:
+            nk3 = -(-self.d_out // 16)
+            with torch.cuda.stream(self.stream):
+                pk = dict(w2b=torch.empty(self.d_h * self.d_h, dtype=torch.bfloat16, device=self.device),
+                          w3b=torch.empty(self.d_h * 16 * nk3, dtype=torch.bfloat16, device=self.device))
+                L.check(self.lib.pdlp_mlp_dz_pack(self._p("W2", self.wb), self._p("W3", self.wb), self.d_h,
+                                                  self.d_out, pk["w2b"].data_ptr(), pk["w3b"].data_ptr(), self.sh),
+                        "mlp dz pack")
+            self._packed_dz = pk
+        return self._packed_dz
+
     def infer(self, xb: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
         """[B, d_out] FP32 output of the fused inference kernel (the three layers in one
         launch; the same BF16 rounding points as :meth:`forward`, no activations kept).
@@ -221,7 +243,7 @@ class WarmStartMLP:
             L.check(self.lib.pdlp_adam_step(self.w.data_ptr(), self.g.data_ptr(), self.m.data_ptr(),
                                             self.v.data_ptr(), self.wb.data_ptr(), self.n_params, lr, beta1, beta2,
                                             eps, self.step, self.sh), "adam")
-        self._packed = None
+        self._packed = self._packed_dz = None
 
     def save(self, path: str, extra: Optional[dict] = None):
         """Weights (and extra arrays, e.g. target normalisation) as safetensors."""

@@ -47,8 +47,8 @@ def test_struct_sizes_match_header():
     assert C.sizeof(L.PdlpTune) == 10 * 4
     # PdlpGemm: 10 int32 (M..accumulate), alpha, split_k, 7 pointers (A, B, bias, mask, C, Cb, work)
     assert C.sizeof(L.PdlpGemm) == 10 * 4 + 4 + 4 + 7 * 8
-    # PdlpMlp: 4 int32, 23 pointers, then the backward work space (pointer, int64)
-    assert C.sizeof(L.PdlpMlp) == 4 * 4 + 23 * 8 + 8 + 8
+    # PdlpMlp: 4 int32, 23 pointers, then the backward work space (pointer, int64), w2b, w3b
+    assert C.sizeof(L.PdlpMlp) == 4 * 4 + 23 * 8 + 8 + 8 + 2 * 8
     # PdlpNodeGather: 6 int32, src2_len, count, seed, 10 pointers; PdlpNodeAssembly: 6 int32, 7 pointers
     assert C.sizeof(L.PdlpNodeGather) == 6 * 4 + 3 * 8 + 10 * 8
     assert C.sizeof(L.PdlpNodeAssembly) == 6 * 4 + 7 * 8

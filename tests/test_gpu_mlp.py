@@ -277,13 +277,15 @@ def test_fused_training_forward_stores_hidden_layers_exactly(gpu, rows, d_in):
     assert np.array_equal(bu["h2"].float().cpu().numpy().astype(np.float64), h2)
 
 
-def test_node_shape_gradients_vs_oracle(gpu):
+@pytest.mark.parametrize("fused", [True, False])
+def test_node_shape_gradients_vs_oracle(gpu, fused):
     """A training step at the node model's shapes (d_in 88, d_h 512, d_out 8, 4,096 rows:
     fused forward, K-split weight gradients) against the FP64 oracle with the kernels'
     BF16 rounding points."""
     rng = np.random.default_rng(8)
     B, di, dh, do = 4096, 88, 512, 8
     mlp = WarmStartMLP(di, dh, do, seed=2)
+    mlp.fused = fused                        # fused forward and backward kernels, or layer by layer
     P = mlp.params_numpy()
     P["b1"] = rng.standard_normal(dh) * 0.1
     P["b2"] = rng.standard_normal(dh) * 0.1
