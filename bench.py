@@ -211,10 +211,16 @@ def cpu_baseline(kind: str, ids, tol: float, cap_s: float, what: str):
 
 
 # --------------------------------------------------------------------- timing
-def primal_bytes(form, elem: int) -> float:
-    """Algorithmic bytes of one fused primal-step launch."""
+def primal_bytes(form, elem: int, sell_entries: int = 0) -> float:
+    """Algorithmic bytes of one fused primal-step launch.  K^T as CSR (values, row
+    indices, column pointers) or, with ``sell_entries`` > 0, as the SELL-64 copy the
+    step then reads (values and row indices of every slot, padding included, and a
+    pointer per slice of 64 columns, no column pointers)."""
     n, m, nnz = form.n, form.m, form.nnz
-    b = nnz * (elem + 4) + (n + 1) * 4        # K^T CSR: values, column indices, row pointers
+    if sell_entries:
+        b = sell_entries * (elem + 4) + (-(-n // 64) + 1) * 4
+    else:
+        b = nnz * (elem + 4) + (n + 1) * 4    # K^T CSR: values, column indices, row pointers
     b += m * elem                             # y, gathered
     b += n * elem                             # c
     if form.lo is not None:
@@ -223,6 +229,11 @@ def primal_bytes(form, elem: int) -> float:
     b += n * elem                             # xbar write
     b += 2 * n * 4                            # FP32 partial sum read + write
     return float(b)
+
+
+def pkern(eng) -> str:
+    """Name of the primal-step kernel the engine's tuning launches."""
+    return "primal_sell_kernel" if eng.tune.sell_primal else "primal_kernel"
 
 
 def dual_bytes(form, elem: int) -> float:
@@ -608,7 +619,8 @@ def run_transport(args, sources=1000, sinks=2000, cpu=True):
     obj_ref = highs_objective(p) if not args.no_highs else None
     us_p = eng.time_kernel("primal", eng.tune, reps=50)
     us_d = eng.time_kernel("dual", eng.tune, reps=50)
-    bp, bd = primal_bytes(form, 4), dual_bytes(form, 4)
+    bp = primal_bytes(form, 4, int(eng.t_sell_ptr[-1].item()) if eng.tune.sell_primal else 0)
+    bd = dual_bytes(form, 4)
     gp, gd = bp / (us_p * 1e-6) / 1e9, bd / (us_d * 1e-6) / 1e9
     spmv = spmv_rates(eng)
     log(f"primal step {us_p:.2f} us ({gp:.0f} GB/s), dual step {us_d:.2f} us ({gd:.0f} GB/s); SpMV K^T y "
@@ -624,9 +636,11 @@ def run_transport(args, sources=1000, sinks=2000, cpu=True):
         "rel_kkt": results[-1]["rel_kkt"], "objective": float(obj),
         "objective_rel_err_vs_fp64": None if obj_ref is None else abs(obj - obj_ref) / (1 + abs(obj_ref)),
         "host_to_host": h2h, "tune": tune.__dict__, "cadence": cadence,
-        "roofline": {"bound": "hbm", "kernel": "primal_kernel (fused K^T y + step + projection + sums)",
+        "roofline": {"bound": "hbm",
+                     "kernel": ("primal_sell_kernel (SELL-64 copy of K^T" if eng.tune.sell_primal else
+                                "primal_kernel (CSR of K^T") + "; fused K^T y + step + projection + sums)",
                      "achieved": gp, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gp / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(f"{key}/primal_kernel"), "bytes_per_launch": bp,
+                     "traffic": measured_traffic(f"{key}/{pkern(eng)}"), "bytes_per_launch": bp,
                      "us_per_launch": us_p,
                      "dual_kernel": {"achieved": gd, "frac": gd / HBM_PEAK_GBS, "bytes_per_launch": bd,
                                      "us_per_launch": us_d, "traffic": measured_traffic(f"{key}/dual_kernel")},
@@ -905,7 +919,8 @@ def run_beyond_mall(args, sources=4000, sinks=8000):
     log(f"transport {sources}x{sinks} (n={form.n}, nnz={form.nnz}) built and tuned in {time.perf_counter() - t0:.1f}s")
     us_p = eng.time_kernel("primal", eng.tune, reps=10)
     us_d = eng.time_kernel("dual", eng.tune, reps=10)
-    bp, bd = primal_bytes(form, 4), dual_bytes(form, 4)
+    bp = primal_bytes(form, 4, int(eng.t_sell_ptr[-1].item()) if eng.tune.sell_primal else 0)
+    bd = dual_bytes(form, 4)
     gp, gd = bp / (us_p * 1e-6) / 1e9, bd / (us_d * 1e-6) / 1e9
     key = f"transport_{sources}x{sinks}"
     spmv = spmv_rates(eng, reps=10, kx_tpr=64, traffic_key=key)
@@ -916,8 +931,8 @@ def run_beyond_mall(args, sources=4000, sinks=8000):
                     f"0.5 GB each, working set of a step ~{bp / 1e9:.2f} GB > 256 MB Infinity Cache",
         "clock": "HIP graphs of back-to-back launches at the cold-start state, events on the engine stream",
         "tune": tune.__dict__,
-        "primal_kernel": {"achieved": gp, "frac": gp / HBM_PEAK_GBS, "bytes_per_launch": bp, "us_per_launch": us_p,
-                          "traffic": measured_traffic(f"{key}/primal_kernel")},
+        "primal_kernel": {"kernel": pkern(eng), "achieved": gp, "frac": gp / HBM_PEAK_GBS, "bytes_per_launch": bp,
+                          "us_per_launch": us_p, "traffic": measured_traffic(f"{key}/{pkern(eng)}")},
         "dual_kernel": {"achieved": gd, "frac": gd / HBM_PEAK_GBS, "bytes_per_launch": bd, "us_per_launch": us_d,
                         "traffic": measured_traffic(f"{key}/dual_kernel")},
         "spmv_standalone": spmv,

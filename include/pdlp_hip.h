@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PDLP_ABI_VERSION 11
+#define PDLP_ABI_VERSION 12
 
 /* precision of iterates, matrix values and problem vectors */
 #define PDLP_F32 0
@@ -167,6 +167,17 @@ typedef struct PdlpDev {
        The distances of the restart criteria stay in the scaled space.  NULL = unscaled. */
     const double *row_scale;  /* m: D_r */
     const double *col_scale;  /* n: D_c */
+    /* optional SELL-64 copy of K^T for the primal step (PdlpTune.sell_primal): slice s
+       holds columns 64 s .. 64 s + 63 in their natural order, one per lane, its entries
+       column-major (entry k of lane l at t_sell_ptr[s] + 64 k + l; padding value 0,
+       index 0), so no column pointer is read and every load of an entry is a whole
+       line; summed in CSR order, it gives the one-lane CSR kernel's result
+       (built by pdlp_csr_to_sell64 from the K^T CSR) */
+    int32_t t_sell_slices;    /* ceil(n / 64), or 0: no SELL copy                   */
+    int32_t t_sell_reserved;
+    const int32_t *t_sell_ptr;  /* t_sell_slices + 1                                 */
+    const int32_t *t_sell_idx;  /* t_sell_ptr[t_sell_slices]: rows of K (padding 0)   */
+    const void *t_sell_val;     /* same length, prec (padding 0)                      */
 } PdlpDev;
 
 /* Launch shape, chosen by the autotuner (sovereign_solver_amd.engine.tune). */
@@ -193,10 +204,21 @@ typedef struct PdlpTune {
                              per k entries of a slice, partial sums added in order */
     int32_t no_ray;       /* 1: skip the unboundedness (ray of decrease) test of the
                              adaptive checks; 0: test at every check */
+    int32_t sell_primal;  /* 1: primal step over the SELL-64 copy of K^T (PdlpDev.t_sell_*),
+                             one lane per column, `block` threads per workgroup; 0: CSR */
 } PdlpTune;
 
 int         pdlp_abi_version(void);
 const char* pdlp_last_error(void);
+
+/* SELL-64 copy of a CSR matrix's rows (the K^T of PdlpDev.t_sell_*): slice s holds
+ * rows 64 s .. 64 s + 63, its width the longest of them; entry k of row 64 s + l goes
+ * to sell_ptr[s] + 64 k + l.  sell_ptr (slices + 1, slices = ceil(rows / 64)) is the
+ * caller's: sell_ptr[s + 1] - sell_ptr[s] = 64 * width.  sell_idx / sell_val must be
+ * zero-filled beforehand (the padding).  No reference counterpart (the reference
+ * keeps CSR; pdhg_gpu.py:137,161). */
+int pdlp_csr_to_sell64(int32_t prec, int32_t rows, const int32_t* row_ptr, const int32_t* col_idx,
+                       const void* val, const int32_t* sell_ptr, int32_t* sell_idx, void* sell_val, void* stream);
 
 /* Device CSR transpose for problem ingestion: K^T (cols x rows) from K
  * (rows x cols, CSR with ascending column indices per row), values of `prec`.

@@ -91,12 +91,49 @@ int layout(int32_t cols, int32_t nnz, Layout& l) {
     return 0;
 }
 
+// Row r's entries into SELL-64 slot order: entry k to sell_ptr[r / 64] + 64 k + r % 64.
+template <typename T>
+__global__ void __launch_bounds__(256) csr_to_sell64_kernel(int rows, const int32_t* __restrict__ row_ptr,
+                                                            const int32_t* __restrict__ col_idx,
+                                                            const T* __restrict__ val,
+                                                            const int32_t* __restrict__ sell_ptr,
+                                                            int32_t* __restrict__ sell_idx, T* __restrict__ sell_val) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const int beg = row_ptr[r], end = row_ptr[r + 1];
+    const int base = sell_ptr[r >> 6] + (r & 63), lim = sell_ptr[(r >> 6) + 1];
+    for (int e = beg; e < end; ++e) {
+        const int q = base + 64 * (e - beg);
+        if (q >= lim) break;                 // a slice narrower than its rows: never past the slice
+        sell_idx[q] = col_idx[e];
+        sell_val[q] = val[e];
+    }
+}
+
 }  // namespace
 }  // namespace pdlp
 
 using namespace pdlp;
 
 extern "C" {
+
+int pdlp_csr_to_sell64(int32_t prec, int32_t rows, const int32_t* row_ptr, const int32_t* col_idx, const void* val,
+                       const int32_t* sell_ptr, int32_t* sell_idx, void* sell_val, void* stream) {
+    if (prec != PDLP_F32 && prec != PDLP_F64) return fail(PDLP_E_ARG, "csr_to_sell64: prec must be PDLP_F32 or PDLP_F64");
+    if (rows < 0) return fail(PDLP_E_SHAPE, "csr_to_sell64: rows >= 0");
+    if (rows == 0) return 0;
+    if (!row_ptr || !col_idx || !val || !sell_ptr || !sell_idx || !sell_val)
+        return fail(PDLP_E_ARG, "csr_to_sell64: NULL pointer");
+    const unsigned g = (unsigned)((rows + 255) / 256);
+    hipStream_t s = as_stream(stream);
+    if (prec == PDLP_F32)
+        csr_to_sell64_kernel<float><<<g, 256, 0, s>>>(rows, row_ptr, col_idx, (const float*)val, sell_ptr, sell_idx,
+                                                      (float*)sell_val);
+    else
+        csr_to_sell64_kernel<double><<<g, 256, 0, s>>>(rows, row_ptr, col_idx, (const double*)val, sell_ptr,
+                                                       sell_idx, (double*)sell_val);
+    return hip_fail(hipGetLastError(), "csr_to_sell64_kernel launch");
+}
 
 int pdlp_csr_transpose_work_bytes(int32_t rows, int32_t cols, int32_t nnz, size_t* bytes) {
     if (bytes == nullptr) return fail(PDLP_E_ARG, "bytes is NULL");

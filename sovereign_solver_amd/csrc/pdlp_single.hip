@@ -97,6 +97,68 @@ __global__ void __launch_bounds__(1024) primal_kernel(PdlpDev d, int remap) {
     else d.x_sum[j] = psd + (double)xn;
 }
 
+// Primal step over the SELL-64 copy of K^T: one lane per column, no column
+// pointer to read; slice s = 64 columns, entries column-major, so every entry
+// load is a whole line.  The column's entries are summed in CSR order (padding
+// adds zeros), which is the one-lane CSR kernel's arithmetic.
+template <typename T>
+__global__ void __launch_bounds__(1024) primal_sell_kernel(PdlpDev d, int remap) {
+    const PdlpCtrl* ctrl = d.ctrl;
+    if (ctrl->status != PDLP_RUNNING) return;
+    const T tau_scale = (T)ctrl->tau_scale;
+    const int j = block_index(remap) * blockDim.x + threadIdx.x;
+    const int s = __builtin_amdgcn_readfirstlane(j >> 6);
+    if (s >= d.t_sell_slices) return;               // whole wavefronts leave together
+    const bool live = j < d.n;
+    const int jj = live ? j : d.n - 1;
+    const T* __restrict__ y = (const T*)d.y;
+    const T* c = (const T*)d.c;
+    const T* lo = (const T*)d.lo;
+    const T* hi = (const T*)d.hi;
+    // the column's own data first, in flight with the entries
+    const T xj = ((const T*)d.x)[jj], cj = c[jj];
+    const T lj = lo != nullptr ? lo[jj] : T(0), hj = hi != nullptr ? hi[jj] : T(0);
+    const float psf = d.x_psum != nullptr ? d.x_psum[jj] : 0.f;
+    const double psd = d.x_psum == nullptr ? d.x_sum[jj] : 0.0;
+    const int beg = __builtin_amdgcn_readfirstlane(d.t_sell_ptr[s]);
+    const int width = __builtin_amdgcn_readfirstlane((d.t_sell_ptr[s + 1] - beg) >> 6);
+    const int32_t* __restrict__ idx = d.t_sell_idx + beg + (j & 63);
+    const T* __restrict__ val = (const T*)d.t_sell_val + beg + (j & 63);
+    T aty = T(0), asum = T(0);
+    int k = 0;
+    for (; k + kRowBatch <= width; k += kRowBatch) {
+        T a[kRowBatch];
+        int r[kRowBatch];
+#pragma unroll
+        for (int u = 0; u < kRowBatch; ++u) {
+            a[u] = val[64 * (k + u)];
+            r[u] = idx[64 * (k + u)];
+        }
+#pragma unroll
+        for (int u = 0; u < kRowBatch; ++u) {
+            aty += a[u] * y[r[u]];
+            asum += fabs(a[u]);
+        }
+    }
+    for (; k < width; ++k) {
+        const T a = val[64 * k];
+        aty += a * y[idx[64 * k]];
+        asum += fabs(a);
+    }
+    if (!live) return;
+    const T colsum = asum > T(1e-4) ? asum : T(1e-4);
+    T tau = T(0.999) / colsum;
+    if (tau_scale != T(1)) tau *= tau_scale;
+    T xn = xj - tau * (cj - aty);
+    if (lo != nullptr) xn = xn < lj ? lj : xn;
+    else xn = xn < T(0) ? T(0) : xn;
+    if (hi != nullptr) xn = xn > hj ? hj : xn;
+    ((T*)d.xbar)[j] = T(2) * xn - xj;
+    ((T*)d.x)[j] = xn;
+    if (d.x_psum != nullptr) d.x_psum[j] = psf + (float)xn;
+    else d.x_sum[j] = psd + (double)xn;
+}
+
 // ------------------------------------------------------------------ dual step
 // Dual update of row i from ax = K_i xbar and asum = sum_j |K_ij|.
 template <typename T, bool EXACT>
@@ -742,6 +804,9 @@ int check_dev(const PdlpDev* d) {
                                (d->n_csr_rows > 0 && !d->csr_rows) || !d->sell_part || !d->sell_asum ||
                                d->sell_max_width <= 0))
         return fail(PDLP_E_ARG, "SELL slices need sell_ptr/rows/col/val/part/asum, csr_rows and sell_max_width");
+    if (d->t_sell_slices != 0 && (d->t_sell_slices != (d->n + 63) / 64 || !d->t_sell_ptr || !d->t_sell_idx ||
+                                  !d->t_sell_val))
+        return fail(PDLP_E_ARG, "the SELL copy of K^T needs ceil(n / 64) slices and t_sell_ptr/idx/val");
     return 0;
 }
 
@@ -762,6 +827,8 @@ int check_tune(const PdlpTune* t) {
         return fail(PDLP_E_TUNE, "sell must be 0..7 or a multiple of 8");
     if (t->exact && t->scheme != PDLP_SCHEME_REFERENCE)
         return fail(PDLP_E_TUNE, "exact mode reproduces the reference scheme only");
+    if (t->sell_primal != 0 && t->sell_primal != 1) return fail(PDLP_E_TUNE, "sell_primal must be 0 or 1");
+    if (t->sell_primal && t->exact) return fail(PDLP_E_TUNE, "exact mode keeps the CSR primal step");
     return 0;
 }
 
@@ -770,6 +837,8 @@ int check_all(const PdlpDev* d, const PdlpTune* t) {
     if (rc || (rc = check_tune(t))) return rc;
     if (t->scheme != PDLP_SCHEME_REFERENCE && (d->x_last == nullptr || d->y_last == nullptr))
         return fail(PDLP_E_ARG, "the adaptive scheme needs x_last and y_last");
+    if (t->sell_primal && d->t_sell_slices == 0)
+        return fail(PDLP_E_TUNE, "sell_primal needs the SELL copy of K^T (PdlpDev.t_sell_*)");
     return 0;
 }
 
@@ -793,6 +862,13 @@ template <typename T, bool EXACT>
 int launch_primal_t(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
     const int remap = t.xcd_remap & 1;
     unsigned g, nt;
+    if constexpr (!EXACT) {
+        if (t.sell_primal) {                          // one lane per column over the SELL-64 slices
+            shape_for(d.n, 1, t.block, remap, g, nt);
+            primal_sell_kernel<T><<<g, nt, 0, s>>>(d, remap);
+            return hip_fail(hipGetLastError(), "primal_sell_kernel launch");
+        }
+    }
     shape_for(d.n, t.tpr_primal, t.block, remap, g, nt);
     if constexpr (EXACT) {
         primal_kernel<T, 1, true><<<g, nt, 0, s>>>(d, remap);

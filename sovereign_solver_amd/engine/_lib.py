@@ -13,7 +13,7 @@ import threading
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "_lib", "libpdlp_hip.so")
 
-PDLP_ABI_VERSION = 11
+PDLP_ABI_VERSION = 12
 PDLP_F32, PDLP_F64 = 0, 1
 PDLP_E_ARG, PDLP_E_TUNE, PDLP_E_SHAPE, PDLP_E_LIMIT = -1, -2, -3, -4
 PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL, PDLP_INFEASIBLE, PDLP_UNBOUNDED, PDLP_SKIPPED = 0, 1, 3, 4, 5, 6
@@ -57,6 +57,8 @@ class PdlpDev(C.Structure):
         ("sell_col", _p), ("sell_val", _p), ("csr_rows", _p), ("sell_max_width", C.c_int32),
         ("sell_reserved", C.c_int32), ("sell_part", _p), ("sell_asum", _p),
         ("row_scale", _p), ("col_scale", _p),
+        ("t_sell_slices", C.c_int32), ("t_sell_reserved", C.c_int32), ("t_sell_ptr", _p), ("t_sell_idx", _p),
+        ("t_sell_val", _p),
     ]
 
 
@@ -64,7 +66,7 @@ class PdlpTune(C.Structure):
     _fields_ = [
         ("tpr_primal", C.c_int32), ("tpr_dual", C.c_int32), ("block", C.c_int32),
         ("check_every", C.c_int32), ("xcd_remap", C.c_int32), ("exact", C.c_int32), ("block_dual", C.c_int32),
-        ("scheme", C.c_int32), ("sell", C.c_int32), ("no_ray", C.c_int32),
+        ("scheme", C.c_int32), ("sell", C.c_int32), ("no_ray", C.c_int32), ("sell_primal", C.c_int32),
     ]
 
 
@@ -177,6 +179,7 @@ _SIGS = {
     "pdlp_csr_transpose": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, _p, _p, _p, _p, _p, _p, _p,
                                      C.c_size_t, _p]),
     "pdlp_spmv_csr": (C.c_int, [C.c_int32, _p, _p, _p, _p, _p, C.c_int32, C.c_int32, _p]),
+    "pdlp_csr_to_sell64": (C.c_int, [C.c_int32, C.c_int32, _p, _p, _p, _p, _p, _p, _p]),
     "pdlp_spmv_csr_f64acc": (C.c_int, [_p, _p, _p, _p, _p, C.c_int32, C.c_int32, _p]),
     "pdlp_init": (C.c_int, [C.POINTER(PdlpDev), C.POINTER(PdlpTune), _p]),
     "pdlp_primal_step": (C.c_int, [C.POINTER(PdlpDev), C.POINTER(PdlpTune), _p]),

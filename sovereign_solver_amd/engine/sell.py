@@ -98,3 +98,31 @@ def build(K: sp.csr_matrix, line_elems: int = 32, max_lines: float = 4.0, max_pa
                           np.arange(m, dtype=np.int32))
     return SellSlices(np.asarray(ptr, np.int32), np.concatenate(rows_out), np.concatenate(col_out),
                       np.concatenate(val_out), np.flatnonzero(~in_slice).astype(np.int32))
+
+
+# Largest padding (entries of the SELL copy beyond nnz, relative to nnz) for which
+# the primal step's SELL copy of K^T is built.
+COLUMN_PADDING_MAX = 0.125
+
+
+def build_columns(K: sp.csr_matrix, padding_max=None):
+    """Slice pointers of a SELL-64 copy of K^T (K's columns, natural order): slice s holds
+    columns 64 s .. 64 s + 63 and is as wide as the longest of them; the copy's entries
+    are filled on the device (``pdlp_csr_to_sell64``).  None when the padding would exceed
+    ``padding_max`` (default :data:`COLUMN_PADDING_MAX`) of K's entries, when K has no
+    entries, or beyond int32 indexing."""
+    if padding_max is None:
+        padding_max = COLUMN_PADDING_MAX
+    n = K.shape[1]
+    if K.nnz == 0 or n == 0:
+        return None
+    lens = np.bincount(K.indices, minlength=n)
+    slices = -(-n // SLICE)
+    padded = np.zeros(slices * SLICE, np.int64)
+    padded[:n] = lens
+    width = padded.reshape(slices, SLICE).max(axis=1)
+    ptr = np.zeros(slices + 1, np.int64)
+    ptr[1:] = np.cumsum(SLICE * width)
+    if ptr[-1] >= 2**31 or ptr[-1] > (1.0 + padding_max) * K.nnz:
+        return None
+    return ptr.astype(np.int32)

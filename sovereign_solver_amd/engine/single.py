@@ -116,6 +116,20 @@ class SingleEngine:
                 self.sell_part = self.sell_asum = None
                 self.n_csr_rows = 0
                 self.sell_max_width = 0
+            # SELL-64 copy of K^T for the primal step (columns in natural order), when the
+            # columns of each slice have nearly equal lengths (padding at most 1/8 of K)
+            self.t_sell_slices = 0
+            self.t_sell_ptr = self.t_sell_idx = self.t_sell_val = None
+            tp = None if exact else sell.build_columns(K)
+            if tp is not None:
+                self.t_sell_slices = len(tp) - 1
+                self.t_sell_ptr = dev_tensor(tp, i32, dv)
+                self.t_sell_idx = torch.zeros(int(tp[-1]), dtype=i32, device=dv)
+                self.t_sell_val = torch.zeros(int(tp[-1]), dtype=T, device=dv)
+                L.check(self.lib.pdlp_csr_to_sell64(L.PDLP_F32 if precision == "fp32" else L.PDLP_F64, self.n,
+                                                    ptr(self.t_row_ptr), ptr(self.t_col_idx), ptr(self.t_val),
+                                                    ptr(self.t_sell_ptr), ptr(self.t_sell_idx), ptr(self.t_sell_val),
+                                                    stream_handle(self.stream)), "pdlp_csr_to_sell64")
         self.ctrl_host = torch.zeros((2, C.sizeof(L.PdlpCtrl)), dtype=torch.uint8).pin_memory()
         self.stream.synchronize()
         self.dev = L.PdlpDev(
@@ -130,7 +144,9 @@ class SingleEngine:
             sell_slices=self.sell_slices, n_csr_rows=self.n_csr_rows, sell_ptr=ptr(self.sell_ptr),
             sell_rows=ptr(self.sell_rows), sell_col=ptr(self.sell_col), sell_val=ptr(self.sell_val),
             csr_rows=ptr(self.csr_rows), sell_max_width=self.sell_max_width, sell_part=ptr(self.sell_part),
-            sell_asum=ptr(self.sell_asum), row_scale=ptr(self.row_scale), col_scale=ptr(self.col_scale))
+            sell_asum=ptr(self.sell_asum), row_scale=ptr(self.row_scale), col_scale=ptr(self.col_scale),
+            t_sell_slices=self.t_sell_slices, t_sell_ptr=ptr(self.t_sell_ptr), t_sell_idx=ptr(self.t_sell_idx),
+            t_sell_val=ptr(self.t_sell_val))
         self._graphs = {}
         self._graph_tune = None
 

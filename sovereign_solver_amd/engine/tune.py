@@ -56,11 +56,12 @@ class Tune:
                                 # workgroup per CSR row and per slice piece (threads/rows: 2 1024/64, 3 512/64,
                                 # 4 1024/16, 5 1024/8, 6 1024/32, 7 512/16), k >= 8 a wavefront per k entries
     no_ray: int = 0             # 1: the adaptive checks skip the unboundedness test
+    sell_primal: int = 0        # 1: primal step over the SELL-64 copy of K^T (one lane per column)
 
     def to_c(self):
         from ._lib import PdlpTune
         return PdlpTune(self.tpr_primal, self.tpr_dual, self.block, self.check_every, self.xcd_remap, self.exact,
-                        self.block_dual, self.scheme, self.sell, self.no_ray)
+                        self.block_dual, self.scheme, self.sell, self.no_ray, self.sell_primal)
 
     def with_(self, **kw) -> "Tune":
         d = asdict(self)
@@ -162,6 +163,16 @@ def autotune(engine, reps: int = 20, use_cache: bool = True) -> Tune:
             timings[f"primal/tpr{tp}/remap{remap}"] = us
             if us < best_pt:
                 best_p, best_pt = (tp, remap), us
+    # the SELL-64 copy of K^T (one lane per column, no column pointers), when the engine has one
+    sell_primal, block_p = 0, 256
+    if getattr(engine, "t_sell_slices", 0) > 0:
+        for bp in (256, 512, 1024):
+            for remap in (0, 1):
+                t = Tune(1, base.tpr_dual, bp, base.check_every, remap, 0, sell_primal=1)
+                us = engine.time_kernel("primal", t, reps)
+                timings[f"primal/sell/block{bp}/remap{remap}"] = us
+                if us < best_pt:
+                    best_p, best_pt, sell_primal, block_p = (1, remap), us, 1, bp
     best_d, best_dt = None, float("inf")
     for td in _neighbours(base.tpr_dual):
         # several rows per workgroup share gathered cache lines in L1 when rows are short enough
@@ -184,7 +195,9 @@ def autotune(engine, reps: int = 20, use_cache: bool = True) -> Tune:
                     timings[f"dual/sell{sv}/tpr{td}/remap{remap}"] = us
                     if us < best_dt:
                         best_d, best_dt, sell = (td, remap, 0), us, sv
-    t = Tune(best_p[0], best_d[0], 256, base.check_every, best_p[1] | (best_d[1] << 1), 0, best_d[2], 0, sell)
+    # `block` is the primal step's (the dual step has block_dual, 0 = 256 here)
+    t = Tune(best_p[0], best_d[0], block_p, base.check_every, best_p[1] | (best_d[1] << 1), 0,
+             best_d[2] or (256 if block_p != 256 and best_d[0] <= 64 else 0), 0, sell, sell_primal=sell_primal)
     _store_cache(key, t, timings)
     return t.with_(scheme=engine.tune.scheme)
 

@@ -42,9 +42,10 @@ def test_every_declared_symbol_exported_and_bound():
 def test_struct_sizes_match_header():
     # PdlpCtrl: 8 int32 + 12 double + (double, 2 int32, double); PdlpDev: 4 int32 + 27 pointers; PdlpTune: 8 int32
     assert C.sizeof(L.PdlpCtrl) == 8 * 4 + 13 * 8 + 2 * 4 + 8
-    # ... then the SELL-64 block (2 int32, 5 pointers, 2 int32, 2 pointers) and the two scale vectors
-    assert C.sizeof(L.PdlpDev) == 4 * 4 + 27 * 8 + 2 * 4 + 5 * 8 + 2 * 4 + 2 * 8 + 2 * 8
-    assert C.sizeof(L.PdlpTune) == 10 * 4
+    # ... then the SELL-64 block (2 int32, 5 pointers, 2 int32, 2 pointers), the two scale vectors
+    # and the SELL-64 copy of K^T (2 int32, 3 pointers)
+    assert C.sizeof(L.PdlpDev) == 4 * 4 + 27 * 8 + 2 * 4 + 5 * 8 + 2 * 4 + 2 * 8 + 2 * 8 + 2 * 4 + 3 * 8
+    assert C.sizeof(L.PdlpTune) == 11 * 4
     # PdlpGemm: 10 int32 (M..accumulate), alpha, split_k, 7 pointers (A, B, bias, mask, C, Cb, work)
     assert C.sizeof(L.PdlpGemm) == 10 * 4 + 4 + 4 + 7 * 8
     # PdlpMlp: 4 int32, 23 pointers, then the backward work space (pointer, int64), w2b, w3b
@@ -164,7 +165,32 @@ def test_noalloc_rejects_oversized_sell_scratch_without_device():
     # the size check comes before any device call
     lib = L.lib()
     d = L.PdlpDev(m=1, n=1, nnz=1, prec=L.PDLP_F32, sell_slices=4, sell_max_width=1 << 20)
-    t = L.PdlpTune(1, 1, 256, 50, 0, 0, 0, 1, 8, 0)
+    t = L.PdlpTune(1, 1, 256, 50, 0, 0, 0, 1, 8, 0, 0)
     assert lib.pdlp_run_noalloc(C.byref(d), C.byref(t), 1, 1, None) == L.PDLP_E_LIMIT
     assert b"scratch" in lib.pdlp_last_error()
     assert lib.pdlp_run_noalloc(None, C.byref(t), 1, 1, None) == L.PDLP_E_ARG
+
+
+def test_sell_columns_layout_and_padding_limit():
+    import numpy as np
+    import scipy.sparse as sp
+    from sovereign_solver_amd.engine.sell import build_columns
+    # 3 rows x 130 columns: column j has j % 3 + 1 entries except the last two (empty)
+    rows, cols = [], []
+    for j in range(128):
+        for i in range(j % 3 + 1):
+            rows.append(i)
+            cols.append(j)
+    K = sp.csr_matrix((np.ones(len(rows)), (rows, cols)), shape=(3, 130))
+    ptr = build_columns(K, padding_max=10.0)
+    # three slices (64, 64, 2 columns) of widths 3, 3, 0 entries per lane
+    assert ptr.tolist() == [0, 192, 384, 384]
+    assert build_columns(K, padding_max=0.5) is None              # 384 > 1.5 * 256 entries
+    assert build_columns(sp.csr_matrix((3, 5))) is None
+
+
+def test_csr_to_sell64_rejects_bad_arguments_without_device():
+    lib = L.lib()
+    assert lib.pdlp_csr_to_sell64(9, 4, None, None, None, None, None, None, None) == L.PDLP_E_ARG
+    assert lib.pdlp_csr_to_sell64(L.PDLP_F32, -1, None, None, None, None, None, None, None) == L.PDLP_E_SHAPE
+    assert lib.pdlp_csr_to_sell64(L.PDLP_F32, 4, None, None, None, None, None, None, None) == L.PDLP_E_ARG

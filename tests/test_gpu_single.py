@@ -288,3 +288,37 @@ def test_split_sell_dual_step_agrees(gpu, kc):
     assert yardstick.rel_kkt(p, b["x"], b["y"])[0] <= 1.02e-4
     ref = _highs_obj(p)
     assert_objective_parity(p, float(p.c @ b["x"]), ref=ref, x=b["x"], y=b["y"])
+
+
+@pytest.mark.parametrize("lp", ["transport", "random"])
+def test_sell_primal_step_same_bits_as_csr(gpu, lp, monkeypatch):
+    """The primal step over the SELL-64 copy of K^T (one lane per column) sums each
+    column in CSR order, so whole solves give the bits of the one-lane CSR kernel:
+    a transportation LP (every column 2 entries, no padding) and a random LP whose
+    unequal columns pad the slices (the padding limit lifted to build it)."""
+    from sovereign_solver_amd.engine import sell as SL
+    from sovereign_solver_amd.engine.forms import box_form
+    from sovereign_solver_amd.engine.single import SingleEngine
+    from sovereign_solver_amd.engine.tune import Tune
+    monkeypatch.setattr(SL, "COLUMN_PADDING_MAX", 100.0)
+    p = transportation_lp(60, 200, seed=5) if lp == "transport" else random_sparse_lp(150, 400, 0.03, seed=7)
+    f = box_form(p)
+    base = Tune(tpr_primal=1, tpr_dual=8, block=256, check_every=50, scheme=1)
+    runs = []
+    for sp in (0, 1):
+        eng = SingleEngine(f, "fp32", tune=base.with_(sell_primal=sp))
+        assert eng.t_sell_slices == -(-f.n // 64)
+        r = eng.solve(1e-4, 50_000)
+        runs.append(r)
+        eng.drop_graphs()
+    a, b = runs
+    assert a["status"] == b["status"] == "optimal" and a["iterations"] == b["iterations"]
+    assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["y"], b["y"])
+    e = yardstick.rel_kkt(p, *extract_xy(f, p, b))[0]
+    assert e <= 1.02e-4
+
+
+def extract_xy(f, p, r):
+    from sovereign_solver_amd.engine.forms import extract
+    x, _, y = extract(f, p, r["x"], r["y"])
+    return x, y
