@@ -124,9 +124,11 @@ class PackedBatch:
             if key not in uniq:
                 K = p.A_eq if p.n_ineq == 0 else (p.A_ub if p.n_eq == 0 else
                                                   sp.vstack([p.A_ub, p.A_eq], format="csr"))
-                K = sp.csr_matrix(K, dtype=np.float64)
-                K.sum_duplicates()
-                K.sort_indices()
+                if not (sp.isspmatrix_csr(K) and K.dtype == np.float64):
+                    K = sp.csr_matrix(K, dtype=np.float64)
+                if not K.has_canonical_format:       # sums duplicates and sorts each row's indices
+                    K = K.copy()
+                    K.sum_duplicates()
                 uniq[key] = len(stored)
                 stored.append(K)
             mat_of[k] = uniq[key]
@@ -161,18 +163,29 @@ class PackedBatch:
             stored_orig, stored = stored, [S.scale_matrix(K, dr, dc) for K, (dr, dc) in zip(stored, sc)]
         else:
             stored_orig = stored
-        self.row_ptr = np.concatenate([K.indptr.astype(np.int32) for K in stored])
-        self.col_idx = np.concatenate([K.indices.astype(np.uint16) for K in stored])
+        self.row_ptr = np.concatenate([K.indptr for K in stored]).astype(np.int32)
+        col_local = np.concatenate([K.indices for K in stored])
+        self.col_idx = col_local.astype(np.uint16)
         self.val = np.concatenate([K.data for K in stored])
-        # K^T by rows (K by columns), same entry count per matrix
-        stored_t = [K.T.tocsr() for K in stored]
-        for Kt in stored_t:
-            Kt.sort_indices()
+        # K^T by rows (K by columns) of every stored matrix at once: one transpose of their
+        # block-diagonal union (SciPy's counting sort; each column's entries stay in row order)
+        s_ro = np.concatenate([[0], np.cumsum(s_m)[:-1]])        # first row / column of each matrix
+        s_co = np.concatenate([[0], np.cumsum(s_n)[:-1]])
+        gptr = np.concatenate([np.concatenate([K.indptr[:-1] + z for K, z in zip(stored, s_nz)]),
+                               [int(s_z.sum())]]).astype(np.int32)
+        gcol = col_local.astype(np.int32)
+        gcol += np.repeat(s_co.astype(np.int32), s_z)
+        BT = sp.csr_matrix((self.val, gcol, gptr), shape=(int(s_m.sum()), int(s_n.sum()))).T.tocsr()
+        col_mat = np.repeat(np.arange(len(stored)), s_n)            # the matrix of each global column
+        rows = BT.indices.astype(np.int32, copy=False)
+        rows -= np.repeat(s_ro[col_mat].astype(np.int32), np.diff(BT.indptr))
+        self.t_row_idx = rows.astype(np.uint16)
+        self.t_val = BT.data
+        # per matrix its n + 1 column pointers, relative to its first entry
+        first = np.repeat(s_co, s_n + 1) + np.concatenate([np.arange(n + 1) for n in s_n])
+        self.t_col_ptr = (BT.indptr[first] - np.repeat(BT.indptr[s_co], s_n + 1)).astype(np.int32)
         s_cp = np.concatenate([[0], np.cumsum(s_n + 1)[:-1]])
         self.cp_off = s_cp[mat_of]
-        self.t_col_ptr = np.concatenate([Kt.indptr.astype(np.int32) for Kt in stored_t])
-        self.t_row_idx = np.concatenate([Kt.indices.astype(np.uint16) for Kt in stored_t])
-        self.t_val = np.concatenate([Kt.data for Kt in stored_t])
 
         P = self.problems
         c = np.concatenate([np.asarray(p.c, float) for p in P])
