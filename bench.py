@@ -622,7 +622,7 @@ def run_transport(args, sources=1000, sinks=2000, cpu=True):
     bp = primal_bytes(form, 4, int(eng.t_sell_ptr[-1].item()) if eng.tune.sell_primal else 0)
     bd = dual_bytes(form, 4)
     gp, gd = bp / (us_p * 1e-6) / 1e9, bd / (us_d * 1e-6) / 1e9
-    spmv = spmv_rates(eng)
+    spmv = spmv_rates(eng, traffic_key=f"transport_{sources}x{sinks}")
     log(f"primal step {us_p:.2f} us ({gp:.0f} GB/s), dual step {us_d:.2f} us ({gd:.0f} GB/s); SpMV K^T y "
         f"{spmv['KT_y']['achieved_gbs']:.0f}, K x {spmv['K_x']['achieved_gbs']:.0f} GB/s")
     key = f"transport_{sources}x{sinks}"
