@@ -242,10 +242,17 @@ class DispatchFamily:
         cum = np.cumsum(pm, axis=1)
         cheaper = np.empty_like(c)
         np.put_along_axis(cheaper, order, cum - pm, axis=1)
-        marg = np.minimum((cum[:, None, :] < d[:, :, None]).sum(axis=2), G - 1)       # [B, T]
-        price = np.take_along_axis(np.take_along_axis(c, order, axis=1), marg, axis=1)
-        xm = np.clip(d[:, :, None] - cheaper[:, None, :], 0.0, self.pmax) / self.pmax  # [B, T, G]
         v = np.empty((B, L), np.float32)
+        # [B, T, G] terms in blocks of instances (the temporaries stay in cache; same values)
+        marg = np.empty((B, T), np.int64)
+        for s in range(0, B, 32):
+            e = min(B, s + 32)
+            marg[s:e] = np.minimum((cum[s:e, None, :] < d[s:e, :, None]).sum(axis=2), G - 1)
+            t = d[s:e, :, None] - cheaper[s:e, None, :]
+            np.clip(t, 0.0, self.pmax, out=t)
+            t /= self.pmax
+            v[s:e, o["X"]:] = t.reshape(e - s, T * G)
+        price = np.take_along_axis(np.take_along_axis(c, order, axis=1), marg, axis=1)
         v[:, o["D"]:o["D"] + T] = 4.0 * d / S
         v[:, o["C"]:o["C"] + G] = c / 60.0
         v[:, o["U"]:o["U"] + G] = c / self.cost - 1.0
@@ -253,7 +260,6 @@ class DispatchFamily:
         v[:, o["CC2"]:o["CC2"] + G] = 2.0 * (cheaper + self.pmax) / S
         v[:, o["P"]:o["P"] + T] = price / 60.0
         v[:, o["M"]:o["M"] + T] = d / self.demand - 1.0
-        v[:, o["X"]:] = xm.reshape(B, T * G)
         return v
 
     def node_inputs(self, ids) -> np.ndarray:
