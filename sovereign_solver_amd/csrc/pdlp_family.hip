@@ -67,6 +67,23 @@ __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 
+// d += val[q] * v(LP lane, idx[q]) for the S entries q0 .. q0 + S - 1, in ascending q
+template <int S>
+__device__ __forceinline__ void tail_dot(const int32_t* __restrict__ idx, const float* __restrict__ val, int q0,
+                                         const float* __restrict__ v, int tile, int len, int lane, float& d) {
+    float g[S], w[S];
+    int c[S];
+#pragma unroll
+    for (int u = 0; u < S; ++u) {
+        c[u] = idx[q0 + u];
+        w[u] = val[q0 + u];
+    }
+#pragma unroll
+    for (int u = 0; u < S; ++u) g[u] = v[at(tile, len, c[u], lane)];
+#pragma unroll
+    for (int u = 0; u < S; ++u) d += w[u] * g[u];
+}
+
 // sum over q in [beg, end) of val[q] * v(LP lane, index idx[q]) in ascending q.
 // The (uniform) index/value pairs are taken 8 at a time so that 8 gathers are
 // in flight together; the summation order stays sequential.
@@ -95,18 +112,17 @@ __device__ __forceinline__ float gather_dot(const int32_t* __restrict__ idx, con
 #pragma unroll
         for (int u = 0; u < 8; ++u) d += w[u] * g[u];
     }
-    // the rest (fewer than 8): loads issued together, added in the same order
-    if (q < end) {
-        float g[8], w[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const bool ok = q + u < end;
-            w[u] = ok ? val[q + u] : 0.f;
-            g[u] = ok ? v[at(tile, len, idx[q + u], lane)] : 0.f;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (q + u < end) d += w[u] * g[u];
+    // the rest (fewer than 8): a path per count, so the index loads and then the
+    // gathers of exactly those entries go out together (added in the same order)
+    switch (end - q) {
+        case 1: tail_dot<1>(idx, val, q, v, tile, len, lane, d); break;
+        case 2: tail_dot<2>(idx, val, q, v, tile, len, lane, d); break;
+        case 3: tail_dot<3>(idx, val, q, v, tile, len, lane, d); break;
+        case 4: tail_dot<4>(idx, val, q, v, tile, len, lane, d); break;
+        case 5: tail_dot<5>(idx, val, q, v, tile, len, lane, d); break;
+        case 6: tail_dot<6>(idx, val, q, v, tile, len, lane, d); break;
+        case 7: tail_dot<7>(idx, val, q, v, tile, len, lane, d); break;
+        default: break;
     }
     return d;
 }
