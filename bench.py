@@ -256,15 +256,21 @@ def spmv_rates(eng, reps: int = 50, kx_tpr: int = 64, traffic_key=None):
     lib = eng.lib
     out = {}
     s = torch.cuda.Stream(device=eng.device)
-    for name, rp, ci, val, x, rows, cols, tpr in (
-            ("KT_y", eng.t_row_ptr, eng.t_col_idx, eng.t_val, eng.y, eng.n, eng.m, 1),
-            ("K_x", eng.row_ptr, eng.col_idx, eng.val, eng.x, eng.m, eng.n, kx_tpr)):
+    cases = [("KT_y", eng.t_row_ptr, eng.t_col_idx, eng.t_val, eng.y, eng.n, eng.m, 1),
+             ("K_x", eng.row_ptr, eng.col_idx, eng.val, eng.x, eng.m, eng.n, kx_tpr)]
+    if getattr(eng, "t_sell_slices", 0):       # K^T y over the SELL-64 copy of K^T (pdlp_spmv_sell64)
+        cases.insert(1, ("KT_y_sell", eng.t_sell_ptr, eng.t_sell_idx, eng.t_sell_val, eng.y, eng.n, eng.m, 0))
+    for name, rp, ci, val, x, rows, cols, tpr in cases:
         with torch.cuda.stream(s):
             y = torch.empty(rows, dtype=torch.float32, device=eng.device)
 
         def launch():
-            L.check(lib.pdlp_spmv_csr(L.PDLP_F32, ptr(rp), ptr(ci), ptr(val), ptr(x), ptr(y), rows, tpr,
-                                      stream_handle(torch.cuda.current_stream())), "pdlp_spmv_csr")
+            if tpr == 0:
+                L.check(lib.pdlp_spmv_sell64(L.PDLP_F32, ptr(rp), ptr(ci), ptr(val), ptr(x), ptr(y), rows,
+                                             stream_handle(torch.cuda.current_stream())), "pdlp_spmv_sell64")
+            else:
+                L.check(lib.pdlp_spmv_csr(L.PDLP_F32, ptr(rp), ptr(ci), ptr(val), ptr(x), ptr(y), rows, tpr,
+                                          stream_handle(torch.cuda.current_stream())), "pdlp_spmv_csr")
         with torch.cuda.stream(s):
             launch()
         s.synchronize()
@@ -280,8 +286,9 @@ def spmv_rates(eng, reps: int = 50, kx_tpr: int = 64, traffic_key=None):
             e1.record(s)
         s.synchronize()
         us = e0.elapsed_time(e1) * 1e3 / reps
-        nnz = int(val.numel())
-        nbytes = 8 * nnz + 4 * (rows + 1) + 4 * cols + 4 * rows
+        nnz = int(val.numel())                   # SELL: slots, padding included
+        ptrs = (int(rp.numel()) if tpr == 0 else rows + 1)
+        nbytes = 8 * nnz + 4 * ptrs + 4 * cols + 4 * rows
         gbs = nbytes / (us * 1e-6) / 1e9
         out[name] = {"rows": rows, "nnz": nnz, "lanes_per_row": tpr, "bytes": nbytes, "us": us,
                      "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS,

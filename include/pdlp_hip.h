@@ -239,6 +239,16 @@ int pdlp_spmv_csr(int32_t prec, const int32_t* row_ptr, const int32_t* col_idx,
                   const void* val, const void* x, void* y, int32_t nrows,
                   int32_t tpr, void* stream);
 
+/* y = A x with A in SELL-64 slices of its rows in natural order (the layout of
+ * pdlp_csr_to_sell64: slice s = rows 64 s .. 64 s + 63, entry k of row 64 s + l at
+ * sell_ptr[s] + 64 k + l, padding 0): one lane per row, no row pointer read,
+ * each row summed in CSR order, so y equals pdlp_spmv_csr's with one lane per
+ * row.  For matrices of many short rows of similar length (K^T of a
+ * transportation LP: 2 entries per row).  Replaces the same products as
+ * pdlp_spmv_csr (pdhg.py:211; pdhg_gpu.py:215,245). */
+int pdlp_spmv_sell64(int32_t prec, const int32_t* sell_ptr, const int32_t* sell_idx, const void* sell_val,
+                     const void* x, void* y, int32_t nrows, void* stream);
+
 /* y = A x with FP32 A, FP64 x and FP64 accumulation: the products of the KKT
  * check (pdhg.py:186,188) when the iterate engine runs in FP32. */
 int pdlp_spmv_csr_f64acc(const int32_t* row_ptr, const int32_t* col_idx,

@@ -60,6 +60,37 @@ int launch_spmv(const int32_t* rp, const int32_t* ci, const V* val, const X* x, 
     return hip_fail(hipGetLastError(), "spmv_kernel launch");
 }
 
+// y = A x with A as SELL-64 slices of its rows in natural order (pdlp_csr_to_sell64):
+// one lane per row, no row pointer to read, every entry load a whole line; each
+// row summed in CSR order (padding adds zeros), as the one-lane CSR kernel does.
+template <typename T>
+__global__ void __launch_bounds__(256) spmv_sell_kernel(const int32_t* __restrict__ sp, const int32_t* __restrict__ si,
+                                                        const T* __restrict__ sv, const T* __restrict__ x,
+                                                        T* __restrict__ y, int nrows) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = __builtin_amdgcn_readfirstlane(r >> 6);
+    if (s >= (nrows + 63) >> 6) return;                // whole wavefronts leave together
+    const int beg = __builtin_amdgcn_readfirstlane(sp[s]);
+    const int width = __builtin_amdgcn_readfirstlane((sp[s + 1] - beg) >> 6);
+    const int32_t* __restrict__ idx = si + beg + (r & 63);
+    const T* __restrict__ val = sv + beg + (r & 63);
+    T d = T(0);
+    int k = 0;
+    for (; k + kRowBatch <= width; k += kRowBatch) {
+        T a[kRowBatch];
+        int c[kRowBatch];
+#pragma unroll
+        for (int u = 0; u < kRowBatch; ++u) {
+            a[u] = val[64 * (k + u)];
+            c[u] = idx[64 * (k + u)];
+        }
+#pragma unroll
+        for (int u = 0; u < kRowBatch; ++u) d += a[u] * x[c[u]];
+    }
+    for (; k < width; ++k) d += val[64 * k] * x[idx[64 * k]];
+    if (r < nrows) y[r] = d;
+}
+
 }  // namespace
 }  // namespace pdlp
 
@@ -82,6 +113,24 @@ int pdlp_spmv_csr(int32_t prec, const int32_t* row_ptr, const int32_t* col_idx, 
         return launch_spmv<double, double, double>(row_ptr, col_idx, (const double*)val, (const double*)x,
                                                    (double*)y, nrows, tpr, as_stream(stream));
     return fail(PDLP_E_ARG, "prec must be PDLP_F32 or PDLP_F64");
+}
+
+int pdlp_spmv_sell64(int32_t prec, const int32_t* sell_ptr, const int32_t* sell_idx, const void* sell_val,
+                     const void* x, void* y, int32_t nrows, void* stream) {
+    if (nrows < 0) return fail(PDLP_E_ARG, "nrows must be >= 0");
+    if (nrows == 0) return 0;
+    if (!sell_ptr || !sell_idx || !sell_val || !x || !y) return fail(PDLP_E_ARG, "NULL pointer");
+    const unsigned g = (unsigned)((nrows + 255) / 256);
+    hipStream_t s = as_stream(stream);
+    if (prec == PDLP_F32)
+        spmv_sell_kernel<float><<<g, 256, 0, s>>>(sell_ptr, sell_idx, (const float*)sell_val, (const float*)x,
+                                                  (float*)y, nrows);
+    else if (prec == PDLP_F64)
+        spmv_sell_kernel<double><<<g, 256, 0, s>>>(sell_ptr, sell_idx, (const double*)sell_val, (const double*)x,
+                                                   (double*)y, nrows);
+    else
+        return fail(PDLP_E_ARG, "prec must be PDLP_F32 or PDLP_F64");
+    return hip_fail(hipGetLastError(), "spmv_sell_kernel launch");
 }
 
 int pdlp_spmv_csr_f64acc(const int32_t* row_ptr, const int32_t* col_idx, const float* val, const double* x,

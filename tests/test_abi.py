@@ -191,6 +191,9 @@ def test_sell_columns_layout_and_padding_limit():
 
 def test_csr_to_sell64_rejects_bad_arguments_without_device():
     lib = L.lib()
+    assert lib.pdlp_spmv_sell64(L.PDLP_F32, None, None, None, None, None, 5, None) == L.PDLP_E_ARG
+    assert lib.pdlp_spmv_sell64(L.PDLP_F32, None, None, None, None, None, -1, None) == L.PDLP_E_ARG
+    assert lib.pdlp_spmv_sell64(L.PDLP_F32, None, None, None, None, None, 0, None) == 0
     assert lib.pdlp_csr_to_sell64(9, 4, None, None, None, None, None, None, None) == L.PDLP_E_ARG
     assert lib.pdlp_csr_to_sell64(L.PDLP_F32, -1, None, None, None, None, None, None, None) == L.PDLP_E_SHAPE
     assert lib.pdlp_csr_to_sell64(L.PDLP_F32, 4, None, None, None, None, None, None, None) == L.PDLP_E_ARG

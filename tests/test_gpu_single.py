@@ -322,3 +322,32 @@ def extract_xy(f, p, r):
     from sovereign_solver_amd.engine.forms import extract
     x, _, y = extract(f, p, r["x"], r["y"])
     return x, y
+
+
+@pytest.mark.parametrize("shape", ["transport", "random"])
+def test_spmv_sell64_same_bits_as_one_lane_csr(gpu, shape, monkeypatch):
+    """pdlp_spmv_sell64 over the engine's SELL-64 copy of K^T gives pdlp_spmv_csr's
+    one-lane-per-row result bit for bit (each row summed in CSR order)."""
+    import torch
+    from sovereign_solver_amd.engine import _lib as L
+    from sovereign_solver_amd.engine import sell as SL
+    from sovereign_solver_amd.engine.device import ptr, stream_handle
+    from sovereign_solver_amd.engine.forms import box_form
+    from sovereign_solver_amd.engine.single import SingleEngine
+    monkeypatch.setattr(SL, "COLUMN_PADDING_MAX", 100.0)
+    p = transportation_lp(70, 150, seed=2) if shape == "transport" else random_sparse_lp(300, 700, 0.02, seed=4)
+    eng = SingleEngine(box_form(p), "fp32")
+    assert eng.t_sell_slices > 0
+    rng = np.random.default_rng(0)
+    x = torch.from_numpy(rng.standard_normal(eng.m).astype(np.float32)).to(gpu)
+    a = torch.zeros(eng.n, dtype=torch.float32, device=gpu)
+    b = torch.zeros(eng.n, dtype=torch.float32, device=gpu)
+    sh = stream_handle(torch.cuda.current_stream())
+    L.check(L.lib().pdlp_spmv_csr(L.PDLP_F32, ptr(eng.t_row_ptr), ptr(eng.t_col_idx), ptr(eng.t_val), ptr(x),
+                                  ptr(a), eng.n, 1, sh), "spmv_csr")
+    L.check(L.lib().pdlp_spmv_sell64(L.PDLP_F32, ptr(eng.t_sell_ptr), ptr(eng.t_sell_idx), ptr(eng.t_sell_val),
+                                     ptr(x), ptr(b), eng.n, sh), "spmv_sell64")
+    torch.cuda.synchronize()
+    assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
+    want = (p.A_eq if p.n_ineq == 0 else sp.vstack([p.A_ub, p.A_eq])).T.tocsr() @ x.cpu().numpy().astype(np.float64)
+    assert np.allclose(b.cpu().numpy(), want, rtol=1e-5, atol=1e-4)
