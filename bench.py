@@ -510,36 +510,68 @@ def run_headline(args):
     log(f"cold batched, resident: {cold['lps_per_s']:.0f} LPs/s (iterations mean {cold['iters_mean']:.0f}, "
         f"max {cold['iters_max']})")
 
-    # host to host: problems on the host -> features, packing, inference, solve -> results and objectives
-    def learned_h2h():
-        V = fam.node_inputs_from(probs)
-        eng.load(probs)
-        model.start_engine(eng, tol, V)
-        eng.run(mi)
-        o, X, Y = eng.results()
-        return o, eng.objectives(X)
-
-    def cold_h2h():
-        eng.load(probs)
-        eng.start(tol)
-        eng.run(mi)
-        o, X, Y = eng.results()
-        return o, eng.objectives(X)
-
+    # host to host: problems on the host -> staging and upload, network inputs (on the device,
+    # from the staged vectors), inference, solve -> results and objectives; one batch at a time,
+    # then a serving loop over consecutive batches whose host work overlaps the device work
+    from sovereign_solver_amd.lp.serve import FamilyServer
+    kw = {"items_per_wave": shape["items_per_wave"], "xcd_affinity": bool(shape["xcd_affinity"]),
+          "check_every": eng.check_every}
+    servers = {"learned": FamilyServer(probs, tol, mi, model=model, features=fam.node_inputs_staged, engine_kw=kw),
+               "cold": FamilyServer(probs, tol, mi, engine_kw=kw)}
     h2h = {}
-    for name, fn in (("learned", learned_h2h), ("cold", cold_h2h)):
-        fn()
+    for name, srv in servers.items():
+        srv.solve(probs)                              # graphs captured, buffers warm
         torch.cuda.synchronize()
         t = time.perf_counter()
         reps = 3
         for _ in range(reps):
-            o, obj = fn()
+            o, X, Y, obj = srv.solve(probs)
         dt = (time.perf_counter() - t) / reps
         h2h[name] = {"lps_per_s": int((o["status"] == 1).sum()) / dt, "ms_per_batch": dt * 1e3}
         if name == "learned":
             learned_obj = obj.copy()
     log(f"host to host: learned {h2h['learned']['lps_per_s']:.0f} LPs/s ({h2h['learned']['ms_per_batch']:.1f} ms "
-        f"per batch), cold {h2h['cold']['lps_per_s']:.0f} LPs/s")
+        f"per batch), cold {h2h['cold']['lps_per_s']:.0f} LPs/s ({h2h['cold']['ms_per_batch']:.1f} ms)")
+    # one more learned batch with a synchronisation after each stage: where the time goes
+    srv = servers["learned"]
+    e0 = srv.engines[0]
+    stages = {}
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for name, fn in (("load", lambda: e0.load(probs)),
+                     ("inputs", lambda: fam.node_inputs_staged(probs, e0)),
+                     ("inference", lambda: model.starts(V)),
+                     ("start_solve", lambda: (e0.start_from_device(tol, S0[0], x_off=0, y_off=n,
+                                                                  w_col=n + m if model.log_weight is not None
+                                                                  else -1, ready=S0[1]), e0.run(mi))),
+                     ("results", lambda: e0.results()),
+                     ("objectives", lambda: e0.objectives(X))):
+        r = fn()
+        if name == "inputs":
+            V = r
+        if name == "inference":
+            S0 = r
+        if name == "results":
+            X = r[1]
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        stages[name] = (t1 - t) * 1e3
+        t = t1
+    h2h["learned"]["stages_ms"] = stages
+    log("host to host, learned, by stage (ms): " + ", ".join(f"{k} {v:.1f}" for k, v in stages.items()))
+    nb = args.serve_batches
+    batches = [[fam.problem(int(10**6 + len(probs) * (j + 1) + k)) for k in range(len(probs))] for j in range(nb)]
+    for name, srv in servers.items():
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ok = sum(int((o["status"] == 1).sum()) for o, X, Y, obj in srv.solve_stream(batches))
+        dt = time.perf_counter() - t
+        h2h[name]["pipelined"] = {"lps_per_s": ok / dt, "ms_per_batch": dt * 1e3 / nb, "batches": nb,
+                                  "optimal": ok}
+    del servers, srv, e0
+    log(f"host to host, serving loop over {nb} batches: learned {h2h['learned']['pipelined']['lps_per_s']:.0f} "
+        f"LPs/s ({h2h['learned']['pipelined']['ms_per_batch']:.1f} ms per batch), cold "
+        f"{h2h['cold']['pipelined']['lps_per_s']:.0f} LPs/s ({h2h['cold']['pipelined']['ms_per_batch']:.1f} ms)")
     roof = family_passes(eng, tol, f"dispatch_{len(probs)}")
     del eng
     plain = plain_rates(probs[: args.plain_sample], tol, mi)
@@ -1024,6 +1056,8 @@ def main():
     ap.add_argument("--skip", default="", help="comma list of sub-records to leave out: c1,c2,c4,beyond_mall")
     ap.add_argument("--train", type=int, default=50_000)
     ap.add_argument("--eval", type=int, default=1024)
+    ap.add_argument("--serve-batches", type=int, default=4,
+                    help="held-out batches of the serving loop (host to host, overlapped)")
     ap.add_argument("--stages", type=int, default=3, help="node-model stages")
     ap.add_argument("--node-steps", type=int, default=60_000, help="training steps per stage")
     ap.add_argument("--node-rows", type=int, default=32768, help="node rows per training step")

@@ -196,12 +196,13 @@ typedef struct PdlpTune {
     int32_t scheme;       /* PDLP_SCHEME_* restart scheme */
     int32_t sell;         /* dual step on the SELL-64 slices of PdlpDev (if any): 0 off;
                              1 one wavefront per slice; 2..7 slices and the CSR rows in
-                             one launch, a workgroup per CSR row and per slice piece
-                             (threads / rows of a piece: 2: 1024/64, 3: 512/64,
-                             4: 1024/16, 5: 1024/8, 6: 1024/32, 7: 512/16), the piece's
-                             entries split among its wavefronts and their sums added
-                             in a fixed order; k >= 8 (multiple of 8) a wavefront
-                             per k entries of a slice, partial sums added in order */
+                             one launch, a workgroup per slice piece (threads / rows of
+                             a piece: 2: 1024/64, 3: 512/64, 4: 1024/16, 5: 1024/8,
+                             6: 1024/32, 7: 512/16), the piece's entries split among
+                             its wavefronts and their sums added in a fixed order, and
+                             tpr_dual threads per CSR row (at least 64, at most the
+                             workgroup); k >= 8 (multiple of 8) a wavefront per k
+                             entries of a slice, partial sums added in order */
     int32_t no_ray;       /* 1: skip the unboundedness (ray of decrease) test of the
                              adaptive checks; 0: test at every check */
     int32_t sell_primal;  /* 1: primal step over the SELL-64 copy of K^T (PdlpDev.t_sell_*),
@@ -735,6 +736,36 @@ typedef struct PdlpNodeAssembly {
 } PdlpNodeAssembly;
 
 int pdlp_node_assemble(const PdlpNodeAssembly* a, void* stream);
+
+/* Network inputs of the economic-dispatch family (G generators, T periods) from
+ * the LPs' own vectors, one row of ld_out floats per instance k: with c_g =
+ * c[k*ld_c + g] (the first period's costs), d_t = -b[k*ld_b + t] (the demand rows'
+ * right-hand sides), S = pmax_sum, the merit order = the stable ascending order of
+ * c, cheaper_g = the capacity strictly before g in it (a sequential FP64 sum along
+ * the order), marg_t = min(#{r : cum_r < d_t}, G - 1) (cum = the cumulated
+ * capacity along the order) and price_t = the cost at position marg_t:
+ *   out[off_d + t] = 4 d_t / S        out[off_c + g] = c_g / 60
+ *   out[off_u + g] = c_g / cost_g - 1 out[off_cc + g] = 2 cheaper_g / S
+ *   out[off_cc2 + g] = 2 (cheaper_g + pmax_g) / S
+ *   out[off_p + t] = price_t / 60     out[off_m + t] = d_t / demand_t - 1
+ *   out[off_x + t G + g] = clip(d_t - cheaper_g, 0, pmax_g) / pmax_g
+ * each an FP64 expression rounded once to FP32 (bitwise the host definition);
+ * other entries are left as they are.  G <= 1024, T <= 256.  The reference has
+ * no learned component: no counterpart. */
+typedef struct PdlpDispatchInputs {
+    int32_t count, G, T;
+    int32_t ld_c, ld_b, ld_out;
+    int32_t off_d, off_c, off_u, off_cc, off_cc2, off_p, off_m, off_x;
+    double pmax_sum;
+    const double* c;               /* [count][ld_c] */
+    const double* b;               /* [count][ld_b] */
+    const double* pmax;            /* [G] */
+    const double* cost;            /* [G] reference costs */
+    const double* demand;          /* [T] reference demand */
+    float* out;                    /* [count][ld_out] */
+} PdlpDispatchInputs;
+
+int pdlp_dispatch_inputs(const PdlpDispatchInputs* a, void* stream);
 
 #ifdef __cplusplus
 }

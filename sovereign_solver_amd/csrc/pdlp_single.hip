@@ -273,21 +273,22 @@ __global__ void __launch_bounds__(256) dual_sell_kernel(PdlpDev d) {
     dual_update<T, false>(d, sigma_scale, i, ax, asum);
 }
 
-// SELL-64 slices and CSR rows in one launch, one workgroup of NT threads per
-// item.  The first slices * (64 / R) workgroups each take R consecutive rows
-// of a slice: a wavefront's lane l works on row l % R of them at entry phase
-// l / R, so one access reads entry k .. k + 64/R - 1 of the R rows as 64/R
-// contiguous pieces of R values (and their gathers of xbar share cache
-// lines); the slice's entries are split into NT / 64 contiguous ranges, one
-// per wavefront.  The partial sums meet in LDS and are added in a fixed order
-// (wavefront, then phase).  The remaining workgroups each take one row of
-// csr_rows with all NT threads.  Block 0 counts the iteration.
-template <typename T, int NT, int R>
+// SELL-64 slices and CSR rows in one launch, workgroups of NT threads.  The
+// first slices * (64 / R) workgroups each take R consecutive rows of a slice:
+// a wavefront's lane l works on row l % R of them at entry phase l / R, so one
+// access reads entry k .. k + 64/R - 1 of the R rows as 64/R contiguous pieces
+// of R values (and their gathers of xbar share cache lines); the slice's
+// entries are split into NT / 64 contiguous ranges, one per wavefront.  The
+// partial sums meet in LDS and are added in a fixed order (wavefront, then
+// phase).  The remaining workgroups take NT / TC rows of csr_rows each, TC
+// threads per row (TC / 64 wavefronts; their sums meet in LDS in wavefront
+// order).  Block 0 counts the iteration.
+template <typename T, int NT, int R, int TC>
 __global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d) {
     constexpr int NW = NT / 64, P = 64 / R;
     static_assert(R >= 1 && R <= 64 && 64 % R == 0, "rows per workgroup divide the slice");
+    static_assert(TC >= 64 && TC <= NT && NT % TC == 0 && TC % 64 == 0, "threads per CSR row");
     __shared__ T part[2][NW][64];
-    __shared__ T red[2 * 16];
     PdlpCtrl* ctrl = d.ctrl;
     if (ctrl->status != PDLP_RUNNING) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) {   // single writer; read only by later kernels
@@ -333,13 +334,54 @@ __global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d) {
         }
         return;
     }
-    const int rr = b - d.sell_slices * P;
-    if (rr >= d.n_csr_rows) return;
-    const int i = d.csr_rows[rr];
-    const RowOwn<T> o = row_own<T>(d, i, threadIdx.x == 0);
-    T ax, asum;
-    row_dot_block<NT, T>(d.row_ptr, d.col_idx, (const T*)d.val, i, [&](int k) { return xbar[k]; }, ax, asum, red);
-    if (threadIdx.x == 0) dual_update_own<T, false>(d, sigma_scale, i, ax, asum, o);
+    // CSR rows: every thread reaches the barrier below, rows past the end included
+    const int rr = (b - d.sell_slices * P) * (NT / TC) + (int)threadIdx.x / TC;
+    const int tc = threadIdx.x % TC;
+    const int i = rr < d.n_csr_rows ? d.csr_rows[rr] : -1;
+    const RowOwn<T> o = row_own<T>(d, i, i >= 0 && tc == 0);
+    T ax = T(0), asum = T(0);
+    if (i >= 0) {
+        const int beg = d.row_ptr[i], end = d.row_ptr[i + 1];
+        const int32_t* __restrict__ ci = d.col_idx;
+        const T* __restrict__ val = (const T*)d.val;
+        for (int k0 = beg + tc; k0 < end; k0 += TC * kRowBatch) {
+            T a[kRowBatch];
+            int c[kRowBatch];
+#pragma unroll
+            for (int u = 0; u < kRowBatch; ++u) {
+                const int k = k0 + u * TC;
+                a[u] = k < end ? val[k] : T(0);
+                c[u] = k < end ? ci[k] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < kRowBatch; ++u) {
+                if (k0 + u * TC < end) {
+                    ax += a[u] * xbar[c[u]];
+                    asum += fabs(a[u]);
+                }
+            }
+        }
+    }
+    ax = group_sum<kWave>(ax);
+    asum = group_sum<kWave>(asum);
+    if constexpr (TC > 64) {
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            part[0][w][0] = ax;
+            part[1][w][0] = asum;
+        }
+        __syncthreads();
+        if (tc == 0) {
+            ax = T(0);
+            asum = T(0);
+#pragma unroll
+            for (int v = 0; v < TC / 64; ++v) {
+                ax += part[0][w + v][0];
+                asum += part[1][w + v][0];
+            }
+        }
+    }
+    if (i >= 0 && tc == 0) dual_update_own<T, false>(d, sigma_scale, i, ax, asum, o);
 }
 
 // ------------------------------------------------------------- KKT partials
@@ -935,16 +977,31 @@ int launch_dual_t(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
         dual_kernel<T, 1, true><<<g, nt, 0, s>>>(d, remap, 0);
     } else {
         if (sell && t.sell >= 2 && t.sell <= 7) {
-            // slices and CSR rows in one launch, a workgroup per item (it counts the iteration)
+            // slices and CSR rows in one launch (block 0 counts the iteration)
             static constexpr int kRows[8] = {0, 0, 64, 64, 16, 8, 32, 16};
-            const unsigned items = (unsigned)(d.sell_slices * (64 / kRows[t.sell]) + d.n_csr_rows);
+            static constexpr int kThreads[8] = {0, 0, 1024, 512, 1024, 1024, 1024, 512};
+            const int nt = kThreads[t.sell];
+            const int tc = std::min(std::max(t.tpr_dual, 64), nt);
+            const int per = nt / tc;
+            const unsigned items =
+                (unsigned)(d.sell_slices * (64 / kRows[t.sell]) + (d.n_csr_rows + per - 1) / per);
+            auto go = [&](auto nt_c, auto r_c) {
+                constexpr int NT = decltype(nt_c)::value, R = decltype(r_c)::value;
+                switch (tc) {
+                    case 64: dual_sell_wg_kernel<T, NT, R, 64><<<items, NT, 0, s>>>(d); break;
+                    case 128: dual_sell_wg_kernel<T, NT, R, 128><<<items, NT, 0, s>>>(d); break;
+                    case 256: dual_sell_wg_kernel<T, NT, R, 256><<<items, NT, 0, s>>>(d); break;
+                    case 512: dual_sell_wg_kernel<T, NT, R, 512><<<items, NT, 0, s>>>(d); break;
+                    default: dual_sell_wg_kernel<T, NT, R, NT><<<items, NT, 0, s>>>(d); break;
+                }
+            };
             switch (t.sell) {
-                case 2: dual_sell_wg_kernel<T, 1024, 64><<<items, 1024, 0, s>>>(d); break;
-                case 3: dual_sell_wg_kernel<T, 512, 64><<<items, 512, 0, s>>>(d); break;
-                case 4: dual_sell_wg_kernel<T, 1024, 16><<<items, 1024, 0, s>>>(d); break;
-                case 5: dual_sell_wg_kernel<T, 1024, 8><<<items, 1024, 0, s>>>(d); break;
-                case 6: dual_sell_wg_kernel<T, 1024, 32><<<items, 1024, 0, s>>>(d); break;
-                default: dual_sell_wg_kernel<T, 512, 16><<<items, 512, 0, s>>>(d); break;
+                case 2: go(ic<1024>{}, ic<64>{}); break;
+                case 3: go(ic<512>{}, ic<64>{}); break;
+                case 4: go(ic<1024>{}, ic<16>{}); break;
+                case 5: go(ic<1024>{}, ic<8>{}); break;
+                case 6: go(ic<1024>{}, ic<32>{}); break;
+                default: go(ic<512>{}, ic<16>{}); break;
             }
             return hip_fail(hipGetLastError(), "dual_sell_wg_kernel launch");
         }

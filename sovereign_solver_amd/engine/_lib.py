@@ -164,6 +164,17 @@ class PdlpNodeAssembly(C.Structure):
     ]
 
 
+class PdlpDispatchInputs(C.Structure):
+    _fields_ = [
+        ("count", C.c_int32), ("G", C.c_int32), ("T", C.c_int32),
+        ("ld_c", C.c_int32), ("ld_b", C.c_int32), ("ld_out", C.c_int32),
+        ("off_d", C.c_int32), ("off_c", C.c_int32), ("off_u", C.c_int32), ("off_cc", C.c_int32),
+        ("off_cc2", C.c_int32), ("off_p", C.c_int32), ("off_m", C.c_int32), ("off_x", C.c_int32),
+        ("pmax_sum", C.c_double),
+        ("c", _p), ("b", _p), ("pmax", _p), ("cost", _p), ("demand", _p), ("out", _p),
+    ]
+
+
 class PdlpError(RuntimeError):
     pass
 
@@ -216,6 +227,7 @@ _SIGS = {
     "pdlp_cast_bf16": (C.c_int, [_p, _p, C.c_int64, _p]),
     "pdlp_node_gather": (C.c_int, [C.POINTER(PdlpNodeGather), _p]),
     "pdlp_node_assemble": (C.c_int, [C.POINTER(PdlpNodeAssembly), _p]),
+    "pdlp_dispatch_inputs": (C.c_int, [C.POINTER(PdlpDispatchInputs), _p]),
 }
 
 

@@ -11,7 +11,6 @@ and termination, exactly as in the persistent batched engine.
 
 from __future__ import annotations
 
-import concurrent.futures as cf
 import ctypes as C
 from typing import Optional, Sequence
 
@@ -337,33 +336,26 @@ class FamilyEngine:
     _COLS = ("c", "lo", "hi", "c64", "lo_shift", "ub_span")
 
     def _gather(self, problems) -> None:
-        """Per-LP sources into the pinned LP-major staging, in parallel threads."""
+        """Per-LP sources into the pinned LP-major staging: one NumPy view of each
+        staging array, then one row copy per LP and vector (one thread: the
+        copies are short and threads only contend for the interpreter)."""
         src = self._src
         if not src:
             return
-        B = len(problems)
         m_ub = self.fb.m_ub
-
-        def fill(lo_k, hi_k):
-            for k in range(lo_k, hi_k):
-                p = problems[k]
-                if "c" in src:
-                    src["c"][0][k].numpy()[:] = p.c
-                if "b" in src:
-                    row = src["b"][0][k].numpy()
-                    row[:m_ub] = p.b_ub
-                    row[m_ub:] = p.b_eq
-                if "lo" in src:
-                    src["lo"][0][k].numpy()[:] = p.lb
-                    src["hi"][0][k].numpy()[:] = p.ub
-
-        workers = max(1, min(8, B // 64))
-        if workers == 1:
-            fill(0, B)
-            return
-        step = (B + workers - 1) // workers
-        with cf.ThreadPoolExecutor(workers) as ex:
-            list(ex.map(lambda a: fill(a, min(B, a + step)), range(0, B, step)))
+        C_ = src["c"][0].numpy() if "c" in src else None
+        B_ = src["b"][0].numpy() if "b" in src else None
+        LO = src["lo"][0].numpy() if "lo" in src else None
+        HI = src["hi"][0].numpy() if "lo" in src else None
+        for k, p in enumerate(problems):
+            if C_ is not None:
+                C_[k] = p.c
+            if B_ is not None:
+                B_[k, :m_ub] = p.b_ub
+                B_[k, m_ub:] = p.b_eq
+            if LO is not None:
+                LO[k] = p.lb
+                HI[k] = p.ub
 
     def load(self, problems: Optional[Sequence[Problem]] = None) -> None:
         """Problem data of the batch's LPs (or of ``problems``: the same family, count and
@@ -391,6 +383,12 @@ class FamilyEngine:
         self.stream.synchronize()
         nv = self.norms_host.numpy().reshape(-1, 3)
         self.norm_b, self.norm_c, self.init_weight = nv[:, 0].copy(), nv[:, 1].copy(), nv[:, 2].copy()
+
+    def staged(self, name: str) -> Optional[torch.Tensor]:
+        """The device copy [B, len] FP64 of the per-LP source ``name`` (``"c"``, ``"b"``,
+        ``"lo"``, ``"hi"``) as the last :meth:`load` staged it, row k = LP k in its own
+        order (``b`` = [b_ub, b_eq]); None when the batch stores that vector once."""
+        return self._src[name][1] if name in self._src else None
 
     def _restore_data(self) -> None:
         """Per-LP problem vectors back in the original slot order (after compaction)."""

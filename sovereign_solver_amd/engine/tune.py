@@ -38,6 +38,9 @@ from dataclasses import asdict, dataclass
 from typing import Callable, Optional
 
 ARCH = "gfx950"
+# Version of the single-instance launch choices stored in the cache (raised when a
+# stored field changes meaning: 2 = tpr_dual sets the CSR rows' threads in SELL modes 2..7)
+SINGLE_LAYOUT = 2
 TPR_CHOICES = (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024)
 CADENCES = (25, 50, 100, 200)          # iterations between restart checks
 
@@ -53,8 +56,9 @@ class Tune:
     block_dual: int = 0
     scheme: int = 0             # 0 reference restart rule, 1 adaptive (see include/pdlp_hip.h)
     sell: int = 0               # SELL-64 dual step: 0 off, 1 a wavefront per slice, 2..7 one launch with a
-                                # workgroup per CSR row and per slice piece (threads/rows: 2 1024/64, 3 512/64,
-                                # 4 1024/16, 5 1024/8, 6 1024/32, 7 512/16), k >= 8 a wavefront per k entries
+                                # workgroup per slice piece (threads/rows: 2 1024/64, 3 512/64, 4 1024/16,
+                                # 5 1024/8, 6 1024/32, 7 512/16) and tpr_dual (>= 64) threads per CSR row,
+                                # k >= 8 a wavefront per k entries
     no_ray: int = 0             # 1: the adaptive checks skip the unboundedness test
     sell_primal: int = 0        # 1: primal step over the SELL-64 copy of K^T (one lane per column)
 
@@ -88,7 +92,7 @@ def heuristic_tune(m: int, n: int, nnz: int, check_every: int = 50, exact: bool 
 
 def _bucket(m: int, n: int, nnz: int, prec: str) -> str:
     lg = lambda v: int(round(math.log2(max(v, 1))))
-    return f"{ARCH}/{prec}/m{lg(m)}/n{lg(n)}/z{lg(nnz)}"
+    return f"{ARCH}/{prec}/s{SINGLE_LAYOUT}/m{lg(m)}/n{lg(n)}/z{lg(nnz)}"
 
 
 def _cache_path() -> str:
@@ -188,8 +192,12 @@ def autotune(engine, reps: int = 20, use_cache: bool = True) -> Tune:
     if getattr(engine, "sell_slices", 0) > 0:
         # the SELL slices take the coalescing rows; the CSR kernel keeps the rest
         for sv in (1, 2, 3, 4, 5, 6, 7, 16, 32, 64, 128):
-            for td in _neighbours(base.tpr_dual):
-                for remap in (0, 1):
+            # one-launch modes: threads per CSR row (no remap); others: the CSR kernel's lanes
+            one = 2 <= sv <= 7
+            tds = [w for w in (64, 128, 256, 512, 1024) if w <= (512 if sv in (3, 7) else 1024)] if one \
+                else _neighbours(base.tpr_dual)
+            for td in tds:
+                for remap in ((0,) if one else (0, 1)):
                     t = Tune(best_p[0], td, 256, base.check_every, remap << 1, 0, 0, 0, sv)
                     us = engine.time_kernel("dual", t, reps)
                     timings[f"dual/sell{sv}/tpr{td}/remap{remap}"] = us

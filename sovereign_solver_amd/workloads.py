@@ -262,6 +262,50 @@ class DispatchFamily:
         v[:, o["M"]:o["M"] + T] = d / self.demand - 1.0
         return v
 
+    def node_inputs_device(self, c_dev, b_dev, stream=None):
+        """:meth:`node_inputs_params` on the device, bitwise, from the LPs' own cost and
+        right-hand-side rows already in HBM: ``c_dev`` [B, >= G] and ``b_dev`` [B, >= T]
+        FP64 device tensors (row k = instance k's ``c`` and ``b_ub``, e.g. the family
+        engine's staging, :meth:`FamilyEngine.staged`).  Returns V [B, L] FP32 on
+        the device, written on ``stream`` (default: the current stream)."""
+        import ctypes as C
+        import torch
+        from .engine import _lib as L
+        T, G = self.T, self.G
+        B = c_dev.shape[0]
+        for t_, w in ((c_dev, G), (b_dev, T)):
+            if (t_.dtype != torch.float64 or t_.dim() != 2 or t_.shape[0] != B or t_.shape[1] < w
+                    or t_.stride(1) != 1):
+                raise ValueError("c_dev / b_dev must be [B, >= G] / [B, >= T] float64 device rows")
+        dv = c_dev.device
+        key = str(dv)
+        if getattr(self, "_dev_consts", None) is None:
+            self._dev_consts = {}
+        if key not in self._dev_consts:
+            self._dev_consts[key] = tuple(torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(dv)
+                                          for a in (self.pmax, self.cost, self.demand))
+        pm, co, de = self._dev_consts[key]
+        o, L_ = self._node_offsets()
+        st = stream if stream is not None else torch.cuda.current_stream(dv)
+        with torch.cuda.stream(st):
+            V = torch.empty((B, L_), dtype=torch.float32, device=dv)
+        a = L.PdlpDispatchInputs(count=B, G=G, T=T, ld_c=c_dev.stride(0), ld_b=b_dev.stride(0), ld_out=L_,
+                                 off_d=o["D"], off_c=o["C"], off_u=o["U"], off_cc=o["CC"], off_cc2=o["CC2"],
+                                 off_p=o["P"], off_m=o["M"], off_x=o["X"], pmax_sum=float(self.pmax.sum()),
+                                 c=c_dev.data_ptr(), b=b_dev.data_ptr(), pmax=pm.data_ptr(), cost=co.data_ptr(),
+                                 demand=de.data_ptr(), out=V.data_ptr())
+        L.check(L.lib().pdlp_dispatch_inputs(C.byref(a), C.c_void_p(st.cuda_stream)), "pdlp_dispatch_inputs")
+        return V
+
+    def node_inputs_staged(self, problems, eng):
+        """Network inputs of a batch the family engine ``eng`` has just loaded: on the
+        device from its staged costs and right-hand sides (:meth:`node_inputs_device`,
+        on the engine's stream), or on the host when the engine stores either once."""
+        c_dev, b_dev = eng.staged("c"), eng.staged("b")
+        if c_dev is None or b_dev is None:
+            return self.node_inputs_from(problems)
+        return self.node_inputs_device(c_dev, b_dev, eng.stream)
+
     def node_inputs(self, ids) -> np.ndarray:
         P = [self.params(int(k)) for k in ids]
         return self.node_inputs_params(np.stack([p[0] for p in P]), np.stack([p[1] for p in P]))

@@ -87,6 +87,27 @@ def test_node_kernels_reject_bad_descriptors_without_device():
     assert lib.pdlp_mlp_bwd_bf16(C.byref(m), None) == L.PDLP_E_ARG
 
 
+def test_dispatch_inputs_rejects_bad_descriptors_without_device():
+    lib = L.lib()
+    assert lib.pdlp_dispatch_inputs(None, None) == L.PDLP_E_ARG
+    T, G = 24, 200
+    ok = dict(count=4, G=G, T=T, ld_c=4800, ld_b=9224, ld_out=3 * T + 4 * G + T * G, off_d=0, off_c=T, off_u=T + G,
+              off_cc=T + 2 * G, off_cc2=T + 3 * G, off_p=T + 4 * G, off_m=2 * T + 4 * G, off_x=3 * T + 4 * G,
+              pmax_sum=1.0)
+    a = L.PdlpDispatchInputs(**ok)                                       # NULL arrays
+    assert lib.pdlp_dispatch_inputs(C.byref(a), None) == L.PDLP_E_ARG
+    a = L.PdlpDispatchInputs(**dict(ok, count=0))                       # nothing to do
+    assert lib.pdlp_dispatch_inputs(C.byref(a), None) == 0
+    a = L.PdlpDispatchInputs(**dict(ok, G=1025, ld_c=5000))
+    assert lib.pdlp_dispatch_inputs(C.byref(a), None) == L.PDLP_E_LIMIT
+    a = L.PdlpDispatchInputs(**dict(ok, ld_out=ok["ld_out"] - 1))       # the X block overruns the row
+    assert lib.pdlp_dispatch_inputs(C.byref(a), None) == L.PDLP_E_SHAPE
+    a = L.PdlpDispatchInputs(**dict(ok, ld_b=T - 1))
+    assert lib.pdlp_dispatch_inputs(C.byref(a), None) == L.PDLP_E_SHAPE
+    a = L.PdlpDispatchInputs(**dict(ok, pmax_sum=0.0))
+    assert lib.pdlp_dispatch_inputs(C.byref(a), None) == L.PDLP_E_ARG
+
+
 def test_argument_errors_without_device():
     lib = L.lib()
     rc = lib.pdlp_run(None, None, 1, 1, None)

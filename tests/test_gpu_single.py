@@ -246,11 +246,13 @@ def test_sell_dual_step_matches_csr_bitwise(gpu):
     assert c["status"] == "optimal" and abs(c["iterations"] - a["iterations"]) <= 500
 
 
+@pytest.mark.parametrize("tc", [64, 256, 1024])
 @pytest.mark.parametrize("mode", [2, 3, 4, 5, 6, 7])
-def test_workgroup_sell_dual_step_agrees(gpu, mode):
-    """Slices and CSR rows in one launch, a workgroup per slice (entries split among
-    its wavefronts) or per row: the same optimum as the CSR step, every run the
-    same bits, and the iteration counted once per step."""
+def test_workgroup_sell_dual_step_agrees(gpu, mode, tc):
+    """Slices and CSR rows in one launch, a workgroup per slice piece (entries split
+    among its wavefronts) and tc threads per CSR row (a wavefront, several, or the
+    whole workgroup): the same optimum as the CSR step, every run the same bits,
+    and the iteration counted once per step."""
     from sovereign_solver_amd.engine.forms import box_form
     from sovereign_solver_amd.engine.single import SingleEngine
     from sovereign_solver_amd.engine.tune import Tune
@@ -258,7 +260,7 @@ def test_workgroup_sell_dual_step_agrees(gpu, mode):
     f = box_form(p)
     base = Tune(tpr_primal=1, tpr_dual=8, block=256, check_every=50)
     a = SingleEngine(f, "fp32", tune=base).solve(1e-4, 50_000)
-    eng = SingleEngine(f, "fp32", tune=base.with_(sell=mode))
+    eng = SingleEngine(f, "fp32", tune=base.with_(sell=mode, tpr_dual=tc))
     assert eng.sell_slices >= 1 and eng.n_csr_rows > 0
     b = eng.solve(1e-4, 50_000)
     c = eng.solve(1e-4, 50_000)
