@@ -305,3 +305,21 @@ def test_family_reload_matches_a_fresh_engine(gpu):
     o2, x2, y2 = FamilyEngine(FamilyBatch(b)).solve(1e-4, 100_000)
     assert np.all(o1["status"] == 1)
     assert np.array_equal(o1, o2) and np.array_equal(x1, x2) and np.array_equal(y1, y2)
+
+
+def test_rows_per_wavefront_same_bits():
+    """Rows / columns per wavefront change where each row and column runs, not its
+    arithmetic: every choice gives the same bits."""
+    fam = DispatchFamily(G=40, T=12)
+    probs = [fam.problem(10**6 + k) for k in range(70)]
+    res = []
+    for ipw in (1, 2, 3, 4):
+        eng = FamilyEngine(FamilyBatch(probs), check_every=50, items_per_wave=ipw)
+        o, X, Y = eng.solve(1e-4, 100_000)
+        res.append((o.copy(), X.copy(), Y.copy()))
+    o0, X0, Y0 = res[0]
+    assert np.all(o0["status"] == 1)
+    for o, X, Y in res[1:]:
+        assert np.array_equal(o["iterations"], o0["iterations"])
+        assert np.array_equal(X, X0) and np.array_equal(Y, Y0)
+    assert yardstick.rel_kkt(probs[0], X0[0], Y0[0])[0] <= 1.02e-4
