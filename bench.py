@@ -250,6 +250,7 @@ def spmv_rates(eng, reps: int = 50, kx_tpr: int = 64, traffic_key=None):
     graph of ``reps`` back-to-back launches timed with events on its stream.
     Algorithmic bytes per product: 8 nnz + 4 (rows + 1) + 4 cols + 4 rows (FP32
     values, int32 indices, x gathered once, y written once)."""
+    import ctypes as C
     import torch
     from sovereign_solver_amd.engine import _lib as L
     from sovereign_solver_amd.engine.device import ptr, stream_handle
@@ -260,12 +261,20 @@ def spmv_rates(eng, reps: int = 50, kx_tpr: int = 64, traffic_key=None):
              ("K_x", eng.row_ptr, eng.col_idx, eng.val, eng.x, eng.m, eng.n, kx_tpr)]
     if getattr(eng, "t_sell_slices", 0):       # K^T y over the SELL-64 copy of K^T (pdlp_spmv_sell64)
         cases.insert(1, ("KT_y_sell", eng.t_sell_ptr, eng.t_sell_idx, eng.t_sell_val, eng.y, eng.n, eng.m, 0))
+    tune_c = None
+    if getattr(eng, "sell_slices", 0) and 2 <= eng.tune.sell <= 7:
+        # K x over the engine's SELL-64 rows and CSR rows in its tuned one-launch layout (pdlp_spmv_dev)
+        cases.append(("K_x_sell", eng.row_ptr, eng.col_idx, eng.val, eng.x, eng.m, eng.n, -1))
+        tune_c = eng.tune.to_c()
     for name, rp, ci, val, x, rows, cols, tpr in cases:
         with torch.cuda.stream(s):
             y = torch.empty(rows, dtype=torch.float32, device=eng.device)
 
         def launch():
-            if tpr == 0:
+            if tpr == -1:
+                L.check(lib.pdlp_spmv_dev(C.byref(eng.dev), C.byref(tune_c), ptr(x), ptr(y),
+                                          stream_handle(torch.cuda.current_stream())), "pdlp_spmv_dev")
+            elif tpr == 0:
                 L.check(lib.pdlp_spmv_sell64(L.PDLP_F32, ptr(rp), ptr(ci), ptr(val), ptr(x), ptr(y), rows,
                                              stream_handle(torch.cuda.current_stream())), "pdlp_spmv_sell64")
             else:
@@ -290,7 +299,8 @@ def spmv_rates(eng, reps: int = 50, kx_tpr: int = 64, traffic_key=None):
         ptrs = (int(rp.numel()) if tpr == 0 else rows + 1)
         nbytes = 8 * nnz + 4 * ptrs + 4 * cols + 4 * rows
         gbs = nbytes / (us * 1e-6) / 1e9
-        out[name] = {"rows": rows, "nnz": nnz, "lanes_per_row": tpr, "bytes": nbytes, "us": us,
+        out[name] = {"rows": rows, "nnz": nnz, "lanes_per_row": tpr if tpr >= 0 else "sell_wg", "bytes": nbytes,
+                     "us": us,
                      "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS,
                      "traffic": measured_traffic(f"{traffic_key}/spmv_{name}") if traffic_key else None}
         del g
@@ -662,8 +672,8 @@ def run_transport(args, sources=1000, sinks=2000, cpu=True):
     bd = dual_bytes(form, 4)
     gp, gd = bp / (us_p * 1e-6) / 1e9, bd / (us_d * 1e-6) / 1e9
     spmv = spmv_rates(eng, traffic_key=f"transport_{sources}x{sinks}")
-    log(f"primal step {us_p:.2f} us ({gp:.0f} GB/s), dual step {us_d:.2f} us ({gd:.0f} GB/s); SpMV K^T y "
-        f"{spmv['KT_y']['achieved_gbs']:.0f}, K x {spmv['K_x']['achieved_gbs']:.0f} GB/s")
+    log(f"primal step {us_p:.2f} us ({gp:.0f} GB/s), dual step {us_d:.2f} us ({gd:.0f} GB/s); SpMV "
+        + ", ".join(f"{k} {v['achieved_gbs']:.0f}" for k, v in spmv.items()) + " GB/s")
     key = f"transport_{sources}x{sinks}"
     rec = {
         "workload": f"C1 single transportation LP {sources}x{sinks} (m={form.m}, n={form.n}, nnz={form.nnz}), "
@@ -963,8 +973,8 @@ def run_beyond_mall(args, sources=4000, sinks=8000):
     gp, gd = bp / (us_p * 1e-6) / 1e9, bd / (us_d * 1e-6) / 1e9
     key = f"transport_{sources}x{sinks}"
     spmv = spmv_rates(eng, reps=10, kx_tpr=64, traffic_key=key)
-    log(f"beyond MALL: primal step {us_p:.1f} us ({gp:.0f} GB/s), dual step {us_d:.1f} us ({gd:.0f} GB/s), "
-        f"SpMV K^T y {spmv['KT_y']['achieved_gbs']:.0f} GB/s, K x {spmv['K_x']['achieved_gbs']:.0f} GB/s")
+    log(f"beyond MALL: primal step {us_p:.1f} us ({gp:.0f} GB/s), dual step {us_d:.1f} us ({gd:.0f} GB/s), SpMV "
+        + ", ".join(f"{k} {v['achieved_gbs']:.0f}" for k, v in spmv.items()) + " GB/s")
     rec = {
         "workload": f"transportation LP {sources}x{sinks} (m={form.m}, n={form.n}, nnz={form.nnz}): K and K^T "
                     f"0.5 GB each, working set of a step ~{bp / 1e9:.2f} GB > 256 MB Infinity Cache",

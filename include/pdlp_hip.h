@@ -278,6 +278,13 @@ int pdlp_dual_step(const PdlpDev* dev, const PdlpTune* tune, void* stream);
  * replacing pdhg_gpu.py:381-427 and its host round trip). */
 int pdlp_kkt_check(const PdlpDev* dev, const PdlpTune* tune, void* stream);
 
+/* y = K x (prec of dev) over the rows of dev: its SELL-64 slices (PdlpDev.sell_*) and
+ * the remaining CSR rows (csr_rows), in the one-launch layout of tune->sell (2..7)
+ * with tune->tpr_dual threads per CSR row — the product K xbar of the dual step
+ * (pdhg.py:217 / pdhg_gpu.py:222), the same sums, without its update.  Needs
+ * sell_slices > 0 (else pdlp_spmv_csr).  x: n entries, y: m entries. */
+int pdlp_spmv_dev(const PdlpDev* dev, const PdlpTune* tune, const void* x, void* y, void* stream);
+
 /* iters x (primal, dual), then a KKT check when check != 0. */
 int pdlp_run(const PdlpDev* dev, const PdlpTune* tune, int32_t iters,
              int32_t check, void* stream);

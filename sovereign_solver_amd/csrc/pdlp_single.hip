@@ -282,21 +282,25 @@ __global__ void __launch_bounds__(256) dual_sell_kernel(PdlpDev d) {
 // partial sums meet in LDS and are added in a fixed order (wavefront, then
 // phase).  The remaining workgroups take NT / TC rows of csr_rows each, TC
 // threads per row (TC / 64 wavefronts; their sums meet in LDS in wavefront
-// order).  Block 0 counts the iteration.
-template <typename T, int NT, int R, int TC>
-__global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d) {
+// order).  Block 0 counts the iteration.  SPMV: the same sums of K vx stored in
+// vy (pdlp_spmv_dev), no dual update and no iteration state.
+template <typename T, int NT, int R, int TC, bool SPMV = false>
+__global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d, const T* __restrict__ vx, T* __restrict__ vy) {
     constexpr int NW = NT / 64, P = 64 / R;
     static_assert(R >= 1 && R <= 64 && 64 % R == 0, "rows per workgroup divide the slice");
     static_assert(TC >= 64 && TC <= NT && NT % TC == 0 && TC % 64 == 0, "threads per CSR row");
     __shared__ T part[2][NW][64];
-    PdlpCtrl* ctrl = d.ctrl;
-    if (ctrl->status != PDLP_RUNNING) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {   // single writer; read only by later kernels
-        ctrl->iter += 1;
-        ctrl->window += 1;
+    T sigma_scale = T(1);
+    if constexpr (!SPMV) {
+        PdlpCtrl* ctrl = d.ctrl;
+        if (ctrl->status != PDLP_RUNNING) return;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {   // single writer; read only by later kernels
+            ctrl->iter += 1;
+            ctrl->window += 1;
+        }
+        sigma_scale = (T)ctrl->sigma_scale;
     }
-    const T sigma_scale = (T)ctrl->sigma_scale;
-    const T* __restrict__ xbar = (const T*)d.xbar;
+    const T* __restrict__ xbar = SPMV ? vx : (const T*)d.xbar;
     const int b = blockIdx.x;
     if (b < d.sell_slices * P) {
         const int sl = b / P, rb = (b % P) * R;
@@ -307,7 +311,7 @@ __global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d) {
         const int beg = d.sell_ptr[sl], width = (d.sell_ptr[sl + 1] - beg) >> 6;
         // the rows' own data, by the threads that apply the updates
         const int ir = threadIdx.x < R ? d.sell_rows[sl * 64 + rb + threadIdx.x] : -1;
-        const RowOwn<T> o = row_own<T>(d, ir, ir >= 0);
+        const RowOwn<T> o = row_own<T>(d, ir, !SPMV && ir >= 0);
         const int kc = (width + NW - 1) / NW;
         const int k0 = min(width, w * kc), k1 = min(width, k0 + kc);
         T ax = T(0), asum = T(0);
@@ -330,7 +334,10 @@ __global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d) {
                     tx += part[0][v][f * R + threadIdx.x];
                     ta += part[1][v][f * R + threadIdx.x];
                 }
-            if (ir >= 0) dual_update_own<T, false>(d, sigma_scale, ir, tx, ta, o);
+            if (ir >= 0) {
+                if constexpr (SPMV) vy[ir] = tx;
+                else dual_update_own<T, false>(d, sigma_scale, ir, tx, ta, o);
+            }
         }
         return;
     }
@@ -338,7 +345,7 @@ __global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d) {
     const int rr = (b - d.sell_slices * P) * (NT / TC) + (int)threadIdx.x / TC;
     const int tc = threadIdx.x % TC;
     const int i = rr < d.n_csr_rows ? d.csr_rows[rr] : -1;
-    const RowOwn<T> o = row_own<T>(d, i, i >= 0 && tc == 0);
+    const RowOwn<T> o = row_own<T>(d, i, !SPMV && i >= 0 && tc == 0);
     T ax = T(0), asum = T(0);
     if (i >= 0) {
         const int beg = d.row_ptr[i], end = d.row_ptr[i + 1];
@@ -381,7 +388,10 @@ __global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d) {
             }
         }
     }
-    if (i >= 0 && tc == 0) dual_update_own<T, false>(d, sigma_scale, i, ax, asum, o);
+    if (i >= 0 && tc == 0) {
+        if constexpr (SPMV) vy[i] = ax;
+        else dual_update_own<T, false>(d, sigma_scale, i, ax, asum, o);
+    }
 }
 
 // ------------------------------------------------------------- KKT partials
@@ -965,6 +975,36 @@ __global__ void __launch_bounds__(256) dual_sell_finish_kernel(PdlpDev d, int kc
     dual_update<T, false>(d, (T)ctrl->sigma_scale, i, ax, ((const T*)d.sell_asum)[s * 64 + lane]);
 }
 
+// The one-launch SELL kernel in the layout of t.sell (2..7): NT threads, R rows
+// per slice piece, tpr_dual threads per CSR row (64 .. NT).
+template <typename T, bool SPMV>
+void launch_sell_wg(const PdlpDev& d, const PdlpTune& t, const T* vx, T* vy, hipStream_t s) {
+    static constexpr int kRows[8] = {0, 0, 64, 64, 16, 8, 32, 16};
+    static constexpr int kThreads[8] = {0, 0, 1024, 512, 1024, 1024, 1024, 512};
+    const int nt = kThreads[t.sell];
+    const int tc = std::min(std::max(t.tpr_dual, 64), nt);
+    const int per = nt / tc;
+    const unsigned items = (unsigned)(d.sell_slices * (64 / kRows[t.sell]) + (d.n_csr_rows + per - 1) / per);
+    auto go = [&](auto nt_c, auto r_c) {
+        constexpr int NT = decltype(nt_c)::value, R = decltype(r_c)::value;
+        switch (tc) {
+            case 64: dual_sell_wg_kernel<T, NT, R, 64, SPMV><<<items, NT, 0, s>>>(d, vx, vy); break;
+            case 128: dual_sell_wg_kernel<T, NT, R, 128, SPMV><<<items, NT, 0, s>>>(d, vx, vy); break;
+            case 256: dual_sell_wg_kernel<T, NT, R, 256, SPMV><<<items, NT, 0, s>>>(d, vx, vy); break;
+            case 512: dual_sell_wg_kernel<T, NT, R, 512, SPMV><<<items, NT, 0, s>>>(d, vx, vy); break;
+            default: dual_sell_wg_kernel<T, NT, R, NT, SPMV><<<items, NT, 0, s>>>(d, vx, vy); break;
+        }
+    };
+    switch (t.sell) {
+        case 2: go(ic<1024>{}, ic<64>{}); break;
+        case 3: go(ic<512>{}, ic<64>{}); break;
+        case 4: go(ic<1024>{}, ic<16>{}); break;
+        case 5: go(ic<1024>{}, ic<8>{}); break;
+        case 6: go(ic<1024>{}, ic<32>{}); break;
+        default: go(ic<512>{}, ic<16>{}); break;
+    }
+}
+
 template <typename T, bool EXACT>
 int launch_dual_t(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
     const int remap = (t.xcd_remap >> 1) & 1;
@@ -978,31 +1018,7 @@ int launch_dual_t(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
     } else {
         if (sell && t.sell >= 2 && t.sell <= 7) {
             // slices and CSR rows in one launch (block 0 counts the iteration)
-            static constexpr int kRows[8] = {0, 0, 64, 64, 16, 8, 32, 16};
-            static constexpr int kThreads[8] = {0, 0, 1024, 512, 1024, 1024, 1024, 512};
-            const int nt = kThreads[t.sell];
-            const int tc = std::min(std::max(t.tpr_dual, 64), nt);
-            const int per = nt / tc;
-            const unsigned items =
-                (unsigned)(d.sell_slices * (64 / kRows[t.sell]) + (d.n_csr_rows + per - 1) / per);
-            auto go = [&](auto nt_c, auto r_c) {
-                constexpr int NT = decltype(nt_c)::value, R = decltype(r_c)::value;
-                switch (tc) {
-                    case 64: dual_sell_wg_kernel<T, NT, R, 64><<<items, NT, 0, s>>>(d); break;
-                    case 128: dual_sell_wg_kernel<T, NT, R, 128><<<items, NT, 0, s>>>(d); break;
-                    case 256: dual_sell_wg_kernel<T, NT, R, 256><<<items, NT, 0, s>>>(d); break;
-                    case 512: dual_sell_wg_kernel<T, NT, R, 512><<<items, NT, 0, s>>>(d); break;
-                    default: dual_sell_wg_kernel<T, NT, R, NT><<<items, NT, 0, s>>>(d); break;
-                }
-            };
-            switch (t.sell) {
-                case 2: go(ic<1024>{}, ic<64>{}); break;
-                case 3: go(ic<512>{}, ic<64>{}); break;
-                case 4: go(ic<1024>{}, ic<16>{}); break;
-                case 5: go(ic<1024>{}, ic<8>{}); break;
-                case 6: go(ic<1024>{}, ic<32>{}); break;
-                default: go(ic<512>{}, ic<16>{}); break;
-            }
+            launch_sell_wg<T, false>(d, t, nullptr, nullptr, s);
             return hip_fail(hipGetLastError(), "dual_sell_wg_kernel launch");
         }
         if (sell && t.sell == 1) {
@@ -1193,6 +1209,18 @@ int pdlp_graph_capture_step(const PdlpDev* dev, const PdlpTune* tune, int32_t wh
     if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
     *exec = (void*)ge;
     return 0;
+}
+
+int pdlp_spmv_dev(const PdlpDev* dev, const PdlpTune* tune, const void* x, void* y, void* stream) {
+    if (int rc = check_dev(dev)) return rc;
+    if (int rc = check_tune(tune)) return rc;
+    if (tune->sell < 2 || tune->sell > 7) return fail(PDLP_E_TUNE, "spmv_dev: tune->sell must be 2..7");
+    if (dev->sell_slices <= 0) return fail(PDLP_E_ARG, "spmv_dev: dev has no SELL-64 rows (use pdlp_spmv_csr)");
+    if (x == nullptr || y == nullptr) return fail(PDLP_E_ARG, "spmv_dev: x or y is NULL");
+    hipStream_t s = as_stream(stream);
+    if (dev->prec == PDLP_F32) launch_sell_wg<float, true>(*dev, *tune, (const float*)x, (float*)y, s);
+    else launch_sell_wg<double, true>(*dev, *tune, (const double*)x, (double*)y, s);
+    return hip_fail(hipGetLastError(), "spmv_dev launch");
 }
 
 int pdlp_graph_launch(void* exec, void* stream) {

@@ -198,6 +198,7 @@ _SIGS = {
     "pdlp_dual_step": (C.c_int, [C.POINTER(PdlpDev), C.POINTER(PdlpTune), _p]),
     "pdlp_kkt_check": (C.c_int, [C.POINTER(PdlpDev), C.POINTER(PdlpTune), _p]),
     "pdlp_run": (C.c_int, [C.POINTER(PdlpDev), C.POINTER(PdlpTune), C.c_int32, C.c_int32, _p]),
+    "pdlp_spmv_dev": (C.c_int, [C.POINTER(PdlpDev), C.POINTER(PdlpTune), _p, _p, _p]),
     "pdlp_run_noalloc": (C.c_int, [C.POINTER(PdlpDev), C.POINTER(PdlpTune), C.c_int32, C.c_int32, _p]),
     "pdlp_graph_capture": (C.c_int, [C.POINTER(PdlpDev), C.POINTER(PdlpTune), C.c_int32, _p,
                                      C.POINTER(C.c_void_p)]),

@@ -353,3 +353,42 @@ def test_spmv_sell64_same_bits_as_one_lane_csr(gpu, shape, monkeypatch):
     assert np.array_equal(a.cpu().numpy(), b.cpu().numpy())
     want = (p.A_eq if p.n_ineq == 0 else sp.vstack([p.A_ub, p.A_eq])).T.tocsr() @ x.cpu().numpy().astype(np.float64)
     assert np.allclose(b.cpu().numpy(), want, rtol=1e-5, atol=1e-4)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_spmv_dev_over_slices_and_csr_rows(gpu, prec):
+    """pdlp_spmv_dev: y = K x over an engine's SELL-64 rows and CSR rows in every
+    one-launch layout (sell 2..7) and CSR-row width: within the rounding bound of
+    the row's length against the FP64 product, and the same bits on a repeat."""
+    import torch
+    from sovereign_solver_amd.engine.device import ptr, stream_handle
+    from sovereign_solver_amd.engine.forms import box_form
+    from sovereign_solver_amd.engine.single import SingleEngine
+    from sovereign_solver_amd.engine.tune import Tune
+    p = transportation_lp(70, 150, seed=2)
+    eng = SingleEngine(box_form(p), prec)
+    assert eng.sell_slices > 0 and eng.n_csr_rows > 0
+    dt = torch.float32 if prec == "fp32" else torch.float64
+    rng = np.random.default_rng(1)
+    x = torch.from_numpy(rng.standard_normal(eng.n)).to(gpu, dt)
+    K = sp.csr_matrix((eng.val.double().cpu().numpy(), eng.col_idx.cpu().numpy(), eng.row_ptr.cpu().numpy()),
+                      shape=(eng.m, eng.n))
+    xd = x.double().cpu().numpy()
+    want = K @ xd
+    eps = 2.0**-24 if prec == "fp32" else 2.0**-53
+    bound = (np.diff(K.indptr).max() + 2) * eps * (abs(K) @ abs(xd)) + 1e-300
+    sh = stream_handle(torch.cuda.current_stream())
+    for mode in (2, 3, 4, 5, 6, 7):
+        for tc in (64, 256, 1024):
+            t = Tune(tpr_primal=1, tpr_dual=tc, block=256, check_every=50, sell=mode).to_c()
+            ys = []
+            for _ in range(2):
+                y = torch.full((eng.m,), float("nan"), dtype=dt, device=gpu)
+                L.check(L.lib().pdlp_spmv_dev(C.byref(eng.dev), C.byref(t), ptr(x), ptr(y), sh), "spmv_dev")
+                ys.append(y)
+            torch.cuda.synchronize()
+            a, b = ys[0].double().cpu().numpy(), ys[1].double().cpu().numpy()
+            assert np.array_equal(a, b), (mode, tc)
+            assert np.all(np.abs(a - want) <= bound), (mode, tc, np.abs(a - want).max())
+    bad = Tune(tpr_primal=1, tpr_dual=64, block=256, check_every=50, sell=1).to_c()
+    assert L.lib().pdlp_spmv_dev(C.byref(eng.dev), C.byref(bad), ptr(x), ptr(x), sh) == L.PDLP_E_TUNE
