@@ -212,6 +212,13 @@ typedef struct PdlpTune {
 int         pdlp_abi_version(void);
 const char* pdlp_last_error(void);
 
+/* width[s] = the longest row of rows 64 s .. 64 s + 63 of a CSR matrix (rows past
+ * the end count 0), for s < ceil(rows / 64): the slice widths of its SELL-64 copy
+ * (the host's sell_ptr is 64 times their running sum, as in pdlp_csr_to_sell64).
+ * The reference builds no such copy (pdhg_gpu.py:137,161,186 keep CSR): no
+ * counterpart. */
+int pdlp_sell64_widths(int32_t rows, const int32_t* row_ptr, int32_t* width, void* stream);
+
 /* SELL-64 copy of a CSR matrix's rows (the K^T of PdlpDev.t_sell_*): slice s holds
  * rows 64 s .. 64 s + 63, its width the longest of them; entry k of row 64 s + l goes
  * to sell_ptr[s] + 64 k + l.  sell_ptr (slices + 1, slices = ceil(rows / 64)) is the

@@ -110,6 +110,17 @@ __global__ void __launch_bounds__(256) csr_to_sell64_kernel(int rows, const int3
     }
 }
 
+// Width of each SELL-64 slice: the longest of its 64 rows (rows past the end count 0).
+__global__ void __launch_bounds__(256) sell64_widths_kernel(int rows, const int32_t* __restrict__ row_ptr,
+                                                            int32_t* __restrict__ width) {
+    const int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int r = s * 64 + (threadIdx.x & 63);
+    int len = r < rows ? row_ptr[r + 1] - row_ptr[r] : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o, 64));
+    if ((threadIdx.x & 63) == 0 && s * 64 < rows) width[s] = len;
+}
+
 }  // namespace
 }  // namespace pdlp
 
@@ -133,6 +144,15 @@ int pdlp_csr_to_sell64(int32_t prec, int32_t rows, const int32_t* row_ptr, const
         csr_to_sell64_kernel<double><<<g, 256, 0, s>>>(rows, row_ptr, col_idx, (const double*)val, sell_ptr,
                                                        sell_idx, (double*)sell_val);
     return hip_fail(hipGetLastError(), "csr_to_sell64_kernel launch");
+}
+
+int pdlp_sell64_widths(int32_t rows, const int32_t* row_ptr, int32_t* width, void* stream) {
+    if (rows < 0) return fail(PDLP_E_SHAPE, "sell64_widths: rows >= 0");
+    if (rows == 0) return 0;
+    if (!row_ptr || !width) return fail(PDLP_E_ARG, "sell64_widths: NULL pointer");
+    const int slices = (rows + 63) / 64;
+    sell64_widths_kernel<<<(unsigned)((slices + 3) / 4), 256, 0, as_stream(stream)>>>(rows, row_ptr, width);
+    return hip_fail(hipGetLastError(), "sell64_widths_kernel launch");
 }
 
 int pdlp_csr_transpose_work_bytes(int32_t rows, int32_t cols, int32_t nnz, size_t* bytes) {

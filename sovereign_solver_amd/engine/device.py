@@ -3,6 +3,8 @@
 from __future__ import annotations
 
 import ctypes as C
+import threading
+from typing import Optional
 
 import numpy as np
 import torch
@@ -21,6 +23,65 @@ def dev_tensor(a, dtype, device) -> torch.Tensor:
     """Upload a host array (contiguous) as a device tensor of ``dtype``."""
     t = torch.from_numpy(np.ascontiguousarray(a))
     return t.to(device=device, dtype=dtype, non_blocking=False).contiguous()
+
+
+_NP_OF = {torch.float32: np.float32, torch.float64: np.float64, torch.int32: np.int32, torch.int8: np.int8,
+          torch.int16: np.int16, torch.uint8: np.uint8, torch.int64: np.int64}
+_STAGING_KEEP = 512 << 20          # pinned staging kept between uploads up to this size
+
+
+class _Staging:
+    """One pinned host buffer reused by uploads (grown on demand), and the event of
+    the last copies out of it."""
+    lock = threading.Lock()
+    buf: Optional[torch.Tensor] = None
+    ev: Optional[torch.cuda.Event] = None
+
+
+def upload(items, device, stream: torch.cuda.Stream) -> list:
+    """Device tensors of host arrays ``items`` = [(array or None, torch dtype), ...]: each
+    converted to its dtype while copied into one pinned staging buffer, then copied
+    to the device asynchronously on ``stream`` (None stays None).  The staging buffer
+    is kept for the next upload (up to 512 MB); an upload first waits for the
+    previous one's copies out of it."""
+    arrs, offs, total = [], [], 0
+    for a, dt in items:
+        if a is None:
+            arrs.append(None)
+            offs.append(0)
+            continue
+        a = np.asarray(a).reshape(-1)
+        arrs.append((a, dt))
+        offs.append(total)
+        total += (a.size * torch.empty(0, dtype=dt).element_size() + 255) // 256 * 256
+    out = []
+    with _Staging.lock:
+        if _Staging.ev is not None:
+            _Staging.ev.synchronize()
+        buf = _Staging.buf
+        if buf is None or buf.numel() < total:
+            buf = torch.empty(max(total, 1 << 20), dtype=torch.uint8).pin_memory()
+        host = buf.numpy()
+        with torch.cuda.stream(stream):
+            for it, off in zip(arrs, offs):
+                if it is None:
+                    out.append(None)
+                    continue
+                a, dt = it
+                nb = a.size * torch.empty(0, dtype=dt).element_size()
+                np.copyto(host[off: off + nb].view(_NP_OF[dt]), a, casting="unsafe")
+                d = torch.empty(a.size, dtype=dt, device=device)
+                if a.size:
+                    d.copy_(buf[off: off + nb].view(dt), non_blocking=True)
+                out.append(d)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        if buf.numel() <= _STAGING_KEEP:
+            _Staging.buf, _Staging.ev = buf, ev
+        else:                              # too large to keep: released once its copies are done
+            ev.synchronize()
+            _Staging.ev = None
+    return out
 
 
 def ptr(t) -> int:

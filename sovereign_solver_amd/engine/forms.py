@@ -121,7 +121,7 @@ def box_form(p: Problem, scaling: str = "none") -> HostForm:
     lb_f = np.where(np.isfinite(lb), lb, 0.0)
     fin_ub = np.isfinite(ub)
     kind = np.concatenate([np.full(m_ub, ROW_LE, np.int8), np.full(m_eq, ROW_EQ, np.int8)])
-    b_shift = b - K @ lb_f
+    b_shift = b - K @ lb_f if lb_f.any() else b.copy()
     span = np.where(fin_ub, ub - lb_f, np.inf)
     norm_b = max(1.0, float(np.sqrt(b_shift @ b_shift + np.sum(span[fin_ub] ** 2))))
     c = np.asarray(p.c, float)

@@ -75,23 +75,30 @@ def build(K: sp.csr_matrix, line_elems: int = 32, max_lines: float = 4.0, max_pa
         W = int(lens.max())
         if W * SLICE > max_pad * lens.sum():
             continue
-        k = np.arange(W)
-        valid = k[None, :] < lens[:, None]                                  # [rows, W]
-        pos = np.minimum(indptr[r][:, None] + k[None, :], len(indices) - 1)
-        cols = np.where(valid, indices[pos], -1)
-        lines = np.sort(np.where(valid, cols // line_elems, -1), axis=0)
-        distinct = 1 + (np.diff(lines, axis=0) != 0).sum(axis=0)           # per k (padding counts once)
+        if lens.min() == W and r[-1] - r[0] == r.size - 1:
+            # consecutive rows of equal length: their entries are one block, [rows, W]
+            e0 = indptr[r[0]]
+            cols = indices[e0: e0 + r.size * W].reshape(r.size, W).T       # [W, rows]: entry-major
+            vals = data[e0: e0 + r.size * W].reshape(r.size, W).T
+            lines = np.sort(cols // line_elems, axis=1)
+        else:
+            k = np.arange(W)
+            valid = k[:, None] < lens[None, :]
+            pos = np.minimum(indptr[r][None, :] + k[:, None], len(indices) - 1)
+            cols = np.where(valid, indices[pos], 0)
+            vals = np.where(valid, data[pos], 0.0)
+            lines = np.sort(np.where(valid, cols // line_elems, -1), axis=1)
+        distinct = 1 + (np.diff(lines, axis=1) != 0).sum(axis=1)           # per k (padding counts once)
         if distinct.mean() > max_lines:
             continue
-        pad = SLICE - r.size                                                  # lanes without a row
         in_slice[r] = True
-        rows_out.append(np.concatenate([r, np.full(pad, -1)]).astype(np.int32))
-        c = np.zeros((SLICE, W), np.int32)
-        v = np.zeros((SLICE, W))
-        c[: r.size] = np.where(valid, cols, 0)
-        v[: r.size] = np.where(valid, data[pos], 0.0)
-        col_out.append(c.T.ravel())                                           # column-major: [W, 64]
-        val_out.append(v.T.ravel())
+        rows_out.append(np.concatenate([r, np.full(SLICE - r.size, -1)]).astype(np.int32))  # -1: no row
+        c = np.zeros((W, SLICE), np.int32)                                  # column-major: [W, 64]
+        v = np.zeros((W, SLICE))
+        c[:, : r.size] = cols
+        v[:, : r.size] = vals
+        col_out.append(c.ravel())
+        val_out.append(v.ravel())
         ptr.append(ptr[-1] + W * SLICE)
     if len(ptr) == 1:
         return SellSlices(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0),
@@ -111,8 +118,6 @@ def build_columns(K: sp.csr_matrix, padding_max=None):
     are filled on the device (``pdlp_csr_to_sell64``).  None when the padding would exceed
     ``padding_max`` (default :data:`COLUMN_PADDING_MAX`) of K's entries, when K has no
     entries, or beyond int32 indexing."""
-    if padding_max is None:
-        padding_max = COLUMN_PADDING_MAX
     n = K.shape[1]
     if K.nnz == 0 or n == 0:
         return None
@@ -120,9 +125,34 @@ def build_columns(K: sp.csr_matrix, padding_max=None):
     slices = -(-n // SLICE)
     padded = np.zeros(slices * SLICE, np.int64)
     padded[:n] = lens
-    width = padded.reshape(slices, SLICE).max(axis=1)
-    ptr = np.zeros(slices + 1, np.int64)
-    ptr[1:] = np.cumsum(SLICE * width)
-    if ptr[-1] >= 2**31 or ptr[-1] > (1.0 + padding_max) * K.nnz:
+    return _column_pointers(padded.reshape(slices, SLICE).max(axis=1), K.nnz, padding_max)
+
+
+def _column_pointers(width: np.ndarray, nnz: int, padding_max=None):
+    if padding_max is None:
+        padding_max = COLUMN_PADDING_MAX
+    ptr = np.zeros(width.size + 1, np.int64)
+    ptr[1:] = np.cumsum(SLICE * np.asarray(width, np.int64))
+    if ptr[-1] >= 2**31 or ptr[-1] > (1.0 + padding_max) * nnz:
         return None
     return ptr.astype(np.int32)
+
+
+def build_columns_device(t_row_ptr, nnz: int, stream, padding_max=None):
+    """:func:`build_columns` from K^T's row pointers on the device (``t_row_ptr``, a
+    torch int32 tensor of n + 1 entries): the slice widths come from
+    ``pdlp_sell64_widths`` (one wavefront per slice), the pointers from their running
+    sum on the host.  Same result as :func:`build_columns` on K."""
+    import torch
+    from . import _lib as L
+    from .device import stream_handle
+    n = t_row_ptr.numel() - 1
+    if nnz == 0 or n == 0:
+        return None
+    slices = -(-n // SLICE)
+    with torch.cuda.stream(stream):
+        width = torch.empty(slices, dtype=torch.int32, device=t_row_ptr.device)
+        L.check(L.lib().pdlp_sell64_widths(n, t_row_ptr.data_ptr(), width.data_ptr(), stream_handle(stream)),
+                "pdlp_sell64_widths")
+        host = width.cpu()
+    return _column_pointers(host.numpy(), nnz, padding_max)

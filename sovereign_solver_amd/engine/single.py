@@ -19,7 +19,7 @@ import torch
 
 from . import _lib as L
 from . import sell
-from .device import csr_transpose, dev_tensor, ptr, require_gpu, stream_handle, struct_from_bytes
+from .device import csr_transpose, dev_tensor, ptr, require_gpu, stream_handle, struct_from_bytes, upload
 from ..config import WARMSTART_FAILOVER_ITERS
 from .forms import HostForm
 from .tune import Tune, heuristic_tune
@@ -59,25 +59,17 @@ class SingleEngine:
         i32 = torch.int32
         with torch.cuda.stream(self.stream):
             K = form.K
-            self.row_ptr = dev_tensor(K.indptr, i32, dv)
-            self.col_idx = dev_tensor(K.indices, i32, dv)
-            self.val = dev_tensor(K.data, T, dv)
+            # the LP's arrays through one pinned staging buffer, converted on the way
+            # (the scale vectors D_r, D_c of a scaled form serve the original LP's KKT)
+            (self.row_ptr, self.col_idx, self.val, self.c, self.lo, self.hi, self.b, self.row_kind, self.c64,
+             self.b64, self.b_shift, self.lo_shift, self.ub_span, self.row_scale, self.col_scale) = upload(
+                [(K.indptr, i32), (K.indices, i32), (K.data, T), (form.c, T), (form.lo, T), (form.hi, T),
+                 (form.b, T), (form.row_kind, torch.int8), (form.c, f64), (form.b, f64), (form.b_shift, f64),
+                 (form.lo_shift, f64), (form.ub_span, f64), (form.row_scale, f64), (form.col_scale, f64)],
+                dv, self.stream)
             # K^T for the primal step, built on the device from the uploaded K
             self.t_row_ptr, self.t_col_idx, self.t_val = csr_transpose(self.row_ptr, self.col_idx, self.val,
                                                                        self.n, self.stream)
-            self.c = dev_tensor(form.c, T, dv)
-            self.lo = None if form.lo is None else dev_tensor(form.lo, T, dv)
-            self.hi = None if form.hi is None else dev_tensor(form.hi, T, dv)
-            self.b = dev_tensor(form.b, T, dv)
-            self.row_kind = None if form.row_kind is None else dev_tensor(form.row_kind, torch.int8, dv)
-            self.c64 = dev_tensor(form.c, f64, dv)
-            self.b64 = dev_tensor(form.b, f64, dv)
-            self.b_shift = dev_tensor(form.b_shift, f64, dv)
-            self.lo_shift = None if form.lo_shift is None else dev_tensor(form.lo_shift, f64, dv)
-            self.ub_span = None if form.ub_span is None else dev_tensor(form.ub_span, f64, dv)
-            # scaled form: D_r, D_c for the KKT yardstick of the original LP
-            self.row_scale = None if form.row_scale is None else dev_tensor(form.row_scale, f64, dv)
-            self.col_scale = None if form.col_scale is None else dev_tensor(form.col_scale, f64, dv)
             self.x = torch.zeros(self.n, dtype=T, device=dv)
             self.xbar = torch.zeros(self.n, dtype=T, device=dv)
             self.y = torch.zeros(self.m, dtype=T, device=dv)
@@ -97,11 +89,9 @@ class SingleEngine:
             sl = None if exact else sell.build(K)
             self.sell_slices = 0 if sl is None else sl.slices
             if self.sell_slices:
-                self.sell_ptr = dev_tensor(sl.ptr, i32, dv)
-                self.sell_rows = dev_tensor(sl.rows, i32, dv)
-                self.sell_col = dev_tensor(sl.col, i32, dv)
-                self.sell_val = dev_tensor(sl.val, T, dv)
-                self.csr_rows = dev_tensor(sl.csr_rows, i32, dv) if sl.csr_rows.size else None
+                self.sell_ptr, self.sell_rows, self.sell_col, self.sell_val, self.csr_rows = upload(
+                    [(sl.ptr, i32), (sl.rows, i32), (sl.col, i32), (sl.val, T),
+                     (sl.csr_rows if sl.csr_rows.size else None, i32)], dv, self.stream)
                 self.n_csr_rows = int(sl.csr_rows.size)
                 widths = np.diff(sl.ptr) // sell.SLICE
                 self.sell_max_width = int(widths.max())
@@ -120,7 +110,7 @@ class SingleEngine:
             # columns of each slice have nearly equal lengths (padding at most 1/8 of K)
             self.t_sell_slices = 0
             self.t_sell_ptr = self.t_sell_idx = self.t_sell_val = None
-            tp = None if exact else sell.build_columns(K)
+            tp = None if exact else sell.build_columns_device(self.t_row_ptr, self.nnz, self.stream)
             if tp is not None:
                 self.t_sell_slices = len(tp) - 1
                 self.t_sell_ptr = dev_tensor(tp, i32, dv)

@@ -212,9 +212,50 @@ def test_sell_columns_layout_and_padding_limit():
 
 def test_csr_to_sell64_rejects_bad_arguments_without_device():
     lib = L.lib()
+    assert lib.pdlp_sell64_widths(-1, None, None, None) == L.PDLP_E_SHAPE
+    assert lib.pdlp_sell64_widths(5, None, None, None) == L.PDLP_E_ARG
+    assert lib.pdlp_sell64_widths(0, None, None, None) == 0
     assert lib.pdlp_spmv_sell64(L.PDLP_F32, None, None, None, None, None, 5, None) == L.PDLP_E_ARG
     assert lib.pdlp_spmv_sell64(L.PDLP_F32, None, None, None, None, None, -1, None) == L.PDLP_E_ARG
     assert lib.pdlp_spmv_sell64(L.PDLP_F32, None, None, None, None, None, 0, None) == 0
     assert lib.pdlp_csr_to_sell64(9, 4, None, None, None, None, None, None, None) == L.PDLP_E_ARG
     assert lib.pdlp_csr_to_sell64(L.PDLP_F32, -1, None, None, None, None, None, None, None) == L.PDLP_E_SHAPE
     assert lib.pdlp_csr_to_sell64(L.PDLP_F32, 4, None, None, None, None, None, None, None) == L.PDLP_E_ARG
+
+
+def test_sell_row_slices_hold_each_row_in_csr_order():
+    """engine.sell.build: every row is in exactly one slice lane or in csr_rows, and a
+    slice lane holds its row's entries in CSR order, then zero padding — for equal
+    consecutive rows (one block of entries) and for rows of unequal length."""
+    import numpy as np
+    import scipy.sparse as sp
+    from sovereign_solver_amd.engine.forms import box_form
+    from sovereign_solver_amd.engine.sell import SLICE, build
+    from sovereign_solver_amd.workloads import transportation_lp
+    rng = np.random.default_rng(3)
+    m = 300
+    lens = rng.integers(3, 6, m)                                   # unequal rows, neighbouring columns
+    rows = np.repeat(np.arange(m), lens)
+    cols = np.concatenate([i + 400 * np.arange(k) for i, k in zip(range(m), lens)])
+    ragged = sp.csr_matrix((rng.standard_normal(rows.size), (rows, cols)), shape=(m, 400 * 6))
+    for K in (box_form(transportation_lp(100, 300, seed=1)).K, ragged):
+        K = sp.csr_matrix(K)
+        K.sort_indices()
+        sl = build(K)
+        assert sl.slices > 0
+        seen = np.zeros(K.shape[0], int)
+        seen[sl.csr_rows] += 1
+        for s in range(sl.slices):
+            w = (sl.ptr[s + 1] - sl.ptr[s]) // SLICE
+            c = sl.col[sl.ptr[s]: sl.ptr[s + 1]].reshape(w, SLICE)
+            v = sl.val[sl.ptr[s]: sl.ptr[s + 1]].reshape(w, SLICE)
+            for lane in range(SLICE):
+                r = sl.rows[s * SLICE + lane]
+                if r < 0:
+                    assert not v[:, lane].any()
+                    continue
+                seen[r] += 1
+                a, b = K.indptr[r], K.indptr[r + 1]
+                assert np.array_equal(c[: b - a, lane], K.indices[a:b])
+                assert np.array_equal(v[: b - a, lane], K.data[a:b]) and not v[b - a:, lane].any()
+        assert np.all(seen == 1)

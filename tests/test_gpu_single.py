@@ -392,3 +392,23 @@ def test_spmv_dev_over_slices_and_csr_rows(gpu, prec):
             assert np.all(np.abs(a - want) <= bound), (mode, tc, np.abs(a - want).max())
     bad = Tune(tpr_primal=1, tpr_dual=64, block=256, check_every=50, sell=1).to_c()
     assert L.lib().pdlp_spmv_dev(C.byref(eng.dev), C.byref(bad), ptr(x), ptr(x), sh) == L.PDLP_E_TUNE
+
+
+@pytest.mark.parametrize("shape", ["transport", "random"])
+def test_sell_columns_from_device_widths(gpu, shape):
+    """The SELL-64 copy's slice pointers from the device's slice widths
+    (pdlp_sell64_widths over K^T's row pointers) equal the host's from K."""
+    import torch
+    from sovereign_solver_amd.engine import sell as SL
+    from sovereign_solver_amd.engine.forms import box_form
+    p = transportation_lp(70, 150, seed=2) if shape == "transport" else random_sparse_lp(300, 700, 0.02, seed=4)
+    K = box_form(p).K
+    KT = K.T.tocsr()
+    s = torch.cuda.Stream()
+    trp = torch.from_numpy(KT.indptr.astype(np.int32)).to(gpu)
+    for pm in (10.0, 0.0):
+        a = SL.build_columns(K, padding_max=pm)
+        b = SL.build_columns_device(trp, K.nnz, s, padding_max=pm)
+        assert (a is None) == (b is None)
+        if a is not None:
+            assert np.array_equal(a, b)
