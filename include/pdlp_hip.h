@@ -228,6 +228,25 @@ int pdlp_sell64_widths(int32_t rows, const int32_t* row_ptr, int32_t* width, voi
 int pdlp_csr_to_sell64(int32_t prec, int32_t rows, const int32_t* row_ptr, const int32_t* col_idx,
                        const void* val, const int32_t* sell_ptr, int32_t* sell_idx, void* sell_val, void* stream);
 
+/* K^T of an arena of `mats` CSR matrices stored one after another (the persistent
+ * batch's layout, PdlpBatchLP.rp_off / nnz_off / cp_off): matrix s has rows
+ * row_off[s] .. row_off[s+1]-1 and columns col_off[s] .. col_off[s+1]-1 of the arena's
+ * block-diagonal union (rows, cols in total), its entries at nz_off[s] .. nz_off[s+1]-1
+ * (nnz in total) with 16-bit column indices local to the matrix, and its m + 1 row
+ * pointers at row_ptr[rp_off[s]..] relative to its first entry.  Writes each matrix's
+ * n + 1 column pointers at t_col_ptr[cp_off[s]..] (relative), and its K^T entries
+ * (16-bit local row indices, values) at the same entry range — per matrix bitwise
+ * SciPy's canonical transpose (the host restatement is one transpose of the union,
+ * engine/batch.py).  All offset arrays have mats + 1 entries.  `work` holds at least
+ * pdlp_arena_transpose_work_bytes(...) bytes.  Replaces the per-LP host transposes
+ * of pdhg.py:179 for a batch; the reference has no batch: no direct counterpart. */
+int pdlp_arena_transpose_work_bytes(int32_t mats, int32_t rows, int32_t cols, int32_t nnz, size_t* bytes);
+int pdlp_arena_transpose(int32_t prec, int32_t mats, int32_t rows, int32_t cols, int32_t nnz,
+                         const int32_t* row_off, const int32_t* col_off, const int32_t* nz_off,
+                         const int32_t* rp_off, const int32_t* cp_off, const int32_t* row_ptr,
+                         const uint16_t* col_idx, const void* val, int32_t* t_col_ptr, uint16_t* t_row_idx,
+                         void* t_val, void* work, size_t work_bytes, void* stream);
+
 /* Device CSR transpose for problem ingestion: K^T (cols x rows) from K
  * (rows x cols, CSR with ascending column indices per row), values of `prec`.
  * Replaces the host transposes `AT = A.T.tocsr()` of pdhg.py:179 and

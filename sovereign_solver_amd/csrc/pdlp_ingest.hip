@@ -110,6 +110,85 @@ __global__ void __launch_bounds__(256) csr_to_sell64_kernel(int rows, const int3
     }
 }
 
+// Arena transpose (many CSR matrices stored one after another, 16-bit local
+// indices): the block-diagonal union's global row pointers and column indices,
+// then the one transpose, then local row indices and per-matrix column pointers.
+// In the union, matrix s's entries occupy [nz_off[s], nz_off[s+1]) both in K and in
+// K^T, so K^T's values need no further move.
+
+// index s with off[s] <= v < off[s+1] (off ascending, off[0] = 0, v < off[S]);
+// empty ranges are skipped (the last s with off[s] <= v)
+__device__ __forceinline__ int range_of(const int32_t* __restrict__ off, int S, int v) {
+    int lo = 0, hi = S - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) arena_rows_kernel(int S, int rows, const int32_t* __restrict__ row_off,
+                                                         const int32_t* __restrict__ nz_off,
+                                                         const int32_t* __restrict__ rp_off,
+                                                         const int32_t* __restrict__ row_ptr,
+                                                         int32_t* __restrict__ gptr) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g > rows) return;
+    if (g == rows) {
+        gptr[rows] = nz_off[S];
+        return;
+    }
+    const int s = range_of(row_off, S, g);
+    gptr[g] = nz_off[s] + row_ptr[rp_off[s] + (g - row_off[s])];
+}
+
+__global__ void __launch_bounds__(256) arena_cols_kernel(int S, int nnz, const int32_t* __restrict__ nz_off,
+                                                         const int32_t* __restrict__ col_off,
+                                                         const uint16_t* __restrict__ col_idx,
+                                                         int32_t* __restrict__ gcol) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    gcol[e] = (int32_t)col_idx[e] + col_off[range_of(nz_off, S, e)];
+}
+
+__global__ void __launch_bounds__(256) arena_local_rows_kernel(int S, int nnz, const int32_t* __restrict__ nz_off,
+                                                               const int32_t* __restrict__ row_off,
+                                                               const int32_t* __restrict__ grow,
+                                                               uint16_t* __restrict__ t_row_idx) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nnz) return;
+    t_row_idx[q] = (uint16_t)(grow[q] - row_off[range_of(nz_off, S, q)]);
+}
+
+__global__ void __launch_bounds__(256) arena_col_ptr_kernel(int S, int slots, const int32_t* __restrict__ cp_off,
+                                                            const int32_t* __restrict__ col_off,
+                                                            const int32_t* __restrict__ nz_off,
+                                                            const int32_t* __restrict__ btp,
+                                                            int32_t* __restrict__ t_col_ptr) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= slots) return;
+    const int s = range_of(cp_off, S, q);
+    t_col_ptr[q] = btp[col_off[s] + (q - cp_off[s])] - nz_off[s];
+}
+
+struct ArenaLayout {
+    size_t gptr, gcol, btp, btc, work, work_bytes, total;
+};
+
+int arena_layout(int32_t rows, int32_t cols, int32_t nnz, ArenaLayout& a) {
+    size_t tw = 0;
+    if (int rc = pdlp_csr_transpose_work_bytes(rows, cols, nnz, &tw)) return rc;
+    a.gptr = 0;
+    a.gcol = align_up((size_t)(rows + 1) * 4);
+    a.btp = a.gcol + align_up((size_t)nnz * 4);
+    a.btc = a.btp + align_up((size_t)(cols + 1) * 4);
+    a.work = a.btc + align_up((size_t)nnz * 4);
+    a.work_bytes = tw;
+    a.total = a.work + align_up(tw);
+    return 0;
+}
+
 // Width of each SELL-64 slice: the longest of its 64 rows (rows past the end count 0).
 __global__ void __launch_bounds__(256) sell64_widths_kernel(int rows, const int32_t* __restrict__ row_ptr,
                                                             int32_t* __restrict__ width) {
@@ -153,6 +232,53 @@ int pdlp_sell64_widths(int32_t rows, const int32_t* row_ptr, int32_t* width, voi
     const int slices = (rows + 63) / 64;
     sell64_widths_kernel<<<(unsigned)((slices + 3) / 4), 256, 0, as_stream(stream)>>>(rows, row_ptr, width);
     return hip_fail(hipGetLastError(), "sell64_widths_kernel launch");
+}
+
+int pdlp_arena_transpose_work_bytes(int32_t mats, int32_t rows, int32_t cols, int32_t nnz, size_t* bytes) {
+    if (bytes == nullptr) return fail(PDLP_E_ARG, "arena_transpose: bytes is NULL");
+    if (mats < 1 || rows < 0 || cols < 0 || nnz < 0) return fail(PDLP_E_SHAPE, "arena_transpose: mats >= 1, sizes >= 0");
+    ArenaLayout a;
+    if (int rc = arena_layout(rows, cols, nnz, a)) return rc;
+    *bytes = a.total;
+    return 0;
+}
+
+int pdlp_arena_transpose(int32_t prec, int32_t mats, int32_t rows, int32_t cols, int32_t nnz,
+                         const int32_t* row_off, const int32_t* col_off, const int32_t* nz_off,
+                         const int32_t* rp_off, const int32_t* cp_off, const int32_t* row_ptr,
+                         const uint16_t* col_idx, const void* val, int32_t* t_col_ptr, uint16_t* t_row_idx,
+                         void* t_val, void* work, size_t work_bytes, void* stream) {
+    if (prec != PDLP_F32 && prec != PDLP_F64) return fail(PDLP_E_ARG, "arena_transpose: prec must be PDLP_F32 or PDLP_F64");
+    if (mats < 1 || rows < 0 || cols < 0 || nnz < 0) return fail(PDLP_E_SHAPE, "arena_transpose: mats >= 1, sizes >= 0");
+    if (!row_off || !col_off || !nz_off || !rp_off || !cp_off || !row_ptr || !t_col_ptr ||
+        (nnz > 0 && (!col_idx || !val || !t_row_idx || !t_val)))
+        return fail(PDLP_E_ARG, "arena_transpose: NULL array");
+    ArenaLayout a;
+    if (int rc = arena_layout(rows, cols, nnz, a)) return rc;
+    if (work == nullptr || work_bytes < a.total)
+        return fail(PDLP_E_ARG, "arena_transpose: work is smaller than pdlp_arena_transpose_work_bytes");
+    const hipStream_t s = as_stream(stream);
+    char* w = (char*)work;
+    int32_t* gptr = (int32_t*)(w + a.gptr);
+    int32_t* gcol = (int32_t*)(w + a.gcol);
+    int32_t* btp = (int32_t*)(w + a.btp);
+    int32_t* btc = (int32_t*)(w + a.btc);
+    constexpr int B = 256;
+    arena_rows_kernel<<<(unsigned)((rows + 1 + B - 1) / B), B, 0, s>>>(mats, rows, row_off, nz_off, rp_off, row_ptr,
+                                                                      gptr);
+    if (nnz > 0)
+        arena_cols_kernel<<<(unsigned)((nnz + B - 1) / B), B, 0, s>>>(mats, nnz, nz_off, col_off, col_idx, gcol);
+    if (int rc = hip_fail(hipGetLastError(), "arena index kernels launch")) return rc;
+    if (int rc = pdlp_csr_transpose(prec, rows, cols, nnz, gptr, gcol, val, btp, btc, t_val, w + a.work,
+                                    a.work_bytes, stream))
+        return rc;
+    if (nnz > 0)
+        arena_local_rows_kernel<<<(unsigned)((nnz + B - 1) / B), B, 0, s>>>(mats, nnz, nz_off, row_off, btc,
+                                                                           t_row_idx);
+    const int slots = cols + mats;                       // n + 1 pointers per matrix
+    arena_col_ptr_kernel<<<(unsigned)((slots + B - 1) / B), B, 0, s>>>(mats, slots, cp_off, col_off, nz_off, btp,
+                                                                       t_col_ptr);
+    return hip_fail(hipGetLastError(), "arena pointer kernels launch");
 }
 
 int pdlp_csr_transpose_work_bytes(int32_t rows, int32_t cols, int32_t nnz, size_t* bytes) {

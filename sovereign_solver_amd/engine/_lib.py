@@ -192,6 +192,10 @@ _SIGS = {
     "pdlp_spmv_csr": (C.c_int, [C.c_int32, _p, _p, _p, _p, _p, C.c_int32, C.c_int32, _p]),
     "pdlp_csr_to_sell64": (C.c_int, [C.c_int32, C.c_int32, _p, _p, _p, _p, _p, _p, _p]),
     "pdlp_sell64_widths": (C.c_int, [C.c_int32, _p, _p, _p]),
+    "pdlp_arena_transpose_work_bytes": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                                  C.POINTER(C.c_size_t)]),
+    "pdlp_arena_transpose": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _p, _p, _p, _p, _p,
+                                       _p, _p, _p, _p, _p, _p, _p, C.c_size_t, _p]),
     "pdlp_spmv_sell64": (C.c_int, [C.c_int32, _p, _p, _p, _p, _p, C.c_int32, _p]),
     "pdlp_spmv_csr_f64acc": (C.c_int, [_p, _p, _p, _p, _p, C.c_int32, C.c_int32, _p]),
     "pdlp_init": (C.c_int, [C.POINTER(PdlpDev), C.POINTER(PdlpTune), _p]),

@@ -109,9 +109,11 @@ class PackedBatch:
     normalisers stay the original LPs', and the engine maps starting points
     and solutions between the two spaces."""
 
-    def __init__(self, problems: Sequence[Problem], scaling: str = "none"):
+    def __init__(self, problems: Sequence[Problem], scaling: str = "none", transpose: str = "device"):
         if scaling not in S.METHODS:
             raise ValueError(f"scaling must be one of {S.METHODS}, got {scaling!r}")
+        if transpose not in ("device", "host"):
+            raise ValueError("transpose must be 'device' or 'host'")
         self.problems = list(problems)
         self.scaling = scaling
         B = len(self.problems)
@@ -167,25 +169,34 @@ class PackedBatch:
         col_local = np.concatenate([K.indices for K in stored])
         self.col_idx = col_local.astype(np.uint16)
         self.val = np.concatenate([K.data for K in stored])
-        # K^T by rows (K by columns) of every stored matrix at once: one transpose of their
-        # block-diagonal union (SciPy's counting sort; each column's entries stay in row order)
         s_ro = np.concatenate([[0], np.cumsum(s_m)[:-1]])        # first row / column of each matrix
         s_co = np.concatenate([[0], np.cumsum(s_n)[:-1]])
-        gptr = np.concatenate([np.concatenate([K.indptr[:-1] + z for K, z in zip(stored, s_nz)]),
-                               [int(s_z.sum())]]).astype(np.int32)
-        gcol = col_local.astype(np.int32)
-        gcol += np.repeat(s_co.astype(np.int32), s_z)
-        BT = sp.csr_matrix((self.val, gcol, gptr), shape=(int(s_m.sum()), int(s_n.sum()))).T.tocsr()
-        col_mat = np.repeat(np.arange(len(stored)), s_n)            # the matrix of each global column
-        rows = BT.indices.astype(np.int32, copy=False)
-        rows -= np.repeat(s_ro[col_mat].astype(np.int32), np.diff(BT.indptr))
-        self.t_row_idx = rows.astype(np.uint16)
-        self.t_val = BT.data
-        # per matrix its n + 1 column pointers, relative to its first entry
-        first = np.repeat(s_co, s_n + 1) + np.concatenate([np.arange(n + 1) for n in s_n])
-        self.t_col_ptr = (BT.indptr[first] - np.repeat(BT.indptr[s_co], s_n + 1)).astype(np.int32)
         s_cp = np.concatenate([[0], np.cumsum(s_n + 1)[:-1]])
         self.cp_off = s_cp[mat_of]
+        # per stored matrix (mats + 1 running offsets): rows, columns and entries in the
+        # block-diagonal union, row pointers and column pointers in the arena
+        cum = lambda v: np.concatenate([[0], np.cumsum(v)]).astype(np.int32)
+        self.arena = {"row_off": cum(s_m), "col_off": cum(s_n), "nz_off": cum(s_z), "rp_off": cum(s_m + 1),
+                      "cp_off": cum(s_n + 1)}
+        if transpose == "device":
+            # K^T built by the engine on the device (pdlp_arena_transpose), bitwise the host's below
+            self.t_row_idx = self.t_val = self.t_col_ptr = None
+        else:
+            # K^T by rows (K by columns) of every stored matrix at once: one transpose of their
+            # block-diagonal union (SciPy's counting sort; each column's entries stay in row order)
+            gptr = np.concatenate([np.concatenate([K.indptr[:-1] + z for K, z in zip(stored, s_nz)]),
+                                   [int(s_z.sum())]]).astype(np.int32)
+            gcol = col_local.astype(np.int32)
+            gcol += np.repeat(s_co.astype(np.int32), s_z)
+            BT = sp.csr_matrix((self.val, gcol, gptr), shape=(int(s_m.sum()), int(s_n.sum()))).T.tocsr()
+            col_mat = np.repeat(np.arange(len(stored)), s_n)            # the matrix of each global column
+            rows = BT.indices.astype(np.int32, copy=False)
+            rows -= np.repeat(s_ro[col_mat].astype(np.int32), np.diff(BT.indptr))
+            self.t_row_idx = rows.astype(np.uint16)
+            self.t_val = BT.data
+            # per matrix its n + 1 column pointers, relative to its first entry
+            first = np.repeat(s_co, s_n + 1) + np.concatenate([np.arange(n + 1) for n in s_n])
+            self.t_col_ptr = (BT.indptr[first] - np.repeat(BT.indptr[s_co], s_n + 1)).astype(np.int32)
 
         P = self.problems
         c = np.concatenate([np.asarray(p.c, float) for p in P])
@@ -295,9 +306,6 @@ class BatchEngine:
                 "row_ptr": dev_tensor(pk.row_ptr, i32, dv),
                 "col_idx": dev_tensor(pk.col_idx.view(np.int16), torch.int16, dv),
                 "val": dev_tensor(pk.val, f32, dv),
-                "t_col_ptr": dev_tensor(pk.t_col_ptr, i32, dv),
-                "t_row_idx": dev_tensor(pk.t_row_idx.view(np.int16), torch.int16, dv),
-                "t_val": dev_tensor(pk.t_val, f32, dv),
                 "c": dev_tensor(pk.c, f32, dv), "lo": dev_tensor(pk.lo, f32, dv), "hi": dev_tensor(pk.hi, f32, dv),
                 "c64": dev_tensor(pk.c, f64, dv), "lo_shift": dev_tensor(pk.lo_shift, f64, dv),
                 "ub_span": dev_tensor(pk.ub_span, f64, dv),
@@ -313,7 +321,34 @@ class BatchEngine:
             if pk.row_scale is not None:
                 self.t["row_scale"] = dev_tensor(pk.row_scale, f64, dv)
                 self.t["col_scale"] = dev_tensor(pk.col_scale, f64, dv)
+            if pk.t_row_idx is not None:
+                self.t["t_col_ptr"] = dev_tensor(pk.t_col_ptr, i32, dv)
+                self.t["t_row_idx"] = dev_tensor(pk.t_row_idx.view(np.int16), torch.int16, dv)
+                self.t["t_val"] = dev_tensor(pk.t_val, f32, dv)
+            else:
+                self._arena_transpose()
         self.stream.synchronize()
+
+    def _arena_transpose(self) -> None:
+        """K^T of every stored matrix on the device (``pdlp_arena_transpose``) from the
+        uploaded K: per matrix bitwise the host packer's transpose."""
+        pk, dv, t = self.pk, self.device, self.t
+        a = {k: dev_tensor(v, torch.int32, dv) for k, v in pk.arena.items()}
+        mats = len(pk.arena["row_off"]) - 1
+        rows, cols, nnz = int(pk.arena["row_off"][-1]), int(pk.arena["col_off"][-1]), int(pk.arena["nz_off"][-1])
+        need = C.c_size_t(0)
+        L.check(self.lib.pdlp_arena_transpose_work_bytes(mats, rows, cols, nnz, C.byref(need)),
+                "pdlp_arena_transpose_work_bytes")
+        work = torch.empty(max(int(need.value), 1), dtype=torch.uint8, device=dv)
+        t["t_col_ptr"] = torch.empty(int(pk.arena["cp_off"][-1]), dtype=torch.int32, device=dv)
+        t["t_row_idx"] = torch.empty(nnz, dtype=torch.int16, device=dv)
+        t["t_val"] = torch.empty(nnz, dtype=torch.float32, device=dv)
+        L.check(self.lib.pdlp_arena_transpose(
+            L.PDLP_F32, mats, rows, cols, nnz, ptr(a["row_off"]), ptr(a["col_off"]), ptr(a["nz_off"]),
+            ptr(a["rp_off"]), ptr(a["cp_off"]), ptr(t["row_ptr"]), ptr(t["col_idx"]), ptr(t["val"]),
+            ptr(t["t_col_ptr"]), ptr(t["t_row_idx"]), ptr(t["t_val"]), ptr(work), int(need.value),
+            stream_handle(self.stream)), "pdlp_arena_transpose")
+        self.stream.synchronize()                # work and the offsets are freed on return
 
     def configure(self, check_every: Optional[int] = None, block: Optional[int] = None,
                   lines: Optional[int] = None, split: Optional[int] = None,

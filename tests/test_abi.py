@@ -210,6 +210,18 @@ def test_sell_columns_layout_and_padding_limit():
     assert build_columns(sp.csr_matrix((3, 5))) is None
 
 
+def test_arena_transpose_rejects_bad_arguments_without_device():
+    lib = L.lib()
+    need = C.c_size_t(0)
+    assert lib.pdlp_arena_transpose_work_bytes(0, 1, 1, 1, C.byref(need)) == L.PDLP_E_SHAPE
+    assert lib.pdlp_arena_transpose_work_bytes(2, 10, 12, 30, None) == L.PDLP_E_ARG
+    assert lib.pdlp_arena_transpose_work_bytes(2, 10, 12, 30, C.byref(need)) == 0 and need.value > 30 * 8
+    args = [L.PDLP_F32, 2, 10, 12, 30] + [None] * 11 + [None, 0, None]
+    assert lib.pdlp_arena_transpose(*args) == L.PDLP_E_ARG                  # NULL arrays
+    args[0] = 7
+    assert lib.pdlp_arena_transpose(*args) == L.PDLP_E_ARG                  # precision
+
+
 def test_csr_to_sell64_rejects_bad_arguments_without_device():
     lib = L.lib()
     assert lib.pdlp_sell64_widths(-1, None, None, None) == L.PDLP_E_SHAPE

@@ -21,7 +21,10 @@ def test_descriptor_layout_matches_header():
 
 def test_packing_offsets_and_contents():
     probs = [transportation_lp(3, 4, seed=0), random_sparse_lp(5, 9, 0.3, seed=1), random_sparse_lp(7, 6, 0.4, seed=2)]
-    pk = PackedBatch(probs)
+    pk = PackedBatch(probs, transpose="host")
+    assert PackedBatch(probs).t_row_idx is None            # the default leaves K^T to the device
+    for k in ("row_off", "col_off", "nz_off"):
+        assert pk.arena[k][-1] == {"row_off": pk.M, "col_off": pk.N, "nz_off": pk.Z}[k]
     assert pk.M == sum(p.n_constraints for p in probs) and pk.N == sum(p.n_vars for p in probs)
     for k, p in enumerate(probs):
         d = pk.desc[k]
@@ -113,7 +116,7 @@ def test_scaled_batch_is_the_same_lps():
     from sovereign_solver_amd.engine import scaling as S
     probs = [random_sparse_lp(30 + 7 * k, 80 + 11 * k, 0.08, seed=k, family=2) for k in range(5)]
     plain = PackedBatch(probs)
-    sc = PackedBatch(probs, scaling="ruiz_pc")
+    sc = PackedBatch(probs, scaling="ruiz_pc", transpose="host")
     np.testing.assert_array_equal(plain.desc["norm_b"], sc.desc["norm_b"])
     np.testing.assert_array_equal(plain.desc["norm_c"], sc.desc["norm_c"])
     for k, p in enumerate(probs):

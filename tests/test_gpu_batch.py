@@ -173,3 +173,25 @@ def test_scaled_batch(gpu, method):
         assert abs(err - r.rel_kkt) <= 1e-3 * err + 1e-9     # the device measured the original LP too
         assert_objective_parity(p, r.objective, ref=_highs(p), x=r.x, y=r.y)
     assert sum(r.iterations for r in scaled[:3]) < 0.3 * sum(r.iterations for r in plain[:3])
+
+
+def test_device_arena_transpose_bitwise_host(gpu):
+    """K^T of every stored matrix built on the device (pdlp_arena_transpose) equals the
+    host packer's one-transpose-of-the-union, bit for bit: column pointers, 16-bit
+    row indices, FP32 values — with matrices shared between LPs and an LP whose
+    matrix has an empty row and an empty column."""
+    import scipy.sparse as sp
+    from sovereign_solver_amd.engine.batch import BatchEngine, PackedBatch
+    from sovereign_solver_amd.problem import Problem
+    from sovereign_solver_amd.workloads import uneven_batch
+    probs = uneven_batch(48, seed=7)
+    probs += [probs[3]] * 3                                       # one matrix stored once for 4 LPs
+    A = sp.csr_matrix(np.array([[1.0, 0.0, 2.0], [0.0, 0.0, 0.0], [3.0, 0.0, -1.0]]))
+    probs.append(Problem(c=np.ones(3), A_ub=A, b_ub=np.ones(3), A_eq=sp.csr_matrix((0, 3)), b_eq=np.zeros(0),
+                         lb=np.zeros(3), ub=np.full(3, np.inf), integer_mask=np.zeros(3, bool)))
+    host = PackedBatch(probs, transpose="host")
+    eng = BatchEngine(PackedBatch(probs))
+    t = eng.t
+    assert np.array_equal(t["t_col_ptr"].cpu().numpy(), host.t_col_ptr)
+    assert np.array_equal(t["t_row_idx"].cpu().numpy().view(np.uint16), host.t_row_idx)
+    assert np.array_equal(t["t_val"].cpu().numpy(), host.t_val.astype(np.float32))
