@@ -211,14 +211,15 @@ def cpu_baseline(kind: str, ids, tol: float, cap_s: float, what: str):
 
 
 # --------------------------------------------------------------------- timing
-def primal_bytes(form, elem: int, sell_entries: int = 0) -> float:
+def primal_bytes(form, elem: int, sell_entries: int = 0, values: bool = True) -> float:
     """Algorithmic bytes of one fused primal-step launch.  K^T as CSR (values, row
     indices, column pointers) or, with ``sell_entries`` > 0, as the SELL-64 copy the
     step then reads (values and row indices of every slot, padding included, and a
-    pointer per slice of 64 columns, no column pointers)."""
+    pointer per slice of 64 columns, no column pointers; no values when ``values`` is
+    false: every entry holds one constant, PdlpDev.k_const)."""
     n, m, nnz = form.n, form.m, form.nnz
     if sell_entries:
-        b = sell_entries * (elem + 4) + (-(-n // 64) + 1) * 4
+        b = sell_entries * ((elem if values else 0) + 4) + (-(-n // 64) + 1) * 4
     else:
         b = nnz * (elem + 4) + (n + 1) * 4    # K^T CSR: values, column indices, row pointers
     b += m * elem                             # y, gathered
@@ -236,9 +237,10 @@ def pkern(eng) -> str:
     return "primal_sell_kernel" if eng.tune.sell_primal else "primal_kernel"
 
 
-def dual_bytes(form, elem: int) -> float:
+def dual_bytes(form, elem: int, values: bool = True) -> float:
+    """Algorithmic bytes of one fused dual-step launch (K's values only when ``values``)."""
     n, m, nnz = form.n, form.m, form.nnz
-    b = nnz * (elem + 4) + (m + 1) * 4 + n * elem   # K CSR + xbar gathered once
+    b = nnz * ((elem if values else 0) + 4) + (m + 1) * 4 + n * elem   # K CSR + xbar gathered once
     b += m * elem + 2 * m * elem + 2 * m * 4        # b, y r/w, FP32 partial sum r/w
     if form.row_kind is not None:
         b += m
@@ -298,6 +300,8 @@ def spmv_rates(eng, reps: int = 50, kx_tpr: int = 64, traffic_key=None):
         nnz = int(val.numel())                   # SELL: slots, padding included
         ptrs = (int(rp.numel()) if tpr == 0 else rows + 1)
         nbytes = 8 * nnz + 4 * ptrs + 4 * cols + 4 * rows
+        if tpr == -1 and eng.k_const:            # every entry one constant: no values read
+            nbytes -= 4 * nnz
         gbs = nbytes / (us * 1e-6) / 1e9
         out[name] = {"rows": rows, "nnz": nnz, "lanes_per_row": tpr if tpr >= 0 else "sell_wg", "bytes": nbytes,
                      "us": us,
@@ -668,8 +672,9 @@ def run_transport(args, sources=1000, sinks=2000, cpu=True):
     obj_ref = highs_objective(p) if not args.no_highs else None
     us_p = eng.time_kernel("primal", eng.tune, reps=50)
     us_d = eng.time_kernel("dual", eng.tune, reps=50)
-    bp = primal_bytes(form, 4, int(eng.t_sell_ptr[-1].item()) if eng.tune.sell_primal else 0)
-    bd = dual_bytes(form, 4)
+    bp = primal_bytes(form, 4, int(eng.t_sell_ptr[-1].item()) if eng.tune.sell_primal else 0,
+                      values=not (eng.k_const and eng.tune.sell_primal))
+    bd = dual_bytes(form, 4, values=not (eng.k_const and 2 <= eng.tune.sell <= 7))
     gp, gd = bp / (us_p * 1e-6) / 1e9, bd / (us_d * 1e-6) / 1e9
     spmv = spmv_rates(eng, traffic_key=f"transport_{sources}x{sinks}")
     log(f"primal step {us_p:.2f} us ({gp:.0f} GB/s), dual step {us_d:.2f} us ({gd:.0f} GB/s); SpMV "
@@ -968,8 +973,9 @@ def run_beyond_mall(args, sources=4000, sinks=8000):
     log(f"transport {sources}x{sinks} (n={form.n}, nnz={form.nnz}) built and tuned in {time.perf_counter() - t0:.1f}s")
     us_p = eng.time_kernel("primal", eng.tune, reps=10)
     us_d = eng.time_kernel("dual", eng.tune, reps=10)
-    bp = primal_bytes(form, 4, int(eng.t_sell_ptr[-1].item()) if eng.tune.sell_primal else 0)
-    bd = dual_bytes(form, 4)
+    bp = primal_bytes(form, 4, int(eng.t_sell_ptr[-1].item()) if eng.tune.sell_primal else 0,
+                      values=not (eng.k_const and eng.tune.sell_primal))
+    bd = dual_bytes(form, 4, values=not (eng.k_const and 2 <= eng.tune.sell <= 7))
     gp, gd = bp / (us_p * 1e-6) / 1e9, bd / (us_d * 1e-6) / 1e9
     key = f"transport_{sources}x{sinks}"
     spmv = spmv_rates(eng, reps=10, kx_tpr=64, traffic_key=key)

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PDLP_ABI_VERSION 12
+#define PDLP_ABI_VERSION 13
 
 /* precision of iterates, matrix values and problem vectors */
 #define PDLP_F32 0
@@ -156,7 +156,10 @@ typedef struct PdlpDev {
     const void *sell_val;     /* same length, prec (padding 0)                      */
     const int32_t *csr_rows;  /* the rows outside the slices                        */
     int32_t sell_max_width;   /* largest slice width (entries per lane)              */
-    int32_t sell_reserved;
+    int32_t k_const;          /* 0, or the value (an integer) every entry of K holds, when the
+                                 SELL-64 copies (rows and K^T) hold no padding inside a row:
+                                 the SELL primal and dual steps then read no values (the
+                                 arrays still hold them; a transportation LP: 1) */
     void *sell_part;          /* prec, 64 * sell_slices * ceil(sell_max_width / 8): partial
                                  dot products of the split SELL step (PdlpTune.sell >= 8) */
     const void *sell_asum;    /* prec, 64 * sell_slices: sum_j |K_ij| of each lane's row */

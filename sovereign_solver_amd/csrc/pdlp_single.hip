@@ -124,6 +124,7 @@ __global__ void __launch_bounds__(1024) primal_sell_kernel(PdlpDev d, int remap)
     const int width = __builtin_amdgcn_readfirstlane((d.t_sell_ptr[s + 1] - beg) >> 6);
     const int32_t* __restrict__ idx = d.t_sell_idx + beg + (j & 63);
     const T* __restrict__ val = (const T*)d.t_sell_val + beg + (j & 63);
+    const T kc = (T)d.k_const;                      // nonzero: every entry holds kc (no values read)
     T aty = T(0), asum = T(0);
     int k = 0;
     for (; k + kRowBatch <= width; k += kRowBatch) {
@@ -131,7 +132,7 @@ __global__ void __launch_bounds__(1024) primal_sell_kernel(PdlpDev d, int remap)
         int r[kRowBatch];
 #pragma unroll
         for (int u = 0; u < kRowBatch; ++u) {
-            a[u] = val[64 * (k + u)];
+            a[u] = kc != T(0) ? kc : val[64 * (k + u)];
             r[u] = idx[64 * (k + u)];
         }
 #pragma unroll
@@ -141,7 +142,7 @@ __global__ void __launch_bounds__(1024) primal_sell_kernel(PdlpDev d, int remap)
         }
     }
     for (; k < width; ++k) {
-        const T a = val[64 * k];
+        const T a = kc != T(0) ? kc : val[64 * k];
         aty += a * y[idx[64 * k]];
         asum += fabs(a);
     }
@@ -314,13 +315,22 @@ __global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d, const T* __
         const RowOwn<T> o = row_own<T>(d, ir, !SPMV && ir >= 0);
         const int kc = (width + NW - 1) / NW;
         const int k0 = min(width, w * kc), k1 = min(width, k0 + kc);
+        const T kv = (T)d.k_const;                  // nonzero: every entry holds kv (no values read)
         T ax = T(0), asum = T(0);
         int q = beg + (k0 + ph) * 64 + rb + r;
+        if (kv != T(0)) {
 #pragma unroll 8
-        for (int k = k0 + ph; k < k1; k += P, q += 64 * P) {
-            const T a = val[q];
-            ax += a * xbar[col[q]];
-            asum += fabs(a);
+            for (int k = k0 + ph; k < k1; k += P, q += 64 * P) {
+                ax += kv * xbar[col[q]];
+                asum += fabs(kv);
+            }
+        } else {
+#pragma unroll 8
+            for (int k = k0 + ph; k < k1; k += P, q += 64 * P) {
+                const T a = val[q];
+                ax += a * xbar[col[q]];
+                asum += fabs(a);
+            }
         }
         part[0][w][lane] = ax;
         part[1][w][lane] = asum;
@@ -351,13 +361,15 @@ __global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d, const T* __
         const int beg = d.row_ptr[i], end = d.row_ptr[i + 1];
         const int32_t* __restrict__ ci = d.col_idx;
         const T* __restrict__ val = (const T*)d.val;
+        const T kv = (T)d.k_const;
+        const bool uni = kv != T(0);
         for (int k0 = beg + tc; k0 < end; k0 += TC * kRowBatch) {
             T a[kRowBatch];
             int c[kRowBatch];
 #pragma unroll
             for (int u = 0; u < kRowBatch; ++u) {
                 const int k = k0 + u * TC;
-                a[u] = k < end ? val[k] : T(0);
+                a[u] = k < end ? (uni ? kv : val[k]) : T(0);
                 c[u] = k < end ? ci[k] : 0;
             }
 #pragma unroll

@@ -13,7 +13,7 @@ import threading
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "_lib", "libpdlp_hip.so")
 
-PDLP_ABI_VERSION = 12
+PDLP_ABI_VERSION = 13
 PDLP_F32, PDLP_F64 = 0, 1
 PDLP_E_ARG, PDLP_E_TUNE, PDLP_E_SHAPE, PDLP_E_LIMIT = -1, -2, -3, -4
 PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL, PDLP_INFEASIBLE, PDLP_UNBOUNDED, PDLP_SKIPPED = 0, 1, 3, 4, 5, 6
@@ -55,7 +55,7 @@ class PdlpDev(C.Structure):
         ("x_psum", _p), ("y_psum", _p), ("partials", _p), ("ctrl", _p), ("x_last", _p), ("y_last", _p),
         ("sell_slices", C.c_int32), ("n_csr_rows", C.c_int32), ("sell_ptr", _p), ("sell_rows", _p),
         ("sell_col", _p), ("sell_val", _p), ("csr_rows", _p), ("sell_max_width", C.c_int32),
-        ("sell_reserved", C.c_int32), ("sell_part", _p), ("sell_asum", _p),
+        ("k_const", C.c_int32), ("sell_part", _p), ("sell_asum", _p),
         ("row_scale", _p), ("col_scale", _p),
         ("t_sell_slices", C.c_int32), ("t_sell_reserved", C.c_int32), ("t_sell_ptr", _p), ("t_sell_idx", _p),
         ("t_sell_val", _p),

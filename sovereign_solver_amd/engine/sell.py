@@ -28,6 +28,7 @@ class SellSlices:
     col: np.ndarray        # int32, entries (padding 0)
     val: np.ndarray        # float64, entries (padding 0)
     csr_rows: np.ndarray   # int32: rows outside the slices
+    padded: bool = False   # some slice pads a row (rows of unequal length)
 
     @property
     def slices(self) -> int:
@@ -68,6 +69,7 @@ def build(K: sp.csr_matrix, line_elems: int = 32, max_lines: float = 4.0, max_pa
     ptr = [0]
     rows_out, col_out, val_out = [], [], []
     in_slice = np.zeros(m, bool)
+    padded = False
     for r in _windows(rl):
         lens = rl[r]
         if lens.min() == 0:
@@ -92,6 +94,7 @@ def build(K: sp.csr_matrix, line_elems: int = 32, max_lines: float = 4.0, max_pa
         if distinct.mean() > max_lines:
             continue
         in_slice[r] = True
+        padded |= bool(lens.min() != W)
         rows_out.append(np.concatenate([r, np.full(SLICE - r.size, -1)]).astype(np.int32))  # -1: no row
         c = np.zeros((W, SLICE), np.int32)                                  # column-major: [W, 64]
         v = np.zeros((W, SLICE))
@@ -104,7 +107,7 @@ def build(K: sp.csr_matrix, line_elems: int = 32, max_lines: float = 4.0, max_pa
         return SellSlices(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0),
                           np.arange(m, dtype=np.int32))
     return SellSlices(np.asarray(ptr, np.int32), np.concatenate(rows_out), np.concatenate(col_out),
-                      np.concatenate(val_out), np.flatnonzero(~in_slice).astype(np.int32))
+                      np.concatenate(val_out), np.flatnonzero(~in_slice).astype(np.int32), padded)
 
 
 # Largest padding (entries of the SELL copy beyond nnz, relative to nnz) for which

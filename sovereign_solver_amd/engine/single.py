@@ -120,6 +120,18 @@ class SingleEngine:
                                                     ptr(self.t_row_ptr), ptr(self.t_col_idx), ptr(self.t_val),
                                                     ptr(self.t_sell_ptr), ptr(self.t_sell_idx), ptr(self.t_sell_val),
                                                     stream_handle(self.stream)), "pdlp_csr_to_sell64")
+        # every entry of K one integer and no padding inside a row of the SELL copies: the SELL
+        # steps read no values (k_const)
+        self.k_const = 0
+        if not exact and K.nnz and (self.sell_slices or self.t_sell_slices):
+            v = float(K.data[0])
+            ok = v != 0.0 and v == round(v) and abs(v) < 2**24 and bool(np.all(K.data == v))
+            if ok and self.sell_slices:
+                ok = not sl.padded
+            if ok and self.t_sell_slices:
+                w_last = int(tp[-1] - tp[-2]) // sell.SLICE
+                ok = int(tp[-1]) - (sell.SLICE * self.t_sell_slices - self.n) * w_last == self.nnz
+            self.k_const = int(v) if ok else 0
         self.ctrl_host = torch.zeros((2, C.sizeof(L.PdlpCtrl)), dtype=torch.uint8).pin_memory()
         self.stream.synchronize()
         self.dev = L.PdlpDev(
@@ -136,7 +148,7 @@ class SingleEngine:
             csr_rows=ptr(self.csr_rows), sell_max_width=self.sell_max_width, sell_part=ptr(self.sell_part),
             sell_asum=ptr(self.sell_asum), row_scale=ptr(self.row_scale), col_scale=ptr(self.col_scale),
             t_sell_slices=self.t_sell_slices, t_sell_ptr=ptr(self.t_sell_ptr), t_sell_idx=ptr(self.t_sell_idx),
-            t_sell_val=ptr(self.t_sell_val))
+            t_sell_val=ptr(self.t_sell_val), k_const=self.k_const)
         self._graphs = {}
         self._graph_tune = None
 

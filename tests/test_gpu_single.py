@@ -412,3 +412,23 @@ def test_sell_columns_from_device_widths(gpu, shape):
         assert (a is None) == (b is None)
         if a is not None:
             assert np.array_equal(a, b)
+
+
+def test_uniform_values_read_none_same_bits(gpu):
+    """A matrix whose entries are all one integer (a transportation LP: 1) with unpadded
+    SELL copies runs the SELL steps without reading values (PdlpDev.k_const): the same
+    bits as reading them; a padded or non-uniform matrix keeps k_const = 0."""
+    from sovereign_solver_amd.engine.forms import box_form
+    from sovereign_solver_amd.engine.single import SingleEngine
+    from sovereign_solver_amd.engine.tune import Tune
+    p = transportation_lp(60, 200, seed=2)
+    t = Tune(tpr_primal=1, tpr_dual=256, block=256, check_every=50, sell=6, sell_primal=1)
+    eng = SingleEngine(box_form(p), "fp32", tune=t)
+    assert eng.k_const == 1 and eng.sell_slices > 0 and eng.t_sell_slices > 0
+    a = eng.solve(1e-4, 50_000)
+    eng.dev.k_const = 0
+    eng.drop_graphs()
+    b = eng.solve(1e-4, 50_000)
+    assert a["status"] == b["status"] == "optimal" and a["iterations"] == b["iterations"]
+    assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["y"], b["y"])
+    assert SingleEngine(box_form(random_sparse_lp(300, 700, 0.02, seed=4)), "fp32").k_const == 0
