@@ -234,7 +234,7 @@ def primal_bytes(form, elem: int, sell_entries: int = 0, values: bool = True) ->
 
 def pkern(eng) -> str:
     """Name of the primal-step kernel the engine's tuning launches."""
-    return "primal_sell_kernel" if eng.tune.sell_primal else "primal_kernel"
+    return {0: "primal_kernel", 1: "primal_sell_kernel", 2: "primal_sell2_kernel"}[eng.tune.sell_primal]
 
 
 def dual_bytes(form, elem: int, values: bool = True) -> float:
@@ -691,8 +691,8 @@ def run_transport(args, sources=1000, sinks=2000, cpu=True):
         "objective_rel_err_vs_fp64": None if obj_ref is None else abs(obj - obj_ref) / (1 + abs(obj_ref)),
         "host_to_host": h2h, "tune": tune.__dict__, "cadence": cadence,
         "roofline": {"bound": "hbm",
-                     "kernel": ("primal_sell_kernel (SELL-64 copy of K^T" if eng.tune.sell_primal else
-                                "primal_kernel (CSR of K^T") + "; fused K^T y + step + projection + sums)",
+                     "kernel": pkern(eng) + (" (SELL-64 copy of K^T" if eng.tune.sell_primal else " (CSR of K^T")
+                               + "; fused K^T y + step + projection + sums)",
                      "achieved": gp, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gp / HBM_PEAK_GBS,
                      "traffic": measured_traffic(f"{key}/{pkern(eng)}"), "bytes_per_launch": bp,
                      "us_per_launch": us_p,

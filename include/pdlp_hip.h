@@ -208,8 +208,10 @@ typedef struct PdlpTune {
                              entries of a slice, partial sums added in order */
     int32_t no_ray;       /* 1: skip the unboundedness (ray of decrease) test of the
                              adaptive checks; 0: test at every check */
-    int32_t sell_primal;  /* 1: primal step over the SELL-64 copy of K^T (PdlpDev.t_sell_*),
-                             one lane per column, `block` threads per workgroup; 0: CSR */
+    int32_t sell_primal;  /* primal step over the SELL-64 copy of K^T (PdlpDev.t_sell_*), one
+                             lane per column, `block` threads per workgroup: 1 one slice of
+                             64 columns per wavefront, 2 two (a lane's two columns issued
+                             together; same sums); 0: CSR */
 } PdlpTune;
 
 int         pdlp_abi_version(void);

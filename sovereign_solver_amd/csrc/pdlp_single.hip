@@ -160,6 +160,92 @@ __global__ void __launch_bounds__(1024) primal_sell_kernel(PdlpDev d, int remap)
     else d.x_sum[j] = psd + (double)xn;
 }
 
+// The same step with two slices per wavefront (columns 128 w .. 128 w + 127 for
+// wavefront w, lane l owning columns 128 w + l and 128 w + 64 + l): both columns'
+// own data, row indices and gathers are issued together, so a lane keeps twice
+// the loads in flight; each column is summed in the same order as above (same
+// bits).
+template <typename T>
+__global__ void __launch_bounds__(1024) primal_sell2_kernel(PdlpDev d, int remap) {
+    const PdlpCtrl* ctrl = d.ctrl;
+    if (ctrl->status != PDLP_RUNNING) return;
+    const T tau_scale = (T)ctrl->tau_scale;
+    const int wv = block_index(remap) * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+    const int s0 = __builtin_amdgcn_readfirstlane(2 * wv);
+    if (s0 >= d.t_sell_slices) return;              // whole wavefronts leave together
+    const int ns = s0 + 1 < d.t_sell_slices ? 2 : 1;
+    const int lane = threadIdx.x & 63;
+    const T* __restrict__ y = (const T*)d.y;
+    const T* c = (const T*)d.c;
+    const T* lo = (const T*)d.lo;
+    const T* hi = (const T*)d.hi;
+    const T kc = (T)d.k_const;                      // nonzero: every entry holds kc (no values read)
+    int j[2], beg[2], width[2];
+    bool live[2];
+    T xj[2], cj[2], lj[2], hj[2];
+    float psf[2];
+    double psd[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        j[u] = (s0 + u) * 64 + lane;
+        live[u] = u < ns && j[u] < d.n;
+        const int jj = j[u] < d.n ? j[u] : d.n - 1;
+        xj[u] = ((const T*)d.x)[jj];
+        cj[u] = c[jj];
+        lj[u] = lo != nullptr ? lo[jj] : T(0);
+        hj[u] = hi != nullptr ? hi[jj] : T(0);
+        psf[u] = d.x_psum != nullptr ? d.x_psum[jj] : 0.f;
+        psd[u] = d.x_psum == nullptr ? d.x_sum[jj] : 0.0;
+        const int sl = u < ns ? s0 + u : s0;
+        beg[u] = __builtin_amdgcn_readfirstlane(d.t_sell_ptr[sl]);
+        width[u] = u < ns ? __builtin_amdgcn_readfirstlane((d.t_sell_ptr[sl + 1] - beg[u]) >> 6) : 0;
+    }
+    const int32_t* __restrict__ idx0 = d.t_sell_idx + beg[0] + lane;
+    const int32_t* __restrict__ idx1 = d.t_sell_idx + beg[1] + lane;
+    const T* __restrict__ val0 = (const T*)d.t_sell_val + beg[0] + lane;
+    const T* __restrict__ val1 = (const T*)d.t_sell_val + beg[1] + lane;
+    const int wmax = width[0] > width[1] ? width[0] : width[1];
+    T aty[2] = {T(0), T(0)}, asum[2] = {T(0), T(0)};
+    for (int k = 0; k < wmax; k += kRowBatch) {
+        T a[2][kRowBatch];
+        int r[2][kRowBatch];
+#pragma unroll
+        for (int u = 0; u < kRowBatch; ++u) {
+            const bool in0 = k + u < width[0], in1 = k + u < width[1];
+            a[0][u] = in0 ? (kc != T(0) ? kc : val0[64 * (k + u)]) : T(0);
+            r[0][u] = in0 ? idx0[64 * (k + u)] : 0;
+            a[1][u] = in1 ? (kc != T(0) ? kc : val1[64 * (k + u)]) : T(0);
+            r[1][u] = in1 ? idx1[64 * (k + u)] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kRowBatch; ++u) {
+            if (k + u < width[0]) {
+                aty[0] += a[0][u] * y[r[0][u]];
+                asum[0] += fabs(a[0][u]);
+            }
+            if (k + u < width[1]) {
+                aty[1] += a[1][u] * y[r[1][u]];
+                asum[1] += fabs(a[1][u]);
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        if (!live[u]) continue;
+        const T colsum = asum[u] > T(1e-4) ? asum[u] : T(1e-4);
+        T tau = T(0.999) / colsum;
+        if (tau_scale != T(1)) tau *= tau_scale;
+        T xn = xj[u] - tau * (cj[u] - aty[u]);
+        if (lo != nullptr) xn = xn < lj[u] ? lj[u] : xn;
+        else xn = xn < T(0) ? T(0) : xn;
+        if (hi != nullptr) xn = xn > hj[u] ? hj[u] : xn;
+        ((T*)d.xbar)[j[u]] = T(2) * xn - xj[u];
+        ((T*)d.x)[j[u]] = xn;
+        if (d.x_psum != nullptr) d.x_psum[j[u]] = psf[u] + (float)xn;
+        else d.x_sum[j[u]] = psd[u] + (double)xn;
+    }
+}
+
 // ------------------------------------------------------------------ dual step
 // Dual update of row i from ax = K_i xbar and asum = sum_j |K_ij|.
 template <typename T, bool EXACT>
@@ -891,7 +977,7 @@ int check_tune(const PdlpTune* t) {
         return fail(PDLP_E_TUNE, "sell must be 0..7 or a multiple of 8");
     if (t->exact && t->scheme != PDLP_SCHEME_REFERENCE)
         return fail(PDLP_E_TUNE, "exact mode reproduces the reference scheme only");
-    if (t->sell_primal != 0 && t->sell_primal != 1) return fail(PDLP_E_TUNE, "sell_primal must be 0 or 1");
+    if (t->sell_primal < 0 || t->sell_primal > 2) return fail(PDLP_E_TUNE, "sell_primal must be 0, 1 or 2");
     if (t->sell_primal && t->exact) return fail(PDLP_E_TUNE, "exact mode keeps the CSR primal step");
     return 0;
 }
@@ -927,6 +1013,11 @@ int launch_primal_t(const PdlpDev& d, const PdlpTune& t, hipStream_t s) {
     const int remap = t.xcd_remap & 1;
     unsigned g, nt;
     if constexpr (!EXACT) {
+        if (t.sell_primal == 2) {                     // two slices per wavefront
+            shape_for((d.t_sell_slices + 1) / 2 * 64, 1, t.block, remap, g, nt);   // a wavefront per slice pair
+            primal_sell2_kernel<T><<<g, nt, 0, s>>>(d, remap);
+            return hip_fail(hipGetLastError(), "primal_sell2_kernel launch");
+        }
         if (t.sell_primal) {                          // one lane per column over the SELL-64 slices
             shape_for(d.n, 1, t.block, remap, g, nt);
             primal_sell_kernel<T><<<g, nt, 0, s>>>(d, remap);

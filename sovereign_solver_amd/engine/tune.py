@@ -60,7 +60,7 @@ class Tune:
                                 # 5 1024/8, 6 1024/32, 7 512/16) and tpr_dual (>= 64) threads per CSR row,
                                 # k >= 8 a wavefront per k entries
     no_ray: int = 0             # 1: the adaptive checks skip the unboundedness test
-    sell_primal: int = 0        # 1: primal step over the SELL-64 copy of K^T (one lane per column)
+    sell_primal: int = 0        # primal step over the SELL-64 copy of K^T: 1 one slice, 2 two slices per wavefront
 
     def to_c(self):
         from ._lib import PdlpTune
@@ -170,13 +170,14 @@ def autotune(engine, reps: int = 20, use_cache: bool = True) -> Tune:
     # the SELL-64 copy of K^T (one lane per column, no column pointers), when the engine has one
     sell_primal, block_p = 0, 256
     if getattr(engine, "t_sell_slices", 0) > 0:
-        for bp in (256, 512, 1024):
-            for remap in (0, 1):
-                t = Tune(1, base.tpr_dual, bp, base.check_every, remap, 0, sell_primal=1)
-                us = engine.time_kernel("primal", t, reps)
-                timings[f"primal/sell/block{bp}/remap{remap}"] = us
-                if us < best_pt:
-                    best_p, best_pt, sell_primal, block_p = (1, remap), us, 1, bp
+        for sp in (1, 2):                 # one or two slices per wavefront
+            for bp in (256, 512, 1024):
+                for remap in (0, 1):
+                    t = Tune(1, base.tpr_dual, bp, base.check_every, remap, 0, sell_primal=sp)
+                    us = engine.time_kernel("primal", t, reps)
+                    timings[f"primal/sell{sp}/block{bp}/remap{remap}"] = us
+                    if us < best_pt:
+                        best_p, best_pt, sell_primal, block_p = (1, remap), us, sp, bp
     best_d, best_dt = None, float("inf")
     for td in _neighbours(base.tpr_dual):
         # several rows per workgroup share gathered cache lines in L1 when rows are short enough

@@ -294,8 +294,9 @@ def test_split_sell_dual_step_agrees(gpu, kc):
 
 @pytest.mark.parametrize("lp", ["transport", "random"])
 def test_sell_primal_step_same_bits_as_csr(gpu, lp, monkeypatch):
-    """The primal step over the SELL-64 copy of K^T (one lane per column) sums each
-    column in CSR order, so whole solves give the bits of the one-lane CSR kernel:
+    """The primal step over the SELL-64 copy of K^T (one lane per column, one or two
+    slices per wavefront) sums each column in CSR order, so whole solves give the
+    bits of the one-lane CSR kernel:
     a transportation LP (every column 2 entries, no padding) and a random LP whose
     unequal columns pad the slices (the padding limit lifted to build it)."""
     from sovereign_solver_amd.engine import sell as SL
@@ -307,15 +308,17 @@ def test_sell_primal_step_same_bits_as_csr(gpu, lp, monkeypatch):
     f = box_form(p)
     base = Tune(tpr_primal=1, tpr_dual=8, block=256, check_every=50, scheme=1)
     runs = []
-    for sp in (0, 1):
+    for sp in (0, 1, 2):                     # CSR, one slice per wavefront, two slices per wavefront
         eng = SingleEngine(f, "fp32", tune=base.with_(sell_primal=sp))
         assert eng.t_sell_slices == -(-f.n // 64)
         r = eng.solve(1e-4, 50_000)
         runs.append(r)
         eng.drop_graphs()
-    a, b = runs
-    assert a["status"] == b["status"] == "optimal" and a["iterations"] == b["iterations"]
+    a, b, c = runs
+    assert a["status"] == b["status"] == c["status"] == "optimal"
+    assert a["iterations"] == b["iterations"] == c["iterations"]
     assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["y"], b["y"])
+    assert np.array_equal(a["x"], c["x"]) and np.array_equal(a["y"], c["y"])
     e = yardstick.rel_kkt(p, *extract_xy(f, p, b))[0]
     assert e <= 1.02e-4
 
