@@ -523,6 +523,19 @@ def run_headline(args):
             "iters_max": int(rec_c["iterations"].max())}
     log(f"cold batched, resident: {cold['lps_per_s']:.0f} LPs/s (iterations mean {cold['iters_mean']:.0f}, "
         f"max {cold['iters_max']})")
+    # iterations at the reference's check cadence (50, config.py:38), where an LP can stop at any
+    # multiple of 50: the learned start's gain in iterations without the tuned cadence's rounding
+    ce_tuned = eng.check_every
+    eng.set_check_every(50)
+    at50 = {}
+    for name, fn in (("learned", learned_resident), ("cold", cold_resident)):
+        _, _, r50 = timed(fn, 1)
+        at50[name] = {"iters_mean": float(r50["iterations"].mean()), "iters_max": int(r50["iterations"].max())}
+    eng.set_check_every(ce_tuned)
+    cold["iterations_at_cadence_50"] = at50
+    log(f"iterations at cadence 50: learned mean {at50['learned']['iters_mean']:.0f} (max "
+        f"{at50['learned']['iters_max']}), cold mean {at50['cold']['iters_mean']:.0f} (max {at50['cold']['iters_max']}) "
+        f"-> {at50['cold']['iters_mean'] / at50['learned']['iters_mean']:.2f}x")
 
     # host to host: problems on the host -> staging and upload, network inputs (on the device,
     # from the staged vectors), inference, solve -> results and objectives; one batch at a time,

@@ -349,19 +349,29 @@ class NodeModel:
                               ready=ev)
 
     def calibrate_weight(self, fam, ids, tol: float, max_iters: int = 100_000,
-                         candidates=(0.5, 1.0, 2.0, 4.0, 8.0), log=None) -> float:
+                         candidates=(0.5, 1.0, 2.0, 4.0, 8.0), refine: bool = True, log=None) -> float:
         """Set the primal weight to the candidate (and the engine default, None) with the
         fewest mean iterations on instances ``ids`` (training instances held out of
-        the fit's rows or not: the weight is one scalar); returns it."""
+        the fit's rows or not: the weight is one scalar), then (``refine``) try the
+        best candidate times and divided by sqrt(2); returns it."""
         best, best_it = None, None
-        for w in (None,) + tuple(candidates):
+
+        def trial(w):
+            nonlocal best, best_it
             self.log_weight = None if w is None else float(np.log(w))
             res, _ = solve_nodes(fam, ids, self, tol, max_iters)
             it = float(np.mean([r.iterations for r in res]))
             if log:
-                log(f"weight {w}: mean iterations {it:.1f}")
+                log(f"weight {w if w is None else round(w, 3)}: mean iterations {it:.1f}")
             if best_it is None or it < best_it:
                 best, best_it = self.log_weight, it
+
+        for w in (None,) + tuple(candidates):
+            trial(w)
+        if refine and best is not None:
+            w0 = float(np.exp(best))
+            for w in (w0 / np.sqrt(2.0), w0 * np.sqrt(2.0)):
+                trial(w)
         self.log_weight = best
         return best
 
