@@ -237,10 +237,13 @@ def pkern(eng) -> str:
     return {0: "primal_kernel", 1: "primal_sell_kernel", 2: "primal_sell2_kernel"}[eng.tune.sell_primal]
 
 
-def dual_bytes(form, elem: int, values: bool = True) -> float:
-    """Algorithmic bytes of one fused dual-step launch (K's values only when ``values``)."""
+def dual_bytes(form, elem: int, values: bool = True, index_loads=None) -> float:
+    """Algorithmic bytes of one fused dual-step launch (K's values only when ``values``;
+    ``index_loads`` column indices, default one per entry: a row whose columns step by
+    a constant reads only its first)."""
     n, m, nnz = form.n, form.m, form.nnz
-    b = nnz * ((elem if values else 0) + 4) + (m + 1) * 4 + n * elem   # K CSR + xbar gathered once
+    il = nnz if index_loads is None else index_loads
+    b = nnz * (elem if values else 0) + il * 4 + (m + 1) * 4 + n * elem   # K CSR + xbar gathered once
     b += m * elem + 2 * m * elem + 2 * m * 4        # b, y r/w, FP32 partial sum r/w
     if form.row_kind is not None:
         b += m
@@ -300,8 +303,10 @@ def spmv_rates(eng, reps: int = 50, kx_tpr: int = 64, traffic_key=None):
         nnz = int(val.numel())                   # SELL: slots, padding included
         ptrs = (int(rp.numel()) if tpr == 0 else rows + 1)
         nbytes = 8 * nnz + 4 * ptrs + 4 * cols + 4 * rows
-        if tpr == -1 and eng.k_const:            # every entry one constant: no values read
-            nbytes -= 4 * nnz
+        if tpr == -1:                            # the dual step's layout: index runs, constant values
+            nbytes -= 4 * (nnz - eng.dual_index_loads)
+            if eng.k_const:
+                nbytes -= 4 * nnz
         gbs = nbytes / (us * 1e-6) / 1e9
         out[name] = {"rows": rows, "nnz": nnz, "lanes_per_row": tpr if tpr >= 0 else "sell_wg", "bytes": nbytes,
                      "us": us,
@@ -687,7 +692,8 @@ def run_transport(args, sources=1000, sinks=2000, cpu=True):
     us_d = eng.time_kernel("dual", eng.tune, reps=50)
     bp = primal_bytes(form, 4, int(eng.t_sell_ptr[-1].item()) if eng.tune.sell_primal else 0,
                       values=not (eng.k_const and eng.tune.sell_primal))
-    bd = dual_bytes(form, 4, values=not (eng.k_const and 2 <= eng.tune.sell <= 7))
+    one = 2 <= eng.tune.sell <= 7
+    bd = dual_bytes(form, 4, values=not (eng.k_const and one), index_loads=eng.dual_index_loads if one else None)
     gp, gd = bp / (us_p * 1e-6) / 1e9, bd / (us_d * 1e-6) / 1e9
     spmv = spmv_rates(eng, traffic_key=f"transport_{sources}x{sinks}")
     log(f"primal step {us_p:.2f} us ({gp:.0f} GB/s), dual step {us_d:.2f} us ({gd:.0f} GB/s); SpMV "
@@ -988,7 +994,8 @@ def run_beyond_mall(args, sources=4000, sinks=8000):
     us_d = eng.time_kernel("dual", eng.tune, reps=10)
     bp = primal_bytes(form, 4, int(eng.t_sell_ptr[-1].item()) if eng.tune.sell_primal else 0,
                       values=not (eng.k_const and eng.tune.sell_primal))
-    bd = dual_bytes(form, 4, values=not (eng.k_const and 2 <= eng.tune.sell <= 7))
+    one = 2 <= eng.tune.sell <= 7
+    bd = dual_bytes(form, 4, values=not (eng.k_const and one), index_loads=eng.dual_index_loads if one else None)
     gp, gd = bp / (us_p * 1e-6) / 1e9, bd / (us_d * 1e-6) / 1e9
     key = f"transport_{sources}x{sinks}"
     spmv = spmv_rates(eng, reps=10, kx_tpr=64, traffic_key=key)

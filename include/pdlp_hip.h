@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PDLP_ABI_VERSION 13
+#define PDLP_ABI_VERSION 14
 
 /* precision of iterates, matrix values and problem vectors */
 #define PDLP_F32 0
@@ -181,6 +181,13 @@ typedef struct PdlpDev {
     const int32_t *t_sell_ptr;  /* t_sell_slices + 1                                 */
     const int32_t *t_sell_idx;  /* t_sell_ptr[t_sell_slices]: rows of K (padding 0)   */
     const void *t_sell_val;     /* same length, prec (padding 0)                      */
+    /* optional index runs for the one-launch SELL dual step (PdlpTune.sell 2..7): a slice
+       whose rows all have its full width and whose every row's column indices step by one
+       constant, col_k = col_0 + k * stride, has that stride in sell_stride (else 0), and
+       a row of csr_rows whose indices step so has it in csr_stride (else 0); the step
+       then reads only each row's first index.  NULL: every index is read. */
+    const int32_t *sell_stride;   /* sell_slices                                     */
+    const int32_t *csr_stride;    /* n_csr_rows, in the order of csr_rows            */
 } PdlpDev;
 
 /* Launch shape, chosen by the autotuner (sovereign_solver_amd.engine.tune). */
@@ -216,6 +223,12 @@ typedef struct PdlpTune {
 
 int         pdlp_abi_version(void);
 const char* pdlp_last_error(void);
+
+/* stride[k] = the column step s of CSR row rows[k] when its columns are col_0 + j s
+ * for every entry j (1 for a single entry), else 0 (no entries, or steps that differ):
+ * the PdlpDev.csr_stride of the listed rows.  No reference counterpart. */
+int pdlp_row_strides(int32_t count, const int32_t* rows, const int32_t* row_ptr, const int32_t* col_idx,
+                     int32_t* stride, void* stream);
 
 /* width[s] = the longest row of rows 64 s .. 64 s + 63 of a CSR matrix (rows past
  * the end count 0), for s < ceil(rows / 64): the slice widths of its SELL-64 copy

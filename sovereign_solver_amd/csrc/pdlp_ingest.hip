@@ -189,6 +189,28 @@ int arena_layout(int32_t rows, int32_t cols, int32_t nnz, ArenaLayout& a) {
     return 0;
 }
 
+// Column step of each listed row: one wavefront per row compares every step between
+// neighbouring entries with the first; 1 for a single entry, 0 for none or no constant step.
+__global__ void __launch_bounds__(256) row_strides_kernel(int count, const int32_t* __restrict__ rows,
+                                                          const int32_t* __restrict__ row_ptr,
+                                                          const int32_t* __restrict__ col_idx,
+                                                          int32_t* __restrict__ stride) {
+    const int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= count) return;                         // whole wavefronts leave together
+    const int lane = threadIdx.x & 63;
+    const int r = rows[w];
+    const int beg = row_ptr[r], end = row_ptr[r + 1];
+    if (end - beg < 2) {
+        if (lane == 0) stride[w] = end - beg;       // 0 entries: 0, one entry: 1
+        return;
+    }
+    const int st = col_idx[beg + 1] - col_idx[beg];
+    bool same = true;
+    for (int e = beg + 1 + lane; e < end; e += 64) same = same && (col_idx[e] - col_idx[e - 1] == st);
+    const bool all = __all(same);
+    if (lane == 0) stride[w] = all ? st : 0;
+}
+
 // Width of each SELL-64 slice: the longest of its 64 rows (rows past the end count 0).
 __global__ void __launch_bounds__(256) sell64_widths_kernel(int rows, const int32_t* __restrict__ row_ptr,
                                                             int32_t* __restrict__ width) {
@@ -223,6 +245,16 @@ int pdlp_csr_to_sell64(int32_t prec, int32_t rows, const int32_t* row_ptr, const
         csr_to_sell64_kernel<double><<<g, 256, 0, s>>>(rows, row_ptr, col_idx, (const double*)val, sell_ptr,
                                                        sell_idx, (double*)sell_val);
     return hip_fail(hipGetLastError(), "csr_to_sell64_kernel launch");
+}
+
+int pdlp_row_strides(int32_t count, const int32_t* rows, const int32_t* row_ptr, const int32_t* col_idx,
+                     int32_t* stride, void* stream) {
+    if (count < 0) return fail(PDLP_E_SHAPE, "row_strides: count >= 0");
+    if (count == 0) return 0;
+    if (!rows || !row_ptr || !col_idx || !stride) return fail(PDLP_E_ARG, "row_strides: NULL pointer");
+    row_strides_kernel<<<(unsigned)((count + 3) / 4), 256, 0, as_stream(stream)>>>(count, rows, row_ptr, col_idx,
+                                                                                   stride);
+    return hip_fail(hipGetLastError(), "row_strides_kernel launch");
 }
 
 int pdlp_sell64_widths(int32_t rows, const int32_t* row_ptr, int32_t* width, void* stream) {

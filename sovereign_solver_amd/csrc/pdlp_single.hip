@@ -402,9 +402,30 @@ __global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d, const T* __
         const int kc = (width + NW - 1) / NW;
         const int k0 = min(width, w * kc), k1 = min(width, k0 + kc);
         const T kv = (T)d.k_const;                  // nonzero: every entry holds kv (no values read)
+        // nonzero: the slice's rows step their columns by st (only each row's first index read)
+        const int st = d.sell_stride != nullptr ? d.sell_stride[sl] : 0;
         T ax = T(0), asum = T(0);
         int q = beg + (k0 + ph) * 64 + rb + r;
-        if (kv != T(0)) {
+        if (st != 0) {
+            // a lane without a row (the slice's padding lanes) gathers column 0
+            const bool row = d.sell_rows[sl * 64 + rb + r] >= 0;
+            const int c0 = row ? col[beg + rb + r] : 0, dc = row ? st : 0;
+            int cc = c0 + (k0 + ph) * dc;
+            if (kv != T(0)) {
+#pragma unroll 8
+                for (int k = k0 + ph; k < k1; k += P, cc += P * dc) {
+                    ax += kv * xbar[cc];
+                    asum += fabs(kv);
+                }
+            } else {
+#pragma unroll 8
+                for (int k = k0 + ph; k < k1; k += P, q += 64 * P, cc += P * dc) {
+                    const T a = val[q];
+                    ax += a * xbar[cc];
+                    asum += fabs(a);
+                }
+            }
+        } else if (kv != T(0)) {
 #pragma unroll 8
             for (int k = k0 + ph; k < k1; k += P, q += 64 * P) {
                 ax += kv * xbar[col[q]];
@@ -449,6 +470,9 @@ __global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d, const T* __
         const T* __restrict__ val = (const T*)d.val;
         const T kv = (T)d.k_const;
         const bool uni = kv != T(0);
+        // nonzero: the row's columns step by rs from its first (the only index read)
+        const int rs = d.csr_stride != nullptr ? d.csr_stride[rr] : 0;
+        const int c0 = rs != 0 && end > beg ? ci[beg] : 0;
         for (int k0 = beg + tc; k0 < end; k0 += TC * kRowBatch) {
             T a[kRowBatch];
             int c[kRowBatch];
@@ -456,7 +480,7 @@ __global__ void __launch_bounds__(NT) dual_sell_wg_kernel(PdlpDev d, const T* __
             for (int u = 0; u < kRowBatch; ++u) {
                 const int k = k0 + u * TC;
                 a[u] = k < end ? (uni ? kv : val[k]) : T(0);
-                c[u] = k < end ? ci[k] : 0;
+                c[u] = k < end ? (rs != 0 ? c0 + (k - beg) * rs : ci[k]) : 0;
             }
 #pragma unroll
             for (int u = 0; u < kRowBatch; ++u) {

@@ -13,7 +13,7 @@ import threading
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "_lib", "libpdlp_hip.so")
 
-PDLP_ABI_VERSION = 13
+PDLP_ABI_VERSION = 14
 PDLP_F32, PDLP_F64 = 0, 1
 PDLP_E_ARG, PDLP_E_TUNE, PDLP_E_SHAPE, PDLP_E_LIMIT = -1, -2, -3, -4
 PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL, PDLP_INFEASIBLE, PDLP_UNBOUNDED, PDLP_SKIPPED = 0, 1, 3, 4, 5, 6
@@ -58,7 +58,7 @@ class PdlpDev(C.Structure):
         ("k_const", C.c_int32), ("sell_part", _p), ("sell_asum", _p),
         ("row_scale", _p), ("col_scale", _p),
         ("t_sell_slices", C.c_int32), ("t_sell_reserved", C.c_int32), ("t_sell_ptr", _p), ("t_sell_idx", _p),
-        ("t_sell_val", _p),
+        ("t_sell_val", _p), ("sell_stride", _p), ("csr_stride", _p),
     ]
 
 
@@ -192,6 +192,7 @@ _SIGS = {
     "pdlp_spmv_csr": (C.c_int, [C.c_int32, _p, _p, _p, _p, _p, C.c_int32, C.c_int32, _p]),
     "pdlp_csr_to_sell64": (C.c_int, [C.c_int32, C.c_int32, _p, _p, _p, _p, _p, _p, _p]),
     "pdlp_sell64_widths": (C.c_int, [C.c_int32, _p, _p, _p]),
+    "pdlp_row_strides": (C.c_int, [C.c_int32, _p, _p, _p, _p, _p]),
     "pdlp_arena_transpose_work_bytes": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                                   C.POINTER(C.c_size_t)]),
     "pdlp_arena_transpose": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _p, _p, _p, _p, _p,

@@ -29,6 +29,7 @@ class SellSlices:
     val: np.ndarray        # float64, entries (padding 0)
     csr_rows: np.ndarray   # int32: rows outside the slices
     padded: bool = False   # some slice pads a row (rows of unequal length)
+    stride: np.ndarray = None   # int32 per slice: the column step of every row (col_k = col_0 + k stride), or 0
 
     @property
     def slices(self) -> int:
@@ -70,6 +71,7 @@ def build(K: sp.csr_matrix, line_elems: int = 32, max_lines: float = 4.0, max_pa
     rows_out, col_out, val_out = [], [], []
     in_slice = np.zeros(m, bool)
     padded = False
+    strides = []
     for r in _windows(rl):
         lens = rl[r]
         if lens.min() == 0:
@@ -95,6 +97,12 @@ def build(K: sp.csr_matrix, line_elems: int = 32, max_lines: float = 4.0, max_pa
             continue
         in_slice[r] = True
         padded |= bool(lens.min() != W)
+        st = 0                                                            # index run: rows of full width only
+        if lens.min() == W:
+            st = 1 if W == 1 else int(cols[1, 0] - cols[0, 0])
+            if W > 1 and not np.all(np.diff(cols, axis=0) == st):
+                st = 0
+        strides.append(st)
         rows_out.append(np.concatenate([r, np.full(SLICE - r.size, -1)]).astype(np.int32))  # -1: no row
         c = np.zeros((W, SLICE), np.int32)                                  # column-major: [W, 64]
         v = np.zeros((W, SLICE))
@@ -105,9 +113,54 @@ def build(K: sp.csr_matrix, line_elems: int = 32, max_lines: float = 4.0, max_pa
         ptr.append(ptr[-1] + W * SLICE)
     if len(ptr) == 1:
         return SellSlices(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0),
-                          np.arange(m, dtype=np.int32))
+                          np.arange(m, dtype=np.int32), False, np.zeros(0, np.int32))
     return SellSlices(np.asarray(ptr, np.int32), np.concatenate(rows_out), np.concatenate(col_out),
-                      np.concatenate(val_out), np.flatnonzero(~in_slice).astype(np.int32), padded)
+                      np.concatenate(val_out), np.flatnonzero(~in_slice).astype(np.int32), padded,
+                      np.asarray(strides, np.int32))
+
+
+def row_strides(K: sp.csr_matrix, rows: np.ndarray) -> np.ndarray:
+    """For each row of ``rows``: the step s with col_k = col_0 + k s for all its entries
+    (1 for a single entry), or 0 (no entries, or no constant step): the smallest and
+    largest step between neighbouring entries of each row agree."""
+    rows = np.asarray(rows, np.int64)
+    out = np.zeros(rows.size, np.int32)
+    if rows.size == 0:
+        return out
+    ip = K.indptr.astype(np.int64)
+    lens = ip[rows + 1] - ip[rows]
+    out[lens == 1] = 1
+    two = lens >= 2
+    if not two.any() or K.nnz < 2:
+        return out
+    d = np.diff(K.indices.astype(np.int64))                  # d[e] = col[e + 1] - col[e]
+    cross = ip[1:-1] - 1                                      # steps from one row into the next
+    cross = cross[(cross >= 0) & (cross < d.size)]
+    lo, hi = d.copy(), d.copy()
+    lo[cross], hi[cross] = np.iinfo(np.int64).max, np.iinfo(np.int64).min
+    st = ip[rows[two]]                                        # a row of >= 2 entries: steps [st, end - 1)
+    smin = np.minimum.reduceat(lo, st)
+    smax = np.maximum.reduceat(hi, st)
+    # reduceat runs each segment to the next start: clip it at the row's own end
+    # (later rows' steps are masked as crossings only at row boundaries, so recompute
+    # rows whose next listed start is not their own end)
+    nxt = np.append(st[1:], d.size)
+    own_end = ip[rows[two] + 1] - 1
+    fix = np.flatnonzero(nxt != own_end)
+    for k in fix:
+        seg = d[st[k]: own_end[k]]
+        smin[k], smax[k] = seg.min(), seg.max()
+    out[np.flatnonzero(two)] = np.where(smin == smax, smin, 0).astype(np.int32)
+    return out
+    r2, l2 = rows[two], lens[two]
+    step = ind[ip[r2] + 1] - ind[ip[r2]]
+    # every entry of those rows against col_0 + k step
+    seg = np.repeat(np.arange(r2.size), l2)
+    k = np.arange(int(l2.sum())) - np.repeat(np.cumsum(l2) - l2, l2)
+    e = np.repeat(ip[r2], l2) + k
+    bad = np.bincount(seg[ind[e] != ind[ip[r2]][seg] + k * step[seg]], minlength=r2.size)
+    out[np.flatnonzero(two)] = np.where(bad == 0, step, 0).astype(np.int32)
+    return out
 
 
 # Largest padding (entries of the SELL copy beyond nnz, relative to nnz) for which

@@ -89,9 +89,21 @@ class SingleEngine:
             sl = None if exact else sell.build(K)
             self.sell_slices = 0 if sl is None else sl.slices
             if self.sell_slices:
-                self.sell_ptr, self.sell_rows, self.sell_col, self.sell_val, self.csr_rows = upload(
+                # column runs (col_k = col_0 + k step) of the slices and the CSR rows: the one-launch
+                # dual step then reads one index per row
+                (self.sell_ptr, self.sell_rows, self.sell_col, self.sell_val, self.csr_rows,
+                 self.sell_stride) = upload(
                     [(sl.ptr, i32), (sl.rows, i32), (sl.col, i32), (sl.val, T),
-                     (sl.csr_rows if sl.csr_rows.size else None, i32)], dv, self.stream)
+                     (sl.csr_rows if sl.csr_rows.size else None, i32), (sl.stride, i32)], dv, self.stream)
+                # the CSR rows' column steps, found on the device (pdlp_row_strides)
+                self.csr_stride = None
+                if sl.csr_rows.size:
+                    self.csr_stride = torch.empty(sl.csr_rows.size, dtype=i32, device=dv)
+                    L.check(self.lib.pdlp_row_strides(int(sl.csr_rows.size), ptr(self.csr_rows), ptr(self.row_ptr),
+                                                      ptr(self.col_idx), ptr(self.csr_stride),
+                                                      stream_handle(self.stream)), "pdlp_row_strides")
+                self._sell_widths = np.diff(sl.ptr) // sell.SLICE
+                self._sell_stride_host = sl.stride
                 self.n_csr_rows = int(sl.csr_rows.size)
                 widths = np.diff(sl.ptr) // sell.SLICE
                 self.sell_max_width = int(widths.max())
@@ -103,7 +115,7 @@ class SingleEngine:
                 self.sell_asum = dev_tensor(np.where(sl.rows >= 0, rs[np.maximum(sl.rows, 0)], 1.0), T, dv)
             else:
                 self.sell_ptr = self.sell_rows = self.sell_col = self.sell_val = self.csr_rows = None
-                self.sell_part = self.sell_asum = None
+                self.sell_part = self.sell_asum = self.sell_stride = self.csr_stride = None
                 self.n_csr_rows = 0
                 self.sell_max_width = 0
             # SELL-64 copy of K^T for the primal step (columns in natural order), when the
@@ -148,7 +160,8 @@ class SingleEngine:
             csr_rows=ptr(self.csr_rows), sell_max_width=self.sell_max_width, sell_part=ptr(self.sell_part),
             sell_asum=ptr(self.sell_asum), row_scale=ptr(self.row_scale), col_scale=ptr(self.col_scale),
             t_sell_slices=self.t_sell_slices, t_sell_ptr=ptr(self.t_sell_ptr), t_sell_idx=ptr(self.t_sell_idx),
-            t_sell_val=ptr(self.t_sell_val), k_const=self.k_const)
+            t_sell_val=ptr(self.t_sell_val), k_const=self.k_const, sell_stride=ptr(self.sell_stride),
+            csr_stride=ptr(self.csr_stride))
         self._graphs = {}
         self._graph_tune = None
 
@@ -159,6 +172,19 @@ class SingleEngine:
 
     def _ctune(self, t: Optional[Tune] = None):
         return (t or self.tune).to_c()
+
+    @property
+    def dual_index_loads(self) -> int:
+        """Column indices one launch of the one-launch SELL dual step reads: every entry's,
+        except that a run row (PdlpDev.sell_stride / csr_stride) reads only its first."""
+        if not self.sell_slices:
+            return int(self.nnz)
+        n = int(np.sum(np.where(self._sell_stride_host != 0, sell.SLICE, sell.SLICE * self._sell_widths)))
+        if self.csr_stride is not None:
+            rl = (self.row_ptr[1:] - self.row_ptr[:-1])[self.csr_rows.long()]
+            with torch.cuda.stream(self.stream):
+                n += int(torch.where(self.csr_stride != 0, torch.clamp(rl, max=1), rl).sum().item())
+        return n
 
     def set_tune(self, t: Tune) -> None:
         self.tune = t

@@ -43,8 +43,8 @@ def test_struct_sizes_match_header():
     # PdlpCtrl: 8 int32 + 12 double + (double, 2 int32, double); PdlpDev: 4 int32 + 27 pointers; PdlpTune: 8 int32
     assert C.sizeof(L.PdlpCtrl) == 8 * 4 + 13 * 8 + 2 * 4 + 8
     # ... then the SELL-64 block (2 int32, 5 pointers, 2 int32, 2 pointers), the two scale vectors
-    # and the SELL-64 copy of K^T (2 int32, 3 pointers)
-    assert C.sizeof(L.PdlpDev) == 4 * 4 + 27 * 8 + 2 * 4 + 5 * 8 + 2 * 4 + 2 * 8 + 2 * 8 + 2 * 4 + 3 * 8
+    # the SELL-64 copy of K^T (2 int32, 3 pointers) and the index runs (2 pointers)
+    assert C.sizeof(L.PdlpDev) == 4 * 4 + 27 * 8 + 2 * 4 + 5 * 8 + 2 * 4 + 2 * 8 + 2 * 8 + 2 * 4 + 3 * 8 + 2 * 8
     assert C.sizeof(L.PdlpTune) == 11 * 4
     # PdlpGemm: 10 int32 (M..accumulate), alpha, split_k, 7 pointers (A, B, bias, mask, C, Cb, work)
     assert C.sizeof(L.PdlpGemm) == 10 * 4 + 4 + 4 + 7 * 8
@@ -227,12 +227,37 @@ def test_csr_to_sell64_rejects_bad_arguments_without_device():
     assert lib.pdlp_sell64_widths(-1, None, None, None) == L.PDLP_E_SHAPE
     assert lib.pdlp_sell64_widths(5, None, None, None) == L.PDLP_E_ARG
     assert lib.pdlp_sell64_widths(0, None, None, None) == 0
+    assert lib.pdlp_row_strides(-1, None, None, None, None, None) == L.PDLP_E_SHAPE
+    assert lib.pdlp_row_strides(3, None, None, None, None, None) == L.PDLP_E_ARG
+    assert lib.pdlp_row_strides(0, None, None, None, None, None) == 0
     assert lib.pdlp_spmv_sell64(L.PDLP_F32, None, None, None, None, None, 5, None) == L.PDLP_E_ARG
     assert lib.pdlp_spmv_sell64(L.PDLP_F32, None, None, None, None, None, -1, None) == L.PDLP_E_ARG
     assert lib.pdlp_spmv_sell64(L.PDLP_F32, None, None, None, None, None, 0, None) == 0
     assert lib.pdlp_csr_to_sell64(9, 4, None, None, None, None, None, None, None) == L.PDLP_E_ARG
     assert lib.pdlp_csr_to_sell64(L.PDLP_F32, -1, None, None, None, None, None, None, None) == L.PDLP_E_SHAPE
     assert lib.pdlp_csr_to_sell64(L.PDLP_F32, 4, None, None, None, None, None, None, None) == L.PDLP_E_ARG
+
+
+def test_index_runs_of_rows_and_slices():
+    """engine.sell.row_strides and SellSlices.stride: the constant column step of a row
+    (of every row of a slice, at full width) or 0."""
+    import numpy as np
+    import scipy.sparse as sp
+    from sovereign_solver_amd.engine.forms import box_form
+    from sovereign_solver_amd.engine.sell import build, row_strides
+    from sovereign_solver_amd.workloads import transportation_lp
+    A = sp.csr_matrix(np.array([[0, 1, 0, 1, 0, 1], [1, 1, 0, 0, 0, 0], [0] * 6, [0, 0, 5, 0, 0, 0],
+                                [1, 0, 1, 1, 0, 0]], float))
+    assert list(row_strides(A, np.arange(5))) == [2, 1, 0, 1, 0]
+    assert list(row_strides(A, np.array([4, 0]))) == [0, 2]
+    K = box_form(transportation_lp(30, 150, seed=0)).K
+    sl = build(K)
+    assert sl.slices == 2 and np.all(sl.stride == 150)          # demand row j: columns j, j + 150, ...
+    rs = row_strides(K, sl.csr_rows)
+    assert np.all(rs[sl.csr_rows < 30] == 1)                     # supply rows: contiguous columns
+    for i, s_ in zip(sl.csr_rows, rs):
+        idx = K.indices[K.indptr[i]: K.indptr[i + 1]]
+        assert s_ == 0 or np.array_equal(idx, idx[0] + s_ * np.arange(idx.size))
 
 
 def test_sell_row_slices_hold_each_row_in_csr_order():

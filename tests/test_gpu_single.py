@@ -419,8 +419,10 @@ def test_sell_columns_from_device_widths(gpu, shape):
 
 def test_uniform_values_read_none_same_bits(gpu):
     """A matrix whose entries are all one integer (a transportation LP: 1) with unpadded
-    SELL copies runs the SELL steps without reading values (PdlpDev.k_const): the same
-    bits as reading them; a padded or non-uniform matrix keeps k_const = 0."""
+    SELL copies runs the SELL steps without reading values (PdlpDev.k_const), and rows
+    whose columns step by a constant without reading their indices past the first
+    (PdlpDev.sell_stride / csr_stride): the same bits as reading them; a padded or
+    non-uniform matrix keeps k_const = 0."""
     from sovereign_solver_amd.engine.forms import box_form
     from sovereign_solver_amd.engine.single import SingleEngine
     from sovereign_solver_amd.engine.tune import Tune
@@ -429,9 +431,51 @@ def test_uniform_values_read_none_same_bits(gpu):
     eng = SingleEngine(box_form(p), "fp32", tune=t)
     assert eng.k_const == 1 and eng.sell_slices > 0 and eng.t_sell_slices > 0
     a = eng.solve(1e-4, 50_000)
+    assert eng.sell_stride is not None and int(eng.sell_stride.min()) == 200     # demand rows step by 200
+    assert eng.csr_stride is not None and int(eng.csr_stride.max()) >= 1
     eng.dev.k_const = 0
     eng.drop_graphs()
     b = eng.solve(1e-4, 50_000)
-    assert a["status"] == b["status"] == "optimal" and a["iterations"] == b["iterations"]
+    eng.dev.sell_stride = eng.dev.csr_stride = None          # every index read as well
+    eng.drop_graphs()
+    c = eng.solve(1e-4, 50_000)
+    assert a["status"] == b["status"] == c["status"] == "optimal"
+    assert a["iterations"] == b["iterations"] == c["iterations"]
     assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["y"], b["y"])
+    assert np.array_equal(a["x"], c["x"]) and np.array_equal(a["y"], c["y"])
     assert SingleEngine(box_form(random_sparse_lp(300, 700, 0.02, seed=4)), "fp32").k_const == 0
+
+
+def test_device_row_strides_match_host(gpu):
+    """pdlp_row_strides (the CSR rows' PdlpDev.csr_stride, found on the device) equals
+    engine.sell.row_strides, which a direct check of each row pins, on rows with no, one,
+    evenly and unevenly spaced entries, including rows longer than one wavefront."""
+    from sovereign_solver_amd.engine import _lib as L
+    from sovereign_solver_amd.engine.forms import box_form
+    from sovereign_solver_amd.engine.sell import row_strides
+    lib = L.lib()
+    rows = []
+    for j in range(200):                                     # lengths 0 .. 199, steps 0 .. 6
+        n, st = j % 150, 1 + j % 7
+        rows.append(list(range(0, n * st, st)))
+    rows[5] = [0, 3, 6, 10]                                  # one uneven step at the end
+    rows[150] = list(range(0, 300, 2)) + [301]               # past the first wavefront
+    rows[151] = [7]
+    ind = [np.array(r, np.int32) for r in rows]
+    ptrs = np.concatenate([[0], np.cumsum([len(r) for r in ind])]).astype(np.int32)
+    A = sp.csr_matrix((np.ones(ptrs[-1]), np.concatenate(ind), ptrs), shape=(len(rows), 1200))
+    for K in (A, box_form(transportation_lp(60, 200, seed=2)).K,
+              box_form(random_sparse_lp(300, 700, 0.02, seed=4)).K):
+        sel = np.arange(K.shape[0], dtype=np.int32)[::-1].copy()
+        want = row_strides(K, sel)
+        for r, s_ in zip(sel, want):
+            idx = K.indices[K.indptr[r]: K.indptr[r + 1]]
+            assert s_ == (0 if idx.size == 0 else 1 if idx.size == 1 else
+                          (idx[1] - idx[0] if np.array_equal(idx, idx[0] + (idx[1] - idx[0]) * np.arange(idx.size))
+                           else 0))
+        d = [torch.from_numpy(a).to(gpu) for a in (sel, K.indptr.astype(np.int32), K.indices.astype(np.int32))]
+        out = torch.full((sel.size,), -7, dtype=torch.int32, device=gpu)
+        assert lib.pdlp_row_strides(int(sel.size), d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(),
+                                    out.data_ptr(), None) == 0
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out.cpu().numpy(), want)
