@@ -325,8 +325,8 @@ def family_passes(eng, tol, traffic_key=None, reps: int = 20):
     b = {"primal": fb.column_pass_bytes(), "dual": fb.row_pass_bytes()}
     gbs = {w: b[w] / (us[w] * 1e-6) / 1e9 for w in us}
     dom = max(us, key=us.get)
-    names = {"primal": "family column pass (primal_kernel + primal_long_kernel)",
-             "dual": "family row pass (dual_kernel + dual_long_kernel)"}
+    names = {"primal": "family column pass (primal_kernel: long and short columns in one launch)",
+             "dual": "family row pass (dual_kernel: long and short rows in one launch)"}
     log(f"family passes: column {us['primal']:.1f} us ({b['primal'] / 1e6:.1f} MB, {gbs['primal']:.0f} GB/s), "
         f"row {us['dual']:.1f} us ({b['dual'] / 1e6:.1f} MB, {gbs['dual']:.0f} GB/s)")
     other = "dual" if dom == "primal" else "primal"

@@ -192,9 +192,34 @@ __device__ __forceinline__ void row_update(const PdlpFamily& f, int tile, int la
     }
 }
 
-// Column pass over the short columns [0, n - long_cols),
-// items_per_wave columns per wavefront.
-__global__ void __launch_bounds__(NT) primal_kernel(PdlpFamily f, int P) {
+// One long row (or column) per workgroup of NL wavefronts: each wavefront
+// takes a contiguous share of the nonzeros, the shares meet in LDS in
+// wavefront order, and wavefront 0 applies the update.
+constexpr int NL = 8;
+
+__device__ __forceinline__ float split_dot(const int32_t* __restrict__ idx, const float* __restrict__ val, int beg,
+                                           int end, const float* __restrict__ v, int tile, int len, int lane, int w) {
+    __shared__ float part[NL][T];
+    const int per = (end - beg + NL - 1) / NL;
+    const int b = min(beg + w * per, end), e = min(b + per, end);
+    part[w][lane] = gather_dot(idx, val, b, e, v, tile, len, lane);
+    __syncthreads();
+    float d = 0.f;
+    if (w == 0)
+#pragma unroll
+        for (int u = 0; u < NL; ++u) d += part[u][lane];
+    return d;
+}
+
+// Column pass in one launch.  Chunks [0, long_cols) of a tile take one long
+// column each (the last long_cols columns) on the whole workgroup, which then
+// has NL wavefronts (NTH = NL * 64: the launcher's choice whenever there are
+// long columns); the other chunks take NTH / 64 * items_per_wave short
+// columns, items_per_wave per wavefront.  A long column's sum is split over
+// NL wavefronts and a short column's is one lane's at every launch shape, so
+// the shape changes no result.  Long and short columns run side by side.
+template <int NTH>
+__global__ void __launch_bounds__(NTH) primal_kernel(PdlpFamily f, int P) {
     const Place pl = place(f, P);
     if (!pl.ok) return;
     const int tile = pl.tile;
@@ -204,8 +229,19 @@ __global__ void __launch_bounds__(NT) primal_kernel(PdlpFamily f, int P) {
     const bool act = (mask >> lane) & 1ull;
     const int k = tile * T + lane;
     const float wgt = act ? f.lp[k].weight : 1.f;
-    const int p0 = (pl.chunk * NWV + w) * f.items_per_wave;
-    const int p1 = min(p0 + f.items_per_wave, f.n - f.long_cols);
+    const int nl = f.long_cols;
+    if (pl.chunk < nl) {                               // uniform over the workgroup
+        if constexpr (NTH == NL * kWave) {
+            const int j = f.n - nl + pl.chunk;
+            const float aty =
+                split_dot(f.t_col_idx, f.t_val, f.t_row_ptr[j], f.t_row_ptr[j + 1], f.y, tile, f.m, lane, w);
+            if (w == 0) column_update(f, tile, lane, act, wgt, j, aty);
+        }
+        return;
+    }
+    constexpr int NW = NTH / kWave;
+    const int p0 = ((pl.chunk - nl) * NW + w) * f.items_per_wave;
+    const int p1 = min(p0 + f.items_per_wave, f.n - nl);
     const bool shc = f.shared & PDLP_FAM_SHARED_C, shl = f.shared & PDLP_FAM_SHARED_LO,
                shh = f.shared & PDLP_FAM_SHARED_HI;
     for (int p = p0; p < p1; ++p) {
@@ -227,8 +263,11 @@ __global__ void __launch_bounds__(NT) primal_kernel(PdlpFamily f, int P) {
     }
 }
 
-// Row pass over the short rows.
-__global__ void __launch_bounds__(NT) dual_kernel(PdlpFamily f, int P) {
+// Row pass in one launch, laid out as the column pass: chunks [0, long_rows)
+// take one long row each (the last long_rows rows) on NL wavefronts, the
+// others short rows, items_per_wave per wavefront.
+template <int NTH>
+__global__ void __launch_bounds__(NTH) dual_kernel(PdlpFamily f, int P) {
     const Place pl = place(f, P);
     if (!pl.ok) return;
     const int tile = pl.tile;
@@ -238,8 +277,18 @@ __global__ void __launch_bounds__(NT) dual_kernel(PdlpFamily f, int P) {
     const bool act = (mask >> lane) & 1ull;
     const int k = tile * T + lane;
     const float inv_wgt = act ? 1.f / f.lp[k].weight : 1.f;
-    const int p0 = (pl.chunk * NWV + w) * f.items_per_wave;
-    const int p1 = min(p0 + f.items_per_wave, f.m - f.long_rows);
+    const int nl = f.long_rows;
+    if (pl.chunk < nl) {                               // uniform over the workgroup
+        if constexpr (NTH == NL * kWave) {
+            const int i = f.m - nl + pl.chunk;
+            const float d = split_dot(f.col_idx, f.val, f.row_ptr[i], f.row_ptr[i + 1], f.xbar, tile, f.n, lane, w);
+            if (w == 0) row_update(f, tile, lane, act, inv_wgt, i, d);
+        }
+        return;
+    }
+    constexpr int NW = NTH / kWave;
+    const int p0 = ((pl.chunk - nl) * NW + w) * f.items_per_wave;
+    const int p1 = min(p0 + f.items_per_wave, f.m - nl);
     const bool shb = f.shared & PDLP_FAM_SHARED_B;
     for (int p = p0; p < p1; ++p) {
         const int i = p;
@@ -256,51 +305,6 @@ __global__ void __launch_bounds__(NT) dual_kernel(PdlpFamily f, int P) {
             f.y_psum[ix] = ps + yn;
         }
     }
-}
-
-// One long row (or column) per workgroup of NL wavefronts: each wavefront
-// takes a contiguous share of the nonzeros, the shares meet in LDS in
-// wavefront order, and wavefront 0 applies the update.
-constexpr int NL = 8;
-
-__device__ __forceinline__ float split_dot(const int32_t* __restrict__ idx, const float* __restrict__ val, int beg,
-                                           int end, const float* __restrict__ v, int tile, int len, int lane, int w) {
-    __shared__ float part[NL][T];
-    const int per = (end - beg + NL - 1) / NL;
-    const int b = min(beg + w * per, end), e = min(b + per, end);
-    part[w][lane] = gather_dot(idx, val, b, e, v, tile, len, lane);
-    __syncthreads();
-    float d = 0.f;
-    if (w == 0)
-#pragma unroll
-        for (int u = 0; u < NL; ++u) d += part[u][lane];
-    return d;
-}
-
-__global__ void __launch_bounds__(NL * kWave) primal_long_kernel(PdlpFamily f, int P) {
-    const Place pl = place(f, P);
-    if (!pl.ok) return;
-    const int tile = pl.tile;
-    const uint64_t mask = f.run_mask[tile];
-    if (mask == 0) return;
-    const int lane = lane_id(), w = wave_id();
-    const bool act = (mask >> lane) & 1ull;
-    const int j = f.n - f.long_cols + pl.chunk;
-    const float aty = split_dot(f.t_col_idx, f.t_val, f.t_row_ptr[j], f.t_row_ptr[j + 1], f.y, tile, f.m, lane, w);
-    if (w == 0) column_update(f, tile, lane, act, act ? f.lp[tile * T + lane].weight : 1.f, j, aty);
-}
-
-__global__ void __launch_bounds__(NL * kWave) dual_long_kernel(PdlpFamily f, int P) {
-    const Place pl = place(f, P);
-    if (!pl.ok) return;
-    const int tile = pl.tile;
-    const uint64_t mask = f.run_mask[tile];
-    if (mask == 0) return;
-    const int lane = lane_id(), w = wave_id();
-    const bool act = (mask >> lane) & 1ull;
-    const int i = f.m - f.long_rows + pl.chunk;
-    const float d = split_dot(f.col_idx, f.val, f.row_ptr[i], f.row_ptr[i + 1], f.xbar, tile, f.n, lane, w);
-    if (w == 0) row_update(f, tile, lane, act, act ? 1.f / f.lp[tile * T + lane].weight : 1.f, i, d);
 }
 
 // ------------------------------------------------------------------- checks
@@ -1011,9 +1015,11 @@ dim3 tile_grid(const PdlpFamily& f, int P) {
     return dim3((unsigned)(8 * per_xcd), 1u);
 }
 
-int step_chunks(const PdlpFamily& f, int items) {
-    const int per_block = NWV * f.items_per_wave;
-    return (items + per_block - 1) / per_block;
+// Chunks per tile of a pass over nl long items and `items` short ones, nw
+// wavefronts per workgroup.
+int pass_chunks(const PdlpFamily& f, int items, int nl, int nw) {
+    const int per_block = nw * f.items_per_wave;
+    return nl + (items + per_block - 1) / per_block;
 }
 
 dim3 vec_grid(const PdlpFamily& f) {
@@ -1021,16 +1027,27 @@ dim3 vec_grid(const PdlpFamily& f) {
     return dim3((unsigned)((f.n + f.m + per_block - 1) / per_block), (unsigned)f.tiles);
 }
 
+// With long items the pass runs on NL-wavefront workgroups, else on NT threads.
 void launch_column_pass(const PdlpFamily& f, hipStream_t s) {
-    const int sc = f.n - f.long_cols, pc = step_chunks(f, sc);
-    if (f.long_cols) primal_long_kernel<<<tile_grid(f, f.long_cols), NL * kWave, 0, s>>>(f, f.long_cols);
-    if (sc) primal_kernel<<<tile_grid(f, pc), NT, 0, s>>>(f, pc);
+    const int nl = f.long_cols, sc = f.n - nl;
+    if (nl) {
+        const int P = pass_chunks(f, sc, nl, NL);
+        primal_kernel<NL * kWave><<<tile_grid(f, P), NL * kWave, 0, s>>>(f, P);
+    } else if (sc) {
+        const int P = pass_chunks(f, sc, 0, NWV);
+        primal_kernel<NT><<<tile_grid(f, P), NT, 0, s>>>(f, P);
+    }
 }
 
 void launch_row_pass(const PdlpFamily& f, hipStream_t s) {
-    const int sr = f.m - f.long_rows, pr = step_chunks(f, sr);
-    if (f.long_rows) dual_long_kernel<<<tile_grid(f, f.long_rows), NL * kWave, 0, s>>>(f, f.long_rows);
-    if (sr) dual_kernel<<<tile_grid(f, pr), NT, 0, s>>>(f, pr);
+    const int nl = f.long_rows, sr = f.m - nl;
+    if (nl) {
+        const int P = pass_chunks(f, sr, nl, NL);
+        dual_kernel<NL * kWave><<<tile_grid(f, P), NL * kWave, 0, s>>>(f, P);
+    } else if (sr) {
+        const int P = pass_chunks(f, sr, 0, NWV);
+        dual_kernel<NT><<<tile_grid(f, P), NT, 0, s>>>(f, P);
+    }
 }
 
 void launch_iterations(const PdlpFamily& f, int iters, hipStream_t s) {

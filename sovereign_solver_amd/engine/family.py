@@ -520,6 +520,9 @@ class FamilyEngine:
             self.stream.wait_stream(torch.cuda.current_stream())
         L.check(self.lib.pdlp_family_set_start(C.byref(self.args), src.data_ptr(), d, x_off, y_off, w_col,
                                                ptr(scale), ptr(shift), self.sh), "pdlp_family_set_start")
+        for t_ in (src, scale, shift):          # read on this stream: not reused before it has been
+            if t_ is not None:
+                t_.record_stream(self.stream)
         L.check(self.lib.pdlp_family_init(C.byref(self.args), self.sh), "pdlp_family_init")
 
     def run(self, max_iters: int) -> int:
