@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PDLP_ABI_VERSION 14
+#define PDLP_ABI_VERSION 15
 
 /* precision of iterates, matrix values and problem vectors */
 #define PDLP_F32 0
@@ -553,6 +553,13 @@ typedef struct PdlpFamily {
                                fallen below its starting value and is at least a tenth of the
                                cold start's restarts from the cold start (weight_cold, sums
                                cleared; iter0 = that iteration) */
+    int32_t max_iters;      /* 0: none (the caller stops the run, pdlp_family_finish); > 0: an LP
+                               still running at a check at or past max_iters of its own
+                               iterations stops there with PDLP_RUNNING and its iterate after
+                               that check's action as solution, as pdlp_family_finish with tail
+                               0 would leave it (for streams whose LPs start at different checks;
+                               a multiple of check_every gives the batch run's limit exactly) */
+    int32_t reserved;
     double tol;
     /* K and K^T, CSR, FP32 values */
     const int32_t* row_ptr; const int32_t* col_idx; const float* val;
@@ -597,6 +604,16 @@ int pdlp_family_load(const PdlpFamily* f, const double* c, const double* b, cons
 /* Set up every LP from (x, y), lp[k].norm_b/norm_c/weight as filled by the
  * host: sums cleared, restart point = start, KKT of the start. */
 int pdlp_family_init(const PdlpFamily* f, void* stream);
+/* The records lp[0 .. count) of LPs just loaded (pdlp_family_load's norms), on
+ * the device: every field 0 except norm_b = norms[3k], norm_c = norms[3k+1],
+ * weight = weight0 = weight_cold = the cold weight norms[3k+2] for the adaptive
+ * schemes (flags bit 0) and 1 for the reference one, warm = `warm`.  With
+ * pdlp_family_load, pdlp_family_set_start and pdlp_family_init on a view of f
+ * whose per-LP pointers start at a tile t0 (tiled arrays advanced by
+ * t0 * 64 * length, lp / out by t0 * 64 records, run_mask by t0 words), this
+ * admits new LPs into free tiles while the LPs of the other tiles keep running
+ * (a stream of LPs through one engine).  No reference counterpart. */
+int pdlp_family_records(const PdlpFamily* f, const double* norms, int32_t warm, void* stream);
 /* Starting points from a row-major FP32 matrix src[count][ld] (src may be
  * NULL when x_off, y_off and w_col are all negative: cold start) (e.g. the
  * warm-start network's output, already on the device): v = src[k*ld + col]
@@ -621,7 +638,7 @@ int pdlp_family_finish(const PdlpFamily* f, int32_t tail, void* stream);
  * launch with pdlp_graph_launch, free with pdlp_graph_destroy. */
 int pdlp_family_graph_capture(const PdlpFamily* f, int32_t blocks, void* stream, void** exec);
 /* HIP graph of `reps` column passes (which = 0) or row passes (which = 1),
- * each the long-item and the short-item kernel, for timing. */
+ * one launch each (long and short items together), for timing. */
 int pdlp_family_graph_capture_step(const PdlpFamily* f, int32_t which, int32_t reps, void* stream, void** exec);
 
 /* ----------------------------------------------------------------------------

@@ -39,8 +39,11 @@ constexpr int KCH = PDLP_FAM_KKT_CHUNKS;
 static_assert(T == kWave, "one tile per wavefront");
 
 // ACT_COLD: restart from the cold start (x = clip(0, lo, hi), y = 0), window sums cleared.
+// ACT_*_STOP: that action, then stop at the LP's iteration limit with the resulting
+// iterate as solution (what ACT_DONE_CUR at pdlp_family_finish would leave).
 enum Action : int32_t {
-    ACT_NONE = 0, ACT_RESTART_AVG = 1, ACT_RESTART_CUR = 2, ACT_DONE_AVG = 3, ACT_DONE_CUR = 4, ACT_COLD = 5
+    ACT_NONE = 0, ACT_RESTART_AVG = 1, ACT_RESTART_CUR = 2, ACT_DONE_AVG = 3, ACT_DONE_CUR = 4, ACT_COLD = 5,
+    ACT_RESTART_AVG_STOP = 6, ACT_RESTART_CUR_STOP = 7, ACT_COLD_STOP = 8
 };
 
 // The cold start's x_j = clip(0, lo_j, hi_j) for the lane's LP.
@@ -583,7 +586,7 @@ __global__ void __launch_bounds__(NSC * T) decide_kernel(PdlpFamily f, int mode,
     }
     __syncthreads();
     if (w != 0) return;                                   // no barrier below
-    bool running = false;
+    bool running = false, stop = false;
     if (k < f.count) {
         PdlpFamilyLP s = f.lp[k];
         if (!act) {
@@ -686,9 +689,17 @@ __global__ void __launch_bounds__(NSC * T) decide_kernel(PdlpFamily f, int mode,
                         s.warm = 0;
                     }
                 }
+                if (s.status == PDLP_RUNNING && f.max_iters > 0 && it >= f.max_iters) {   // the LP's own limit
+                    stop = true;
+                    s.done_iter = it;
+                    s.action = s.action == ACT_RESTART_AVG   ? ACT_RESTART_AVG_STOP
+                               : s.action == ACT_RESTART_CUR ? ACT_RESTART_CUR_STOP
+                               : s.action == ACT_COLD        ? ACT_COLD_STOP
+                                                             : ACT_DONE_CUR;
+                }
                 write_out(&f.out[k], s, ka, s.status == PDLP_RUNNING ? it : s.done_iter);
             }
-            running = s.status == PDLP_RUNNING;
+            running = s.status == PDLP_RUNNING && !stop;   // a stopped LP keeps PDLP_RUNNING
         }
         f.lp[k] = s;
     }
@@ -739,6 +750,24 @@ __global__ void __launch_bounds__(NT) apply_kernel(PdlpFamily f) {
                 last[ix] = nv;
                 // a start that ends the solve at once (status set at init) keeps it as solution
                 sum[ix] = f.lp[k].status == PDLP_RUNNING ? 0.0 : (double)nv;
+                break;
+            }
+            case ACT_RESTART_AVG_STOP: {
+                const float nv = (float)(sum[ix] / wd);
+                v[ix] = nv;
+                if (move_ref) last[ix] = nv;
+                sum[ix] = (double)nv;
+                break;
+            }
+            case ACT_RESTART_CUR_STOP:
+                if (move_ref) last[ix] = v[ix];
+                sum[ix] = (double)v[ix];
+                break;
+            case ACT_COLD_STOP: {
+                const float nv = col ? cold_x(f, tile, d, lane) : 0.f;
+                v[ix] = nv;
+                last[ix] = nv;
+                sum[ix] = (double)nv;
                 break;
             }
             default:
@@ -805,6 +834,19 @@ __global__ void __launch_bounds__(T) init_lp_kernel(PdlpFamily f) {
     }
     const uint64_t run = __ballot(valid);
     if (lane == 0) f.run_mask[tile] = run;
+}
+
+// Records of freshly loaded LPs from their norms (pdlp_family_records).
+__global__ void __launch_bounds__(T) records_kernel(PdlpFamily f, const double* __restrict__ norms, int warm) {
+    const int k = blockIdx.x * T + threadIdx.x;
+    if (k >= f.count) return;
+    PdlpFamilyLP s = {};
+    s.norm_b = norms[3 * (size_t)k];
+    s.norm_c = norms[3 * (size_t)k + 1];
+    const float w = (f.flags & 1) ? (float)norms[3 * (size_t)k + 2] : 1.f;
+    s.weight = s.weight0 = s.weight_cold = w;
+    s.warm = warm;
+    f.lp[k] = s;
 }
 
 // LPs still running at the end: iteration limit, current iterate as solution.
@@ -991,6 +1033,7 @@ int check_family(const PdlpFamily* f) {
     if (f->tiles != (f->count + T - 1) / T) return fail(PDLP_E_SHAPE, "tiles must be ceil(count / 64)");
     if (f->tiles > 65535) return fail(PDLP_E_LIMIT, "at most 65535 tiles of 64 LPs");
     if (f->check_every <= 0) return fail(PDLP_E_ARG, "check_every must be > 0");
+    if (f->max_iters < 0) return fail(PDLP_E_ARG, "max_iters must be >= 0");
     if (f->items_per_wave < 1 || f->items_per_wave > 64) return fail(PDLP_E_TUNE, "items_per_wave in [1, 64]");
     if (f->long_rows < 0 || f->long_rows > f->m || f->long_cols < 0 || f->long_cols > f->n)
         return fail(PDLP_E_SHAPE, "long_rows in [0, m] and long_cols in [0, n]");
@@ -1147,6 +1190,14 @@ extern "C" int pdlp_family_load(const PdlpFamily* f, const double* c, const doub
         load_shift_kernel<<<dim3((unsigned)((f->m + NWV - 1) / NWV), (unsigned)f->tiles), NT, 0, s>>>(*f);
     load_norm_kernel<<<f->tiles, NWN * T, 0, s>>>(*f, norms);
     return hip_fail(hipGetLastError(), "pdlp_family_load launch");
+}
+
+extern "C" int pdlp_family_records(const PdlpFamily* f, const double* norms, int32_t warm, void* stream) {
+    if (int e = check_family(f)) return e;
+    if (norms == nullptr) return fail(PDLP_E_ARG, "norms is NULL");
+    if (warm < 0 || warm > 3) return fail(PDLP_E_ARG, "warm must be 0 .. 3");
+    records_kernel<<<f->tiles, T, 0, as_stream(stream)>>>(*f, norms, warm);
+    return hip_fail(hipGetLastError(), "records_kernel launch");
 }
 
 extern "C" int pdlp_family_finish(const PdlpFamily* f, int32_t tail, void* stream) {

@@ -13,7 +13,7 @@ import threading
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "_lib", "libpdlp_hip.so")
 
-PDLP_ABI_VERSION = 14
+PDLP_ABI_VERSION = 15
 PDLP_F32, PDLP_F64 = 0, 1
 PDLP_E_ARG, PDLP_E_TUNE, PDLP_E_SHAPE, PDLP_E_LIMIT = -1, -2, -3, -4
 PDLP_RUNNING, PDLP_OPTIMAL, PDLP_NUMERICAL, PDLP_INFEASIBLE, PDLP_UNBOUNDED, PDLP_SKIPPED = 0, 1, 3, 4, 5, 6
@@ -106,7 +106,7 @@ class PdlpFamily(C.Structure):
         ("m", C.c_int32), ("n", C.c_int32), ("count", C.c_int32), ("tiles", C.c_int32),
         ("check_every", C.c_int32), ("flags", C.c_int32), ("shared", C.c_int32), ("items_per_wave", C.c_int32),
         ("long_rows", C.c_int32), ("long_cols", C.c_int32), ("xcd_affinity", C.c_int32),
-        ("failover_iters", C.c_int32),
+        ("failover_iters", C.c_int32), ("max_iters", C.c_int32), ("reserved", C.c_int32),
         ("tol", C.c_double),
         ("row_ptr", _p), ("col_idx", _p), ("val", _p), ("t_row_ptr", _p), ("t_col_idx", _p), ("t_val", _p),
         ("row_kind", _p), ("tau_hat", _p), ("sigma_hat", _p), ("row_metric", _p), ("row_orig", _p),
@@ -214,6 +214,7 @@ _SIGS = {
                                           C.POINTER(C.c_void_p)]),
     "pdlp_batch_solve": (C.c_int, [C.POINTER(PdlpBatch), _p]),
     "pdlp_family_init": (C.c_int, [C.POINTER(PdlpFamily), _p]),
+    "pdlp_family_records": (C.c_int, [C.POINTER(PdlpFamily), _p, C.c_int32, _p]),
     "pdlp_family_load": (C.c_int, [C.POINTER(PdlpFamily), _p, _p, _p, _p, _p, _p]),
     "pdlp_family_run": (C.c_int, [C.POINTER(PdlpFamily), C.c_int32, _p]),
     "pdlp_family_iterate": (C.c_int, [C.POINTER(PdlpFamily), C.c_int32, _p]),

@@ -12,7 +12,8 @@ and termination, exactly as in the persistent batched engine.
 from __future__ import annotations
 
 import ctypes as C
-from typing import Optional, Sequence
+from dataclasses import dataclass
+from typing import Dict, Optional, Sequence
 
 import numpy as np
 import scipy.sparse as sp
@@ -242,6 +243,35 @@ class FamilyBatch:
         return self.column_pass_bytes() + self.row_pass_bytes()
 
 
+@dataclass
+class StreamBatch:
+    """One batch of LPs of a stream through a :class:`FamilyEngine`
+    (:meth:`FamilyEngine.run_stream`): ``src`` the LP-major FP64 device sources
+    [k, len] of the per-LP vectors, named as :meth:`FamilyEngine.staged` names
+    them (exactly the engine's non-shared ones); ``start`` None (cold) or a
+    contiguous FP32 device matrix [k, d] read as by
+    :meth:`FamilyEngine.start_from_device` (``x_off``, ``y_off``, ``w_col``),
+    complete once ``ready`` (an event, or None) has passed.  ``count``: the
+    number of LPs, needed only when neither sources nor a start give it."""
+    src: Dict[str, torch.Tensor]
+    start: Optional[torch.Tensor] = None
+    ready: Optional[torch.cuda.Event] = None
+    x_off: int = 0
+    y_off: int = -1
+    w_col: int = -1
+    count: Optional[int] = None
+
+    def __post_init__(self):
+        sizes = {int(v.shape[0]) for v in self.src.values()}
+        if self.start is not None:
+            sizes.add(int(self.start.shape[0]))
+        if self.count is not None:
+            sizes.add(int(self.count))
+        if len(sizes) != 1:
+            raise ValueError("a stream batch needs its sources, start and count for the same LPs")
+        self.count = sizes.pop()
+
+
 class FamilyEngine:
     """Device state of a :class:`FamilyBatch` and the lockstep solve loop."""
 
@@ -287,6 +317,7 @@ class FamilyEngine:
             t["out"] = torch.zeros(B * OUT_DTYPE.itemsize, dtype=torch.uint8, device=dv)
             t["run_mask"] = torch.zeros(fb.tiles, dtype=torch.int64, device=dv)
             t["running"] = torch.zeros(1, dtype=i32, device=dv)
+            t["running_sub"] = torch.zeros(1, dtype=i32, device=dv)     # the count of a view's own check
             t["X_out"] = torch.zeros((B, n), dtype=f64, device=dv)
             t["Y_out"] = torch.zeros((B, m), dtype=f64, device=dv)
             t["out_final"] = torch.zeros(B * OUT_DTYPE.itemsize, dtype=torch.uint8, device=dv)
@@ -324,7 +355,8 @@ class FamilyEngine:
                                  shared=fb.shared, items_per_wave=items_per_wave, long_rows=fb.long_rows,
                                  long_cols=fb.long_cols, xcd_affinity=int(xcd_affinity),
                                  failover_iters=int(failover_iters), tol=0.0)
-        for name, _ in L.PdlpFamily._fields_[13:]:
+        first = [name for name, _ in L.PdlpFamily._fields_].index("row_ptr")
+        for name, _ in L.PdlpFamily._fields_[first:]:
             setattr(self.args, name, ptr(t[name]))
         self._graphs = {}
         self._host = None
@@ -579,6 +611,171 @@ class FamilyEngine:
         L.check(self.lib.pdlp_family_finish(C.byref(self.args), tail, self.sh), "pdlp_family_finish")
         return done + tail
 
+    # ---------------------------------------------------------------- streams
+    def _tiled(self):
+        """(name, length, element bytes) of every per-LP tiled array."""
+        fb = self.fb
+        out = [("x", fb.n, 4), ("xbar", fb.n, 4), ("x_psum", fb.n, 4), ("x_last", fb.n, 4),
+               ("y", fb.m, 4), ("y_psum", fb.m, 4), ("y_last", fb.m, 4), ("x_sum", fb.n, 8), ("y_sum", fb.m, 8)]
+        for name, dt in self._DATA.items():
+            if not fb.is_shared(name):
+                out.append((name, fb.n if name in self._COLS else fb.m, 8 if dt == torch.float64 else 4))
+        return out
+
+    def _view(self, t0: int, count: int) -> L.PdlpFamily:
+        """Argument block of ``count`` LPs in the slots from tile ``t0`` on: every per-LP
+        pointer advanced to that tile, the view's running count in its own word."""
+        a = L.PdlpFamily.from_buffer_copy(self.args)
+        a.count, a.tiles = count, (count + TILE - 1) // TILE
+        for name, ln, es in self._tiled():
+            setattr(a, name, self.t[name].data_ptr() + t0 * TILE * ln * es)
+        a.lp = self.t["lp"].data_ptr() + t0 * TILE * C.sizeof(L.PdlpFamilyLP)
+        a.out = self.t["out"].data_ptr() + t0 * TILE * OUT_DTYPE.itemsize
+        a.run_mask = self.t["run_mask"].data_ptr() + t0 * 8
+        a.running = self.t["running_sub"].data_ptr()
+        return a
+
+    def _admit(self, t0: int, b: StreamBatch, first: int, k: int, lp0: int) -> None:
+        """Rows first .. first + k of stream batch ``b`` (stream LPs lp0 ..) into the
+        slots from tile t0: their data, records, start and initial check, on the
+        device (``pdlp_family_load`` / ``_records`` / ``_set_start`` / ``_init`` on a
+        view of those tiles), while the LPs of the other tiles keep their state."""
+        v = self._view(t0, k)
+        norms = self.t["norms"].data_ptr() + t0 * TILE * 3 * 8
+        row = lambda name: (b.src[name][first:].data_ptr() if name in b.src else None)
+        L.check(self.lib.pdlp_family_load(C.byref(v), row("c"), row("b"), row("lo"), row("hi"), norms, self.sh),
+                "pdlp_family_load")
+        warm = 0
+        if b.start is not None:
+            warm = (1 if b.x_off >= 0 or b.y_off >= 0 or b.w_col >= 0 else 0) | (2 if b.y_off >= 0 else 0)
+        L.check(self.lib.pdlp_family_records(C.byref(v), norms, warm, self.sh), "pdlp_family_records")
+        if b.start is None:
+            L.check(self.lib.pdlp_family_set_start(C.byref(v), None, 0, -1, -1, -1, None, None, self.sh),
+                    "pdlp_family_set_start")
+        else:
+            if b.ready is not None:
+                self.stream.wait_event(b.ready)
+            d = b.start.shape[1]
+            L.check(self.lib.pdlp_family_set_start(C.byref(v), b.start[first:].data_ptr(), d, b.x_off, b.y_off,
+                                                   b.w_col, None, None, self.sh), "pdlp_family_set_start")
+        L.check(self.lib.pdlp_family_init(C.byref(v), self.sh), "pdlp_family_init")
+        end = t0 * TILE + k
+        pad = -(-end // TILE) * TILE
+        if self.slot_lp.size < pad:
+            self.slot_lp = np.concatenate([self.slot_lp, np.full(pad - self.slot_lp.size, -1)])
+        self.slot_lp[t0 * TILE: pad] = -1
+        self.slot_lp[t0 * TILE: end] = np.arange(lp0, lp0 + k)
+
+    def run_stream(self, tol: float, max_iters: int, batches: Sequence[StreamBatch], admit_tiles: int = 0):
+        """Solve a stream of batches of LPs of this engine's family through its slots:
+        LPs enter free tiles as earlier ones finish (at a check: the running LPs
+        are packed into the leading tiles, the next LPs of the stream loaded into
+        the tiles behind them), so the device runs full tiles until the stream
+        runs dry instead of waiting, batch by batch, on each batch's slowest LPs.
+
+        Each LP runs exactly the iterations, checks and decisions it would run in
+        a batch solve from the same start (its checks fall every check_every of
+        its own iterations; ``max_iters``, a multiple of check_every, is its own
+        limit), so its result is the same bit for bit.  ``admit_tiles``: free whole
+        tiles that trigger an admission (0: a quarter of the engine's tiles).
+        Returns (out records [N] (uint8 view of OUT_DTYPE), X [N, n], Y [N, m]) on
+        the device, LPs in stream order (problems' own row and column order);
+        the engine's next solve reloads its own batch."""
+        fb = self.fb
+        ce = self.check_every
+        if max_iters <= 0 or max_iters % ce:
+            raise ValueError("a stream's max_iters must be a positive multiple of check_every")
+        need_src = set(self._src)
+        for b in batches:
+            if set(b.src) != need_src:
+                raise ValueError(f"stream batch sources must be exactly {sorted(need_src)}")
+            for name, v in b.src.items():
+                ln = fb.n if name in ("c", "lo", "hi") else fb.m
+                if v.dtype != torch.float64 or v.dim() != 2 or v.shape[1] != ln or not v.is_contiguous():
+                    raise ValueError(f"source {name} must be a contiguous [k, {ln}] float64 device tensor")
+            if b.start is not None and (b.start.dtype != torch.float32 or b.start.dim() != 2
+                                        or not b.start.is_contiguous()):
+                raise ValueError("a stream batch's start must be a contiguous [k, d] float32 device tensor")
+        total = sum(b.count for b in batches)
+        B, tiles = fb.count, fb.tiles
+        admit_tiles = admit_tiles or max(1, tiles // 4)
+        dv = self.device
+        keep = (self.t["X_out"], self.t["Y_out"], self.t["out_final"])
+        with torch.cuda.stream(self.stream):
+            X = torch.zeros((total, fb.n), dtype=torch.float64, device=dv)
+            Y = torch.zeros((total, fb.m), dtype=torch.float64, device=dv)
+            O = torch.zeros(total * OUT_DTYPE.itemsize, dtype=torch.uint8, device=dv)
+        self.t["X_out"], self.t["Y_out"], self.t["out_final"] = X, Y, O
+        self.set_tol(tol)
+        self.args.max_iters = int(max_iters)
+        self.compactions += 1             # the slots' data is the stream's from here: reload on the next solve
+        self.drop_graphs()
+        self.saved = np.zeros(total, bool)
+        self.slot_lp = np.full(tiles * TILE, -1)
+        queue = [(b, 0) for b in batches if b.count]
+        lp_next = 0
+
+        def fill(t0: int) -> int:
+            """Admit the next LPs into tiles t0 ..; returns the slots now in use."""
+            nonlocal lp_next
+            slot = t0 * TILE
+            while queue and slot < B:
+                b, first = queue[0]
+                k = min(b.count - first, B - slot)
+                self._admit(slot // TILE, b, first, k, lp_next)
+                lp_next += k
+                if first + k == b.count:
+                    queue.pop(0)
+                else:
+                    queue[0] = (b, first + k)
+                slot += -(-k // TILE) * TILE          # a batch's last LPs may leave a tile part empty
+            return min(slot, B) if slot > t0 * TILE else t0 * TILE
+
+        try:
+            used = fill(0)
+            self.args.count, self.args.tiles = used, -(-used // TILE)
+            inflight, slot = [], 0
+            while self.args.count > 0:
+                with torch.cuda.stream(self.stream):
+                    L.check(self.lib.pdlp_graph_launch(self._graph(1), self.sh), "pdlp_graph_launch")
+                    self.running_host[slot].copy_(self.t["running"], non_blocking=True)
+                    self.mask_host[slot, : self.args.tiles].copy_(self.t["run_mask"][: self.args.tiles],
+                                                                  non_blocking=True)
+                    ev = torch.cuda.Event()
+                    ev.record(self.stream)
+                inflight.append((slot, ev, self.args.tiles))
+                slot ^= 1
+                if len(inflight) < 2:
+                    continue
+                s0, e0, t0 = inflight.pop(0)
+                e0.synchronize()
+                running = int(self.running_host[s0, 0])
+                need = -(-running // TILE)
+                if queue and (tiles - need >= admit_tiles or running == 0):
+                    self.stream.synchronize()
+                    inflight.clear()
+                    self.compact()                     # running LPs into tiles [0, need'), finished saved
+                    t_run = self.args.tiles
+                    used = fill(t_run)
+                    if used > t_run * TILE:
+                        self.args.count, self.args.tiles = used, -(-used // TILE)
+                    self.drop_graphs()
+                elif not queue:
+                    if running == 0:
+                        break
+                    live = int(np.count_nonzero(self.mask_host[s0, :t0].numpy()))
+                    if self.compaction and live >= 2 and need <= 0.75 * live:
+                        self.stream.synchronize()
+                        inflight.clear()
+                        self.compact()
+            self.collect()
+            self.stream.synchronize()
+        finally:
+            self.t["X_out"], self.t["Y_out"], self.t["out_final"] = keep
+            self.args.max_iters = 0
+            self.drop_graphs()
+        return O, X, Y
+
     # ------------------------------------------------------------ compaction
     def _slot_index(self, perm: torch.Tensor, ln: int, new_tiles: int) -> torch.Tensor:
         """Flat source index of every element of the first ``new_tiles`` tiles after
@@ -619,9 +816,12 @@ class FamilyEngine:
             mask = self.t["run_mask"][:T].cpu().numpy().view(np.uint64)
         bits = ((mask[:, None] >> np.arange(TILE, dtype=np.uint64)[None, :]) & np.uint64(1)).astype(bool).ravel()
         slots = np.arange(T * TILE)
-        valid = slots < cnt
+        lp_of = np.full(T * TILE, -1)
+        have = min(T * TILE, self.slot_lp.size)
+        lp_of[:have] = self.slot_lp[:have]
+        valid = (slots < cnt) & (lp_of >= 0)           # a stream may leave empty slots inside count
         run = bits & valid
-        done_slots = slots[valid & ~run & ~self.saved[np.maximum(self.slot_lp, 0)]]
+        done_slots = slots[valid & ~run & ~self.saved[np.maximum(lp_of, 0)]]
         self._save(done_slots)
         keep = slots[run]
         R = int(keep.size)
@@ -644,7 +844,7 @@ class FamilyEngine:
                 new_mask[t_] = np.uint64((1 << lanes) - 1) if lanes < 64 else np.uint64(0xFFFFFFFFFFFFFFFF)
             self.t["run_mask"][:new_T].copy_(torch.from_numpy(new_mask.view(np.int64)).to(self.device))
         self.stream.synchronize()
-        self.slot_lp = np.concatenate([self.slot_lp[perm_np[:R]], np.full(new_T * TILE - R, -1)])
+        self.slot_lp = np.concatenate([lp_of[perm_np[:R]], np.full(new_T * TILE - R, -1)])
         self.args.tiles = new_T
         self.args.count = R
         self.compactions += 1
@@ -654,9 +854,9 @@ class FamilyEngine:
         """Gather every LP's result into the LP-major device outputs (``X_out``, ``Y_out``,
         ``out_final``; problems' own order), leaving them in HBM."""
         self.stream.synchronize()
-        cnt = self.args.count
-        live = np.arange(cnt)[~self.saved[self.slot_lp[:cnt]]]
-        self._save(live)
+        lp_of = self.slot_lp[: self.args.count]
+        has = np.flatnonzero(lp_of >= 0)           # a stream may leave empty slots
+        self._save(has[~self.saved[lp_of[has]]])
 
     def results(self):
         """(out records, X [B, n] FP64 solutions, Y [B, m]) by LP, in the problems'

@@ -92,7 +92,7 @@ def test_struct_layouts():
     import ctypes as C
     # 8 int32, 5 doubles, weight, weight0, weight_cold, warm, iter0, reserved, start_err, cold_err
     assert C.sizeof(L.PdlpFamilyLP) == 8 * 4 + 5 * 8 + 4 * 6 + 8 + 8
-    assert C.sizeof(L.PdlpFamily) == 12 * 4 + 8 + 35 * 8
+    assert C.sizeof(L.PdlpFamily) == 14 * 4 + 8 + 35 * 8
     lib = L.lib()
     assert lib.pdlp_family_init(None, None) == -1                               # PDLP_E_ARG
     assert b"NULL" in lib.pdlp_last_error()
