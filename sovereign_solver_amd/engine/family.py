@@ -50,6 +50,12 @@ def _same_matrix(A, B) -> bool:
             and np.array_equal(A.data, B.data))
 
 
+def _h2d(a: np.ndarray, dv) -> torch.Tensor:
+    """A host array on the device, copied on the current stream from pinned memory
+    without waiting for that stream's earlier work."""
+    return torch.from_numpy(np.ascontiguousarray(a)).pin_memory().to(dv, non_blocking=True)
+
+
 def to_tiles(a: np.ndarray, tiles: int) -> np.ndarray:
     """[B, L] (LP-major) -> flat tiled layout [tiles, L, 64], padded LPs zero."""
     B, Ln = a.shape
@@ -359,6 +365,7 @@ class FamilyEngine:
         for name, _ in L.PdlpFamily._fields_[first:]:
             setattr(self.args, name, ptr(t[name]))
         self._graphs = {}
+        self._cap_stream = torch.cuda.Stream(device=dv)   # graphs are captured here, off the busy stream
         self._host = None
         self.load()
 
@@ -447,7 +454,8 @@ class FamilyEngine:
         g = self._graphs.get(blocks)
         if g is None:
             h = C.c_void_p()
-            L.check(self.lib.pdlp_family_graph_capture(C.byref(self.args), blocks, self.sh, C.byref(h)),
+            L.check(self.lib.pdlp_family_graph_capture(C.byref(self.args), blocks,
+                                                       stream_handle(self._cap_stream), C.byref(h)),
                     "pdlp_family_graph_capture")
             g = self._graphs[blocks] = h
         return g
@@ -754,7 +762,7 @@ class FamilyEngine:
                 if queue and (tiles - need >= admit_tiles or running == 0):
                     self.stream.synchronize()
                     inflight.clear()
-                    self.compact()                     # running LPs into tiles [0, need'), finished saved
+                    self.compact()                     # running LPs to the front, finished ones saved
                     t_run = self.args.tiles
                     used = fill(t_run)
                     if used > t_run * TILE:
@@ -795,8 +803,8 @@ class FamilyEngine:
         # every temporary is allocated on the engine stream that uses it, so the
         # caching allocator cannot hand its memory to other work before that use
         with torch.cuda.stream(self.stream):
-            sl = torch.from_numpy(slots.astype(np.int64)).to(dv)
-            li = torch.from_numpy(lps.astype(np.int64)).to(dv)
+            sl = _h2d(slots.astype(np.int64), dv)
+            li = _h2d(lps.astype(np.int64), dv)
             for name, ln, dst, inv in (("x_sum", fb.n, "X_out", "col_dev"), ("y_sum", fb.m, "Y_out", "row_dev")):
                 # element (LP, problem index j) sits at device index inv[j]
                 base = (sl // TILE) * (ln * TILE) + sl % TILE
@@ -809,7 +817,7 @@ class FamilyEngine:
 
     def compact(self) -> None:
         """Save the finished LPs of the live tiles and move the running LPs to the
-        first ceil(running / 64) tiles (stream idle, at a check boundary)."""
+        first ceil(running / 64) tiles (stream drained, at a check boundary)."""
         fb = self.fb
         T, cnt = self.args.tiles, self.args.count
         with torch.cuda.stream(self.stream):
@@ -827,8 +835,9 @@ class FamilyEngine:
         R = int(keep.size)
         new_T = (R + TILE - 1) // TILE
         perm_np = np.concatenate([keep, np.full(new_T * TILE - R, keep[0] if R else 0)]).astype(np.int64)
+        dv = self.device
         with torch.cuda.stream(self.stream):
-            perm = torch.from_numpy(perm_np).to(self.device)
+            perm = _h2d(perm_np, dv)
             idx = {fb.n: self._slot_index(perm, fb.n, new_T), fb.m: self._slot_index(perm, fb.m, new_T)}
             for name, ln in self._per_lp:
                 t = self.t[name]
@@ -842,8 +851,7 @@ class FamilyEngine:
             for t_ in range(new_T):
                 lanes = min(TILE, R - t_ * TILE)
                 new_mask[t_] = np.uint64((1 << lanes) - 1) if lanes < 64 else np.uint64(0xFFFFFFFFFFFFFFFF)
-            self.t["run_mask"][:new_T].copy_(torch.from_numpy(new_mask.view(np.int64)).to(self.device))
-        self.stream.synchronize()
+            self.t["run_mask"][:new_T].copy_(_h2d(new_mask.view(np.int64), dv))
         self.slot_lp = np.concatenate([lp_of[perm_np[:R]], np.full(new_T * TILE - R, -1)])
         self.args.tiles = new_T
         self.args.count = R
