@@ -413,7 +413,6 @@ class FamilyEngine:
             if self.compactions or self.args.tiles != fb.tiles:
                 self.args.tiles, self.args.count = fb.tiles, fb.count
                 self.compactions = 0
-                self.drop_graphs()
             L.check(self.lib.pdlp_family_load(C.byref(self.args), dp("c"), dp("b"), dp("lo"), dp("hi"),
                                               self.t["norms"].data_ptr(), self.sh), "pdlp_family_load")
             for name, orig in self._orig.items():
@@ -450,14 +449,25 @@ class FamilyEngine:
         except Exception:
             pass
 
+    MAX_GRAPHS = 24
+
     def _graph(self, blocks: int):
-        g = self._graphs.get(blocks)
+        """The graph of ``blocks`` blocks for the current argument block (which it holds by
+        value), captured once per distinct argument block and kept, least recently used
+        dropped first, so that the tile counts a solve passes through are reused by the next."""
+        key = (blocks, C.string_at(C.addressof(self.args), C.sizeof(self.args)))
+        g = self._graphs.pop(key, None)
         if g is None:
             h = C.c_void_p()
             L.check(self.lib.pdlp_family_graph_capture(C.byref(self.args), blocks,
                                                        stream_handle(self._cap_stream), C.byref(h)),
                     "pdlp_family_graph_capture")
-            g = self._graphs[blocks] = h
+            g = h
+            if len(self._graphs) >= self.MAX_GRAPHS:
+                self.stream.synchronize()              # the oldest may still be running
+                old = next(iter(self._graphs))
+                self.lib.pdlp_graph_destroy(self._graphs.pop(old))
+        self._graphs[key] = g
         return g
 
     def set_tol(self, tol: float) -> None:
@@ -483,7 +493,6 @@ class FamilyEngine:
             self._restore_data()
             self.args.tiles, self.args.count = fb.tiles, fb.count
             self.compactions = 0
-            self.drop_graphs()
         self.slot_lp = np.concatenate([np.arange(fb.count), np.full(fb.tiles * TILE - fb.count, -1)])
         self.saved = np.zeros(fb.count, bool)
         w_cold = self.init_weight.copy() if self.flags & 1 else np.ones(fb.count)
@@ -717,7 +726,6 @@ class FamilyEngine:
         self.set_tol(tol)
         self.args.max_iters = int(max_iters)
         self.compactions += 1             # the slots' data is the stream's from here: reload on the next solve
-        self.drop_graphs()
         self.saved = np.zeros(total, bool)
         self.slot_lp = np.full(tiles * TILE, -1)
         queue = [(b, 0) for b in batches if b.count]
@@ -767,7 +775,6 @@ class FamilyEngine:
                     used = fill(t_run)
                     if used > t_run * TILE:
                         self.args.count, self.args.tiles = used, -(-used // TILE)
-                    self.drop_graphs()
                 elif not queue:
                     if running == 0:
                         break
@@ -781,7 +788,6 @@ class FamilyEngine:
         finally:
             self.t["X_out"], self.t["Y_out"], self.t["out_final"] = keep
             self.args.max_iters = 0
-            self.drop_graphs()
         return O, X, Y
 
     # ------------------------------------------------------------ compaction
@@ -856,7 +862,6 @@ class FamilyEngine:
         self.args.tiles = new_T
         self.args.count = R
         self.compactions += 1
-        self.drop_graphs()
 
     def collect(self) -> None:
         """Gather every LP's result into the LP-major device outputs (``X_out``, ``Y_out``,
