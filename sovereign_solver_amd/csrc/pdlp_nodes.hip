@@ -36,11 +36,21 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 
 constexpr int GATHER_THREADS = 256;
 
+// Which optional arrays are given is a template choice, so the element loop
+// has no branch around a load: the table entry, scale, mean and 1/std of an
+// element go out together, then the one source value the entry selects.
+template <bool SCALE, bool MEAN, bool STD, bool BF16>
 __global__ void __launch_bounds__(GATHER_THREADS) node_gather_kernel(PdlpNodeGather g, int64_t count, int lanes) {
     const int lane = threadIdx.x & (lanes - 1);
     const int per_block = GATHER_THREADS / lanes;
     const int64_t groups = (int64_t)gridDim.x * per_block;
     const int64_t total = (int64_t)g.batch * g.nodes;
+    const int32_t* __restrict__ table = g.table;
+    const float* __restrict__ shared = g.shared;
+    const float* __restrict__ scale = g.scale;
+    const float* __restrict__ mean = g.mean;
+    const float* __restrict__ inv_std = g.inv_std;
+    const int base2 = g.src_len + g.shared_len;
     for (int64_t r = blockIdx.x * (int64_t)per_block + threadIdx.x / lanes; r < count; r += groups) {
         int64_t q;
         if (g.rows != nullptr) q = g.rows[r];
@@ -48,25 +58,53 @@ __global__ void __launch_bounds__(GATHER_THREADS) node_gather_kernel(PdlpNodeGat
         else q = r;
         const int64_t b = q / g.nodes;
         const int j = (int)(q - b * g.nodes);
-        const float* src = g.src + b * g.src_len;
-        const float* src2 = g.src2 + b * g.src2_len;
-        const int base2 = g.src_len + g.shared_len;
+        const float* __restrict__ src = g.src + b * g.src_len;
+        const float* __restrict__ src2 = g.src2 + b * g.src2_len;
         for (int f = lane; f < g.ld_out; f += lanes) {
             float v = 0.f;
-            if (f < g.width) {
-                const size_t e = (size_t)j * g.width + f;
-                const int t = g.table[e];
-                float x = t < 0 ? 0.f : t < g.src_len ? src[t] : t < base2 ? g.shared[t - g.src_len] : src2[t - base2];
-                if (g.scale != nullptr) x *= g.scale[e];
-                if (g.mean != nullptr) x -= g.mean[f];
-                if (g.inv_std != nullptr) x *= g.inv_std[f];
-                v = x;
+            if (g.width > 0) {                          // uniform: the table exists
+                const bool in = f < g.width;
+                const int fc = in ? f : 0;
+                const size_t e = (size_t)j * g.width + fc;
+                const int t = table[e];
+                const float sc = SCALE ? scale[e] : 1.f;
+                const float mu = MEAN ? mean[fc] : 0.f;
+                const float is = STD ? inv_std[fc] : 1.f;
+                // the value's address (an unused entry reads its own table word)
+                const float* at = t < 0 ? reinterpret_cast<const float*>(table + e)
+                                  : t < g.src_len ? src + t
+                                  : t < base2     ? shared + (t - g.src_len)
+                                                  : src2 + (t - base2);
+                float x = *at;
+                x = t < 0 ? 0.f : x;
+                if (SCALE) x *= sc;
+                if (MEAN) x -= mu;
+                if (STD) x *= is;
+                v = in ? x : 0.f;
             }
             const size_t o = (size_t)r * g.ld_out + f;
-            if (g.out_bf16 != nullptr) g.out_bf16[o] = to_bf16(v);
+            if (BF16) g.out_bf16[o] = to_bf16(v);
             else g.out_f32[o] = v;
         }
     }
+}
+
+template <bool SCALE, bool MEAN, bool STD>
+void launch_gather3(const PdlpNodeGather& g, int64_t count, int lanes, unsigned blocks, hipStream_t s) {
+    if (g.out_bf16 != nullptr) node_gather_kernel<SCALE, MEAN, STD, true><<<blocks, GATHER_THREADS, 0, s>>>(g, count, lanes);
+    else node_gather_kernel<SCALE, MEAN, STD, false><<<blocks, GATHER_THREADS, 0, s>>>(g, count, lanes);
+}
+
+void launch_gather(const PdlpNodeGather& g, int64_t count, int lanes, unsigned blocks, hipStream_t s) {
+    const bool sc = g.scale != nullptr, mu = g.mean != nullptr, is = g.inv_std != nullptr;
+    if (sc && mu && is) launch_gather3<true, true, true>(g, count, lanes, blocks, s);
+    else if (sc && mu) launch_gather3<true, true, false>(g, count, lanes, blocks, s);
+    else if (sc && is) launch_gather3<true, false, true>(g, count, lanes, blocks, s);
+    else if (sc) launch_gather3<true, false, false>(g, count, lanes, blocks, s);
+    else if (mu && is) launch_gather3<false, true, true>(g, count, lanes, blocks, s);
+    else if (mu) launch_gather3<false, true, false>(g, count, lanes, blocks, s);
+    else if (is) launch_gather3<false, false, true>(g, count, lanes, blocks, s);
+    else launch_gather3<false, false, false>(g, count, lanes, blocks, s);
 }
 
 __global__ void __launch_bounds__(256) node_assemble_kernel(PdlpNodeAssembly a) {
@@ -108,7 +146,7 @@ int pdlp_node_gather(const PdlpNodeGather* g, void* stream) {
     while (lanes < 64 && lanes < g->ld_out) lanes *= 2;
     const int64_t rpb = GATHER_THREADS / lanes;
     const unsigned blocks = (unsigned)std::min<int64_t>((count + rpb - 1) / rpb, 65536);
-    node_gather_kernel<<<blocks, GATHER_THREADS, 0, as_stream(stream)>>>(*g, count, lanes);
+    launch_gather(*g, count, lanes, blocks, as_stream(stream));
     return hip_fail(hipGetLastError(), "node_gather_kernel launch");
 }
 
