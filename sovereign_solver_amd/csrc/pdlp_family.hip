@@ -311,6 +311,10 @@ __global__ void __launch_bounds__(NTH) dual_kernel(PdlpFamily f, int P) {
 }
 
 // ------------------------------------------------------------------- checks
+// The value is computed here, not inside a later branch that alone uses it (the
+// compiler would otherwise move its load into that branch and wait on it there).
+__device__ __forceinline__ void keep_loaded(double v) { asm volatile("" ::"v"(v)); }
+
 // Per-lane sums of both candidates (window average a[0..6), current a[6..12)),
 // reduced over the workgroup's wavefronts and stored as one chunk.
 __device__ __forceinline__ void store_chunk(const PdlpFamily& f, int chunk, int tile, double (&a)[NS2]) {
@@ -383,12 +387,34 @@ __global__ void __launch_bounds__(NT) kkt_cols_kernel(PdlpFamily f, int add, int
         }
         double sa = 0.0, sc = 0.0;
         const int beg = f.t_row_ptr[j], end = f.t_row_ptr[j + 1];
-#pragma unroll 4
-        for (int q = beg; q < end; ++q) {
+        // every entry's y, y_sum and y_psum are loaded whatever the window (no branch
+        // around a load), four entries' loads together, then summed in ascending q
+        int q = beg;
+        for (; q + 4 <= end; q += 4) {
+            double kv[4], yc[4], ys[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                kv[u] = (double)f.t_val[q + u];
+                const size_t iy = at(tile, f.m, f.t_col_idx[q + u], lane);
+                yc[u] = (double)f.y[iy];
+                ys[u] = f.y_sum[iy] + (double)f.y_psum[iy];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) keep_loaded(ys[u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double ya = win > 0 ? ys[u] / wd : yc[u];
+                sa += kv[u] * ya;
+                sc += kv[u] * yc[u];
+            }
+        }
+        for (; q < end; ++q) {
             const double kv = (double)f.t_val[q];
             const size_t iy = at(tile, f.m, f.t_col_idx[q], lane);
             const double yc = (double)f.y[iy];
-            const double ya = win > 0 ? (f.y_sum[iy] + (double)f.y_psum[iy]) / wd : yc;
+            const double ys = f.y_sum[iy] + (double)f.y_psum[iy];
+            keep_loaded(ys);
+            const double ya = win > 0 ? ys / wd : yc;
             sa += kv * ya;
             sc += kv * yc;
         }
@@ -456,12 +482,33 @@ __device__ __forceinline__ void row_products(const PdlpFamily& f, int tile, int 
                                              int end, double& da, double& dc) {
     da = 0.0;
     dc = 0.0;
-#pragma unroll 4
-    for (int q = beg; q < end; ++q) {
+    // x and x_sum loaded whatever the window, four entries' loads together
+    int q = beg;
+    for (; q + 4 <= end; q += 4) {
+        double kv[4], xc[4], xs[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            kv[u] = (double)f.val[q + u];
+            const size_t jx = at(tile, f.n, f.col_idx[q + u], lane);
+            xc[u] = (double)f.x[jx];
+            xs[u] = f.x_sum[jx];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) keep_loaded(xs[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double xa = win > 0 ? xs[u] / wd : xc[u];
+            da += kv[u] * xa;
+            dc += kv[u] * xc[u];
+        }
+    }
+    for (; q < end; ++q) {
         const double kv = (double)f.val[q];
         const size_t jx = at(tile, f.n, f.col_idx[q], lane);
         const double xc = (double)f.x[jx];
-        const double xa = win > 0 ? f.x_sum[jx] / wd : xc;
+        const double xs = f.x_sum[jx];
+        keep_loaded(xs);
+        const double xa = win > 0 ? xs / wd : xc;
         da += kv * xa;
         dc += kv * xc;
     }
