@@ -635,6 +635,17 @@ def run_headline(args):
         dt = time.perf_counter() - t
         h2h[name]["pipelined"] = {"lps_per_s": ok / dt, "ms_per_batch": dt * 1e3 / nb, "batches": nb,
                                   "optimal": ok}
+    # the same batches through one engine as one stream, host to host (FamilyServer.solve_streamed)
+    for name, srv in servers.items():
+        srv.solve_streamed(batches[:1])                 # graphs, buffers
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ok = sum(int((o["status"] == 1).sum()) for o, X, Y, obj in srv.solve_streamed(batches))
+        dt = time.perf_counter() - t
+        h2h[name]["streamed"] = {"lps_per_s": ok / dt, "ms_per_batch": dt * 1e3 / nb, "batches": nb,
+                                 "optimal": ok}
+    log(f"host to host, streamed through one engine over {nb} batches: learned "
+        f"{h2h['learned']['streamed']['lps_per_s']:.0f} LPs/s, cold {h2h['cold']['streamed']['lps_per_s']:.0f} LPs/s")
     del servers, srv, e0
     log(f"host to host, serving loop over {nb} batches: learned {h2h['learned']['pipelined']['lps_per_s']:.0f} "
         f"LPs/s ({h2h['learned']['pipelined']['ms_per_batch']:.1f} ms per batch), cold "

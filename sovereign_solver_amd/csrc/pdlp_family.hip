@@ -624,9 +624,10 @@ __global__ void __launch_bounds__(NSC * T) decide_kernel(PdlpFamily f, int mode,
         const size_t cpad = (size_t)f.tiles * T;
         double t = 0.0;
         if (act) {
-#pragma unroll 8
+            // 32 chunks' loads in flight at a time, added in chunk order
+#pragma unroll 32
             for (int c = 0; c < col_chunks; ++c) t += f.partials[((size_t)c * NS2 + q) * cpad + k];
-#pragma unroll 8
+#pragma unroll 32
             for (int c = 0; c < row_chunks; ++c) t += f.partials[((size_t)(KCH + c) * NS2 + q) * cpad + k];
         }
         red[q][lane] = t;

@@ -374,18 +374,28 @@ class FamilyEngine:
              "b_shift": torch.float64}
     _COLS = ("c", "lo", "hi", "c64", "lo_shift", "ub_span")
 
-    def _gather(self, problems) -> None:
-        """Per-LP sources into the pinned LP-major staging: one NumPy view of each
-        staging array, then one row copy per LP and vector (one thread: the
-        copies are short and threads only contend for the interpreter)."""
-        src = self._src
+    def host_sources(self, problems) -> Dict[str, torch.Tensor]:
+        """Fresh pinned LP-major FP64 copies [len(problems), len] of the per-LP vectors
+        this engine does not store once (``c``, ``b`` = [b_ub, b_eq], ``lo``, ``hi``),
+        e.g. to upload as a :class:`StreamBatch`'s sources."""
+        fb = self.fb
+        out = {name: torch.empty((len(problems), fb.n if name != "b" else fb.m), dtype=torch.float64).pin_memory()
+               for name in self._src}
+        self._gather(problems, out)
+        return out
+
+    def _gather(self, problems, dst: Optional[Dict[str, torch.Tensor]] = None) -> None:
+        """Per-LP sources into the pinned LP-major staging (or into ``dst``): one
+        NumPy view of each staging array, then one row copy per LP and vector (one
+        thread: the copies are short and threads only contend for the interpreter)."""
+        src = {k: v[0] for k, v in self._src.items()} if dst is None else dst
         if not src:
             return
         m_ub = self.fb.m_ub
-        C_ = src["c"][0].numpy() if "c" in src else None
-        B_ = src["b"][0].numpy() if "b" in src else None
-        LO = src["lo"][0].numpy() if "lo" in src else None
-        HI = src["hi"][0].numpy() if "lo" in src else None
+        C_ = src["c"].numpy() if "c" in src else None
+        B_ = src["b"].numpy() if "b" in src else None
+        LO = src["lo"].numpy() if "lo" in src else None
+        HI = src["hi"].numpy() if "lo" in src else None
         for k, p in enumerate(problems):
             if C_ is not None:
                 C_[k] = p.c
@@ -658,6 +668,8 @@ class FamilyEngine:
         device (``pdlp_family_load`` / ``_records`` / ``_set_start`` / ``_init`` on a
         view of those tiles), while the LPs of the other tiles keep their state."""
         v = self._view(t0, k)
+        for t_ in list(b.src.values()) + ([b.start] if b.start is not None else []):
+            t_.record_stream(self.stream)               # read on this stream: not reused before it has been
         norms = self.t["norms"].data_ptr() + t0 * TILE * 3 * 8
         row = lambda name: (b.src[name][first:].data_ptr() if name in b.src else None)
         L.check(self.lib.pdlp_family_load(C.byref(v), row("c"), row("b"), row("lo"), row("hi"), norms, self.sh),
@@ -683,7 +695,7 @@ class FamilyEngine:
         self.slot_lp[t0 * TILE: pad] = -1
         self.slot_lp[t0 * TILE: end] = np.arange(lp0, lp0 + k)
 
-    def run_stream(self, tol: float, max_iters: int, batches: Sequence[StreamBatch], admit_tiles: int = 0):
+    def run_stream(self, tol: float, max_iters: int, batches, admit_tiles: int = 0, total: Optional[int] = None):
         """Solve a stream of batches of LPs of this engine's family through its slots:
         LPs enter free tiles as earlier ones finish (at a check: the running LPs
         are packed into the leading tiles, the next LPs of the stream loaded into
@@ -695,6 +707,9 @@ class FamilyEngine:
         its own iterations; ``max_iters``, a multiple of check_every, is its own
         limit), so its result is the same bit for bit.  ``admit_tiles``: free whole
         tiles that trigger an admission (0: a quarter of the engine's tiles).
+        ``batches``: a sequence of :class:`StreamBatch`, or any iterable of them
+        (read only as LPs are admitted, so its batches may be prepared while
+        earlier ones run) together with ``total``, the number of LPs it yields.
         Returns (out records [N] (uint8 view of OUT_DTYPE), X [N, n], Y [N, m]) on
         the device, LPs in stream order (problems' own row and column order);
         the engine's next solve reloads its own batch."""
@@ -703,7 +718,8 @@ class FamilyEngine:
         if max_iters <= 0 or max_iters % ce:
             raise ValueError("a stream's max_iters must be a positive multiple of check_every")
         need_src = set(self._src)
-        for b in batches:
+
+        def check(b: StreamBatch) -> StreamBatch:
             if set(b.src) != need_src:
                 raise ValueError(f"stream batch sources must be exactly {sorted(need_src)}")
             for name, v in b.src.items():
@@ -713,7 +729,12 @@ class FamilyEngine:
             if b.start is not None and (b.start.dtype != torch.float32 or b.start.dim() != 2
                                         or not b.start.is_contiguous()):
                 raise ValueError("a stream batch's start must be a contiguous [k, d] float32 device tensor")
-        total = sum(b.count for b in batches)
+            return b
+
+        if total is None:
+            batches = [check(b) for b in batches]
+            total = sum(b.count for b in batches)
+        source = iter(batches)
         B, tiles = fb.count, fb.tiles
         admit_tiles = admit_tiles or max(1, tiles // 4)
         dv = self.device
@@ -728,16 +749,31 @@ class FamilyEngine:
         self.compactions += 1             # the slots' data is the stream's from here: reload on the next solve
         self.saved = np.zeros(total, bool)
         self.slot_lp = np.full(tiles * TILE, -1)
-        queue = [(b, 0) for b in batches if b.count]
+        queue = []
+        more = [True]                                  # the source may still yield batches
         lp_next = 0
+
+        def pull() -> None:
+            """The next batch with LPs onto the queue (waits for a lazy source)."""
+            while more[0] and not queue:
+                b = next(source, None)
+                if b is None:
+                    more[0] = False
+                elif check(b).count:
+                    queue.append((b, 0))
 
         def fill(t0: int) -> int:
             """Admit the next LPs into tiles t0 ..; returns the slots now in use."""
             nonlocal lp_next
             slot = t0 * TILE
-            while queue and slot < B:
+            while slot < B:
+                pull()
+                if not queue:
+                    break
                 b, first = queue[0]
                 k = min(b.count - first, B - slot)
+                if lp_next + k > total:
+                    raise ValueError(f"the stream yields more than total = {total} LPs")
                 self._admit(slot // TILE, b, first, k, lp_next)
                 lp_next += k
                 if first + k == b.count:
@@ -767,6 +803,8 @@ class FamilyEngine:
                 e0.synchronize()
                 running = int(self.running_host[s0, 0])
                 need = -(-running // TILE)
+                if tiles - need >= admit_tiles or running == 0:
+                    pull()                             # a lazy source's next batch, while blocks run
                 if queue and (tiles - need >= admit_tiles or running == 0):
                     self.stream.synchronize()
                     inflight.clear()
@@ -775,7 +813,7 @@ class FamilyEngine:
                     used = fill(t_run)
                     if used > t_run * TILE:
                         self.args.count, self.args.tiles = used, -(-used // TILE)
-                elif not queue:
+                elif not queue and not more[0]:
                     if running == 0:
                         break
                     live = int(np.count_nonzero(self.mask_host[s0, :t0].numpy()))

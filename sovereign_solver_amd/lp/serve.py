@@ -22,9 +22,23 @@ from typing import Callable, Iterable, Iterator, Optional, Sequence
 
 import torch
 
+import numpy as np
+
 from ..config import MAX_PDHG_ITERS
-from ..engine.family import FamilyBatch, FamilyEngine
+from ..engine.batch import OUT_DTYPE
+from ..engine.family import FamilyBatch, FamilyEngine, StreamBatch
 from ..problem import Problem
+
+
+class _Staged:
+    """What a features function reads of an engine (:meth:`FamilyEngine.staged` and
+    its stream), for sources uploaded on their own stream."""
+
+    def __init__(self, src, stream):
+        self._src, self.stream = src, stream
+
+    def staged(self, name):
+        return self._src.get(name)
 
 
 class FamilyServer:
@@ -52,6 +66,7 @@ class FamilyServer:
         self.tol, self.max_iters = float(tol), int(max_iters)
         self.model, self.features = model, features
         self.count = fb.count
+        self._up = torch.cuda.Stream(device=self.engines[0].device)   # uploads of streamed batches
 
     def _prepare(self, k: int, problems: Sequence[Problem]):
         """Host and upload work of batch k (worker thread): staging and
@@ -81,6 +96,74 @@ class FamilyServer:
         eng.run(self.max_iters)
         o, X, Y = eng.results()
         return o, X, Y, eng.objectives(X)
+
+    def _stream_batch(self, problems: Sequence[Problem]):
+        """A batch as a :class:`StreamBatch` (worker thread): its sources gathered into
+        fresh pinned memory and uploaded on the server's upload stream, then, with a
+        model, its network inputs and starts; returns (batch, host copy of c or None)."""
+        eng = self.engines[0]
+        eng.fb.with_problems(problems)                 # same family, count and sharing
+        host = eng.host_sources(problems)
+        with torch.cuda.stream(self._up):
+            src = {k: h.to(eng.device, non_blocking=True) for k, h in host.items()}
+            up = torch.cuda.Event()
+            up.record(self._up)
+        if self.model is None:
+            return StreamBatch(src, ready=up, count=len(problems)), host.get("c")
+        V = self.features(problems, _Staged(src, self._up))
+        ready = torch.cuda.Event()                     # the upload and device-side inputs
+        ready.record(self._up)
+        self.model.mlp.stream.wait_event(ready)        # so the starts' event covers them too
+        if isinstance(V, torch.Tensor):
+            V.record_stream(self.model.mlp.stream)
+        out, ev = self.model.starts(V)
+        n, m = self.model.n, self.model.m
+        w_col = n + m if self.model.log_weight is not None else -1
+        return StreamBatch(src, start=out, ready=ev, x_off=0, y_off=n, w_col=w_col), host.get("c")
+
+    def solve_streamed(self, batches: Sequence[Sequence[Problem]]):
+        """Every batch through one engine as one stream (:meth:`FamilyEngine.run_stream`):
+        a worker thread gathers, uploads and starts batch k+1 while the engine runs,
+        and the LPs of a batch enter tiles as earlier LPs finish, so no batch waits on
+        another's slowest LPs.  Returns [(out records, X, Y, objectives)] per batch,
+        host arrays of the caller's own; each LP's result is bitwise :meth:`solve`'s
+        on its batch.  ``max_iters`` must be a multiple of the engines' check_every."""
+        batches = [list(b) for b in batches]
+        if not batches:
+            return []
+        eng = self.engines[0]
+        if self.max_iters % eng.check_every:
+            raise ValueError("a stream's max_iters must be a multiple of check_every")
+        total = sum(len(b) for b in batches)
+        costs = []
+        with cf.ThreadPoolExecutor(1) as ex:
+            futs = [ex.submit(self._stream_batch, probs) for probs in batches]
+
+            def source():
+                for f in futs:
+                    sb, c_host = f.result()
+                    costs.append(c_host)
+                    yield sb
+            try:
+                O, X, Y = eng.run_stream(self.tol, self.max_iters, source(), total=total)
+            except BaseException:
+                for f in futs:
+                    f.cancel()
+                cf.wait(futs)
+                raise
+        with torch.cuda.stream(eng.stream):
+            o_all = np.frombuffer(O.cpu().numpy().tobytes(), dtype=OUT_DTYPE)
+            Xh, Yh = X.cpu().numpy(), Y.cpu().numpy()
+        res, lo = [], 0
+        for probs, c_host in zip(batches, costs):
+            hi = lo + len(probs)
+            Xb = Xh[lo:hi]
+            obj = (np.einsum("ij,ij->i", c_host.numpy(), Xb) if c_host is not None
+                   else Xb @ np.asarray(probs[0].c, float))
+            obj = obj * np.array([-1.0 if p.sense == "max" else 1.0 for p in probs])
+            res.append((o_all[lo:hi].copy(), Xb.copy(), Yh[lo:hi].copy(), obj))
+            lo = hi
+        return res
 
     def solve(self, problems: Sequence[Problem]):
         """One batch, without overlap: (out records, X [B, n], Y [B, m], objectives)."""

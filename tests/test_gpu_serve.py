@@ -74,6 +74,25 @@ def test_serving_loop_learned_same_bits_as_alone(gpu, small_model, where):
         assert np.array_equal(X, alone[1][1]) and np.array_equal(o["iterations"], alone[1][0]["iterations"])
 
 
+@pytest.mark.parametrize("learned", [False, True])
+def test_streamed_batches_same_bits_as_alone(gpu, small_model, learned):
+    """All batches through one engine as one stream (FamilyServer.solve_streamed:
+    uploads and network starts on a worker thread, LPs entering tiles as earlier
+    ones finish) give each batch's bits when solved alone."""
+    fam, model = small_model
+    bs = _batches(fam, 3, 64)
+    kw = {"model": model, "features": fam.node_inputs_staged} if learned else {}
+    srv = FamilyServer(bs[0], 1e-4, 100_000, engine_kw={"check_every": 50}, **kw)
+    streamed = srv.solve_streamed(bs)
+    assert len(streamed) == 3
+    alone = []
+    for probs in bs:
+        o, X, Y, obj = srv.solve(probs)
+        alone.append((o.copy(), X.copy(), Y.copy(), obj.copy()))
+    _check(bs, streamed, alone)
+    assert srv.solve_streamed([]) == []
+
+
 @pytest.mark.parametrize("G,T", [(40, 12), (200, 24)])
 def test_device_node_inputs_bitwise_host(gpu, G, T):
     """The dispatch family's network inputs computed on the device from the staged
