@@ -76,6 +76,20 @@ def test_stream_cold_same_bits_as_batches(gpu, admit_tiles):
     assert np.array_equal(o, ref[0]) and np.array_equal(X, ref[1]) and np.array_equal(Y, ref[2])
 
 
+def test_stream_engine_with_a_part_tile_same_bits(gpu):
+    """An engine of 100 slots (its last tile part empty) streams batches of other
+    sizes: admissions never reach past its slots, and every LP keeps its bits."""
+    fam = DispatchFamily(G=12, T=5)
+    sets = [[fam.problem(2000 + k) for k in range(100)], [fam.problem(2500 + k) for k in range(128)],
+            [fam.problem(2900 + k) for k in range(40)]]
+    eng = FamilyEngine(FamilyBatch(sets[0]), check_every=50)
+    got = _host(*eng.run_stream(1e-4, 100_000, [StreamBatch(_sources(eng, s, gpu)) for s in sets], admit_tiles=1))
+    lo = 0
+    for s in sets:
+        _same(got, _batch(s), lo, lo + len(s))
+        lo += len(s)
+
+
 def test_stream_device_starts_same_bits_as_batches(gpu):
     """Warm starts read on the device per batch (garbage and NaN rows included,
     which the failover judges) give the batch solve's bits."""
