@@ -51,40 +51,64 @@ __global__ void __launch_bounds__(GATHER_THREADS) node_gather_kernel(PdlpNodeGat
     const float* __restrict__ mean = g.mean;
     const float* __restrict__ inv_std = g.inv_std;
     const int base2 = g.src_len + g.shared_len;
-    for (int64_t r = blockIdx.x * (int64_t)per_block + threadIdx.x / lanes; r < count; r += groups) {
-        int64_t q;
-        if (g.rows != nullptr) q = g.rows[r];
-        else if (g.seed != 0) q = (int64_t)(mix64(g.seed ^ mix64((uint64_t)r)) % (uint64_t)total);
-        else q = r;
-        const int64_t b = q / g.nodes;
-        const int j = (int)(q - b * g.nodes);
-        const float* __restrict__ src = g.src + b * g.src_len;
-        const float* __restrict__ src2 = g.src2 + b * g.src2_len;
+    // two rows per pass (r and r + groups), each element computed as alone
+    for (int64_t r0 = blockIdx.x * (int64_t)per_block + threadIdx.x / lanes; r0 < count; r0 += 2 * groups) {
+        int64_t rr[2];
+        int j[2];
+        const float* __restrict__ src[2];
+        const float* __restrict__ src2[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            rr[u] = r0 + u * groups;
+            const int64_t r = rr[u] < count ? rr[u] : r0;          // a missing second row repeats the first
+            int64_t q;
+            if (g.rows != nullptr) q = g.rows[r];
+            else if (g.seed != 0) q = (int64_t)(mix64(g.seed ^ mix64((uint64_t)r)) % (uint64_t)total);
+            else q = r;
+            const int64_t b = q / g.nodes;
+            j[u] = (int)(q - b * g.nodes);
+            src[u] = g.src + b * g.src_len;
+            src2[u] = g.src2 + b * g.src2_len;
+        }
         for (int f = lane; f < g.ld_out; f += lanes) {
-            float v = 0.f;
+            float v[2] = {0.f, 0.f};
             if (g.width > 0) {                          // uniform: the table exists
                 const bool in = f < g.width;
                 const int fc = in ? f : 0;
-                const size_t e = (size_t)j * g.width + fc;
-                const int t = table[e];
-                const float sc = SCALE ? scale[e] : 1.f;
                 const float mu = MEAN ? mean[fc] : 0.f;
                 const float is = STD ? inv_std[fc] : 1.f;
-                // the value's address (an unused entry reads its own table word)
-                const float* at = t < 0 ? reinterpret_cast<const float*>(table + e)
-                                  : t < g.src_len ? src + t
-                                  : t < base2     ? shared + (t - g.src_len)
-                                                  : src2 + (t - base2);
-                float x = *at;
-                x = t < 0 ? 0.f : x;
-                if (SCALE) x *= sc;
-                if (MEAN) x -= mu;
-                if (STD) x *= is;
-                v = in ? x : 0.f;
+                size_t e[2];
+                int t[2];
+                float sc[2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    e[u] = (size_t)j[u] * g.width + fc;
+                    t[u] = table[e[u]];
+                    sc[u] = SCALE ? scale[e[u]] : 1.f;
+                }
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    // the value's address (an unused entry reads its own table word)
+                    const int tu = t[u];
+                    const float* at = tu < 0 ? reinterpret_cast<const float*>(table + e[u])
+                                      : tu < g.src_len ? src[u] + tu
+                                      : tu < base2     ? shared + (tu - g.src_len)
+                                                       : src2[u] + (tu - base2);
+                    float x = *at;
+                    x = tu < 0 ? 0.f : x;
+                    if (SCALE) x *= sc[u];
+                    if (MEAN) x -= mu;
+                    if (STD) x *= is;
+                    v[u] = in ? x : 0.f;
+                }
             }
-            const size_t o = (size_t)r * g.ld_out + f;
-            if (BF16) g.out_bf16[o] = to_bf16(v);
-            else g.out_f32[o] = v;
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (rr[u] >= count) continue;
+                const size_t o = (size_t)rr[u] * g.ld_out + f;
+                if (BF16) g.out_bf16[o] = to_bf16(v[u]);
+                else g.out_f32[o] = v[u];
+            }
         }
     }
 }
@@ -144,7 +168,7 @@ int pdlp_node_gather(const PdlpNodeGather* g, void* stream) {
     if (count == 0 || g->ld_out == 0) return 0;
     int lanes = 1;
     while (lanes < 64 && lanes < g->ld_out) lanes *= 2;
-    const int64_t rpb = GATHER_THREADS / lanes;
+    const int64_t rpb = GATHER_THREADS / lanes * 2;              // two rows per lane group and pass
     const unsigned blocks = (unsigned)std::min<int64_t>((count + rpb - 1) / rpb, 65536);
     launch_gather(*g, count, lanes, blocks, as_stream(stream));
     return hip_fail(hipGetLastError(), "node_gather_kernel launch");
