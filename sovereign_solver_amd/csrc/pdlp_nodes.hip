@@ -36,6 +36,8 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 
 constexpr int GATHER_THREADS = 256;
 
+constexpr int GATHER_ROWS = 4;   // rows of a lane group whose loads go out together
+
 // Which optional arrays are given is a template choice, so the element loop
 // has no branch around a load: the table entry, scale, mean and 1/std of an
 // element go out together, then the one source value the entry selects.
@@ -51,16 +53,17 @@ __global__ void __launch_bounds__(GATHER_THREADS) node_gather_kernel(PdlpNodeGat
     const float* __restrict__ mean = g.mean;
     const float* __restrict__ inv_std = g.inv_std;
     const int base2 = g.src_len + g.shared_len;
-    // two rows per pass (r and r + groups), each element computed as alone
-    for (int64_t r0 = blockIdx.x * (int64_t)per_block + threadIdx.x / lanes; r0 < count; r0 += 2 * groups) {
-        int64_t rr[2];
-        int j[2];
-        const float* __restrict__ src[2];
-        const float* __restrict__ src2[2];
+    // GATHER_ROWS rows per pass (r0, r0 + groups, ...), each element computed as alone
+    for (int64_t r0 = blockIdx.x * (int64_t)per_block + threadIdx.x / lanes; r0 < count;
+         r0 += GATHER_ROWS * groups) {
+        int64_t rr[GATHER_ROWS];
+        int j[GATHER_ROWS];
+        const float* __restrict__ src[GATHER_ROWS];
+        const float* __restrict__ src2[GATHER_ROWS];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < GATHER_ROWS; ++u) {
             rr[u] = r0 + u * groups;
-            const int64_t r = rr[u] < count ? rr[u] : r0;          // a missing second row repeats the first
+            const int64_t r = rr[u] < count ? rr[u] : r0;          // a missing row repeats the first
             int64_t q;
             if (g.rows != nullptr) q = g.rows[r];
             else if (g.seed != 0) q = (int64_t)(mix64(g.seed ^ mix64((uint64_t)r)) % (uint64_t)total);
@@ -71,23 +74,25 @@ __global__ void __launch_bounds__(GATHER_THREADS) node_gather_kernel(PdlpNodeGat
             src2[u] = g.src2 + b * g.src2_len;
         }
         for (int f = lane; f < g.ld_out; f += lanes) {
-            float v[2] = {0.f, 0.f};
+            float v[GATHER_ROWS];
+#pragma unroll
+            for (int u = 0; u < GATHER_ROWS; ++u) v[u] = 0.f;
             if (g.width > 0) {                          // uniform: the table exists
                 const bool in = f < g.width;
                 const int fc = in ? f : 0;
                 const float mu = MEAN ? mean[fc] : 0.f;
                 const float is = STD ? inv_std[fc] : 1.f;
-                size_t e[2];
-                int t[2];
-                float sc[2];
+                size_t e[GATHER_ROWS];
+                int t[GATHER_ROWS];
+                float sc[GATHER_ROWS];
 #pragma unroll
-                for (int u = 0; u < 2; ++u) {
+                for (int u = 0; u < GATHER_ROWS; ++u) {
                     e[u] = (size_t)j[u] * g.width + fc;
                     t[u] = table[e[u]];
                     sc[u] = SCALE ? scale[e[u]] : 1.f;
                 }
 #pragma unroll
-                for (int u = 0; u < 2; ++u) {
+                for (int u = 0; u < GATHER_ROWS; ++u) {
                     // the value's address (an unused entry reads its own table word)
                     const int tu = t[u];
                     const float* at = tu < 0 ? reinterpret_cast<const float*>(table + e[u])
@@ -103,7 +108,7 @@ __global__ void __launch_bounds__(GATHER_THREADS) node_gather_kernel(PdlpNodeGat
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
+            for (int u = 0; u < GATHER_ROWS; ++u) {
                 if (rr[u] >= count) continue;
                 const size_t o = (size_t)rr[u] * g.ld_out + f;
                 if (BF16) g.out_bf16[o] = to_bf16(v[u]);
@@ -168,7 +173,7 @@ int pdlp_node_gather(const PdlpNodeGather* g, void* stream) {
     if (count == 0 || g->ld_out == 0) return 0;
     int lanes = 1;
     while (lanes < 64 && lanes < g->ld_out) lanes *= 2;
-    const int64_t rpb = GATHER_THREADS / lanes * 2;              // two rows per lane group and pass
+    const int64_t rpb = GATHER_THREADS / lanes * GATHER_ROWS;    // rows per lane group and pass
     const unsigned blocks = (unsigned)std::min<int64_t>((count + rpb - 1) / rpb, 65536);
     launch_gather(*g, count, lanes, blocks, as_stream(stream));
     return hip_fail(hipGetLastError(), "node_gather_kernel launch");
