@@ -1044,16 +1044,40 @@ __global__ void __launch_bounds__(NWN * T) load_norm_kernel(PdlpFamily f, double
     const bool shc = f.shared & PDLP_FAM_SHARED_C64, shu = f.shared & PDLP_FAM_SHARED_UBSPAN,
                shb = f.shared & PDLP_FAM_SHARED_B64, shs = f.shared & PDLP_FAM_SHARED_BSHIFT;
     double bs2 = 0.0, c2 = 0.0, bw2 = 0.0, cw2 = 0.0;
-    for (int d = w; d < f.n; d += NWN) {
-        const double c = ldv(f.c64, shc, tile, f.n, d, lane), sp = ldv(f.ub_span, shu, tile, f.n, d, lane);
-        if (isfinite(sp)) bs2 += sp * sp;
-        c2 += c * c;
-        cw2 += c * c * (double)f.tau_hat[d];
+    // the wavefront's indices w, w + NWN, ... in order, four indices' loads together
+    constexpr int U = 4;
+    for (int d0 = w; d0 < f.n; d0 += U * NWN) {
+        double c[U], sp[U], th[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int d = d0 + u * NWN < f.n ? d0 + u * NWN : d0;
+            c[u] = ldv(f.c64, shc, tile, f.n, d, lane);
+            sp[u] = ldv(f.ub_span, shu, tile, f.n, d, lane);
+            th[u] = (double)f.tau_hat[d];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (d0 + u * NWN >= f.n) break;
+            if (isfinite(sp[u])) bs2 += sp[u] * sp[u];
+            c2 += c[u] * c[u];
+            cw2 += c[u] * c[u] * th[u];
+        }
     }
-    for (int i = w; i < f.m; i += NWN) {
-        const double b = ldv(f.b64, shb, tile, f.m, i, lane), bs = ldv(f.b_shift, shs, tile, f.m, i, lane);
-        bs2 += bs * bs;
-        bw2 += b * b / f.row_metric[i];
+    for (int i0 = w; i0 < f.m; i0 += U * NWN) {
+        double b[U], bs[U], rm[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = i0 + u * NWN < f.m ? i0 + u * NWN : i0;
+            b[u] = ldv(f.b64, shb, tile, f.m, i, lane);
+            bs[u] = ldv(f.b_shift, shs, tile, f.m, i, lane);
+            rm[u] = f.row_metric[i];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (i0 + u * NWN >= f.m) break;
+            bs2 += bs[u] * bs[u];
+            bw2 += b[u] * b[u] / rm[u];
+        }
     }
     red[w][0][lane] = bs2;
     red[w][1][lane] = c2;
