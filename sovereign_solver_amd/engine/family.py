@@ -706,7 +706,7 @@ class FamilyEngine:
         a batch solve from the same start (its checks fall every check_every of
         its own iterations; ``max_iters``, a multiple of check_every, is its own
         limit), so its result is the same bit for bit.  ``admit_tiles``: free whole
-        tiles that trigger an admission (0: a quarter of the engine's tiles).
+        tiles that trigger an admission (0: an eighth of the engine's tiles).
         ``batches``: a sequence of :class:`StreamBatch`, or any iterable of them
         (read only as LPs are admitted, so its batches may be prepared while
         earlier ones run) together with ``total``, the number of LPs it yields.
@@ -736,7 +736,7 @@ class FamilyEngine:
             total = sum(b.count for b in batches)
         source = iter(batches)
         B, tiles = fb.count, fb.tiles
-        admit_tiles = admit_tiles or max(1, tiles // 4)
+        admit_tiles = admit_tiles or max(1, tiles // 8)
         dv = self.device
         keep = (self.t["X_out"], self.t["Y_out"], self.t["out_final"])
         with torch.cuda.stream(self.stream):
